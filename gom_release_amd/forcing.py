@@ -1,0 +1,100 @@
+"""Host-side forcing of one step: the 2-D boundary values the reference computes on the
+host before the hot path (GOM_Rv1.1.0/source/main.f90:84-161) and the per-sub-cycle
+boundary concentrations of the TVD transport.
+
+These stay on the host by design (O(boundary) work per step); their results are handed to
+the device through ``gomgpu_set_forcing``.
+"""
+from __future__ import annotations
+
+import math
+from typing import Optional, Sequence
+
+import numpy as np
+
+from .gomio import Case, Series
+
+DEG2RAD = math.pi / 180.0
+
+
+def series_value(s: Series, u2: float, ref_diff_days: int = 0) -> Optional[float]:
+    """Linear interpolation of a series at time u2 [s], as the reference does it
+    (solve_transport_equation_v19.f90:245-258, compute_boundary_eta.f90:96-108)."""
+    t = [v * 86400.0 for v in s.times_days]
+    for i in range(1, len(t)):
+        u1 = t[i - 1] - ref_diff_days * 86400.0
+        u3 = t[i] - ref_diff_days * 86400.0
+        if u1 <= u2 <= u3:
+            v1, v3 = float(s.values[i - 1]), float(s.values[i])
+            return (u2 - u1) * (v3 - v1) / (u3 - u1) + v1
+    return None
+
+
+def spinup(flag: int, period_days: float, elapsed: float) -> float:
+    """setup_spinup.f90:10-54."""
+    if flag == 1:
+        per = period_days * 86400.0
+        if elapsed < per:
+            return math.tanh(2.0 * elapsed / per)
+    return 1.0
+
+
+def julian_day0(case: Case) -> float:
+    """Day-of-year (0-based, fractional) of the start time, julian.f90."""
+    y, mo, d, h, mi, s = case.start
+    cum = [0, 31, 59, 90, 120, 151, 181, 212, 243, 273, 304, 334][mo - 1]
+    jd = float(cum)
+    if y % 4 == 0 and y != 1900 and mo >= 3:
+        jd = jd + 1
+    return jd + (d - 1) + h / 24.0 + mi / 1440.0 + s / 86400.0
+
+
+def boundary_eta(case: Case, it: int, sp_tide: float):
+    """eta_at_ob_old/new for step `it` (compute_boundary_eta.f90:72-90, harmonic type)."""
+    dt = case.dt
+    elapsed = it * dt
+    nob = len(case.open_boundaries)
+    old = np.zeros(max(nob, 1))
+    new = np.zeros(max(nob, 1))
+    for i, b in enumerate(case.open_boundaries):
+        if b.eta_type != 1:
+            raise NotImplementedError("only harmonic open boundaries are generated")
+        shift, cons = case.harmonics[b.harmonic_id - 1]
+        o = 0.0
+        n = 0.0
+        for (per, amp, ph, fac, v0, _nm) in cons:
+            w = 2.0 * math.pi / (per * 3600.0)
+            o = o + sp_tide * amp * fac * math.cos(w * (elapsed - dt) + v0 * DEG2RAD - ph * DEG2RAD - shift * DEG2RAD)
+            n = n + sp_tide * amp * fac * math.cos(w * elapsed + v0 * DEG2RAD - ph * DEG2RAD - shift * DEG2RAD)
+        old[i] = o
+        new[i] = n
+    return old, new
+
+
+def tvd_boundary_concentrations(case: Case, julian_day: float, sp_baroclinic: float,
+                                ntran: int, tran_ids: Sequence[int]):
+    """con_at_ob (Nt, ntran, OB) and con_at_Qb (Nt, ntran, QB), Fortran order, for the
+    sub-cycles of one step (solve_transport_equation_v19.f90:229-397)."""
+    nt = max(case.trans_sub_iter, 1)
+    dt2 = case.dt / nt
+    nob = max(len(case.open_boundaries), 1)
+    nqb = max(len(case.rivers), 1)
+    ob = np.zeros((nt, max(ntran, 1), nob))
+    qb = np.zeros((nt, max(ntran, 1), nqb))
+    for sub in range(nt):
+        u2 = (julian_day * 86400.0 - case.dt) + (sub + 1) * dt2
+        for t in range(ntran):
+            tid = tran_ids[t]
+            for i, b in enumerate(case.open_boundaries):
+                sid = b.salt_ser_id if tid == 1 else b.temp_ser_id
+                ser = case.salt_series if tid == 1 else case.temp_series
+                if sid > 0:
+                    v = series_value(ser[sid - 1], u2)
+                    ob[sub, t, i] = sp_baroclinic * (v if v is not None else 0.0)
+            for i, r in enumerate(case.rivers):
+                sid = r.salt_ser_id if tid == 1 else r.temp_ser_id
+                ser = case.salt_series if tid == 1 else case.temp_series
+                if sid > 0:
+                    v = series_value(ser[sid - 1], u2)
+                    qb[sub, t, i] = sp_baroclinic * (v if v is not None else 0.0)
+    return ob.ravel(order="F"), qb.ravel(order="F")

@@ -1,0 +1,187 @@
+"""Synthetic meshes and cases for the benchmark configurations.
+
+* ``channel_case``  rectangular channel of counter-clockwise quads, uniform depth, M2 tide
+  on the west open boundary, other sides closed (the 64x32, 200 m, 10-layer configuration
+  and smaller variants of it).
+* ``estuary_case``  mixed orthogonal mesh: rows of 100x100 m squares stitched to rows of
+  equilateral 100 m triangles, Gaussian thalweg bathymetry with seeded noise, salinity
+  ramp from river to sea, river inflow at the east end, M2+K1 tide at the west end.
+
+Cells are written counter-clockwise; node and cell ids are 1-based as in the reference.
+"""
+from __future__ import annotations
+
+import math
+from typing import List, Optional, Sequence, Tuple
+
+import numpy as np
+
+from .gomio import Case, OpenBoundary, RiverBoundary, Series
+
+M2_HOURS = 12.42
+K1_HOURS = 23.93
+
+
+def uniform_levels(depth: float, nlayer: int, top: float, surf_gap: float) -> np.ndarray:
+    """z_level(0..L): L-1 equal layers from -depth up to -surf_gap, then a top layer to `top`."""
+    z = np.empty(nlayer + 1)
+    z[0] = -depth
+    lower = np.linspace(-depth, -surf_gap, nlayer)
+    z[:nlayer] = lower
+    z[nlayer] = top
+    return z
+
+
+def channel_case(nx: int = 64, ny: int = 32, dx: float = 200.0, depth: float = 5.0,
+                 nlayer: int = 10, amp: float = 0.5, dt: float = 60.0, ndt: int = 1440,
+                 salt: float = 30.0, name: str = "channel", **kw) -> Case:
+    """Rectangular channel of nx x ny quads; the west side (x = 0) is an open boundary."""
+    ii, jj = np.meshgrid(np.arange(nx + 1), np.arange(ny + 1), indexing="xy")
+    x = (ii * dx).astype(np.float64).ravel()
+    y = (jj * dx).astype(np.float64).ravel()
+    nid = lambda i, j: j * (nx + 1) + i + 1          # noqa: E731
+    cells = np.zeros((nx * ny, 4), dtype=np.int64)
+    c = 0
+    for j in range(ny):
+        for i in range(nx):
+            cells[c] = (nid(i, j), nid(i + 1, j), nid(i + 1, j + 1), nid(i, j + 1))
+            c += 1
+    P = x.shape[0]
+    z = uniform_levels(depth, nlayer, top=max(3.0, 4 * amp), surf_gap=max(0.4, 2.0 * amp + 0.4))
+    obs = [OpenBoundary(nid(0, j + 1), nid(0, j), eta_type=1, harmonic_id=1, salt_ser_id=1)
+           for j in range(ny)]
+    case = Case(name=name, x=x, y=y, depth=np.full(P, depth), cells=cells, z_level=z,
+                dt=dt, ndt=ndt,
+                salt_init=np.full((P, nlayer), salt),
+                salt_series=[Series([-1000.0, 100000.0], [salt, salt])],
+                open_boundaries=obs,
+                harmonics=[(0.0, [(M2_HOURS, amp, 0.0, 1.0, 0.0, "M2")])])
+    for k, v in kw.items():
+        setattr(case, k, v)
+    return case
+
+
+def estuary_mesh(ncol: int, nrep: int, pattern: str = "STT", dx: float = 100.0):
+    """Rows stacked in y following `pattern` repeated `nrep` times.
+
+    'S' is a row of ncol squares (dx x dx); 'T' a row of equilateral triangles of side dx.
+    Triangle rows alternate the x offset of their node rows; rows touching the x = 0 and
+    x = ncol*dx sides are closed by right triangles so the domain stays rectangular.
+    Returns (x, y, cells) with 1-based node ids.
+    """
+    h_tri = dx * math.sqrt(3.0) / 2.0
+    rows = pattern * nrep
+    xs: List[float] = []
+    ys: List[float] = []
+    cells: List[Tuple[int, int, int, int]] = []
+
+    def add_row(yv: float, offset: bool) -> List[int]:
+        ids = []
+        if offset:
+            pts = [0.0] + [dx * (i + 0.5) for i in range(ncol)] + [dx * ncol]
+        else:
+            pts = [dx * i for i in range(ncol + 1)]
+        for px in pts:
+            xs.append(px)
+            ys.append(yv)
+            ids.append(len(xs))
+        return ids
+
+    yv = 0.0
+    offset = False
+    cur = add_row(yv, offset)
+    for r in rows:
+        if r == "S":
+            if offset:
+                raise ValueError("square row needs an aligned node row below it")
+            yv += dx
+            nxt = add_row(yv, False)
+            for i in range(ncol):
+                cells.append((cur[i], cur[i + 1], nxt[i + 1], nxt[i]))
+            cur = nxt
+        else:
+            yv += h_tri
+            nxt = add_row(yv, not offset)
+            if not offset:
+                # lower row aligned (ncol+1 pts), upper offset (ncol+2 pts incl. side pts)
+                up = nxt
+                cells.append((cur[0], up[1], up[0], 0))
+                for i in range(ncol):
+                    cells.append((cur[i], cur[i + 1], up[i + 1], 0))
+                    if i + 1 < ncol:
+                        cells.append((cur[i + 1], up[i + 2], up[i + 1], 0))
+                cells.append((cur[ncol], up[ncol + 1], up[ncol], 0))
+            else:
+                lo = cur
+                up = nxt
+                cells.append((lo[0], lo[1], up[0], 0))
+                for i in range(ncol):
+                    cells.append((lo[i + 1], up[i + 1], up[i], 0))
+                    cells.append((lo[i + 1], lo[i + 2], up[i + 1], 0))
+            cur = nxt
+            offset = not offset
+    return np.array(xs), np.array(ys), np.array(cells, dtype=np.int64)
+
+
+def estuary_case(ncol: int = 40, nrep: int = 4, nlayer: int = 5, dx: float = 100.0,
+                 depth_axis: float = 20.0, depth_bank: float = 2.0, noise: float = 0.5,
+                 seed: int = 1, river_q: float = 500.0, wind: float = 0.0,
+                 dt: float = 120.0, ndt: int = 10, dry_land: bool = False,
+                 tide_phase: float = 0.0,
+                 name: str = "estuary", **kw) -> Case:
+    """Synthetic estuary on the stitched square/triangle mesh (axis along x)."""
+    x, y, cells = estuary_mesh(ncol, nrep, dx=dx)
+    rng = np.random.default_rng(seed)
+    ymid = 0.5 * (y.min() + y.max())
+    width = y.max() - y.min()
+    sig = width / 5.0
+    depth = depth_bank + (depth_axis - depth_bank) * np.exp(-((y - ymid) / sig) ** 2)
+    depth = depth + rng.uniform(-noise, noise, size=depth.shape)
+    if dry_land:
+        # intertidal banks: beyond 0.3 of the width from the axis the bed rises linearly
+        # from 1 m depth to 2 cm depth at the sides; with a falling tide they dry out
+        rel = np.abs(y - ymid) / (0.5 * width)
+        bank = rel > 0.6
+        depth[bank] = 1.0 - (rel[bank] - 0.6) / 0.4 * 0.98
+    depth = np.round(depth, 6)
+    zb = float(np.ceil(depth.max() + 0.5))
+    z = uniform_levels(zb, nlayer, top=4.0, surf_gap=1.5)
+    P = x.shape[0]
+    L = nlayer
+    xr = (x - x.min()) / (x.max() - x.min())
+    s_node = 35.0 * (1.0 - xr)
+    salt_init = np.repeat(np.round(s_node, 6)[:, None], L, axis=1)
+    # open boundary at x = 0 (west): faces on the x=0 side in their cell's CCW order
+    obs: List[OpenBoundary] = []
+    rivers: List[RiverBoundary] = []
+    xmax = x.max()
+    west: List[Tuple[int, int]] = []
+    east: List[Tuple[int, int]] = []
+    for c in cells:
+        n = 4 if c[3] else 3
+        for l in range(n):
+            a, b = int(c[l]), int(c[(l + 1) % n])
+            if x[a - 1] == 0.0 and x[b - 1] == 0.0:
+                west.append((a, b))
+            if x[a - 1] == xmax and x[b - 1] == xmax:
+                east.append((a, b))
+    for (a, b) in west:
+        obs.append(OpenBoundary(a, b, eta_type=1, harmonic_id=1, salt_ser_id=1))
+    # river: the east-side faces nearest to the axis (3 faces)
+    east.sort(key=lambda ab: abs(0.5 * (y[ab[0] - 1] + y[ab[1] - 1]) - ymid))
+    nriv = min(3, len(east))
+    for (a, b) in east[:nriv]:
+        rivers.append(RiverBoundary(a, b, q_ser_id=1, portion=1.0 / nriv, salt_ser_id=2))
+    case = Case(name=name, x=x, y=y, depth=depth, cells=cells, z_level=z, dt=dt, ndt=ndt,
+                salt_init=salt_init,
+                salt_series=[Series([-1000.0, 100000.0], [35.0, 35.0]),
+                             Series([-1000.0, 100000.0], [0.0, 0.0])],
+                open_boundaries=obs,
+                harmonics=[(0.0, [(M2_HOURS, 0.5, tide_phase, 1.0, 0.0, "M2"),
+                                  (K1_HOURS, 0.2, tide_phase + 30.0, 1.0, 0.0, "K1")])],
+                rivers=rivers,
+                q_series=[Series([-1000.0, 100000.0], [river_q, river_q])] if rivers else [],
+                trans_sub_iter=2, h_flux_limiter=1, v_flux_limiter=1)
+    for k, v in kw.items():
+        setattr(case, k, v)
+    return case

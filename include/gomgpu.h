@@ -1,0 +1,229 @@
+/*
+ * gomgpu.h - C ABI of the MI355X implementation of GOM's semi-implicit time step.
+ *
+ * The reference calls its step as argument-less Fortran subroutines that exchange data
+ * through the module mod_global_variables (GOM_Rv1.1.0/source/main.f90:179-272). This ABI
+ * replaces them one for one, with the module arrays passed explicitly:
+ *
+ *   gomgpu_stage(ctx, GOMGPU_STAGE_FRICTION)  calculate_bottom_friction   main.f90:95
+ *   gomgpu_stage(ctx, GOMGPU_STAGE_ELM)       solve_nonlinear_advection   main.f90:167-180
+ *   gomgpu_stage(ctx, GOMGPU_STAGE_MOMENTUM)  solve_momentum_equation     main.f90:193
+ *   gomgpu_stage(ctx, GOMGPU_STAGE_FREESURF)  solve_free_surface_equation main.f90:201
+ *   gomgpu_stage(ctx, GOMGPU_STAGE_VELOCITY)  solve_velocities            main.f90:207
+ *   gomgpu_stage(ctx, GOMGPU_STAGE_LAYERS)    update_vertical_layers_1    main.f90:210
+ *   gomgpu_stage(ctx, GOMGPU_STAGE_NODEVEL)   calculate_velocity_at_node  main.f90:213
+ *   gomgpu_stage(ctx, GOMGPU_STAGE_TRANSPORT) solve_transport_equation_v19 main.f90:231
+ *   gomgpu_stage(ctx, GOMGPU_STAGE_EOS)       calculate_density_full/_linear main.f90:261,264
+ *   gomgpu_stage(ctx, GOMGPU_STAGE_UPDATE)    update_variables            main.f90:272
+ *   gomgpu_step(ctx)                          all of the above, in that order
+ *
+ * Layout conventions (identical to the reference module arrays, so a Fortran caller passes
+ * its allocatables with c_loc): every 2-D array is Fortran column-major (layer, entity),
+ * entity ids are 1-based int32 with 0 meaning "none". Arrays whose Fortran lower bound is
+ * 0 (dzhalf_*, wn_cell*, u/v/w_node, u/v_face_level, Av, Kv) have leading dimension
+ * maxlayer+1, the others maxlayer. Host arrays are only read (or written) during the call;
+ * no pointer is kept. Device state is owned by the context and stays resident between
+ * steps; host copies are refreshed only by gomgpu_get_state.
+ *
+ * Errors: every function returns 0 on success, a positive GOMGPU_E* code on failure. A
+ * failing device check (the reference's `stop` conditions) is latched with the first
+ * failing entity and reported by gomgpu_last_error. The PCG never fails silently:
+ * non-convergence is reported as GOMGPU_W_PCG_NOT_CONVERGED by gomgpu_step, and the
+ * iteration count of the last solve is available from gomgpu_pcg_iterations.
+ */
+#ifndef GOMGPU_H
+#define GOMGPU_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---------------------------------------------------------------------------------------
+ * Field lists. Each entry is X(name, extent) where extent names the array shape:
+ *   P nodes, E cells, J faces, L maxlayer, L1 maxlayer+1, NB max_no_neighbor_node,
+ *   NB1 max_no_neighbor_node+1, OB num_ob_cell, QB num_Qb_cell.
+ * ------------------------------------------------------------------------------------- */
+
+/* Static geometry, from set_geometry_1.f90 / set_geometry_2.f90 / read_*_inp.f90. */
+#define GOMGPU_MESH_REAL(X) \
+  X(x_node, P) X(y_node, P) X(h_node, P) \
+  X(x_cell, E) X(y_cell, E) X(h_cell, E) X(area, E) \
+  X(face_length, J) X(x_face, J) X(y_face, J) X(h_face, J) X(delta_j, J) \
+  X(cos_theta, J) X(sin_theta, J) X(cos_theta2, J) X(sin_theta2, J) \
+  X(coriolis_factor, J) X(manning, J) X(bf_height, J) \
+  X(z_level, L1) X(delta_z, L)
+
+#define GOMGPU_MESH_INT(X) \
+  X(tri_or_quad, E) X(nodenum_at_cell, 4E) X(adj_cellnum_at_cell, 4E) \
+  X(facenum_at_cell, 4E) X(sign_in_outflow, 4E) \
+  X(adj_cellnum_at_face, 2J) X(nodenum_at_face, 2J) \
+  X(adj_cells_at_node, P) X(adj_cellnum_at_node, NBP) X(node_count_each_element, NBP) \
+  X(adj_nodes_at_node, P) X(adj_nodenum_at_node, NB1P) \
+  X(boundary_type_of_face, J) X(isflowside3, J) \
+  X(ob_element_flag, E) X(Qb_element_flag, E) \
+  X(bottom_layer_at_node, P) X(bottom_layer_at_element, E) X(bottom_layer_at_face, J) \
+  X(initial_wetdry_node, P) X(ob_cell_id, OB) X(Qb_cell_id, QB)
+
+/* Prognostic and diagnostic state (mod_global_variables.f90). */
+#define GOMGPU_STATE_REAL(X) \
+  X(eta_cell, E) X(eta_cell_new, E) X(eta_node, P) X(Gamma_B, J) X(Cdb, J) \
+  X(un_face, LJ) X(vn_face, LJ) X(un_face_new, LJ) X(vn_face_new, LJ) \
+  X(un_ELM, LJ) X(vn_ELM, LJ) X(dz_face, LJ) X(dz_face_new, LJ) \
+  X(AinvG1, LJ) X(AinvG2, LJ) X(AinvDeltaZ1, LJ) X(AinvDeltaZ2, LJ) X(Kh, LJ) \
+  X(salt_face, LJ) X(temp_face, LJ) X(rho_face, LJ) \
+  X(dzhalf_face, L1J) X(dzhalf_face_new, L1J) X(Av, L1J) \
+  X(u_face_level, L1J) X(v_face_level, L1J) \
+  X(dz_cell, LE) X(dz_cell_new, LE) X(salt_cell, LE) X(salt_cell_new, LE) \
+  X(temp_cell, LE) X(temp_cell_new, LE) X(rho_cell, LE) \
+  X(dzhalf_cell, L1E) X(dzhalf_cell_new, L1E) X(wn_cell, L1E) X(wn_cell_new, L1E) X(Kv, L1E) \
+  X(dz_node, LP) X(dz_node_new, LP) X(salt_node, LP) X(temp_node, LP) X(rho_node, LP) \
+  X(dzhalf_node, L1P) X(dzhalf_node_new, L1P) \
+  X(u_node, L1P) X(v_node, L1P) X(w_node, L1P) \
+  X(ubar_node, P) X(vbar_node, P) X(sbar_node, P) X(tbar_node, P) X(rbar_node, P)
+
+#define GOMGPU_STATE_INT(X) \
+  X(top_layer_at_node, P) X(top_layer_at_element, E) X(top_layer_at_face, J) \
+  X(wetdry_node, P) X(num_sub_elm_iteration, LP)
+
+/* Per-step forcing computed by the host's 2-D boundary routines (main.f90:84-161,
+ * heat_term_by_term.f90) and the boundary concentrations of the transport sub-cycles
+ * (solve_transport_equation_v19.f90:229-534): con_at_ob is (Nt, maxtran2, OB) and
+ * con_at_Qb (Nt, maxtran2, QB), fastest index first. */
+#define GOMGPU_FORCING_REAL(X) \
+  X(Gamma_T, J) X(wind_stress_normal, J) X(wind_stress_tangnt, J) X(air_p1, P) \
+  X(eta_at_ob_old, OB) X(eta_at_ob_new, OB) X(Qu_boundary, QB) X(phi_sz, LE) \
+  X(con_at_ob, NTOB) X(con_at_Qb, NTQB)
+
+/* Scalars of main.inp and set-up (read_main_inp.f90). */
+#define GOMGPU_PARAMS_INT(X) \
+  X(maxnod) X(maxele) X(maxface) X(maxlayer) X(max_no_neighbor_node) \
+  X(num_ob_cell) X(num_Qb_cell) X(maxtran2) X(tran_id1) X(tran_id2) \
+  X(advection_flag) X(ELM_backtrace_flag) X(ELM_min_iter) X(ELM_max_iter) \
+  X(bf_flag) X(baroclinic_flag) X(ana_density) X(i_density_flag) \
+  X(transport_flag) X(trans_sub_iter) X(h_flux_limiter) X(v_flux_limiter) \
+  X(heat_option) X(max_iteration_pcg) X(precision)
+
+#define GOMGPU_PARAMS_REAL(X) \
+  X(dt) X(theta) X(gravity) X(rho_o) X(MSL) X(dry_depth) X(adv_onoff_depth) \
+  X(smagorinsky_parameter) X(Ah_0) X(Kh_0) X(von_Karman) X(error_tolerance) \
+  X(denominator_min_for_matrix) X(delta_z_min)
+
+#define GOMGPU_FORCING_SCALARS(X) \
+  X(elapsed_time) X(spinup_function_tide) X(spinup_function_baroclinic)
+
+#define GOMGPU_DECL_RPTR(name, ext) double *name;
+#define GOMGPU_DECL_IPTR(name, ext) int32_t *name;
+#define GOMGPU_DECL_CRPTR(name, ext) const double *name;
+#define GOMGPU_DECL_CIPTR(name, ext) const int32_t *name;
+#define GOMGPU_DECL_INT(name) int32_t name;
+#define GOMGPU_DECL_REAL(name) double name;
+
+typedef struct gomgpu_mesh {
+  GOMGPU_MESH_REAL(GOMGPU_DECL_CRPTR)
+  GOMGPU_MESH_INT(GOMGPU_DECL_CIPTR)
+} gomgpu_mesh;
+
+typedef struct gomgpu_params {
+  GOMGPU_PARAMS_INT(GOMGPU_DECL_INT)
+  GOMGPU_PARAMS_REAL(GOMGPU_DECL_REAL)
+} gomgpu_params;
+
+/* Fields may be NULL in gomgpu_get_state (not copied) and in gomgpu_set_state (left as
+ * they are on the device; a new context starts from zeros). */
+typedef struct gomgpu_state {
+  GOMGPU_STATE_REAL(GOMGPU_DECL_RPTR)
+  GOMGPU_STATE_INT(GOMGPU_DECL_IPTR)
+} gomgpu_state;
+
+typedef struct gomgpu_forcing {
+  GOMGPU_FORCING_SCALARS(GOMGPU_DECL_REAL)
+  GOMGPU_FORCING_REAL(GOMGPU_DECL_CRPTR)
+} gomgpu_forcing;
+
+/* Precision of the bulk arithmetic (gomgpu_params.precision). */
+enum { GOMGPU_FP64 = 0, GOMGPU_MIXED = 1 };
+
+enum gomgpu_stage_id {
+  GOMGPU_STAGE_FRICTION = 0,
+  GOMGPU_STAGE_ELM = 1,
+  GOMGPU_STAGE_MOMENTUM = 2,
+  GOMGPU_STAGE_FREESURF = 3,
+  GOMGPU_STAGE_VELOCITY = 4,
+  GOMGPU_STAGE_LAYERS = 5,
+  GOMGPU_STAGE_NODEVEL = 6,
+  GOMGPU_STAGE_TRANSPORT = 7,
+  GOMGPU_STAGE_EOS = 8,
+  GOMGPU_STAGE_UPDATE = 9,
+  GOMGPU_NUM_STAGES = 10
+};
+
+enum gomgpu_status {
+  GOMGPU_OK = 0,
+  GOMGPU_E_ARG = 1,         /* bad argument or unsupported option */
+  GOMGPU_E_HIP = 2,         /* HIP runtime error */
+  GOMGPU_E_NOMEM = 3,       /* device allocation failed */
+  GOMGPU_E_MODEL = 4,       /* a reference `stop` condition fired on the device */
+  GOMGPU_W_PCG_NOT_CONVERGED = 5
+};
+
+/* Device error codes latched by the kernels (the reference's stop conditions). */
+enum gomgpu_model_error {
+  GOMGPU_ERR_NONE = 0,
+  GOMGPU_ERR_TRIDIAG_PIVOT = 1,      /* tridiagonal_solver.f90:32-35,47-50 */
+  GOMGPU_ERR_NOT_SPD = 2,            /* solve_free_surface_equation.f90:100-104 */
+  GOMGPU_ERR_LAYER_CELL = 3,         /* update_vertical_layers_1.f90:66-81 */
+  GOMGPU_ERR_LAYER_FACE = 4,         /* update_vertical_layers_1.f90:166-176 */
+  GOMGPU_ERR_LAYER_NODE = 5,         /* update_vertical_layers_1.f90:224-234 */
+  GOMGPU_ERR_WET_DRY_SIDES = 6,      /* update_vertical_layers_1.f90:273-276 */
+  GOMGPU_ERR_DZHALF_MIN = 7,         /* update_vertical_layers_1.f90:281-330 */
+  GOMGPU_ERR_ELM_DRY_START = 8,      /* search_straight_line.f90:37-46 */
+  GOMGPU_ERR_ELM_NOT_IN_CELL = 9,    /* search_straight_line.f90:82-89 */
+  GOMGPU_ERR_ELM_ZERO_PATH = 10,     /* search_straight_line.f90:129-139 */
+  GOMGPU_ERR_ELM_NO_EDGE = 11,       /* search_straight_line.f90:162-169, 343-353 */
+  GOMGPU_ERR_ELM_OVERSHOT = 12,      /* search_straight_line.f90:199-207 */
+  GOMGPU_ERR_ELM_TARGET = 13,        /* search_straight_line.f90:218-222 */
+  GOMGPU_ERR_ELM_WRONG_SIDE = 14,    /* search_straight_line.f90:237-242 */
+  GOMGPU_ERR_ELM_DRY_END = 15,       /* search_straight_line.f90:363-372 */
+  GOMGPU_ERR_ELM_VERTICAL = 16,      /* search_straight_line.f90:379-385, ELM_backtrace_v1.f90:107-113 */
+  GOMGPU_ERR_ELM_OUTSIDE_QUAD = 17,  /* ELM_backtrace_v1.f90:164-170 */
+  GOMGPU_ERR_BILINEAR = 18,          /* bilinear_interpolation.f90:41-112 */
+  GOMGPU_ERR_EOS_RANGE = 19,         /* calculate_density_full.f90:43-53,103-107 */
+  GOMGPU_ERR_EOS_DRY_NODES = 20      /* calculate_density_full.f90:223-225 */
+};
+
+typedef struct gomgpu_ctx gomgpu_ctx;
+
+/* Create a context on the current HIP device; uploads the static mesh once. */
+int gomgpu_create(const gomgpu_mesh *mesh, const gomgpu_params *params, gomgpu_ctx **out);
+/* Upload state fields (non-NULL ones) after prepare_gom or a restart. */
+int gomgpu_set_state(gomgpu_ctx *ctx, const gomgpu_state *state);
+/* Upload the host forcing of the coming step (main.f90:84-161). */
+int gomgpu_set_forcing(gomgpu_ctx *ctx, const gomgpu_forcing *forcing);
+/* Run one stage (for per-stage parity) or the whole step. Both are asynchronous on the
+ * context's stream except where a stage needs a device-to-host value (none in FP64). */
+int gomgpu_stage(gomgpu_ctx *ctx, int stage_id);
+int gomgpu_step(gomgpu_ctx *ctx);
+/* Advance n steps with the forcing already uploaded (benchmark and ensemble use: the
+ * forcing is held fixed). */
+int gomgpu_run(gomgpu_ctx *ctx, int nsteps);
+/* Synchronise and download the non-NULL fields. */
+int gomgpu_get_state(gomgpu_ctx *ctx, gomgpu_state *state);
+/* Total volume sum(area*(h+eta)) over wet cells and salt sum(S*area*dz) in FP64. */
+int gomgpu_budgets(gomgpu_ctx *ctx, double *volume, double *salt);
+/* Iterations taken by the last PCG solve (0 if it did not converge). */
+int gomgpu_pcg_iterations(gomgpu_ctx *ctx);
+/* Wait for all queued work; returns the latched model error as GOMGPU_E_MODEL. */
+int gomgpu_sync(gomgpu_ctx *ctx);
+/* Text of the last error (static storage of the context). */
+const char *gomgpu_last_error(gomgpu_ctx *ctx);
+/* Per-kernel event timing of the last step: fills up to n (name, milliseconds) pairs. */
+int gomgpu_timings(gomgpu_ctx *ctx, int enable, const char **names, double *ms, int n);
+void gomgpu_destroy(gomgpu_ctx *ctx);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* GOMGPU_H */
