@@ -1,0 +1,120 @@
+// gomgpu_dev.h - device-side context and layout helpers of the HIP step.
+//
+// Device layout: every (layer, entity) field of the reference is stored layer-major with
+// the entity index fastest, d[(k - lo) * N + e] with e 0-based, so that one thread per
+// entity (column kernels) or per (layer, entity) reads consecutive addresses across a
+// wavefront. Connectivity tables (4, E), (2, J), (NB, P) are stored the same way
+// ([slot][entity]). Entity ids inside the tables stay 1-based with 0 = none, as in the
+// reference, so the host uploads them unchanged.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "../../include/gomgpu.h"
+
+#define GOM_LMAX 64  // largest supported maxlayer (per-thread column arrays)
+
+struct DevMesh {
+#define DM_R(name, ext) const double *name;
+#define DM_I(name, ext) const int32_t *name;
+  GOMGPU_MESH_REAL(DM_R)
+  GOMGPU_MESH_INT(DM_I)
+#undef DM_R
+#undef DM_I
+  const double *Cdb_static;  // Manning drag from the still-water depth (bf_flag 1)
+};
+
+struct DevState {
+#define DS_R(name, ext) double *name;
+#define DS_I(name, ext) int32_t *name;
+  GOMGPU_STATE_REAL(DS_R)
+  GOMGPU_STATE_INT(DS_I)
+#undef DS_R
+#undef DS_I
+};
+
+struct DevForcing {
+#define DF_R(name, ext) const double *name;
+  GOMGPU_FORCING_REAL(DF_R)
+#undef DF_R
+};
+
+// Scalars every kernel needs, passed by value.
+struct DevDims {
+  int P, E, J, L, NB, NT, NOB, NQB;
+  int tid[2];
+};
+
+// Work arrays of the step (device-owned, sized at create time).
+struct DevWork {
+  // free surface / PCG (E rows, ELL with the diagonal and 4 neighbour slots)
+  double *fs_diag, *fs_off, *fs_rhs, *pcg_x, *pcg_r, *pcg_d0, *pcg_d1, *pcg_q, *pcg_s, *pcg_minv;
+  double *pcg_part;       // per-block partial sums
+  int *pcg_info;          // [0] iterations of the last solve, [1] converged flag
+  double *face_sdz;       // per face sum_k dz * AinvDeltaZ1 (top down)
+  double *face_sun;       // per face sum_k dz * un
+  double *face_sg;        // per face sum_k dz * AinvG1
+  // transport
+  double *con, *con_new;  // con (0:L+1, E, NT) and con_new (L, E, NT), layer-major
+  double *Qj, *dj, *phij; // (L, J)
+  double *Qi, *di, *phii; // (0:L, E)
+  double *rj, *ri;        // (L, J, NT), (0:L, E, NT)
+  double *PSIj, *PSIi;
+  double *s1a, *s2a;      // (L, E, NT) limiter sums per cell-layer
+  double *obk, *qbk;      // (L, OB, NT), (L, QB, NT)
+  double *node_new;       // (L, P, NT)
+  double *face_new;       // (L, J, NT)
+  // budgets
+  double *budget;         // [0] volume, [1] salt
+  int *err;               // [0] code, [1] entity
+};
+
+// ------------------------------------------------------------------ index helpers
+__device__ __forceinline__ size_t ix(int krow, int e, int n) { return (size_t)krow * n + e; }
+// 1-based layer k of an (L, N) field
+#define AL(a, k, e, n) (a)[ix((k) - 1, (e), (n))]
+// 0-based layer k of an (0:L, N) field
+#define A0(a, k, e, n) (a)[ix((k), (e), (n))]
+// 1-based slot l of a connectivity table [slot][entity]
+#define TB(a, l, e, n) (a)[ix((l) - 1, (e), (n))]
+
+__device__ __forceinline__ int sen(int N, int l, int w) {
+  int v = l + w;
+  return v > N ? v - N : v;
+}
+
+__device__ __forceinline__ double carea(double x1, double x2, double x3, double y1, double y2, double y3) {
+  return fabs((x1 - x3) * (y2 - y3) - (x2 - x3) * (y1 - y3)) / 2.0;
+}
+
+__device__ __forceinline__ double dmaxf(double a, double b) { return a > b ? a : b; }
+__device__ __forceinline__ double dminf(double a, double b) { return a < b ? a : b; }
+
+__device__ __forceinline__ void latch(int *err, int code, int entity) {
+  if (atomicCAS(err, 0, code) == 0) err[1] = entity;
+}
+
+// Kernel argument blocks shared by the launchers and the kernels.
+struct MomParams {
+  double dt, g, th, rho_o, smag, Ah0, Kh0, spinup_bc;
+  int baroclinic;
+};
+
+struct TvdArgs {
+  double dt, th, rho_o, dt2;
+  int Nt, sub, hlim, vlim, heat;
+};
+
+struct PcgArgs {
+  const double *diag, *off, *b;
+  const int32_t *col;     // adj_cellnum_at_cell, [slot][cell], 1-based, 0 = none
+  const int32_t *nfaces;  // tri_or_quad
+  double *x, *r, *d0, *d1, *q, *s, *minv, *partA, *partB;
+  int *info;
+  int n, maxit;
+  double tol;
+};
+
+// Launch helpers
+static inline int nblk(long n, int bs) { return (int)((n + bs - 1) / bs); }

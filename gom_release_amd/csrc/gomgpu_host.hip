@@ -1,0 +1,594 @@
+// gomgpu_host.hip - the C ABI of include/gomgpu.h: context, uploads/downloads in the
+// reference's Fortran layout, and the stage sequence of one time step
+// (GOM_Rv1.1.0/source/main.f90:95-272).
+#include <math.h>
+#include <stdio.h>
+#include <string.h>
+
+#include <string>
+#include <vector>
+
+#include "gomgpu_dev.h"
+
+// kernels (k_hydro.hip, k_transport.hip, k_pcg.hip)
+__global__ void k_friction(DevMesh, DevState, DevDims, int, double, double);
+__global__ void k_elm_substeps(DevMesh, DevState, DevDims, double, double, int, int);
+__global__ void k_elm(DevMesh, DevState, DevDims, double, double, double, int *);
+__global__ void k_elm_copy(DevState, long);
+__global__ void k_momentum(DevMesh, DevState, DevForcing, DevDims, MomParams, int *);
+__global__ void k_fs_face(DevMesh, DevState, DevWork, DevDims, double);
+__global__ void k_fs_cell(DevMesh, DevState, DevForcing, DevWork, DevDims, double, double, int *);
+__global__ void k_eta_node(DevMesh, DevState, DevDims);
+__global__ void k_eta_node_dry(DevMesh, DevState, DevDims);
+__global__ void k_velocity_face(DevMesh, DevState, DevForcing, DevDims, double);
+__global__ void k_velocity_w(DevMesh, DevState, DevDims);
+__global__ void k_layers_cell(DevMesh, DevState, DevDims, double, double, int *);
+__global__ void k_layers_face(DevMesh, DevState, DevDims, double, double, int *);
+__global__ void k_layers_node(DevMesh, DevState, DevDims, double, double, int *);
+__global__ void k_layers_check_cell(DevMesh, DevState, DevDims, double, int *);
+__global__ void k_layers_check_fn(DevMesh, DevState, DevDims, double, int *);
+__global__ void k_face_levels(DevMesh, DevState, DevDims);
+__global__ void k_node_vel(DevMesh, DevState, DevDims);
+__global__ void k_tvd_init(DevMesh, DevState, DevForcing, DevWork, DevDims, TvdArgs);
+__global__ void k_tvd_bnd(DevMesh, DevState, DevForcing, DevWork, DevDims, TvdArgs);
+__global__ void k_tvd_coef_face(DevMesh, DevState, DevWork, DevDims, TvdArgs);
+__global__ void k_tvd_coef_cell(DevMesh, DevState, DevWork, DevDims, TvdArgs);
+__global__ void k_tvd_lim_cell(DevMesh, DevState, DevWork, DevDims);
+__global__ void k_tvd_lim_face(DevMesh, DevState, DevWork, DevDims, int);
+__global__ void k_tvd_psi_cell(DevMesh, DevState, DevWork, DevDims, int);
+__global__ void k_tvd_column(DevMesh, DevState, DevWork, DevDims, TvdArgs, int *);
+__global__ void k_tvd_copyback(DevMesh, DevState, DevWork, DevDims);
+__global__ void k_tvd_node(DevMesh, DevState, DevWork, DevDims);
+__global__ void k_tvd_face_out(DevMesh, DevState, DevWork, DevDims);
+__global__ void k_tvd_cellnode_out(DevState, DevWork, DevDims);
+__global__ void k_eos_nodeface(DevMesh, DevState, DevDims, double, int, int, int *);
+__global__ void k_eos_cell(DevMesh, DevState, DevDims, int *);
+__global__ void k_bars(DevMesh, DevState, DevDims);
+__global__ void k_budget(DevMesh, DevState, DevDims, double *);
+__global__ void k_copy(double *, const double *, int);
+extern "C" hipError_t gom_launch_pcg(const PcgArgs *args, int max_blocks, hipStream_t st);
+extern "C" int gom_pcg_max_blocks(int device);
+
+static const char *kStageNames[GOMGPU_NUM_STAGES] = {"friction", "elm", "momentum", "freesurf", "velocity",
+                                                      "layers", "nodevel", "transport", "eos", "update"};
+
+struct gomgpu_ctx {
+  int device = 0;
+  hipStream_t st = nullptr;
+  gomgpu_params p;
+  DevDims d;
+  DevMesh m;
+  DevState s;
+  DevForcing f;
+  DevWork w;
+  double elapsed = 0.0, spin_tide = 1.0, spin_bc = 1.0;
+  std::vector<void *> allocs;
+  int pcg_blocks = 0;
+  int last_iters = 0;
+  std::string err;
+  bool timing = false;
+  hipEvent_t ev[GOMGPU_NUM_STAGES + 1];
+  double stage_ms[GOMGPU_NUM_STAGES];
+  int *h_err = nullptr;  // pinned
+  bool pcg_ran = false;  // a free-surface solve ran since the last sync
+};
+
+static int seterr(gomgpu_ctx *c, int code, const std::string &msg) {
+  if (c) c->err = msg;
+  return code;
+}
+
+#define HIPCHK(ctx, call)                                                                     \
+  do {                                                                                        \
+    hipError_t e_ = (call);                                                                   \
+    if (e_ != hipSuccess)                                                                     \
+      return seterr(ctx, GOMGPU_E_HIP, std::string(#call) + ": " + hipGetErrorString(e_));    \
+  } while (0)
+
+// ------------------------------------------------------------------ shapes
+struct Shape {
+  long ld, n;  // Fortran leading dimension (slots/layers) and entity count
+};
+
+static Shape shape_of(const char *code, const gomgpu_params &p) {
+  long P = p.maxnod, E = p.maxele, J = p.maxface, L = p.maxlayer, NB = p.max_no_neighbor_node;
+  long OB = p.num_ob_cell > 0 ? p.num_ob_cell : 1, QB = p.num_Qb_cell > 0 ? p.num_Qb_cell : 1;
+  long NT = p.maxtran2 > 0 ? p.maxtran2 : 1, Nt = p.trans_sub_iter > 0 ? p.trans_sub_iter : 1;
+  std::string c(code);
+  if (c == "P") return {1, P};
+  if (c == "E") return {1, E};
+  if (c == "J") return {1, J};
+  if (c == "L") return {1, L};
+  if (c == "L1") return {1, L + 1};
+  if (c == "4E") return {4, E};
+  if (c == "2J") return {2, J};
+  if (c == "NBP") return {NB, P};
+  if (c == "NB1P") return {NB + 1, P};
+  if (c == "OB") return {1, OB};
+  if (c == "QB") return {1, QB};
+  if (c == "LJ") return {L, J};
+  if (c == "L1J") return {L + 1, J};
+  if (c == "LE") return {L, E};
+  if (c == "L1E") return {L + 1, E};
+  if (c == "LP") return {L, P};
+  if (c == "L1P") return {L + 1, P};
+  if (c == "NTOB") return {1, Nt * NT * OB};
+  if (c == "NTQB") return {1, Nt * NT * QB};
+  return {0, 0};
+}
+
+template <typename T>
+static void to_device_layout(const T *src, T *dst, Shape sh) {
+  // Fortran (ld, n) column-major -> [slot][entity]
+  for (long e = 0; e < sh.n; e++)
+    for (long l = 0; l < sh.ld; l++) dst[l * sh.n + e] = src[l + e * sh.ld];
+}
+
+template <typename T>
+static void from_device_layout(const T *src, T *dst, Shape sh) {
+  for (long e = 0; e < sh.n; e++)
+    for (long l = 0; l < sh.ld; l++) dst[l + e * sh.ld] = src[l * sh.n + e];
+}
+
+template <typename T>
+static int dalloc(gomgpu_ctx *c, T **ptr, size_t count) {
+  void *p = nullptr;
+  if (count == 0) count = 1;
+  if (hipMalloc(&p, count * sizeof(T)) != hipSuccess) return seterr(c, GOMGPU_E_NOMEM, "hipMalloc failed");
+  if (hipMemset(p, 0, count * sizeof(T)) != hipSuccess) return seterr(c, GOMGPU_E_HIP, "hipMemset failed");
+  c->allocs.push_back(p);
+  *ptr = (T *)p;
+  return 0;
+}
+
+template <typename T>
+static int upload(gomgpu_ctx *c, T *dptr, const T *host, Shape sh) {
+  std::vector<T> tmp((size_t)(sh.ld * sh.n));
+  to_device_layout(host, tmp.data(), sh);
+  HIPCHK(c, hipMemcpyAsync(dptr, tmp.data(), tmp.size() * sizeof(T), hipMemcpyHostToDevice, c->st));
+  HIPCHK(c, hipStreamSynchronize(c->st));
+  return 0;
+}
+
+template <typename T>
+static int download(gomgpu_ctx *c, T *host, const T *dptr, Shape sh) {
+  std::vector<T> tmp((size_t)(sh.ld * sh.n));
+  HIPCHK(c, hipMemcpyAsync(tmp.data(), dptr, tmp.size() * sizeof(T), hipMemcpyDeviceToHost, c->st));
+  HIPCHK(c, hipStreamSynchronize(c->st));
+  from_device_layout(tmp.data(), host, sh);
+  return 0;
+}
+
+// ------------------------------------------------------------------ create
+static int validate(const gomgpu_params *p, std::string &why) {
+  if (p->maxnod <= 0 || p->maxele <= 0 || p->maxface <= 0) return why = "empty mesh", 1;
+  if (p->maxlayer < 1 || p->maxlayer > GOM_LMAX) return why = "maxlayer must be in [1, 64]", 1;
+  if (p->max_no_neighbor_node < 1) return why = "max_no_neighbor_node < 1", 1;
+  if (p->transport_flag == 1) {
+    if (p->maxtran2 < 1 || p->maxtran2 > 2) return why = "maxtran2 must be 1 or 2", 1;
+    if (p->tran_id1 != 1)
+      return why = "the first transported tracer must be salinity (the reference copies the "
+                   "temperature result from tracer slot 2)", 1;
+    if (p->maxtran2 == 2 && p->tran_id2 != 2) return why = "tracer 2 must be temperature", 1;
+    if (p->heat_option == 2) return why = "heat_option 2 is not supported", 1;
+  }
+  if (p->baroclinic_flag == 1 && p->ana_density != 0 && p->ana_density != 1)
+    return why = "ana_density must be 0 or 1", 1;
+  if (p->precision != GOMGPU_FP64 && p->precision != GOMGPU_MIXED) return why = "bad precision", 1;
+  return 0;
+}
+
+extern "C" int gomgpu_create(const gomgpu_mesh *mesh, const gomgpu_params *params, gomgpu_ctx **out) {
+  if (!mesh || !params || !out) return GOMGPU_E_ARG;
+  *out = nullptr;
+  std::string why;
+  if (validate(params, why)) {
+    fprintf(stderr, "gomgpu_create: %s\n", why.c_str());
+    return GOMGPU_E_ARG;
+  }
+  gomgpu_ctx *c = new gomgpu_ctx();
+  c->p = *params;
+  const gomgpu_params &p = c->p;
+  int rc;
+  if (hipGetDevice(&c->device) != hipSuccess || hipStreamCreateWithFlags(&c->st, hipStreamNonBlocking) != hipSuccess) {
+    delete c;
+    return GOMGPU_E_HIP;
+  }
+  c->d.P = p.maxnod;
+  c->d.E = p.maxele;
+  c->d.J = p.maxface;
+  c->d.L = p.maxlayer;
+  c->d.NB = p.max_no_neighbor_node;
+  c->d.NT = p.transport_flag == 1 ? p.maxtran2 : 0;
+  c->d.NOB = p.num_ob_cell;
+  c->d.NQB = p.num_Qb_cell;
+  c->d.tid[0] = p.tran_id1;
+  c->d.tid[1] = p.tran_id2;
+  // mesh
+#define UP_MR(name, ext)                                                             \
+  {                                                                                  \
+    Shape sh = shape_of(#ext, p);                                                    \
+    double *dp;                                                                      \
+    if ((rc = dalloc(c, &dp, sh.ld * sh.n))) goto fail;                              \
+    if (mesh->name && (rc = upload(c, dp, mesh->name, sh))) goto fail;               \
+    c->m.name = dp;                                                                  \
+  }
+#define UP_MI(name, ext)                                                             \
+  {                                                                                  \
+    Shape sh = shape_of(#ext, p);                                                    \
+    int32_t *dp;                                                                     \
+    if ((rc = dalloc(c, &dp, sh.ld * sh.n))) goto fail;                              \
+    if (mesh->name && (rc = upload(c, dp, mesh->name, sh))) goto fail;               \
+    c->m.name = dp;                                                                  \
+  }
+  GOMGPU_MESH_REAL(UP_MR)
+  GOMGPU_MESH_INT(UP_MI)
+#undef UP_MR
+#undef UP_MI
+  {
+    // Manning drag of the still-water face depth (calculate_bottom_friction.f90:27-37),
+    // evaluated with the host libm pow as the reference does.
+    std::vector<double> cdb(p.maxface, 0.0);
+    if (p.bf_flag == 1) {
+      const double expo = 1.0 / 6.0;
+      for (int j = 0; j < p.maxface; j++) {
+        double hr = 1.0 > mesh->h_face[j] ? 1.0 : mesh->h_face[j];
+        double chezy = pow(hr, expo) / mesh->manning[j];
+        cdb[j] = p.gravity / (chezy * chezy);
+      }
+    }
+    double *dp;
+    if ((rc = dalloc(c, &dp, p.maxface))) goto fail;
+    if ((rc = upload(c, dp, cdb.data(), Shape{1, p.maxface}))) goto fail;
+    c->m.Cdb_static = dp;
+  }
+  // state
+#define AL_SR(name, ext)                                                             \
+  {                                                                                  \
+    Shape sh = shape_of(#ext, p);                                                    \
+    if ((rc = dalloc(c, &c->s.name, sh.ld * sh.n))) goto fail;                       \
+  }
+  GOMGPU_STATE_REAL(AL_SR)
+  GOMGPU_STATE_INT(AL_SR)
+#undef AL_SR
+  // forcing
+#define AL_FR(name, ext)                                                             \
+  {                                                                                  \
+    Shape sh = shape_of(#ext, p);                                                    \
+    double *dp;                                                                      \
+    if ((rc = dalloc(c, &dp, sh.ld * sh.n))) goto fail;                              \
+    c->f.name = dp;                                                                  \
+  }
+  GOMGPU_FORCING_REAL(AL_FR)
+#undef AL_FR
+  {
+    long E = p.maxele, J = p.maxface, P = p.maxnod, L = p.maxlayer;
+    long NT = c->d.NT > 0 ? c->d.NT : 1, OB = p.num_ob_cell > 0 ? p.num_ob_cell : 1,
+         QB = p.num_Qb_cell > 0 ? p.num_Qb_cell : 1;
+    c->pcg_blocks = gom_pcg_max_blocks(c->device);
+    if (c->pcg_blocks <= 0) {
+      rc = seterr(c, GOMGPU_E_HIP, "cannot size the PCG grid");
+      goto fail;
+    }
+    DevWork &w = c->w;
+    if ((rc = dalloc(c, &w.fs_diag, E)) || (rc = dalloc(c, &w.fs_off, 4 * E)) || (rc = dalloc(c, &w.fs_rhs, E)) ||
+        (rc = dalloc(c, &w.pcg_x, E)) || (rc = dalloc(c, &w.pcg_r, E)) || (rc = dalloc(c, &w.pcg_d0, E)) ||
+        (rc = dalloc(c, &w.pcg_d1, E)) || (rc = dalloc(c, &w.pcg_q, E)) || (rc = dalloc(c, &w.pcg_s, E)) ||
+        (rc = dalloc(c, &w.pcg_minv, E)) || (rc = dalloc(c, &w.pcg_part, 2 * (long)c->pcg_blocks)) ||
+        (rc = dalloc(c, &w.pcg_info, 2)) || (rc = dalloc(c, &w.face_sdz, J)) || (rc = dalloc(c, &w.face_sun, J)) ||
+        (rc = dalloc(c, &w.face_sg, J)))
+      goto fail;
+    if ((rc = dalloc(c, &w.con, NT * (L + 2) * E)) || (rc = dalloc(c, &w.con_new, NT * L * E)) ||
+        (rc = dalloc(c, &w.Qj, L * J)) || (rc = dalloc(c, &w.dj, L * J)) || (rc = dalloc(c, &w.phij, L * J)) ||
+        (rc = dalloc(c, &w.Qi, (L + 1) * E)) || (rc = dalloc(c, &w.di, (L + 1) * E)) ||
+        (rc = dalloc(c, &w.phii, (L + 1) * E)) || (rc = dalloc(c, &w.rj, NT * L * J)) ||
+        (rc = dalloc(c, &w.ri, NT * (L + 1) * E)) || (rc = dalloc(c, &w.PSIj, NT * L * J)) ||
+        (rc = dalloc(c, &w.PSIi, NT * (L + 1) * E)) || (rc = dalloc(c, &w.s1a, NT * L * E)) ||
+        (rc = dalloc(c, &w.s2a, NT * L * E)) || (rc = dalloc(c, &w.obk, NT * L * OB)) ||
+        (rc = dalloc(c, &w.qbk, NT * L * QB)) || (rc = dalloc(c, &w.node_new, NT * L * P)) ||
+        (rc = dalloc(c, &w.face_new, NT * L * J)) || (rc = dalloc(c, &w.budget, 2 * ((E + 255) / 256) + 2)) ||
+        (rc = dalloc(c, &w.err, 2)))
+      goto fail;
+  }
+  if (hipHostMalloc((void **)&c->h_err, 4 * sizeof(int)) != hipSuccess) {
+    rc = seterr(c, GOMGPU_E_NOMEM, "hipHostMalloc failed");
+    goto fail;
+  }
+  for (int i = 0; i <= GOMGPU_NUM_STAGES; i++) hipEventCreate(&c->ev[i]);
+  *out = c;
+  return 0;
+fail:
+  fprintf(stderr, "gomgpu_create: %s\n", c->err.c_str());
+  gomgpu_destroy(c);
+  return rc ? rc : GOMGPU_E_HIP;
+}
+
+extern "C" void gomgpu_destroy(gomgpu_ctx *c) {
+  if (!c) return;
+  if (c->st) hipStreamSynchronize(c->st);
+  for (void *p : c->allocs) hipFree(p);
+  if (c->h_err) hipHostFree(c->h_err);
+  for (int i = 0; i <= GOMGPU_NUM_STAGES; i++)
+    if (c->ev[i]) hipEventDestroy(c->ev[i]);
+  if (c->st) hipStreamDestroy(c->st);
+  delete c;
+}
+
+extern "C" int gomgpu_set_state(gomgpu_ctx *c, const gomgpu_state *st) {
+  if (!c || !st) return GOMGPU_E_ARG;
+  const gomgpu_params &p = c->p;
+  int rc;
+#define UP_S(name, ext)                                                              \
+  if (st->name) {                                                                    \
+    if ((rc = upload(c, c->s.name, st->name, shape_of(#ext, p)))) return rc;         \
+  }
+  GOMGPU_STATE_REAL(UP_S)
+  GOMGPU_STATE_INT(UP_S)
+#undef UP_S
+  return 0;
+}
+
+extern "C" int gomgpu_get_state(gomgpu_ctx *c, gomgpu_state *st) {
+  if (!c || !st) return GOMGPU_E_ARG;
+  const gomgpu_params &p = c->p;
+  int rc;
+  HIPCHK(c, hipStreamSynchronize(c->st));
+#define DN_S(name, ext)                                                              \
+  if (st->name) {                                                                    \
+    if ((rc = download(c, st->name, c->s.name, shape_of(#ext, p)))) return rc;       \
+  }
+  GOMGPU_STATE_REAL(DN_S)
+  GOMGPU_STATE_INT(DN_S)
+#undef DN_S
+  return 0;
+}
+
+extern "C" int gomgpu_set_forcing(gomgpu_ctx *c, const gomgpu_forcing *f) {
+  if (!c || !f) return GOMGPU_E_ARG;
+  const gomgpu_params &p = c->p;
+  int rc;
+  c->elapsed = f->elapsed_time;
+  c->spin_tide = f->spinup_function_tide;
+  c->spin_bc = f->spinup_function_baroclinic;
+#define UP_F(name, ext)                                                                      \
+  if (f->name) {                                                                             \
+    if ((rc = upload(c, (double *)c->f.name, f->name, shape_of(#ext, p)))) return rc;        \
+  }
+  GOMGPU_FORCING_REAL(UP_F)
+#undef UP_F
+  return 0;
+}
+
+// ------------------------------------------------------------------ stages
+static int stage_impl(gomgpu_ctx *c, int id) {
+  const gomgpu_params &p = c->p;
+  const DevDims &d = c->d;
+  const int BS = 256;
+  hipStream_t st = c->st;
+  int *err = c->w.err;
+  switch (id) {
+    case GOMGPU_STAGE_FRICTION:
+      k_friction<<<nblk(d.J, BS), BS, 0, st>>>(c->m, c->s, d, p.bf_flag, p.dry_depth, p.von_Karman);
+      break;
+    case GOMGPU_STAGE_ELM:
+      if (p.advection_flag == 0) {
+        long n = (long)d.L * d.J;
+        k_elm_copy<<<nblk(n, BS), BS, 0, st>>>(c->s, n);
+      } else {
+        if (p.ELM_backtrace_flag == 1)
+          k_elm_substeps<<<nblk((long)d.P * d.L, BS), BS, 0, st>>>(c->m, c->s, d, p.dt, p.delta_z_min,
+                                                                   p.ELM_min_iter, p.ELM_max_iter);
+        k_elm<<<nblk((long)d.J * d.L, 128), 128, 0, st>>>(c->m, c->s, d, p.MSL, p.dt, p.adv_onoff_depth, err);
+      }
+      break;
+    case GOMGPU_STAGE_MOMENTUM: {
+      MomParams q{p.dt, p.gravity, p.theta, p.rho_o, p.smagorinsky_parameter, p.Ah_0, p.Kh_0, c->spin_bc,
+                  p.baroclinic_flag};
+      k_momentum<<<nblk(d.J, 128), 128, 0, st>>>(c->m, c->s, c->f, d, q, err);
+      break;
+    }
+    case GOMGPU_STAGE_FREESURF: {
+      double gdt2th2 = p.gravity * (p.dt * p.dt) * (p.theta * p.theta);
+      k_fs_face<<<nblk(d.J, BS), BS, 0, st>>>(c->m, c->s, c->w, d, gdt2th2);
+      k_fs_cell<<<nblk(d.E, BS), BS, 0, st>>>(c->m, c->s, c->f, c->w, d, p.dt, p.theta, err);
+      PcgArgs a;
+      a.diag = c->w.fs_diag;
+      a.off = c->w.fs_off;
+      a.b = c->w.fs_rhs;
+      a.col = c->m.adj_cellnum_at_cell;
+      a.nfaces = c->m.tri_or_quad;
+      a.x = c->w.pcg_x;
+      a.r = c->w.pcg_r;
+      a.d0 = c->w.pcg_d0;
+      a.d1 = c->w.pcg_d1;
+      a.q = c->w.pcg_q;
+      a.s = c->w.pcg_s;
+      a.minv = c->w.pcg_minv;
+      a.partA = c->w.pcg_part;
+      a.partB = c->w.pcg_part + c->pcg_blocks;
+      a.info = c->w.pcg_info;
+      a.n = d.E;
+      a.maxit = p.max_iteration_pcg;
+      a.tol = p.error_tolerance;
+      HIPCHK(c, gom_launch_pcg(&a, c->pcg_blocks, st));
+      c->pcg_ran = true;
+      k_copy<<<nblk(d.E, BS), BS, 0, st>>>(c->s.eta_cell_new, c->w.pcg_x, d.E);
+      k_eta_node<<<nblk(d.P, BS), BS, 0, st>>>(c->m, c->s, d);
+      k_eta_node_dry<<<nblk(d.P, BS), BS, 0, st>>>(c->m, c->s, d);
+      break;
+    }
+    case GOMGPU_STAGE_VELOCITY:
+      k_velocity_face<<<nblk(d.J, BS), BS, 0, st>>>(c->m, c->s, c->f, d, p.theta * p.gravity * p.dt);
+      k_velocity_w<<<nblk(d.E, BS), BS, 0, st>>>(c->m, c->s, d);
+      break;
+    case GOMGPU_STAGE_LAYERS:
+      k_layers_cell<<<nblk(d.E, BS), BS, 0, st>>>(c->m, c->s, d, p.MSL, p.dry_depth, err);
+      k_layers_face<<<nblk(d.J, BS), BS, 0, st>>>(c->m, c->s, d, p.MSL, p.dry_depth, err);
+      k_layers_node<<<nblk(d.P, BS), BS, 0, st>>>(c->m, c->s, d, p.MSL, p.dry_depth, err);
+      k_layers_check_cell<<<nblk(d.E, BS), BS, 0, st>>>(c->m, c->s, d, p.denominator_min_for_matrix, err);
+      k_layers_check_fn<<<nblk((long)d.J + d.P, BS), BS, 0, st>>>(c->m, c->s, d, p.denominator_min_for_matrix, err);
+      break;
+    case GOMGPU_STAGE_NODEVEL:
+      k_face_levels<<<nblk(d.J, BS), BS, 0, st>>>(c->m, c->s, d);
+      k_node_vel<<<nblk((long)d.P * (d.L + 1), BS), BS, 0, st>>>(c->m, c->s, d);
+      break;
+    case GOMGPU_STAGE_TRANSPORT: {
+      if (p.transport_flag != 1 || d.NT == 0) break;
+      TvdArgs a;
+      a.Nt = p.trans_sub_iter > 0 ? p.trans_sub_iter : 1;
+      a.dt = p.dt;
+      a.dt2 = p.dt / a.Nt;
+      a.th = p.theta;
+      a.rho_o = p.rho_o;
+      a.hlim = p.h_flux_limiter;
+      a.vlim = p.v_flux_limiter;
+      a.heat = p.heat_option;
+      a.sub = 0;
+      long nE2 = (long)d.E * (d.L + 2);
+      HIPCHK(c, hipMemsetAsync(c->w.con_new, 0, sizeof(double) * (size_t)d.NT * d.L * d.E, st));
+      k_tvd_init<<<nblk(nE2, BS), BS, 0, st>>>(c->m, c->s, c->f, c->w, d, a);
+      for (int sub = 0; sub < a.Nt; sub++) {
+        a.sub = sub;
+        long nb = (long)d.L * (d.NOB + d.NQB);
+        if (nb > 0) k_tvd_bnd<<<nblk(nb, BS), BS, 0, st>>>(c->m, c->s, c->f, c->w, d, a);
+        k_tvd_coef_face<<<nblk((long)d.J * d.L, BS), BS, 0, st>>>(c->m, c->s, c->w, d, a);
+        k_tvd_coef_cell<<<nblk((long)d.E * (d.L + 1), BS), BS, 0, st>>>(c->m, c->s, c->w, d, a);
+        if (p.h_flux_limiter > 0 || p.v_flux_limiter > 0) {
+          k_tvd_lim_cell<<<nblk(d.E, 128), 128, 0, st>>>(c->m, c->s, c->w, d);
+          k_tvd_lim_face<<<nblk((long)d.J * d.L, BS), BS, 0, st>>>(c->m, c->s, c->w, d, p.h_flux_limiter);
+          k_tvd_psi_cell<<<nblk((long)d.E * (d.L + 1), BS), BS, 0, st>>>(c->m, c->s, c->w, d, p.h_flux_limiter);
+        }
+        k_tvd_column<<<nblk(d.E, 128), 128, 0, st>>>(c->m, c->s, c->w, d, a, err);
+        k_tvd_copyback<<<nblk(nE2, BS), BS, 0, st>>>(c->m, c->s, c->w, d);
+      }
+      k_tvd_node<<<nblk(d.P, BS), BS, 0, st>>>(c->m, c->s, c->w, d);
+      k_tvd_face_out<<<nblk((long)d.J * d.L, BS), BS, 0, st>>>(c->m, c->s, c->w, d);
+      long nmax = (long)d.L * (d.E > d.P ? d.E : d.P);
+      k_tvd_cellnode_out<<<nblk(nmax, BS), BS, 0, st>>>(c->s, c->w, d);
+      break;
+    }
+    case GOMGPU_STAGE_EOS:
+      if (p.baroclinic_flag != 1) break;
+      k_eos_nodeface<<<nblk((long)d.P + d.J, 128), 128, 0, st>>>(c->m, c->s, d, p.MSL, p.ana_density == 0,
+                                                                 p.i_density_flag, err);
+      k_eos_cell<<<nblk(d.E, BS), BS, 0, st>>>(c->m, c->s, d, err);
+      break;
+    case GOMGPU_STAGE_UPDATE: {
+      // update_variables.f90:16-31
+      size_t E = d.E, J = d.J, P = d.P, L = d.L;
+      struct {
+        double *dst;
+        const double *src;
+        size_t n;
+      } cp[] = {{c->s.eta_cell, c->s.eta_cell_new, E},
+                {c->s.wn_cell, c->s.wn_cell_new, (L + 1) * E},
+                {c->s.dz_cell, c->s.dz_cell_new, L * E},
+                {c->s.dzhalf_cell, c->s.dzhalf_cell_new, (L + 1) * E},
+                {c->s.salt_cell, c->s.salt_cell_new, L * E},
+                {c->s.temp_cell, c->s.temp_cell_new, L * E},
+                {c->s.un_face, c->s.un_face_new, L * J},
+                {c->s.vn_face, c->s.vn_face_new, L * J},
+                {c->s.dz_face, c->s.dz_face_new, L * J},
+                {c->s.dzhalf_face, c->s.dzhalf_face_new, (L + 1) * J},
+                {c->s.dz_node, c->s.dz_node_new, L * P},
+                {c->s.dzhalf_node, c->s.dzhalf_node_new, (L + 1) * P}};
+      for (auto &x : cp) HIPCHK(c, hipMemcpyAsync(x.dst, x.src, x.n * sizeof(double), hipMemcpyDeviceToDevice, st));
+      k_bars<<<nblk(d.P, BS), BS, 0, st>>>(c->m, c->s, d);
+      break;
+    }
+    default:
+      return seterr(c, GOMGPU_E_ARG, "bad stage id");
+  }
+  HIPCHK(c, hipGetLastError());
+  return 0;
+}
+
+extern "C" int gomgpu_stage(gomgpu_ctx *c, int id) {
+  if (!c) return GOMGPU_E_ARG;
+  return stage_impl(c, id);
+}
+
+extern "C" int gomgpu_step(gomgpu_ctx *c) {
+  if (!c) return GOMGPU_E_ARG;
+  if (c->timing) HIPCHK(c, hipEventRecord(c->ev[0], c->st));
+  for (int id = 0; id < GOMGPU_NUM_STAGES; id++) {
+    int rc = stage_impl(c, id);
+    if (rc) return rc;
+    if (c->timing) HIPCHK(c, hipEventRecord(c->ev[id + 1], c->st));
+  }
+  if (c->timing) {
+    HIPCHK(c, hipEventSynchronize(c->ev[GOMGPU_NUM_STAGES]));
+    for (int id = 0; id < GOMGPU_NUM_STAGES; id++) {
+      float ms = 0;
+      hipEventElapsedTime(&ms, c->ev[id], c->ev[id + 1]);
+      c->stage_ms[id] = ms;
+    }
+  }
+  return 0;
+}
+
+extern "C" int gomgpu_run(gomgpu_ctx *c, int nsteps) {
+  for (int i = 0; i < nsteps; i++) {
+    int rc = gomgpu_step(c);
+    if (rc) return rc;
+  }
+  return 0;
+}
+
+extern "C" int gomgpu_sync(gomgpu_ctx *c) {
+  if (!c) return GOMGPU_E_ARG;
+  HIPCHK(c, hipMemcpyAsync(c->h_err, c->w.err, 2 * sizeof(int), hipMemcpyDeviceToHost, c->st));
+  HIPCHK(c, hipMemcpyAsync(c->h_err + 2, c->w.pcg_info, 2 * sizeof(int), hipMemcpyDeviceToHost, c->st));
+  HIPCHK(c, hipStreamSynchronize(c->st));
+  bool ran = c->pcg_ran;
+  c->pcg_ran = false;
+  if (ran) c->last_iters = c->h_err[2];
+  if (c->h_err[0] != 0) {
+    char buf[160];
+    snprintf(buf, sizeof buf, "model error %d at entity %d", c->h_err[0], c->h_err[1]);
+    c->err = buf;
+    return GOMGPU_E_MODEL;
+  }
+  if (ran && c->h_err[3] == 0) {
+    c->err = "PCG did not converge within max_iteration_pcg";
+    return GOMGPU_W_PCG_NOT_CONVERGED;
+  }
+  return 0;
+}
+
+extern "C" int gomgpu_pcg_iterations(gomgpu_ctx *c) {
+  if (!c) return -1;
+  gomgpu_sync(c);
+  return c->last_iters;
+}
+
+extern "C" int gomgpu_budgets(gomgpu_ctx *c, double *volume, double *salt) {
+  if (!c || !volume || !salt) return GOMGPU_E_ARG;
+  int nb = nblk(c->d.E, 256);
+  k_budget<<<nb, 256, 0, c->st>>>(c->m, c->s, c->d, c->w.budget);
+  HIPCHK(c, hipGetLastError());
+  std::vector<double> part(2 * (size_t)nb);
+  HIPCHK(c, hipMemcpyAsync(part.data(), c->w.budget, part.size() * sizeof(double), hipMemcpyDeviceToHost, c->st));
+  HIPCHK(c, hipStreamSynchronize(c->st));
+  double v = 0.0, s = 0.0;
+  for (int b = 0; b < nb; b++) {
+    v += part[2 * b];
+    s += part[2 * b + 1];
+  }
+  *volume = v;
+  *salt = s;
+  return 0;
+}
+
+extern "C" const char *gomgpu_last_error(gomgpu_ctx *c) { return c ? c->err.c_str() : "null context"; }
+
+extern "C" int gomgpu_timings(gomgpu_ctx *c, int enable, const char **names, double *ms, int n) {
+  if (!c) return GOMGPU_E_ARG;
+  c->timing = enable != 0;
+  int k = 0;
+  for (; k < n && k < GOMGPU_NUM_STAGES; k++) {
+    if (names) names[k] = kStageNames[k];
+    if (ms) ms[k] = c->stage_ms[k];
+  }
+  return k;
+}

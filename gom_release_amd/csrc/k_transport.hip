@@ -1,0 +1,660 @@
+// k_transport.hip - TVD scalar transport (solve_transport_equation_v19.f90), equation of
+// state (calculate_density_full.f90 / calculate_density_linear.f90), the new->old update
+// (update_variables.f90) and FP64 budgets. Paths relative to GOM_Rv1.1.0/source.
+//
+// Work arrays are layer-major with the entity fastest. Reads the reference makes of its
+// automatic arrays outside their written ranges are defined as zero here (coefficients
+// are written for every layer, zero outside the active range).
+#include "gomgpu_dev.h"
+
+
+__device__ __forceinline__ size_t cix(int t, int k, int e, int L2, int n) {  // con (0:L+1)
+  return ((size_t)t * L2 + k) * n + e;
+}
+__device__ __forceinline__ size_t nix(int t, int k, int e, int L, int n) {  // (L) fields, k 1-based
+  return ((size_t)t * L + (k - 1)) * n + e;
+}
+__device__ __forceinline__ size_t zix(int t, int k, int e, int L1, int n) {  // (0:L) fields
+  return ((size_t)t * L1 + k) * n + e;
+}
+
+// con_cell from salt/temp (+ surface heat source), solve_transport_equation_v19.f90:88-173
+__global__ void k_tvd_init(DevMesh m, DevState s, DevForcing f, DevWork w, DevDims d, TvdArgs a) {
+  int q = blockIdx.x * blockDim.x + threadIdx.x;
+  const int E = d.E, L = d.L, L2 = d.L + 2;
+  if (q >= E * L2) return;
+  int e = q % E, k = q / E;
+  for (int t = 0; t < d.NT; t++) {
+    double v = 0.0;
+    if (k >= 1 && k <= L) {
+      const double *src = d.tid[t] == 1 ? s.salt_cell : s.temp_cell;
+      v = AL(src, k, e, E);
+      if (d.tid[t] == 2 && a.heat == 1 && f.phi_sz) {
+        int tl = s.top_layer_at_element[e], bl = m.bottom_layer_at_element[e];
+        if (k >= bl && k <= tl) {
+          const double cp = 4186.0;
+          v = v + a.dt * AL(f.phi_sz, k, e, E) / (a.rho_o * cp * AL(s.dz_cell, k, e, E));
+        }
+      }
+    }
+    w.con[cix(t, k, e, L2, E)] = v;
+  }
+}
+
+// Boundary concentrations of open-boundary and river cells for sub-cycle a.sub
+// (solve_transport_equation_v19.f90:291-397); one thread per (layer, boundary cell).
+__global__ void k_tvd_bnd(DevMesh m, DevState s, DevForcing f, DevWork w, DevDims d, TvdArgs a) {
+  int q = blockIdx.x * blockDim.x + threadIdx.x;
+  const int L = d.L, E = d.E, J = d.J, L2 = d.L + 2;
+  int nob = d.NOB, nqb = d.NQB;
+  if (q < L * nob) {
+    int i = q % nob, k = q / nob + 1;
+    int i2 = m.ob_cell_id[i] - 1;
+    int tl = s.top_layer_at_element[i2], bl = m.bottom_layer_at_element[i2];
+    double v[2] = {0.0, 0.0};
+    if (k >= bl && k <= tl) {
+      int N = m.tri_or_quad[i2];
+      for (int l = 1; l <= N; l++) {
+        int j2 = TB(m.facenum_at_cell, l, i2, E) - 1;
+        if (m.boundary_type_of_face[j2] > 0) {
+          double dzf = AL(s.dz_face, k, j2, J);
+          double dzo = dzf + (AL(s.dz_face_new, k, j2, J) - dzf) * a.sub / a.Nt;
+          double sg = (double)TB(m.sign_in_outflow, l, i2, E);
+          double Q2 = m.face_length[j2] * dzo *
+                      ((1.0 - a.th) * AL(s.un_face, k, j2, J) * sg + a.th * AL(s.un_face_new, k, j2, J) * sg);
+          for (int t = 0; t < d.NT; t++)
+            v[t] = (Q2 <= 0.0) ? f.con_at_ob[a.sub + (size_t)a.Nt * (t + (size_t)d.NT * i)]
+                               : w.con[cix(t, k, i2, L2, E)];
+        }
+      }
+    }
+    for (int t = 0; t < d.NT; t++) w.obk[nix(t, k, i, L, nob)] = v[t];
+  } else if (q < L * nob + L * nqb) {
+    int r = q - L * nob;
+    int i = r % nqb, k = r / nqb + 1;
+    int i2 = m.Qb_cell_id[i] - 1;
+    int tl = s.top_layer_at_element[i2], bl = m.bottom_layer_at_element[i2];
+    for (int t = 0; t < d.NT; t++)
+      w.qbk[nix(t, k, i, L, nqb)] =
+          (k >= bl && k <= tl) ? f.con_at_Qb[a.sub + (size_t)a.Nt * (t + (size_t)d.NT * i)] : 0.0;
+  }
+}
+
+// Face coefficients Q, d, phi (solve_transport_equation_v19.f90:552-581)
+__global__ void k_tvd_coef_face(DevMesh m, DevState s, DevWork w, DevDims d, TvdArgs a) {
+  int q = blockIdx.x * blockDim.x + threadIdx.x;
+  const int J = d.J;
+  if (q >= J * d.L) return;
+  int j = q % J, k = q / J + 1;
+  double Q = 0.0, dd = 0.0, phi = 0.0;
+  if (k >= m.bottom_layer_at_face[j] && k <= s.top_layer_at_face[j]) {
+    double dzf = AL(s.dz_face, k, j, J);
+    double dzo = dzf + (AL(s.dz_face_new, k, j, J) - dzf) * a.sub / a.Nt;
+    Q = m.face_length[j] * dzo * ((1.0 - a.th) * AL(s.un_face, k, j, J) + a.th * AL(s.un_face_new, k, j, J));
+    double D = m.face_length[j] * dzo * AL(s.Kh, k, j, J) / m.delta_j[j];
+    dd = dmaxf(0.0, D - 0.5 * fabs(Q));
+    phi = (Q == 0.0) ? 1.0 : dminf(1.0, 2.0 * D / fabs(Q));
+  }
+  AL(w.Qj, k, j, J) = Q;
+  AL(w.dj, k, j, J) = dd;
+  AL(w.phij, k, j, J) = phi;
+}
+
+// Cell (vertical) coefficients (solve_transport_equation_v19.f90:586-609)
+__global__ void k_tvd_coef_cell(DevMesh m, DevState s, DevWork w, DevDims d, TvdArgs a) {
+  int q = blockIdx.x * blockDim.x + threadIdx.x;
+  const int E = d.E;
+  if (q >= E * (d.L + 1)) return;
+  int e = q % E, k = q / E;
+  double Q = 0.0, dd = 0.0, phi = 0.0;
+  if (k >= m.bottom_layer_at_element[e] && k <= s.top_layer_at_element[e] - 1) {
+    double ar = m.area[e];
+    Q = ar * ((1.0 - a.th) * A0(s.wn_cell, k, e, E) + a.th * A0(s.wn_cell_new, k, e, E));
+    double dh = A0(s.dzhalf_cell, k, e, E);
+    double dzo = dh + (A0(s.dzhalf_cell_new, k, e, E) - dh) * a.sub / a.Nt;
+    double D = ar * A0(s.Kv, k, e, E) / dzo;
+    dd = dmaxf(0.0, D - 0.5 * fabs(Q));
+    phi = (Q == 0.0) ? 1.0 : dminf(1.0, 2.0 * D / fabs(Q));
+  }
+  A0(w.Qi, k, e, E) = Q;
+  A0(w.di, k, e, E) = dd;
+  A0(w.phii, k, e, E) = phi;
+}
+
+// Concentration across face slot l of cell e at layer k (boundary value for no neighbour).
+__device__ __forceinline__ double con_across(const DevMesh &m, const DevWork &w, const DevDims &d, int e,
+                                             int ii, int k, int t) {
+  const int E = d.E, L = d.L, L2 = d.L + 2;
+  if (ii > 0) return w.con[cix(t, k, ii - 1, L2, E)];
+  int iqb = m.Qb_element_flag[e];
+  if (iqb > 0) return w.qbk[nix(t, k, iqb - 1, L, d.NQB)];
+  int iob = m.ob_element_flag[e];
+  if (iob > 0) return w.obk[nix(t, k, iob - 1, L, d.NOB)];
+  return 0.0;
+}
+
+__device__ __forceinline__ bool face_active(const DevMesh &m, int j) {
+  return m.boundary_type_of_face[j] != -1 || m.isflowside3[j] > 0;
+}
+
+__device__ __forceinline__ double limiter(int which, double r, double phi) {
+  double PHI;
+  if (which == 1)
+    PHI = dmaxf(phi, dminf(1.0, r));
+  else if (which == 2)
+    PHI = dmaxf(phi, (r + fabs(r)) / (1.0 + fabs(r)));
+  else
+    PHI = dmaxf(dmaxf(phi, dminf(1.0, 2.0 * r)), dminf(2.0, r));
+  return PHI - phi;
+}
+
+// Limiter ratios per cell column (solve_transport_equation_v19.f90:624-842): the vertical
+// ratios r_ik, and the per-layer upwind sums that the face ratios r_jk need.
+__global__ void k_tvd_lim_cell(DevMesh m, DevState s, DevWork w, DevDims d) {
+  int e = blockIdx.x * blockDim.x + threadIdx.x;
+  const int E = d.E, L = d.L, L1 = d.L + 1, L2 = d.L + 2, J = d.J;
+  if (e >= E) return;
+  int tl = s.top_layer_at_element[e], bl = m.bottom_layer_at_element[e];
+  for (int t = 0; t < d.NT; t++)
+    for (int k = 0; k <= L; k++) w.ri[zix(t, k, e, L1, E)] = 1.0;
+  if (tl == 0) return;
+  int nv = tl - bl + 1, N = m.tri_or_quad[e];
+  double sum1[2] = {0, 0}, sum2[2] = {0, 0};
+  for (int k = bl; k <= tl; k++) {
+    double s1a[2] = {0, 0}, s2a[2] = {0, 0};
+    sum1[0] = sum1[1] = 0.0;
+    sum2[0] = sum2[1] = 0.0;
+    for (int t = 0; t < d.NT; t++) {
+      double c0 = w.con[cix(t, k, e, L2, E)];
+      if (nv != 1) {
+        if (k == bl) {
+          double q = A0(w.Qi, k, e, E);
+          double r2 = fabs(0.5 * (q + fabs(q)));
+          s1a[t] = r2 * (c0 - w.con[cix(t, k + 1, e, L2, E)]);
+          s2a[t] = r2;
+        } else if (k == tl) {
+          double q = A0(w.Qi, k - 1, e, E);
+          double r1 = fabs(0.5 * (q + fabs(q)));
+          s1a[t] = r1 * (c0 - w.con[cix(t, k - 1, e, L2, E)]);
+          s2a[t] = r1;
+        } else {
+          double q1 = A0(w.Qi, k - 1, e, E), q2 = A0(w.Qi, k, e, E);
+          double r1 = fabs(0.5 * (q1 + fabs(q1))), r2 = fabs(0.5 * (q2 + fabs(q2)));
+          s1a[t] = r1 * (c0 - w.con[cix(t, k - 1, e, L2, E)]) + r2 * (c0 - w.con[cix(t, k + 1, e, L2, E)]);
+          s2a[t] = r1 + r2;
+        }
+      }
+    }
+    for (int l = 1; l <= N; l++) {
+      int j = TB(m.facenum_at_cell, l, e, E) - 1, ii = TB(m.adj_cellnum_at_cell, l, e, E);
+      if (face_active(m, j)) {
+        double Q2 = AL(w.Qj, k, j, J) * (double)TB(m.sign_in_outflow, l, e, E);
+        if (Q2 < 0.0) {
+          for (int t = 0; t < d.NT; t++) {
+            double ci = con_across(m, w, d, e, ii, k, t);
+            sum1[t] = sum1[t] + fabs(Q2) * (w.con[cix(t, k, e, L2, E)] - ci);
+            sum2[t] = sum2[t] + fabs(Q2);
+          }
+        }
+      }
+    }
+    for (int t = 0; t < d.NT; t++) {
+      w.s1a[nix(t, k, e, L, E)] = s1a[t] + sum1[t];
+      w.s2a[nix(t, k, e, L, E)] = s2a[t] + sum2[t];
+    }
+  }
+  for (int t = 0; t < d.NT; t++) {
+    if (A0(w.Qi, tl, e, E) >= 0.0) {
+      for (int k = bl; k <= tl - 1; k++) {
+        double qm = A0(w.Qi, k - 1, e, E);
+        double c0 = w.con[cix(t, k, e, L2, E)];
+        double rt1 = sum1[t] + fabs(qm) * (c0 - w.con[cix(t, k - 1, e, L2, E)]);
+        double rt2 = sum2[t] + fabs(qm);
+        double rt = (w.con[cix(t, k + 1, e, L2, E)] - c0) * rt2;
+        w.ri[zix(t, k, e, L1, E)] = (rt == 0.0) ? 1.0 : rt1 / rt;
+      }
+    } else {
+      for (int k = bl + 1; k <= tl; k++) {
+        double qk = A0(w.Qi, k, e, E);
+        double c0 = w.con[cix(t, k, e, L2, E)];
+        double rt1 = sum1[t] + fabs(qk) * (c0 - w.con[cix(t, k + 1, e, L2, E)]);
+        double rt2 = sum2[t] + fabs(qk);
+        double rt = (w.con[cix(t, k - 1, e, L2, E)] - c0) * rt2;
+        w.ri[zix(t, k - 1, e, L1, E)] = (rt == 0.0) ? 1.0 : rt1 / rt;
+      }
+    }
+    w.ri[zix(t, tl, e, L1, E)] = 1.0;
+    w.ri[zix(t, bl - 1, e, L1, E)] = 1.0;
+  }
+}
+
+// Face ratio r_jk and PSI_jk. In the reference each adjacent cell whose outflow the face
+// carries (Q2 >= 0 from its side) writes r_jk in cell order, so the higher cell id wins
+// a tie (Q = 0) -- the same writer is chosen here.
+__global__ void k_tvd_lim_face(DevMesh m, DevState s, DevWork w, DevDims d, int hlim) {
+  int q = blockIdx.x * blockDim.x + threadIdx.x;
+  const int J = d.J, E = d.E, L = d.L, L2 = d.L + 2;
+  if (q >= J * L) return;
+  int j = q % J, k = q / J + 1;
+  double r[2] = {1.0, 1.0};
+  if (face_active(m, j)) {
+    double Q = AL(w.Qj, k, j, J);
+    for (int side = 1; side <= 2; side++) {
+      int c = TB(m.adj_cellnum_at_face, side, j, J);
+      if (c == 0) continue;
+      int e = c - 1;
+      int tl = s.top_layer_at_element[e], bl = m.bottom_layer_at_element[e];
+      if (tl == 0 || k < bl || k > tl) continue;
+      double sg = side == 1 ? 1.0 : -1.0;  // sign_in_outflow of face j seen from this cell
+      if (Q * sg >= 0.0) {
+        int other = TB(m.adj_cellnum_at_face, 3 - side, j, J);
+        for (int t = 0; t < d.NT; t++) {
+          double ci = con_across(m, w, d, e, other, k, t);
+          double rt = w.s2a[nix(t, k, e, L, E)] * (ci - w.con[cix(t, k, e, L2, E)]);
+          r[t] = (rt == 0.0) ? 1.0 : w.s1a[nix(t, k, e, L, E)] / rt;
+        }
+      }
+    }
+  }
+  bool in = (k >= m.bottom_layer_at_face[j] && k <= s.top_layer_at_face[j]);
+  for (int t = 0; t < d.NT; t++) {
+    w.rj[nix(t, k, j, L, J)] = r[t];
+    w.PSIj[nix(t, k, j, L, J)] = (in && hlim != 0) ? limiter(hlim, r[t], AL(w.phij, k, j, J)) : 0.0;
+  }
+}
+
+__global__ void k_tvd_psi_cell(DevMesh m, DevState s, DevWork w, DevDims d, int hlim) {
+  int q = blockIdx.x * blockDim.x + threadIdx.x;
+  const int E = d.E, L1 = d.L + 1;
+  if (q >= E * L1) return;
+  int e = q % E, k = q / E;
+  bool in = (k >= m.bottom_layer_at_element[e] && k <= s.top_layer_at_element[e] - 1);
+  for (int t = 0; t < d.NT; t++)
+    w.PSIi[zix(t, k, e, L1, E)] = (in && hlim != 0) ? limiter(hlim, w.ri[zix(t, k, e, L1, E)], A0(w.phii, k, e, E)) : 0.0;
+}
+
+// Per-column implicit update (solve_transport_equation_v19.f90:921-1303): horizontal
+// upwind advection, diffusion and limiter fluxes explicit; vertical advection explicit;
+// vertical diffusion implicit (Thomas); clamp to [0, 50].
+__global__ void k_tvd_column(DevMesh m, DevState s, DevWork w, DevDims d, TvdArgs a, int *err) {
+  int e = blockIdx.x * blockDim.x + threadIdx.x;
+  const int E = d.E, L = d.L, L1 = d.L + 1, L2 = d.L + 2, J = d.J;
+  if (e >= E) return;
+  int bl = m.bottom_layer_at_element[e], tl = s.top_layer_at_element[e];
+  if (tl == 0) return;
+  const int NT = d.NT, Nt = a.Nt, sub = a.sub;
+  const double dt2 = a.dt2;
+  int nv = tl - bl + 1, N = m.tri_or_quad[e];
+  double ar = m.area[e];
+  double av[GOM_LMAX], bv[GOM_LMAX], cv[GOM_LMAX], x[GOM_LMAX][2];
+  for (int k = bl + 1; k <= tl - 1; k++) {
+    int r = tl - k;
+    av[r] = -dt2 * A0(w.di, k, e, E);
+    cv[r] = -dt2 * A0(w.di, k - 1, e, E);
+    double dz0 = AL(s.dz_cell, k, e, E);
+    double dzn = dz0 + (AL(s.dz_cell_new, k, e, E) - dz0) * (sub + 1) / Nt;
+    bv[r] = -av[r] + ar * dzn - cv[r];
+  }
+  if (tl == bl) {
+    double dz0 = AL(s.dz_cell, tl, e, E);
+    double dzn = dz0 + (AL(s.dz_cell_new, tl, e, E) - dz0) * (sub + 1) / Nt;
+    bv[0] = ar * dzn;
+  } else {
+    cv[0] = -dt2 * A0(w.di, tl - 1, e, E);
+    double dz0 = AL(s.dz_cell, tl, e, E);
+    double dzn = dz0 + (AL(s.dz_cell_new, tl, e, E) - dz0) * (sub + 1) / Nt;
+    bv[0] = ar * dzn - cv[0];
+    av[nv - 1] = -dt2 * A0(w.di, bl, e, E);
+    dz0 = AL(s.dz_cell, bl, e, E);
+    dzn = dz0 + (AL(s.dz_cell_new, bl, e, E) - dz0) * (sub + 1) / Nt;
+    bv[nv - 1] = -av[nv - 1] + ar * dzn;
+  }
+  for (int k = bl; k <= tl; k++) {
+    int r = tl - k;
+    double sqi[2] = {0, 0}, sqii[2] = {0, 0}, shd[2] = {0, 0}, shl[2] = {0, 0};
+    for (int l = 1; l <= N; l++) {
+      int j = TB(m.facenum_at_cell, l, e, E) - 1, ii = TB(m.adj_cellnum_at_cell, l, e, E);
+      if (!face_active(m, j)) continue;
+      double Q2 = AL(w.Qj, k, j, J) * (double)TB(m.sign_in_outflow, l, e, E);
+      double dfj = AL(w.dj, k, j, J);
+      for (int t = 0; t < NT; t++) {
+        double c0 = w.con[cix(t, k, e, L2, E)];
+        double ci = con_across(m, w, d, e, ii, k, t);
+        if (Q2 > 0.0)
+          sqi[t] = sqi[t] + fabs(Q2) * c0;
+        else if (Q2 < 0.0)
+          sqii[t] = sqii[t] + fabs(Q2) * ci;
+        shd[t] = shd[t] + dfj * (ci - c0);
+        shl[t] = shl[t] + w.PSIj[nix(t, k, j, L, J)] * fabs(Q2) * (c0 - ci);
+      }
+    }
+    double dz0 = AL(s.dz_cell, k, e, E);
+    double dzo = dz0 + (AL(s.dz_cell_new, k, e, E) - dz0) * sub / Nt;
+    for (int t = 0; t < NT; t++) {
+      double h_adv = dt2 * (sqii[t] - sqi[t]);
+      double h_diff = dt2 * shd[t];
+      double h_flux = 0.5 * dt2 * shl[t];
+      double rhs = ar * dzo * w.con[cix(t, k, e, L2, E)] + h_adv + h_diff;
+      if (a.hlim > 0) rhs = rhs + h_flux;
+      x[r][t] = rhs;
+    }
+  }
+  if (nv != 1) {
+    for (int k = bl; k <= tl; k++) {
+      int r = tl - k;
+      for (int t = 0; t < NT; t++) {
+        double ck = w.con[cix(t, k, e, L2, E)];
+        double v_adv, vft = 0.0;
+        if (k == bl) {
+          double Q = A0(w.Qi, k, e, E), ckp = w.con[cix(t, k + 1, e, L2, E)];
+          double Ap = fabs(0.5 * (Q + fabs(Q))), Am = fabs(0.5 * (Q - fabs(Q)));
+          v_adv = dt2 * (-(Ap * ck) + Am * ckp);
+          if (a.vlim > 0) {
+            double ps = w.PSIi[zix(t, k, e, L1, E)];
+            vft = 0.5 * dt2 * (-(ps * Ap * (ckp - ck)) + ps * Am * (ck - ckp));
+          }
+        } else if (k == tl) {
+          double Q = A0(w.Qi, k - 1, e, E), ckm = w.con[cix(t, k - 1, e, L2, E)];
+          double Ap = fabs(0.5 * (Q + fabs(Q))), Am = fabs(0.5 * (Q - fabs(Q)));
+          v_adv = dt2 * (Ap * ckm - Am * ck);
+          if (a.vlim > 0) {
+            double ps = w.PSIi[zix(t, k - 1, e, L1, E)];
+            vft = 0.5 * dt2 * (ps * Ap * (ck - ckm) - ps * Am * (ckm - ck));
+          }
+        } else {
+          double Q1 = A0(w.Qi, k - 1, e, E), Q2 = A0(w.Qi, k, e, E);
+          double ckm = w.con[cix(t, k - 1, e, L2, E)], ckp = w.con[cix(t, k + 1, e, L2, E)];
+          double A1p = fabs(0.5 * (Q1 + fabs(Q1))), A1m = fabs(0.5 * (Q1 - fabs(Q1)));
+          double A2p = fabs(0.5 * (Q2 + fabs(Q2))), A2m = fabs(0.5 * (Q2 - fabs(Q2)));
+          v_adv = dt2 * (A1p * ckm - A2p * ck + A2m * ckp - A1m * ck);
+          if (a.vlim > 0) {
+            double p1 = w.PSIi[zix(t, k - 1, e, L1, E)], p2 = w.PSIi[zix(t, k, e, L1, E)];
+            vft = 0.5 * dt2 * (p1 * A1p * (ck - ckm) - p2 * A2p * (ckp - ck) + p2 * A2m * (ck - ckp) - p1 * A1m * (ckm - ck));
+          }
+        }
+        x[r][t] = x[r][t] + v_adv;
+        if (a.vlim > 0) x[r][t] = x[r][t] + vft;
+      }
+    }
+  }
+  // Thomas, tridiagonal_solver.f90:32-61, with NT right-hand sides
+  double gam[GOM_LMAX];
+  if (bv[0] == 0.0) {
+    latch(err, GOMGPU_ERR_TRIDIAG_PIVOT, e + 1);
+    return;
+  }
+  double den = bv[0];
+  for (int t = 0; t < NT; t++) x[0][t] = x[0][t] / den;
+  for (int jr = 1; jr < nv; jr++) {
+    gam[jr] = cv[jr - 1] / den;
+    den = bv[jr] - av[jr] * gam[jr];
+    if (den == 0.0) {
+      latch(err, GOMGPU_ERR_TRIDIAG_PIVOT, e + 1);
+      return;
+    }
+    for (int t = 0; t < NT; t++) x[jr][t] = (x[jr][t] - av[jr] * x[jr - 1][t]) / den;
+  }
+  for (int jr = nv - 2; jr >= 0; jr--)
+    for (int t = 0; t < NT; t++) x[jr][t] = x[jr][t] - gam[jr + 1] * x[jr + 1][t];
+  for (int k = bl; k <= tl; k++) {
+    for (int t = 0; t < NT; t++) {
+      double v = x[tl - k][t];
+      size_t o = nix(t, k, e, L, E);
+      if (v >= 0.0 && v <= 50.0)
+        w.con_new[o] = v;
+      else if (v < 0.0)
+        w.con_new[o] = 0.0;
+      else if (v > 50.0)
+        w.con_new[o] = 50.0;
+    }
+  }
+}
+
+// con <- con_new in the active range, zero elsewhere (solve_transport_equation_v19.f90:1352-1371)
+__global__ void k_tvd_copyback(DevMesh m, DevState s, DevWork w, DevDims d) {
+  int q = blockIdx.x * blockDim.x + threadIdx.x;
+  const int E = d.E, L = d.L, L2 = d.L + 2;
+  if (q >= E * L2) return;
+  int e = q % E, k = q / E;
+  int bl = m.bottom_layer_at_element[e], tl = s.top_layer_at_element[e];
+  for (int t = 0; t < d.NT; t++)
+    w.con[cix(t, k, e, L2, E)] = (k >= bl && k <= tl) ? w.con_new[nix(t, k, e, L, E)] : 0.0;
+}
+
+// Node values: area*dz weighted cell means, top down (solve_transport_equation_v19.f90:1403-1453)
+__global__ void k_tvd_node(DevMesh m, DevState s, DevWork w, DevDims d) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  const int P = d.P, E = d.E, L = d.L;
+  if (i >= P) return;
+  int tn = s.top_layer_at_node[i], bn = m.bottom_layer_at_node[i];
+  int nadj = m.adj_cells_at_node[i];
+  for (int t = 0; t < d.NT; t++) {
+    double above = 0.0;
+    for (int k = L; k >= 1; k--) {
+      double v = 0.0;
+      if (k <= tn && k >= bn) {
+        double s11 = 0.0, s22 = 0.0;
+        int count1 = 0;
+        for (int l = 1; l <= nadj; l++) {
+          int ii = TB(m.adj_cellnum_at_node, l, i, P) - 1;
+          if (k >= m.bottom_layer_at_element[ii] && k <= s.top_layer_at_element[ii]) {
+            double wgt = m.area[ii] * AL(s.dz_cell_new, k, ii, E);
+            s11 = s11 + wgt;
+            s22 = s22 + wgt * w.con_new[nix(t, k, ii, L, E)];
+          } else {
+            count1++;
+          }
+        }
+        v = (s11 == 0.0) ? 0.0 : s22 / s11;
+        if (count1 == nadj && k != tn) v = above;
+      }
+      w.node_new[nix(t, k, i, L, P)] = v;
+      above = v;
+    }
+  }
+}
+
+// Face values and the copy to the model arrays (solve_transport_equation_v19.f90:1457-1482)
+__global__ void k_tvd_face_out(DevMesh m, DevState s, DevWork w, DevDims d) {
+  int q = blockIdx.x * blockDim.x + threadIdx.x;
+  const int J = d.J, P = d.P, L = d.L;
+  if (q >= J * L) return;
+  int j = q % J, k = q / J + 1;
+  int n1 = TB(m.nodenum_at_face, 1, j, J) - 1, n2 = TB(m.nodenum_at_face, 2, j, J) - 1;
+  bool in = k >= m.bottom_layer_at_face[j] && k <= s.top_layer_at_face[j];
+  for (int t = 0; t < d.NT; t++) {
+    double v = in ? (w.node_new[nix(t, k, n1, L, P)] + w.node_new[nix(t, k, n2, L, P)]) * 0.5 : 0.0;
+    double *dst = d.tid[t] == 1 ? s.salt_face : s.temp_face;
+    AL(dst, k, j, J) = v;
+  }
+}
+
+__global__ void k_tvd_cellnode_out(DevState s, DevWork w, DevDims d) {
+  int q = blockIdx.x * blockDim.x + threadIdx.x;
+  const int E = d.E, P = d.P, L = d.L;
+  long nc = (long)E * L, nn = (long)P * L;
+  for (int t = 0; t < d.NT; t++) {
+    if (q < nc) {
+      double *dst = d.tid[t] == 1 ? s.salt_cell_new : s.temp_cell_new;
+      dst[q] = w.con_new[(size_t)t * nc + q];
+    }
+    if (q < nn) {
+      double *dst = d.tid[t] == 1 ? s.salt_node : s.temp_node;
+      dst[q] = w.node_new[(size_t)t * nn + q];
+    }
+  }
+}
+
+// ================================================================ equation of state
+__device__ __forceinline__ double rho_unesco(double t, double s) {
+  double t2 = t * t, t3 = t2 * t, t4 = t3 * t, t5 = t4 * t;
+  double sq = sqrt(s), sq3 = sq * sq * sq;
+  return (double)(1000.0f - 0.157406f) + (double)6.793952e-2f * t - (double)9.095290e-3f * t2 +
+         (double)1.001685e-4f * t3 - (double)1.120083e-6f * t4 + (double)6.536332e-9f * t5 +
+         s * ((double)0.824493f - (double)4.089900e-3f * t + (double)7.643800e-5f * t2 - (double)8.246700e-7f * t3 +
+              (double)5.387500e-9f * t4) +
+         sq3 * ((double)(-5.72466e-3f) + (double)1.022700e-4f * t - (double)1.654600e-6f * t2) +
+         (double)4.831400e-4f * (s * s);
+}
+
+__device__ double rho_pressure(double rho_surface, double t, double s, double pres) {
+  double t2 = t * t, t3 = t2 * t, t4 = t3 * t;
+  double sq = sqrt(s), sq3 = sq * sq * sq, p2 = pres * pres;
+  double sbm = (double)19652.21f + (double)148.4206f * t - (double)2.327105f * t2 + (double)1.360477e-2f * t3 -
+               (double)5.155288e-5f * t4 +
+               pres * ((double)3.239908f + (double)1.43713e-3f * t + (double)1.16092e-4f * t2 - (double)5.77905e-7f * t3) +
+               p2 * ((double)8.50935e-5f - (double)6.12293e-6f * t + (double)5.2787e-8f * t2) +
+               s * ((double)54.6746f - (double)0.603459f * t + (double)1.09987e-2f * t2 - (double)6.1670e-5f * t3) +
+               sq3 * ((double)7.944e-2f + (double)1.6483e-2f * t - (double)5.3009e-4f * t2) +
+               pres * s * ((double)2.2838e-3f - (double)1.0981e-5f * t - (double)1.6078e-6f * t2) +
+               (double)1.91075e-4f * pres * sq3 - (double)9.9348e-7f * p2 * s + (double)2.08160e-8f * t * p2 * s +
+               (double)9.1697e-10f * t2 * p2 * s;
+  double ratio = 1.0 / (1.0 - pres / sbm);
+  return rho_surface * ratio;
+}
+
+// One column of calculate_density_full.f90:36-117 / calculate_density_linear.f90:33-71.
+__device__ void eos_column(const DevMesh &m, double MSL, int full, int compute, int bl, int tl, const double *T,
+                           const double *S, double *R, const double *dz, int e, int n, int L, int *err, int ent) {
+  for (int k = bl; k <= tl; k++) {
+    double rt = AL(T, k, e, n), rs = AL(S, k, e, n);
+    if (rt < -5.0 || rt > 100.0 || rs < -5.0 || rs > 100.0) {
+      latch(err, GOMGPU_ERR_EOS_RANGE, ent);
+      return;
+    }
+    if (compute) {
+      double r;
+      if (full) {
+        r = rho_unesco(rt, rs);
+        if (MSL - m.z_level[k] > 100.0) {
+          double pres = 0.0;
+          for (int kk = k; kk <= tl; kk++) {
+            double rr = (kk == k) ? r : AL(R, kk, e, n);
+            double ptmp = (double)9.81f * rr * AL(dz, kk, e, n);
+            pres = (kk == k) ? pres + (double)1.0e-5f * 0.5 * ptmp : pres + (double)1.0e-5f * ptmp;
+          }
+          r = rho_pressure(r, rt, rs, pres);
+        }
+      } else {
+        r = 1000.0 + (double)0.7f * rs;
+      }
+      AL(R, k, e, n) = r;
+    }
+    if (AL(R, k, e, n) < 980.0) {
+      latch(err, GOMGPU_ERR_EOS_RANGE, ent);
+      return;
+    }
+  }
+  double rb = AL(R, bl, e, n), rtp = AL(R, tl, e, n);
+  for (int k = 1; k <= bl - 1; k++) AL(R, k, e, n) = rb;
+  for (int k = tl + 1; k <= L; k++) AL(R, k, e, n) = rtp;
+}
+
+__global__ void k_eos_nodeface(DevMesh m, DevState s, DevDims d, double MSL, int full, int idens, int *err) {
+  int q = blockIdx.x * blockDim.x + threadIdx.x;
+  const int P = d.P, J = d.J, L = d.L;
+  if (q < P) {
+    int tl = s.top_layer_at_node[q];
+    if (tl == 0) return;
+    eos_column(m, MSL, full, 1, m.bottom_layer_at_node[q], tl, s.temp_node, s.salt_node, s.rho_node, s.dz_node, q, P,
+               L, err, q + 1);
+  } else if (q < P + J) {
+    int j = q - P;
+    int tl = s.top_layer_at_face[j];
+    if (tl == 0) return;
+    eos_column(m, MSL, full, full || idens == 1, m.bottom_layer_at_face[j], tl, s.temp_face, s.salt_face, s.rho_face,
+               s.dz_face, j, J, L, err, j + 1);
+  }
+}
+
+__global__ void k_eos_cell(DevMesh m, DevState s, DevDims d, int *err) {
+  int e = blockIdx.x * blockDim.x + threadIdx.x;
+  const int E = d.E, P = d.P, L = d.L;
+  if (e >= E) return;
+  int tl = s.top_layer_at_element[e];
+  if (tl == 0) return;
+  int bl = m.bottom_layer_at_element[e], N = m.tri_or_quad[e];
+  int nd[4], wet[4];
+  for (int l = 0; l < N; l++) {
+    nd[l] = TB(m.nodenum_at_cell, l + 1, e, E) - 1;
+    wet[l] = s.top_layer_at_node[nd[l]] != 0;
+  }
+  for (int k = bl; k <= tl; k++) {
+    double r = 0.0;
+    int icount = 0;
+    for (int l = 0; l < N; l++)
+      if (wet[l]) {
+        icount++;
+        r = r + AL(s.rho_node, k, nd[l], P);
+      }
+    if (icount == 0) {
+      latch(err, GOMGPU_ERR_EOS_DRY_NODES, e + 1);
+      return;
+    }
+    AL(s.rho_cell, k, e, E) = r / icount;
+  }
+  double rb = AL(s.rho_cell, bl, e, E), rtp = AL(s.rho_cell, tl, e, E);
+  for (int k = 1; k <= bl - 1; k++) AL(s.rho_cell, k, e, E) = rb;
+  for (int k = tl + 1; k <= L; k++) AL(s.rho_cell, k, e, E) = rtp;
+}
+
+// ================================================================ update
+// Depth averages at nodes, update_variables.f90:37-74
+__global__ void k_bars(DevMesh m, DevState s, DevDims d) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  const int P = d.P;
+  if (i >= P) return;
+  int tl = s.top_layer_at_node[i], bl = m.bottom_layer_at_node[i];
+  int nv = tl - bl + 1;
+  if (nv == 0) {
+    s.ubar_node[i] = s.vbar_node[i] = s.sbar_node[i] = s.tbar_node[i] = s.rbar_node[i] = 0.0;
+    return;
+  }
+  double su = 0.0, sv = 0.0;
+  for (int k = bl - 1; k <= tl; k++) {
+    su = su + A0(s.u_node, k, i, P);
+    sv = sv + A0(s.v_node, k, i, P);
+  }
+  s.ubar_node[i] = su / (nv + 1);
+  s.vbar_node[i] = sv / (nv + 1);
+  double ss = 0.0, st = 0.0, sr = 0.0;
+  for (int k = bl; k <= tl; k++) {
+    ss = ss + AL(s.salt_node, k, i, P);
+    st = st + AL(s.temp_node, k, i, P);
+    sr = sr + AL(s.rho_node, k, i, P);
+  }
+  s.sbar_node[i] = ss / nv;
+  s.tbar_node[i] = st / nv;
+  s.rbar_node[i] = sr / nv;
+}
+
+// FP64 volume and salt over wet cell layers; one block partial per block, summed on the
+// host in block order (deterministic).
+__global__ void k_budget(DevMesh m, DevState s, DevDims d, double *part) {
+  __shared__ double lv[256], ls[256];
+  int e = blockIdx.x * blockDim.x + threadIdx.x;
+  const int E = d.E;
+  double vol = 0.0, salt = 0.0;
+  if (e < E) {
+    int tl = s.top_layer_at_element[e], bl = m.bottom_layer_at_element[e];
+    for (int k = bl; k <= tl; k++) {
+      double v = m.area[e] * AL(s.dz_cell, k, e, E);
+      vol = vol + v;
+      salt = salt + v * AL(s.salt_cell, k, e, E);
+    }
+  }
+  lv[threadIdx.x] = vol;
+  ls[threadIdx.x] = salt;
+  __syncthreads();
+  for (int st = blockDim.x / 2; st > 0; st >>= 1) {
+    if ((int)threadIdx.x < st) {
+      lv[threadIdx.x] += lv[threadIdx.x + st];
+      ls[threadIdx.x] += ls[threadIdx.x + st];
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    part[2 * blockIdx.x] = lv[0];
+    part[2 * blockIdx.x + 1] = ls[0];
+  }
+}

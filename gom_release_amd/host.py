@@ -1,0 +1,146 @@
+"""Python host of the HIP step: ctypes binding of libgomgpu.so (the C ABI of
+include/gomgpu.h).
+
+This plays the role of the Fortran time loop (GOM_Rv1.1.0/source/main.f90:53-304) for
+tests, the benchmark and the serving loop: it owns host copies of the module arrays in the
+reference's layout and calls the same per-stage entry points a Fortran host would bind with
+ISO_C_BINDING. There is no CPU fallback: if the library is missing or no GPU is visible,
+construction fails with an error.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+from typing import Dict, Optional, Sequence
+
+import numpy as np
+
+from . import abi
+
+PKG = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(PKG, "libgomgpu.so")
+_lib = None
+
+STAGE_IDS = {name: i for i, name in enumerate(abi.STAGES)}
+
+
+class GomGpuError(RuntimeError):
+    def __init__(self, code: int, msg: str):
+        super().__init__(f"gomgpu error {code}: {msg}")
+        self.code = code
+
+
+def lib():
+    """Load libgomgpu.so (built in-tree by gom_release_amd.build)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(f"{LIB_PATH} is missing: run `python -m gom_release_amd.build` first")
+        L = C.CDLL(LIB_PATH)
+        vp = C.c_void_p
+        L.gomgpu_create.argtypes = [C.POINTER(abi.Mesh), C.POINTER(abi.Params), C.POINTER(vp)]
+        L.gomgpu_set_state.argtypes = [vp, C.POINTER(abi.State)]
+        L.gomgpu_get_state.argtypes = [vp, C.POINTER(abi.State)]
+        L.gomgpu_set_forcing.argtypes = [vp, C.POINTER(abi.Forcing)]
+        L.gomgpu_stage.argtypes = [vp, C.c_int]
+        L.gomgpu_step.argtypes = [vp]
+        L.gomgpu_run.argtypes = [vp, C.c_int]
+        L.gomgpu_sync.argtypes = [vp]
+        L.gomgpu_budgets.argtypes = [vp, C.POINTER(C.c_double), C.POINTER(C.c_double)]
+        L.gomgpu_pcg_iterations.argtypes = [vp]
+        L.gomgpu_last_error.argtypes = [vp]
+        L.gomgpu_last_error.restype = C.c_char_p
+        L.gomgpu_timings.argtypes = [vp, C.c_int, C.POINTER(C.c_char_p), C.POINTER(C.c_double), C.c_int]
+        L.gomgpu_destroy.argtypes = [vp]
+        L.gomgpu_destroy.restype = None
+        _lib = L
+    return _lib
+
+
+EXPORTED = ["gomgpu_create", "gomgpu_set_state", "gomgpu_set_forcing", "gomgpu_stage",
+            "gomgpu_step", "gomgpu_run", "gomgpu_get_state", "gomgpu_budgets",
+            "gomgpu_pcg_iterations", "gomgpu_sync", "gomgpu_last_error", "gomgpu_timings",
+            "gomgpu_destroy"]
+
+
+def device_count() -> int:
+    """Number of visible HIP devices (through the HIP runtime the library links)."""
+    n = C.c_int(0)
+    rc = lib().hipGetDeviceCount(C.byref(n))
+    return n.value if rc == 0 else 0
+
+
+class GpuModel:
+    """Device-resident model state driven through the C ABI."""
+
+    def __init__(self, mesh: "abi.Arrays", params: "abi.Params"):
+        self.params = params
+        self.mesh = mesh
+        self._L = lib()
+        self._ctx = C.c_void_p()
+        ms = mesh.struct()
+        rc = self._L.gomgpu_create(C.byref(ms), C.byref(params), C.byref(self._ctx))
+        if rc != 0:
+            raise GomGpuError(rc, "gomgpu_create failed (see stderr)")
+
+    # -------------------------------------------------------------- data movement
+    def set_state(self, state: "abi.Arrays", only: Optional[Sequence[str]] = None) -> None:
+        st = state.struct(only)
+        self._check(self._L.gomgpu_set_state(self._ctx, C.byref(st)))
+
+    def get_state(self, state: "abi.Arrays", only: Optional[Sequence[str]] = None) -> None:
+        st = state.struct(only)
+        self._check(self._L.gomgpu_get_state(self._ctx, C.byref(st)))
+
+    def set_forcing(self, forcing: "abi.ForcingArrays", only: Optional[Sequence[str]] = None) -> None:
+        f = forcing.struct(only)
+        self._check(self._L.gomgpu_set_forcing(self._ctx, C.byref(f)))
+
+    # -------------------------------------------------------------- execution
+    def stage(self, name: str) -> None:
+        self._check(self._L.gomgpu_stage(self._ctx, STAGE_IDS[name]))
+
+    def step(self) -> None:
+        self._check(self._L.gomgpu_step(self._ctx))
+
+    def run(self, nsteps: int) -> None:
+        self._check(self._L.gomgpu_run(self._ctx, int(nsteps)))
+
+    def sync(self, allow_pcg_warning: bool = False) -> int:
+        rc = self._L.gomgpu_sync(self._ctx)
+        if rc == 5 and allow_pcg_warning:
+            return rc
+        self._check(rc)
+        return rc
+
+    def pcg_iterations(self) -> int:
+        return int(self._L.gomgpu_pcg_iterations(self._ctx))
+
+    def budgets(self):
+        v, s = C.c_double(), C.c_double()
+        self._check(self._L.gomgpu_budgets(self._ctx, C.byref(v), C.byref(s)))
+        return v.value, s.value
+
+    def timings(self, enable: bool = True) -> Dict[str, float]:
+        names = (C.c_char_p * 16)()
+        ms = (C.c_double * 16)()
+        n = self._L.gomgpu_timings(self._ctx, 1 if enable else 0, names, ms, 16)
+        return {names[i].decode(): ms[i] for i in range(n)}
+
+    def last_error(self) -> str:
+        return self._L.gomgpu_last_error(self._ctx).decode()
+
+    def close(self) -> None:
+        if self._ctx:
+            self._L.gomgpu_destroy(self._ctx)
+            self._ctx = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _check(self, rc: int) -> None:
+        if rc != 0:
+            raise GomGpuError(rc, self.last_error())

@@ -75,8 +75,10 @@ def run_reference(project: str, dumpdir: Optional[str], first: int = 1, last: in
     # A Fortran STOP exits with status 0: detect it from the log of the run.
     log = os.path.join(project, "output", "run.log")
     tail = open(log).read()[-400:] if os.path.exists(log) else ""
-    if dumpdir is not None and last >= first and not os.path.exists(
-            os.path.join(dumpdir, f"s{last}_post.bin")):
+    want = [last] if last >= first else []
+    want += [e for e in extra]
+    if dumpdir is not None and any(not os.path.exists(os.path.join(dumpdir, f"s{w}_post.bin"))
+                                   for w in want if w > 0):
         raise RuntimeError("reference stopped early: " + tail)
     return p.stdout
 
