@@ -1,0 +1,168 @@
+"""Benchmark of the GOM semi-implicit step on one MI355X.
+
+Workload: the synthetic estuary of the benchmark configurations -- a mixed orthogonal mesh
+of 200,200 cells (equilateral 100 m triangles stitched to 100x100 m squares), 20 z-layers,
+Gaussian thalweg 20 m -> 2 m with seeded +-0.5 m noise, salinity 0 -> 35 psu along the
+axis, 500 m^3/s river, M2+K1 tide, uniform 10 m/s wind, TVD limiters (minmod) with 2
+sub-cycles, dt = 120 s. Inputs are generated (no network) and resident on the device when
+the timed region starts; per step only the O(boundary) host forcing crosses the bus.
+
+A "step" is one full semi-implicit time step (friction, ELM, momentum, free surface + PCG,
+velocities, layers, node velocities, transport, EOS, update). The metric is active
+cell-layer-steps per second: sum over wet cells of their layer count, per step.
+
+Prints one JSON line. Extra fields: "roofline" for the dominant kernel (HBM-bound; bytes
+are algorithmic, durations from HIP events on the context stream) and "cpu_baseline" (the
+CPU restatement of the same step, single thread, on a bounded sample of the same workload).
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+
+
+def make_case(cells: str):
+    from gom_release_amd import meshgen
+    if cells == "200k":
+        return meshgen.estuary_case(ncol=400, nrep=100, nlayer=20, dt=120.0, river_q=500.0,
+                                    trans_sub_iter=2, h_flux_limiter=1, v_flux_limiter=1)
+    if cells == "20k":
+        return meshgen.estuary_case(ncol=100, nrep=40, nlayer=20, dt=120.0, river_q=500.0,
+                                    trans_sub_iter=2, h_flux_limiter=1, v_flux_limiter=1)
+    raise ValueError(cells)
+
+
+def pcg_bytes_per_iteration(mesh, params) -> float:
+    """Algorithmic bytes of one PCG iteration: E*(12*zbar + 108), zbar = nnz per row."""
+    import numpy as np
+    adj = mesh.view2("adj_cellnum_at_cell")
+    nnz = params.maxele + int(np.count_nonzero(adj))
+    zbar = nnz / params.maxele
+    return params.maxele * (12.0 * zbar + 108.0)
+
+
+def cpu_baseline(case, wind, budget_s: float = 15.0, max_steps: int = 12):
+    """Time the CPU restatement (oracle, one thread) on the same workload."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    from oracle import Oracle
+    from gom_release_amd import driver, prep
+    p, m, s = prep.build(case)
+    hf = driver.HostForcing(case, p, m, wind)
+    o = Oracle(m, p, s, hf.forcing)
+    if o.run("eos") != 0:
+        raise RuntimeError("oracle eos failed")
+    t_total = 0.0
+    work = 0
+    n = 0
+    import numpy as np
+    bot = m["bottom_layer_at_element"].astype(np.int64)
+    while n < max_steps and (t_total < budget_s or n == 0):
+        hf.advance(n + 1, eta_node=s["eta_node"])
+        top = s["top_layer_at_element"].astype(np.int64)
+        t0 = time.perf_counter()
+        rc = o.step()
+        t_total += time.perf_counter() - t0
+        if rc != 0:
+            raise RuntimeError(f"oracle step failed: {rc}")
+        work += int(np.sum(np.where(top > 0, top - bot + 1, 0)))
+        n += 1
+    return {"value": work / t_total, "unit": "cell-layer-steps/s", "cores": 1, "kind": "port",
+            "sample": f"first {n} steps of the same 200k-cell x 20-layer estuary, FP64 C restatement, "
+                      f"single thread, {t_total:.1f} s"}
+
+
+def main() -> int:
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=60)
+    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--precision", choices=["fp64", "mixed"], default="fp64")
+    ap.add_argument("--cells", default="200k")
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--profile-steps", type=int, default=5)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py: this build measures one GPU only (got --gpus {args.gpus}); "
+              "multi-GPU sharding is described in DESIGN.md but not run.", file=sys.stderr)
+        return 2
+    from gom_release_amd import host
+    from gom_release_amd.driver import Scenario
+    if host.device_count() < 1:
+        print("bench.py: no GPU visible", file=sys.stderr)
+        return 1
+    wind = (10.0, 0.0)
+    case = make_case(args.cells)
+    prec = 0 if args.precision == "fp64" else 1
+    print(f"[bench] building {case.maxele}-cell x {case.maxlayer}-layer case", file=sys.stderr, flush=True)
+    sc = Scenario(case, precision=prec, wind=wind)
+    model = sc.model
+    sc.run(args.warmup)
+    model.sync()
+    print(f"[bench] warmup done ({args.warmup} steps)", file=sys.stderr, flush=True)
+    a0 = sc.active_cell_layers()
+    model.sync()
+    t0 = time.perf_counter()
+    sc.run(args.steps)
+    model.sync()
+    t1 = time.perf_counter()
+    a1 = sc.active_cell_layers()
+    elapsed = t1 - t0
+    active = 0.5 * (a0 + a1) * args.steps
+    value = active / elapsed
+    print(f"[bench] timed {args.steps} steps in {elapsed:.3f} s", file=sys.stderr, flush=True)
+
+    # measurement pass: per-stage and per-kernel HIP event times (synchronising)
+    model.timings(True)
+    iters = []
+    stage_tot = {}
+    for _ in range(args.profile_steps):
+        sc.step()
+        iters.append(model.pcg_iterations())
+        for k, v in model.timings(True).items():
+            stage_tot[k] = stage_tot.get(k, 0.0) + v
+    tim = model.timings(False)
+    kernels = {k: tim[k] for k in ("k_pcg", "k_elm", "k_momentum", "k_tvd_column")}
+    stages = {k: stage_tot[k] / args.profile_steps for k in stage_tot if not k.startswith("k_")}
+    it_mean = sum(iters) / len(iters)
+    pcg_bytes = pcg_bytes_per_iteration(sc.mesh, sc.params) * it_mean
+    pcg_ms = kernels["k_pcg"]
+    achieved = pcg_bytes / (pcg_ms * 1e-3) / 1e9
+    roof = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+            "kernel": "k_pcg", "pcg_iterations": round(it_mean, 1),
+            "kernel_ms": {k: round(v, 4) for k, v in kernels.items()},
+            "stage_ms": {k: round(v, 4) for k, v in stages.items()}}
+    vol, salt = model.budgets()
+    out = {
+        "metric": "cell-layer-steps/s at 1 GPU + %HBM roofline; p50/p99 scenario latency",
+        "value": value, "unit": "cell-layer-steps/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": 1e3 * elapsed / args.steps, "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f64" if prec == 0 else "f32+f64",
+        "data": "synthetic",
+        "config": {"workload": f"synthetic estuary, {case.maxele} mixed tri/quad cells x {case.maxlayer} "
+                               f"z-layers, dt=120 s, TVD minmod x2 sub-cycles, M2+K1, 500 m3/s river, "
+                               f"10 m/s wind, constant Av/Kv",
+                   "cells": case.maxele, "faces": sc.params.maxface, "nodes": case.maxnod,
+                   "layers": case.maxlayer, "active_cell_layers": int(0.5 * (a0 + a1)),
+                   "nominal_cell_layer_steps_per_s": case.maxele * case.maxlayer * args.steps / elapsed,
+                   "pcg_tolerance": case.error_tolerance, "precision": args.precision,
+                   "volume_m3": vol, "salt_psu_m3": salt},
+        "roofline": roof,
+    }
+    if not args.no_cpu_baseline:
+        print("[bench] timing the CPU restatement", file=sys.stderr, flush=True)
+        out["cpu_baseline"] = cpu_baseline(case, wind)
+    print(json.dumps(out))
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

@@ -84,6 +84,14 @@ def leading(code: str, p: Params) -> int:
         return p.maxlayer
     if code in ("L1J", "L1E", "L1P"):
         return p.maxlayer + 1
+    if code == "4E":
+        return 4
+    if code == "2J":
+        return 2
+    if code == "NBP":
+        return p.max_no_neighbor_node
+    if code == "NB1P":
+        return p.max_no_neighbor_node + 1
     return 1
 
 

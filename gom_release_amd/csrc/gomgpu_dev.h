@@ -52,6 +52,7 @@ struct DevWork {
   double *fs_diag, *fs_off, *fs_rhs, *pcg_x, *pcg_r, *pcg_d0, *pcg_d1, *pcg_q, *pcg_s, *pcg_minv;
   double *pcg_part;       // per-block partial sums
   int *pcg_info;          // [0] iterations of the last solve, [1] converged flag
+  unsigned *pcg_bar;      // grid barrier state
   double *face_sdz;       // per face sum_k dz * AinvDeltaZ1 (top down)
   double *face_sun;       // per face sum_k dz * un
   double *face_sg;        // per face sum_k dz * AinvG1
@@ -112,7 +113,9 @@ struct PcgArgs {
   const int32_t *nfaces;  // tri_or_quad
   double *x, *r, *d0, *d1, *q, *s, *minv, *partA, *partB;
   int *info;
+  unsigned *bar;      // [0] arrival counter, [1] generation (custom grid barrier)
   int n, maxit;
+  int barrier_mode;   // 0 cooperative-groups grid sync, 1 custom barrier
   double tol;
 };
 

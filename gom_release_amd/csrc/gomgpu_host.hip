@@ -3,6 +3,7 @@
 // (GOM_Rv1.1.0/source/main.f90:95-272).
 #include <math.h>
 #include <stdio.h>
+#include <stdlib.h>
 #include <string.h>
 
 #include <string>
@@ -16,6 +17,7 @@ __global__ void k_elm_substeps(DevMesh, DevState, DevDims, double, double, int, 
 __global__ void k_elm(DevMesh, DevState, DevDims, double, double, double, int *);
 __global__ void k_elm_copy(DevState, long);
 __global__ void k_momentum(DevMesh, DevState, DevForcing, DevDims, MomParams, int *);
+__global__ void k_river_velocity(DevMesh, DevState, DevForcing, DevDims, double *);
 __global__ void k_fs_face(DevMesh, DevState, DevWork, DevDims, double);
 __global__ void k_fs_cell(DevMesh, DevState, DevForcing, DevWork, DevDims, double, double, int *);
 __global__ void k_eta_node(DevMesh, DevState, DevDims);
@@ -46,9 +48,11 @@ __global__ void k_eos_cell(DevMesh, DevState, DevDims, int *);
 __global__ void k_bars(DevMesh, DevState, DevDims);
 __global__ void k_budget(DevMesh, DevState, DevDims, double *);
 __global__ void k_copy(double *, const double *, int);
-extern "C" hipError_t gom_launch_pcg(const PcgArgs *args, int max_blocks, hipStream_t st);
+extern "C" hipError_t gom_launch_pcg(const PcgArgs *args, int max_blocks, int cooperative, hipStream_t st);
 extern "C" int gom_pcg_max_blocks(int device);
 
+enum { KT_PCG = 0, KT_ELM = 1, KT_MOMENTUM = 2, KT_TVD_COLUMN = 3 };
+static const char *kKernelNames[GOMGPU_NUM_KTIMERS] = {"k_pcg", "k_elm", "k_momentum", "k_tvd_column"};
 static const char *kStageNames[GOMGPU_NUM_STAGES] = {"friction", "elm", "momentum", "freesurf", "velocity",
                                                       "layers", "nodevel", "transport", "eos", "update"};
 
@@ -63,14 +67,25 @@ struct gomgpu_ctx {
   DevWork w;
   double elapsed = 0.0, spin_tide = 1.0, spin_bc = 1.0;
   std::vector<void *> allocs;
-  int pcg_blocks = 0;
+  int pcg_blocks = 0;      // co-resident capacity of the PCG kernel
+  int pcg_cap = 0;         // tuned cap of the PCG grid (0 = capacity)
+  int pcg_barrier = 1;     // 0 cooperative-groups sync, 1 custom barrier
+  int pcg_coop = 1;        // launch through hipLaunchCooperativeKernel
+  int ncu = 0;
   int last_iters = 0;
   std::string err;
   bool timing = false;
-  hipEvent_t ev[GOMGPU_NUM_STAGES + 1];
+  hipEvent_t ev[GOMGPU_NUM_STAGES + 1] = {};
   double stage_ms[GOMGPU_NUM_STAGES];
+  // per-kernel timers of the hot kernels (event pairs on the context stream)
+  hipEvent_t kev[GOMGPU_NUM_KTIMERS][2] = {};
+  double kernel_ms[GOMGPU_NUM_KTIMERS];
+  int kernel_calls[GOMGPU_NUM_KTIMERS];
   int *h_err = nullptr;  // pinned
   bool pcg_ran = false;  // a free-surface solve ran since the last sync
+  double *h_stage = nullptr;  // pinned staging of the forcing arrays
+  size_t stage_len = 0;
+  hipEvent_t fev = nullptr;   // completion of the last forcing upload
 };
 
 static int seterr(gomgpu_ctx *c, int code, const std::string &msg) {
@@ -266,6 +281,10 @@ extern "C" int gomgpu_create(const gomgpu_mesh *mesh, const gomgpu_params *param
     long NT = c->d.NT > 0 ? c->d.NT : 1, OB = p.num_ob_cell > 0 ? p.num_ob_cell : 1,
          QB = p.num_Qb_cell > 0 ? p.num_Qb_cell : 1;
     c->pcg_blocks = gom_pcg_max_blocks(c->device);
+    hipDeviceGetAttribute(&c->ncu, hipDeviceAttributeMultiprocessorCount, c->device);
+    if (const char *e = getenv("GOMGPU_PCG_BLOCKS")) c->pcg_cap = atoi(e);
+    if (const char *e = getenv("GOMGPU_PCG_BARRIER")) c->pcg_barrier = atoi(e);
+    if (const char *e = getenv("GOMGPU_PCG_COOPERATIVE")) c->pcg_coop = atoi(e);
     if (c->pcg_blocks <= 0) {
       rc = seterr(c, GOMGPU_E_HIP, "cannot size the PCG grid");
       goto fail;
@@ -275,7 +294,7 @@ extern "C" int gomgpu_create(const gomgpu_mesh *mesh, const gomgpu_params *param
         (rc = dalloc(c, &w.pcg_x, E)) || (rc = dalloc(c, &w.pcg_r, E)) || (rc = dalloc(c, &w.pcg_d0, E)) ||
         (rc = dalloc(c, &w.pcg_d1, E)) || (rc = dalloc(c, &w.pcg_q, E)) || (rc = dalloc(c, &w.pcg_s, E)) ||
         (rc = dalloc(c, &w.pcg_minv, E)) || (rc = dalloc(c, &w.pcg_part, 2 * (long)c->pcg_blocks)) ||
-        (rc = dalloc(c, &w.pcg_info, 2)) || (rc = dalloc(c, &w.face_sdz, J)) || (rc = dalloc(c, &w.face_sun, J)) ||
+        (rc = dalloc(c, &w.pcg_info, 2)) || (rc = dalloc(c, &w.pcg_bar, 2)) || (rc = dalloc(c, &w.face_sdz, J)) || (rc = dalloc(c, &w.face_sun, J)) ||
         (rc = dalloc(c, &w.face_sg, J)))
       goto fail;
     if ((rc = dalloc(c, &w.con, NT * (L + 2) * E)) || (rc = dalloc(c, &w.con_new, NT * L * E)) ||
@@ -295,6 +314,27 @@ extern "C" int gomgpu_create(const gomgpu_mesh *mesh, const gomgpu_params *param
     goto fail;
   }
   for (int i = 0; i <= GOMGPU_NUM_STAGES; i++) hipEventCreate(&c->ev[i]);
+  for (int i = 0; i < GOMGPU_NUM_KTIMERS; i++) {
+    hipEventCreate(&c->kev[i][0]);
+    hipEventCreate(&c->kev[i][1]);
+  }
+  {
+    size_t tot = 0;
+#define SZ_F(name, ext)                                                              \
+  {                                                                                  \
+    Shape sh = shape_of(#ext, p);                                                    \
+    tot += (size_t)(sh.ld * sh.n);                                                   \
+  }
+    GOMGPU_FORCING_REAL(SZ_F)
+#undef SZ_F
+    c->stage_len = tot;
+    if (hipHostMalloc((void **)&c->h_stage, tot * sizeof(double)) != hipSuccess ||
+        hipEventCreateWithFlags(&c->fev, hipEventDisableTiming) != hipSuccess ||
+        hipEventRecord(c->fev, c->st) != hipSuccess) {
+      rc = seterr(c, GOMGPU_E_NOMEM, "cannot allocate the forcing staging buffer");
+      goto fail;
+    }
+  }
   *out = c;
   return 0;
 fail:
@@ -308,8 +348,13 @@ extern "C" void gomgpu_destroy(gomgpu_ctx *c) {
   if (c->st) hipStreamSynchronize(c->st);
   for (void *p : c->allocs) hipFree(p);
   if (c->h_err) hipHostFree(c->h_err);
+  if (c->h_stage) hipHostFree(c->h_stage);
+  if (c->fev) hipEventDestroy(c->fev);
   for (int i = 0; i <= GOMGPU_NUM_STAGES; i++)
     if (c->ev[i]) hipEventDestroy(c->ev[i]);
+  for (int i = 0; i < GOMGPU_NUM_KTIMERS; i++)
+    for (int k = 0; k < 2; k++)
+      if (c->kev[i][k]) hipEventDestroy(c->kev[i][k]);
   if (c->st) hipStreamDestroy(c->st);
   delete c;
 }
@@ -343,23 +388,46 @@ extern "C" int gomgpu_get_state(gomgpu_ctx *c, gomgpu_state *st) {
   return 0;
 }
 
+// Forcing uploads are asynchronous: the arrays go through one pinned staging buffer; the
+// host waits only for the previous forcing copy before reusing it, so it can run one step
+// ahead of the device.
 extern "C" int gomgpu_set_forcing(gomgpu_ctx *c, const gomgpu_forcing *f) {
   if (!c || !f) return GOMGPU_E_ARG;
   const gomgpu_params &p = c->p;
-  int rc;
   c->elapsed = f->elapsed_time;
   c->spin_tide = f->spinup_function_tide;
   c->spin_bc = f->spinup_function_baroclinic;
-#define UP_F(name, ext)                                                                      \
-  if (f->name) {                                                                             \
-    if ((rc = upload(c, (double *)c->f.name, f->name, shape_of(#ext, p)))) return rc;        \
+  HIPCHK(c, hipEventSynchronize(c->fev));
+  size_t off = 0;
+#define UP_F(name, ext)                                                                              \
+  if (f->name) {                                                                                     \
+    Shape sh = shape_of(#ext, p);                                                                    \
+    size_t n = (size_t)(sh.ld * sh.n);                                                               \
+    if (off + n > c->stage_len) return seterr(c, GOMGPU_E_ARG, "forcing staging overflow");         \
+    to_device_layout(f->name, c->h_stage + off, sh);                                                 \
+    HIPCHK(c, hipMemcpyAsync((double *)c->f.name, c->h_stage + off, n * sizeof(double),              \
+                             hipMemcpyHostToDevice, c->st));                                         \
+    off += n;                                                                                        \
   }
   GOMGPU_FORCING_REAL(UP_F)
 #undef UP_F
+  HIPCHK(c, hipEventRecord(c->fev, c->st));
   return 0;
 }
 
 // ------------------------------------------------------------------ stages
+#define KT_BEGIN(i) \
+  if (c->timing) hipEventRecord(c->kev[i][0], st)
+#define KT_END(i)                                             \
+  if (c->timing) {                                            \
+    hipEventRecord(c->kev[i][1], st);                         \
+    hipEventSynchronize(c->kev[i][1]);                        \
+    float ms_ = 0;                                            \
+    hipEventElapsedTime(&ms_, c->kev[i][0], c->kev[i][1]);    \
+    c->kernel_ms[i] += ms_;                                   \
+    c->kernel_calls[i] += 1;                                  \
+  }
+
 static int stage_impl(gomgpu_ctx *c, int id) {
   const gomgpu_params &p = c->p;
   const DevDims &d = c->d;
@@ -371,6 +439,10 @@ static int stage_impl(gomgpu_ctx *c, int id) {
       k_friction<<<nblk(d.J, BS), BS, 0, st>>>(c->m, c->s, d, p.bf_flag, p.dry_depth, p.von_Karman);
       break;
     case GOMGPU_STAGE_ELM:
+      // compute_boundary_velocity (main.f90:152) runs on the host before the hot path in
+      // the reference; with river_on_device it is done here, before its first use.
+      if (p.river_on_device && d.NQB > 0)
+        k_river_velocity<<<nblk(d.NQB, 64), 64, 0, st>>>(c->m, c->s, c->f, d, (double *)c->f.Qu_boundary);
       if (p.advection_flag == 0) {
         long n = (long)d.L * d.J;
         k_elm_copy<<<nblk(n, BS), BS, 0, st>>>(c->s, n);
@@ -378,13 +450,17 @@ static int stage_impl(gomgpu_ctx *c, int id) {
         if (p.ELM_backtrace_flag == 1)
           k_elm_substeps<<<nblk((long)d.P * d.L, BS), BS, 0, st>>>(c->m, c->s, d, p.dt, p.delta_z_min,
                                                                    p.ELM_min_iter, p.ELM_max_iter);
+        KT_BEGIN(KT_ELM);
         k_elm<<<nblk((long)d.J * d.L, 128), 128, 0, st>>>(c->m, c->s, d, p.MSL, p.dt, p.adv_onoff_depth, err);
+        KT_END(KT_ELM);
       }
       break;
     case GOMGPU_STAGE_MOMENTUM: {
       MomParams q{p.dt, p.gravity, p.theta, p.rho_o, p.smagorinsky_parameter, p.Ah_0, p.Kh_0, c->spin_bc,
                   p.baroclinic_flag};
+      KT_BEGIN(KT_MOMENTUM);
       k_momentum<<<nblk(d.J, 128), 128, 0, st>>>(c->m, c->s, c->f, d, q, err);
+      KT_END(KT_MOMENTUM);
       break;
     }
     case GOMGPU_STAGE_FREESURF: {
@@ -407,10 +483,19 @@ static int stage_impl(gomgpu_ctx *c, int id) {
       a.partA = c->w.pcg_part;
       a.partB = c->w.pcg_part + c->pcg_blocks;
       a.info = c->w.pcg_info;
+      a.bar = c->w.pcg_bar;
+      a.barrier_mode = c->pcg_barrier;
       a.n = d.E;
       a.maxit = p.max_iteration_pcg;
       a.tol = p.error_tolerance;
-      HIPCHK(c, gom_launch_pcg(&a, c->pcg_blocks, st));
+      KT_BEGIN(KT_PCG);
+      {
+        int cap = c->pcg_blocks;
+        if (c->pcg_cap > 0 && c->pcg_cap < cap) cap = c->pcg_cap;
+        if (!c->pcg_coop && cap > c->ncu) cap = c->ncu;  // plain launch: one block per CU
+        HIPCHK(c, gom_launch_pcg(&a, cap, c->pcg_coop, st));
+      }
+      KT_END(KT_PCG);
       c->pcg_ran = true;
       k_copy<<<nblk(d.E, BS), BS, 0, st>>>(c->s.eta_cell_new, c->w.pcg_x, d.E);
       k_eta_node<<<nblk(d.P, BS), BS, 0, st>>>(c->m, c->s, d);
@@ -458,7 +543,9 @@ static int stage_impl(gomgpu_ctx *c, int id) {
           k_tvd_lim_face<<<nblk((long)d.J * d.L, BS), BS, 0, st>>>(c->m, c->s, c->w, d, p.h_flux_limiter);
           k_tvd_psi_cell<<<nblk((long)d.E * (d.L + 1), BS), BS, 0, st>>>(c->m, c->s, c->w, d, p.h_flux_limiter);
         }
+        KT_BEGIN(KT_TVD_COLUMN);
         k_tvd_column<<<nblk(d.E, 128), 128, 0, st>>>(c->m, c->s, c->w, d, a, err);
+        KT_END(KT_TVD_COLUMN);
         k_tvd_copyback<<<nblk(nE2, BS), BS, 0, st>>>(c->m, c->s, c->w, d);
       }
       k_tvd_node<<<nblk(d.P, BS), BS, 0, st>>>(c->m, c->s, c->w, d);
@@ -584,11 +671,21 @@ extern "C" const char *gomgpu_last_error(gomgpu_ctx *c) { return c ? c->err.c_st
 
 extern "C" int gomgpu_timings(gomgpu_ctx *c, int enable, const char **names, double *ms, int n) {
   if (!c) return GOMGPU_E_ARG;
+  bool was = c->timing;
   c->timing = enable != 0;
   int k = 0;
   for (; k < n && k < GOMGPU_NUM_STAGES; k++) {
     if (names) names[k] = kStageNames[k];
     if (ms) ms[k] = c->stage_ms[k];
   }
+  for (int i = 0; i < GOMGPU_NUM_KTIMERS && k < n; i++, k++) {
+    if (names) names[k] = kKernelNames[i];
+    if (ms) ms[k] = c->kernel_calls[i] ? c->kernel_ms[i] / c->kernel_calls[i] : 0.0;
+  }
+  if (c->timing && !was)
+    for (int i = 0; i < GOMGPU_NUM_KTIMERS; i++) {
+      c->kernel_ms[i] = 0.0;
+      c->kernel_calls[i] = 0;
+    }
   return k;
 }

@@ -1053,3 +1053,23 @@ __global__ void k_node_vel(DevMesh m, DevState s, DevDims d) {
   A0(s.v_node, k, i, P) = v;
   A0(s.w_node, k, i, P) = w;
 }
+
+// River face velocity from the discharge (compute_boundary_velocity.f90:32-46):
+// Qu = -Q_add / (face_length * (h_face + mean eta of the face nodes)).
+__global__ void k_river_velocity(DevMesh m, DevState s, DevForcing f, DevDims d, double *qu) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  const int J = d.J;
+  if (i >= d.NQB) return;
+  // the river face is the face of the river cell flagged with this river id
+  int e = m.Qb_cell_id[i] - 1;
+  int N = m.tri_or_quad[e];
+  for (int l = 1; l <= N; l++) {
+    int j = TB(m.facenum_at_cell, l, e, d.E) - 1;
+    if (m.isflowside3[j] == i + 1) {
+      int n1 = TB(m.nodenum_at_face, 1, j, J) - 1, n2 = TB(m.nodenum_at_face, 2, j, J) - 1;
+      double depth = m.h_face[j] + (s.eta_node[n1] + s.eta_node[n2]) * 0.5;
+      qu[i] = -f.Q_add[i] / (m.face_length[j] * depth);
+      return;
+    }
+  }
+}

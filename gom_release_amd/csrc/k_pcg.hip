@@ -75,9 +75,46 @@ __device__ __forceinline__ double row_apply_dir(const PcgArgs &a, int i, const d
   return sum;
 }
 
+// Grid-wide barrier for a co-resident grid: one device-scope atomic per block on arrival,
+// the last block to arrive resets the counter and bumps the generation; the others spin
+// on the generation with a short sleep. Releases and acquires are agent scope.
+__device__ __forceinline__ void bar_sync(unsigned *bar, unsigned nb, unsigned &gen, int mode) {
+  if (mode == 0) {
+    cg::this_grid().sync();
+    return;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    unsigned g = gen;
+    __threadfence();
+    unsigned arrived = atomicAdd(&bar[0], 1u);
+    if (arrived == nb - 1) {
+      atomicExch(&bar[0], 0u);
+      __threadfence();
+      atomicAdd(&bar[1], 1u);
+    } else {
+      while (__hip_atomic_load(&bar[1], __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) == g)
+        __builtin_amdgcn_s_sleep(1);
+    }
+    __threadfence();
+  }
+  gen = gen + 1;
+  __syncthreads();
+}
+
 __global__ void __launch_bounds__(PCG_BS) k_pcg(PcgArgs a) {
   __shared__ double lds[16];
-  cg::grid_group grid = cg::this_grid();
+  __shared__ unsigned gen0;
+  if (threadIdx.x == 0) gen0 = __hip_atomic_load(&a.bar[1], __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT);
+  __syncthreads();
+  unsigned gen = gen0;
+  struct {
+    unsigned *bar;
+    unsigned nb;
+    unsigned *gen;
+    int mode;
+    __device__ void sync() { bar_sync(bar, nb, *gen, mode); }
+  } grid{a.bar, gridDim.x, &gen, a.barrier_mode};
   const int n = a.n, nb = gridDim.x;
   const int stride = gridDim.x * blockDim.x;
   const int t0 = blockIdx.x * blockDim.x + threadIdx.x;
@@ -164,13 +201,22 @@ __global__ void k_copy(double *dst, const double *src, int n) {
 }
 
 // Host launcher: sizes the grid to the co-resident capacity (cooperative launch).
-extern "C" hipError_t gom_launch_pcg(const PcgArgs *args, int max_blocks, hipStream_t st) {
+// Host launcher. The grid is at most max_blocks (the co-resident capacity, or a smaller
+// tuned cap). cooperative != 0 launches through hipLaunchCooperativeKernel, which checks
+// co-residency; otherwise a plain launch is used with a grid of at most one block per
+// compute unit, which an otherwise idle device always holds at once.
+extern "C" hipError_t gom_launch_pcg(const PcgArgs *args, int max_blocks, int cooperative, hipStream_t st) {
   PcgArgs a = *args;
   int nb = (a.n + PCG_BS - 1) / PCG_BS;
   if (nb > max_blocks) nb = max_blocks;
   if (nb < 1) nb = 1;
-  void *kargs[] = {&a};
-  return hipLaunchCooperativeKernel((const void *)k_pcg, dim3(nb), dim3(PCG_BS), kargs, 0, st);
+  if (cooperative) {
+    void *kargs[] = {&a};
+    return hipLaunchCooperativeKernel((const void *)k_pcg, dim3(nb), dim3(PCG_BS), kargs, 0, st);
+  }
+  if (a.barrier_mode == 0) return hipErrorInvalidValue;  // grid sync needs a cooperative launch
+  hipLaunchKernelGGL(k_pcg, dim3(nb), dim3(PCG_BS), 0, st, a);
+  return hipGetLastError();
 }
 
 extern "C" int gom_pcg_max_blocks(int device) {

@@ -1,0 +1,102 @@
+"""Time loop of one scenario on the device (the role of main.f90:53-304).
+
+A Scenario owns a device context, computes the host forcing of each step (spin-up,
+harmonic open-boundary elevation, boundary concentrations of the transport sub-cycles,
+river discharge, uniform wind) and advances the step through the C ABI. The state stays
+on the device; only O(boundary) forcing crosses the bus each step.
+"""
+from __future__ import annotations
+
+from typing import Optional
+
+import numpy as np
+
+from . import abi, forcing as fz, host, prep
+from .gomio import Case
+
+DYNAMIC = ["eta_at_ob_old", "eta_at_ob_new", "con_at_ob", "con_at_Qb", "Q_add"]
+
+
+class HostForcing:
+    """Per-step host forcing of a generated case (the 2-D routines of main.f90:84-161)."""
+
+    def __init__(self, case: Case, params: "abi.Params", mesh: "abi.Arrays", wind=(0.0, 0.0)):
+        self.case, self.params, self.mesh = case, params, mesh
+        self.forcing = abi.ForcingArrays(params)
+        if wind[0] or wind[1]:
+            gt, wsn, wst = fz.wind_stress(mesh, *wind)
+            self.forcing["Gamma_T"] = gt
+            self.forcing["wind_stress_normal"] = wsn
+            self.forcing["wind_stress_tangnt"] = wst
+        self.julian_day = fz.julian_day0(case)
+        self.tran_ids = [params.tran_id1, params.tran_id2]
+
+    def advance(self, it: int, eta_node: Optional[np.ndarray] = None) -> "abi.ForcingArrays":
+        """Fill the forcing of step `it`. With eta_node the river face velocity is computed
+        here (compute_boundary_velocity.f90:32-46); otherwise the device computes it."""
+        c = self.case
+        elapsed = it * c.dt
+        self.julian_day = self.julian_day + c.dt / 86400.0
+        sp_tide = fz.spinup(c.tide_spinup, c.tide_spinup_period, elapsed)
+        sp_bc = fz.spinup(c.baroclinic_spinup, c.baroclinic_spinup_period, elapsed) \
+            if c.baroclinic_flag == 1 else 1.0
+        f = self.forcing
+        f.scalars["elapsed_time"] = elapsed
+        f.scalars["spinup_function_tide"] = sp_tide
+        f.scalars["spinup_function_baroclinic"] = sp_bc
+        old, new = fz.boundary_eta(c, it, sp_tide)
+        f["eta_at_ob_old"] = old
+        f["eta_at_ob_new"] = new
+        ob, qb = fz.tvd_boundary_concentrations(c, self.julian_day, sp_bc, self.params.maxtran2,
+                                                self.tran_ids)
+        f["con_at_ob"] = ob
+        f["con_at_Qb"] = qb
+        if c.rivers:
+            q = fz.river_discharge(c, self.julian_day)
+            f["Q_add"] = q
+            if eta_node is not None:
+                m = self.mesh
+                isf = m["isflowside3"]
+                nf = m.view2("nodenum_at_face")
+                qu = np.zeros(max(len(c.rivers), 1))
+                for j in np.flatnonzero(isf > 0):
+                    i = isf[j] - 1
+                    depth = m["h_face"][j] + (eta_node[nf[0, j] - 1] + eta_node[nf[1, j] - 1]) * 0.5
+                    qu[i] = -q[i] / (m["face_length"][j] * depth)
+                f["Qu_boundary"] = qu
+        return f
+
+
+class Scenario:
+    """One scenario advanced on the device."""
+
+    def __init__(self, case: Case, precision: int = 0, wind=(0.0, 0.0), model: Optional[host.GpuModel] = None):
+        self.case = case
+        self.params, self.mesh, state0 = prep.build(case, precision)
+        if case.rivers:
+            self.params.river_on_device = 1
+        self.model = model or host.GpuModel(self.mesh, self.params)
+        self.model.set_state(state0)
+        self.model.stage("eos")          # initial density, as prepare_gom does
+        self.host_forcing = HostForcing(case, self.params, self.mesh, wind)
+        self.forcing = self.host_forcing.forcing
+        self.model.set_forcing(self.forcing)   # static fields once
+        self.it = 0
+
+    def step(self) -> None:
+        self.it += 1
+        self.host_forcing.advance(self.it)
+        self.model.set_forcing(self.forcing, only=DYNAMIC)
+        self.model.step()
+
+    def run(self, n: int) -> None:
+        for _ in range(n):
+            self.step()
+
+    def active_cell_layers(self) -> int:
+        """Sum over wet cells of (top - bottom + 1) for the current state."""
+        s = abi.state_arrays(self.params)
+        self.model.get_state(s, only=["top_layer_at_element"])
+        top = s["top_layer_at_element"].astype(np.int64)
+        bot = self.mesh["bottom_layer_at_element"].astype(np.int64)
+        return int(np.sum(np.where(top > 0, top - bot + 1, 0)))

@@ -98,3 +98,37 @@ def tvd_boundary_concentrations(case: Case, julian_day: float, sp_baroclinic: fl
                     v = series_value(ser[sid - 1], u2)
                     qb[sub, t, i] = sp_baroclinic * (v if v is not None else 0.0)
     return ob.ravel(order="F"), qb.ravel(order="F")
+
+
+def wind_stress(mesh, u: float, v: float, rho_a: float = 1.29, rho_o: float = 1025.0,
+                formula: int = 2, spinup_wind: float = 1.0):
+    """Face wind stress for a uniform wind (calculate_wind_stress.f90:19-66, wind_flag 1).
+
+    Returns (Gamma_T, wind_stress_normal, wind_stress_tangnt) per face.
+    """
+    f32 = np.float32
+    rho_ratio = rho_a / rho_o
+    coef = {1: (0.75, 0.067, 4.0), 2: (0.61, 0.063, 6.0), 3: (0.80, 0.065, 1.0)}[formula]
+    # Cda_min/max: the inner sums are default-real (single precision) in the reference
+    cmin = 0.001 * float(f32(coef[0]) + f32(coef[1]) * f32(coef[2]))
+    cmax = 0.001 * float(f32(coef[0]) + f32(coef[1]) * f32(50.0))
+    ct = mesh["cos_theta"]
+    st = mesh["sin_theta"]
+    ws = math.sqrt(u * u + v * v)
+    wn = u * ct + v * st
+    wt = -u * st + v * ct
+    cda = 0.001 * (coef[0] + coef[1] * ws)
+    cda = min(max(cda, cmin), cmax)
+    gt = cda * rho_ratio * ws
+    n = ct.shape[0]
+    return np.full(n, gt), gt * wn * spinup_wind, gt * wt * spinup_wind
+
+
+def river_discharge(case: Case, julian_day: float):
+    """Q_add per river cell (calculate_Q.f90:12-30, linear interpolation)."""
+    q = np.zeros(max(len(case.rivers), 1))
+    u2 = julian_day * 86400.0
+    for i, r in enumerate(case.rivers):
+        v = series_value(case.q_series[r.q_ser_id - 1], u2)
+        q[i] = (v if v is not None else 0.0) * r.portion
+    return q

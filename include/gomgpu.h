@@ -90,11 +90,13 @@ extern "C" {
 /* Per-step forcing computed by the host's 2-D boundary routines (main.f90:84-161,
  * heat_term_by_term.f90) and the boundary concentrations of the transport sub-cycles
  * (solve_transport_equation_v19.f90:229-534): con_at_ob is (Nt, maxtran2, OB) and
- * con_at_Qb (Nt, maxtran2, QB), fastest index first. */
+ * con_at_Qb (Nt, maxtran2, QB), fastest index first. With params.river_on_device = 1 the
+ * river face velocity Qu_boundary is computed on the device from the discharge Q_add and
+ * the current face depth (compute_boundary_velocity.f90:32-46) instead of being passed. */
 #define GOMGPU_FORCING_REAL(X) \
   X(Gamma_T, J) X(wind_stress_normal, J) X(wind_stress_tangnt, J) X(air_p1, P) \
   X(eta_at_ob_old, OB) X(eta_at_ob_new, OB) X(Qu_boundary, QB) X(phi_sz, LE) \
-  X(con_at_ob, NTOB) X(con_at_Qb, NTQB)
+  X(con_at_ob, NTOB) X(con_at_Qb, NTQB) X(Q_add, QB)
 
 /* Scalars of main.inp and set-up (read_main_inp.f90). */
 #define GOMGPU_PARAMS_INT(X) \
@@ -103,7 +105,7 @@ extern "C" {
   X(advection_flag) X(ELM_backtrace_flag) X(ELM_min_iter) X(ELM_max_iter) \
   X(bf_flag) X(baroclinic_flag) X(ana_density) X(i_density_flag) \
   X(transport_flag) X(trans_sub_iter) X(h_flux_limiter) X(v_flux_limiter) \
-  X(heat_option) X(max_iteration_pcg) X(precision)
+  X(heat_option) X(max_iteration_pcg) X(precision) X(river_on_device)
 
 #define GOMGPU_PARAMS_REAL(X) \
   X(dt) X(theta) X(gravity) X(rho_o) X(MSL) X(dry_depth) X(adv_onoff_depth) \
@@ -218,7 +220,12 @@ int gomgpu_pcg_iterations(gomgpu_ctx *ctx);
 int gomgpu_sync(gomgpu_ctx *ctx);
 /* Text of the last error (static storage of the context). */
 const char *gomgpu_last_error(gomgpu_ctx *ctx);
-/* Per-kernel event timing of the last step: fills up to n (name, milliseconds) pairs. */
+/* Event timing. enable != 0 turns it on (turning it on resets the kernel averages);
+ * fills up to n (name, milliseconds) pairs: the GOMGPU_NUM_STAGES stage times of the last
+ * step, then the average launch time of each hot kernel (k_pcg, k_elm, k_momentum,
+ * k_tvd_column) since timing was turned on. Timing synchronises after every timed kernel,
+ * so it is for measurement passes only. */
+#define GOMGPU_NUM_KTIMERS 4
 int gomgpu_timings(gomgpu_ctx *ctx, int enable, const char **names, double *ms, int n);
 void gomgpu_destroy(gomgpu_ctx *ctx);
 
