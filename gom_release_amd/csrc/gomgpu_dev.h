@@ -50,7 +50,8 @@ struct DevDims {
 struct DevWork {
   // free surface / PCG (E rows, ELL with the diagonal and 4 neighbour slots)
   double *fs_diag, *fs_off, *fs_rhs, *pcg_x, *pcg_r, *pcg_d0, *pcg_d1, *pcg_q, *pcg_s, *pcg_minv;
-  double *pcg_part;       // per-block partial sums
+  double *pcg_part;       // per-block partial sums (4 buffers)
+  double *pcg_pub;        // 6 E published row values (single-barrier PCG)
   int *pcg_info;          // [0] iterations of the last solve, [1] converged flag
   unsigned *pcg_bar;      // grid barrier state
   double *face_sdz;       // per face sum_k dz * AinvDeltaZ1 (top down)
@@ -111,7 +112,8 @@ struct PcgArgs {
   const double *diag, *off, *b;
   const int32_t *col;     // adj_cellnum_at_cell, [slot][cell], 1-based, 0 = none
   const int32_t *nfaces;  // tri_or_quad
-  double *x, *r, *d0, *d1, *q, *s, *minv, *partA, *partB;
+  double *x, *r, *d0, *d1, *q, *s, *minv, *partA, *partB, *partC, *partD;
+  double *pub;        // 6n published row values of the single-barrier variant
   int *info;
   unsigned *bar;      // [0] arrival counter, [1] generation (custom grid barrier)
   int n, maxit;

@@ -50,6 +50,10 @@ __global__ void k_budget(DevMesh, DevState, DevDims, double *);
 __global__ void k_copy(double *, const double *, int);
 extern "C" hipError_t gom_launch_pcg(const PcgArgs *args, int max_blocks, int cooperative, hipStream_t st);
 extern "C" int gom_pcg_max_blocks(int device);
+extern "C" hipError_t gom_launch_pcg_reg(const PcgArgs *args, int nb, int bs, int cooperative, hipStream_t st);
+extern "C" int gom_pcg_reg_capacity(int device, int n, int nb, int bs);
+extern "C" hipError_t gom_launch_pcg_cg1(const PcgArgs *args, int nb, int bs, int cooperative, hipStream_t st);
+extern "C" int gom_pcg_cg1_capacity(int device, int n, int nb, int bs);
 
 enum { KT_PCG = 0, KT_ELM = 1, KT_MOMENTUM = 2, KT_TVD_COLUMN = 3 };
 static const char *kKernelNames[GOMGPU_NUM_KTIMERS] = {"k_pcg", "k_elm", "k_momentum", "k_tvd_column"};
@@ -71,6 +75,9 @@ struct gomgpu_ctx {
   int pcg_cap = 0;         // tuned cap of the PCG grid (0 = capacity)
   int pcg_barrier = 1;     // 0 cooperative-groups sync, 1 custom barrier
   int pcg_coop = 1;        // launch through hipLaunchCooperativeKernel
+  int pcg_mode = 1;        // 0 strided k_pcg, 1 register-resident k_pcg_reg, 2 single-barrier k_pcg_cg1
+  int pcg_bs = 1024;       // threads per block of k_pcg_reg
+  int pcg_nb = 0;          // blocks of k_pcg_reg (0 = rows / (4 * bs))
   int ncu = 0;
   int last_iters = 0;
   std::string err;
@@ -285,6 +292,9 @@ extern "C" int gomgpu_create(const gomgpu_mesh *mesh, const gomgpu_params *param
     if (const char *e = getenv("GOMGPU_PCG_BLOCKS")) c->pcg_cap = atoi(e);
     if (const char *e = getenv("GOMGPU_PCG_BARRIER")) c->pcg_barrier = atoi(e);
     if (const char *e = getenv("GOMGPU_PCG_COOPERATIVE")) c->pcg_coop = atoi(e);
+    if (const char *e = getenv("GOMGPU_PCG_MODE")) c->pcg_mode = atoi(e);
+    if (const char *e = getenv("GOMGPU_PCG_BS")) c->pcg_bs = atoi(e);
+    if (const char *e = getenv("GOMGPU_PCG_NB")) c->pcg_nb = atoi(e);
     if (c->pcg_blocks <= 0) {
       rc = seterr(c, GOMGPU_E_HIP, "cannot size the PCG grid");
       goto fail;
@@ -293,8 +303,9 @@ extern "C" int gomgpu_create(const gomgpu_mesh *mesh, const gomgpu_params *param
     if ((rc = dalloc(c, &w.fs_diag, E)) || (rc = dalloc(c, &w.fs_off, 4 * E)) || (rc = dalloc(c, &w.fs_rhs, E)) ||
         (rc = dalloc(c, &w.pcg_x, E)) || (rc = dalloc(c, &w.pcg_r, E)) || (rc = dalloc(c, &w.pcg_d0, E)) ||
         (rc = dalloc(c, &w.pcg_d1, E)) || (rc = dalloc(c, &w.pcg_q, E)) || (rc = dalloc(c, &w.pcg_s, E)) ||
-        (rc = dalloc(c, &w.pcg_minv, E)) || (rc = dalloc(c, &w.pcg_part, 2 * (long)c->pcg_blocks)) ||
-        (rc = dalloc(c, &w.pcg_info, 2)) || (rc = dalloc(c, &w.pcg_bar, 2)) || (rc = dalloc(c, &w.face_sdz, J)) || (rc = dalloc(c, &w.face_sun, J)) ||
+        (rc = dalloc(c, &w.pcg_minv, E)) || (rc = dalloc(c, &w.pcg_part, 4 * (long)(c->pcg_blocks > 4096 ? c->pcg_blocks : 4096))) ||
+        (rc = dalloc(c, &w.pcg_pub, 6 * E)) ||
+        (rc = dalloc(c, &w.pcg_info, 2)) || (rc = dalloc(c, &w.pcg_bar, 512)) || (rc = dalloc(c, &w.face_sdz, J)) || (rc = dalloc(c, &w.face_sun, J)) ||
         (rc = dalloc(c, &w.face_sg, J)))
       goto fail;
     if ((rc = dalloc(c, &w.con, NT * (L + 2) * E)) || (rc = dalloc(c, &w.con_new, NT * L * E)) ||
@@ -481,7 +492,13 @@ static int stage_impl(gomgpu_ctx *c, int id) {
       a.s = c->w.pcg_s;
       a.minv = c->w.pcg_minv;
       a.partA = c->w.pcg_part;
-      a.partB = c->w.pcg_part + c->pcg_blocks;
+      {
+        long pstride = c->pcg_blocks > 4096 ? c->pcg_blocks : 4096;
+        a.partB = c->w.pcg_part + pstride;
+        a.partC = c->w.pcg_part + 2 * pstride;
+        a.partD = c->w.pcg_part + 3 * pstride;
+      }
+      a.pub = c->w.pcg_pub;
       a.info = c->w.pcg_info;
       a.bar = c->w.pcg_bar;
       a.barrier_mode = c->pcg_barrier;
@@ -490,10 +507,27 @@ static int stage_impl(gomgpu_ctx *c, int id) {
       a.tol = p.error_tolerance;
       KT_BEGIN(KT_PCG);
       {
-        int cap = c->pcg_blocks;
-        if (c->pcg_cap > 0 && c->pcg_cap < cap) cap = c->pcg_cap;
-        if (!c->pcg_coop && cap > c->ncu) cap = c->ncu;  // plain launch: one block per CU
-        HIPCHK(c, gom_launch_pcg(&a, cap, c->pcg_coop, st));
+        hipError_t le = hipErrorInvalidValue;
+        if (c->pcg_mode == 1 || c->pcg_mode == 2) {
+          int rmax = c->pcg_bs >= 1024 ? 2 : c->pcg_bs >= 512 ? 4 : 8;
+          int nb = c->pcg_nb > 0 ? c->pcg_nb : (int)((d.E + (long)rmax * c->pcg_bs - 1) / ((long)rmax * c->pcg_bs));
+          if (nb < 1) nb = 1;
+          if (nb > 4096) nb = 4096;
+          if (!c->pcg_coop && nb > c->ncu) nb = c->ncu;  // plain launch: at most one block per CU
+          if (c->pcg_mode == 2) {
+            if (nb <= gom_pcg_cg1_capacity(c->device, d.E, nb, c->pcg_bs))
+              le = gom_launch_pcg_cg1(&a, nb, c->pcg_bs, c->pcg_coop, st);
+          } else if (nb <= gom_pcg_reg_capacity(c->device, d.E, nb, c->pcg_bs)) {
+            le = gom_launch_pcg_reg(&a, nb, c->pcg_bs, c->pcg_coop, st);
+          }
+        }
+        if (le == hipErrorInvalidValue) {
+          int cap = c->pcg_blocks;
+          if (c->pcg_cap > 0 && c->pcg_cap < cap) cap = c->pcg_cap;
+          if (!c->pcg_coop && cap > c->ncu) cap = c->ncu;
+          le = gom_launch_pcg(&a, cap, c->pcg_coop, st);
+        }
+        HIPCHK(c, le);
       }
       KT_END(KT_PCG);
       c->pcg_ran = true;

@@ -225,3 +225,434 @@ extern "C" int gom_pcg_max_blocks(int device) {
   if (hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, device) != hipSuccess) return 0;
   return per_cu * ncu;
 }
+
+// ---------------------------------------------------------------------------------------
+// Register-resident PCG. Each thread owns up to R rows for the whole solve and keeps their
+// matrix entries and x, r, M^-1, d (current direction) in registers; only the quantities
+// the neighbouring rows read -- s and the direction -- go through memory (d is
+// double-buffered because a row's new direction is read by its neighbours in the next
+// SpMV). Per iteration and row this moves ~32 bytes instead of ~170. Rows are assigned in
+// contiguous chunks, grouped so that the blocks the dispatcher places on one XCD (block id
+// modulo 8) own one contiguous eighth of the rows, which keeps most neighbour reads in that
+// XCD's cache. The per-row arithmetic is the same as k_pcg, statement for statement.
+//
+// Barrier: two levels -- the blocks of one XCD group arrive on their own counter (own
+// 128-byte line), the last of them arrives on the top counter, the last group bumps the
+// generation; waiters spin on the generation.
+#define PCG_XCD 8
+
+__device__ __forceinline__ void bar2_sync(unsigned *bar, unsigned nb, unsigned &gen) {
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    unsigned g = gen;
+    unsigned grp = blockIdx.x % PCG_XCD;
+    unsigned ngrp = nb < PCG_XCD ? nb : PCG_XCD;
+    unsigned gsize = nb / PCG_XCD + (grp < nb % PCG_XCD ? 1u : 0u);
+    unsigned *cnt = bar + 32 * (grp + 1);  // 128-byte apart
+    __threadfence();
+    unsigned a = atomicAdd(cnt, 1u);
+    bool last = false;
+    if (a == gsize - 1) {
+      atomicExch(cnt, 0u);
+      __threadfence();
+      unsigned b = atomicAdd(&bar[0], 1u);
+      if (b == ngrp - 1) {
+        atomicExch(&bar[0], 0u);
+        __threadfence();
+        atomicAdd(&bar[1], 1u);
+        last = true;
+      }
+    }
+    if (!last)
+      while (__hip_atomic_load(&bar[1], __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) == g) __builtin_amdgcn_s_sleep(1);
+    __threadfence();
+  }
+  gen = gen + 1;
+  __syncthreads();
+}
+
+__device__ double block_sum_bs(double v, double *lds) {
+  v = wave_sum(v);
+  int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+  if (lane == 0) lds[wid] = v;
+  __syncthreads();
+  double r = 0.0;
+  if (threadIdx.x == 0)
+    for (int w = 0; w < (int)(blockDim.x >> 6); w++) r += lds[w];
+  __syncthreads();
+  return r;
+}
+
+__device__ double grid_total_bs(const double *part, int nb, double *lds) {
+  double v = 0.0;
+  for (int b = threadIdx.x; b < nb; b += blockDim.x) v += part[b];
+  double r = block_sum_bs(v, lds);
+  if (threadIdx.x == 0) lds[16] = r;
+  __syncthreads();
+  r = lds[16];
+  __syncthreads();
+  return r;
+}
+
+template <int R, int BS>
+__global__ void __launch_bounds__(BS) k_pcg_reg(PcgArgs a) {
+  __shared__ double lds[24];
+  __shared__ unsigned gen0;
+  if (threadIdx.x == 0) gen0 = __hip_atomic_load(&a.bar[1], __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT);
+  __syncthreads();
+  unsigned gen = gen0;
+  const int n = a.n, nb = gridDim.x, bs = blockDim.x;
+  // chunk of this block: XCD group g = b % 8 owns a contiguous run of chunks
+  const int b = blockIdx.x;
+  const int ng = nb < PCG_XCD ? nb : PCG_XCD;
+  const int g = b % ng, slot = b / ng;
+  int chunk = 0;
+  for (int q = 0; q < g; q++) chunk += nb / ng + (q < nb % ng ? 1 : 0);
+  chunk += slot;
+  const int rows_per_block = (n + nb - 1) / nb;
+  const int row0 = chunk * rows_per_block;
+  int rid[R];
+  double dg[R], of[R][4], xr[R], rr[R], mi[R], dc[R], qv[R];
+  int cl[R][4], nf[R];
+#pragma unroll
+  for (int k = 0; k < R; k++) {
+    int i = row0 + threadIdx.x + k * bs;
+    bool in = (threadIdx.x + k * bs < rows_per_block) && i < n;
+    rid[k] = in ? i : -1;
+    nf[k] = in ? a.nfaces[i] : 0;
+    dg[k] = in ? a.diag[i] : 1.0;
+#pragma unroll
+    for (int l = 0; l < 4; l++) {
+      cl[k][l] = (in && l < nf[k]) ? a.col[ix(l, i, n)] : 0;
+      of[k][l] = (in && l < nf[k]) ? a.off[ix(l, i, n)] : 0.0;
+    }
+    xr[k] = in ? a.x[i] : 0.0;
+  }
+  // r = b - A x ; M^-1 ; d = M^-1 r ; delta = r.d
+  double part = 0.0;
+#pragma unroll
+  for (int k = 0; k < R; k++) {
+    if (rid[k] < 0) continue;
+    int i = rid[k];
+    double sum = dg[k] * xr[k];
+#pragma unroll
+    for (int l = 0; l < 4; l++)
+      if (cl[k][l] != 0) sum = sum + of[k][l] * a.x[cl[k][l] - 1];
+    mi[k] = 1.0 / dg[k];
+    rr[k] = a.b[i] - sum;
+    dc[k] = 0.0 + mi[k] * rr[k];
+    a.d0[i] = dc[k];
+    part = part + rr[k] * dc[k];
+  }
+  double bsum = block_sum_bs(part, lds);
+  if (threadIdx.x == 0) a.partB[b] = bsum;
+  bar2_sync(a.bar, nb, gen);
+  double delta_new = grid_total_bs(a.partB, nb, lds);
+  double delta_0 = delta_new;
+  double beta = 0.0;
+  double *dcur = a.d0, *dnext = a.d1;
+  bool first = true;
+  int cgit2 = 0;
+  for (int cgit = 1; cgit <= a.maxit; cgit++) {
+    part = 0.0;
+#pragma unroll
+    for (int k = 0; k < R; k++) {
+      if (rid[k] < 0) continue;
+      double di = first ? dc[k] : a.s[rid[k]] + beta * dc[k];
+      double sum = dg[k] * di;
+#pragma unroll
+      for (int l = 0; l < 4; l++) {
+        int c = cl[k][l];
+        if (c != 0) {
+          double dn = first ? dcur[c - 1] : a.s[c - 1] + beta * dcur[c - 1];
+          sum = sum + of[k][l] * dn;
+        }
+      }
+      if (!first) {
+        dnext[rid[k]] = di;
+        dc[k] = di;
+      }
+      qv[k] = sum;
+      part = part + di * sum;
+    }
+    if (!first) {
+      double *t = dcur;
+      dcur = dnext;
+      dnext = t;
+    }
+    bsum = block_sum_bs(part, lds);
+    if (threadIdx.x == 0) a.partA[b] = bsum;
+    bar2_sync(a.bar, nb, gen);
+    double dq = grid_total_bs(a.partA, nb, lds);
+    double alpha = delta_new / dq;
+    part = 0.0;
+#pragma unroll
+    for (int k = 0; k < R; k++) {
+      if (rid[k] < 0) continue;
+      xr[k] = xr[k] + alpha * dc[k];
+      rr[k] = rr[k] - alpha * qv[k];
+      double si = 0.0 + mi[k] * rr[k];
+      a.s[rid[k]] = si;
+      part = part + rr[k] * si;
+    }
+    bsum = block_sum_bs(part, lds);
+    if (threadIdx.x == 0) a.partB[b] = bsum;
+    bar2_sync(a.bar, nb, gen);
+    double delta_old = delta_new;
+    delta_new = grid_total_bs(a.partB, nb, lds);
+    beta = delta_new / delta_old;
+    first = false;
+    if (sqrt(delta_new * delta_new / (delta_0 * delta_0)) < a.tol) {
+      cgit2 = cgit;
+      break;
+    }
+  }
+#pragma unroll
+  for (int k = 0; k < R; k++)
+    if (rid[k] >= 0) a.x[rid[k]] = xr[k];
+  if (b == 0 && threadIdx.x == 0) {
+    a.info[0] = cgit2;
+    a.info[1] = cgit2 > 0 ? 1 : 0;
+  }
+}
+
+// Launch of the register-resident PCG with bs threads per block (256, 512 or 1024) and the
+// smallest R (rows per thread) that lets nb blocks cover the system; R is limited so the
+// per-thread rows stay in registers (R <= 2 at 1024 threads, 4 at 512, 8 at 256). Returns
+// hipErrorInvalidValue when the shape cannot cover the system (the caller then uses k_pcg).
+static const void *pcg_reg_kernel(int bs, int R) {
+  if (bs == 1024) {
+    if (R <= 1) return (const void *)k_pcg_reg<1, 1024>;
+    if (R <= 2) return (const void *)k_pcg_reg<2, 1024>;
+  } else if (bs == 512) {
+    if (R <= 1) return (const void *)k_pcg_reg<1, 512>;
+    if (R <= 2) return (const void *)k_pcg_reg<2, 512>;
+    if (R <= 4) return (const void *)k_pcg_reg<4, 512>;
+  } else if (bs == 256) {
+    if (R <= 1) return (const void *)k_pcg_reg<1, 256>;
+    if (R <= 2) return (const void *)k_pcg_reg<2, 256>;
+    if (R <= 4) return (const void *)k_pcg_reg<4, 256>;
+    if (R <= 8) return (const void *)k_pcg_reg<8, 256>;
+  }
+  return nullptr;
+}
+
+extern "C" hipError_t gom_launch_pcg_reg(const PcgArgs *args, int nb, int bs, int cooperative, hipStream_t st) {
+  PcgArgs a = *args;
+  long per_block = ((long)a.n + nb - 1) / nb;
+  int R = (int)((per_block + bs - 1) / bs);
+  const void *kfun = pcg_reg_kernel(bs, R);
+  if (!kfun) return hipErrorInvalidValue;
+  void *kargs[] = {&a};
+  if (cooperative) return hipLaunchCooperativeKernel(kfun, dim3(nb), dim3(bs), kargs, 0, st);
+  return hipLaunchKernel(kfun, dim3(nb), dim3(bs), kargs, 0, st);
+}
+
+// Co-resident capacity (blocks) of the variant used for n rows at bs threads and nb blocks.
+extern "C" int gom_pcg_reg_capacity(int device, int n, int nb, int bs) {
+  long per_block = ((long)n + nb - 1) / nb;
+  int R = (int)((per_block + bs - 1) / bs);
+  const void *k = pcg_reg_kernel(bs, R);
+  if (!k) return 0;
+  int per_cu = 0, ncu = 0;
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k, bs, 0) != hipSuccess) return 0;
+  if (hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, device) != hipSuccess) return 0;
+  return per_cu * ncu;
+}
+
+// ---------------------------------------------------------------------------------------
+// Single-barrier variant (Chronopoulos-Gear form of the same Jacobi-preconditioned CG).
+// In exact arithmetic it produces the iterates of pre_conj_grad.f90 (same x_i, r_i, the
+// same gamma_i = r_i . M^-1 r_i that the stop test uses); the two dot products of an
+// iteration are fused into one reduction, and a row forms the new preconditioned residual
+// u_j = M^-1_j (r_j - alpha (w_j + beta s_j)) of each neighbour itself from the neighbour's
+// published (r, w, s), so the iteration needs one grid barrier instead of two.
+//   p = u + beta p ; s = w + beta s ; x += alpha p ; r -= alpha s ; u = M^-1 r ; w = A u
+//   gamma = r.u ; delta = w.u ; beta' = gamma'/gamma ; alpha' = gamma'/(delta' - beta' gamma'/alpha)
+template <int R, int BS>
+__global__ void __launch_bounds__(BS) k_pcg_cg1(PcgArgs a) {
+  __shared__ double lds[48];
+  __shared__ unsigned gen0;
+  if (threadIdx.x == 0) gen0 = __hip_atomic_load(&a.bar[1], __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT);
+  __syncthreads();
+  unsigned gen = gen0;
+  const int n = a.n, nb = gridDim.x, bs = blockDim.x;
+  const int b = blockIdx.x;
+  const int ng = nb < PCG_XCD ? nb : PCG_XCD;
+  const int g = b % ng, slot = b / ng;
+  int chunk = 0;
+  for (int q = 0; q < g; q++) chunk += nb / ng + (q < nb % ng ? 1 : 0);
+  chunk += slot;
+  const int rows_per_block = (n + nb - 1) / nb;
+  const int row0 = chunk * rows_per_block;
+  // published per-row values, double-buffered by iteration parity: [r | w | s] x 2
+  double *pub = a.pub;
+  int rid[R], cl[R][4];
+  double dg[R], of[R][4], mi[R], xr[R], rr[R], uu[R], ww[R], pp[R], ss[R];
+#pragma unroll
+  for (int k = 0; k < R; k++) {
+    int i = row0 + threadIdx.x + k * bs;
+    bool in = (threadIdx.x + k * bs < rows_per_block) && i < n;
+    rid[k] = in ? i : -1;
+    int nf = in ? a.nfaces[i] : 0;
+    dg[k] = in ? a.diag[i] : 1.0;
+#pragma unroll
+    for (int l = 0; l < 4; l++) {
+      cl[k][l] = (in && l < nf) ? a.col[ix(l, i, n)] : 0;
+      of[k][l] = (in && l < nf) ? a.off[ix(l, i, n)] : 0.0;
+    }
+    xr[k] = in ? a.x[i] : 0.0;
+    pp[k] = 0.0;
+    ss[k] = 0.0;
+  }
+  // r0 = b - A x0 (pre_conj_grad.f90:48-60); publish r0
+#pragma unroll
+  for (int k = 0; k < R; k++) {
+    if (rid[k] < 0) continue;
+    int i = rid[k];
+    double sum = dg[k] * xr[k];
+#pragma unroll
+    for (int l = 0; l < 4; l++)
+      if (cl[k][l] != 0) sum = sum + of[k][l] * a.x[cl[k][l] - 1];
+    mi[k] = 1.0 / dg[k];
+    rr[k] = a.b[i] - sum;
+    pub[i] = rr[k];
+  }
+  bar2_sync(a.bar, nb, gen);
+  // u0 = M^-1 r0 ; w0 = A u0 ; gamma0 = r0.u0 ; delta0 = w0.u0
+  double pg = 0.0, pd = 0.0;
+#pragma unroll
+  for (int k = 0; k < R; k++) {
+    if (rid[k] < 0) continue;
+    uu[k] = 0.0 + mi[k] * rr[k];
+    double sum = dg[k] * uu[k];
+#pragma unroll
+    for (int l = 0; l < 4; l++) {
+      int c = cl[k][l];
+      if (c != 0) sum = sum + of[k][l] * (0.0 + (1.0 / a.diag[c - 1]) * pub[c - 1]);
+    }
+    ww[k] = sum;
+    pg = pg + rr[k] * uu[k];
+    pd = pd + ww[k] * uu[k];
+  }
+  int par = 1;  // buffer the next publication goes to
+#pragma unroll
+  for (int k = 0; k < R; k++) {
+    if (rid[k] < 0) continue;
+    size_t o = (size_t)par * 3 * n;
+    pub[o + rid[k]] = rr[k];
+    pub[o + n + rid[k]] = ww[k];
+    pub[o + 2 * (size_t)n + rid[k]] = ss[k];
+  }
+  double bg = block_sum_bs(pg, lds), bd = block_sum_bs(pd, lds + 16);
+  if (threadIdx.x == 0) {
+    a.partA[b] = bg;
+    a.partB[b] = bd;
+  }
+  bar2_sync(a.bar, nb, gen);
+  double gamma = grid_total_bs(a.partA, nb, lds);
+  double delta = grid_total_bs(a.partB, nb, lds + 24);
+  const double gamma_0 = gamma;
+  double alpha = gamma / delta, beta = 0.0;
+  int cgit2 = 0;
+  for (int cgit = 1; cgit <= a.maxit; cgit++) {
+    const size_t oin = (size_t)par * 3 * n;       // neighbours' (r, w, s) of this iteration
+    par ^= 1;
+    const size_t oout = (size_t)par * 3 * n;
+    pg = 0.0;
+    pd = 0.0;
+#pragma unroll
+    for (int k = 0; k < R; k++) {
+      if (rid[k] < 0) continue;
+      pp[k] = uu[k] + beta * pp[k];
+      ss[k] = ww[k] + beta * ss[k];
+      xr[k] = xr[k] + alpha * pp[k];
+      rr[k] = rr[k] - alpha * ss[k];
+      uu[k] = 0.0 + mi[k] * rr[k];
+      double sum = dg[k] * uu[k];
+#pragma unroll
+      for (int l = 0; l < 4; l++) {
+        int c = cl[k][l];
+        if (c != 0) {
+          double rj = pub[oin + c - 1], wj = pub[oin + n + c - 1], sj = pub[oin + 2 * (size_t)n + c - 1];
+          double sn = wj + beta * sj;
+          double rn = rj - alpha * sn;
+          double un = 0.0 + (1.0 / a.diag[c - 1]) * rn;
+          sum = sum + of[k][l] * un;
+        }
+      }
+      ww[k] = sum;
+      pg = pg + rr[k] * uu[k];
+      pd = pd + ww[k] * uu[k];
+      pub[oout + rid[k]] = rr[k];
+      pub[oout + n + rid[k]] = ww[k];
+      pub[oout + 2 * (size_t)n + rid[k]] = ss[k];
+    }
+    bg = block_sum_bs(pg, lds);
+    bd = block_sum_bs(pd, lds + 16);
+    // alternate partial buffers so a slow block still reading the previous totals is safe
+    double *PA = (cgit & 1) ? a.partC : a.partA, *PB = (cgit & 1) ? a.partD : a.partB;
+    if (threadIdx.x == 0) {
+      PA[b] = bg;
+      PB[b] = bd;
+    }
+    bar2_sync(a.bar, nb, gen);
+    double gamma_new = grid_total_bs(PA, nb, lds);
+    double delta_new = grid_total_bs(PB, nb, lds + 24);
+    double beta_new = gamma_new / gamma;
+    double alpha_new = gamma_new / (delta_new - beta_new * gamma_new / alpha);
+    gamma = gamma_new;
+    beta = beta_new;
+    alpha = alpha_new;
+    if (sqrt(gamma * gamma / (gamma_0 * gamma_0)) < a.tol) {
+      cgit2 = cgit;
+      break;
+    }
+  }
+#pragma unroll
+  for (int k = 0; k < R; k++)
+    if (rid[k] >= 0) a.x[rid[k]] = xr[k];
+  if (b == 0 && threadIdx.x == 0) {
+    a.info[0] = cgit2;
+    a.info[1] = cgit2 > 0 ? 1 : 0;
+  }
+}
+
+static const void *pcg_cg1_kernel(int bs, int R) {
+  if (bs == 1024) {
+    if (R <= 1) return (const void *)k_pcg_cg1<1, 1024>;
+    if (R <= 2) return (const void *)k_pcg_cg1<2, 1024>;
+  } else if (bs == 512) {
+    if (R <= 1) return (const void *)k_pcg_cg1<1, 512>;
+    if (R <= 2) return (const void *)k_pcg_cg1<2, 512>;
+    if (R <= 4) return (const void *)k_pcg_cg1<4, 512>;
+  } else if (bs == 256) {
+    if (R <= 1) return (const void *)k_pcg_cg1<1, 256>;
+    if (R <= 2) return (const void *)k_pcg_cg1<2, 256>;
+    if (R <= 4) return (const void *)k_pcg_cg1<4, 256>;
+    if (R <= 8) return (const void *)k_pcg_cg1<8, 256>;
+  }
+  return nullptr;
+}
+
+extern "C" hipError_t gom_launch_pcg_cg1(const PcgArgs *args, int nb, int bs, int cooperative, hipStream_t st) {
+  PcgArgs a = *args;
+  long per_block = ((long)a.n + nb - 1) / nb;
+  int R = (int)((per_block + bs - 1) / bs);
+  const void *kfun = pcg_cg1_kernel(bs, R);
+  if (!kfun) return hipErrorInvalidValue;
+  void *kargs[] = {&a};
+  if (cooperative) return hipLaunchCooperativeKernel(kfun, dim3(nb), dim3(bs), kargs, 0, st);
+  return hipLaunchKernel(kfun, dim3(nb), dim3(bs), kargs, 0, st);
+}
+
+extern "C" int gom_pcg_cg1_capacity(int device, int n, int nb, int bs) {
+  long per_block = ((long)n + nb - 1) / nb;
+  int R = (int)((per_block + bs - 1) / bs);
+  const void *k = pcg_cg1_kernel(bs, R);
+  if (!k) return 0;
+  int per_cu = 0, ncu = 0;
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k, bs, 0) != hipSuccess) return 0;
+  if (hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, device) != hipSuccess) return 0;
+  return per_cu * ncu;
+}
