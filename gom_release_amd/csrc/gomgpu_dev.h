@@ -14,6 +14,7 @@
 #include "../../include/gomgpu.h"
 
 #define GOM_LMAX 64  // largest supported maxlayer (per-thread column arrays)
+#define PCG_MAX_SLOT_BLOCKS 1024  // most blocks of a slot-exchange PCG launch
 
 struct DevMesh {
 #define DM_R(name, ext) const double *name;
@@ -52,7 +53,8 @@ struct DevWork {
   double *fs_diag, *fs_off, *fs_rhs, *pcg_x, *pcg_r, *pcg_d0, *pcg_d1, *pcg_q, *pcg_s, *pcg_minv;
   double *pcg_part;       // per-block partial sums (4 buffers)
   double *pcg_pub;        // 6 E published row values (single-barrier PCG)
-  int *pcg_info;          // [0] iterations of the last solve, [1] converged flag
+  unsigned long long *pcg_slots;  // 6 * PCG_MAX_SLOT_BLOCKS partial-sum slots
+  int *pcg_info;          // [0] iterations of the last solve, [1] converged flag, [2] exchange abort
   unsigned *pcg_bar;      // grid barrier state
   double *face_sdz;       // per face sum_k dz * AinvDeltaZ1 (top down)
   double *face_sun;       // per face sum_k dz * un
@@ -114,6 +116,7 @@ struct PcgArgs {
   const int32_t *nfaces;  // tri_or_quad
   double *x, *r, *d0, *d1, *q, *s, *minv, *partA, *partB, *partC, *partD;
   double *pub;        // 6n published row values of the single-barrier variant
+  unsigned long long *slots;  // 3 x PCG_MAX_SLOT_BLOCKS x 2 partial-sum slots (slot exchange)
   int *info;
   unsigned *bar;      // [0] arrival counter, [1] generation (custom grid barrier)
   int n, maxit;

@@ -52,8 +52,9 @@ extern "C" hipError_t gom_launch_pcg(const PcgArgs *args, int max_blocks, int co
 extern "C" int gom_pcg_max_blocks(int device);
 extern "C" hipError_t gom_launch_pcg_reg(const PcgArgs *args, int nb, int bs, int cooperative, hipStream_t st);
 extern "C" int gom_pcg_reg_capacity(int device, int n, int nb, int bs);
-extern "C" hipError_t gom_launch_pcg_cg1(const PcgArgs *args, int nb, int bs, int cooperative, hipStream_t st);
-extern "C" int gom_pcg_cg1_capacity(int device, int n, int nb, int bs);
+extern "C" hipError_t gom_launch_pcg_cg1(const PcgArgs *args, int nb, int bs, int slots, int cooperative,
+                                         hipStream_t st);
+extern "C" int gom_pcg_cg1_capacity(int device, int n, int nb, int bs, int slots);
 
 enum { KT_PCG = 0, KT_ELM = 1, KT_MOMENTUM = 2, KT_TVD_COLUMN = 3 };
 static const char *kKernelNames[GOMGPU_NUM_KTIMERS] = {"k_pcg", "k_elm", "k_momentum", "k_tvd_column"};
@@ -75,8 +76,9 @@ struct gomgpu_ctx {
   int pcg_cap = 0;         // tuned cap of the PCG grid (0 = capacity)
   int pcg_barrier = 1;     // 0 cooperative-groups sync, 1 custom barrier
   int pcg_coop = 1;        // launch through hipLaunchCooperativeKernel
-  int pcg_mode = 1;        // 0 strided k_pcg, 1 register-resident k_pcg_reg, 2 single-barrier k_pcg_cg1
-  int pcg_bs = 1024;       // threads per block of k_pcg_reg
+  int pcg_mode = 3;        // 0 strided k_pcg, 1 register-resident k_pcg_reg, 2 single-barrier k_pcg_cg1,
+                           // 3 k_pcg_cg1 with the slot exchange instead of the grid barrier
+  int pcg_bs = 512;        // threads per block of k_pcg_reg / k_pcg_cg1
   int pcg_nb = 0;          // blocks of k_pcg_reg (0 = rows / (4 * bs))
   int ncu = 0;
   int last_iters = 0;
@@ -304,8 +306,8 @@ extern "C" int gomgpu_create(const gomgpu_mesh *mesh, const gomgpu_params *param
         (rc = dalloc(c, &w.pcg_x, E)) || (rc = dalloc(c, &w.pcg_r, E)) || (rc = dalloc(c, &w.pcg_d0, E)) ||
         (rc = dalloc(c, &w.pcg_d1, E)) || (rc = dalloc(c, &w.pcg_q, E)) || (rc = dalloc(c, &w.pcg_s, E)) ||
         (rc = dalloc(c, &w.pcg_minv, E)) || (rc = dalloc(c, &w.pcg_part, 4 * (long)(c->pcg_blocks > 4096 ? c->pcg_blocks : 4096))) ||
-        (rc = dalloc(c, &w.pcg_pub, 6 * E)) ||
-        (rc = dalloc(c, &w.pcg_info, 2)) || (rc = dalloc(c, &w.pcg_bar, 512)) || (rc = dalloc(c, &w.face_sdz, J)) || (rc = dalloc(c, &w.face_sun, J)) ||
+        (rc = dalloc(c, &w.pcg_pub, 6 * E)) || (rc = dalloc(c, &w.pcg_slots, 6 * PCG_MAX_SLOT_BLOCKS)) ||
+        (rc = dalloc(c, &w.pcg_info, 3)) || (rc = dalloc(c, &w.pcg_bar, 512)) || (rc = dalloc(c, &w.face_sdz, J)) || (rc = dalloc(c, &w.face_sun, J)) ||
         (rc = dalloc(c, &w.face_sg, J)))
       goto fail;
     if ((rc = dalloc(c, &w.con, NT * (L + 2) * E)) || (rc = dalloc(c, &w.con_new, NT * L * E)) ||
@@ -320,7 +322,7 @@ extern "C" int gomgpu_create(const gomgpu_mesh *mesh, const gomgpu_params *param
         (rc = dalloc(c, &w.err, 2)))
       goto fail;
   }
-  if (hipHostMalloc((void **)&c->h_err, 4 * sizeof(int)) != hipSuccess) {
+  if (hipHostMalloc((void **)&c->h_err, 5 * sizeof(int)) != hipSuccess) {
     rc = seterr(c, GOMGPU_E_NOMEM, "hipHostMalloc failed");
     goto fail;
   }
@@ -499,6 +501,7 @@ static int stage_impl(gomgpu_ctx *c, int id) {
         a.partD = c->w.pcg_part + 3 * pstride;
       }
       a.pub = c->w.pcg_pub;
+      a.slots = c->w.pcg_slots;
       a.info = c->w.pcg_info;
       a.bar = c->w.pcg_bar;
       a.barrier_mode = c->pcg_barrier;
@@ -508,15 +511,16 @@ static int stage_impl(gomgpu_ctx *c, int id) {
       KT_BEGIN(KT_PCG);
       {
         hipError_t le = hipErrorInvalidValue;
-        if (c->pcg_mode == 1 || c->pcg_mode == 2) {
+        if (c->pcg_mode >= 1 && c->pcg_mode <= 3) {
           int rmax = c->pcg_bs >= 1024 ? 2 : c->pcg_bs >= 512 ? 4 : 8;
           int nb = c->pcg_nb > 0 ? c->pcg_nb : (int)((d.E + (long)rmax * c->pcg_bs - 1) / ((long)rmax * c->pcg_bs));
           if (nb < 1) nb = 1;
           if (nb > 4096) nb = 4096;
           if (!c->pcg_coop && nb > c->ncu) nb = c->ncu;  // plain launch: at most one block per CU
-          if (c->pcg_mode == 2) {
-            if (nb <= gom_pcg_cg1_capacity(c->device, d.E, nb, c->pcg_bs))
-              le = gom_launch_pcg_cg1(&a, nb, c->pcg_bs, c->pcg_coop, st);
+          if (c->pcg_mode == 2 || c->pcg_mode == 3) {
+            int slots = c->pcg_mode == 3 && nb <= c->pcg_bs && nb <= PCG_MAX_SLOT_BLOCKS;
+            if (nb <= gom_pcg_cg1_capacity(c->device, d.E, nb, c->pcg_bs, slots))
+              le = gom_launch_pcg_cg1(&a, nb, c->pcg_bs, slots, c->pcg_coop, st);
           } else if (nb <= gom_pcg_reg_capacity(c->device, d.E, nb, c->pcg_bs)) {
             le = gom_launch_pcg_reg(&a, nb, c->pcg_bs, c->pcg_coop, st);
           }
@@ -659,7 +663,7 @@ extern "C" int gomgpu_run(gomgpu_ctx *c, int nsteps) {
 extern "C" int gomgpu_sync(gomgpu_ctx *c) {
   if (!c) return GOMGPU_E_ARG;
   HIPCHK(c, hipMemcpyAsync(c->h_err, c->w.err, 2 * sizeof(int), hipMemcpyDeviceToHost, c->st));
-  HIPCHK(c, hipMemcpyAsync(c->h_err + 2, c->w.pcg_info, 2 * sizeof(int), hipMemcpyDeviceToHost, c->st));
+  HIPCHK(c, hipMemcpyAsync(c->h_err + 2, c->w.pcg_info, 3 * sizeof(int), hipMemcpyDeviceToHost, c->st));
   HIPCHK(c, hipStreamSynchronize(c->st));
   bool ran = c->pcg_ran;
   c->pcg_ran = false;
@@ -669,6 +673,10 @@ extern "C" int gomgpu_sync(gomgpu_ctx *c) {
     snprintf(buf, sizeof buf, "model error %d at entity %d", c->h_err[0], c->h_err[1]);
     c->err = buf;
     return GOMGPU_E_MODEL;
+  }
+  if (ran && c->h_err[4] != 0) {
+    c->err = "PCG slot exchange timed out (blocks not co-resident)";
+    return GOMGPU_E_HIP;
   }
   if (ran && c->h_err[3] == 0) {
     c->err = "PCG did not converge within max_iteration_pcg";

@@ -242,6 +242,9 @@ extern "C" int gom_pcg_max_blocks(int device) {
 #define PCG_XCD 8
 
 __device__ __forceinline__ void bar2_sync(unsigned *bar, unsigned nb, unsigned &gen) {
+  // every wave's global stores acknowledged by L2 before thread 0's agent-scope release
+  // writes L2 back (CDNA counts stores in vmcnt; the workgroup barrier does not wait)
+  __builtin_amdgcn_s_waitcnt(0);
   __syncthreads();
   if (threadIdx.x == 0) {
     unsigned g = gen;
@@ -469,7 +472,79 @@ extern "C" int gom_pcg_reg_capacity(int device, int n, int nb, int bs) {
 // published (r, w, s), so the iteration needs one grid barrier instead of two.
 //   p = u + beta p ; s = w + beta s ; x += alpha p ; r -= alpha s ; u = M^-1 r ; w = A u
 //   gamma = r.u ; delta = w.u ; beta' = gamma'/gamma ; alpha' = gamma'/(delta' - beta' gamma'/alpha)
-template <int R, int BS>
+// Slot exchange of the per-block partial sums, used instead of a grid barrier inside the
+// single-barrier PCG loop. slots holds 3 x nb pairs of 64-bit words; a pair is either the
+// sentinel or the block's (gamma, delta) partials of one iteration. In iteration it block b
+// (1) resets its pair of iteration it+1 to the sentinel, (2) releases (its row values and
+// the reset become visible), (3) stores its pair of iteration it, then polls the nb pairs of
+// iteration it until none is the sentinel and acquires. A pair is reset only after every
+// block has published the iteration after the one it held, i.e. after every block has read
+// it. The totals are summed in the same order as grid_total_bs.
+#define PCG_SLOT_SENTINEL 0x7ff4c0ffee0f1a6bull
+#define PCG_CANONICAL_NAN 0x7ff8000000000000ull
+
+__device__ __forceinline__ unsigned long long slot_bits(double v) {
+  unsigned long long u = __double_as_longlong(v);
+  return u == PCG_SLOT_SENTINEL ? PCG_CANONICAL_NAN : u;
+}
+
+__device__ __forceinline__ double block_total_bcast(double v, double *lds) {
+  double r = block_sum_bs(v, lds);
+  if (threadIdx.x == 0) lds[16] = r;
+  __syncthreads();
+  r = lds[16];
+  __syncthreads();
+  return r;
+}
+
+// A poll gives up after PCG_SLOT_SPIN_LIMIT spins (seconds) and raises the abort word;
+// every poll then stops waiting, the solve runs out its iterations on NaN totals and the
+// host reports the abort. With all blocks co-resident this never triggers.
+#define PCG_SLOT_SPIN_LIMIT (1u << 22)
+
+__device__ void slot_exchange(unsigned long long *slots, int nb, int it, double bg, double bd,
+                              double *lds, int *abort, double &gamma, double &delta) {
+  __builtin_amdgcn_s_waitcnt(0);  // this wave's row-value stores (and loads) complete
+  __syncthreads();
+  const int cur = it % 3, nxt = (it + 1) % 3;
+  if (threadIdx.x == 0) {
+    unsigned long long *rst = slots + 2 * ((size_t)nxt * nb + blockIdx.x);
+    __hip_atomic_store(rst, PCG_SLOT_SENTINEL, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    __hip_atomic_store(rst + 1, PCG_SLOT_SENTINEL, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+    unsigned long long *out = slots + 2 * ((size_t)cur * nb + blockIdx.x);
+    __hip_atomic_store(out, slot_bits(bg), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    __hip_atomic_store(out + 1, slot_bits(bd), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  }
+  double vg = 0.0, vd = 0.0;
+  if ((int)threadIdx.x < nb) {
+    for (int q = threadIdx.x; q < nb; q += blockDim.x) {
+      const unsigned long long *src = slots + 2 * ((size_t)cur * nb + q);
+      unsigned long long x, y;
+      unsigned spins = 0;
+      for (;;) {
+        x = __hip_atomic_load(src, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        y = __hip_atomic_load(src + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        if (x != PCG_SLOT_SENTINEL && y != PCG_SLOT_SENTINEL) break;
+        if ((++spins & 255u) == 0) {
+          if (__hip_atomic_load(abort, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0) break;
+          if (spins >= PCG_SLOT_SPIN_LIMIT) {
+            __hip_atomic_store(abort, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            break;
+          }
+        }
+        __builtin_amdgcn_s_sleep(1);
+      }
+      vg += __longlong_as_double(x);
+      vd += __longlong_as_double(y);
+    }
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+  }
+  gamma = block_total_bcast(vg, lds);
+  delta = block_total_bcast(vd, lds + 24);
+}
+
+template <int R, int BS, bool SLOTS>
 __global__ void __launch_bounds__(BS) k_pcg_cg1(PcgArgs a) {
   __shared__ double lds[48];
   __shared__ unsigned gen0;
@@ -485,6 +560,10 @@ __global__ void __launch_bounds__(BS) k_pcg_cg1(PcgArgs a) {
   chunk += slot;
   const int rows_per_block = (n + nb - 1) / nb;
   const int row0 = chunk * rows_per_block;
+  if (SLOTS && b == 0 && threadIdx.x == 0) a.info[2] = 0;  // abort word, ordered by the first barrier
+  if (SLOTS && threadIdx.x < 6)  // this block's pair in each of the 3 slots
+    __hip_atomic_store(a.slots + 2 * ((size_t)(threadIdx.x >> 1) * nb + b) + (threadIdx.x & 1),
+                       PCG_SLOT_SENTINEL, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
   // published per-row values, double-buffered by iteration parity: [r | w | s] x 2
   double *pub = a.pub;
   int rid[R], cl[R][4];
@@ -590,15 +669,20 @@ __global__ void __launch_bounds__(BS) k_pcg_cg1(PcgArgs a) {
     }
     bg = block_sum_bs(pg, lds);
     bd = block_sum_bs(pd, lds + 16);
-    // alternate partial buffers so a slow block still reading the previous totals is safe
-    double *PA = (cgit & 1) ? a.partC : a.partA, *PB = (cgit & 1) ? a.partD : a.partB;
-    if (threadIdx.x == 0) {
-      PA[b] = bg;
-      PB[b] = bd;
+    double gamma_new, delta_new;
+    if (SLOTS) {
+      slot_exchange(a.slots, nb, cgit, bg, bd, lds, a.info + 2, gamma_new, delta_new);
+    } else {
+      // alternate partial buffers so a slow block still reading the previous totals is safe
+      double *PA = (cgit & 1) ? a.partC : a.partA, *PB = (cgit & 1) ? a.partD : a.partB;
+      if (threadIdx.x == 0) {
+        PA[b] = bg;
+        PB[b] = bd;
+      }
+      bar2_sync(a.bar, nb, gen);
+      gamma_new = grid_total_bs(PA, nb, lds);
+      delta_new = grid_total_bs(PB, nb, lds + 24);
     }
-    bar2_sync(a.bar, nb, gen);
-    double gamma_new = grid_total_bs(PA, nb, lds);
-    double delta_new = grid_total_bs(PB, nb, lds + 24);
     double beta_new = gamma_new / gamma;
     double alpha_new = gamma_new / (delta_new - beta_new * gamma_new / alpha);
     gamma = gamma_new;
@@ -618,38 +702,45 @@ __global__ void __launch_bounds__(BS) k_pcg_cg1(PcgArgs a) {
   }
 }
 
-static const void *pcg_cg1_kernel(int bs, int R) {
+template <bool SLOTS>
+static const void *pcg_cg1_kernel_t(int bs, int R) {
   if (bs == 1024) {
-    if (R <= 1) return (const void *)k_pcg_cg1<1, 1024>;
-    if (R <= 2) return (const void *)k_pcg_cg1<2, 1024>;
+    if (R <= 1) return (const void *)k_pcg_cg1<1, 1024, SLOTS>;
+    if (R <= 2) return (const void *)k_pcg_cg1<2, 1024, SLOTS>;
   } else if (bs == 512) {
-    if (R <= 1) return (const void *)k_pcg_cg1<1, 512>;
-    if (R <= 2) return (const void *)k_pcg_cg1<2, 512>;
-    if (R <= 4) return (const void *)k_pcg_cg1<4, 512>;
+    if (R <= 1) return (const void *)k_pcg_cg1<1, 512, SLOTS>;
+    if (R <= 2) return (const void *)k_pcg_cg1<2, 512, SLOTS>;
+    if (R <= 4) return (const void *)k_pcg_cg1<4, 512, SLOTS>;
   } else if (bs == 256) {
-    if (R <= 1) return (const void *)k_pcg_cg1<1, 256>;
-    if (R <= 2) return (const void *)k_pcg_cg1<2, 256>;
-    if (R <= 4) return (const void *)k_pcg_cg1<4, 256>;
-    if (R <= 8) return (const void *)k_pcg_cg1<8, 256>;
+    if (R <= 1) return (const void *)k_pcg_cg1<1, 256, SLOTS>;
+    if (R <= 2) return (const void *)k_pcg_cg1<2, 256, SLOTS>;
+    if (R <= 4) return (const void *)k_pcg_cg1<4, 256, SLOTS>;
+    if (R <= 8) return (const void *)k_pcg_cg1<8, 256, SLOTS>;
   }
   return nullptr;
 }
 
-extern "C" hipError_t gom_launch_pcg_cg1(const PcgArgs *args, int nb, int bs, int cooperative, hipStream_t st) {
+static const void *pcg_cg1_kernel(int bs, int R, int slots) {
+  return slots ? pcg_cg1_kernel_t<true>(bs, R) : pcg_cg1_kernel_t<false>(bs, R);
+}
+
+extern "C" hipError_t gom_launch_pcg_cg1(const PcgArgs *args, int nb, int bs, int slots, int cooperative,
+                                         hipStream_t st) {
   PcgArgs a = *args;
   long per_block = ((long)a.n + nb - 1) / nb;
   int R = (int)((per_block + bs - 1) / bs);
-  const void *kfun = pcg_cg1_kernel(bs, R);
+  if (slots && (nb > bs || nb > PCG_MAX_SLOT_BLOCKS)) return hipErrorInvalidValue;
+  const void *kfun = pcg_cg1_kernel(bs, R, slots);
   if (!kfun) return hipErrorInvalidValue;
   void *kargs[] = {&a};
   if (cooperative) return hipLaunchCooperativeKernel(kfun, dim3(nb), dim3(bs), kargs, 0, st);
   return hipLaunchKernel(kfun, dim3(nb), dim3(bs), kargs, 0, st);
 }
 
-extern "C" int gom_pcg_cg1_capacity(int device, int n, int nb, int bs) {
+extern "C" int gom_pcg_cg1_capacity(int device, int n, int nb, int bs, int slots) {
   long per_block = ((long)n + nb - 1) / nb;
   int R = (int)((per_block + bs - 1) / bs);
-  const void *k = pcg_cg1_kernel(bs, R);
+  const void *k = pcg_cg1_kernel(bs, R, slots);
   if (!k) return 0;
   int per_cu = 0, ncu = 0;
   if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k, bs, 0) != hipSuccess) return 0;
