@@ -49,6 +49,15 @@ def pcg_bytes_per_iteration(mesh, params) -> float:
     return params.maxele * (12.0 * zbar + 108.0)
 
 
+def step_bytes(F: int, C: int, N: int, J: int, E: int, zbar: float, nt: int, ntr: int, iters: float,
+               s: int = 8) -> float:
+    """Algorithmic (compulsory) HBM bytes of one step: every stage's irreducible 3-D inputs
+    read once and outputs written once, dz/dzhalf derived, the new->old swap free; F, C, N
+    active face-, cell- and node-layers, T tracers, Nt sub-cycles, I PCG iterations."""
+    b3 = s * ((20 + 3 * (nt - 1)) * F + (15 + (3 + 2 * ntr) * (nt - 1)) * C + 12 * N)
+    return b3 + 24.0 * J + E * (12.0 * zbar + 20.0) + iters * E * (12.0 * zbar + 108.0)
+
+
 def cpu_baseline(case, wind, budget_s: float = 15.0, max_steps: int = 12):
     """Time the CPU restatement (oracle, one thread) on the same workload."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
@@ -79,6 +88,25 @@ def cpu_baseline(case, wind, budget_s: float = 15.0, max_steps: int = 12):
                       f"single thread, {t_total:.1f} s"}
 
 
+def copy_calibration(nbytes: int) -> None:
+    """One device-to-device hipMemcpy of nbytes (reads nbytes, writes nbytes) through the
+    HIP runtime the library links, so profiler counters can be scaled to known traffic."""
+    import ctypes as C
+    from gom_release_amd import host
+    hip = host.lib()
+    src, dst = C.c_void_p(), C.c_void_p()
+    if hip.hipMalloc(C.byref(src), C.c_size_t(nbytes)) or hip.hipMalloc(C.byref(dst), C.c_size_t(nbytes)):
+        raise RuntimeError("hipMalloc failed")
+    try:
+        if hip.hipMemset(src, 1, C.c_size_t(nbytes)) or hip.hipDeviceSynchronize():
+            raise RuntimeError("hipMemset failed")
+        if hip.hipMemcpy(dst, src, C.c_size_t(nbytes), 3) or hip.hipDeviceSynchronize():  # 3: device to device
+            raise RuntimeError("hipMemcpy failed")
+    finally:
+        hip.hipFree(src)
+        hip.hipFree(dst)
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -88,6 +116,9 @@ def main() -> int:
     ap.add_argument("--cells", default="200k")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--profile-steps", type=int, default=5)
+    ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
+                    help="also run one device-to-device copy of MIB MiB (calibrates rocprofv3 "
+                         "FETCH_SIZE/WRITE_SIZE against known traffic)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build measures one GPU only (got --gpus {args.gpus}); "
@@ -98,6 +129,9 @@ def main() -> int:
     if host.device_count() < 1:
         print("bench.py: no GPU visible", file=sys.stderr)
         return 1
+    if args.copy_calibration > 0:
+        copy_calibration(args.copy_calibration << 20)
+        print(f"[bench] calibration copy of {args.copy_calibration} MiB done", file=sys.stderr, flush=True)
     wind = (10.0, 0.0)
     case = make_case(args.cells)
     prec = 0 if args.precision == "fp64" else 1
@@ -137,9 +171,17 @@ def main() -> int:
     achieved = pcg_bytes / (pcg_ms * 1e-3) / 1e9
     roof = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
-            "kernel": "k_pcg", "pcg_iterations": round(it_mean, 1),
+            "kernel": "k_pcg_cg1 (PCG solve; timer k_pcg)", "pcg_iterations": round(it_mean, 1),
             "kernel_ms": {k: round(v, 4) for k, v in kernels.items()},
             "stage_ms": {k: round(v, 4) for k, v in stages.items()}}
+    F, Cl, N = sc.active_layers()
+    zbar = pcg_bytes_per_iteration(sc.mesh, sc.params) / sc.params.maxele / 12.0 - 9.0
+    bstep = step_bytes(F, Cl, N, sc.params.maxface, sc.params.maxele, zbar, max(case.trans_sub_iter, 1),
+                       2, it_mean)
+    step_ms = 1e3 * elapsed / args.steps
+    roof["step"] = {"algorithmic_bytes": round(bstep), "achieved": round(bstep / (step_ms * 1e-3) / 1e9, 1),
+                    "frac": round(bstep / (step_ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
+                    "active_face_cell_node_layers": [F, Cl, N]}
     vol, salt = model.budgets()
     out = {
         "metric": "cell-layer-steps/s at 1 GPU + %HBM roofline; p50/p99 scenario latency",

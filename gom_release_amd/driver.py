@@ -93,6 +93,17 @@ class Scenario:
         for _ in range(n):
             self.step()
 
+    def active_layers(self):
+        """(face-layers, cell-layers, node-layers) of the wet entities in the current state."""
+        s = abi.state_arrays(self.params)
+        self.model.get_state(s, only=["top_layer_at_element", "top_layer_at_face", "top_layer_at_node"])
+        out = []
+        for ent in ("face", "element", "node"):
+            top = s[f"top_layer_at_{ent}"].astype(np.int64)
+            bot = self.mesh[f"bottom_layer_at_{ent}"].astype(np.int64)
+            out.append(int(np.sum(np.where(top > 0, top - bot + 1, 0))))
+        return tuple(out)
+
     def active_cell_layers(self) -> int:
         """Sum over wet cells of (top - bottom + 1) for the current state."""
         s = abi.state_arrays(self.params)

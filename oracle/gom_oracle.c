@@ -1628,7 +1628,7 @@ double oracle_rho_unesco(double t, double s) {
 }
 
 /* pressure correction, calculate_density_full.f90:75-101 (parity unpinned) */
-static double rho_pressure(double rho_surface, double t, double s, double pres) {
+double oracle_rho_pressure(double rho_surface, double t, double s, double pres) {
   double t2 = t * t, t3 = t2 * t, t4 = t3 * t;
   double sq = sqrt(s), sq3 = sq * sq * sq, p2 = pres * pres;
   double sbm = (double)19652.21f + (double)148.4206f * t - (double)2.327105f * t2 + (double)1.360477e-2f * t3 -
@@ -1662,7 +1662,7 @@ static int eos_column(const gomgpu_mesh *m, const gomgpu_params *p, int L, int f
           else
             pres = pres + (double)1.0e-5f * ptmp;
         }
-        rcol[k - 1] = rho_pressure(rcol[k - 1], rt, rs, pres);
+        rcol[k - 1] = oracle_rho_pressure(rcol[k - 1], rt, rs, pres);
       }
     } else {
       rcol[k - 1] = 1000.0 + (double)0.7f * rs;

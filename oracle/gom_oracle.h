@@ -65,6 +65,8 @@ int oracle_bilinear(double x1, double x2, double x3, double x4, double y1, doubl
                     double y3, double y4, double x, double y, double *xi, double *eta,
                     double *shapef);
 double oracle_rho_unesco(double t, double s);
+/* UNESCO-1980 secant-bulk-modulus correction, pres in bar (calculate_density_full.f90:75-101). */
+double oracle_rho_pressure(double rho_surface, double t, double s, double pres);
 /* Last PCG iteration count and the first failing entity of the last error. */
 int oracle_last_pcg_iterations(void);
 int oracle_last_error_entity(void);
