@@ -198,7 +198,8 @@ static int validate(const gomgpu_params *p, std::string &why) {
   }
   if (p->baroclinic_flag == 1 && p->ana_density != 0 && p->ana_density != 1)
     return why = "ana_density must be 0 or 1", 1;
-  if (p->precision != GOMGPU_FP64 && p->precision != GOMGPU_MIXED) return why = "bad precision", 1;
+  if (p->precision == GOMGPU_MIXED) return why = "GOMGPU_MIXED is not implemented in this build", 1;
+  if (p->precision != GOMGPU_FP64) return why = "bad precision", 1;
   return 0;
 }
 
