@@ -59,7 +59,7 @@ class Forcing(C.Structure):
 
 
 STAGES = ["friction", "elm", "momentum", "freesurf", "velocity", "layers", "nodevel",
-          "transport", "eos", "update"]
+          "transport", "eos", "update", "turbulence"]
 
 
 def extent(code: str, p: Params) -> int:

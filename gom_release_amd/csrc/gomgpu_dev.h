@@ -69,6 +69,8 @@ struct DevWork {
   double *obk, *qbk;      // (L, OB, NT), (L, QB, NT)
   double *node_new;       // (L, P, NT)
   double *face_new;       // (L, J, NT)
+  // turbulence closure: face diffusivity (0:L, J) before the face -> cell map
+  double *turb_kvf;
   // budgets
   double *budget;         // [0] volume, [1] salt
   int *err;               // [0] code, [1] entity
@@ -103,6 +105,16 @@ __device__ __forceinline__ void latch(int *err, int code, int entity) {
 struct MomParams {
   double dt, g, th, rho_o, smag, Ah0, Kh0, spinup_bc;
   int baroclinic;
+};
+
+// Closure parameters (the module variables of turbulence_closure_v1.f90) and the
+// single-precision constant 16.6**(2.0/3) evaluated on the host.
+struct TurbParams {
+  double dt, g, rho_o, MSL, fs_coef;
+  double q2min, psimin, eps_min, bgdiff, diffmax, xlmin1, xlmin2;
+  double cmiu0, cpsi1, cpsi2, rpub, rmub, rnub, schk, schpsi;
+  double h1_pp, h2_pp, vdmax_pp1, vdmax_pp2, vdmin_pp1, vdmin_pp2, tdmin_pp;
+  int flag, model, stab;
 };
 
 struct TvdArgs {

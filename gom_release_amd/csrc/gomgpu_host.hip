@@ -16,7 +16,7 @@ __global__ void k_friction(DevMesh, DevState, DevDims, int, double, double);
 __global__ void k_elm_substeps(DevMesh, DevState, DevDims, double, double, int, int);
 __global__ void k_elm(DevMesh, DevState, DevDims, double, double, double, int *);
 __global__ void k_elm_copy(DevState, long);
-__global__ void k_momentum(DevMesh, DevState, DevForcing, DevDims, MomParams, int *);
+__global__ void __launch_bounds__(256) k_momentum(DevMesh, DevState, DevForcing, DevDims, MomParams, int *);
 __global__ void k_river_velocity(DevMesh, DevState, DevForcing, DevDims, double *);
 __global__ void k_fs_face(DevMesh, DevState, DevWork, DevDims, double);
 __global__ void k_fs_cell(DevMesh, DevState, DevForcing, DevWork, DevDims, double, double, int *);
@@ -46,6 +46,8 @@ __global__ void k_tvd_cellnode_out(DevState, DevWork, DevDims);
 __global__ void k_eos_nodeface(DevMesh, DevState, DevDims, double, int, int, int *);
 __global__ void k_eos_cell(DevMesh, DevState, DevDims, int *);
 __global__ void k_bars(DevMesh, DevState, DevDims);
+__global__ void k_turbulence(DevMesh, DevState, DevForcing, DevWork, DevDims, TurbParams, int *);
+__global__ void k_turb_kv(DevMesh, DevState, DevWork, DevDims);
 __global__ void k_budget(DevMesh, DevState, DevDims, double *);
 __global__ void k_copy(double *, const double *, int);
 extern "C" hipError_t gom_launch_pcg(const PcgArgs *args, int max_blocks, int cooperative, hipStream_t st);
@@ -58,8 +60,14 @@ extern "C" int gom_pcg_cg1_capacity(int device, int n, int nb, int bs, int slots
 
 enum { KT_PCG = 0, KT_ELM = 1, KT_MOMENTUM = 2, KT_TVD_COLUMN = 3 };
 static const char *kKernelNames[GOMGPU_NUM_KTIMERS] = {"k_pcg", "k_elm", "k_momentum", "k_tvd_column"};
-static const char *kStageNames[GOMGPU_NUM_STAGES] = {"friction", "elm", "momentum", "freesurf", "velocity",
-                                                      "layers", "nodevel", "transport", "eos", "update"};
+static const char *kStageNames[GOMGPU_NUM_STAGES] = {"friction", "elm",       "momentum", "freesurf",
+                                                      "velocity", "layers",    "nodevel",  "transport",
+                                                      "eos",      "update",    "turbulence"};
+// Order of the stages in one step (main.f90:95-272, the closure before the transport).
+static const int kStepOrder[GOMGPU_NUM_STAGES] = {
+    GOMGPU_STAGE_FRICTION, GOMGPU_STAGE_ELM,        GOMGPU_STAGE_MOMENTUM,  GOMGPU_STAGE_FREESURF,
+    GOMGPU_STAGE_VELOCITY, GOMGPU_STAGE_LAYERS,     GOMGPU_STAGE_NODEVEL,   GOMGPU_STAGE_TURBULENCE,
+    GOMGPU_STAGE_TRANSPORT, GOMGPU_STAGE_EOS,       GOMGPU_STAGE_UPDATE};
 
 struct gomgpu_ctx {
   int device = 0;
@@ -198,6 +206,13 @@ static int validate(const gomgpu_params *p, std::string &why) {
   }
   if (p->baroclinic_flag == 1 && p->ana_density != 0 && p->ana_density != 1)
     return why = "ana_density must be 0 or 1", 1;
+  if (p->turbulence_flag < 0 || p->turbulence_flag > 3) return why = "turbulence_flag must be 0..3", 1;
+  if (p->turbulence_flag == 3) {
+    if (p->turbulence_model < 0 || p->turbulence_model > 4) return why = "turbulence_model must be 0..4", 1;
+    if (p->stability_function < 0 || p->stability_function > 1) return why = "stability_function must be 0 or 1", 1;
+    if (!(p->q2min > 0.0) || !(p->xlmin2 > 0.0) || !(p->eps_min > 0.0) || p->rnub == 0.0)
+      return why = "closure needs q2min > 0, xlmin2 > 0, eps_min > 0 and rnub != 0", 1;
+  }
   if (p->precision == GOMGPU_MIXED) return why = "GOMGPU_MIXED is not implemented in this build", 1;
   if (p->precision != GOMGPU_FP64) return why = "bad precision", 1;
   return 0;
@@ -320,7 +335,7 @@ extern "C" int gomgpu_create(const gomgpu_mesh *mesh, const gomgpu_params *param
         (rc = dalloc(c, &w.s2a, NT * L * E)) || (rc = dalloc(c, &w.obk, NT * L * OB)) ||
         (rc = dalloc(c, &w.qbk, NT * L * QB)) || (rc = dalloc(c, &w.node_new, NT * L * P)) ||
         (rc = dalloc(c, &w.face_new, NT * L * J)) || (rc = dalloc(c, &w.budget, 2 * ((E + 255) / 256) + 2)) ||
-        (rc = dalloc(c, &w.err, 2)))
+        (rc = dalloc(c, &w.err, 2)) || (rc = dalloc(c, &w.turb_kvf, (L + 1) * J)))
       goto fail;
   }
   if (hipHostMalloc((void **)&c->h_err, 5 * sizeof(int)) != hipSuccess) {
@@ -622,6 +637,43 @@ static int stage_impl(gomgpu_ctx *c, int id) {
       k_bars<<<nblk(d.P, BS), BS, 0, st>>>(c->m, c->s, d);
       break;
     }
+    case GOMGPU_STAGE_TURBULENCE: {
+      if (p.turbulence_flag == 0) break;
+      TurbParams tp;
+      tp.dt = p.dt;
+      tp.g = p.gravity;
+      tp.rho_o = p.rho_o;
+      tp.MSL = p.MSL;
+      tp.fs_coef = (double)powf(16.6f, 2.0f / 3.0f);  // 16.6**(2.0/3) in default real
+      tp.q2min = p.q2min;
+      tp.psimin = p.psimin;
+      tp.eps_min = p.eps_min;
+      tp.bgdiff = p.bgdiff;
+      tp.diffmax = p.diffmax;
+      tp.xlmin1 = p.xlmin1;
+      tp.xlmin2 = p.xlmin2;
+      tp.cmiu0 = p.cmiu0;
+      tp.cpsi1 = p.cpsi1;
+      tp.cpsi2 = p.cpsi2;
+      tp.rpub = p.rpub;
+      tp.rmub = p.rmub;
+      tp.rnub = p.rnub;
+      tp.schk = p.schk;
+      tp.schpsi = p.schpsi;
+      tp.h1_pp = p.h1_pp;
+      tp.h2_pp = p.h2_pp;
+      tp.vdmax_pp1 = p.vdmax_pp1;
+      tp.vdmax_pp2 = p.vdmax_pp2;
+      tp.vdmin_pp1 = p.vdmin_pp1;
+      tp.vdmin_pp2 = p.vdmin_pp2;
+      tp.tdmin_pp = p.tdmin_pp;
+      tp.flag = p.turbulence_flag;
+      tp.model = p.turbulence_model;
+      tp.stab = p.stability_function;
+      k_turbulence<<<nblk(d.J, 64), 64, 0, st>>>(c->m, c->s, c->f, c->w, d, tp, err);
+      k_turb_kv<<<nblk((long)(d.L + 1) * d.E, BS), BS, 0, st>>>(c->m, c->s, c->w, d);
+      break;
+    }
     default:
       return seterr(c, GOMGPU_E_ARG, "bad stage id");
   }
@@ -637,17 +689,17 @@ extern "C" int gomgpu_stage(gomgpu_ctx *c, int id) {
 extern "C" int gomgpu_step(gomgpu_ctx *c) {
   if (!c) return GOMGPU_E_ARG;
   if (c->timing) HIPCHK(c, hipEventRecord(c->ev[0], c->st));
-  for (int id = 0; id < GOMGPU_NUM_STAGES; id++) {
-    int rc = stage_impl(c, id);
+  for (int pos = 0; pos < GOMGPU_NUM_STAGES; pos++) {
+    int rc = stage_impl(c, kStepOrder[pos]);
     if (rc) return rc;
-    if (c->timing) HIPCHK(c, hipEventRecord(c->ev[id + 1], c->st));
+    if (c->timing) HIPCHK(c, hipEventRecord(c->ev[pos + 1], c->st));
   }
   if (c->timing) {
     HIPCHK(c, hipEventSynchronize(c->ev[GOMGPU_NUM_STAGES]));
-    for (int id = 0; id < GOMGPU_NUM_STAGES; id++) {
+    for (int pos = 0; pos < GOMGPU_NUM_STAGES; pos++) {
       float ms = 0;
-      hipEventElapsedTime(&ms, c->ev[id], c->ev[id + 1]);
-      c->stage_ms[id] = ms;
+      hipEventElapsedTime(&ms, c->ev[pos], c->ev[pos + 1]);
+      c->stage_ms[kStepOrder[pos]] = ms;
     }
   }
   return 0;

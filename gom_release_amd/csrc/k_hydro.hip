@@ -439,7 +439,7 @@ __device__ __forceinline__ int thomas(int n, const double *a, const double *b, c
 
 
 // solve_momentum_equation.f90:67-655; one thread per face column.
-__global__ void k_momentum(DevMesh m, DevState s, DevForcing f, DevDims d, MomParams q, int *err) {
+__global__ void __launch_bounds__(256) k_momentum(DevMesh m, DevState s, DevForcing f, DevDims d, MomParams q, int *err) {
   int j = blockIdx.x * blockDim.x + threadIdx.x;
   const int J = d.J, E = d.E, P = d.P, L = d.L;
   if (j >= J) return;

@@ -1,0 +1,390 @@
+// k_turbulence.hip - vertical turbulence closure (turbulence_closure_v1.f90:9-492).
+//
+// One thread per face column. Scheme 1 (Richardson step) and 2 (Pacanowski-Philander) are
+// per-level algebra; scheme 3 solves the k (q2) and psi equations of the generic length
+// scale closure as two tridiagonal systems per column (the Thomas recurrence of
+// tridiagonal_solver.f90, as in the momentum kernel), converts back to whole levels and
+// evaluates the quasi-equilibrium stability functions (Galperin et al. 1988 or Kantha &
+// Clayson 1994; the reference's algebraic_stress is undefined). The face diffusivity is
+// then averaged onto cells for the transport (Kv). The arithmetic follows the CPU
+// restatement statement for statement.
+#include "gomgpu_dev.h"
+
+#define TK_A1 0.92
+#define TK_A2 0.74
+#define TK_B1 16.6
+#define TK_B2 10.1
+#define TK_C1 0.08
+#define TK_C2 0.7
+#define TK_C3 0.2
+#define TLEV (GOM_LMAX + 2)
+
+/* x**e for the GLS exponents: the ones the models use (0, +-1, 2, 3, +-0.5, 1.5) by exact
+ * arithmetic, so host and device agree bit for bit; others through pow. */
+__device__ __forceinline__ double gls_pow(double x, double e) {
+  if (e == 0.0) return 1.0;
+  if (e == 1.0) return x;
+  if (e == -1.0) return 1.0 / x;
+  if (e == 2.0) return x * x;
+  if (e == 3.0) return x * x * x;
+  if (e == 0.5) return sqrt(x);
+  if (e == -0.5) return 1.0 / sqrt(x);
+  if (e == 1.5) return x * sqrt(x);
+  if (e == -1.5) return 1.0 / (x * sqrt(x));
+  if (e == -3.0) return 1.0 / (x * x * x);
+  return pow(x, e);
+}
+
+__device__ __forceinline__ void stability(int kind, double gh, double &sm, double &sh) {
+  if (gh > 0.0233) gh = 0.0233;
+  if (gh < -0.28) gh = -0.28;
+  double b2 = kind == GOMGPU_STAB_KC ? TK_B2 * (1.0 - TK_C3) : TK_B2;
+  double a2 = kind == GOMGPU_STAB_KC ? TK_A2 * (1.0 - TK_C2) : TK_A2;
+  double h = TK_A2 * (1.0 - 6.0 * TK_A1 / TK_B1) / (1.0 - 3.0 * TK_A2 * gh * (6.0 * TK_A1 + b2));
+  double m = (TK_A1 * (1.0 - 3.0 * TK_C1 - 6.0 * TK_A1 / TK_B1) + 9.0 * TK_A1 * (2.0 * TK_A1 + a2) * h * gh) /
+             (1.0 - 9.0 * TK_A1 * TK_A2 * gh);
+  sm = m;
+  sh = h;
+}
+
+// algebraic_stress at one whole level: vd, td, qd, qd2 clipped to [bgdiff, diffmax]
+__device__ __forceinline__ void alg_stress(const TurbParams &p, double q2, double xl, double rz, double &vd,
+                                           double &td, double &qd, double &qd2) {
+  double gh = rz * (xl * xl) / (2.0 * q2);
+  double sm, sh;
+  stability(p.stab, gh, sm, sh);
+  double q = sqrt(2.0 * q2);
+  double v = sm * q * xl, t = sh * q * xl, a, b;
+  if (p.model == GOMGPU_GLS_MY || p.model == GOMGPU_GLS_KL) {
+    a = 0.2 * q * xl;
+    b = a;
+  } else {
+    a = v / p.schk;
+    b = v / p.schpsi;
+  }
+  vd = dminf(p.diffmax, dmaxf(p.bgdiff, v));
+  td = dminf(p.diffmax, dmaxf(p.bgdiff, t));
+  qd = dminf(p.diffmax, dmaxf(p.bgdiff, a));
+  qd2 = dminf(p.diffmax, dmaxf(p.bgdiff, b));
+}
+
+__device__ __forceinline__ double rz_level(const TurbParams &p, const DevState &s, int J, int j, int k, int kt) {
+  if (k >= kt) return 0.0;
+  double drhodz = (AL(s.rho_face, k + 1, j, J) - AL(s.rho_face, k, j, J)) / A0(s.dzhalf_face_new, k, j, J);
+  return p.g / p.rho_o * drhodz;
+}
+
+// Thomas recurrence of tridiagonal_solver.f90:26-60 (one right-hand side), rows 0..n-1
+__device__ __forceinline__ int thomas1(int n, const double *a, const double *b, const double *c, const double *r,
+                                       double *x, double *gam) {
+  if (b[0] == 0.0) return GOMGPU_ERR_TRIDIAG_PIVOT;
+  double den = b[0];
+  x[0] = r[0] / den;
+  for (int j = 2; j <= n; j++) {
+    gam[j - 1] = c[j - 2] / den;
+    den = b[j - 1] - a[j - 1] * gam[j - 1];
+    if (den == 0.0) return GOMGPU_ERR_TRIDIAG_PIVOT;
+    x[j - 1] = (r[j - 1] - a[j - 1] * x[j - 2]) / den;
+  }
+  for (int j = n - 1; j >= 1; j--) x[j - 1] = x[j - 1] - gam[j] * x[j];
+  return 0;
+}
+
+__global__ void k_turbulence(DevMesh m, DevState s, DevForcing f, DevWork w, DevDims d, TurbParams p, int *err) {
+  int j = blockIdx.x * blockDim.x + threadIdx.x;
+  const int J = d.J, L = d.L;
+  if (j >= J) return;
+  int kt = s.top_layer_at_face[j];
+  if (kt == 0) return;
+  int kb = m.bottom_layer_at_face[j];
+  const double dt = p.dt, g = p.g;
+  double *KV = w.turb_kvf;
+  if (p.flag == 1 || p.flag == 2) {
+    double vmax = 0.0, vmin = 0.0;
+    if (p.flag == 2) {
+      double h = m.h_face[j];
+      if (h <= p.h1_pp) {
+        vmax = p.vdmax_pp1;
+        vmin = p.vdmin_pp1;
+      } else if (h < p.h2_pp) {
+        vmax = p.vdmax_pp1 + (p.vdmax_pp2 - p.vdmax_pp1) * (h - p.h1_pp) / (p.h2_pp - p.h1_pp);
+        vmin = p.vdmin_pp1 + (p.vdmin_pp2 - p.vdmin_pp1) * (h - p.h1_pp) / (p.h2_pp - p.h1_pp);
+      } else {
+        vmax = p.vdmax_pp2;
+        vmin = p.vdmin_pp2;
+      }
+    }
+    for (int k = kb; k <= kt; k++) {
+      double ri;
+      if (k == kt) {
+        ri = 0.0;
+      } else {
+        double dzh = A0(s.dzhalf_face_new, k, j, J);
+        double drhodz = (AL(s.rho_face, k + 1, j, J) - AL(s.rho_face, k, j, J)) / dzh;
+        double bvf = -g * (drhodz / p.rho_o + g / 2250000.0);
+        double dundz = (AL(s.un_face_new, k + 1, j, J) - AL(s.un_face_new, k, j, J)) / dzh;
+        double dutdz = (AL(s.vn_face_new, k + 1, j, J) - AL(s.vn_face_new, k, j, J)) / dzh;
+        double shear2 = dundz * dundz + dutdz * dutdz;
+        shear2 = dmaxf(shear2, 1.0e-10);
+        ri = dmaxf(bvf / shear2, 0.0);
+      }
+      double av, kv;
+      if (p.flag == 1) {
+        if (ri < 0.25) {
+          av = 1.0e-3;
+          kv = 1.0 * 1.0e-3;
+        } else {
+          av = 1.0e-5;
+          kv = 1.0e-5;
+        }
+      } else {
+        double t = 1.0 + 5.0 * ri;
+        av = vmax / (t * t) + vmin;
+        kv = av / (1.0 + 5.0 * ri) + p.tdmin_pp;
+      }
+      A0(s.Av, k, j, J) = av;
+      A0(KV, k, j, J) = kv;
+    }
+    return;
+  }
+  // scheme 3
+  double avc[TLEV], kvc[TLEV], qdc[TLEV], qd2c[TLEV];
+  double q2bt[TLEV], xlbt[TLEV], rzbt[TLEV], shearbt[TLEV], q2tmp[TLEV], xltmp[TLEV];
+  double a[TLEV], b[TLEV], c[TLEV], rhs[TLEV], sol[TLEV], gam[TLEV];
+  for (int k = kb; k <= kt; k++)
+    alg_stress(p, A0(s.q2, k, j, J), A0(s.xl, k, j, J), rz_level(p, s, J, j, k, kt), avc[k], kvc[k], qdc[k], qd2c[k]);
+  avc[kb - 1] = avc[kb];
+  kvc[kb - 1] = kvc[kb];
+  int n1 = TB(m.nodenum_at_face, 1, j, J) - 1, n2 = TB(m.nodenum_at_face, 2, j, J) - 1;
+  const int P = d.P;
+  for (int k = kb; k <= kt; k++) {
+    q2bt[k] = (A0(s.q2, k - 1, j, J) + A0(s.q2, k, j, J)) / 2.0;
+    xlbt[k] = (A0(s.xl, k - 1, j, J) + A0(s.xl, k, j, J)) / 2.0;
+    if (xlbt[k] < p.xlmin2) {
+      latch(err, GOMGPU_ERR_TURB_XL, j + 1);
+      return;
+    }
+    double udown = (A0(s.u_node, k - 1, n1, P) + A0(s.u_node, k - 1, n2, P)) / 2.0;
+    double uup = (A0(s.u_node, k, n1, P) + A0(s.u_node, k, n2, P)) / 2.0;
+    double vdown = (A0(s.v_node, k - 1, n1, P) + A0(s.v_node, k - 1, n2, P)) / 2.0;
+    double vup = (A0(s.v_node, k, n1, P) + A0(s.v_node, k, n2, P)) / 2.0;
+    double dzk = AL(s.dz_face_new, k, j, J);
+    double dudz = (uup - udown) / dzk, dvdz = (vup - vdown) / dzk;
+    shearbt[k] = dudz * dudz + dvdz * dvdz;
+    double drhodz;
+    if (k == kt || k == kb) {
+      drhodz = 0.0;
+    } else {
+      double rho_up = AL(s.rho_face, k, j, J) + dzk / 2.0 / A0(s.dzhalf_face_new, k, j, J) *
+                                                    (AL(s.rho_face, k + 1, j, J) - AL(s.rho_face, k, j, J));
+      double rho_down = AL(s.rho_face, k - 1, j, J) + AL(s.dz_face_new, k - 1, j, J) / 2.0 /
+                                                          A0(s.dzhalf_face_new, k - 1, j, J) *
+                                                          (AL(s.rho_face, k, j, J) - AL(s.rho_face, k - 1, j, J));
+      drhodz = (rho_up - rho_down) / dzk;
+    }
+    rzbt[k] = g / p.rho_o * drhodz;
+  }
+  int a1 = TB(m.adj_cellnum_at_face, 1, j, J), a2 = TB(m.adj_cellnum_at_face, 2, j, J);
+  double etam = a2 != 0 ? dmaxf(s.eta_cell_new[a1 - 1], s.eta_cell_new[a2 - 1]) : s.eta_cell_new[a1 - 1];
+  double hf = m.h_face[j];
+  for (int k = kb; k <= kt; k++) {
+    double dzk = AL(s.dz_face_new, k, j, J);
+    double zctr = (k == kt) ? p.MSL + etam - dzk / 2.0 : m.z_level[k] - dzk / 2.0;
+    double dists = p.MSL + etam - zctr, distb = zctr - (p.MSL - hf);
+    double xlmax = (double)0.4f * dists * distb / (hf + etam);
+    if (xlmax <= 0.0) {
+      latch(err, GOMGPU_ERR_TURB_DIST, j + 1);
+      return;
+    }
+  }
+  double wn = f.wind_stress_normal[j], wt = f.wind_stress_tangnt[j];
+  double q2fs = p.fs_coef * sqrt(wn * wn + wt * wt) / 2.0;
+  q2fs = dmaxf(q2fs, p.q2min);
+  double ubn = AL(s.un_face_new, kb, j, J), ubt = AL(s.vn_face_new, kb, j, J);
+  double q2bot = p.fs_coef * s.Cdb[j] * sqrt(ubn * ubn + ubt * ubt) / 2.0;
+  q2bot = dmaxf(q2bot, p.q2min);
+  double xlfs = dmaxf(p.xlmin1, AL(s.dz_face_new, kt, j, J) / 2.0 * (double)0.4f);
+  double xlbot = dmaxf(p.xlmin2, AL(s.dz_face_new, kb, j, J) / 2.0 * (double)0.4f);
+  int nq = kt - kb + 1;
+  double cm3 = p.cmiu0 * p.cmiu0 * p.cmiu0;
+  for (int k = 1; k <= nq; k++) {
+    int kl = kt - k + 1;
+    double dzk = AL(s.dz_face_new, kl, j, J);
+    double eleft = 0.0, eright = 0.0;
+    if (k > 1) {
+      eleft = -(qdc[kl] * dt / A0(s.dzhalf_face_new, kl, j, J));
+      a[k - 1] = eleft;
+    }
+    if (k < nq) {
+      eright = -(qdc[kl - 1] * dt / A0(s.dzhalf_face_new, kl - 1, j, J));
+      c[k - 1] = eright;
+    }
+    double prod = (avc[kl] + avc[kl - 1]) / 2.0 * shearbt[kl];
+    double buoy = (kvc[kl] + kvc[kl - 1]) / 2.0 * rzbt[kl];
+    double diss = cm3 * sqrt(q2bt[kl]) / xlbt[kl];
+    double pplus, pminus;
+    if (prod + buoy > 0.0) {
+      pplus = prod + buoy;
+      pminus = 0.0;
+    } else {
+      pplus = 0.0;
+      pminus = -prod - buoy;
+    }
+    b[k - 1] = dzk - eleft - eright + (diss + pminus / q2bt[kl]) * dt * dzk;
+    rhs[k - 1] = dzk * (q2bt[kl] + dt * pplus);
+  }
+  a[0] = 0.0;
+  c[nq - 1] = 0.0;
+  int e = thomas1(nq, a, b, c, rhs, sol, gam);
+  if (e) {
+    latch(err, e, j + 1);
+    return;
+  }
+  for (int k = 1; k <= nq; k++) {
+    int kl = kt - k + 1;
+    q2tmp[kl] = kl == kt ? q2fs : (kl == kb ? q2bot : dmaxf(sol[k - 1], p.q2min));
+  }
+  for (int k = 1; k <= nq; k++) {
+    int kl = kt - k + 1;
+    double dzk = AL(s.dz_face_new, kl, j, J);
+    double eleft = 0.0, eright = 0.0;
+    if (k > 1) {
+      eleft = -(qd2c[kl] * dt / A0(s.dzhalf_face_new, kl, j, J));
+      a[k - 1] = eleft;
+    }
+    if (k < nq) {
+      eright = -(qd2c[kl - 1] * dt / A0(s.dzhalf_face_new, kl - 1, j, J));
+      c[k - 1] = eright;
+    }
+    double bb = dzk - eleft - eright;
+    double cpsi2p;
+    if (p.model == GOMGPU_GLS_MY || p.model == GOMGPU_GLS_KL) {
+      double zctr = m.z_level[kl] - dzk / 2.0;
+      double dists = dmaxf(p.MSL + etam - zctr, p.xlmin2);
+      double distb = dmaxf(zctr - (p.MSL - hf), p.xlmin2);
+      double r1 = xlbt[kl] / (double)0.4f / distb, r2 = xlbt[kl] / (double)0.4f / dists;
+      double psi = 1.0 + (double)1.33f * (r1 * r1) + (double)0.25f * (r2 * r2);
+      cpsi2p = psi * p.cpsi2;
+    } else {
+      cpsi2p = p.cpsi2;
+    }
+    double diss = cpsi2p * cm3 * sqrt(q2bt[kl]) / xlbt[kl];
+    bb = bb + dt * dzk * diss;
+    double prod = p.cpsi1 * (avc[kl] + avc[kl - 1]) / 2.0 * shearbt[kl];
+    double cpsi3;
+    if (p.model == GOMGPU_GLS_MY) {
+      cpsi3 = (double)0.9f;
+    } else if (rzbt[kl] > 0.0) {
+      cpsi3 = 1.0;
+    } else {
+      switch (p.model) {
+        case GOMGPU_GLS_KL: cpsi3 = (double)2.53f; break;
+        case GOMGPU_GLS_KE: cpsi3 = (double)(-0.52f); break;
+        case GOMGPU_GLS_KW: cpsi3 = (double)(-0.58f); break;
+        default: cpsi3 = (double)0.1f; break;
+      }
+    }
+    double buoy = cpsi3 * (kvc[kl] + kvc[kl - 1]) / 2.0 * rzbt[kl];
+    double pplus, pminus;
+    if (prod + buoy > 0.0) {
+      pplus = prod + buoy;
+      pminus = 0.0;
+    } else {
+      pplus = 0.0;
+      pminus = -prod - buoy;
+    }
+    bb = bb + pminus / q2bt[kl] * dt * dzk;
+    if (kl == kb || kl == kt) bb = bb + dt * (double)0.4f * p.rnub * qd2c[kl] / xlbt[kl];
+    b[k - 1] = bb;
+    double tmp = gls_pow(p.cmiu0, p.rpub) * gls_pow(q2bt[kl], p.rmub) * gls_pow(xlbt[kl], p.rnub);
+    rhs[k - 1] = dzk * tmp * (1.0 + dt * pplus / q2bt[kl]);
+  }
+  a[0] = 0.0;
+  c[nq - 1] = 0.0;
+  e = thomas1(nq, a, b, c, rhs, sol, gam);
+  if (e) {
+    latch(err, e, j + 1);
+    return;
+  }
+  for (int k = 1; k <= nq; k++) {
+    int kl = kt - k + 1;
+    double q2l = dmaxf(sol[k - 1], p.psimin);
+    double x;
+    if (kl == kt)
+      x = xlfs;
+    else if (kl == kb)
+      x = xlbot;
+    else
+      x = gls_pow(q2l * gls_pow(p.cmiu0, -p.rpub) * gls_pow(q2tmp[kl], -p.rmub), 1.0 / p.rnub);
+    if (rzbt[kl] < 0.0) {
+      double upper = sqrt(-(double)0.56f * q2tmp[kl] / rzbt[kl]);
+      x = dminf(x, upper);
+    }
+    double cq = p.cmiu0 * sqrt(q2tmp[kl]);
+    double xl_max = cq * cq * cq / p.eps_min;
+    xltmp[kl] = dmaxf(p.xlmin2, dminf(xl_max, x));
+  }
+  for (int k = kb - 1; k <= kt; k++) {
+    double q, x;
+    if (k == kb - 1) {
+      q = q2bot;
+      x = xlbot;
+    } else if (k == kt) {
+      q = q2fs;
+      x = xlfs;
+    } else {
+      double r = AL(s.dz_face_new, k, j, J) / 2.0 / A0(s.dzhalf_face_new, k, j, J);
+      q = q2tmp[k] + r * (q2tmp[k + 1] - q2tmp[k]);
+      x = xltmp[k] + r * (xltmp[k + 1] - xltmp[k]);
+      if (q < 0.0 || x < p.xlmin2) {
+        latch(err, GOMGPU_ERR_TURB_NEGATIVE, j + 1);
+        return;
+      }
+    }
+    A0(s.q2, k, j, J) = q;
+    A0(s.xl, k, j, J) = x;
+  }
+  for (int k = kb; k <= kt; k++) {
+    double vd, td, qd, qd2;
+    alg_stress(p, A0(s.q2, k, j, J), A0(s.xl, k, j, J), rz_level(p, s, J, j, k, kt), vd, td, qd, qd2);
+    A0(s.Av, k, j, J) = vd;
+    A0(KV, k, j, J) = td;
+  }
+  for (int k = 0; k <= kb - 2; k++) {
+    A0(s.q2, k, j, J) = A0(s.q2, kb - 1, j, J);
+    A0(s.xl, k, j, J) = A0(s.xl, kb - 1, j, J);
+  }
+  for (int k = 1; k <= kb - 1; k++) {
+    A0(s.Av, k, j, J) = A0(s.Av, kb, j, J);
+    A0(KV, k, j, J) = A0(KV, kb, j, J);
+  }
+  for (int k = kt + 1; k <= L; k++) {
+    A0(s.q2, k, j, J) = A0(s.q2, kt, j, J);
+    A0(s.xl, k, j, J) = A0(s.xl, kt, j, J);
+    A0(s.Av, k, j, J) = A0(s.Av, kt, j, J);
+    A0(KV, k, j, J) = A0(KV, kt, j, J);
+  }
+}
+
+// Face diffusivity -> cell Kv: mean over the cell's wet faces at the level clamped to each
+// face's active range (1.0e-2 for a cell with no wet face); one thread per (level, cell).
+__global__ void k_turb_kv(DevMesh m, DevState s, DevWork w, DevDims d) {
+  long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  const int E = d.E, J = d.J, L = d.L;
+  if (t >= (long)(L + 1) * E) return;
+  int i = (int)(t % E), k = (int)(t / E);
+  if (s.top_layer_at_element[i] == 0) return;
+  int N = m.tri_or_quad[i];
+  double sum = 0.0;
+  int cnt = 0;
+  for (int l = 1; l <= N; l++) {
+    int f = TB(m.facenum_at_cell, l, i, E) - 1;
+    int kt = s.top_layer_at_face[f];
+    if (kt == 0) continue;
+    int kb = m.bottom_layer_at_face[f];
+    int kk = k < kb ? kb : (k > kt ? kt : k);
+    sum = sum + A0(w.turb_kvf, kk, f, J);
+    cnt++;
+  }
+  A0(s.Kv, k, i, E) = cnt == 0 ? (double)1.0e-2f : sum / cnt;
+}

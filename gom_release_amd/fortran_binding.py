@@ -14,8 +14,7 @@ import sys
 
 from . import abi
 
-STAGES = ["FRICTION", "ELM", "MOMENTUM", "FREESURF", "VELOCITY", "LAYERS", "NODEVEL",
-          "TRANSPORT", "EOS", "UPDATE"]
+STAGES = [s.upper() for s in abi.STAGES]
 
 
 def _enum_values():

@@ -116,6 +116,21 @@ class Case:
     rivers: List[RiverBoundary] = field(default_factory=list)
     q_series: List[Series] = field(default_factory=list)
     start: Tuple[int, int, int, int, int, int] = (2020, 1, 1, 0, 0, 0)
+    # vertical turbulence closure (turbulence_closure_v1.f90; not read by read_main_inp.f90,
+    # so these never reach the GOM input files): 0 constant Av/Kv, 1 Richardson step,
+    # 2 Pacanowski-Philander, 3 GLS; model 'my','kl','ke','kw','ub'; stability 'GA','KC'
+    turbulence_flag: int = 0
+    turbulence_model: str = "kl"
+    stability_function: str = "GA"
+    q2min: float = 5.0e-6
+    eps_min: float = 1.0e-12
+    bgdiff: float = 1.0e-6
+    diffmax: float = 1.0
+    xlmin: float = 1.0e-3
+    pp_depths: Tuple[float, float] = (20.0, 50.0)
+    pp_vdmax: Tuple[float, float] = (1.0e-2, 1.0e-2)
+    pp_vdmin: Tuple[float, float] = (1.0e-5, 1.0e-5)
+    pp_tdmin: float = 1.0e-5
 
     @property
     def maxnod(self) -> int:

@@ -55,7 +55,42 @@ def params_for(case: Case, precision: int = 0) -> "abi.Params":
         denominator_min_for_matrix=min(case.dry_depth * float(np.float32(0.5) - np.float32(1.0e-4)),
                                        float(dz.min()) * float(np.float32(0.5) - np.float32(1.0e-4))),
         delta_z_min=float(dz.min()))
+    d.update(turbulence_params(case))
     return abi.params_from_dict(d)
+
+
+# Second-equation constants of the generic length scale closure (Umlauf & Burchard 2003,
+# J. Mar. Res. 61, Table 1; MY and k-kl as in Mellor & Yamada 1982): psi = cmiu0^p k^m l^n.
+GLS_MODELS = {
+    "my": dict(rpub=0.0, rmub=1.0, rnub=1.0, cpsi1=0.9, cpsi2=0.5, schk=2.44, schpsi=2.44),
+    "kl": dict(rpub=0.0, rmub=1.0, rnub=1.0, cpsi1=0.9, cpsi2=0.5, schk=2.44, schpsi=2.44),
+    "ke": dict(rpub=3.0, rmub=1.5, rnub=-1.0, cpsi1=1.44, cpsi2=1.92, schk=1.0, schpsi=1.3),
+    "kw": dict(rpub=-1.0, rmub=0.5, rnub=-1.0, cpsi1=0.555, cpsi2=0.833, schk=2.0, schpsi=2.0),
+    "ub": dict(rpub=2.0, rmub=1.0, rnub=-0.67, cpsi1=1.0, cpsi2=1.22, schk=0.8, schpsi=1.07),
+}
+GLS_IDS = {"my": 0, "kl": 1, "ke": 2, "kw": 3, "ub": 4}
+TURBULENCE_PARAMS = ("turbulence_flag", "turbulence_model", "stability_function", "q2min", "psimin",
+                     "eps_min", "bgdiff", "diffmax", "xlmin1", "xlmin2", "cmiu0", "cpsi1", "cpsi2",
+                     "rpub", "rmub", "rnub", "schk", "schpsi", "h1_pp", "h2_pp", "vdmax_pp1",
+                     "vdmax_pp2", "vdmin_pp1", "vdmin_pp2", "tdmin_pp")
+
+
+def turbulence_params(case: Case) -> dict:
+    """Closure parameters of the ABI (the module variables of turbulence_closure_v1.f90).
+
+    cmiu0 = (2^1.5 / B1)^(1/3) with B1 = 16.6 makes the dissipation cmiu0^3 k^1.5 / l equal
+    to Mellor-Yamada's q^3 / (B1 l) with q^2 = 2 k."""
+    mdl = GLS_MODELS[case.turbulence_model]
+    cmiu0 = (2.0 ** 1.5 / 16.6) ** (1.0 / 3.0)
+    psimin = cmiu0 ** mdl["rpub"] * case.q2min ** mdl["rmub"] * case.xlmin ** mdl["rnub"]
+    return dict(
+        turbulence_flag=case.turbulence_flag, turbulence_model=GLS_IDS[case.turbulence_model],
+        stability_function={"GA": 0, "KC": 1}[case.stability_function],
+        q2min=case.q2min, psimin=psimin, eps_min=case.eps_min, bgdiff=case.bgdiff,
+        diffmax=case.diffmax, xlmin1=case.xlmin, xlmin2=case.xlmin, cmiu0=cmiu0,
+        h1_pp=case.pp_depths[0], h2_pp=case.pp_depths[1], vdmax_pp1=case.pp_vdmax[0],
+        vdmax_pp2=case.pp_vdmax[1], vdmin_pp1=case.pp_vdmin[0], vdmin_pp2=case.pp_vdmin[1],
+        tdmin_pp=case.pp_tdmin, **mdl)
 
 
 def build(case: Case, precision: int = 0) -> Tuple["abi.Params", "abi.Arrays", "abi.Arrays"]:
@@ -374,6 +409,9 @@ def cold_start(case: Case, p: "abi.Params", mesh: "abi.Arrays") -> "abi.Arrays":
     kv = np.zeros((L + 1, E))
     kv[1:, :] = case.kv0
     s["Kv"] = kv
+    # two-equation closure state: k at its floor, length scale at its minimum
+    s["q2"] = np.full((L + 1, J), case.q2min)
+    s["xl"] = np.full((L + 1, J), case.xlmin)
     # tracers (allocate_variables.f90:...: salt 0, temp 20; read_salt_init.f90:25-48)
     salt_node = np.zeros((L, P))
     temp_node = np.full((L, P), 20.0)

@@ -12,6 +12,9 @@
  *   gomgpu_stage(ctx, GOMGPU_STAGE_VELOCITY)  solve_velocities            main.f90:207
  *   gomgpu_stage(ctx, GOMGPU_STAGE_LAYERS)    update_vertical_layers_1    main.f90:210
  *   gomgpu_stage(ctx, GOMGPU_STAGE_NODEVEL)   calculate_velocity_at_node  main.f90:213
+ *   gomgpu_stage(ctx, GOMGPU_STAGE_TURBULENCE) subroutine turbulence      turbulence_closure_v1.f90:9-492
+ *                                             (not called by main.f90; runs between NODEVEL and
+ *                                             TRANSPORT when turbulence_flag != 0)
  *   gomgpu_stage(ctx, GOMGPU_STAGE_TRANSPORT) solve_transport_equation_v19 main.f90:231
  *   gomgpu_stage(ctx, GOMGPU_STAGE_EOS)       calculate_density_full/_linear main.f90:261,264
  *   gomgpu_stage(ctx, GOMGPU_STAGE_UPDATE)    update_variables            main.f90:272
@@ -73,7 +76,7 @@ extern "C" {
   X(un_ELM, LJ) X(vn_ELM, LJ) X(dz_face, LJ) X(dz_face_new, LJ) \
   X(AinvG1, LJ) X(AinvG2, LJ) X(AinvDeltaZ1, LJ) X(AinvDeltaZ2, LJ) X(Kh, LJ) \
   X(salt_face, LJ) X(temp_face, LJ) X(rho_face, LJ) \
-  X(dzhalf_face, L1J) X(dzhalf_face_new, L1J) X(Av, L1J) \
+  X(dzhalf_face, L1J) X(dzhalf_face_new, L1J) X(Av, L1J) X(q2, L1J) X(xl, L1J) \
   X(u_face_level, L1J) X(v_face_level, L1J) \
   X(dz_cell, LE) X(dz_cell_new, LE) X(salt_cell, LE) X(salt_cell_new, LE) \
   X(temp_cell, LE) X(temp_cell_new, LE) X(rho_cell, LE) \
@@ -105,12 +108,16 @@ extern "C" {
   X(advection_flag) X(ELM_backtrace_flag) X(ELM_min_iter) X(ELM_max_iter) \
   X(bf_flag) X(baroclinic_flag) X(ana_density) X(i_density_flag) \
   X(transport_flag) X(trans_sub_iter) X(h_flux_limiter) X(v_flux_limiter) \
-  X(heat_option) X(max_iteration_pcg) X(precision) X(river_on_device)
+  X(heat_option) X(max_iteration_pcg) X(precision) X(river_on_device) \
+  X(turbulence_flag) X(turbulence_model) X(stability_function)
 
 #define GOMGPU_PARAMS_REAL(X) \
   X(dt) X(theta) X(gravity) X(rho_o) X(MSL) X(dry_depth) X(adv_onoff_depth) \
   X(smagorinsky_parameter) X(Ah_0) X(Kh_0) X(von_Karman) X(error_tolerance) \
-  X(denominator_min_for_matrix) X(delta_z_min)
+  X(denominator_min_for_matrix) X(delta_z_min) \
+  X(q2min) X(psimin) X(eps_min) X(bgdiff) X(diffmax) X(xlmin1) X(xlmin2) \
+  X(cmiu0) X(cpsi1) X(cpsi2) X(rpub) X(rmub) X(rnub) X(schk) X(schpsi) \
+  X(h1_pp) X(h2_pp) X(vdmax_pp1) X(vdmax_pp2) X(vdmin_pp1) X(vdmin_pp2) X(tdmin_pp)
 
 #define GOMGPU_FORCING_SCALARS(X) \
   X(elapsed_time) X(spinup_function_tide) X(spinup_function_baroclinic)
@@ -144,6 +151,16 @@ typedef struct gomgpu_forcing {
   GOMGPU_FORCING_REAL(GOMGPU_DECL_CRPTR)
 } gomgpu_forcing;
 
+/* Vertical turbulence closure (gomgpu_params.turbulence_flag; turbulence_closure_v1.f90):
+ * 0 constant Av/Kv, 1 Richardson step function, 2 Pacanowski-Philander, 3 two-equation
+ * GLS / Mellor-Yamada-Galperin. turbulence_model selects the second equation of scheme 3
+ * (i_turbulence_model_name 'my','kl','ke','kw','ub'); stability_function the quasi-
+ * equilibrium stability functions (Galperin et al. 1988 or Kantha & Clayson 1994). q2 is the
+ * turbulent kinetic energy k and xl the length scale, at whole levels 0..maxlayer of faces. */
+enum { GOMGPU_TURB_OFF = 0, GOMGPU_TURB_STEP = 1, GOMGPU_TURB_PP = 2, GOMGPU_TURB_GLS = 3 };
+enum { GOMGPU_GLS_MY = 0, GOMGPU_GLS_KL = 1, GOMGPU_GLS_KE = 2, GOMGPU_GLS_KW = 3, GOMGPU_GLS_UB = 4 };
+enum { GOMGPU_STAB_GA = 0, GOMGPU_STAB_KC = 1 };
+
 /* Precision of the bulk arithmetic (gomgpu_params.precision). */
 enum { GOMGPU_FP64 = 0, GOMGPU_MIXED = 1 };
 
@@ -158,7 +175,8 @@ enum gomgpu_stage_id {
   GOMGPU_STAGE_TRANSPORT = 7,
   GOMGPU_STAGE_EOS = 8,
   GOMGPU_STAGE_UPDATE = 9,
-  GOMGPU_NUM_STAGES = 10
+  GOMGPU_STAGE_TURBULENCE = 10,
+  GOMGPU_NUM_STAGES = 11
 };
 
 enum gomgpu_status {
@@ -192,7 +210,10 @@ enum gomgpu_model_error {
   GOMGPU_ERR_ELM_OUTSIDE_QUAD = 17,  /* ELM_backtrace_v1.f90:164-170 */
   GOMGPU_ERR_BILINEAR = 18,          /* bilinear_interpolation.f90:41-112 */
   GOMGPU_ERR_EOS_RANGE = 19,         /* calculate_density_full.f90:43-53,103-107 */
-  GOMGPU_ERR_EOS_DRY_NODES = 20      /* calculate_density_full.f90:223-225 */
+  GOMGPU_ERR_EOS_DRY_NODES = 20,     /* calculate_density_full.f90:223-225 */
+  GOMGPU_ERR_TURB_XL = 21,           /* turbulence_closure_v1.f90:163-170 */
+  GOMGPU_ERR_TURB_DIST = 22,         /* turbulence_closure_v1.f90:209-214 */
+  GOMGPU_ERR_TURB_NEGATIVE = 23      /* turbulence_closure_v1.f90:384-388 */
 };
 
 typedef struct gomgpu_ctx gomgpu_ctx;

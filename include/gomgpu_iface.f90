@@ -14,6 +14,7 @@ module gomgpu_iface
   integer(c_int), parameter :: GOMGPU_STAGE_TRANSPORT = 7
   integer(c_int), parameter :: GOMGPU_STAGE_EOS = 8
   integer(c_int), parameter :: GOMGPU_STAGE_UPDATE = 9
+  integer(c_int), parameter :: GOMGPU_STAGE_TURBULENCE = 10
   integer(c_int), parameter :: GOMGPU_FP64 = 0
   integer(c_int), parameter :: GOMGPU_E_ARG = 1
   integer(c_int), parameter :: GOMGPU_MIXED = 1
@@ -35,7 +36,7 @@ module gomgpu_iface
   type, bind(C) :: gomgpu_state
     type(c_ptr) :: eta_cell, eta_cell_new, eta_node, Gamma_B, Cdb, un_face, vn_face, un_face_new, &
       & vn_face_new, un_ELM, vn_ELM, dz_face, dz_face_new, AinvG1, AinvG2, AinvDeltaZ1, &
-      & AinvDeltaZ2, Kh, salt_face, temp_face, rho_face, dzhalf_face, dzhalf_face_new, Av, &
+      & AinvDeltaZ2, Kh, salt_face, temp_face, rho_face, dzhalf_face, dzhalf_face_new, Av, q2, xl, &
       & u_face_level, v_face_level, dz_cell, dz_cell_new, salt_cell, salt_cell_new, temp_cell, &
       & temp_cell_new, rho_cell, dzhalf_cell, dzhalf_cell_new, wn_cell, wn_cell_new, Kv, dz_node, &
       & dz_node_new, salt_node, temp_node, rho_node, dzhalf_node, dzhalf_node_new, u_node, v_node, &
@@ -47,10 +48,13 @@ module gomgpu_iface
       & num_Qb_cell, maxtran2, tran_id1, tran_id2, advection_flag, ELM_backtrace_flag, &
       & ELM_min_iter, ELM_max_iter, bf_flag, baroclinic_flag, ana_density, i_density_flag, &
       & transport_flag, trans_sub_iter, h_flux_limiter, v_flux_limiter, heat_option, &
-      & max_iteration_pcg, precision, river_on_device = 0
+      & max_iteration_pcg, precision, river_on_device, turbulence_flag, turbulence_model, &
+      & stability_function = 0
     real(c_double) :: dt, theta, gravity, rho_o, MSL, dry_depth, adv_onoff_depth, &
       & smagorinsky_parameter, Ah_0, Kh_0, von_Karman, error_tolerance, &
-      & denominator_min_for_matrix, delta_z_min = 0.0d0
+      & denominator_min_for_matrix, delta_z_min, q2min, psimin, eps_min, bgdiff, diffmax, xlmin1, &
+      & xlmin2, cmiu0, cpsi1, cpsi2, rpub, rmub, rnub, schk, schpsi, h1_pp, h2_pp, vdmax_pp1, &
+      & vdmax_pp2, vdmin_pp1, vdmin_pp2, tdmin_pp = 0.0d0
   end type gomgpu_params
   type, bind(C) :: gomgpu_forcing
     real(c_double) :: elapsed_time, spinup_function_tide, spinup_function_baroclinic = 0.0d0

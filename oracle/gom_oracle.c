@@ -1726,6 +1726,403 @@ int oracle_eos(const gomgpu_mesh *m, const gomgpu_params *p, gomgpu_state *s,
 }
 
 /* ------------------------------------------------------------------- update -------- */
+/* ------------------------------------------------------------------ turbulence ------ */
+/* Vertical turbulence closure, `subroutine turbulence` of turbulence_closure_v1.f90:9-492.
+ * The reference routine is not called by main.f90 and calls an undefined
+ * algebraic_stress; here it is defined as the quasi-equilibrium stability functions of
+ * Galperin et al. (1988) (stability_function GA) or Kantha & Clayson (1994) (KC), in the
+ * GLS form where q2 is the turbulent kinetic energy k (q = sqrt(2 k) in MY notation):
+ *   Gh = (g/rho_o) (drho/dz) xl^2 / (2 k), clipped to [-0.28, 0.0233];
+ *   Km = Sm q xl, Kh = Sh q xl; k and psi diffusivities 0.2 q xl (MY, k-kl) or Km/schk,
+ *   Km/schpsi (k-eps, k-omega, generic).
+ * Quantities the reference reads from arrays it never sizes (diffmax(j), xlmin1(j),
+ * xlmin2(j)) are scalar parameters; the diffusivities of level bottom-1 used by the
+ * bottom layer are those of level bottom. Parity unpinned by any reference run (dead
+ * code); pinned by closed forms in tests/test_turbulence.py. */
+#define TK_A1 0.92
+#define TK_A2 0.74
+#define TK_B1 16.6
+#define TK_B2 10.1
+#define TK_C1 0.08
+#define TK_C2 0.7
+#define TK_C3 0.2
+
+/* x**e for the GLS exponents: the ones the models use (0, +-1, 2, 3, +-0.5, 1.5) by exact
+ * arithmetic, so host and device agree bit for bit; others through pow. */
+static double gls_pow(double x, double e) {
+  if (e == 0.0) return 1.0;
+  if (e == 1.0) return x;
+  if (e == -1.0) return 1.0 / x;
+  if (e == 2.0) return x * x;
+  if (e == 3.0) return x * x * x;
+  if (e == 0.5) return sqrt(x);
+  if (e == -0.5) return 1.0 / sqrt(x);
+  if (e == 1.5) return x * sqrt(x);
+  if (e == -1.5) return 1.0 / (x * sqrt(x));
+  if (e == -3.0) return 1.0 / (x * x * x);
+  return pow(x, e);
+}
+
+/* Sm, Sh of the quasi-equilibrium closure at Gh (clipped). */
+void oracle_stability(int kind, double gh, double *sm, double *sh) {
+  if (gh > 0.0233) gh = 0.0233;
+  if (gh < -0.28) gh = -0.28;
+  double b2 = kind == GOMGPU_STAB_KC ? TK_B2 * (1.0 - TK_C3) : TK_B2;
+  double a2 = kind == GOMGPU_STAB_KC ? TK_A2 * (1.0 - TK_C2) : TK_A2;
+  double h = TK_A2 * (1.0 - 6.0 * TK_A1 / TK_B1) / (1.0 - 3.0 * TK_A2 * gh * (6.0 * TK_A1 + b2));
+  double m = (TK_A1 * (1.0 - 3.0 * TK_C1 - 6.0 * TK_A1 / TK_B1) + 9.0 * TK_A1 * (2.0 * TK_A1 + a2) * h * gh) /
+             (1.0 - 9.0 * TK_A1 * TK_A2 * gh);
+  *sm = m;
+  *sh = h;
+}
+
+/* algebraic_stress: vd, td, qd, qd2 at one whole level, clipped to [bgdiff, diffmax]. */
+static void alg_stress(const gomgpu_params *p, double q2, double xl, double rz, double *vd, double *td,
+                       double *qd, double *qd2) {
+  double gh = rz * (xl * xl) / (2.0 * q2);
+  double sm, sh;
+  oracle_stability(p->stability_function, gh, &sm, &sh);
+  double q = sqrt(2.0 * q2);
+  double v = sm * q * xl, t = sh * q * xl, a, b;
+  if (p->turbulence_model == GOMGPU_GLS_MY || p->turbulence_model == GOMGPU_GLS_KL) {
+    a = 0.2 * q * xl;
+    b = a;
+  } else {
+    a = v / p->schk;
+    b = v / p->schpsi;
+  }
+  *vd = dmin(p->diffmax, dmax(p->bgdiff, v));
+  *td = dmin(p->diffmax, dmax(p->bgdiff, t));
+  *qd = dmin(p->diffmax, dmax(p->bgdiff, a));
+  *qd2 = dmin(p->diffmax, dmax(p->bgdiff, b));
+}
+
+/* (g/rho_o) drho/dz at whole level k of face j from the face densities (0 at the top). */
+static double rz_level(const gomgpu_params *p, const gomgpu_state *s, int L, int j, int k, int kt) {
+  if (k >= kt) return 0.0;
+  double drhodz = (RL(s->rho_face, k + 1, j) - RL(s->rho_face, k, j)) / R0(s->dzhalf_face_new, k, j);
+  return p->gravity / p->rho_o * drhodz;
+}
+
+/* Face diffusivity -> cell Kv: mean over the cell's wet faces of the face value at the
+ * level clamped to the face's active range; a cell with no wet face gets 1.0e-2 (the
+ * fallback of the node mean at turbulence_closure_v1.f90:456-460). */
+static void kv_cells(const gomgpu_mesh *m, const gomgpu_params *p, gomgpu_state *s, const double *kvf) {
+  const int E = p->maxele, L = p->maxlayer;
+  for (int i = 1; i <= E; i++) {
+    if (R1(s->top_layer_at_element, i) == 0) continue;
+    int N = R1(m->tri_or_quad, i);
+    for (int k = 0; k <= L; k++) {
+      double sum = 0.0;
+      int cnt = 0;
+      for (int l = 1; l <= N; l++) {
+        int f = I4(m->facenum_at_cell, l, i);
+        int kt = R1(s->top_layer_at_face, f);
+        if (kt == 0) continue;
+        int kb = R1(m->bottom_layer_at_face, f);
+        int kk = k < kb ? kb : (k > kt ? kt : k);
+        sum = sum + kvf[kk + (size_t)(f - 1) * (L + 1)];
+        cnt++;
+      }
+      R0(s->Kv, k, i) = cnt == 0 ? (double)1.0e-2f : sum / cnt;
+    }
+  }
+}
+
+int oracle_turbulence(const gomgpu_mesh *m, const gomgpu_params *p, gomgpu_state *s,
+                      const gomgpu_forcing *f) {
+  const int J = p->maxface, L = p->maxlayer;
+  const int flag = p->turbulence_flag;
+  if (flag == 0) return 0;
+  if (flag < 0 || flag > 3 || L + 1 > 65) return GOMGPU_E_ARG;
+  double *kvf = calloc((size_t)(L + 1) * J, sizeof(double));
+  if (!kvf) return GOMGPU_E_NOMEM;
+  const double dt = p->dt, g = p->gravity;
+  const double fs_coef = (double)powf(16.6f, 2.0f / 3.0f);  /* 16.6**(2.0/3), default real */
+  int err = 0;
+  for (int j = 1; j <= J && !err; j++) {
+    int kt = R1(s->top_layer_at_face, j);
+    if (kt == 0) continue;
+    int kb = R1(m->bottom_layer_at_face, j);
+    double *KV = &kvf[(size_t)(j - 1) * (L + 1)];
+    if (flag == 1 || flag == 2) {
+      /* Richardson number at whole levels (:75-89) */
+      double vmax = 0.0, vmin = 0.0;
+      if (flag == 2) { /* :120-134 */
+        double h = R1(m->h_face, j);
+        if (h <= p->h1_pp) {
+          vmax = p->vdmax_pp1;
+          vmin = p->vdmin_pp1;
+        } else if (h < p->h2_pp) {
+          vmax = p->vdmax_pp1 + (p->vdmax_pp2 - p->vdmax_pp1) * (h - p->h1_pp) / (p->h2_pp - p->h1_pp);
+          vmin = p->vdmin_pp1 + (p->vdmin_pp2 - p->vdmin_pp1) * (h - p->h1_pp) / (p->h2_pp - p->h1_pp);
+        } else {
+          vmax = p->vdmax_pp2;
+          vmin = p->vdmin_pp2;
+        }
+      }
+      for (int k = kb; k <= kt; k++) {
+        double ri;
+        if (k == kt) {
+          ri = 0.0;
+        } else {
+          double dzh = R0(s->dzhalf_face_new, k, j);
+          double drhodz = (RL(s->rho_face, k + 1, j) - RL(s->rho_face, k, j)) / dzh;
+          double bvf = -g * (drhodz / p->rho_o + g / 2250000.0);
+          double dundz = (RL(s->un_face_new, k + 1, j) - RL(s->un_face_new, k, j)) / dzh;
+          double dutdz = (RL(s->vn_face_new, k + 1, j) - RL(s->vn_face_new, k, j)) / dzh;
+          double shear2 = dundz * dundz + dutdz * dutdz;
+          shear2 = dmax(shear2, 1.0e-10);
+          ri = dmax(bvf / shear2, 0.0);
+        }
+        double av, kv;
+        if (flag == 1) { /* :94-107 */
+          if (ri < 0.25) {
+            av = 1.0e-3;
+            kv = 1.0 * 1.0e-3;
+          } else {
+            av = 1.0e-5;
+            kv = 1.0e-5;
+          }
+        } else { /* :136-141 */
+          double t = 1.0 + 5.0 * ri;
+          av = vmax / (t * t) + vmin;
+          kv = av / (1.0 + 5.0 * ri) + p->tdmin_pp;
+        }
+        R0(s->Av, k, j) = av;
+        KV[k] = kv;
+      }
+      continue;
+    }
+    /* scheme 3 (:143-429) */
+    double avc[66] = {0}, kvc[66] = {0}, qdc[66], qd2c[66];
+    double q2bt[66], xlbt[66], rzbt[66], shearbt[66], q2tmp[66], xltmp[66];
+    double a[66], b[66], c[66], rhs[66], sol[66];
+    for (int k = kb; k <= kt; k++) /* initial diffusivities (:56-66) */
+      alg_stress(p, R0(s->q2, k, j), R0(s->xl, k, j), rz_level(p, s, L, j, k, kt), &avc[k], &kvc[k], &qdc[k], &qd2c[k]);
+    avc[kb - 1] = avc[kb];
+    kvc[kb - 1] = kvc[kb];
+    int n1 = I2(m->nodenum_at_face, 1, j), n2 = I2(m->nodenum_at_face, 2, j);
+    for (int k = kb; k <= kt; k++) { /* :158-187 */
+      q2bt[k] = (R0(s->q2, k - 1, j) + R0(s->q2, k, j)) / 2.0;
+      xlbt[k] = (R0(s->xl, k - 1, j) + R0(s->xl, k, j)) / 2.0;
+      if (xlbt[k] < p->xlmin2) {
+        err = fail(GOMGPU_ERR_TURB_XL, j);
+        break;
+      }
+      double udown = (R0(s->u_node, k - 1, n1) + R0(s->u_node, k - 1, n2)) / 2.0;
+      double uup = (R0(s->u_node, k, n1) + R0(s->u_node, k, n2)) / 2.0;
+      double vdown = (R0(s->v_node, k - 1, n1) + R0(s->v_node, k - 1, n2)) / 2.0;
+      double vup = (R0(s->v_node, k, n1) + R0(s->v_node, k, n2)) / 2.0;
+      double dzk = RL(s->dz_face_new, k, j);
+      double dudz = (uup - udown) / dzk, dvdz = (vup - vdown) / dzk;
+      shearbt[k] = dudz * dudz + dvdz * dvdz;
+      double drhodz;
+      if (k == kt || k == kb) {
+        drhodz = 0.0;
+      } else {
+        double rho_up = RL(s->rho_face, k, j) + dzk / 2.0 / R0(s->dzhalf_face_new, k, j) *
+                                                    (RL(s->rho_face, k + 1, j) - RL(s->rho_face, k, j));
+        double rho_down = RL(s->rho_face, k - 1, j) + RL(s->dz_face_new, k - 1, j) / 2.0 / R0(s->dzhalf_face_new, k - 1, j) *
+                                                          (RL(s->rho_face, k, j) - RL(s->rho_face, k - 1, j));
+        drhodz = (rho_up - rho_down) / dzk;
+      }
+      rzbt[k] = g / p->rho_o * drhodz;
+    }
+    if (err) break;
+    int a1 = I2(m->adj_cellnum_at_face, 1, j), a2 = I2(m->adj_cellnum_at_face, 2, j);
+    double etam = a2 != 0 ? dmax(R1(s->eta_cell_new, a1), R1(s->eta_cell_new, a2)) : R1(s->eta_cell_new, a1);
+    double hf = R1(m->h_face, j);
+    for (int k = kb; k <= kt; k++) { /* :198-215 */
+      double dzk = RL(s->dz_face_new, k, j);
+      double zctr = (k == kt) ? p->MSL + etam - dzk / 2.0 : ZL(k) - dzk / 2.0;
+      double dists = p->MSL + etam - zctr, distb = zctr - (p->MSL - hf);
+      double xlmax = (double)0.4f * dists * distb / (hf + etam);
+      if (xlmax <= 0.0) {
+        err = fail(GOMGPU_ERR_TURB_DIST, j);
+        break;
+      }
+    }
+    if (err) break;
+    /* boundary values (:219-227) */
+    double wn = f->wind_stress_normal[j - 1], wt = f->wind_stress_tangnt[j - 1];
+    double q2fs = fs_coef * sqrt(wn * wn + wt * wt) / 2.0;
+    q2fs = dmax(q2fs, p->q2min);
+    double ubn = RL(s->un_face_new, kb, j), ubt = RL(s->vn_face_new, kb, j);
+    double q2bot = fs_coef * R1(s->Cdb, j) * sqrt(ubn * ubn + ubt * ubt) / 2.0;
+    q2bot = dmax(q2bot, p->q2min);
+    double xlfs = dmax(p->xlmin1, RL(s->dz_face_new, kt, j) / 2.0 * (double)0.4f);
+    double xlbot = dmax(p->xlmin2, RL(s->dz_face_new, kb, j) / 2.0 * (double)0.4f);
+    int nq = kt - kb + 1;
+    double cm3 = p->cmiu0 * p->cmiu0 * p->cmiu0;
+    for (int k = 1; k <= nq; k++) { /* q2 matrix (:234-264) */
+      int kl = kt - k + 1;
+      double dzk = RL(s->dz_face_new, kl, j);
+      double eleft = 0.0, eright = 0.0;
+      if (k > 1) {
+        eleft = -(qdc[kl] * dt / R0(s->dzhalf_face_new, kl, j));
+        a[k - 1] = eleft;
+      }
+      if (k < nq) {
+        eright = -(qdc[kl - 1] * dt / R0(s->dzhalf_face_new, kl - 1, j));
+        c[k - 1] = eright;
+      }
+      double prod = (avc[kl] + avc[kl - 1]) / 2.0 * shearbt[kl];
+      double buoy = (kvc[kl] + kvc[kl - 1]) / 2.0 * rzbt[kl];
+      double diss = cm3 * sqrt(q2bt[kl]) / xlbt[kl];
+      double pplus, pminus;
+      if (prod + buoy > 0.0) {
+        pplus = prod + buoy;
+        pminus = 0.0;
+      } else {
+        pplus = 0.0;
+        pminus = -prod - buoy;
+      }
+      b[k - 1] = dzk - eleft - eright + (diss + pminus / q2bt[kl]) * dt * dzk;
+      rhs[k - 1] = dzk * (q2bt[kl] + dt * pplus);
+    }
+    if (nq >= 1) {
+      a[0] = 0.0;
+      c[nq - 1] = 0.0;
+    }
+    err = oracle_tridiag(nq, 1, 66, a, b, c, rhs, sol);
+    if (err) {
+      err = fail(err, j);
+      break;
+    }
+    for (int k = 1; k <= nq; k++) { /* :268-278 */
+      int kl = kt - k + 1;
+      q2tmp[kl] = kl == kt ? q2fs : (kl == kb ? q2bot : dmax(sol[k - 1], p->q2min));
+    }
+    for (int k = 1; k <= nq; k++) { /* psi matrix (:281-347) */
+      int kl = kt - k + 1;
+      double dzk = RL(s->dz_face_new, kl, j);
+      double eleft = 0.0, eright = 0.0;
+      if (k > 1) {
+        eleft = -(qd2c[kl] * dt / R0(s->dzhalf_face_new, kl, j));
+        a[k - 1] = eleft;
+      }
+      if (k < nq) {
+        eright = -(qd2c[kl - 1] * dt / R0(s->dzhalf_face_new, kl - 1, j));
+        c[k - 1] = eright;
+      }
+      double bb = dzk - eleft - eright;
+      double cpsi2p;
+      if (p->turbulence_model == GOMGPU_GLS_MY || p->turbulence_model == GOMGPU_GLS_KL) {
+        double zctr = ZL(kl) - dzk / 2.0;
+        double dists = dmax(p->MSL + etam - zctr, p->xlmin2);
+        double distb = dmax(zctr - (p->MSL - hf), p->xlmin2);
+        double r1 = xlbt[kl] / (double)0.4f / distb, r2 = xlbt[kl] / (double)0.4f / dists;
+        double psi = 1.0 + (double)1.33f * (r1 * r1) + (double)0.25f * (r2 * r2);
+        cpsi2p = psi * p->cpsi2;
+      } else {
+        cpsi2p = p->cpsi2;
+      }
+      double diss = cpsi2p * cm3 * sqrt(q2bt[kl]) / xlbt[kl];
+      bb = bb + dt * dzk * diss;
+      double prod = p->cpsi1 * (avc[kl] + avc[kl - 1]) / 2.0 * shearbt[kl];
+      double cpsi3;
+      if (p->turbulence_model == GOMGPU_GLS_MY) {
+        cpsi3 = (double)0.9f;
+      } else if (rzbt[kl] > 0.0) {
+        cpsi3 = 1.0;
+      } else {
+        switch (p->turbulence_model) {
+          case GOMGPU_GLS_KL: cpsi3 = (double)2.53f; break;
+          case GOMGPU_GLS_KE: cpsi3 = (double)(-0.52f); break;
+          case GOMGPU_GLS_KW: cpsi3 = (double)(-0.58f); break;
+          default: cpsi3 = (double)0.1f; break;
+        }
+      }
+      double buoy = cpsi3 * (kvc[kl] + kvc[kl - 1]) / 2.0 * rzbt[kl];
+      double pplus, pminus;
+      if (prod + buoy > 0.0) {
+        pplus = prod + buoy;
+        pminus = 0.0;
+      } else {
+        pplus = 0.0;
+        pminus = -prod - buoy;
+      }
+      bb = bb + pminus / q2bt[kl] * dt * dzk;
+      if (kl == kb || kl == kt) bb = bb + dt * (double)0.4f * p->rnub * qd2c[kl] / xlbt[kl];
+      b[k - 1] = bb;
+      double tmp = gls_pow(p->cmiu0, p->rpub) * gls_pow(q2bt[kl], p->rmub) * gls_pow(xlbt[kl], p->rnub);
+      rhs[k - 1] = dzk * tmp * (1.0 + dt * pplus / q2bt[kl]);
+    }
+    if (nq >= 1) {
+      a[0] = 0.0;
+      c[nq - 1] = 0.0;
+    }
+    err = oracle_tridiag(nq, 1, 66, a, b, c, rhs, sol);
+    if (err) {
+      err = fail(err, j);
+      break;
+    }
+    for (int k = 1; k <= nq; k++) { /* :351-368 */
+      int kl = kt - k + 1;
+      double q2l = dmax(sol[k - 1], p->psimin);
+      double x;
+      if (kl == kt)
+        x = xlfs;
+      else if (kl == kb)
+        x = xlbot;
+      else
+        x = gls_pow(q2l * gls_pow(p->cmiu0, -p->rpub) * gls_pow(q2tmp[kl], -p->rmub), 1.0 / p->rnub);
+      if (rzbt[kl] < 0.0) {
+        double upper = sqrt(-(double)0.56f * q2tmp[kl] / rzbt[kl]);
+        x = dmin(x, upper);
+      }
+      double cq = p->cmiu0 * sqrt(q2tmp[kl]);
+      double xl_max = cq * cq * cq / p->eps_min;
+      xltmp[kl] = dmax(p->xlmin2, dmin(xl_max, x));
+    }
+    for (int k = kb - 1; k <= kt; k++) { /* whole levels (:372-390) */
+      double q, x;
+      if (k == kb - 1) {
+        q = q2bot;
+        x = xlbot;
+      } else if (k == kt) {
+        q = q2fs;
+        x = xlfs;
+      } else {
+        double r = RL(s->dz_face_new, k, j) / 2.0 / R0(s->dzhalf_face_new, k, j);
+        q = q2tmp[k] + r * (q2tmp[k + 1] - q2tmp[k]);
+        x = xltmp[k] + r * (xltmp[k + 1] - xltmp[k]);
+        if (q < 0.0 || x < p->xlmin2) {
+          err = fail(GOMGPU_ERR_TURB_NEGATIVE, j);
+          break;
+        }
+      }
+      R0(s->q2, k, j) = q;
+      R0(s->xl, k, j) = x;
+    }
+    if (err) break;
+    for (int k = kb; k <= kt; k++) { /* new diffusivities (:397-405) */
+      double vd, td, qd, qd2;
+      alg_stress(p, R0(s->q2, k, j), R0(s->xl, k, j), rz_level(p, s, L, j, k, kt), &vd, &td, &qd, &qd2);
+      R0(s->Av, k, j) = vd;
+      KV[k] = td;
+    }
+    for (int k = 0; k <= kb - 2; k++) { /* extension (:408-427) */
+      R0(s->q2, k, j) = R0(s->q2, kb - 1, j);
+      R0(s->xl, k, j) = R0(s->xl, kb - 1, j);
+    }
+    for (int k = 1; k <= kb - 1; k++) {
+      R0(s->Av, k, j) = R0(s->Av, kb, j);
+      KV[k] = KV[kb];
+    }
+    for (int k = kt + 1; k <= L; k++) {
+      R0(s->q2, k, j) = R0(s->q2, kt, j);
+      R0(s->xl, k, j) = R0(s->xl, kt, j);
+      R0(s->Av, k, j) = R0(s->Av, kt, j);
+      KV[k] = KV[kt];
+    }
+  }
+  if (!err) kv_cells(m, p, s, kvf);
+  free(kvf);
+  return err;
+}
+
 /* update_variables.f90:4-77 */
 int oracle_update(const gomgpu_mesh *m, const gomgpu_params *p, gomgpu_state *s,
                   const gomgpu_forcing *f) {
@@ -1778,9 +2175,10 @@ int oracle_step(const gomgpu_mesh *m, const gomgpu_params *p, gomgpu_state *s,
                 const gomgpu_forcing *f) {
   int (*stages[])(const gomgpu_mesh *, const gomgpu_params *, gomgpu_state *, const gomgpu_forcing *) = {
       oracle_friction, oracle_elm, oracle_momentum, oracle_freesurf, oracle_velocity,
-      oracle_layers, oracle_nodevel, oracle_transport, oracle_eos, oracle_update};
-  for (int i = 0; i < 10; i++) {
-    if (i == 7 && p->transport_flag != 1) continue;
+      oracle_layers, oracle_nodevel, oracle_turbulence, oracle_transport, oracle_eos, oracle_update};
+  for (int i = 0; i < 11; i++) {
+    if (i == 7 && p->turbulence_flag == 0) continue;
+    if (i == 8 && p->transport_flag != 1) continue;
     int err = stages[i](m, p, s, f);
     if (err) return err;
   }

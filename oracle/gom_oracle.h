@@ -48,7 +48,13 @@ int oracle_eos(const gomgpu_mesh *m, const gomgpu_params *p, gomgpu_state *s,
                const gomgpu_forcing *f);
 int oracle_update(const gomgpu_mesh *m, const gomgpu_params *p, gomgpu_state *s,
                   const gomgpu_forcing *f);
-/* All stages in the order of main.f90:95-272 (friction first). */
+/* Vertical turbulence closure (turbulence_closure_v1.f90:9-492); see gom_oracle.c. */
+int oracle_turbulence(const gomgpu_mesh *m, const gomgpu_params *p, gomgpu_state *s,
+                      const gomgpu_forcing *f);
+/* Quasi-equilibrium stability functions Sm, Sh at Gh (GOMGPU_STAB_GA or _KC). */
+void oracle_stability(int kind, double gh, double *sm, double *sh);
+/* All stages in the order of main.f90:95-272 (friction first), with the turbulence
+ * closure between the node velocities and the transport when turbulence_flag != 0. */
 int oracle_step(const gomgpu_mesh *m, const gomgpu_params *p, gomgpu_state *s,
                 const gomgpu_forcing *f);
 
