@@ -71,6 +71,9 @@ struct DevWork {
   double *face_new;       // (L, J, NT)
   // turbulence closure: face diffusivity (0:L, J) before the face -> cell map
   double *turb_kvf;
+  // elimination factors of the column tridiagonal solves, (L, max(E, J))
+  double *tri_gam;
+  double *tvd_x;          // (L, E, NT) right-hand sides / solution of the TVD column solve
   // budgets
   double *budget;         // [0] volume, [1] salt
   int *err;               // [0] code, [1] entity

@@ -16,7 +16,8 @@ __global__ void k_friction(DevMesh, DevState, DevDims, int, double, double);
 __global__ void k_elm_substeps(DevMesh, DevState, DevDims, double, double, int, int);
 __global__ void k_elm(DevMesh, DevState, DevDims, double, double, double, int *);
 __global__ void k_elm_copy(DevState, long);
-__global__ void __launch_bounds__(256) k_momentum(DevMesh, DevState, DevForcing, DevDims, MomParams, int *);
+__global__ void k_mom_rhs(DevMesh, DevState, DevForcing, DevDims, MomParams);
+__global__ void k_mom_solve(DevMesh, DevState, DevForcing, DevWork, DevDims, MomParams, int *);
 __global__ void k_river_velocity(DevMesh, DevState, DevForcing, DevDims, double *);
 __global__ void k_fs_face(DevMesh, DevState, DevWork, DevDims, double);
 __global__ void k_fs_cell(DevMesh, DevState, DevForcing, DevWork, DevDims, double, double, int *);
@@ -38,7 +39,8 @@ __global__ void k_tvd_coef_cell(DevMesh, DevState, DevWork, DevDims, TvdArgs);
 __global__ void k_tvd_lim_cell(DevMesh, DevState, DevWork, DevDims);
 __global__ void k_tvd_lim_face(DevMesh, DevState, DevWork, DevDims, int);
 __global__ void k_tvd_psi_cell(DevMesh, DevState, DevWork, DevDims, int);
-__global__ void k_tvd_column(DevMesh, DevState, DevWork, DevDims, TvdArgs, int *);
+__global__ void k_tvd_rhs(DevMesh, DevState, DevWork, DevDims, TvdArgs);
+__global__ void k_tvd_solve(DevMesh, DevState, DevWork, DevDims, TvdArgs, int *);
 __global__ void k_tvd_copyback(DevMesh, DevState, DevWork, DevDims);
 __global__ void k_tvd_node(DevMesh, DevState, DevWork, DevDims);
 __global__ void k_tvd_face_out(DevMesh, DevState, DevWork, DevDims);
@@ -335,7 +337,8 @@ extern "C" int gomgpu_create(const gomgpu_mesh *mesh, const gomgpu_params *param
         (rc = dalloc(c, &w.s2a, NT * L * E)) || (rc = dalloc(c, &w.obk, NT * L * OB)) ||
         (rc = dalloc(c, &w.qbk, NT * L * QB)) || (rc = dalloc(c, &w.node_new, NT * L * P)) ||
         (rc = dalloc(c, &w.face_new, NT * L * J)) || (rc = dalloc(c, &w.budget, 2 * ((E + 255) / 256) + 2)) ||
-        (rc = dalloc(c, &w.err, 2)) || (rc = dalloc(c, &w.turb_kvf, (L + 1) * J)))
+        (rc = dalloc(c, &w.err, 2)) || (rc = dalloc(c, &w.turb_kvf, (L + 1) * J)) ||
+        (rc = dalloc(c, &w.tri_gam, L * (E > J ? E : J))) || (rc = dalloc(c, &w.tvd_x, NT * L * E)))
       goto fail;
   }
   if (hipHostMalloc((void **)&c->h_err, 5 * sizeof(int)) != hipSuccess) {
@@ -488,7 +491,8 @@ static int stage_impl(gomgpu_ctx *c, int id) {
       MomParams q{p.dt, p.gravity, p.theta, p.rho_o, p.smagorinsky_parameter, p.Ah_0, p.Kh_0, c->spin_bc,
                   p.baroclinic_flag};
       KT_BEGIN(KT_MOMENTUM);
-      k_momentum<<<nblk(d.J, 128), 128, 0, st>>>(c->m, c->s, c->f, d, q, err);
+      k_mom_rhs<<<nblk((long)d.L * d.J, BS), BS, 0, st>>>(c->m, c->s, c->f, d, q);
+      k_mom_solve<<<nblk(d.J, 128), 128, 0, st>>>(c->m, c->s, c->f, c->w, d, q, err);
       KT_END(KT_MOMENTUM);
       break;
     }
@@ -598,7 +602,8 @@ static int stage_impl(gomgpu_ctx *c, int id) {
           k_tvd_psi_cell<<<nblk((long)d.E * (d.L + 1), BS), BS, 0, st>>>(c->m, c->s, c->w, d, p.h_flux_limiter);
         }
         KT_BEGIN(KT_TVD_COLUMN);
-        k_tvd_column<<<nblk(d.E, 128), 128, 0, st>>>(c->m, c->s, c->w, d, a, err);
+        k_tvd_rhs<<<nblk((long)d.L * d.E, BS), BS, 0, st>>>(c->m, c->s, c->w, d, a);
+        k_tvd_solve<<<nblk(d.E, 128), 128, 0, st>>>(c->m, c->s, c->w, d, a, err);
         KT_END(KT_TVD_COLUMN);
         k_tvd_copyback<<<nblk(nE2, BS), BS, 0, st>>>(c->m, c->s, c->w, d);
       }

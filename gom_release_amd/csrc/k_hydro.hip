@@ -418,66 +418,29 @@ __global__ void k_elm_copy(DevState s, long n) {
 }
 
 // ================================================================ momentum
-// tridiagonal_solver.f90:15-90 on per-thread arrays; rhs[r][c] with r the 0-based row.
-template <int NC>
-__device__ __forceinline__ int thomas(int n, const double *a, const double *b, const double *c,
-                                      double (*x)[NC]) {
-  double gam[GOM_LMAX];
-  if (b[0] == 0.0) return GOMGPU_ERR_TRIDIAG_PIVOT;
-  double den = b[0];
-  for (int i = 0; i < NC; i++) x[0][i] = x[0][i] / den;
-  for (int j = 1; j < n; j++) {
-    gam[j] = c[j - 1] / den;
-    den = b[j] - a[j] * gam[j];
-    if (den == 0.0) return GOMGPU_ERR_TRIDIAG_PIVOT;
-    for (int i = 0; i < NC; i++) x[j][i] = (x[j][i] - a[j] * x[j - 1][i]) / den;
-  }
-  for (int j = n - 2; j >= 0; j--)
-    for (int i = 0; i < NC; i++) x[j][i] = x[j][i] - gam[j + 1] * x[j + 1][i];
-  return 0;
-}
-
-
-// solve_momentum_equation.f90:67-655; one thread per face column.
-__global__ void __launch_bounds__(256) k_momentum(DevMesh m, DevState s, DevForcing f, DevDims d, MomParams q, int *err) {
-  int j = blockIdx.x * blockDim.x + threadIdx.x;
+// solve_momentum_equation.f90:67-655, in two passes. k_mom_rhs: one thread per (row, face)
+// builds the right-hand sides G1, G2 of tridiagonal row r (local index, surface first) and
+// Kh; k_mom_solve: one thread per face runs the Thomas recurrence of
+// tridiagonal_solver.f90 with the three right-hand sides [G1, G2, dz], in place in the
+// output arrays (the elimination factors go to w.tri_gam). Every value is computed by the
+// same expression sequence as the reference's per-face loop.
+__global__ void k_mom_rhs(DevMesh m, DevState s, DevForcing f, DevDims d, MomParams q) {
+  long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
   const int J = d.J, E = d.E, P = d.P, L = d.L;
-  if (j >= J) return;
+  if (t >= (long)L * J) return;
+  int j = (int)(t % J), k = (int)(t / J) + 1;
   int kt = s.top_layer_at_face[j];
-  if (kt == 0) {
-    for (int k = 1; k <= L; k++) {
-      AL(s.AinvG1, k, j, J) = 0.0;
-      AL(s.AinvG2, k, j, J) = 0.0;
-      AL(s.AinvDeltaZ1, k, j, J) = 0.0;
-      AL(s.AinvDeltaZ2, k, j, J) = 0.0;
-    }
-    return;
-  }
-  const double dt = q.dt, g = q.g, th = q.th;
+  if (kt == 0) return;
   int kb = m.bottom_layer_at_face[j];
-  int node1 = TB(m.nodenum_at_face, 1, j, J) - 1, node2 = TB(m.nodenum_at_face, 2, j, J) - 1;
   int n = kt - kb + 1;
+  if (k > n) return;
+  const double dt = q.dt, g = q.g, th = q.th;
+  int node1 = TB(m.nodenum_at_face, 1, j, J) - 1, node2 = TB(m.nodenum_at_face, 2, j, J) - 1;
   int a1 = TB(m.adj_cellnum_at_face, 1, j, J), a2 = TB(m.adj_cellnum_at_face, 2, j, J);
   int bt = m.boundary_type_of_face[j];
   double dj = m.delta_j[j], Lf = m.face_length[j];
   double ct = m.cos_theta[j], st = m.sin_theta[j], ct2 = m.cos_theta2[j], st2 = m.sin_theta2[j];
   double cor = m.coriolis_factor[j];
-  double a[GOM_LMAX], b[GOM_LMAX], cc[GOM_LMAX], x[GOM_LMAX][3];
-  // matrix (rows 0..n-1 = local k 1..n, surface first)
-  for (int k = kb + 1; k <= kt - 1; k++) {
-    int r = kt - k;
-    a[r] = -(dt * A0(s.Av, k, j, J) / A0(s.dzhalf_face, k, j, J));
-    cc[r] = -(dt * A0(s.Av, k - 1, j, J) / A0(s.dzhalf_face, k - 1, j, J));
-    b[r] = -a[r] + AL(s.dz_face, k, j, J) - cc[r];
-  }
-  if (kt == kb) {
-    b[0] = AL(s.dz_face, kt, j, J) + s.Gamma_B[j] * dt;
-  } else {
-    cc[0] = -(dt * A0(s.Av, kt - 1, j, J) / A0(s.dzhalf_face, kt - 1, j, J));
-    b[0] = f.Gamma_T[j] * dt + AL(s.dz_face, kt, j, J) - cc[0];
-    a[n - 1] = -(dt * A0(s.Av, kb, j, J) / A0(s.dzhalf_face, kb, j, J));
-    b[n - 1] = -a[n - 1] + AL(s.dz_face, kb, j, J) + s.Gamma_B[j] * dt;
-  }
   // air pressure differences are layer independent
   double airc = 0.0;
   if (a2 != 0) {
@@ -493,137 +456,221 @@ __global__ void __launch_bounds__(256) k_momentum(DevMesh m, DevState s, DevForc
   double rtemp3a = -(dt / q.rho_o * ((f.air_p1[node2] - f.air_p1[node1]) / Lf));
   bool wet_nodes = (s.wetdry_node[node1] == 0 && s.wetdry_node[node2] == 0);
   double rtemp0 = 0.0, rtemp1, rtemp2 = 0.0, rtemp3, rtemp4;
-  for (int k = 1; k <= n; k++) {
-    int kk = kt - k + 1;
-    double dz = AL(s.dz_face, kk, j, J);
-    double unf = AL(s.un_face, kk, j, J), vnf = AL(s.vn_face, kk, j, J);
-    double G1 = dz * (AL(s.un_ELM, kk, j, J) + cor * dt * vnf);
-    double G2 = dz * (AL(s.vn_ELM, kk, j, J) - cor * dt * unf);
-    if (a2 != 0) {
-      double temp[13];
-      for (int l = 1; l <= 2; l++) {
-        int nd = TB(m.nodenum_at_face, l, j, J) - 1;
-        double ut = (A0(s.u_node, kk, nd, P) + A0(s.u_node, kk - 1, nd, P)) * 0.5;
-        double vt = (A0(s.v_node, kk, nd, P) + A0(s.v_node, kk - 1, nd, P)) * 0.5;
-        temp[2 * l - 1] = ut * ct + vt * st;
-        temp[2 * l] = -ut * st + vt * ct;
-      }
-      for (int l = 1; l <= 2; l++) {
-        int ie = TB(m.adj_cellnum_at_face, l, j, J) - 1;
-        int N = m.tri_or_quad[ie];
-        double ut = 0.0, vt = 0.0;
-        for (int ll = 1; ll <= N; ll++) {
-          int nd = TB(m.nodenum_at_cell, ll, ie, E) - 1;
-          ut = ut + (A0(s.u_node, kk, nd, P) + A0(s.u_node, kk - 1, nd, P)) * 0.5 / N;
-          vt = vt + (A0(s.v_node, kk, nd, P) + A0(s.v_node, kk - 1, nd, P)) * 0.5 / N;
-        }
-        temp[2 * l + 3] = ut * ct + vt * st;
-        temp[2 * l + 4] = -ut * st + vt * ct;
-      }
-      double dudx = (temp[7] - temp[5]) / dj, dvdx = (temp[8] - temp[6]) / dj;
-      double dudy = (temp[3] - temp[1]) / Lf, dvdy = (temp[4] - temp[2]) / Lf;
-      double sx = dvdx + dudy;
-      double hdiff = q.smag * sqrt(dudx * dudx + dvdy * dvdy + 0.5 * (sx * sx));
-      double kh = hdiff * m.area[a1 - 1];
-      hdiff = hdiff + q.Ah0;
-      AL(s.Kh, k, j, J) = kh + q.Kh0;
-      double d2udy = 4.0 * (temp[3] + temp[1] - 2 * unf) / (Lf * Lf);
-      double d2vdy = 4.0 * (temp[4] + temp[2] - 2 * vnf) / (Lf * Lf);
-      for (int l = 1; l <= 2; l++) {
-        int ie = TB(m.adj_cellnum_at_face, l, j, J) - 1;
-        double ddx = m.x_cell[ie] - m.x_face[j], ddy = m.y_cell[ie] - m.y_face[j];
-        double rl = sqrt(ddx * ddx + ddy * ddy);
-        temp[2 * l + 7] = (temp[2 * l + 3] - unf) / rl;
-        temp[2 * l + 8] = (temp[2 * l + 4] - vnf) / rl;
-      }
-      double d2udx = 2.0 * (temp[9] + temp[11]) / dj;
-      double d2vdx = 2.0 * (temp[10] + temp[12]) / dj;
-      G1 = G1 + dz * dt * hdiff * (d2udx + d2udy);
-      G2 = G2 + dz * dt * hdiff * (d2vdx + d2vdy);
+  int kk = kt - k + 1;
+  double dz = AL(s.dz_face, kk, j, J);
+  double unf = AL(s.un_face, kk, j, J), vnf = AL(s.vn_face, kk, j, J);
+  double G1 = dz * (AL(s.un_ELM, kk, j, J) + cor * dt * vnf);
+  double G2 = dz * (AL(s.vn_ELM, kk, j, J) - cor * dt * unf);
+  if (a2 != 0) {
+    double temp[13];
+    for (int l = 1; l <= 2; l++) {
+      int nd = TB(m.nodenum_at_face, l, j, J) - 1;
+      double ut = (A0(s.u_node, kk, nd, P) + A0(s.u_node, kk - 1, nd, P)) * 0.5;
+      double vt = (A0(s.v_node, kk, nd, P) + A0(s.v_node, kk - 1, nd, P)) * 0.5;
+      temp[2 * l - 1] = ut * ct + vt * st;
+      temp[2 * l] = -ut * st + vt * ct;
     }
-    if (bt == 0) {
-      rtemp0 = g * dt / dj * (1.0 - th) * (s.eta_cell[a2 - 1] - s.eta_cell[a1 - 1]);
-      rtemp1 = rtemp0 * ct2;
-      rtemp2 = rtemp0 * st2;
-      G1 = G1 - dz * rtemp1;
-    } else if (bt > 0) {
-      int ie = m.ob_element_flag[a1 - 1];
-      rtemp1 = g * dt / dj * (1.0 - th) * (f.eta_at_ob_old[ie - 1] - s.eta_cell[a1 - 1]);
-      G1 = G1 - dz * rtemp1;
-    } else if (bt == -1) {
-      G1 = 0.0;
-    }
-    if (wet_nodes) {
-      rtemp0 = g * dt / Lf * (1.0 - th) * (s.eta_node[node2] - s.eta_node[node1]);
-      if (bt == 0)
-        G2 = G2 - dz * (rtemp0 + rtemp2);
-      else
-        G2 = G2 - dz * rtemp0;
-    } else if (a2 == 0) {
-      rtemp0 = 0.0;  // defined value for the reference's stale private variable
-    }
-    if (q.baroclinic == 1) {
-      double sum1 = 0.0, sum2 = 0.0;
-      int cb1 = 0, ct1 = -1, cb2 = 0, ct2c = -1;
-      if (a2 != 0) {
-        cb1 = m.bottom_layer_at_element[a1 - 1];
-        ct1 = s.top_layer_at_element[a1 - 1];
-        cb2 = m.bottom_layer_at_element[a2 - 1];
-        ct2c = s.top_layer_at_element[a2 - 1];
+    for (int l = 1; l <= 2; l++) {
+      int ie = TB(m.adj_cellnum_at_face, l, j, J) - 1;
+      int N = m.tri_or_quad[ie];
+      double ut = 0.0, vt = 0.0;
+      for (int ll = 1; ll <= N; ll++) {
+        int nd = TB(m.nodenum_at_cell, ll, ie, E) - 1;
+        ut = ut + (A0(s.u_node, kk, nd, P) + A0(s.u_node, kk - 1, nd, P)) * 0.5 / N;
+        vt = vt + (A0(s.v_node, kk, nd, P) + A0(s.v_node, kk - 1, nd, P)) * 0.5 / N;
       }
-      int nb1 = m.bottom_layer_at_node[node1], nt1 = s.top_layer_at_node[node1];
-      int nb2 = m.bottom_layer_at_node[node2], nt2 = s.top_layer_at_node[node2];
-      for (int k2 = kk; k2 <= kt; k2++) {
-        double fct = (k2 == kk) ? 0.5 : 1.0;
-        double dz2 = AL(s.dz_face, k2, j, J);
-        if (a2 != 0 && k2 >= cb1 && k2 <= ct1 && k2 >= cb2 && k2 <= ct2c)
-          sum1 = sum1 + fct * (AL(s.rho_cell, k2, a2 - 1, E) - AL(s.rho_cell, k2, a1 - 1, E)) * dz2;
-        if (k2 >= nb1 && k2 <= nt1 && k2 >= nb2 && k2 <= nt2)
-          sum2 = sum2 + fct * (AL(s.rho_node, k2, node2, P) - AL(s.rho_node, k2, node1, P)) * dz2;
-      }
-      double dzl = AL(s.dz_face, k, j, J);
-      rtemp1 = dzl * (-(g * dt / (q.rho_o * dj))) * sum1;
-      rtemp2 = dzl * (-(g * dt / (q.rho_o * Lf))) * sum2;
-      rtemp3 = rtemp1 * ct2;
-      rtemp4 = rtemp1 * st2;
-      G1 = G1 + rtemp3 * q.spinup_bc;
-      G2 = G2 + (rtemp2 + rtemp4) * q.spinup_bc;
+      temp[2 * l + 3] = ut * ct + vt * st;
+      temp[2 * l + 4] = -ut * st + vt * ct;
     }
-    if (k == 1) {
-      G1 = G1 + dt * f.wind_stress_normal[j];
-      G2 = G2 + dt * f.wind_stress_tangnt[j];
+    double dudx = (temp[7] - temp[5]) / dj, dvdx = (temp[8] - temp[6]) / dj;
+    double dudy = (temp[3] - temp[1]) / Lf, dvdy = (temp[4] - temp[2]) / Lf;
+    double sx = dvdx + dudy;
+    double hdiff = q.smag * sqrt(dudx * dudx + dvdy * dvdy + 0.5 * (sx * sx));
+    double kh = hdiff * m.area[a1 - 1];
+    hdiff = hdiff + q.Ah0;
+    AL(s.Kh, k, j, J) = kh + q.Kh0;
+    double d2udy = 4.0 * (temp[3] + temp[1] - 2 * unf) / (Lf * Lf);
+    double d2vdy = 4.0 * (temp[4] + temp[2] - 2 * vnf) / (Lf * Lf);
+    for (int l = 1; l <= 2; l++) {
+      int ie = TB(m.adj_cellnum_at_face, l, j, J) - 1;
+      double ddx = m.x_cell[ie] - m.x_face[j], ddy = m.y_cell[ie] - m.y_face[j];
+      double rl = sqrt(ddx * ddx + ddy * ddy);
+      temp[2 * l + 7] = (temp[2 * l + 3] - unf) / rl;
+      temp[2 * l + 8] = (temp[2 * l + 4] - vnf) / rl;
     }
-    G1 = G1 + 0.0;  // equilibrium tide term (zero without tidal potential species)
-    G2 = G2 + 0.0;
-    if (a2 != 0) rtemp0 = airc;
+    double d2udx = 2.0 * (temp[9] + temp[11]) / dj;
+    double d2vdx = 2.0 * (temp[10] + temp[12]) / dj;
+    G1 = G1 + dz * dt * hdiff * (d2udx + d2udy);
+    G2 = G2 + dz * dt * hdiff * (d2vdx + d2vdy);
+  }
+  if (bt == 0) {
+    rtemp0 = g * dt / dj * (1.0 - th) * (s.eta_cell[a2 - 1] - s.eta_cell[a1 - 1]);
     rtemp1 = rtemp0 * ct2;
     rtemp2 = rtemp0 * st2;
-    G1 = G1 + dz * rtemp1;
-    rtemp3 = rtemp3a;
-    G2 = G2 + dz * (rtemp3 + rtemp2);
-    if (bt == -1) G1 = 0.0;
-    int ibnd = m.isflowside3[j];
-    if (ibnd > 0) G1 = G1 + dz * f.Qu_boundary[ibnd - 1];
-    x[k - 1][0] = G1;
-    x[k - 1][1] = G2;
-    x[k - 1][2] = dz;
+    G1 = G1 - dz * rtemp1;
+  } else if (bt > 0) {
+    int ie = m.ob_element_flag[a1 - 1];
+    rtemp1 = g * dt / dj * (1.0 - th) * (f.eta_at_ob_old[ie - 1] - s.eta_cell[a1 - 1]);
+    G1 = G1 - dz * rtemp1;
+  } else if (bt == -1) {
+    G1 = 0.0;
   }
-  int e = thomas<3>(n, a, b, cc, x);
-  if (e) latch(err, e, j + 1);
+  if (wet_nodes) {
+    rtemp0 = g * dt / Lf * (1.0 - th) * (s.eta_node[node2] - s.eta_node[node1]);
+    if (bt == 0)
+      G2 = G2 - dz * (rtemp0 + rtemp2);
+    else
+      G2 = G2 - dz * rtemp0;
+  } else if (a2 == 0) {
+    rtemp0 = 0.0;  // defined value for the reference's stale private variable
+  }
+  if (q.baroclinic == 1) {
+    double sum1 = 0.0, sum2 = 0.0;
+    int cb1 = 0, ct1 = -1, cb2 = 0, ct2c = -1;
+    if (a2 != 0) {
+      cb1 = m.bottom_layer_at_element[a1 - 1];
+      ct1 = s.top_layer_at_element[a1 - 1];
+      cb2 = m.bottom_layer_at_element[a2 - 1];
+      ct2c = s.top_layer_at_element[a2 - 1];
+    }
+    int nb1 = m.bottom_layer_at_node[node1], nt1 = s.top_layer_at_node[node1];
+    int nb2 = m.bottom_layer_at_node[node2], nt2 = s.top_layer_at_node[node2];
+    for (int k2 = kk; k2 <= kt; k2++) {
+      double fct = (k2 == kk) ? 0.5 : 1.0;
+      double dz2 = AL(s.dz_face, k2, j, J);
+      if (a2 != 0 && k2 >= cb1 && k2 <= ct1 && k2 >= cb2 && k2 <= ct2c)
+        sum1 = sum1 + fct * (AL(s.rho_cell, k2, a2 - 1, E) - AL(s.rho_cell, k2, a1 - 1, E)) * dz2;
+      if (k2 >= nb1 && k2 <= nt1 && k2 >= nb2 && k2 <= nt2)
+        sum2 = sum2 + fct * (AL(s.rho_node, k2, node2, P) - AL(s.rho_node, k2, node1, P)) * dz2;
+    }
+    double dzl = AL(s.dz_face, k, j, J);
+    rtemp1 = dzl * (-(g * dt / (q.rho_o * dj))) * sum1;
+    rtemp2 = dzl * (-(g * dt / (q.rho_o * Lf))) * sum2;
+    rtemp3 = rtemp1 * ct2;
+    rtemp4 = rtemp1 * st2;
+    G1 = G1 + rtemp3 * q.spinup_bc;
+    G2 = G2 + (rtemp2 + rtemp4) * q.spinup_bc;
+  }
+  if (k == 1) {
+    G1 = G1 + dt * f.wind_stress_normal[j];
+    G2 = G2 + dt * f.wind_stress_tangnt[j];
+  }
+  G1 = G1 + 0.0;  // equilibrium tide term (zero without tidal potential species)
+  G2 = G2 + 0.0;
+  if (a2 != 0) rtemp0 = airc;
+  rtemp1 = rtemp0 * ct2;
+  rtemp2 = rtemp0 * st2;
+  G1 = G1 + dz * rtemp1;
+  rtemp3 = rtemp3a;
+  G2 = G2 + dz * (rtemp3 + rtemp2);
+  if (bt == -1) G1 = 0.0;
+  int ibnd = m.isflowside3[j];
+  if (ibnd > 0) G1 = G1 + dz * f.Qu_boundary[ibnd - 1];
+  AL(s.AinvG1, k, j, J) = G1;
+  AL(s.AinvG2, k, j, J) = G2;
+}
+
+// Tridiagonal row r (0-based, surface first) of face j: solve_momentum_equation.f90:120-169.
+__device__ __forceinline__ void mom_row(const DevState &s, const double *gamma_t, int j, int J, int kt, int kb,
+                                        int r, double dt, double &a, double &b, double &c) {
+  int n = kt - kb + 1;
+  a = 0.0;
+  c = 0.0;
+  if (kt == kb) {
+    b = AL(s.dz_face, kt, j, J) + s.Gamma_B[j] * dt;
+  } else if (r == 0) {
+    c = -(dt * A0(s.Av, kt - 1, j, J) / A0(s.dzhalf_face, kt - 1, j, J));
+    b = gamma_t[j] * dt + AL(s.dz_face, kt, j, J) - c;
+  } else if (r == n - 1) {
+    a = -(dt * A0(s.Av, kb, j, J) / A0(s.dzhalf_face, kb, j, J));
+    b = -a + AL(s.dz_face, kb, j, J) + s.Gamma_B[j] * dt;
+  } else {
+    int k = kt - r;
+    a = -(dt * A0(s.Av, k, j, J) / A0(s.dzhalf_face, k, j, J));
+    c = -(dt * A0(s.Av, k - 1, j, J) / A0(s.dzhalf_face, k - 1, j, J));
+    b = -a + AL(s.dz_face, k, j, J) - c;
+  }
+}
+
+__global__ void k_mom_solve(DevMesh m, DevState s, DevForcing f, DevWork w, DevDims d, MomParams q, int *err) {
+  int j = blockIdx.x * blockDim.x + threadIdx.x;
+  const int J = d.J, L = d.L;
+  if (j >= J) return;
+  int kt = s.top_layer_at_face[j];
+  if (kt == 0) {
+    for (int k = 1; k <= L; k++) {
+      AL(s.AinvG1, k, j, J) = 0.0;
+      AL(s.AinvG2, k, j, J) = 0.0;
+      AL(s.AinvDeltaZ1, k, j, J) = 0.0;
+      AL(s.AinvDeltaZ2, k, j, J) = 0.0;
+    }
+    return;
+  }
+  const double dt = q.dt;
+  int kb = m.bottom_layer_at_face[j];
+  int n = kt - kb + 1;
+  int bt = m.boundary_type_of_face[j];
+  // forward elimination (rows 0..n-1, RHS 3 is dz of the row's layer)
+  double a, b, c, cprev;
+  mom_row(s, f.Gamma_T, j, J, kt, kb, 0, dt, a, b, c);
+  int e = 0;
+  double den = b;
+  double x0, x1, x2;
+  if (den == 0.0) {
+    e = GOMGPU_ERR_TRIDIAG_PIVOT;
+  } else {
+    x0 = AL(s.AinvG1, 1, j, J) / den;
+    x1 = AL(s.AinvG2, 1, j, J) / den;
+    x2 = AL(s.dz_face, kt, j, J) / den;
+    AL(s.AinvG1, 1, j, J) = x0;
+    AL(s.AinvG2, 1, j, J) = x1;
+    AL(s.AinvDeltaZ2, 1, j, J) = x2;
+    cprev = c;
+    for (int r = 1; r < n; r++) {
+      mom_row(s, f.Gamma_T, j, J, kt, kb, r, dt, a, b, c);
+      double gam = cprev / den;
+      den = b - a * gam;
+      if (den == 0.0) {
+        e = GOMGPU_ERR_TRIDIAG_PIVOT;
+        break;
+      }
+      x0 = (AL(s.AinvG1, r + 1, j, J) - a * x0) / den;
+      x1 = (AL(s.AinvG2, r + 1, j, J) - a * x1) / den;
+      x2 = (AL(s.dz_face, kt - r, j, J) - a * x2) / den;
+      AL(s.AinvG1, r + 1, j, J) = x0;
+      AL(s.AinvG2, r + 1, j, J) = x1;
+      AL(s.AinvDeltaZ2, r + 1, j, J) = x2;
+      AL(w.tri_gam, r + 1, j, J) = gam;
+      cprev = c;
+    }
+  }
+  if (e) {
+    latch(err, e, j + 1);
+    return;
+  }
+  // back substitution
+  for (int r = n - 2; r >= 0; r--) {
+    double gam = AL(w.tri_gam, r + 2, j, J);
+    x0 = AL(s.AinvG1, r + 1, j, J) - gam * x0;
+    x1 = AL(s.AinvG2, r + 1, j, J) - gam * x1;
+    x2 = AL(s.AinvDeltaZ2, r + 1, j, J) - gam * x2;
+    AL(s.AinvG1, r + 1, j, J) = x0;
+    AL(s.AinvG2, r + 1, j, J) = x1;
+    AL(s.AinvDeltaZ2, r + 1, j, J) = x2;
+  }
   bool land = (bt == -1);
   bool zero_g1 = land && m.isflowside3[j] == 0;
   for (int k = 1; k <= L; k++) {
-    double g1 = 0.0, g2 = 0.0, z1 = 0.0, z2 = 0.0;
     if (k <= n) {
-      g1 = zero_g1 ? 0.0 : x[k - 1][0];
-      g2 = x[k - 1][1];
-      z1 = land ? 0.0 : x[k - 1][2];
-      z2 = x[k - 1][2];
+      if (zero_g1) AL(s.AinvG1, k, j, J) = 0.0;
+      double z = AL(s.AinvDeltaZ2, k, j, J);
+      AL(s.AinvDeltaZ1, k, j, J) = land ? 0.0 : z;
+    } else {
+      AL(s.AinvG1, k, j, J) = 0.0;
+      AL(s.AinvG2, k, j, J) = 0.0;
+      AL(s.AinvDeltaZ1, k, j, J) = 0.0;
+      AL(s.AinvDeltaZ2, k, j, J) = 0.0;
     }
-    AL(s.AinvG1, k, j, J) = g1;
-    AL(s.AinvG2, k, j, J) = g2;
-    AL(s.AinvDeltaZ1, k, j, J) = z1;
-    AL(s.AinvDeltaZ2, k, j, J) = z2;
   }
 }
 

@@ -273,133 +273,175 @@ __global__ void k_tvd_psi_cell(DevMesh m, DevState s, DevWork w, DevDims d, int 
     w.PSIi[zix(t, k, e, L1, E)] = (in && hlim != 0) ? limiter(hlim, w.ri[zix(t, k, e, L1, E)], A0(w.phii, k, e, E)) : 0.0;
 }
 
-// Per-column implicit update (solve_transport_equation_v19.f90:921-1303): horizontal
-// upwind advection, diffusion and limiter fluxes explicit; vertical advection explicit;
-// vertical diffusion implicit (Thomas); clamp to [0, 50].
-__global__ void k_tvd_column(DevMesh m, DevState s, DevWork w, DevDims d, TvdArgs a, int *err) {
-  int e = blockIdx.x * blockDim.x + threadIdx.x;
+// Per-column implicit update (solve_transport_equation_v19.f90:921-1303), in two passes.
+// k_tvd_rhs, one thread per (layer, cell): horizontal upwind advection, diffusion and
+// limiter fluxes and the vertical advection/limiter terms, explicit, into w.tvd_x (row of
+// the layer); k_tvd_solve, one thread per cell column: vertical diffusion implicit (the
+// Thomas recurrence of tridiagonal_solver.f90 with NT right-hand sides, in place in
+// w.tvd_x, elimination factors in w.tri_gam), then the clamp to [0, 50] into con_new.
+__global__ void k_tvd_rhs(DevMesh m, DevState s, DevWork w, DevDims d, TvdArgs a) {
+  long q = (long)blockIdx.x * blockDim.x + threadIdx.x;
   const int E = d.E, L = d.L, L1 = d.L + 1, L2 = d.L + 2, J = d.J;
+  if (q >= (long)L * E) return;
+  int e = (int)(q % E), k = (int)(q / E) + 1;
+  int bl = m.bottom_layer_at_element[e], tl = s.top_layer_at_element[e];
+  if (tl == 0 || k < bl || k > tl) return;
+  const int NT = d.NT, Nt = a.Nt, sub = a.sub;
+  const double dt2 = a.dt2;
+  int nv = tl - bl + 1, N = m.tri_or_quad[e];
+  double ar = m.area[e];
+  double x[2];
+  double sqi[2] = {0, 0}, sqii[2] = {0, 0}, shd[2] = {0, 0}, shl[2] = {0, 0};
+  for (int l = 1; l <= N; l++) {
+    int j = TB(m.facenum_at_cell, l, e, E) - 1, ii = TB(m.adj_cellnum_at_cell, l, e, E);
+    if (!face_active(m, j)) continue;
+    double Q2 = AL(w.Qj, k, j, J) * (double)TB(m.sign_in_outflow, l, e, E);
+    double dfj = AL(w.dj, k, j, J);
+    for (int t = 0; t < NT; t++) {
+      double c0 = w.con[cix(t, k, e, L2, E)];
+      double ci = con_across(m, w, d, e, ii, k, t);
+      if (Q2 > 0.0)
+        sqi[t] = sqi[t] + fabs(Q2) * c0;
+      else if (Q2 < 0.0)
+        sqii[t] = sqii[t] + fabs(Q2) * ci;
+      shd[t] = shd[t] + dfj * (ci - c0);
+      shl[t] = shl[t] + w.PSIj[nix(t, k, j, L, J)] * fabs(Q2) * (c0 - ci);
+    }
+  }
+  double dz0 = AL(s.dz_cell, k, e, E);
+  double dzo = dz0 + (AL(s.dz_cell_new, k, e, E) - dz0) * sub / Nt;
+  for (int t = 0; t < NT; t++) {
+    double h_adv = dt2 * (sqii[t] - sqi[t]);
+    double h_diff = dt2 * shd[t];
+    double h_flux = 0.5 * dt2 * shl[t];
+    double rhs = ar * dzo * w.con[cix(t, k, e, L2, E)] + h_adv + h_diff;
+    if (a.hlim > 0) rhs = rhs + h_flux;
+    x[t] = rhs;
+  }
+  if (nv != 1) {
+    for (int t = 0; t < NT; t++) {
+      double ck = w.con[cix(t, k, e, L2, E)];
+      double v_adv, vft = 0.0;
+      if (k == bl) {
+        double Q = A0(w.Qi, k, e, E), ckp = w.con[cix(t, k + 1, e, L2, E)];
+        double Ap = fabs(0.5 * (Q + fabs(Q))), Am = fabs(0.5 * (Q - fabs(Q)));
+        v_adv = dt2 * (-(Ap * ck) + Am * ckp);
+        if (a.vlim > 0) {
+          double ps = w.PSIi[zix(t, k, e, L1, E)];
+          vft = 0.5 * dt2 * (-(ps * Ap * (ckp - ck)) + ps * Am * (ck - ckp));
+        }
+      } else if (k == tl) {
+        double Q = A0(w.Qi, k - 1, e, E), ckm = w.con[cix(t, k - 1, e, L2, E)];
+        double Ap = fabs(0.5 * (Q + fabs(Q))), Am = fabs(0.5 * (Q - fabs(Q)));
+        v_adv = dt2 * (Ap * ckm - Am * ck);
+        if (a.vlim > 0) {
+          double ps = w.PSIi[zix(t, k - 1, e, L1, E)];
+          vft = 0.5 * dt2 * (ps * Ap * (ck - ckm) - ps * Am * (ckm - ck));
+        }
+      } else {
+        double Q1 = A0(w.Qi, k - 1, e, E), Q2 = A0(w.Qi, k, e, E);
+        double ckm = w.con[cix(t, k - 1, e, L2, E)], ckp = w.con[cix(t, k + 1, e, L2, E)];
+        double A1p = fabs(0.5 * (Q1 + fabs(Q1))), A1m = fabs(0.5 * (Q1 - fabs(Q1)));
+        double A2p = fabs(0.5 * (Q2 + fabs(Q2))), A2m = fabs(0.5 * (Q2 - fabs(Q2)));
+        v_adv = dt2 * (A1p * ckm - A2p * ck + A2m * ckp - A1m * ck);
+        if (a.vlim > 0) {
+          double p1 = w.PSIi[zix(t, k - 1, e, L1, E)], p2 = w.PSIi[zix(t, k, e, L1, E)];
+          vft = 0.5 * dt2 * (p1 * A1p * (ck - ckm) - p2 * A2p * (ckp - ck) + p2 * A2m * (ck - ckp) - p1 * A1m * (ckm - ck));
+        }
+      }
+      x[t] = x[t] + v_adv;
+      if (a.vlim > 0) x[t] = x[t] + vft;
+    }
+  }
+  for (int t = 0; t < NT; t++) w.tvd_x[nix(t, k, e, L, E)] = x[t];
+}
+
+// Tridiagonal row r (0-based, surface first) of cell e: solve_transport_equation_v19.f90:957-1025
+__device__ __forceinline__ void tvd_row(const DevState &s, const DevWork &w, int e, int E, int tl, int bl, int r,
+                                        double ar, double dt2, int sub, int Nt, double &av, double &bv, double &cv) {
+  int nv = tl - bl + 1;
+  av = 0.0;
+  cv = 0.0;
+  if (tl == bl) {
+    double dz0 = AL(s.dz_cell, tl, e, E);
+    double dzn = dz0 + (AL(s.dz_cell_new, tl, e, E) - dz0) * (sub + 1) / Nt;
+    bv = ar * dzn;
+  } else if (r == 0) {
+    cv = -dt2 * A0(w.di, tl - 1, e, E);
+    double dz0 = AL(s.dz_cell, tl, e, E);
+    double dzn = dz0 + (AL(s.dz_cell_new, tl, e, E) - dz0) * (sub + 1) / Nt;
+    bv = ar * dzn - cv;
+  } else if (r == nv - 1) {
+    av = -dt2 * A0(w.di, bl, e, E);
+    double dz0 = AL(s.dz_cell, bl, e, E);
+    double dzn = dz0 + (AL(s.dz_cell_new, bl, e, E) - dz0) * (sub + 1) / Nt;
+    bv = -av + ar * dzn;
+  } else {
+    int k = tl - r;
+    av = -dt2 * A0(w.di, k, e, E);
+    cv = -dt2 * A0(w.di, k - 1, e, E);
+    double dz0 = AL(s.dz_cell, k, e, E);
+    double dzn = dz0 + (AL(s.dz_cell_new, k, e, E) - dz0) * (sub + 1) / Nt;
+    bv = -av + ar * dzn - cv;
+  }
+}
+
+__global__ void k_tvd_solve(DevMesh m, DevState s, DevWork w, DevDims d, TvdArgs a, int *err) {
+  int e = blockIdx.x * blockDim.x + threadIdx.x;
+  const int E = d.E, L = d.L;
   if (e >= E) return;
   int bl = m.bottom_layer_at_element[e], tl = s.top_layer_at_element[e];
   if (tl == 0) return;
   const int NT = d.NT, Nt = a.Nt, sub = a.sub;
   const double dt2 = a.dt2;
-  int nv = tl - bl + 1, N = m.tri_or_quad[e];
+  int nv = tl - bl + 1;
   double ar = m.area[e];
-  double av[GOM_LMAX], bv[GOM_LMAX], cv[GOM_LMAX], x[GOM_LMAX][2];
-  for (int k = bl + 1; k <= tl - 1; k++) {
-    int r = tl - k;
-    av[r] = -dt2 * A0(w.di, k, e, E);
-    cv[r] = -dt2 * A0(w.di, k - 1, e, E);
-    double dz0 = AL(s.dz_cell, k, e, E);
-    double dzn = dz0 + (AL(s.dz_cell_new, k, e, E) - dz0) * (sub + 1) / Nt;
-    bv[r] = -av[r] + ar * dzn - cv[r];
-  }
-  if (tl == bl) {
-    double dz0 = AL(s.dz_cell, tl, e, E);
-    double dzn = dz0 + (AL(s.dz_cell_new, tl, e, E) - dz0) * (sub + 1) / Nt;
-    bv[0] = ar * dzn;
-  } else {
-    cv[0] = -dt2 * A0(w.di, tl - 1, e, E);
-    double dz0 = AL(s.dz_cell, tl, e, E);
-    double dzn = dz0 + (AL(s.dz_cell_new, tl, e, E) - dz0) * (sub + 1) / Nt;
-    bv[0] = ar * dzn - cv[0];
-    av[nv - 1] = -dt2 * A0(w.di, bl, e, E);
-    dz0 = AL(s.dz_cell, bl, e, E);
-    dzn = dz0 + (AL(s.dz_cell_new, bl, e, E) - dz0) * (sub + 1) / Nt;
-    bv[nv - 1] = -av[nv - 1] + ar * dzn;
-  }
-  for (int k = bl; k <= tl; k++) {
-    int r = tl - k;
-    double sqi[2] = {0, 0}, sqii[2] = {0, 0}, shd[2] = {0, 0}, shl[2] = {0, 0};
-    for (int l = 1; l <= N; l++) {
-      int j = TB(m.facenum_at_cell, l, e, E) - 1, ii = TB(m.adj_cellnum_at_cell, l, e, E);
-      if (!face_active(m, j)) continue;
-      double Q2 = AL(w.Qj, k, j, J) * (double)TB(m.sign_in_outflow, l, e, E);
-      double dfj = AL(w.dj, k, j, J);
-      for (int t = 0; t < NT; t++) {
-        double c0 = w.con[cix(t, k, e, L2, E)];
-        double ci = con_across(m, w, d, e, ii, k, t);
-        if (Q2 > 0.0)
-          sqi[t] = sqi[t] + fabs(Q2) * c0;
-        else if (Q2 < 0.0)
-          sqii[t] = sqii[t] + fabs(Q2) * ci;
-        shd[t] = shd[t] + dfj * (ci - c0);
-        shl[t] = shl[t] + w.PSIj[nix(t, k, j, L, J)] * fabs(Q2) * (c0 - ci);
-      }
-    }
-    double dz0 = AL(s.dz_cell, k, e, E);
-    double dzo = dz0 + (AL(s.dz_cell_new, k, e, E) - dz0) * sub / Nt;
-    for (int t = 0; t < NT; t++) {
-      double h_adv = dt2 * (sqii[t] - sqi[t]);
-      double h_diff = dt2 * shd[t];
-      double h_flux = 0.5 * dt2 * shl[t];
-      double rhs = ar * dzo * w.con[cix(t, k, e, L2, E)] + h_adv + h_diff;
-      if (a.hlim > 0) rhs = rhs + h_flux;
-      x[r][t] = rhs;
-    }
-  }
-  if (nv != 1) {
-    for (int k = bl; k <= tl; k++) {
-      int r = tl - k;
-      for (int t = 0; t < NT; t++) {
-        double ck = w.con[cix(t, k, e, L2, E)];
-        double v_adv, vft = 0.0;
-        if (k == bl) {
-          double Q = A0(w.Qi, k, e, E), ckp = w.con[cix(t, k + 1, e, L2, E)];
-          double Ap = fabs(0.5 * (Q + fabs(Q))), Am = fabs(0.5 * (Q - fabs(Q)));
-          v_adv = dt2 * (-(Ap * ck) + Am * ckp);
-          if (a.vlim > 0) {
-            double ps = w.PSIi[zix(t, k, e, L1, E)];
-            vft = 0.5 * dt2 * (-(ps * Ap * (ckp - ck)) + ps * Am * (ck - ckp));
-          }
-        } else if (k == tl) {
-          double Q = A0(w.Qi, k - 1, e, E), ckm = w.con[cix(t, k - 1, e, L2, E)];
-          double Ap = fabs(0.5 * (Q + fabs(Q))), Am = fabs(0.5 * (Q - fabs(Q)));
-          v_adv = dt2 * (Ap * ckm - Am * ck);
-          if (a.vlim > 0) {
-            double ps = w.PSIi[zix(t, k - 1, e, L1, E)];
-            vft = 0.5 * dt2 * (ps * Ap * (ck - ckm) - ps * Am * (ckm - ck));
-          }
-        } else {
-          double Q1 = A0(w.Qi, k - 1, e, E), Q2 = A0(w.Qi, k, e, E);
-          double ckm = w.con[cix(t, k - 1, e, L2, E)], ckp = w.con[cix(t, k + 1, e, L2, E)];
-          double A1p = fabs(0.5 * (Q1 + fabs(Q1))), A1m = fabs(0.5 * (Q1 - fabs(Q1)));
-          double A2p = fabs(0.5 * (Q2 + fabs(Q2))), A2m = fabs(0.5 * (Q2 - fabs(Q2)));
-          v_adv = dt2 * (A1p * ckm - A2p * ck + A2m * ckp - A1m * ck);
-          if (a.vlim > 0) {
-            double p1 = w.PSIi[zix(t, k - 1, e, L1, E)], p2 = w.PSIi[zix(t, k, e, L1, E)];
-            vft = 0.5 * dt2 * (p1 * A1p * (ck - ckm) - p2 * A2p * (ckp - ck) + p2 * A2m * (ck - ckp) - p1 * A1m * (ckm - ck));
-          }
-        }
-        x[r][t] = x[r][t] + v_adv;
-        if (a.vlim > 0) x[r][t] = x[r][t] + vft;
-      }
-    }
-  }
-  // Thomas, tridiagonal_solver.f90:32-61, with NT right-hand sides
-  double gam[GOM_LMAX];
-  if (bv[0] == 0.0) {
+  double av, bv, cv, cprev, x[2];
+  tvd_row(s, w, e, E, tl, bl, 0, ar, dt2, sub, Nt, av, bv, cv);
+  if (bv == 0.0) {
     latch(err, GOMGPU_ERR_TRIDIAG_PIVOT, e + 1);
     return;
   }
-  double den = bv[0];
-  for (int t = 0; t < NT; t++) x[0][t] = x[0][t] / den;
-  for (int jr = 1; jr < nv; jr++) {
-    gam[jr] = cv[jr - 1] / den;
-    den = bv[jr] - av[jr] * gam[jr];
+  double den = bv;
+  for (int t = 0; t < NT; t++) {
+    size_t o = nix(t, tl, e, L, E);
+    x[t] = w.tvd_x[o] / den;
+    w.tvd_x[o] = x[t];
+  }
+  cprev = cv;
+  for (int r = 1; r < nv; r++) {
+    tvd_row(s, w, e, E, tl, bl, r, ar, dt2, sub, Nt, av, bv, cv);
+    double gam = cprev / den;
+    den = bv - av * gam;
     if (den == 0.0) {
       latch(err, GOMGPU_ERR_TRIDIAG_PIVOT, e + 1);
       return;
     }
-    for (int t = 0; t < NT; t++) x[jr][t] = (x[jr][t] - av[jr] * x[jr - 1][t]) / den;
-  }
-  for (int jr = nv - 2; jr >= 0; jr--)
-    for (int t = 0; t < NT; t++) x[jr][t] = x[jr][t] - gam[jr + 1] * x[jr + 1][t];
-  for (int k = bl; k <= tl; k++) {
     for (int t = 0; t < NT; t++) {
-      double v = x[tl - k][t];
-      size_t o = nix(t, k, e, L, E);
+      size_t o = nix(t, tl - r, e, L, E);
+      x[t] = (w.tvd_x[o] - av * x[t]) / den;
+      w.tvd_x[o] = x[t];
+    }
+    AL(w.tri_gam, r + 1, e, E) = gam;
+    cprev = cv;
+  }
+  // back substitution, then the clamp into con_new (rows top down = layers tl .. bl)
+  for (int t = 0; t < NT; t++) {
+    double v = x[t];
+    size_t o = nix(t, bl, e, L, E);
+    if (v >= 0.0 && v <= 50.0)
+      w.con_new[o] = v;
+    else if (v < 0.0)
+      w.con_new[o] = 0.0;
+    else if (v > 50.0)
+      w.con_new[o] = 50.0;
+  }
+  for (int r = nv - 2; r >= 0; r--) {
+    double gam = AL(w.tri_gam, r + 2, e, E);
+    for (int t = 0; t < NT; t++) {
+      size_t o = nix(t, tl - r, e, L, E);
+      x[t] = w.tvd_x[o] - gam * x[t];
+      double v = x[t];
       if (v >= 0.0 && v <= 50.0)
         w.con_new[o] = v;
       else if (v < 0.0)
