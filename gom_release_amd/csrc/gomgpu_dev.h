@@ -74,6 +74,10 @@ struct DevWork {
   // elimination factors of the column tridiagonal solves, (L, max(E, J))
   double *tri_gam;
   double *tvd_x;          // (L, E, NT) right-hand sides / solution of the TVD column solve
+  // two-equation closure (allocated when turbulence_flag == 3): level diffusivities of the
+  // current state (0:L, J) and layer quantities / solutions (L, J)
+  double *tk_av, *tk_kv, *tk_qd, *tk_qd2;
+  double *tk_q2bt, *tk_xlbt, *tk_shear, *tk_rz, *tk_q2tmp, *tk_x;
   // budgets
   double *budget;         // [0] volume, [1] salt
   int *err;               // [0] code, [1] entity

@@ -48,7 +48,10 @@ __global__ void k_tvd_cellnode_out(DevState, DevWork, DevDims);
 __global__ void k_eos_nodeface(DevMesh, DevState, DevDims, double, int, int, int *);
 __global__ void k_eos_cell(DevMesh, DevState, DevDims, int *);
 __global__ void k_bars(DevMesh, DevState, DevDims);
-__global__ void k_turbulence(DevMesh, DevState, DevForcing, DevWork, DevDims, TurbParams, int *);
+__global__ void k_turb_ri(DevMesh, DevState, DevWork, DevDims, TurbParams);
+__global__ void k_turb_prep(DevMesh, DevState, DevWork, DevDims, TurbParams, int *);
+__global__ void k_turb_solve(DevMesh, DevState, DevForcing, DevWork, DevDims, TurbParams, int *);
+__global__ void k_turb_post(DevMesh, DevState, DevWork, DevDims, TurbParams);
 __global__ void k_turb_kv(DevMesh, DevState, DevWork, DevDims);
 __global__ void k_budget(DevMesh, DevState, DevDims, double *);
 __global__ void k_copy(double *, const double *, int);
@@ -339,6 +342,13 @@ extern "C" int gomgpu_create(const gomgpu_mesh *mesh, const gomgpu_params *param
         (rc = dalloc(c, &w.face_new, NT * L * J)) || (rc = dalloc(c, &w.budget, 2 * ((E + 255) / 256) + 2)) ||
         (rc = dalloc(c, &w.err, 2)) || (rc = dalloc(c, &w.turb_kvf, (L + 1) * J)) ||
         (rc = dalloc(c, &w.tri_gam, L * (E > J ? E : J))) || (rc = dalloc(c, &w.tvd_x, NT * L * E)))
+      goto fail;
+    if (p.turbulence_flag == 3 &&
+        ((rc = dalloc(c, &w.tk_av, (L + 1) * J)) || (rc = dalloc(c, &w.tk_kv, (L + 1) * J)) ||
+         (rc = dalloc(c, &w.tk_qd, (L + 1) * J)) || (rc = dalloc(c, &w.tk_qd2, (L + 1) * J)) ||
+         (rc = dalloc(c, &w.tk_q2bt, L * J)) || (rc = dalloc(c, &w.tk_xlbt, L * J)) ||
+         (rc = dalloc(c, &w.tk_shear, L * J)) || (rc = dalloc(c, &w.tk_rz, L * J)) ||
+         (rc = dalloc(c, &w.tk_q2tmp, L * J)) || (rc = dalloc(c, &w.tk_x, L * J))))
       goto fail;
   }
   if (hipHostMalloc((void **)&c->h_err, 5 * sizeof(int)) != hipSuccess) {
@@ -675,7 +685,14 @@ static int stage_impl(gomgpu_ctx *c, int id) {
       tp.flag = p.turbulence_flag;
       tp.model = p.turbulence_model;
       tp.stab = p.stability_function;
-      k_turbulence<<<nblk(d.J, 64), 64, 0, st>>>(c->m, c->s, c->f, c->w, d, tp, err);
+      if (tp.flag == 3) {
+        long nlev = (long)(d.L + 1) * d.J;
+        k_turb_prep<<<nblk(nlev, BS), BS, 0, st>>>(c->m, c->s, c->w, d, tp, err);
+        k_turb_solve<<<nblk(d.J, 128), 128, 0, st>>>(c->m, c->s, c->f, c->w, d, tp, err);
+        k_turb_post<<<nblk(nlev, BS), BS, 0, st>>>(c->m, c->s, c->w, d, tp);
+      } else {
+        k_turb_ri<<<nblk((long)d.L * d.J, BS), BS, 0, st>>>(c->m, c->s, c->w, d, tp);
+      }
       k_turb_kv<<<nblk((long)(d.L + 1) * d.E, BS), BS, 0, st>>>(c->m, c->s, c->w, d);
       break;
     }

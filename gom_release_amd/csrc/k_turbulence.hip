@@ -1,13 +1,15 @@
 // k_turbulence.hip - vertical turbulence closure (turbulence_closure_v1.f90:9-492).
 //
-// One thread per face column. Scheme 1 (Richardson step) and 2 (Pacanowski-Philander) are
-// per-level algebra; scheme 3 solves the k (q2) and psi equations of the generic length
-// scale closure as two tridiagonal systems per column (the Thomas recurrence of
-// tridiagonal_solver.f90, as in the momentum kernel), converts back to whole levels and
-// evaluates the quasi-equilibrium stability functions (Galperin et al. 1988 or Kantha &
-// Clayson 1994; the reference's algebraic_stress is undefined). The face diffusivity is
-// then averaged onto cells for the transport (Kv). The arithmetic follows the CPU
-// restatement statement for statement.
+// Scheme 1 (Richardson step) and 2 (Pacanowski-Philander) are per-level algebra. Scheme 3
+// solves the k (q2) and psi equations of the generic length scale closure as two
+// tridiagonal systems per face column, in three passes: per (level, face) the stability
+// functions of the current state and the layer quantities; per face the two Thomas
+// recurrences of tridiagonal_solver.f90 in global work arrays, the length scale and the
+// conversion to whole levels; per (level, face) the new diffusivities. The stability
+// functions are the quasi-equilibrium ones of Galperin et al. (1988) or Kantha & Clayson
+// (1994) (the reference's algebraic_stress is undefined). The face diffusivity is then
+// averaged onto cells for the transport (Kv). Each value is computed by the same expression
+// sequence as the CPU restatement.
 #include "gomgpu_dev.h"
 
 #define TK_A1 0.92
@@ -74,129 +76,137 @@ __device__ __forceinline__ double rz_level(const TurbParams &p, const DevState &
   return p.g / p.rho_o * drhodz;
 }
 
-// Thomas recurrence of tridiagonal_solver.f90:26-60 (one right-hand side), rows 0..n-1
-__device__ __forceinline__ int thomas1(int n, const double *a, const double *b, const double *c, const double *r,
-                                       double *x, double *gam) {
-  if (b[0] == 0.0) return GOMGPU_ERR_TRIDIAG_PIVOT;
-  double den = b[0];
-  x[0] = r[0] / den;
-  for (int j = 2; j <= n; j++) {
-    gam[j - 1] = c[j - 2] / den;
-    den = b[j - 1] - a[j - 1] * gam[j - 1];
-    if (den == 0.0) return GOMGPU_ERR_TRIDIAG_PIVOT;
-    x[j - 1] = (r[j - 1] - a[j - 1] * x[j - 2]) / den;
+// Schemes 1 and 2 (turbulence_closure_v1.f90:75-141): per-level algebra, one thread per
+// (level, face).
+__global__ void k_turb_ri(DevMesh m, DevState s, DevWork w, DevDims d, TurbParams p) {
+  long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  const int J = d.J, L = d.L;
+  if (t >= (long)L * J) return;
+  int j = (int)(t % J), k = (int)(t / J) + 1;
+  int kt = s.top_layer_at_face[j];
+  if (kt == 0) return;
+  int kb = m.bottom_layer_at_face[j];
+  if (k < kb || k > kt) return;
+  const double g = p.g;
+  double vmax = 0.0, vmin = 0.0;
+  if (p.flag == 2) {
+    double h = m.h_face[j];
+    if (h <= p.h1_pp) {
+      vmax = p.vdmax_pp1;
+      vmin = p.vdmin_pp1;
+    } else if (h < p.h2_pp) {
+      vmax = p.vdmax_pp1 + (p.vdmax_pp2 - p.vdmax_pp1) * (h - p.h1_pp) / (p.h2_pp - p.h1_pp);
+      vmin = p.vdmin_pp1 + (p.vdmin_pp2 - p.vdmin_pp1) * (h - p.h1_pp) / (p.h2_pp - p.h1_pp);
+    } else {
+      vmax = p.vdmax_pp2;
+      vmin = p.vdmin_pp2;
+    }
   }
-  for (int j = n - 1; j >= 1; j--) x[j - 1] = x[j - 1] - gam[j] * x[j];
-  return 0;
+  double ri;
+  if (k == kt) {
+    ri = 0.0;
+  } else {
+    double dzh = A0(s.dzhalf_face_new, k, j, J);
+    double drhodz = (AL(s.rho_face, k + 1, j, J) - AL(s.rho_face, k, j, J)) / dzh;
+    double bvf = -g * (drhodz / p.rho_o + g / 2250000.0);
+    double dundz = (AL(s.un_face_new, k + 1, j, J) - AL(s.un_face_new, k, j, J)) / dzh;
+    double dutdz = (AL(s.vn_face_new, k + 1, j, J) - AL(s.vn_face_new, k, j, J)) / dzh;
+    double shear2 = dundz * dundz + dutdz * dutdz;
+    shear2 = dmaxf(shear2, 1.0e-10);
+    ri = dmaxf(bvf / shear2, 0.0);
+  }
+  double av, kv;
+  if (p.flag == 1) {
+    if (ri < 0.25) {
+      av = 1.0e-3;
+      kv = 1.0 * 1.0e-3;
+    } else {
+      av = 1.0e-5;
+      kv = 1.0e-5;
+    }
+  } else {
+    double tt = 1.0 + 5.0 * ri;
+    av = vmax / (tt * tt) + vmin;
+    kv = av / (1.0 + 5.0 * ri) + p.tdmin_pp;
+  }
+  A0(s.Av, k, j, J) = av;
+  A0(w.turb_kvf, k, j, J) = kv;
 }
 
-__global__ void k_turbulence(DevMesh m, DevState s, DevForcing f, DevWork w, DevDims d, TurbParams p, int *err) {
+// Scheme 3, pass 1 (turbulence_closure_v1.f90:56-66, 158-215), one thread per (level, face):
+// the diffusivities of the current k, l at whole levels kb..kt (level kb-1 takes level
+// kb's), and at layers kb..kt the half-level k, l, the squared shear, (g/rho_o) drho/dz and
+// the two stop checks.
+__global__ void k_turb_prep(DevMesh m, DevState s, DevWork w, DevDims d, TurbParams p, int *err) {
+  long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  const int J = d.J, L = d.L, P = d.P;
+  if (t >= (long)(L + 1) * J) return;
+  int j = (int)(t % J), k = (int)(t / J);
+  int kt = s.top_layer_at_face[j];
+  if (kt == 0) return;
+  int kb = m.bottom_layer_at_face[j];
+  if (k < kb - 1 || k > kt) return;
+  const double g = p.g;
+  {
+    int kl = k == kb - 1 ? kb : k;
+    double vd, td, qd, qd2;
+    alg_stress(p, A0(s.q2, kl, j, J), A0(s.xl, kl, j, J), rz_level(p, s, J, j, kl, kt), vd, td, qd, qd2);
+    A0(w.tk_av, k, j, J) = vd;
+    A0(w.tk_kv, k, j, J) = td;
+    A0(w.tk_qd, k, j, J) = qd;
+    A0(w.tk_qd2, k, j, J) = qd2;
+  }
+  if (k < kb) return;
+  double q2bt = (A0(s.q2, k - 1, j, J) + A0(s.q2, k, j, J)) / 2.0;
+  double xlbt = (A0(s.xl, k - 1, j, J) + A0(s.xl, k, j, J)) / 2.0;
+  if (xlbt < p.xlmin2) latch(err, GOMGPU_ERR_TURB_XL, j + 1);
+  int n1 = TB(m.nodenum_at_face, 1, j, J) - 1, n2 = TB(m.nodenum_at_face, 2, j, J) - 1;
+  double udown = (A0(s.u_node, k - 1, n1, P) + A0(s.u_node, k - 1, n2, P)) / 2.0;
+  double uup = (A0(s.u_node, k, n1, P) + A0(s.u_node, k, n2, P)) / 2.0;
+  double vdown = (A0(s.v_node, k - 1, n1, P) + A0(s.v_node, k - 1, n2, P)) / 2.0;
+  double vup = (A0(s.v_node, k, n1, P) + A0(s.v_node, k, n2, P)) / 2.0;
+  double dzk = AL(s.dz_face_new, k, j, J);
+  double dudz = (uup - udown) / dzk, dvdz = (vup - vdown) / dzk;
+  double shear = dudz * dudz + dvdz * dvdz;
+  double drhodz;
+  if (k == kt || k == kb) {
+    drhodz = 0.0;
+  } else {
+    double rho_up = AL(s.rho_face, k, j, J) + dzk / 2.0 / A0(s.dzhalf_face_new, k, j, J) *
+                                                  (AL(s.rho_face, k + 1, j, J) - AL(s.rho_face, k, j, J));
+    double rho_down = AL(s.rho_face, k - 1, j, J) + AL(s.dz_face_new, k - 1, j, J) / 2.0 /
+                                                        A0(s.dzhalf_face_new, k - 1, j, J) *
+                                                        (AL(s.rho_face, k, j, J) - AL(s.rho_face, k - 1, j, J));
+    drhodz = (rho_up - rho_down) / dzk;
+  }
+  AL(w.tk_q2bt, k, j, J) = q2bt;
+  AL(w.tk_xlbt, k, j, J) = xlbt;
+  AL(w.tk_shear, k, j, J) = shear;
+  AL(w.tk_rz, k, j, J) = g / p.rho_o * drhodz;
+  int a1 = TB(m.adj_cellnum_at_face, 1, j, J), a2 = TB(m.adj_cellnum_at_face, 2, j, J);
+  double etam = a2 != 0 ? dmaxf(s.eta_cell_new[a1 - 1], s.eta_cell_new[a2 - 1]) : s.eta_cell_new[a1 - 1];
+  double hf = m.h_face[j];
+  double zctr = (k == kt) ? p.MSL + etam - dzk / 2.0 : m.z_level[k] - dzk / 2.0;
+  double dists = p.MSL + etam - zctr, distb = zctr - (p.MSL - hf);
+  double xlmax = (double)0.4f * dists * distb / (hf + etam);
+  if (xlmax <= 0.0) latch(err, GOMGPU_ERR_TURB_DIST, j + 1);
+}
+
+// Scheme 3, pass 2 (turbulence_closure_v1.f90:219-390), one thread per face: the k and psi
+// tridiagonal systems (Thomas in place in w.tk_x / w.tk_q2tmp, factors in w.tri_gam), the
+// length scale with Galperin's clipping, and k, l back at whole levels.
+__global__ void k_turb_solve(DevMesh m, DevState s, DevForcing f, DevWork w, DevDims d, TurbParams p,
+                             int *err) {
   int j = blockIdx.x * blockDim.x + threadIdx.x;
-  const int J = d.J, L = d.L;
+  const int J = d.J;
   if (j >= J) return;
   int kt = s.top_layer_at_face[j];
   if (kt == 0) return;
   int kb = m.bottom_layer_at_face[j];
-  const double dt = p.dt, g = p.g;
-  double *KV = w.turb_kvf;
-  if (p.flag == 1 || p.flag == 2) {
-    double vmax = 0.0, vmin = 0.0;
-    if (p.flag == 2) {
-      double h = m.h_face[j];
-      if (h <= p.h1_pp) {
-        vmax = p.vdmax_pp1;
-        vmin = p.vdmin_pp1;
-      } else if (h < p.h2_pp) {
-        vmax = p.vdmax_pp1 + (p.vdmax_pp2 - p.vdmax_pp1) * (h - p.h1_pp) / (p.h2_pp - p.h1_pp);
-        vmin = p.vdmin_pp1 + (p.vdmin_pp2 - p.vdmin_pp1) * (h - p.h1_pp) / (p.h2_pp - p.h1_pp);
-      } else {
-        vmax = p.vdmax_pp2;
-        vmin = p.vdmin_pp2;
-      }
-    }
-    for (int k = kb; k <= kt; k++) {
-      double ri;
-      if (k == kt) {
-        ri = 0.0;
-      } else {
-        double dzh = A0(s.dzhalf_face_new, k, j, J);
-        double drhodz = (AL(s.rho_face, k + 1, j, J) - AL(s.rho_face, k, j, J)) / dzh;
-        double bvf = -g * (drhodz / p.rho_o + g / 2250000.0);
-        double dundz = (AL(s.un_face_new, k + 1, j, J) - AL(s.un_face_new, k, j, J)) / dzh;
-        double dutdz = (AL(s.vn_face_new, k + 1, j, J) - AL(s.vn_face_new, k, j, J)) / dzh;
-        double shear2 = dundz * dundz + dutdz * dutdz;
-        shear2 = dmaxf(shear2, 1.0e-10);
-        ri = dmaxf(bvf / shear2, 0.0);
-      }
-      double av, kv;
-      if (p.flag == 1) {
-        if (ri < 0.25) {
-          av = 1.0e-3;
-          kv = 1.0 * 1.0e-3;
-        } else {
-          av = 1.0e-5;
-          kv = 1.0e-5;
-        }
-      } else {
-        double t = 1.0 + 5.0 * ri;
-        av = vmax / (t * t) + vmin;
-        kv = av / (1.0 + 5.0 * ri) + p.tdmin_pp;
-      }
-      A0(s.Av, k, j, J) = av;
-      A0(KV, k, j, J) = kv;
-    }
-    return;
-  }
-  // scheme 3
-  double avc[TLEV], kvc[TLEV], qdc[TLEV], qd2c[TLEV];
-  double q2bt[TLEV], xlbt[TLEV], rzbt[TLEV], shearbt[TLEV], q2tmp[TLEV], xltmp[TLEV];
-  double a[TLEV], b[TLEV], c[TLEV], rhs[TLEV], sol[TLEV], gam[TLEV];
-  for (int k = kb; k <= kt; k++)
-    alg_stress(p, A0(s.q2, k, j, J), A0(s.xl, k, j, J), rz_level(p, s, J, j, k, kt), avc[k], kvc[k], qdc[k], qd2c[k]);
-  avc[kb - 1] = avc[kb];
-  kvc[kb - 1] = kvc[kb];
-  int n1 = TB(m.nodenum_at_face, 1, j, J) - 1, n2 = TB(m.nodenum_at_face, 2, j, J) - 1;
-  const int P = d.P;
-  for (int k = kb; k <= kt; k++) {
-    q2bt[k] = (A0(s.q2, k - 1, j, J) + A0(s.q2, k, j, J)) / 2.0;
-    xlbt[k] = (A0(s.xl, k - 1, j, J) + A0(s.xl, k, j, J)) / 2.0;
-    if (xlbt[k] < p.xlmin2) {
-      latch(err, GOMGPU_ERR_TURB_XL, j + 1);
-      return;
-    }
-    double udown = (A0(s.u_node, k - 1, n1, P) + A0(s.u_node, k - 1, n2, P)) / 2.0;
-    double uup = (A0(s.u_node, k, n1, P) + A0(s.u_node, k, n2, P)) / 2.0;
-    double vdown = (A0(s.v_node, k - 1, n1, P) + A0(s.v_node, k - 1, n2, P)) / 2.0;
-    double vup = (A0(s.v_node, k, n1, P) + A0(s.v_node, k, n2, P)) / 2.0;
-    double dzk = AL(s.dz_face_new, k, j, J);
-    double dudz = (uup - udown) / dzk, dvdz = (vup - vdown) / dzk;
-    shearbt[k] = dudz * dudz + dvdz * dvdz;
-    double drhodz;
-    if (k == kt || k == kb) {
-      drhodz = 0.0;
-    } else {
-      double rho_up = AL(s.rho_face, k, j, J) + dzk / 2.0 / A0(s.dzhalf_face_new, k, j, J) *
-                                                    (AL(s.rho_face, k + 1, j, J) - AL(s.rho_face, k, j, J));
-      double rho_down = AL(s.rho_face, k - 1, j, J) + AL(s.dz_face_new, k - 1, j, J) / 2.0 /
-                                                          A0(s.dzhalf_face_new, k - 1, j, J) *
-                                                          (AL(s.rho_face, k, j, J) - AL(s.rho_face, k - 1, j, J));
-      drhodz = (rho_up - rho_down) / dzk;
-    }
-    rzbt[k] = g / p.rho_o * drhodz;
-  }
+  const double dt = p.dt;
   int a1 = TB(m.adj_cellnum_at_face, 1, j, J), a2 = TB(m.adj_cellnum_at_face, 2, j, J);
   double etam = a2 != 0 ? dmaxf(s.eta_cell_new[a1 - 1], s.eta_cell_new[a2 - 1]) : s.eta_cell_new[a1 - 1];
   double hf = m.h_face[j];
-  for (int k = kb; k <= kt; k++) {
-    double dzk = AL(s.dz_face_new, k, j, J);
-    double zctr = (k == kt) ? p.MSL + etam - dzk / 2.0 : m.z_level[k] - dzk / 2.0;
-    double dists = p.MSL + etam - zctr, distb = zctr - (p.MSL - hf);
-    double xlmax = (double)0.4f * dists * distb / (hf + etam);
-    if (xlmax <= 0.0) {
-      latch(err, GOMGPU_ERR_TURB_DIST, j + 1);
-      return;
-    }
-  }
   double wn = f.wind_stress_normal[j], wt = f.wind_stress_tangnt[j];
   double q2fs = p.fs_coef * sqrt(wn * wn + wt * wt) / 2.0;
   q2fs = dmaxf(q2fs, p.q2min);
@@ -207,21 +217,24 @@ __global__ void k_turbulence(DevMesh m, DevState s, DevForcing f, DevWork w, Dev
   double xlbot = dmaxf(p.xlmin2, AL(s.dz_face_new, kb, j, J) / 2.0 * (double)0.4f);
   int nq = kt - kb + 1;
   double cm3 = p.cmiu0 * p.cmiu0 * p.cmiu0;
+  // ---- k equation (rows top down; row k <-> layer kl = kt - k + 1)
+  double den = 0.0, cprev = 0.0, x = 0.0;
   for (int k = 1; k <= nq; k++) {
     int kl = kt - k + 1;
     double dzk = AL(s.dz_face_new, kl, j, J);
-    double eleft = 0.0, eright = 0.0;
+    double eleft = 0.0, eright = 0.0, a = 0.0, c = 0.0;
     if (k > 1) {
-      eleft = -(qdc[kl] * dt / A0(s.dzhalf_face_new, kl, j, J));
-      a[k - 1] = eleft;
+      eleft = -(A0(w.tk_qd, kl, j, J) * dt / A0(s.dzhalf_face_new, kl, j, J));
+      a = eleft;
     }
     if (k < nq) {
-      eright = -(qdc[kl - 1] * dt / A0(s.dzhalf_face_new, kl - 1, j, J));
-      c[k - 1] = eright;
+      eright = -(A0(w.tk_qd, kl - 1, j, J) * dt / A0(s.dzhalf_face_new, kl - 1, j, J));
+      c = eright;
     }
-    double prod = (avc[kl] + avc[kl - 1]) / 2.0 * shearbt[kl];
-    double buoy = (kvc[kl] + kvc[kl - 1]) / 2.0 * rzbt[kl];
-    double diss = cm3 * sqrt(q2bt[kl]) / xlbt[kl];
+    double q2bt = AL(w.tk_q2bt, kl, j, J), xlbt = AL(w.tk_xlbt, kl, j, J);
+    double prod = (A0(w.tk_av, kl, j, J) + A0(w.tk_av, kl - 1, j, J)) / 2.0 * AL(w.tk_shear, kl, j, J);
+    double buoy = (A0(w.tk_kv, kl, j, J) + A0(w.tk_kv, kl - 1, j, J)) / 2.0 * AL(w.tk_rz, kl, j, J);
+    double diss = cm3 * sqrt(q2bt) / xlbt;
     double pplus, pminus;
     if (prod + buoy > 0.0) {
       pplus = prod + buoy;
@@ -230,51 +243,71 @@ __global__ void k_turbulence(DevMesh m, DevState s, DevForcing f, DevWork w, Dev
       pplus = 0.0;
       pminus = -prod - buoy;
     }
-    b[k - 1] = dzk - eleft - eright + (diss + pminus / q2bt[kl]) * dt * dzk;
-    rhs[k - 1] = dzk * (q2bt[kl] + dt * pplus);
+    double b = dzk - eleft - eright + (diss + pminus / q2bt) * dt * dzk;
+    double r = dzk * (q2bt + dt * pplus);
+    if (k == 1) {
+      if (b == 0.0) {
+        latch(err, GOMGPU_ERR_TRIDIAG_PIVOT, j + 1);
+        return;
+      }
+      den = b;
+      x = r / den;
+    } else {
+      double gam = cprev / den;
+      den = b - a * gam;
+      if (den == 0.0) {
+        latch(err, GOMGPU_ERR_TRIDIAG_PIVOT, j + 1);
+        return;
+      }
+      x = (r - a * x) / den;
+      AL(w.tri_gam, k, j, J) = gam;
+    }
+    AL(w.tk_q2tmp, kl, j, J) = x;
+    cprev = c;
   }
-  a[0] = 0.0;
-  c[nq - 1] = 0.0;
-  int e = thomas1(nq, a, b, c, rhs, sol, gam);
-  if (e) {
-    latch(err, e, j + 1);
-    return;
+  {  // back substitution and the boundary values (:268-278)
+    int kl = kb;  // row nq
+    AL(w.tk_q2tmp, kl, j, J) = kl == kt ? q2fs : (kl == kb ? q2bot : dmaxf(x, p.q2min));
+    for (int k = nq - 1; k >= 1; k--) {
+      kl = kt - k + 1;
+      x = AL(w.tk_q2tmp, kl, j, J) - AL(w.tri_gam, k + 1, j, J) * x;
+      AL(w.tk_q2tmp, kl, j, J) = kl == kt ? q2fs : (kl == kb ? q2bot : dmaxf(x, p.q2min));
+    }
   }
-  for (int k = 1; k <= nq; k++) {
-    int kl = kt - k + 1;
-    q2tmp[kl] = kl == kt ? q2fs : (kl == kb ? q2bot : dmaxf(sol[k - 1], p.q2min));
-  }
+  // ---- psi equation (:281-347)
   for (int k = 1; k <= nq; k++) {
     int kl = kt - k + 1;
     double dzk = AL(s.dz_face_new, kl, j, J);
-    double eleft = 0.0, eright = 0.0;
+    double eleft = 0.0, eright = 0.0, a = 0.0, c = 0.0;
+    double qd2k = A0(w.tk_qd2, kl, j, J);
     if (k > 1) {
-      eleft = -(qd2c[kl] * dt / A0(s.dzhalf_face_new, kl, j, J));
-      a[k - 1] = eleft;
+      eleft = -(qd2k * dt / A0(s.dzhalf_face_new, kl, j, J));
+      a = eleft;
     }
     if (k < nq) {
-      eright = -(qd2c[kl - 1] * dt / A0(s.dzhalf_face_new, kl - 1, j, J));
-      c[k - 1] = eright;
+      eright = -(A0(w.tk_qd2, kl - 1, j, J) * dt / A0(s.dzhalf_face_new, kl - 1, j, J));
+      c = eright;
     }
+    double q2bt = AL(w.tk_q2bt, kl, j, J), xlbt = AL(w.tk_xlbt, kl, j, J), rz = AL(w.tk_rz, kl, j, J);
     double bb = dzk - eleft - eright;
     double cpsi2p;
     if (p.model == GOMGPU_GLS_MY || p.model == GOMGPU_GLS_KL) {
       double zctr = m.z_level[kl] - dzk / 2.0;
       double dists = dmaxf(p.MSL + etam - zctr, p.xlmin2);
       double distb = dmaxf(zctr - (p.MSL - hf), p.xlmin2);
-      double r1 = xlbt[kl] / (double)0.4f / distb, r2 = xlbt[kl] / (double)0.4f / dists;
+      double r1 = xlbt / (double)0.4f / distb, r2 = xlbt / (double)0.4f / dists;
       double psi = 1.0 + (double)1.33f * (r1 * r1) + (double)0.25f * (r2 * r2);
       cpsi2p = psi * p.cpsi2;
     } else {
       cpsi2p = p.cpsi2;
     }
-    double diss = cpsi2p * cm3 * sqrt(q2bt[kl]) / xlbt[kl];
+    double diss = cpsi2p * cm3 * sqrt(q2bt) / xlbt;
     bb = bb + dt * dzk * diss;
-    double prod = p.cpsi1 * (avc[kl] + avc[kl - 1]) / 2.0 * shearbt[kl];
+    double prod = p.cpsi1 * (A0(w.tk_av, kl, j, J) + A0(w.tk_av, kl - 1, j, J)) / 2.0 * AL(w.tk_shear, kl, j, J);
     double cpsi3;
     if (p.model == GOMGPU_GLS_MY) {
       cpsi3 = (double)0.9f;
-    } else if (rzbt[kl] > 0.0) {
+    } else if (rz > 0.0) {
       cpsi3 = 1.0;
     } else {
       switch (p.model) {
@@ -284,7 +317,7 @@ __global__ void k_turbulence(DevMesh m, DevState s, DevForcing f, DevWork w, Dev
         default: cpsi3 = (double)0.1f; break;
       }
     }
-    double buoy = cpsi3 * (kvc[kl] + kvc[kl - 1]) / 2.0 * rzbt[kl];
+    double buoy = cpsi3 * (A0(w.tk_kv, kl, j, J) + A0(w.tk_kv, kl - 1, j, J)) / 2.0 * rz;
     double pplus, pminus;
     if (prod + buoy > 0.0) {
       pplus = prod + buoy;
@@ -293,77 +326,102 @@ __global__ void k_turbulence(DevMesh m, DevState s, DevForcing f, DevWork w, Dev
       pplus = 0.0;
       pminus = -prod - buoy;
     }
-    bb = bb + pminus / q2bt[kl] * dt * dzk;
-    if (kl == kb || kl == kt) bb = bb + dt * (double)0.4f * p.rnub * qd2c[kl] / xlbt[kl];
-    b[k - 1] = bb;
-    double tmp = gls_pow(p.cmiu0, p.rpub) * gls_pow(q2bt[kl], p.rmub) * gls_pow(xlbt[kl], p.rnub);
-    rhs[k - 1] = dzk * tmp * (1.0 + dt * pplus / q2bt[kl]);
-  }
-  a[0] = 0.0;
-  c[nq - 1] = 0.0;
-  e = thomas1(nq, a, b, c, rhs, sol, gam);
-  if (e) {
-    latch(err, e, j + 1);
-    return;
-  }
-  for (int k = 1; k <= nq; k++) {
-    int kl = kt - k + 1;
-    double q2l = dmaxf(sol[k - 1], p.psimin);
-    double x;
-    if (kl == kt)
-      x = xlfs;
-    else if (kl == kb)
-      x = xlbot;
-    else
-      x = gls_pow(q2l * gls_pow(p.cmiu0, -p.rpub) * gls_pow(q2tmp[kl], -p.rmub), 1.0 / p.rnub);
-    if (rzbt[kl] < 0.0) {
-      double upper = sqrt(-(double)0.56f * q2tmp[kl] / rzbt[kl]);
-      x = dminf(x, upper);
+    bb = bb + pminus / q2bt * dt * dzk;
+    if (kl == kb || kl == kt) bb = bb + dt * (double)0.4f * p.rnub * qd2k / xlbt;
+    double tmp = gls_pow(p.cmiu0, p.rpub) * gls_pow(q2bt, p.rmub) * gls_pow(xlbt, p.rnub);
+    double r = dzk * tmp * (1.0 + dt * pplus / q2bt);
+    if (k == 1) {
+      if (bb == 0.0) {
+        latch(err, GOMGPU_ERR_TRIDIAG_PIVOT, j + 1);
+        return;
+      }
+      den = bb;
+      x = r / den;
+    } else {
+      double gam = cprev / den;
+      den = bb - a * gam;
+      if (den == 0.0) {
+        latch(err, GOMGPU_ERR_TRIDIAG_PIVOT, j + 1);
+        return;
+      }
+      x = (r - a * x) / den;
+      AL(w.tri_gam, k, j, J) = gam;
     }
-    double cq = p.cmiu0 * sqrt(q2tmp[kl]);
-    double xl_max = cq * cq * cq / p.eps_min;
-    xltmp[kl] = dmaxf(p.xlmin2, dminf(xl_max, x));
+    AL(w.tk_x, kl, j, J) = x;
+    cprev = c;
   }
+  // back substitution and the length scale (:351-368)
+  for (int k = nq; k >= 1; k--) {
+    int kl = kt - k + 1;
+    if (k < nq) x = AL(w.tk_x, kl, j, J) - AL(w.tri_gam, k + 1, j, J) * x;
+    double q2l = dmaxf(x, p.psimin);
+    double q2tmp = AL(w.tk_q2tmp, kl, j, J), rz = AL(w.tk_rz, kl, j, J);
+    double xv;
+    if (kl == kt)
+      xv = xlfs;
+    else if (kl == kb)
+      xv = xlbot;
+    else
+      xv = gls_pow(q2l * gls_pow(p.cmiu0, -p.rpub) * gls_pow(q2tmp, -p.rmub), 1.0 / p.rnub);
+    if (rz < 0.0) {
+      double upper = sqrt(-(double)0.56f * q2tmp / rz);
+      xv = dminf(xv, upper);
+    }
+    double cq = p.cmiu0 * sqrt(q2tmp);
+    double xl_max = cq * cq * cq / p.eps_min;
+    // the solution of row k-1 still needs x, so the clipped length goes to the xl array
+    A0(s.xl, kl, j, J) = dmaxf(p.xlmin2, dminf(xl_max, xv));
+  }
+  // whole levels (:372-390). xl[k] holds layer k's length until level k replaces it; in
+  // ascending order level k reads layers k and k+1 before either is overwritten.
   for (int k = kb - 1; k <= kt; k++) {
-    double q, x;
+    double q, xv;
     if (k == kb - 1) {
       q = q2bot;
-      x = xlbot;
+      xv = xlbot;
     } else if (k == kt) {
       q = q2fs;
-      x = xlfs;
+      xv = xlfs;
     } else {
       double r = AL(s.dz_face_new, k, j, J) / 2.0 / A0(s.dzhalf_face_new, k, j, J);
-      q = q2tmp[k] + r * (q2tmp[k + 1] - q2tmp[k]);
-      x = xltmp[k] + r * (xltmp[k + 1] - xltmp[k]);
-      if (q < 0.0 || x < p.xlmin2) {
+      double q2a = AL(w.tk_q2tmp, k, j, J), q2b = AL(w.tk_q2tmp, k + 1, j, J);
+      double xa = A0(s.xl, k, j, J), xb = A0(s.xl, k + 1, j, J);
+      q = q2a + r * (q2b - q2a);
+      xv = xa + r * (xb - xa);
+      if (q < 0.0 || xv < p.xlmin2) {
         latch(err, GOMGPU_ERR_TURB_NEGATIVE, j + 1);
         return;
       }
     }
     A0(s.q2, k, j, J) = q;
-    A0(s.xl, k, j, J) = x;
+    A0(s.xl, k, j, J) = xv;
   }
-  for (int k = kb; k <= kt; k++) {
-    double vd, td, qd, qd2;
-    alg_stress(p, A0(s.q2, k, j, J), A0(s.xl, k, j, J), rz_level(p, s, J, j, k, kt), vd, td, qd, qd2);
-    A0(s.Av, k, j, J) = vd;
-    A0(KV, k, j, J) = td;
-  }
-  for (int k = 0; k <= kb - 2; k++) {
+}
+
+// Scheme 3, pass 3 (turbulence_closure_v1.f90:397-427), one thread per (level, face): the
+// new diffusivities at levels kb..kt and their extension above and below the column.
+__global__ void k_turb_post(DevMesh m, DevState s, DevWork w, DevDims d, TurbParams p) {
+  long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  const int J = d.J, L = d.L;
+  if (t >= (long)(L + 1) * J) return;
+  int j = (int)(t % J), k = (int)(t / J);
+  int kt = s.top_layer_at_face[j];
+  if (kt == 0) return;
+  int kb = m.bottom_layer_at_face[j];
+  if (k <= kb - 2) {
     A0(s.q2, k, j, J) = A0(s.q2, kb - 1, j, J);
     A0(s.xl, k, j, J) = A0(s.xl, kb - 1, j, J);
   }
-  for (int k = 1; k <= kb - 1; k++) {
-    A0(s.Av, k, j, J) = A0(s.Av, kb, j, J);
-    A0(KV, k, j, J) = A0(KV, kb, j, J);
-  }
-  for (int k = kt + 1; k <= L; k++) {
+  if (k > kt) {
     A0(s.q2, k, j, J) = A0(s.q2, kt, j, J);
     A0(s.xl, k, j, J) = A0(s.xl, kt, j, J);
-    A0(s.Av, k, j, J) = A0(s.Av, kt, j, J);
-    A0(KV, k, j, J) = A0(KV, kt, j, J);
   }
+  if (k == 0) return;
+  int kl = k < kb ? kb : (k > kt ? kt : k);
+  double vd, td, qd, qd2;
+  alg_stress(p, A0(s.q2, kl, j, J), A0(s.xl, kl, j, J), rz_level(p, s, J, j, kl, kt), vd, td, qd, qd2);
+  A0(s.Av, k, j, J) = vd;
+  A0(w.turb_kvf, k, j, J) = td;
 }
 
 // Face diffusivity -> cell Kv: mean over the cell's wet faces at the level clamped to each
