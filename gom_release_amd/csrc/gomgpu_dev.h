@@ -39,6 +39,9 @@ struct DevForcing {
 #define DF_R(name, ext) const double *name;
   GOMGPU_FORCING_REAL(DF_R)
 #undef DF_R
+  // per-step scalars in device memory (so a captured step stays valid): [0] baroclinic
+  // spin-up factor, [1] tidal spin-up factor, [2] elapsed time
+  const double *scal;
 };
 
 // Scalars every kernel needs, passed by value.
@@ -110,7 +113,7 @@ __device__ __forceinline__ void latch(int *err, int code, int entity) {
 
 // Kernel argument blocks shared by the launchers and the kernels.
 struct MomParams {
-  double dt, g, th, rho_o, smag, Ah0, Kh0, spinup_bc;
+  double dt, g, th, rho_o, smag, Ah0, Kh0;
   int baroclinic;
 };
 

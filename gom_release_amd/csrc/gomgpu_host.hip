@@ -55,6 +55,7 @@ __global__ void k_turb_post(DevMesh, DevState, DevWork, DevDims, TurbParams);
 __global__ void k_turb_kv(DevMesh, DevState, DevWork, DevDims);
 __global__ void k_budget(DevMesh, DevState, DevDims, double *);
 __global__ void k_copy(double *, const double *, int);
+__global__ void k_fill(double *, double, long);
 extern "C" hipError_t gom_launch_pcg(const PcgArgs *args, int max_blocks, int cooperative, hipStream_t st);
 extern "C" int gom_pcg_max_blocks(int device);
 extern "C" hipError_t gom_launch_pcg_reg(const PcgArgs *args, int nb, int bs, int cooperative, hipStream_t st);
@@ -307,6 +308,16 @@ extern "C" int gomgpu_create(const gomgpu_mesh *mesh, const gomgpu_params *param
   GOMGPU_FORCING_REAL(AL_FR)
 #undef AL_FR
   {
+    double *dp;
+    const double init[4] = {1.0, 1.0, 0.0, 0.0};
+    if ((rc = dalloc(c, &dp, 4))) goto fail;
+    if (hipMemcpy(dp, init, sizeof init, hipMemcpyHostToDevice) != hipSuccess) {
+      rc = seterr(c, GOMGPU_E_HIP, "hipMemcpy failed");
+      goto fail;
+    }
+    c->f.scal = dp;
+  }
+  {
     long E = p.maxele, J = p.maxface, P = p.maxnod, L = p.maxlayer;
     long NT = c->d.NT > 0 ? c->d.NT : 1, OB = p.num_ob_cell > 0 ? p.num_ob_cell : 1,
          QB = p.num_Qb_cell > 0 ? p.num_Qb_cell : 1;
@@ -369,6 +380,7 @@ extern "C" int gomgpu_create(const gomgpu_mesh *mesh, const gomgpu_params *param
   }
     GOMGPU_FORCING_REAL(SZ_F)
 #undef SZ_F
+    tot += 4;  // the per-step scalars
     c->stage_len = tot;
     if (hipHostMalloc((void **)&c->h_stage, tot * sizeof(double)) != hipSuccess ||
         hipEventCreateWithFlags(&c->fev, hipEventDisableTiming) != hipSuccess ||
@@ -453,6 +465,12 @@ extern "C" int gomgpu_set_forcing(gomgpu_ctx *c, const gomgpu_forcing *f) {
   }
   GOMGPU_FORCING_REAL(UP_F)
 #undef UP_F
+  if (off + 4 > c->stage_len) return seterr(c, GOMGPU_E_ARG, "forcing staging overflow");
+  c->h_stage[off] = f->spinup_function_baroclinic;
+  c->h_stage[off + 1] = f->spinup_function_tide;
+  c->h_stage[off + 2] = f->elapsed_time;
+  c->h_stage[off + 3] = 0.0;
+  HIPCHK(c, hipMemcpyAsync((double *)c->f.scal, c->h_stage + off, 4 * sizeof(double), hipMemcpyHostToDevice, c->st));
   HIPCHK(c, hipEventRecord(c->fev, c->st));
   return 0;
 }
@@ -469,6 +487,98 @@ extern "C" int gomgpu_set_forcing(gomgpu_ctx *c, const gomgpu_forcing *f) {
     c->kernel_ms[i] += ms_;                                   \
     c->kernel_calls[i] += 1;                                  \
   }
+
+// ---- free surface: assembly, PCG, node elevations (solve_free_surface_equation.f90)
+static int fs_pre(gomgpu_ctx *c) {
+  const gomgpu_params &p = c->p;
+  const DevDims &d = c->d;
+  const int BS = 256;
+  double gdt2th2 = p.gravity * (p.dt * p.dt) * (p.theta * p.theta);
+  k_fs_face<<<nblk(d.J, BS), BS, 0, c->st>>>(c->m, c->s, c->w, d, gdt2th2);
+  k_fs_cell<<<nblk(d.E, BS), BS, 0, c->st>>>(c->m, c->s, c->f, c->w, d, p.dt, p.theta, c->w.err);
+  HIPCHK(c, hipGetLastError());
+  return 0;
+}
+
+static void pcg_args(gomgpu_ctx *c, PcgArgs &a) {
+  a.diag = c->w.fs_diag;
+  a.off = c->w.fs_off;
+  a.b = c->w.fs_rhs;
+  a.col = c->m.adj_cellnum_at_cell;
+  a.nfaces = c->m.tri_or_quad;
+  a.x = c->w.pcg_x;
+  a.r = c->w.pcg_r;
+  a.d0 = c->w.pcg_d0;
+  a.d1 = c->w.pcg_d1;
+  a.q = c->w.pcg_q;
+  a.s = c->w.pcg_s;
+  a.minv = c->w.pcg_minv;
+  a.partA = c->w.pcg_part;
+  long pstride = c->pcg_blocks > 4096 ? c->pcg_blocks : 4096;
+  a.partB = c->w.pcg_part + pstride;
+  a.partC = c->w.pcg_part + 2 * pstride;
+  a.partD = c->w.pcg_part + 3 * pstride;
+  a.pub = c->w.pcg_pub;
+  a.slots = c->w.pcg_slots;
+  a.info = c->w.pcg_info;
+  a.bar = c->w.pcg_bar;
+  a.barrier_mode = c->pcg_barrier;
+  a.n = c->d.E;
+  a.maxit = c->p.max_iteration_pcg;
+  a.tol = c->p.error_tolerance;
+}
+
+// Blocks of the register-resident solve for n rows at 512 threads and up to 4 rows per
+// thread: a function of n alone, so a system gets the same partition alone and in a batch.
+static int pcg_blocks_for(long n) {
+  int nb = (int)((n + 4L * 512 - 1) / (4L * 512));
+  return nb < 1 ? 1 : nb;
+}
+
+static int fs_pcg(gomgpu_ctx *c) {
+  const DevDims &d = c->d;
+  hipStream_t st = c->st;
+  PcgArgs a;
+  pcg_args(c, a);
+  KT_BEGIN(KT_PCG);
+  {
+    hipError_t le = hipErrorInvalidValue;
+    if (c->pcg_mode >= 1 && c->pcg_mode <= 3) {
+      int rmax = c->pcg_bs >= 1024 ? 2 : c->pcg_bs >= 512 ? 4 : 8;
+      int nb = c->pcg_nb > 0 ? c->pcg_nb : (int)((d.E + (long)rmax * c->pcg_bs - 1) / ((long)rmax * c->pcg_bs));
+      if (nb < 1) nb = 1;
+      if (nb > 4096) nb = 4096;
+      if (!c->pcg_coop && nb > c->ncu) nb = c->ncu;  // plain launch: at most one block per CU
+      if (c->pcg_mode == 2 || c->pcg_mode == 3) {
+        int slots = c->pcg_mode == 3 && nb <= c->pcg_bs && nb <= PCG_MAX_SLOT_BLOCKS;
+        if (nb <= gom_pcg_cg1_capacity(c->device, d.E, nb, c->pcg_bs, slots))
+          le = gom_launch_pcg_cg1(&a, nb, c->pcg_bs, slots, c->pcg_coop, st);
+      } else if (nb <= gom_pcg_reg_capacity(c->device, d.E, nb, c->pcg_bs)) {
+        le = gom_launch_pcg_reg(&a, nb, c->pcg_bs, c->pcg_coop, st);
+      }
+    }
+    if (le == hipErrorInvalidValue) {
+      // memory-resident solve; at most one block per CU (the same grid in a batch)
+      int cap = c->pcg_blocks < c->ncu ? c->pcg_blocks : c->ncu;
+      if (c->pcg_cap > 0 && c->pcg_cap < cap) cap = c->pcg_cap;
+      le = gom_launch_pcg(&a, cap, c->pcg_coop, st);
+    }
+    HIPCHK(c, le);
+  }
+  KT_END(KT_PCG);
+  c->pcg_ran = true;
+  return 0;
+}
+
+static int fs_post(gomgpu_ctx *c) {
+  const DevDims &d = c->d;
+  const int BS = 256;
+  k_copy<<<nblk(d.E, BS), BS, 0, c->st>>>(c->s.eta_cell_new, c->w.pcg_x, d.E);
+  k_eta_node<<<nblk(d.P, BS), BS, 0, c->st>>>(c->m, c->s, d);
+  k_eta_node_dry<<<nblk(d.P, BS), BS, 0, c->st>>>(c->m, c->s, d);
+  HIPCHK(c, hipGetLastError());
+  return 0;
+}
 
 static int stage_impl(gomgpu_ctx *c, int id) {
   const gomgpu_params &p = c->p;
@@ -498,8 +608,7 @@ static int stage_impl(gomgpu_ctx *c, int id) {
       }
       break;
     case GOMGPU_STAGE_MOMENTUM: {
-      MomParams q{p.dt, p.gravity, p.theta, p.rho_o, p.smagorinsky_parameter, p.Ah_0, p.Kh_0, c->spin_bc,
-                  p.baroclinic_flag};
+      MomParams q{p.dt, p.gravity, p.theta, p.rho_o, p.smagorinsky_parameter, p.Ah_0, p.Kh_0, p.baroclinic_flag};
       KT_BEGIN(KT_MOMENTUM);
       k_mom_rhs<<<nblk((long)d.L * d.J, BS), BS, 0, st>>>(c->m, c->s, c->f, d, q);
       k_mom_solve<<<nblk(d.J, 128), 128, 0, st>>>(c->m, c->s, c->f, c->w, d, q, err);
@@ -507,67 +616,10 @@ static int stage_impl(gomgpu_ctx *c, int id) {
       break;
     }
     case GOMGPU_STAGE_FREESURF: {
-      double gdt2th2 = p.gravity * (p.dt * p.dt) * (p.theta * p.theta);
-      k_fs_face<<<nblk(d.J, BS), BS, 0, st>>>(c->m, c->s, c->w, d, gdt2th2);
-      k_fs_cell<<<nblk(d.E, BS), BS, 0, st>>>(c->m, c->s, c->f, c->w, d, p.dt, p.theta, err);
-      PcgArgs a;
-      a.diag = c->w.fs_diag;
-      a.off = c->w.fs_off;
-      a.b = c->w.fs_rhs;
-      a.col = c->m.adj_cellnum_at_cell;
-      a.nfaces = c->m.tri_or_quad;
-      a.x = c->w.pcg_x;
-      a.r = c->w.pcg_r;
-      a.d0 = c->w.pcg_d0;
-      a.d1 = c->w.pcg_d1;
-      a.q = c->w.pcg_q;
-      a.s = c->w.pcg_s;
-      a.minv = c->w.pcg_minv;
-      a.partA = c->w.pcg_part;
-      {
-        long pstride = c->pcg_blocks > 4096 ? c->pcg_blocks : 4096;
-        a.partB = c->w.pcg_part + pstride;
-        a.partC = c->w.pcg_part + 2 * pstride;
-        a.partD = c->w.pcg_part + 3 * pstride;
-      }
-      a.pub = c->w.pcg_pub;
-      a.slots = c->w.pcg_slots;
-      a.info = c->w.pcg_info;
-      a.bar = c->w.pcg_bar;
-      a.barrier_mode = c->pcg_barrier;
-      a.n = d.E;
-      a.maxit = p.max_iteration_pcg;
-      a.tol = p.error_tolerance;
-      KT_BEGIN(KT_PCG);
-      {
-        hipError_t le = hipErrorInvalidValue;
-        if (c->pcg_mode >= 1 && c->pcg_mode <= 3) {
-          int rmax = c->pcg_bs >= 1024 ? 2 : c->pcg_bs >= 512 ? 4 : 8;
-          int nb = c->pcg_nb > 0 ? c->pcg_nb : (int)((d.E + (long)rmax * c->pcg_bs - 1) / ((long)rmax * c->pcg_bs));
-          if (nb < 1) nb = 1;
-          if (nb > 4096) nb = 4096;
-          if (!c->pcg_coop && nb > c->ncu) nb = c->ncu;  // plain launch: at most one block per CU
-          if (c->pcg_mode == 2 || c->pcg_mode == 3) {
-            int slots = c->pcg_mode == 3 && nb <= c->pcg_bs && nb <= PCG_MAX_SLOT_BLOCKS;
-            if (nb <= gom_pcg_cg1_capacity(c->device, d.E, nb, c->pcg_bs, slots))
-              le = gom_launch_pcg_cg1(&a, nb, c->pcg_bs, slots, c->pcg_coop, st);
-          } else if (nb <= gom_pcg_reg_capacity(c->device, d.E, nb, c->pcg_bs)) {
-            le = gom_launch_pcg_reg(&a, nb, c->pcg_bs, c->pcg_coop, st);
-          }
-        }
-        if (le == hipErrorInvalidValue) {
-          int cap = c->pcg_blocks;
-          if (c->pcg_cap > 0 && c->pcg_cap < cap) cap = c->pcg_cap;
-          if (!c->pcg_coop && cap > c->ncu) cap = c->ncu;
-          le = gom_launch_pcg(&a, cap, c->pcg_coop, st);
-        }
-        HIPCHK(c, le);
-      }
-      KT_END(KT_PCG);
-      c->pcg_ran = true;
-      k_copy<<<nblk(d.E, BS), BS, 0, st>>>(c->s.eta_cell_new, c->w.pcg_x, d.E);
-      k_eta_node<<<nblk(d.P, BS), BS, 0, st>>>(c->m, c->s, d);
-      k_eta_node_dry<<<nblk(d.P, BS), BS, 0, st>>>(c->m, c->s, d);
+      int rc = fs_pre(c);
+      if (!rc) rc = fs_pcg(c);
+      if (!rc) rc = fs_post(c);
+      if (rc) return rc;
       break;
     }
     case GOMGPU_STAGE_VELOCITY:
@@ -598,7 +650,7 @@ static int stage_impl(gomgpu_ctx *c, int id) {
       a.heat = p.heat_option;
       a.sub = 0;
       long nE2 = (long)d.E * (d.L + 2);
-      HIPCHK(c, hipMemsetAsync(c->w.con_new, 0, sizeof(double) * (size_t)d.NT * d.L * d.E, st));
+      k_fill<<<nblk((long)d.NT * d.L * d.E, BS), BS, 0, st>>>(c->w.con_new, 0.0, (long)d.NT * d.L * d.E);
       k_tvd_init<<<nblk(nE2, BS), BS, 0, st>>>(c->m, c->s, c->f, c->w, d, a);
       for (int sub = 0; sub < a.Nt; sub++) {
         a.sub = sub;
@@ -648,7 +700,7 @@ static int stage_impl(gomgpu_ctx *c, int id) {
                 {c->s.dzhalf_face, c->s.dzhalf_face_new, (L + 1) * J},
                 {c->s.dz_node, c->s.dz_node_new, L * P},
                 {c->s.dzhalf_node, c->s.dzhalf_node_new, (L + 1) * P}};
-      for (auto &x : cp) HIPCHK(c, hipMemcpyAsync(x.dst, x.src, x.n * sizeof(double), hipMemcpyDeviceToDevice, st));
+      for (auto &x : cp) k_copy<<<nblk((long)x.n, BS), BS, 0, st>>>(x.dst, x.src, (int)x.n);
       k_bars<<<nblk(d.P, BS), BS, 0, st>>>(c->m, c->s, d);
       break;
     }
@@ -805,4 +857,261 @@ extern "C" int gomgpu_timings(gomgpu_ctx *c, int enable, const char **names, dou
       c->kernel_calls[i] = 0;
     }
   return k;
+}
+
+// ------------------------------------------------------------------ batches
+// A batch advances several independent scenarios (contexts on one device) by one step in
+// one submission. Per member the stages run on the member's own stream, so small members
+// overlap on the device; the PCG solves of all members run in fused launches between a
+// join and a fork of those streams (one k_pcg_cg1_batch launch per group of members whose
+// blocks fit the device together; a member too large for the register-resident solve gets
+// the memory-resident k_pcg on the batch stream). With graphs on, the whole submission is
+// captured once per membership and replayed every step.
+extern "C" hipError_t gom_launch_pcg_cg1_batch(const PcgArgs *d_members, const int *d_base, const int *d_nbm,
+                                               int nmem, int total_blocks, hipStream_t st);
+extern "C" int gom_pcg_cg1_batch_capacity(int device);
+
+struct BatchGroup {
+  int nmem = 0, total = 0;
+  PcgArgs *d_args = nullptr;
+  int *d_base = nullptr, *d_nbm = nullptr;
+};
+
+struct gomgpu_batch {
+  int device = 0;
+  hipStream_t st = nullptr;
+  std::vector<gomgpu_ctx *> members;
+  std::vector<hipEvent_t> ev_join;  // one per member, reused
+  hipEvent_t ev_fork = nullptr, ev_done = nullptr;
+  std::vector<BatchGroup> groups;
+  std::vector<gomgpu_ctx *> solo;
+  bool dirty = true, use_graph = true;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+  int capacity = 0;
+  std::string err;
+};
+
+static int bseterr(gomgpu_batch *b, int code, const std::string &msg) {
+  b->err = msg;
+  return code;
+}
+#define BHIP(b, call)                                                                     \
+  do {                                                                                    \
+    hipError_t e_ = (call);                                                               \
+    if (e_ != hipSuccess) return bseterr(b, GOMGPU_E_HIP, std::string(#call) + ": " + hipGetErrorString(e_)); \
+  } while (0)
+
+static void batch_free_plan(gomgpu_batch *b) {
+  for (auto &g : b->groups) {
+    if (g.d_args) hipFree(g.d_args);
+    if (g.d_base) hipFree(g.d_base);
+    if (g.d_nbm) hipFree(g.d_nbm);
+  }
+  b->groups.clear();
+  b->solo.clear();
+  if (b->exec) hipGraphExecDestroy(b->exec);
+  if (b->graph) hipGraphDestroy(b->graph);
+  b->exec = nullptr;
+  b->graph = nullptr;
+}
+
+static int batch_plan(gomgpu_batch *b) {
+  batch_free_plan(b);
+  std::vector<std::vector<gomgpu_ctx *>> grp;
+  int total = 0;
+  for (gomgpu_ctx *c : b->members) {
+    int nb = pcg_blocks_for(c->d.E);
+    if (nb > b->capacity || nb > PCG_MAX_SLOT_BLOCKS) {
+      b->solo.push_back(c);
+      continue;
+    }
+    int padded = (nb + 7) / 8 * 8;
+    if (grp.empty() || total + padded > b->capacity) {
+      grp.emplace_back();
+      total = 0;
+    }
+    grp.back().push_back(c);
+    total += padded;
+  }
+  for (auto &gm : grp) {
+    BatchGroup g;
+    std::vector<PcgArgs> args;
+    std::vector<int> base, nbm;
+    int off = 0;
+    for (gomgpu_ctx *c : gm) {
+      PcgArgs a;
+      pcg_args(c, a);
+      args.push_back(a);
+      int nb = pcg_blocks_for(c->d.E);
+      base.push_back(off);
+      nbm.push_back(nb);
+      off += (nb + 7) / 8 * 8;
+    }
+    g.nmem = (int)gm.size();
+    g.total = off;
+    BHIP(b, hipMalloc(&g.d_args, sizeof(PcgArgs) * args.size()));
+    BHIP(b, hipMalloc(&g.d_base, sizeof(int) * base.size()));
+    BHIP(b, hipMalloc(&g.d_nbm, sizeof(int) * nbm.size()));
+    BHIP(b, hipMemcpy(g.d_args, args.data(), sizeof(PcgArgs) * args.size(), hipMemcpyHostToDevice));
+    BHIP(b, hipMemcpy(g.d_base, base.data(), sizeof(int) * base.size(), hipMemcpyHostToDevice));
+    BHIP(b, hipMemcpy(g.d_nbm, nbm.data(), sizeof(int) * nbm.size(), hipMemcpyHostToDevice));
+    b->groups.push_back(g);
+  }
+  return 0;
+}
+
+static const int kPreStages[] = {GOMGPU_STAGE_FRICTION, GOMGPU_STAGE_ELM, GOMGPU_STAGE_MOMENTUM};
+static const int kPostStages[] = {GOMGPU_STAGE_VELOCITY,   GOMGPU_STAGE_LAYERS,    GOMGPU_STAGE_NODEVEL,
+                                  GOMGPU_STAGE_TURBULENCE, GOMGPU_STAGE_TRANSPORT, GOMGPU_STAGE_EOS,
+                                  GOMGPU_STAGE_UPDATE};
+
+// The submission of one batch step on b->st (captured or run eagerly).
+static int batch_enqueue(gomgpu_batch *b) {
+  BHIP(b, hipEventRecord(b->ev_fork, b->st));
+  for (size_t i = 0; i < b->members.size(); i++) {
+    gomgpu_ctx *c = b->members[i];
+    BHIP(b, hipStreamWaitEvent(c->st, b->ev_fork, 0));
+    for (int id : kPreStages)
+      if (int rc = stage_impl(c, id)) return bseterr(b, rc, c->err);
+    if (int rc = fs_pre(c)) return bseterr(b, rc, c->err);
+    BHIP(b, hipEventRecord(b->ev_join[i], c->st));
+    BHIP(b, hipStreamWaitEvent(b->st, b->ev_join[i], 0));
+  }
+  for (auto &g : b->groups) BHIP(b, gom_launch_pcg_cg1_batch(g.d_args, g.d_base, g.d_nbm, g.nmem, g.total, b->st));
+  for (gomgpu_ctx *c : b->solo) {
+    PcgArgs a;
+    pcg_args(c, a);
+    int cap = c->pcg_blocks < c->ncu ? c->pcg_blocks : c->ncu;
+    BHIP(b, gom_launch_pcg(&a, cap, 0, b->st));
+  }
+  BHIP(b, hipEventRecord(b->ev_fork, b->st));
+  for (size_t i = 0; i < b->members.size(); i++) {
+    gomgpu_ctx *c = b->members[i];
+    BHIP(b, hipStreamWaitEvent(c->st, b->ev_fork, 0));
+    if (int rc = fs_post(c)) return bseterr(b, rc, c->err);
+    for (int id : kPostStages)
+      if (int rc = stage_impl(c, id)) return bseterr(b, rc, c->err);
+    BHIP(b, hipEventRecord(b->ev_join[i], c->st));
+    BHIP(b, hipStreamWaitEvent(b->st, b->ev_join[i], 0));
+  }
+  return 0;
+}
+
+static int batch_capture(gomgpu_batch *b) {
+  BHIP(b, hipStreamBeginCapture(b->st, hipStreamCaptureModeThreadLocal));
+  int rc = batch_enqueue(b);
+  hipGraph_t g = nullptr;
+  hipError_t e = hipStreamEndCapture(b->st, &g);
+  if (rc) {
+    if (g) hipGraphDestroy(g);
+    return rc;
+  }
+  if (e != hipSuccess) return bseterr(b, GOMGPU_E_HIP, std::string("hipStreamEndCapture: ") + hipGetErrorString(e));
+  b->graph = g;
+  BHIP(b, hipGraphInstantiate(&b->exec, g, nullptr, nullptr, 0));
+  return 0;
+}
+
+extern "C" int gomgpu_batch_create(gomgpu_batch **out) {
+  if (!out) return GOMGPU_E_ARG;
+  *out = nullptr;
+  gomgpu_batch *b = new gomgpu_batch();
+  if (hipGetDevice(&b->device) != hipSuccess || hipStreamCreateWithFlags(&b->st, hipStreamNonBlocking) != hipSuccess ||
+      hipEventCreateWithFlags(&b->ev_fork, hipEventDisableTiming) != hipSuccess ||
+      hipEventCreateWithFlags(&b->ev_done, hipEventDisableTiming) != hipSuccess) {
+    if (b->st) hipStreamDestroy(b->st);
+    delete b;
+    return GOMGPU_E_HIP;
+  }
+  b->capacity = gom_pcg_cg1_batch_capacity(b->device);
+  if (const char *e = getenv("GOMGPU_BATCH_GRAPH")) b->use_graph = atoi(e) != 0;
+  *out = b;
+  return 0;
+}
+
+extern "C" int gomgpu_batch_add(gomgpu_batch *b, gomgpu_ctx *c) {
+  if (!b || !c) return GOMGPU_E_ARG;
+  if (c->device != b->device) return bseterr(b, GOMGPU_E_ARG, "member on another device");
+  for (gomgpu_ctx *x : b->members)
+    if (x == c) return bseterr(b, GOMGPU_E_ARG, "context already in the batch");
+  hipEvent_t ev;
+  BHIP(b, hipEventCreateWithFlags(&ev, hipEventDisableTiming));
+  c->timing = false;
+  b->members.push_back(c);
+  b->ev_join.push_back(ev);
+  b->dirty = true;
+  return 0;
+}
+
+extern "C" int gomgpu_batch_remove(gomgpu_batch *b, gomgpu_ctx *c) {
+  if (!b || !c) return GOMGPU_E_ARG;
+  for (size_t i = 0; i < b->members.size(); i++)
+    if (b->members[i] == c) {
+      BHIP(b, hipStreamSynchronize(b->st));
+      hipEventDestroy(b->ev_join[i]);
+      b->members.erase(b->members.begin() + i);
+      b->ev_join.erase(b->ev_join.begin() + i);
+      b->dirty = true;
+      return 0;
+    }
+  return bseterr(b, GOMGPU_E_ARG, "context not in the batch");
+}
+
+extern "C" int gomgpu_batch_size(gomgpu_batch *b) { return b ? (int)b->members.size() : -1; }
+
+extern "C" int gomgpu_batch_step(gomgpu_batch *b) {
+  if (!b) return GOMGPU_E_ARG;
+  if (b->members.empty()) return 0;
+  if (b->dirty) {
+    BHIP(b, hipStreamSynchronize(b->st));
+    if (int rc = batch_plan(b)) return rc;
+    if (b->use_graph)
+      if (int rc = batch_capture(b)) return rc;
+    b->dirty = false;
+  }
+  // the members' own queued work (forcing uploads) precedes the step
+  for (size_t i = 0; i < b->members.size(); i++) {
+    BHIP(b, hipEventRecord(b->ev_join[i], b->members[i]->st));
+    BHIP(b, hipStreamWaitEvent(b->st, b->ev_join[i], 0));
+  }
+  if (b->use_graph) {
+    BHIP(b, hipGraphLaunch(b->exec, b->st));
+  } else if (int rc = batch_enqueue(b)) {
+    return rc;
+  }
+  // and the members' later work (downloads, next forcing) follows it
+  BHIP(b, hipEventRecord(b->ev_done, b->st));
+  for (gomgpu_ctx *c : b->members) {
+    BHIP(b, hipStreamWaitEvent(c->st, b->ev_done, 0));
+    c->pcg_ran = true;
+  }
+  return 0;
+}
+
+extern "C" int gomgpu_batch_sync(gomgpu_batch *b) {
+  if (!b) return GOMGPU_E_ARG;
+  BHIP(b, hipStreamSynchronize(b->st));
+  int first = 0;
+  for (size_t i = 0; i < b->members.size(); i++) {
+    int rc = gomgpu_sync(b->members[i]);
+    if (rc && !first) {
+      first = rc;
+      b->err = "member " + std::to_string(i) + ": " + b->members[i]->err;
+    }
+  }
+  return first;
+}
+
+extern "C" const char *gomgpu_batch_last_error(gomgpu_batch *b) { return b ? b->err.c_str() : "null batch"; }
+
+extern "C" void gomgpu_batch_destroy(gomgpu_batch *b) {
+  if (!b) return;
+  if (b->st) hipStreamSynchronize(b->st);
+  batch_free_plan(b);
+  for (hipEvent_t e : b->ev_join) hipEventDestroy(e);
+  if (b->ev_fork) hipEventDestroy(b->ev_fork);
+  if (b->ev_done) hipEventDestroy(b->ev_done);
+  if (b->st) hipStreamDestroy(b->st);
+  delete b;
 }

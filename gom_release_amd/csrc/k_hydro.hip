@@ -548,8 +548,9 @@ __global__ void k_mom_rhs(DevMesh m, DevState s, DevForcing f, DevDims d, MomPar
     rtemp2 = dzl * (-(g * dt / (q.rho_o * Lf))) * sum2;
     rtemp3 = rtemp1 * ct2;
     rtemp4 = rtemp1 * st2;
-    G1 = G1 + rtemp3 * q.spinup_bc;
-    G2 = G2 + (rtemp2 + rtemp4) * q.spinup_bc;
+    const double spinup_bc = f.scal[0];
+    G1 = G1 + rtemp3 * spinup_bc;
+    G2 = G2 + (rtemp2 + rtemp4) * spinup_bc;
   }
   if (k == 1) {
     G1 = G1 + dt * f.wind_stress_normal[j];

@@ -200,6 +200,11 @@ __global__ void k_copy(double *dst, const double *src, int n) {
   if (i < n) dst[i] = src[i];
 }
 
+__global__ void k_fill(double *dst, double v, long n) {
+  long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) dst[i] = v;
+}
+
 // Host launcher: sizes the grid to the co-resident capacity (cooperative launch).
 // Host launcher. The grid is at most max_blocks (the co-resident capacity, or a smaller
 // tuned cap). cooperative != 0 launches through hipLaunchCooperativeKernel, which checks
@@ -241,14 +246,14 @@ extern "C" int gom_pcg_max_blocks(int device) {
 // generation; waiters spin on the generation.
 #define PCG_XCD 8
 
-__device__ __forceinline__ void bar2_sync(unsigned *bar, unsigned nb, unsigned &gen) {
+__device__ __forceinline__ void bar2_sync(unsigned *bar, unsigned nb, unsigned &gen, unsigned blk) {
   // every wave's global stores acknowledged by L2 before thread 0's agent-scope release
   // writes L2 back (CDNA counts stores in vmcnt; the workgroup barrier does not wait)
   __builtin_amdgcn_s_waitcnt(0);
   __syncthreads();
   if (threadIdx.x == 0) {
     unsigned g = gen;
-    unsigned grp = blockIdx.x % PCG_XCD;
+    unsigned grp = blk % PCG_XCD;
     unsigned ngrp = nb < PCG_XCD ? nb : PCG_XCD;
     unsigned gsize = nb / PCG_XCD + (grp < nb % PCG_XCD ? 1u : 0u);
     unsigned *cnt = bar + 32 * (grp + 1);  // 128-byte apart
@@ -349,7 +354,7 @@ __global__ void __launch_bounds__(BS) k_pcg_reg(PcgArgs a) {
   }
   double bsum = block_sum_bs(part, lds);
   if (threadIdx.x == 0) a.partB[b] = bsum;
-  bar2_sync(a.bar, nb, gen);
+  bar2_sync(a.bar, nb, gen, b);
   double delta_new = grid_total_bs(a.partB, nb, lds);
   double delta_0 = delta_new;
   double beta = 0.0;
@@ -385,7 +390,7 @@ __global__ void __launch_bounds__(BS) k_pcg_reg(PcgArgs a) {
     }
     bsum = block_sum_bs(part, lds);
     if (threadIdx.x == 0) a.partA[b] = bsum;
-    bar2_sync(a.bar, nb, gen);
+    bar2_sync(a.bar, nb, gen, b);
     double dq = grid_total_bs(a.partA, nb, lds);
     double alpha = delta_new / dq;
     part = 0.0;
@@ -400,7 +405,7 @@ __global__ void __launch_bounds__(BS) k_pcg_reg(PcgArgs a) {
     }
     bsum = block_sum_bs(part, lds);
     if (threadIdx.x == 0) a.partB[b] = bsum;
-    bar2_sync(a.bar, nb, gen);
+    bar2_sync(a.bar, nb, gen, b);
     double delta_old = delta_new;
     delta_new = grid_total_bs(a.partB, nb, lds);
     beta = delta_new / delta_old;
@@ -488,71 +493,95 @@ __device__ __forceinline__ unsigned long long slot_bits(double v) {
   return u == PCG_SLOT_SENTINEL ? PCG_CANONICAL_NAN : u;
 }
 
-__device__ __forceinline__ double block_total_bcast(double v, double *lds) {
-  double r = block_sum_bs(v, lds);
-  if (threadIdx.x == 0) lds[16] = r;
-  __syncthreads();
-  r = lds[16];
-  __syncthreads();
-  return r;
-}
-
 // A poll gives up after PCG_SLOT_SPIN_LIMIT spins (seconds) and raises the abort word;
 // every poll then stops waiting, the solve runs out its iterations on NaN totals and the
 // host reports the abort. With all blocks co-resident this never triggers.
 #define PCG_SLOT_SPIN_LIMIT (1u << 22)
 
-__device__ void slot_exchange(unsigned long long *slots, int nb, int it, double bg, double bd,
-                              double *lds, int *abort, double &gamma, double &delta) {
+// One iteration's reduction and exchange with three block barriers: the block's (gamma,
+// delta) partials (S1), the poll of the nb published pairs (S2) and the broadcast of the
+// totals (S3). LDS regions: [0,NW) and [NW,2NW) partials, [2NW,4NW) polled sums, [4NW,4NW+2)
+// totals; each region is rewritten only after a barrier that follows its last read. The
+// sums run in the same order as block_sum_bs / grid_total_bs, so the totals equal those of
+// slot_exchange.
+template <int BS>
+__device__ __forceinline__ void cg1_exchange(unsigned long long *slots, int b, int nb, int it, double pg, double pd,
+                                             double *lds, int *abort, double &gamma, double &delta) {
+  constexpr int NW = BS / 64;
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+  pg = wave_sum(pg);
+  pd = wave_sum(pd);
+  if (lane == 0) {
+    lds[wid] = pg;
+    lds[NW + wid] = pd;
+  }
   __builtin_amdgcn_s_waitcnt(0);  // this wave's row-value stores (and loads) complete
-  __syncthreads();
+  __syncthreads();                // S1
   const int cur = it % 3, nxt = (it + 1) % 3;
   if (threadIdx.x == 0) {
-    unsigned long long *rst = slots + 2 * ((size_t)nxt * nb + blockIdx.x);
+    double bg = 0.0, bd = 0.0;
+    for (int w = 0; w < NW; w++) bg += lds[w];
+    for (int w = 0; w < NW; w++) bd += lds[NW + w];
+    unsigned long long *rst = slots + 2 * ((size_t)nxt * nb + b);
     __hip_atomic_store(rst, PCG_SLOT_SENTINEL, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     __hip_atomic_store(rst + 1, PCG_SLOT_SENTINEL, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
-    unsigned long long *out = slots + 2 * ((size_t)cur * nb + blockIdx.x);
+    unsigned long long *out = slots + 2 * ((size_t)cur * nb + b);
     __hip_atomic_store(out, slot_bits(bg), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     __hip_atomic_store(out + 1, slot_bits(bd), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
   }
   double vg = 0.0, vd = 0.0;
   if ((int)threadIdx.x < nb) {
-    for (int q = threadIdx.x; q < nb; q += blockDim.x) {
-      const unsigned long long *src = slots + 2 * ((size_t)cur * nb + q);
-      unsigned long long x, y;
-      unsigned spins = 0;
-      for (;;) {
-        x = __hip_atomic_load(src, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        y = __hip_atomic_load(src + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        if (x != PCG_SLOT_SENTINEL && y != PCG_SLOT_SENTINEL) break;
-        if ((++spins & 255u) == 0) {
-          if (__hip_atomic_load(abort, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0) break;
-          if (spins >= PCG_SLOT_SPIN_LIMIT) {
-            __hip_atomic_store(abort, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-            break;
-          }
+    const unsigned long long *src = slots + 2 * ((size_t)cur * nb + threadIdx.x);
+    unsigned long long x, y;
+    unsigned spins = 0;
+    for (;;) {
+      x = __hip_atomic_load(src, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      y = __hip_atomic_load(src + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      if (x != PCG_SLOT_SENTINEL && y != PCG_SLOT_SENTINEL) break;
+      if ((++spins & 255u) == 0) {
+        if (__hip_atomic_load(abort, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0) break;
+        if (spins >= PCG_SLOT_SPIN_LIMIT) {
+          __hip_atomic_store(abort, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+          break;
         }
-        __builtin_amdgcn_s_sleep(1);
       }
-      vg += __longlong_as_double(x);
-      vd += __longlong_as_double(y);
+      __builtin_amdgcn_s_sleep(1);
     }
+    vg = 0.0 + __longlong_as_double(x);
+    vd = 0.0 + __longlong_as_double(y);
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
   }
-  gamma = block_total_bcast(vg, lds);
-  delta = block_total_bcast(vd, lds + 24);
+  vg = wave_sum(vg);
+  vd = wave_sum(vd);
+  if (lane == 0) {
+    lds[2 * NW + wid] = vg;
+    lds[3 * NW + wid] = vd;
+  }
+  __syncthreads();  // S2
+  if (threadIdx.x == 0) {
+    double tg = 0.0, td = 0.0;
+    for (int w = 0; w < NW; w++) tg += lds[2 * NW + w];
+    for (int w = 0; w < NW; w++) td += lds[3 * NW + w];
+    lds[4 * NW] = tg;
+    lds[4 * NW + 1] = td;
+  }
+  __syncthreads();  // S3
+  gamma = lds[4 * NW];
+  delta = lds[4 * NW + 1];
 }
 
+#define CG1_LDS(BS) (4 * ((BS) / 64) + 2 > 48 ? 4 * ((BS) / 64) + 2 : 48)
+
+// Block b of nb of one system: the body shared by the single-system kernel and the batched
+// kernel (where b and nb are member-local).
 template <int R, int BS, bool SLOTS>
-__global__ void __launch_bounds__(BS) k_pcg_cg1(PcgArgs a) {
-  __shared__ double lds[48];
-  __shared__ unsigned gen0;
-  if (threadIdx.x == 0) gen0 = __hip_atomic_load(&a.bar[1], __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT);
+__device__ __forceinline__ void cg1_body(const PcgArgs &a, const int b, const int nb, double *lds,
+                                         unsigned *gen0) {
+  if (threadIdx.x == 0) *gen0 = __hip_atomic_load(&a.bar[1], __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT);
   __syncthreads();
-  unsigned gen = gen0;
-  const int n = a.n, nb = gridDim.x, bs = blockDim.x;
-  const int b = blockIdx.x;
+  unsigned gen = *gen0;
+  const int n = a.n, bs = BS;
   const int ng = nb < PCG_XCD ? nb : PCG_XCD;
   const int g = b % ng, slot = b / ng;
   int chunk = 0;
@@ -597,7 +626,7 @@ __global__ void __launch_bounds__(BS) k_pcg_cg1(PcgArgs a) {
     rr[k] = a.b[i] - sum;
     pub[i] = rr[k];
   }
-  bar2_sync(a.bar, nb, gen);
+  bar2_sync(a.bar, nb, gen, b);
   // u0 = M^-1 r0 ; w0 = A u0 ; gamma0 = r0.u0 ; delta0 = w0.u0
   double pg = 0.0, pd = 0.0;
 #pragma unroll
@@ -628,7 +657,7 @@ __global__ void __launch_bounds__(BS) k_pcg_cg1(PcgArgs a) {
     a.partA[b] = bg;
     a.partB[b] = bd;
   }
-  bar2_sync(a.bar, nb, gen);
+  bar2_sync(a.bar, nb, gen, b);
   double gamma = grid_total_bs(a.partA, nb, lds);
   double delta = grid_total_bs(a.partB, nb, lds + 24);
   const double gamma_0 = gamma;
@@ -667,19 +696,19 @@ __global__ void __launch_bounds__(BS) k_pcg_cg1(PcgArgs a) {
       pub[oout + n + rid[k]] = ww[k];
       pub[oout + 2 * (size_t)n + rid[k]] = ss[k];
     }
-    bg = block_sum_bs(pg, lds);
-    bd = block_sum_bs(pd, lds + 16);
     double gamma_new, delta_new;
     if (SLOTS) {
-      slot_exchange(a.slots, nb, cgit, bg, bd, lds, a.info + 2, gamma_new, delta_new);
+      cg1_exchange<BS>(a.slots, b, nb, cgit, pg, pd, lds, a.info + 2, gamma_new, delta_new);
     } else {
+      bg = block_sum_bs(pg, lds);
+      bd = block_sum_bs(pd, lds + 16);
       // alternate partial buffers so a slow block still reading the previous totals is safe
       double *PA = (cgit & 1) ? a.partC : a.partA, *PB = (cgit & 1) ? a.partD : a.partB;
       if (threadIdx.x == 0) {
         PA[b] = bg;
         PB[b] = bd;
       }
-      bar2_sync(a.bar, nb, gen);
+      bar2_sync(a.bar, nb, gen, b);
       gamma_new = grid_total_bs(PA, nb, lds);
       delta_new = grid_total_bs(PB, nb, lds + 24);
     }
@@ -700,6 +729,53 @@ __global__ void __launch_bounds__(BS) k_pcg_cg1(PcgArgs a) {
     a.info[0] = cgit2;
     a.info[1] = cgit2 > 0 ? 1 : 0;
   }
+}
+
+template <int R, int BS, bool SLOTS>
+__global__ void __launch_bounds__(BS) k_pcg_cg1(PcgArgs a) {
+  __shared__ double lds[CG1_LDS(BS)];
+  __shared__ unsigned gen0;
+  cg1_body<R, BS, SLOTS>(a, blockIdx.x, gridDim.x, lds, &gen0);
+}
+
+// Several independent systems in one launch: member m owns blocks base[m] .. base[m] +
+// nbm[m] - 1 (base[m] a multiple of 8, so the member-local block index keeps the XCD
+// grouping); blocks in the padding between members return at once. Each member runs the
+// slot-exchange solve among its own blocks with its own work arrays, exactly as it would
+// alone with the same block count.
+template <int R, int BS>
+__global__ void __launch_bounds__(BS) k_pcg_cg1_batch(const PcgArgs *members, const int *base, const int *nbm,
+                                                      int nmem) {
+  __shared__ double lds[CG1_LDS(BS)];
+  __shared__ unsigned gen0;
+  __shared__ int mm;
+  if (threadIdx.x == 0) {
+    int m = 0;
+    while (m + 1 < nmem && base[m + 1] <= (int)blockIdx.x) m++;
+    mm = m;
+  }
+  __syncthreads();
+  const int m = mm;
+  const int b = (int)blockIdx.x - base[m];
+  const int nb = nbm[m];
+  if (b >= nb) return;
+  const PcgArgs a = members[m];
+  cg1_body<R, BS, true>(a, b, nb, lds, &gen0);
+}
+
+extern "C" hipError_t gom_launch_pcg_cg1_batch(const PcgArgs *d_members, const int *d_base, const int *d_nbm,
+                                               int nmem, int total_blocks, hipStream_t st) {
+  void *kargs[] = {&d_members, &d_base, &d_nbm, &nmem};
+  return hipLaunchKernel((const void *)k_pcg_cg1_batch<4, 512>, dim3(total_blocks), dim3(512), kargs, 0, st);
+}
+
+extern "C" int gom_pcg_cg1_batch_capacity(int device) {
+  int per_cu = 0, ncu = 0;
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, (const void *)k_pcg_cg1_batch<4, 512>, 512, 0) !=
+      hipSuccess)
+    return 0;
+  if (hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, device) != hipSuccess) return 0;
+  return per_cu * ncu;
 }
 
 template <bool SLOTS>

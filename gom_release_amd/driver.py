@@ -83,10 +83,14 @@ class Scenario:
         self.model.set_forcing(self.forcing)   # static fields once
         self.it = 0
 
-    def step(self) -> None:
+    def stage_forcing(self) -> None:
+        """Host forcing of the next step, uploaded asynchronously (before a batch step)."""
         self.it += 1
         self.host_forcing.advance(self.it)
         self.model.set_forcing(self.forcing, only=DYNAMIC)
+
+    def step(self) -> None:
+        self.stage_forcing()
         self.model.step()
 
     def run(self, n: int) -> None:

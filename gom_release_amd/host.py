@@ -53,6 +53,15 @@ def lib():
         L.gomgpu_timings.argtypes = [vp, C.c_int, C.POINTER(C.c_char_p), C.POINTER(C.c_double), C.c_int]
         L.gomgpu_destroy.argtypes = [vp]
         L.gomgpu_destroy.restype = None
+        L.gomgpu_batch_create.argtypes = [C.POINTER(vp)]
+        for fn in ("gomgpu_batch_add", "gomgpu_batch_remove"):
+            getattr(L, fn).argtypes = [vp, vp]
+        for fn in ("gomgpu_batch_size", "gomgpu_batch_step", "gomgpu_batch_sync"):
+            getattr(L, fn).argtypes = [vp]
+        L.gomgpu_batch_last_error.argtypes = [vp]
+        L.gomgpu_batch_last_error.restype = C.c_char_p
+        L.gomgpu_batch_destroy.argtypes = [vp]
+        L.gomgpu_batch_destroy.restype = None
         _lib = L
     return _lib
 
@@ -60,7 +69,9 @@ def lib():
 EXPORTED = ["gomgpu_create", "gomgpu_set_state", "gomgpu_set_forcing", "gomgpu_stage",
             "gomgpu_step", "gomgpu_run", "gomgpu_get_state", "gomgpu_budgets",
             "gomgpu_pcg_iterations", "gomgpu_sync", "gomgpu_last_error", "gomgpu_timings",
-            "gomgpu_destroy"]
+            "gomgpu_destroy", "gomgpu_batch_create", "gomgpu_batch_add", "gomgpu_batch_remove",
+            "gomgpu_batch_size", "gomgpu_batch_step", "gomgpu_batch_sync", "gomgpu_batch_last_error",
+            "gomgpu_batch_destroy"]
 
 
 def device_count() -> int:
@@ -144,3 +155,48 @@ class GpuModel:
     def _check(self, rc: int) -> None:
         if rc != 0:
             raise GomGpuError(rc, self.last_error())
+
+
+class GpuBatch:
+    """Several device-resident models advanced together (gomgpu_batch_*)."""
+
+    def __init__(self):
+        self._L = lib()
+        self._b = C.c_void_p()
+        rc = self._L.gomgpu_batch_create(C.byref(self._b))
+        if rc != 0:
+            raise GomGpuError(rc, "gomgpu_batch_create failed")
+        self.members = []
+
+    def _check(self, rc: int) -> None:
+        if rc != 0:
+            raise GomGpuError(rc, self._L.gomgpu_batch_last_error(self._b).decode())
+
+    def add(self, model: "GpuModel") -> None:
+        self._check(self._L.gomgpu_batch_add(self._b, model._ctx))
+        self.members.append(model)
+
+    def remove(self, model: "GpuModel") -> None:
+        self._check(self._L.gomgpu_batch_remove(self._b, model._ctx))
+        self.members.remove(model)
+
+    def __len__(self) -> int:
+        return self._L.gomgpu_batch_size(self._b)
+
+    def step(self) -> None:
+        self._check(self._L.gomgpu_batch_step(self._b))
+
+    def sync(self) -> None:
+        self._check(self._L.gomgpu_batch_sync(self._b))
+
+    def close(self) -> None:
+        if self._b:
+            self._L.gomgpu_batch_destroy(self._b)
+            self._b = C.c_void_p()
+            self.members = []
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

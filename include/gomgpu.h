@@ -250,6 +250,28 @@ const char *gomgpu_last_error(gomgpu_ctx *ctx);
 int gomgpu_timings(gomgpu_ctx *ctx, int enable, const char **names, double *ms, int n);
 void gomgpu_destroy(gomgpu_ctx *ctx);
 
+/* ---------------------------------------------------------------------------------------
+ * Batches (ensembles and the serving loop; no counterpart in the reference, which runs one
+ * scenario per process). A batch advances several independent contexts of one device by
+ * one step in one submission: each member's stages run on its own stream, the PCG solves of
+ * all members run in fused launches, and the submission is captured as a hipGraph once per
+ * membership and replayed (GOMGPU_BATCH_GRAPH=0 submits eagerly instead). A member's result
+ * is bit for bit the same alone and in any batch. Forcing is uploaded per member with
+ * gomgpu_set_forcing before gomgpu_batch_step; gomgpu_get_state on a member is ordered after
+ * the batch steps submitted before it.
+ * ------------------------------------------------------------------------------------- */
+typedef struct gomgpu_batch gomgpu_batch;
+int gomgpu_batch_create(gomgpu_batch **out);
+int gomgpu_batch_add(gomgpu_batch *batch, gomgpu_ctx *member);
+int gomgpu_batch_remove(gomgpu_batch *batch, gomgpu_ctx *member);
+int gomgpu_batch_size(gomgpu_batch *batch);
+/* One step of every member (asynchronous). */
+int gomgpu_batch_step(gomgpu_batch *batch);
+/* Wait for the batch; returns the first member error (gomgpu_sync semantics). */
+int gomgpu_batch_sync(gomgpu_batch *batch);
+const char *gomgpu_batch_last_error(gomgpu_batch *batch);
+void gomgpu_batch_destroy(gomgpu_batch *batch);
+
 #ifdef __cplusplus
 }
 #endif
