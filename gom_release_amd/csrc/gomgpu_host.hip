@@ -86,6 +86,7 @@ struct gomgpu_ctx {
   DevWork w;
   double elapsed = 0.0, spin_tide = 1.0, spin_bc = 1.0;
   std::vector<void *> allocs;
+  size_t bytes = 0;  // device memory held by the context
   int pcg_blocks = 0;      // co-resident capacity of the PCG kernel
   int pcg_cap = 0;         // tuned cap of the PCG grid (0 = capacity)
   int pcg_barrier = 1;     // 0 cooperative-groups sync, 1 custom barrier
@@ -173,8 +174,11 @@ static int dalloc(gomgpu_ctx *c, T **ptr, size_t count) {
   void *p = nullptr;
   if (count == 0) count = 1;
   if (hipMalloc(&p, count * sizeof(T)) != hipSuccess) return seterr(c, GOMGPU_E_NOMEM, "hipMalloc failed");
-  if (hipMemset(p, 0, count * sizeof(T)) != hipSuccess) return seterr(c, GOMGPU_E_HIP, "hipMemset failed");
+  // zeroed on the context's own stream: the stream is non-blocking, so a fill on the null
+  // stream would not be ordered before the uploads and kernels that follow on it
+  if (hipMemsetAsync(p, 0, count * sizeof(T), c->st) != hipSuccess) return seterr(c, GOMGPU_E_HIP, "hipMemsetAsync failed");
   c->allocs.push_back(p);
+  c->bytes += count * sizeof(T);
   *ptr = (T *)p;
   return 0;
 }
@@ -311,7 +315,8 @@ extern "C" int gomgpu_create(const gomgpu_mesh *mesh, const gomgpu_params *param
     double *dp;
     const double init[4] = {1.0, 1.0, 0.0, 0.0};
     if ((rc = dalloc(c, &dp, 4))) goto fail;
-    if (hipMemcpy(dp, init, sizeof init, hipMemcpyHostToDevice) != hipSuccess) {
+    if (hipMemcpyAsync(dp, init, sizeof init, hipMemcpyHostToDevice, c->st) != hipSuccess ||
+        hipStreamSynchronize(c->st) != hipSuccess) {
       rc = seterr(c, GOMGPU_E_HIP, "hipMemcpy failed");
       goto fail;
     }
@@ -388,6 +393,10 @@ extern "C" int gomgpu_create(const gomgpu_mesh *mesh, const gomgpu_params *param
       rc = seterr(c, GOMGPU_E_NOMEM, "cannot allocate the forcing staging buffer");
       goto fail;
     }
+  }
+  if (hipStreamSynchronize(c->st) != hipSuccess) {
+    rc = seterr(c, GOMGPU_E_HIP, "initialisation failed");
+    goto fail;
   }
   *out = c;
   return 0;
@@ -953,9 +962,10 @@ static int batch_plan(gomgpu_batch *b) {
     BHIP(b, hipMalloc(&g.d_args, sizeof(PcgArgs) * args.size()));
     BHIP(b, hipMalloc(&g.d_base, sizeof(int) * base.size()));
     BHIP(b, hipMalloc(&g.d_nbm, sizeof(int) * nbm.size()));
-    BHIP(b, hipMemcpy(g.d_args, args.data(), sizeof(PcgArgs) * args.size(), hipMemcpyHostToDevice));
-    BHIP(b, hipMemcpy(g.d_base, base.data(), sizeof(int) * base.size(), hipMemcpyHostToDevice));
-    BHIP(b, hipMemcpy(g.d_nbm, nbm.data(), sizeof(int) * nbm.size(), hipMemcpyHostToDevice));
+    BHIP(b, hipMemcpyAsync(g.d_args, args.data(), sizeof(PcgArgs) * args.size(), hipMemcpyHostToDevice, b->st));
+    BHIP(b, hipMemcpyAsync(g.d_base, base.data(), sizeof(int) * base.size(), hipMemcpyHostToDevice, b->st));
+    BHIP(b, hipMemcpyAsync(g.d_nbm, nbm.data(), sizeof(int) * nbm.size(), hipMemcpyHostToDevice, b->st));
+    BHIP(b, hipStreamSynchronize(b->st));
     b->groups.push_back(g);
   }
   return 0;
@@ -1115,3 +1125,65 @@ extern "C" void gomgpu_batch_destroy(gomgpu_batch *b) {
   if (b->st) hipStreamDestroy(b->st);
   delete b;
 }
+
+// ------------------------------------------------------------------ footprint
+// Device bytes gomgpu_create allocates for these parameters (the same list as create, one
+// line per allocation; checked against gomgpu_device_bytes by the tests).
+extern "C" long long gomgpu_footprint(const gomgpu_params *params) {
+  if (!params) return -1;
+  std::string why;
+  if (validate(params, why)) return -1;
+  const gomgpu_params &p = *params;
+  int device = 0;
+  if (hipGetDevice(&device) != hipSuccess) return -1;
+  size_t tot = 0;
+  auto add = [&](size_t count, size_t elem) { tot += (count == 0 ? 1 : count) * elem; };
+#define FP_R(name, ext)                     \
+  {                                         \
+    Shape sh = shape_of(#ext, p);           \
+    add((size_t)(sh.ld * sh.n), sizeof(double)); \
+  }
+#define FP_I(name, ext)                     \
+  {                                         \
+    Shape sh = shape_of(#ext, p);           \
+    add((size_t)(sh.ld * sh.n), sizeof(int32_t)); \
+  }
+  GOMGPU_MESH_REAL(FP_R)
+  GOMGPU_MESH_INT(FP_I)
+  add(p.maxface, sizeof(double));  // Cdb_static
+  GOMGPU_STATE_REAL(FP_R)
+  GOMGPU_STATE_INT(FP_I)
+  GOMGPU_FORCING_REAL(FP_R)
+#undef FP_R
+#undef FP_I
+  add(4, sizeof(double));  // per-step scalars
+  long E = p.maxele, J = p.maxface, P = p.maxnod, L = p.maxlayer;
+  long NT = (p.transport_flag == 1 ? p.maxtran2 : 0);
+  NT = NT > 0 ? NT : 1;
+  long OB = p.num_ob_cell > 0 ? p.num_ob_cell : 1, QB = p.num_Qb_cell > 0 ? p.num_Qb_cell : 1;
+  long pb = gom_pcg_max_blocks(device);
+  const size_t D = sizeof(double);
+  add(E, D); add(4 * E, D); add(E, D);                      // fs_diag, fs_off, fs_rhs
+  for (int i = 0; i < 7; i++) add(E, D);                    // pcg_x .. pcg_minv
+  add(4 * (pb > 4096 ? pb : 4096), D);                      // pcg_part
+  add(6 * E, D); add(6 * PCG_MAX_SLOT_BLOCKS, sizeof(unsigned long long));
+  add(3, sizeof(int)); add(512, sizeof(unsigned));
+  add(J, D); add(J, D); add(J, D);                          // face_sdz, face_sun, face_sg
+  add(NT * (L + 2) * E, D); add(NT * L * E, D);             // con, con_new
+  add(L * J, D); add(L * J, D); add(L * J, D);              // Qj, dj, phij
+  add((L + 1) * E, D); add((L + 1) * E, D); add((L + 1) * E, D);  // Qi, di, phii
+  add(NT * L * J, D); add(NT * (L + 1) * E, D);             // rj, ri
+  add(NT * L * J, D); add(NT * (L + 1) * E, D);             // PSIj, PSIi
+  add(NT * L * E, D); add(NT * L * E, D);                   // s1a, s2a
+  add(NT * L * OB, D); add(NT * L * QB, D);                 // obk, qbk
+  add(NT * L * P, D); add(NT * L * J, D);                   // node_new, face_new
+  add(2 * ((E + 255) / 256) + 2, D); add(2, sizeof(int));   // budget, err
+  add((L + 1) * J, D); add(L * (E > J ? E : J), D); add(NT * L * E, D);  // turb_kvf, tri_gam, tvd_x
+  if (p.turbulence_flag == 3) {
+    for (int i = 0; i < 4; i++) add((L + 1) * J, D);
+    for (int i = 0; i < 6; i++) add(L * J, D);
+  }
+  return (long long)tot;
+}
+
+extern "C" long long gomgpu_device_bytes(gomgpu_ctx *c) { return c ? (long long)c->bytes : -1; }

@@ -67,12 +67,34 @@ class HostForcing:
         return f
 
 
+def pack_prepared(case: Case, precision: int = 0):
+    """prep.build in a picklable form (for preparing scenarios in worker processes)."""
+    p, m, st = prep.build(case, precision)
+    return abi.params_to_dict(p), dict(m.arr), dict(st.arr)
+
+
+def unpack_prepared(packed):
+    pd, marr, sarr = packed
+    p = abi.params_from_dict(pd)
+    m = abi.mesh_arrays(p)
+    s = abi.state_arrays(p)
+    for k, v in marr.items():
+        m.arr[k] = v
+    for k, v in sarr.items():
+        s.arr[k] = v
+    return p, m, s
+
+
 class Scenario:
     """One scenario advanced on the device."""
 
-    def __init__(self, case: Case, precision: int = 0, wind=(0.0, 0.0), model: Optional[host.GpuModel] = None):
+    def __init__(self, case: Case, precision: int = 0, wind=(0.0, 0.0), model: Optional[host.GpuModel] = None,
+                 prepared=None):
         self.case = case
-        self.params, self.mesh, state0 = prep.build(case, precision)
+        if prepared is None:
+            self.params, self.mesh, state0 = prep.build(case, precision)
+        else:
+            self.params, self.mesh, state0 = unpack_prepared(prepared)
         if case.rivers:
             self.params.river_on_device = 1
         self.model = model or host.GpuModel(self.mesh, self.params)

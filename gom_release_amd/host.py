@@ -53,6 +53,10 @@ def lib():
         L.gomgpu_timings.argtypes = [vp, C.c_int, C.POINTER(C.c_char_p), C.POINTER(C.c_double), C.c_int]
         L.gomgpu_destroy.argtypes = [vp]
         L.gomgpu_destroy.restype = None
+        L.gomgpu_footprint.argtypes = [C.POINTER(abi.Params)]
+        L.gomgpu_footprint.restype = C.c_longlong
+        L.gomgpu_device_bytes.argtypes = [vp]
+        L.gomgpu_device_bytes.restype = C.c_longlong
         L.gomgpu_batch_create.argtypes = [C.POINTER(vp)]
         for fn in ("gomgpu_batch_add", "gomgpu_batch_remove"):
             getattr(L, fn).argtypes = [vp, vp]
@@ -69,9 +73,17 @@ def lib():
 EXPORTED = ["gomgpu_create", "gomgpu_set_state", "gomgpu_set_forcing", "gomgpu_stage",
             "gomgpu_step", "gomgpu_run", "gomgpu_get_state", "gomgpu_budgets",
             "gomgpu_pcg_iterations", "gomgpu_sync", "gomgpu_last_error", "gomgpu_timings",
-            "gomgpu_destroy", "gomgpu_batch_create", "gomgpu_batch_add", "gomgpu_batch_remove",
+            "gomgpu_destroy", "gomgpu_footprint", "gomgpu_device_bytes", "gomgpu_batch_create", "gomgpu_batch_add", "gomgpu_batch_remove",
             "gomgpu_batch_size", "gomgpu_batch_step", "gomgpu_batch_sync", "gomgpu_batch_last_error",
             "gomgpu_batch_destroy"]
+
+
+def footprint(params: "abi.Params") -> int:
+    """Device bytes a context for these parameters allocates."""
+    n = lib().gomgpu_footprint(C.byref(params))
+    if n < 0:
+        raise GomGpuError(1, "gomgpu_footprint: invalid parameters or no device")
+    return int(n)
 
 
 def device_count() -> int:
@@ -140,6 +152,9 @@ class GpuModel:
 
     def last_error(self) -> str:
         return self._L.gomgpu_last_error(self._ctx).decode()
+
+    def device_bytes(self) -> int:
+        return int(self._L.gomgpu_device_bytes(self._ctx))
 
     def close(self) -> None:
         if self._ctx:

@@ -260,6 +260,11 @@ void gomgpu_destroy(gomgpu_ctx *ctx);
  * gomgpu_set_forcing before gomgpu_batch_step; gomgpu_get_state on a member is ordered after
  * the batch steps submitted before it.
  * ------------------------------------------------------------------------------------- */
+/* Device bytes gomgpu_create would allocate for these parameters (admission by HBM
+ * footprint), and the bytes a context holds; -1 on bad arguments. */
+long long gomgpu_footprint(const gomgpu_params *params);
+long long gomgpu_device_bytes(gomgpu_ctx *ctx);
+
 typedef struct gomgpu_batch gomgpu_batch;
 int gomgpu_batch_create(gomgpu_batch **out);
 int gomgpu_batch_add(gomgpu_batch *batch, gomgpu_ctx *member);
