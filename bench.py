@@ -23,6 +23,8 @@ import os
 import sys
 import time
 
+import numpy as np
+
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
@@ -115,6 +117,81 @@ def copy_calibration(nbytes: int) -> None:
         hip.hipFree(dst)
 
 
+def _log_uniform(rng, lo, hi, n):
+    return [float(x) for x in np.exp(rng.uniform(np.log(lo), np.log(hi), n))]
+
+
+def serving_sample(n_req: int = 12, rate: float = 2.0, steps: int = 30, seed: int = 5):
+    """Bounded sample of the serving workload: Poisson arrivals at `rate`, scenarios of
+    10k-200k cells and 10-30 layers, `steps` steps each, admission by device footprint
+    against a 64 GiB budget, continuous batching."""
+    from gom_release_amd import serving
+    rng = np.random.default_rng(seed)
+    sizes = _log_uniform(rng, 10e3, 200e3, n_req)
+    layers = rng.integers(10, 31, n_req)
+    arrivals = serving.poisson_trace(n_req, rate, seed=seed)
+    reqs = [serving.Request(i, serving.estuary_of_size(sizes[i], int(layers[i]), seed=100 + i, dt=60.0,
+                                                       river_q=200.0, tide_phase=float(rng.uniform(-180, 180)),
+                                                       turbulence_flag=3),
+                            steps, arrivals[i], wind=(8.0, 2.0)) for i in range(n_req)]
+    results, summary = serving.ServingLoop(64 << 30, workers=4).run(reqs)
+    summary.update({"lambda_per_s": rate, "steps_per_request": steps,
+                    "cells_min_max": [min(r.cells for r in results), max(r.cells for r in results)],
+                    "sample": f"{n_req} requests of 10k-200k cells x 10-30 layers, {steps} steps each "
+                              f"(the 24 h scenarios shortened), GLS closure, 64 GiB budget"})
+    return summary
+
+
+def ensemble_sample(n_mem: int = 8, steps: int = 10, seed: int = 7):
+    """Members of 20k-200k cells and 10-30 layers with their own tidal phase: stepped
+    together in one batch (fused PCG, captured graph) against one after another."""
+    from gom_release_amd import driver, host, serving
+    rng = np.random.default_rng(seed)
+    sizes = _log_uniform(rng, 20e3, 200e3, n_mem)
+    layers = rng.integers(10, 31, n_mem)
+    cases = [serving.estuary_of_size(sizes[i], int(layers[i]), seed=200 + i, dt=60.0, river_q=200.0,
+                                     tide_phase=float(rng.uniform(-180, 180)), turbulence_flag=3)
+             for i in range(n_mem)]
+    scs = [driver.Scenario(c, wind=(8.0, 2.0)) for c in cases]
+    # serial: each member alone
+    for sc in scs:
+        sc.run(2)
+        sc.model.sync()
+    t0 = time.perf_counter()
+    work = 0
+    for sc in scs:
+        a = sc.active_cell_layers()
+        sc.run(steps)
+        sc.model.sync()
+        work += a * steps
+    t_serial = time.perf_counter() - t0
+    b = host.GpuBatch()
+    for sc in scs:
+        b.add(sc.model)
+    for _ in range(2):
+        for sc in scs:
+            sc.stage_forcing()
+        b.step()
+    b.sync()
+    t0 = time.perf_counter()
+    workb = 0
+    for sc in scs:
+        workb += sc.active_cell_layers() * steps
+    for _ in range(steps):
+        for sc in scs:
+            sc.stage_forcing()
+        b.step()
+    b.sync()
+    t_batch = time.perf_counter() - t0
+    b.close()
+    for sc in scs:
+        sc.model.close()
+    return {"members": n_mem, "steps": steps, "cells": [int(c.maxele) for c in cases],
+            "layers": [int(x) for x in layers],
+            "serial_cell_layer_steps_per_s": work / t_serial, "batched_cell_layer_steps_per_s": workb / t_batch,
+            "speedup": (workb / t_batch) / (work / t_serial)}
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -126,6 +203,8 @@ def main() -> int:
                     help="vertical closure of the workload (gls: k-kl GLS, Galperin stability functions)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--profile-steps", type=int, default=5)
+    ap.add_argument("--no-serving", action="store_true", help="skip the serving-loop sample")
+    ap.add_argument("--no-ensemble", action="store_true", help="skip the batched-ensemble sample")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="also run one device-to-device copy of MIB MiB (calibrates rocprofv3 "
                          "FETCH_SIZE/WRITE_SIZE against known traffic)")
@@ -211,6 +290,15 @@ def main() -> int:
                    "volume_m3": vol, "salt_psu_m3": salt},
         "roofline": roof,
     }
+    model.close()
+    if not args.no_ensemble:
+        print("[bench] batched ensemble sample", file=sys.stderr, flush=True)
+        out["ensemble"] = ensemble_sample()
+    if not args.no_serving:
+        print("[bench] serving-loop sample", file=sys.stderr, flush=True)
+        out["serving"] = serving_sample()
+        out["config"]["p50_latency_s"] = out["serving"]["p50_latency_s"]
+        out["config"]["p99_latency_s"] = out["serving"]["p99_latency_s"]
     if not args.no_cpu_baseline:
         print("[bench] timing the CPU restatement", file=sys.stderr, flush=True)
         out["cpu_baseline"] = cpu_baseline(case, wind)

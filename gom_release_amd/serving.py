@@ -91,12 +91,15 @@ class ServingLoop:
         futures = {}
         used = 0
         nxt = 0
-        t0 = time.perf_counter()
-        now = lambda: time.perf_counter() - t0   # noqa: E731
         # fresh interpreters (never a fork of this process, which holds the GPU); the
         # workers do host work only
         ctx = multiprocessing.get_context("spawn")
         with ProcessPoolExecutor(max_workers=self.workers, mp_context=ctx) as pool:
+            # the server is up before the first request: start and warm every worker
+            for f in [pool.submit(_warm) for _ in range(self.workers)]:
+                f.result()
+            t0 = time.perf_counter()
+            now = lambda: time.perf_counter() - t0   # noqa: E731
             while len(results) < len(reqs):
                 t = now()
                 # arrivals go to the preparation pool
@@ -156,6 +159,12 @@ class ServingLoop:
                    "p99_latency_s": float(np.percentile(lat, 99)), "makespan_s": makespan,
                    "cell_layer_steps_per_s": work / makespan if makespan > 0 else 0.0}
         return sorted(results, key=lambda r: r.rid), summary
+
+
+def _warm() -> int:
+    """Imports the preparation code in a worker (run once per worker before serving)."""
+    from . import driver, meshgen, prep  # noqa: F401
+    return 0
 
 
 def poisson_trace(n: int, rate: float, seed: int = 0) -> List[float]:
