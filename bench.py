@@ -203,6 +203,7 @@ def main() -> int:
                     help="vertical closure of the workload (gls: k-kl GLS, Galperin stability functions)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--profile-steps", type=int, default=5)
+    ap.add_argument("--no-autotune", action="store_true", help="keep the default launch shapes")
     ap.add_argument("--no-serving", action="store_true", help="skip the serving-loop sample")
     ap.add_argument("--no-ensemble", action="store_true", help="skip the batched-ensemble sample")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
@@ -227,6 +228,13 @@ def main() -> int:
     print(f"[bench] building {case.maxele}-cell x {case.maxlayer}-layer case", file=sys.stderr, flush=True)
     sc = Scenario(case, precision=prec, wind=wind)
     model = sc.model
+    launch = None
+    if not args.no_autotune:
+        from gom_release_amd import autotune
+        t_tune = time.perf_counter()
+        launch = autotune.tuned(sc, log=lambda m: print(m, file=sys.stderr, flush=True))
+        print(f"[bench] launch shapes tuned in {time.perf_counter() - t_tune:.1f} s: {launch}",
+              file=sys.stderr, flush=True)
     sc.run(args.warmup)
     model.sync()
     print(f"[bench] warmup done ({args.warmup} steps)", file=sys.stderr, flush=True)
@@ -289,6 +297,7 @@ def main() -> int:
                    "pcg_tolerance": case.error_tolerance, "precision": args.precision,
                    "volume_m3": vol, "salt_psu_m3": salt},
         "roofline": roof,
+        "launch": launch,
     }
     model.close()
     if not args.no_ensemble:

@@ -50,7 +50,7 @@ __global__ void k_eos_cell(DevMesh, DevState, DevDims, int *);
 __global__ void k_bars(DevMesh, DevState, DevDims);
 __global__ void k_turb_ri(DevMesh, DevState, DevWork, DevDims, TurbParams);
 __global__ void k_turb_prep(DevMesh, DevState, DevWork, DevDims, TurbParams, int *);
-__global__ void k_turb_solve(DevMesh, DevState, DevForcing, DevWork, DevDims, TurbParams, int *);
+__global__ void __launch_bounds__(256) k_turb_solve(DevMesh, DevState, DevForcing, DevWork, DevDims, TurbParams, int *);
 __global__ void k_turb_post(DevMesh, DevState, DevWork, DevDims, TurbParams);
 __global__ void k_turb_kv(DevMesh, DevState, DevWork, DevDims);
 __global__ void k_budget(DevMesh, DevState, DevDims, double *);
@@ -87,6 +87,8 @@ struct gomgpu_ctx {
   double elapsed = 0.0, spin_tide = 1.0, spin_bc = 1.0;
   std::vector<void *> allocs;
   size_t bytes = 0;  // device memory held by the context
+  // threads per block of the tunable launches (GOMGPU_LAUNCH_*); none changes results
+  int lb[GOMGPU_NUM_LAUNCH] = {128, 256, 128, 256, 128, 128, 256, 256, 128, 256};
   int pcg_blocks = 0;      // co-resident capacity of the PCG kernel
   int pcg_cap = 0;         // tuned cap of the PCG grid (0 = capacity)
   int pcg_barrier = 1;     // 0 cooperative-groups sync, 1 custom barrier
@@ -589,10 +591,12 @@ static int fs_post(gomgpu_ctx *c) {
   return 0;
 }
 
+#define LB(k) (c->lb[GOMGPU_LAUNCH_##k])
+
 static int stage_impl(gomgpu_ctx *c, int id) {
   const gomgpu_params &p = c->p;
   const DevDims &d = c->d;
-  const int BS = 256;
+  const int BS = LB(STREAM);
   hipStream_t st = c->st;
   int *err = c->w.err;
   switch (id) {
@@ -612,15 +616,15 @@ static int stage_impl(gomgpu_ctx *c, int id) {
           k_elm_substeps<<<nblk((long)d.P * d.L, BS), BS, 0, st>>>(c->m, c->s, d, p.dt, p.delta_z_min,
                                                                    p.ELM_min_iter, p.ELM_max_iter);
         KT_BEGIN(KT_ELM);
-        k_elm<<<nblk((long)d.J * d.L, 128), 128, 0, st>>>(c->m, c->s, d, p.MSL, p.dt, p.adv_onoff_depth, err);
+        k_elm<<<nblk((long)d.J * d.L, LB(ELM)), LB(ELM), 0, st>>>(c->m, c->s, d, p.MSL, p.dt, p.adv_onoff_depth, err);
         KT_END(KT_ELM);
       }
       break;
     case GOMGPU_STAGE_MOMENTUM: {
       MomParams q{p.dt, p.gravity, p.theta, p.rho_o, p.smagorinsky_parameter, p.Ah_0, p.Kh_0, p.baroclinic_flag};
       KT_BEGIN(KT_MOMENTUM);
-      k_mom_rhs<<<nblk((long)d.L * d.J, BS), BS, 0, st>>>(c->m, c->s, c->f, d, q);
-      k_mom_solve<<<nblk(d.J, 128), 128, 0, st>>>(c->m, c->s, c->f, c->w, d, q, err);
+      k_mom_rhs<<<nblk((long)d.L * d.J, LB(MOM_RHS)), LB(MOM_RHS), 0, st>>>(c->m, c->s, c->f, d, q);
+      k_mom_solve<<<nblk(d.J, LB(MOM_SOLVE)), LB(MOM_SOLVE), 0, st>>>(c->m, c->s, c->f, c->w, d, q, err);
       KT_END(KT_MOMENTUM);
       break;
     }
@@ -644,7 +648,7 @@ static int stage_impl(gomgpu_ctx *c, int id) {
       break;
     case GOMGPU_STAGE_NODEVEL:
       k_face_levels<<<nblk(d.J, BS), BS, 0, st>>>(c->m, c->s, d);
-      k_node_vel<<<nblk((long)d.P * (d.L + 1), BS), BS, 0, st>>>(c->m, c->s, d);
+      k_node_vel<<<nblk((long)d.P * (d.L + 1), LB(NODE_VEL)), LB(NODE_VEL), 0, st>>>(c->m, c->s, d);
       break;
     case GOMGPU_STAGE_TRANSPORT: {
       if (p.transport_flag != 1 || d.NT == 0) break;
@@ -668,13 +672,13 @@ static int stage_impl(gomgpu_ctx *c, int id) {
         k_tvd_coef_face<<<nblk((long)d.J * d.L, BS), BS, 0, st>>>(c->m, c->s, c->w, d, a);
         k_tvd_coef_cell<<<nblk((long)d.E * (d.L + 1), BS), BS, 0, st>>>(c->m, c->s, c->w, d, a);
         if (p.h_flux_limiter > 0 || p.v_flux_limiter > 0) {
-          k_tvd_lim_cell<<<nblk(d.E, 128), 128, 0, st>>>(c->m, c->s, c->w, d);
+          k_tvd_lim_cell<<<nblk(d.E, LB(TVD_LIM)), LB(TVD_LIM), 0, st>>>(c->m, c->s, c->w, d);
           k_tvd_lim_face<<<nblk((long)d.J * d.L, BS), BS, 0, st>>>(c->m, c->s, c->w, d, p.h_flux_limiter);
           k_tvd_psi_cell<<<nblk((long)d.E * (d.L + 1), BS), BS, 0, st>>>(c->m, c->s, c->w, d, p.h_flux_limiter);
         }
         KT_BEGIN(KT_TVD_COLUMN);
-        k_tvd_rhs<<<nblk((long)d.L * d.E, BS), BS, 0, st>>>(c->m, c->s, c->w, d, a);
-        k_tvd_solve<<<nblk(d.E, 128), 128, 0, st>>>(c->m, c->s, c->w, d, a, err);
+        k_tvd_rhs<<<nblk((long)d.L * d.E, LB(TVD_RHS)), LB(TVD_RHS), 0, st>>>(c->m, c->s, c->w, d, a);
+        k_tvd_solve<<<nblk(d.E, LB(TVD_SOLVE)), LB(TVD_SOLVE), 0, st>>>(c->m, c->s, c->w, d, a, err);
         KT_END(KT_TVD_COLUMN);
         k_tvd_copyback<<<nblk(nE2, BS), BS, 0, st>>>(c->m, c->s, c->w, d);
       }
@@ -748,9 +752,9 @@ static int stage_impl(gomgpu_ctx *c, int id) {
       tp.stab = p.stability_function;
       if (tp.flag == 3) {
         long nlev = (long)(d.L + 1) * d.J;
-        k_turb_prep<<<nblk(nlev, BS), BS, 0, st>>>(c->m, c->s, c->w, d, tp, err);
-        k_turb_solve<<<nblk(d.J, 128), 128, 0, st>>>(c->m, c->s, c->f, c->w, d, tp, err);
-        k_turb_post<<<nblk(nlev, BS), BS, 0, st>>>(c->m, c->s, c->w, d, tp);
+        k_turb_prep<<<nblk(nlev, LB(TURB_PREP)), LB(TURB_PREP), 0, st>>>(c->m, c->s, c->w, d, tp, err);
+        k_turb_solve<<<nblk(d.J, LB(TURB_SOLVE)), LB(TURB_SOLVE), 0, st>>>(c->m, c->s, c->f, c->w, d, tp, err);
+        k_turb_post<<<nblk(nlev, LB(TURB_PREP)), LB(TURB_PREP), 0, st>>>(c->m, c->s, c->w, d, tp);
       } else {
         k_turb_ri<<<nblk((long)d.L * d.J, BS), BS, 0, st>>>(c->m, c->s, c->w, d, tp);
       }
@@ -1187,3 +1191,19 @@ extern "C" long long gomgpu_footprint(const gomgpu_params *params) {
 }
 
 extern "C" long long gomgpu_device_bytes(gomgpu_ctx *c) { return c ? (long long)c->bytes : -1; }
+
+// ------------------------------------------------------------------ launch shapes
+static int launch_max(int which) { return which == GOMGPU_LAUNCH_TURB_SOLVE ? 256 : 1024; }
+
+extern "C" int gomgpu_set_launch(gomgpu_ctx *c, int which, int block) {
+  if (!c || which < 0 || which >= GOMGPU_NUM_LAUNCH) return GOMGPU_E_ARG;
+  if (block < 64 || block > launch_max(which) || (block & (block - 1)) != 0)
+    return seterr(c, GOMGPU_E_ARG, "block size must be a power of two in [64, max]");
+  c->lb[which] = block;
+  return 0;
+}
+
+extern "C" int gomgpu_get_launch(gomgpu_ctx *c, int which) {
+  if (!c || which < 0 || which >= GOMGPU_NUM_LAUNCH) return -1;
+  return c->lb[which];
+}

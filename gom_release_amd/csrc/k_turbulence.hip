@@ -195,7 +195,7 @@ __global__ void k_turb_prep(DevMesh m, DevState s, DevWork w, DevDims d, TurbPar
 // Scheme 3, pass 2 (turbulence_closure_v1.f90:219-390), one thread per face: the k and psi
 // tridiagonal systems (Thomas in place in w.tk_x / w.tk_q2tmp, factors in w.tri_gam), the
 // length scale with Galperin's clipping, and k, l back at whole levels.
-__global__ void k_turb_solve(DevMesh m, DevState s, DevForcing f, DevWork w, DevDims d, TurbParams p,
+__global__ void __launch_bounds__(256) k_turb_solve(DevMesh m, DevState s, DevForcing f, DevWork w, DevDims d, TurbParams p,
                              int *err) {
   int j = blockIdx.x * blockDim.x + threadIdx.x;
   const int J = d.J;

@@ -22,6 +22,10 @@ LIB_PATH = os.path.join(PKG, "libgomgpu.so")
 _lib = None
 
 STAGE_IDS = {name: i for i, name in enumerate(abi.STAGES)}
+# tunable launch shapes (GOMGPU_LAUNCH_* of include/gomgpu.h) and their largest block
+LAUNCH = {"elm": 0, "mom_rhs": 1, "mom_solve": 2, "tvd_rhs": 3, "tvd_solve": 4, "tvd_lim": 5,
+          "node_vel": 6, "turb_prep": 7, "turb_solve": 8, "stream": 9}
+LAUNCH_MAX = {k: (256 if k == "turb_solve" else 1024) for k in LAUNCH}
 
 
 class GomGpuError(RuntimeError):
@@ -53,6 +57,8 @@ def lib():
         L.gomgpu_timings.argtypes = [vp, C.c_int, C.POINTER(C.c_char_p), C.POINTER(C.c_double), C.c_int]
         L.gomgpu_destroy.argtypes = [vp]
         L.gomgpu_destroy.restype = None
+        L.gomgpu_set_launch.argtypes = [vp, C.c_int, C.c_int]
+        L.gomgpu_get_launch.argtypes = [vp, C.c_int]
         L.gomgpu_footprint.argtypes = [C.POINTER(abi.Params)]
         L.gomgpu_footprint.restype = C.c_longlong
         L.gomgpu_device_bytes.argtypes = [vp]
@@ -73,7 +79,8 @@ def lib():
 EXPORTED = ["gomgpu_create", "gomgpu_set_state", "gomgpu_set_forcing", "gomgpu_stage",
             "gomgpu_step", "gomgpu_run", "gomgpu_get_state", "gomgpu_budgets",
             "gomgpu_pcg_iterations", "gomgpu_sync", "gomgpu_last_error", "gomgpu_timings",
-            "gomgpu_destroy", "gomgpu_footprint", "gomgpu_device_bytes", "gomgpu_batch_create", "gomgpu_batch_add", "gomgpu_batch_remove",
+            "gomgpu_destroy", "gomgpu_set_launch", "gomgpu_get_launch", "gomgpu_footprint",
+            "gomgpu_device_bytes", "gomgpu_batch_create", "gomgpu_batch_add", "gomgpu_batch_remove",
             "gomgpu_batch_size", "gomgpu_batch_step", "gomgpu_batch_sync", "gomgpu_batch_last_error",
             "gomgpu_batch_destroy"]
 
@@ -155,6 +162,15 @@ class GpuModel:
 
     def device_bytes(self) -> int:
         return int(self._L.gomgpu_device_bytes(self._ctx))
+
+    def set_launch(self, which: str, block: int) -> None:
+        self._check(self._L.gomgpu_set_launch(self._ctx, LAUNCH[which], int(block)))
+
+    def get_launch(self, which: str) -> int:
+        return int(self._L.gomgpu_get_launch(self._ctx, LAUNCH[which]))
+
+    def launch_config(self) -> Dict[str, int]:
+        return {k: self.get_launch(k) for k in LAUNCH}
 
     def close(self) -> None:
         if self._ctx:

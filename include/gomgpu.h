@@ -260,6 +260,26 @@ void gomgpu_destroy(gomgpu_ctx *ctx);
  * gomgpu_set_forcing before gomgpu_batch_step; gomgpu_get_state on a member is ordered after
  * the batch steps submitted before it.
  * ------------------------------------------------------------------------------------- */
+/* Launch shapes the autotuner may change: threads per block of the kernels whose threads
+ * each compute their own outputs, so no choice changes any result (the PCG partition is a
+ * fixed function of the system size and is not tunable). Powers of two from 64 to 1024
+ * (256 for TURB_SOLVE). No counterpart in the reference. */
+enum {
+  GOMGPU_LAUNCH_ELM = 0,        /* k_elm, one thread per (layer, face) */
+  GOMGPU_LAUNCH_MOM_RHS = 1,    /* k_mom_rhs */
+  GOMGPU_LAUNCH_MOM_SOLVE = 2,  /* k_mom_solve, one thread per face column */
+  GOMGPU_LAUNCH_TVD_RHS = 3,    /* k_tvd_rhs */
+  GOMGPU_LAUNCH_TVD_SOLVE = 4,  /* k_tvd_solve */
+  GOMGPU_LAUNCH_TVD_LIM = 5,    /* k_tvd_lim_cell */
+  GOMGPU_LAUNCH_NODE_VEL = 6,   /* k_node_vel */
+  GOMGPU_LAUNCH_TURB_PREP = 7,  /* k_turb_prep, k_turb_post */
+  GOMGPU_LAUNCH_TURB_SOLVE = 8, /* k_turb_solve */
+  GOMGPU_LAUNCH_STREAM = 9,     /* every other per-entity kernel */
+  GOMGPU_NUM_LAUNCH = 10
+};
+int gomgpu_set_launch(gomgpu_ctx *ctx, int which, int block);
+int gomgpu_get_launch(gomgpu_ctx *ctx, int which);
+
 /* Device bytes gomgpu_create would allocate for these parameters (admission by HBM
  * footprint), and the bytes a context holds; -1 on bad arguments. */
 long long gomgpu_footprint(const gomgpu_params *params);

@@ -33,3 +33,19 @@ def test_prepared_round_trip_is_exact():
     for a, b in ((m1, m0), (s1, s0)):
         for k in b.arr:
             assert np.array_equal(a[k], b[k]), k
+
+
+def test_autotune_cache_round_trip(tmp_path):
+    from gom_release_amd import abi, autotune
+    p = abi.Params()
+    p.maxele, p.maxlayer, p.turbulence_flag = 200200, 20, 3
+    path = str(tmp_path / "launch.json")
+    assert autotune.load(p, path) is None
+    cfg = {"elm": 256, "mom_rhs": 512}
+    autotune.save(p, cfg, path)
+    assert autotune.load(p, path) == cfg
+    q = abi.Params()
+    q.maxele, q.maxlayer, q.turbulence_flag = 190000, 20, 3    # same size class
+    assert autotune.load(q, path) == cfg
+    q.maxlayer = 30
+    assert autotune.load(q, path) is None
