@@ -43,6 +43,9 @@ def make_case(cells: str, turbulence: str = "gls"):
         return meshgen.estuary_case(ncol=400, nrep=100, **kw)
     if cells == "20k":
         return meshgen.estuary_case(ncol=100, nrep=40, **kw)
+    if cells == "1200k":  # the size of the 1.2M-cell x 30-layer configuration
+        kw.update(nlayer=30, dt=60.0)
+        return meshgen.estuary_case(ncol=980, nrep=245, **kw)
     raise ValueError(cells)
 
 
@@ -287,7 +290,7 @@ def main() -> int:
         "scaling": "weak", "vs_baseline": None, "dtype": "f64" if prec == 0 else "f32+f64",
         "data": "synthetic",
         "config": {"workload": f"synthetic estuary, {case.maxele} mixed tri/quad cells x {case.maxlayer} "
-                               f"z-layers, dt=120 s, TVD minmod x2 sub-cycles, M2+K1, 500 m3/s river, "
+                               f"z-layers, dt={case.dt:g} s, TVD minmod x2 sub-cycles, M2+K1, 500 m3/s river, "
                                f"10 m/s wind, " + ("GLS k-kl turbulence closure (Galperin)"
                                                    if args.turbulence == "gls" else "constant Av/Kv"),
                    "turbulence": args.turbulence,

@@ -63,6 +63,9 @@ extern "C" int gom_pcg_reg_capacity(int device, int n, int nb, int bs);
 extern "C" hipError_t gom_launch_pcg_cg1(const PcgArgs *args, int nb, int bs, int slots, int cooperative,
                                          hipStream_t st);
 extern "C" int gom_pcg_cg1_capacity(int device, int n, int nb, int bs, int slots);
+extern "C" int gom_pcg_cg1m_blocks(long n);
+extern "C" hipError_t gom_launch_pcg_cg1m(const PcgArgs *args, int nb, int cooperative, hipStream_t st);
+extern "C" int gom_pcg_cg1m_capacity(int device);
 
 enum { KT_PCG = 0, KT_ELM = 1, KT_MOMENTUM = 2, KT_TVD_COLUMN = 3 };
 static const char *kKernelNames[GOMGPU_NUM_KTIMERS] = {"k_pcg", "k_elm", "k_momentum", "k_tvd_column"};
@@ -94,7 +97,8 @@ struct gomgpu_ctx {
   int pcg_barrier = 1;     // 0 cooperative-groups sync, 1 custom barrier
   int pcg_coop = 1;        // launch through hipLaunchCooperativeKernel
   int pcg_mode = 3;        // 0 strided k_pcg, 1 register-resident k_pcg_reg, 2 single-barrier k_pcg_cg1,
-                           // 3 k_pcg_cg1 with the slot exchange instead of the grid barrier
+                           // 3 k_pcg_cg1 with the slot exchange instead of the grid barrier (and
+                           // k_pcg_cg1m above its register capacity), 4 always k_pcg_cg1m
   int pcg_bs = 512;        // threads per block of k_pcg_reg / k_pcg_cg1
   int pcg_nb = 0;          // blocks of k_pcg_reg (0 = rows / (4 * bs))
   int ncu = 0;
@@ -568,8 +572,13 @@ static int fs_pcg(gomgpu_ctx *c) {
         le = gom_launch_pcg_reg(&a, nb, c->pcg_bs, c->pcg_coop, st);
       }
     }
+    if (le == hipErrorInvalidValue && (c->pcg_mode == 3 || c->pcg_mode == 4)) {
+      // rows beyond the register capacity: the memory-resident single-reduction solve
+      int nb = gom_pcg_cg1m_blocks(d.E);
+      if (nb <= gom_pcg_cg1m_capacity(c->device)) le = gom_launch_pcg_cg1m(&a, nb, c->pcg_coop, st);
+    }
     if (le == hipErrorInvalidValue) {
-      // memory-resident solve; at most one block per CU (the same grid in a batch)
+      // memory-resident two-reduction solve; at most one block per CU (the same grid in a batch)
       int cap = c->pcg_blocks < c->ncu ? c->pcg_blocks : c->ncu;
       if (c->pcg_cap > 0 && c->pcg_cap < cap) cap = c->pcg_cap;
       le = gom_launch_pcg(&a, cap, c->pcg_coop, st);
@@ -993,11 +1002,16 @@ static int batch_enqueue(gomgpu_batch *b) {
     BHIP(b, hipStreamWaitEvent(b->st, b->ev_join[i], 0));
   }
   for (auto &g : b->groups) BHIP(b, gom_launch_pcg_cg1_batch(g.d_args, g.d_base, g.d_nbm, g.nmem, g.total, b->st));
-  for (gomgpu_ctx *c : b->solo) {
+  for (gomgpu_ctx *c : b->solo) {  // the solve each of them runs alone (fs_pcg)
     PcgArgs a;
     pcg_args(c, a);
-    int cap = c->pcg_blocks < c->ncu ? c->pcg_blocks : c->ncu;
-    BHIP(b, gom_launch_pcg(&a, cap, 0, b->st));
+    int nb = gom_pcg_cg1m_blocks(c->d.E);
+    if (nb <= gom_pcg_cg1m_capacity(c->device)) {
+      BHIP(b, gom_launch_pcg_cg1m(&a, nb, 0, b->st));
+    } else {
+      int cap = c->pcg_blocks < c->ncu ? c->pcg_blocks : c->ncu;
+      BHIP(b, gom_launch_pcg(&a, cap, 0, b->st));
+    }
   }
   BHIP(b, hipEventRecord(b->ev_fork, b->st));
   for (size_t i = 0; i < b->members.size(); i++) {

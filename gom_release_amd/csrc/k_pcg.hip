@@ -738,6 +738,163 @@ __global__ void __launch_bounds__(BS) k_pcg_cg1(PcgArgs a) {
   cg1_body<R, BS, SLOTS>(a, blockIdx.x, gridDim.x, lds, &gen0);
 }
 
+// Memory-resident form of the same single-reduction solve, for systems whose rows do not
+// fit in registers (more than 256 blocks x 2048 rows). A thread walks its rows row0 + t +
+// k*BS (the assignment of the register-resident kernel) each iteration; x and the search
+// direction p stay in a.x and a.d0, the published (r, w, s) in a.pub, the matrix is
+// streamed. Per row the arithmetic is that of cg1_body statement for statement.
+template <int BS>
+__device__ __forceinline__ void cg1m_body(const PcgArgs &a, const int b, const int nb, double *lds,
+                                          unsigned *gen0) {
+  if (threadIdx.x == 0) *gen0 = __hip_atomic_load(&a.bar[1], __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT);
+  __syncthreads();
+  unsigned gen = *gen0;
+  const int n = a.n;
+  const int ng = nb < PCG_XCD ? nb : PCG_XCD;
+  const int g = b % ng, slot = b / ng;
+  int chunk = 0;
+  for (int q = 0; q < g; q++) chunk += nb / ng + (q < nb % ng ? 1 : 0);
+  chunk += slot;
+  const int rows_per_block = (n + nb - 1) / nb;
+  const int row0 = chunk * rows_per_block;
+  if (b == 0 && threadIdx.x == 0) a.info[2] = 0;
+  if (threadIdx.x < 6)
+    __hip_atomic_store(a.slots + 2 * ((size_t)(threadIdx.x >> 1) * nb + b) + (threadIdx.x & 1),
+                       PCG_SLOT_SENTINEL, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  double *pub = a.pub;
+  const size_t n1 = (size_t)n, n2 = 2 * (size_t)n;
+#define CG1M_ROWS(i)                                                                     \
+  for (int loc = threadIdx.x, i = row0 + loc; loc < rows_per_block && i < n; loc += BS, i += BS)
+  // r0 = b - A x0 (pre_conj_grad.f90:48-60); publish r0; p = 0
+  CG1M_ROWS(i) {
+    double dg = a.diag[i];
+    int nf = a.nfaces[i];
+    double sum = dg * a.x[i];
+    for (int l = 0; l < nf; l++) {
+      int c = a.col[ix(l, i, n)];
+      if (c != 0) sum = sum + a.off[ix(l, i, n)] * a.x[c - 1];
+    }
+    pub[i] = a.b[i] - sum;
+    a.d0[i] = 0.0;
+  }
+  bar2_sync(a.bar, nb, gen, b);
+  // u0, w0 = A u0, gamma0 = r0.u0, delta0 = w0.u0; publish (r0, w0, 0) to buffer 1
+  double pg = 0.0, pd = 0.0;
+  CG1M_ROWS(i) {
+    double dg = a.diag[i], mi = 1.0 / dg, rr = pub[i];
+    double uu = 0.0 + mi * rr;
+    double sum = dg * uu;
+    int nf = a.nfaces[i];
+    for (int l = 0; l < nf; l++) {
+      int c = a.col[ix(l, i, n)];
+      if (c != 0) sum = sum + a.off[ix(l, i, n)] * (0.0 + (1.0 / a.diag[c - 1]) * pub[c - 1]);
+    }
+    double ww = sum;
+    pg = pg + rr * uu;
+    pd = pd + ww * uu;
+    size_t o = 3 * n1;
+    pub[o + i] = rr;
+    pub[o + n1 + i] = ww;
+    pub[o + n2 + i] = 0.0;
+  }
+  double bg = block_sum_bs(pg, lds), bd = block_sum_bs(pd, lds + 16);
+  if (threadIdx.x == 0) {
+    a.partA[b] = bg;
+    a.partB[b] = bd;
+  }
+  bar2_sync(a.bar, nb, gen, b);
+  double gamma = grid_total_bs(a.partA, nb, lds);
+  double delta = grid_total_bs(a.partB, nb, lds + 24);
+  const double gamma_0 = gamma;
+  double alpha = gamma / delta, beta = 0.0;
+  int cgit2 = 0, par = 1;
+  for (int cgit = 1; cgit <= a.maxit; cgit++) {
+    const size_t oin = (size_t)par * 3 * n1;
+    par ^= 1;
+    const size_t oout = (size_t)par * 3 * n1;
+    pg = 0.0;
+    pd = 0.0;
+    CG1M_ROWS(i) {
+      double dg = a.diag[i], mi = 1.0 / dg;
+      double rr = pub[oin + i], ww = pub[oin + n1 + i], ss = pub[oin + n2 + i];
+      double uu = 0.0 + mi * rr;
+      double pp = uu + beta * a.d0[i];
+      ss = ww + beta * ss;
+      double xr = a.x[i] + alpha * pp;
+      rr = rr - alpha * ss;
+      uu = 0.0 + mi * rr;
+      double sum = dg * uu;
+      int nf = a.nfaces[i];
+      for (int l = 0; l < nf; l++) {
+        int c = a.col[ix(l, i, n)];
+        if (c != 0) {
+          double rj = pub[oin + c - 1], wj = pub[oin + n1 + c - 1], sj = pub[oin + n2 + c - 1];
+          double sn = wj + beta * sj;
+          double rn = rj - alpha * sn;
+          double un = 0.0 + (1.0 / a.diag[c - 1]) * rn;
+          sum = sum + a.off[ix(l, i, n)] * un;
+        }
+      }
+      ww = sum;
+      pg = pg + rr * uu;
+      pd = pd + ww * uu;
+      a.d0[i] = pp;
+      a.x[i] = xr;
+      pub[oout + i] = rr;
+      pub[oout + n1 + i] = ww;
+      pub[oout + n2 + i] = ss;
+    }
+    double gamma_new, delta_new;
+    cg1_exchange<BS>(a.slots, b, nb, cgit, pg, pd, lds, a.info + 2, gamma_new, delta_new);
+    double beta_new = gamma_new / gamma;
+    double alpha_new = gamma_new / (delta_new - beta_new * gamma_new / alpha);
+    gamma = gamma_new;
+    beta = beta_new;
+    alpha = alpha_new;
+    if (sqrt(gamma * gamma / (gamma_0 * gamma_0)) < a.tol) {
+      cgit2 = cgit;
+      break;
+    }
+  }
+#undef CG1M_ROWS
+  if (b == 0 && threadIdx.x == 0) {
+    a.info[0] = cgit2;
+    a.info[1] = cgit2 > 0 ? 1 : 0;
+  }
+}
+
+template <int BS>
+__global__ void __launch_bounds__(BS) k_pcg_cg1m(PcgArgs a) {
+  __shared__ double lds[CG1_LDS(BS)];
+  __shared__ unsigned gen0;
+  cg1m_body<BS>(a, blockIdx.x, gridDim.x, lds, &gen0);
+}
+
+#define CG1M_BS 1024
+// Blocks of the memory-resident solve for n rows: a function of n alone (at most 256).
+extern "C" int gom_pcg_cg1m_blocks(long n) {
+  long nb = (n + 2047) / 2048;
+  return (int)(nb < 1 ? 1 : (nb > 256 ? 256 : nb));
+}
+
+extern "C" hipError_t gom_launch_pcg_cg1m(const PcgArgs *args, int nb, int cooperative, hipStream_t st) {
+  PcgArgs a = *args;
+  if (nb > CG1M_BS || nb > PCG_MAX_SLOT_BLOCKS) return hipErrorInvalidValue;
+  void *kargs[] = {&a};
+  const void *k = (const void *)k_pcg_cg1m<CG1M_BS>;
+  if (cooperative) return hipLaunchCooperativeKernel(k, dim3(nb), dim3(CG1M_BS), kargs, 0, st);
+  return hipLaunchKernel(k, dim3(nb), dim3(CG1M_BS), kargs, 0, st);
+}
+
+extern "C" int gom_pcg_cg1m_capacity(int device) {
+  int per_cu = 0, ncu = 0;
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, (const void *)k_pcg_cg1m<CG1M_BS>, CG1M_BS, 0) !=
+      hipSuccess)
+    return 0;
+  if (hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, device) != hipSuccess) return 0;
+  return per_cu * ncu;
+}
+
 // Several independent systems in one launch: member m owns blocks base[m] .. base[m] +
 // nbm[m] - 1 (base[m] a multiple of 8, so the member-local block index keeps the XCD
 // grouping); blocks in the padding between members return at once. Each member runs the
