@@ -24,6 +24,12 @@ struct DevMesh {
 #undef DM_R
 #undef DM_I
   const double *Cdb_static;  // Manning drag from the still-water depth (bf_flag 1)
+  // Static node stencil of calculate_velocity_at_node.f90:67-138, built at create: for the
+  // jj-th adjacent cell of node i, its two faces touching the node (slots 2jj, 2jj+1) with
+  // their length, boundary factor and bottom layer, and the cell with its area and bottom
+  // layer; [slot][node] like the other tables. ns_weight is the node's sum of fw/L.
+  const int32_t *ns_face, *ns_fkb, *ns_cell, *ns_ckb;
+  const double *ns_Lf, *ns_fw, *ns_area, *ns_weight;
 };
 
 struct DevState {

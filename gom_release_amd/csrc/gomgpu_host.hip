@@ -298,6 +298,59 @@ extern "C" int gomgpu_create(const gomgpu_mesh *mesh, const gomgpu_params *param
     if ((rc = upload(c, dp, cdb.data(), Shape{1, p.maxface}))) goto fail;
     c->m.Cdb_static = dp;
   }
+  {
+    // node stencil (calculate_velocity_at_node.f90:67-138): the faces and cells the node
+    // loop of the reference visits, in its order, with the level-independent factors
+    const long P = p.maxnod, NB = p.max_no_neighbor_node;
+    if (!mesh->adj_cells_at_node || !mesh->adj_cellnum_at_node || !mesh->node_count_each_element ||
+        !mesh->tri_or_quad || !mesh->facenum_at_cell || !mesh->adj_cellnum_at_face ||
+        !mesh->bottom_layer_at_face || !mesh->bottom_layer_at_element || !mesh->face_length || !mesh->area) {
+      rc = seterr(c, GOMGPU_E_ARG, "mesh connectivity missing");
+      goto fail;
+    }
+    std::vector<int32_t> face(2 * NB * P, 0), fkb(2 * NB * P, 0), cell(NB * P, 0), ckb(NB * P, 0);
+    std::vector<double> Lf(2 * NB * P, 1.0), fw(2 * NB * P, 0.0), area(NB * P, 0.0), weight(P, 0.0);
+    for (long i = 0; i < P; i++) {
+      int nadj = mesh->adj_cells_at_node[i];
+      double wsum = 0.0;
+      for (int jj = 1; jj <= nadj && jj <= NB; jj++) {
+        int isn = mesh->adj_cellnum_at_node[(jj - 1) + i * NB] - 1;
+        int nc = mesh->node_count_each_element[(jj - 1) + i * NB];
+        int N = mesh->tri_or_quad[isn];
+        for (int l = N - 2; l <= N - 1; l++) {
+          int v = nc + l;
+          int sl = v > N ? v - N : v;  // start_end_node (set_geometry_1.f90:78-97)
+          int ifn = mesh->facenum_at_cell[(sl - 1) + (long)isn * 4] - 1;
+          double w = mesh->adj_cellnum_at_face[1 + (long)ifn * 2] == 0 ? 2.0 : 1.0;
+          long o = (long)(2 * (jj - 1) + (l - (N - 2))) * P + i;
+          face[o] = ifn;
+          fkb[o] = mesh->bottom_layer_at_face[ifn];
+          Lf[o] = mesh->face_length[ifn];
+          fw[o] = w;
+          wsum = wsum + w / mesh->face_length[ifn];
+        }
+        long oc = (long)(jj - 1) * P + i;
+        cell[oc] = isn;
+        ckb[oc] = mesh->bottom_layer_at_element[isn];
+        area[oc] = mesh->area[isn];
+      }
+      weight[i] = wsum;
+    }
+    int32_t *ip;
+    double *dp;
+#define NS_UP(vec, field, T, ptr)                                                     \
+    if ((rc = dalloc(c, &ptr, vec.size())) || (rc = upload(c, ptr, vec.data(), Shape{1, (long)vec.size()}))) goto fail; \
+    c->m.field = ptr;
+    NS_UP(face, ns_face, int32_t, ip)
+    NS_UP(fkb, ns_fkb, int32_t, ip)
+    NS_UP(cell, ns_cell, int32_t, ip)
+    NS_UP(ckb, ns_ckb, int32_t, ip)
+    NS_UP(Lf, ns_Lf, double, dp)
+    NS_UP(fw, ns_fw, double, dp)
+    NS_UP(area, ns_area, double, dp)
+    NS_UP(weight, ns_weight, double, dp)
+#undef NS_UP
+  }
   // state
 #define AL_SR(name, ext)                                                             \
   {                                                                                  \
@@ -1169,6 +1222,13 @@ extern "C" long long gomgpu_footprint(const gomgpu_params *params) {
   GOMGPU_MESH_REAL(FP_R)
   GOMGPU_MESH_INT(FP_I)
   add(p.maxface, sizeof(double));  // Cdb_static
+  {  // node stencil
+    size_t P = p.maxnod, NB = p.max_no_neighbor_node;
+    add(2 * NB * P, sizeof(int32_t)); add(2 * NB * P, sizeof(int32_t));
+    add(NB * P, sizeof(int32_t)); add(NB * P, sizeof(int32_t));
+    add(2 * NB * P, sizeof(double)); add(2 * NB * P, sizeof(double));
+    add(NB * P, sizeof(double)); add(P, sizeof(double));
+  }
   GOMGPU_STATE_REAL(FP_R)
   GOMGPU_STATE_INT(FP_I)
   GOMGPU_FORCING_REAL(FP_R)

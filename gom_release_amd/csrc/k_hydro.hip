@@ -1058,41 +1058,44 @@ __global__ void k_face_levels(DevMesh m, DevState s, DevDims d) {
 }
 
 // Node u, v (calculate_velocity_at_node.f90:67-115) and w (:120-138); one thread per
-// (level, node).
+// (level, node). The node's faces and cells come from the static stencil tables (one
+// coalesced load each instead of the reference's cell -> face lookups), visited in the
+// reference's order; the weight sum is precomputed in that order.
 __global__ void k_node_vel(DevMesh m, DevState s, DevDims d) {
   int t = blockIdx.x * blockDim.x + threadIdx.x;
   const int P = d.P, E = d.E, J = d.J;
   if (t >= P * (d.L + 1)) return;
   int i = t % P, k = t / P;
   int nadj = m.adj_cells_at_node[i];
-  double u = 0.0, v = 0.0, weight = 0.0;
+  double u = 0.0, v = 0.0;
   int icount = 0;
   double w = 0.0, tw = 0.0;
-  for (int jj = 1; jj <= nadj; jj++) {
-    int isn = TB(m.adj_cellnum_at_node, jj, i, P) - 1;
-    int nc = TB(m.node_count_each_element, jj, i, P);
-    int N = m.tri_or_quad[isn];
-    for (int l = N - 2; l <= N - 1; l++) {
-      int ifn = TB(m.facenum_at_cell, sen(N, nc, l), isn, E) - 1;
-      double fw = (TB(m.adj_cellnum_at_face, 2, ifn, J) == 0) ? 2.0 : 1.0;
+  for (int jj = 0; jj < nadj; jj++) {
+#pragma unroll
+    for (int q = 0; q < 2; q++) {
+      size_t o = (size_t)(2 * jj + q) * P + i;
+      int ifn = m.ns_face[o];
       int kt = s.top_layer_at_face[ifn];
-      double Lf = m.face_length[ifn];
-      if (kt != 0 && k >= m.bottom_layer_at_face[ifn] - 1) {
+      if (kt != 0 && k >= m.ns_fkb[o] - 1) {
         int kvt = k < kt ? k : kt;
+        double Lf = m.ns_Lf[o], fw = m.ns_fw[o];
         u = u + A0(s.u_face_level, kvt, ifn, J) / Lf * fw;
         v = v + A0(s.v_face_level, kvt, ifn, J) / Lf * fw;
         icount++;
       }
-      weight = weight + fw / Lf;
     }
+    size_t oc = (size_t)jj * P + i;
+    int isn = m.ns_cell[oc];
     int te = s.top_layer_at_element[isn];
-    if (te != 0 && k >= m.bottom_layer_at_element[isn] - 1) {
+    if (te != 0 && k >= m.ns_ckb[oc] - 1) {
       int kvt = k < te ? k : te;
-      tw = tw + m.area[isn];
-      w = w + A0(s.wn_cell, kvt, isn, E) * m.area[isn];
+      double ar = m.ns_area[oc];
+      tw = tw + ar;
+      w = w + A0(s.wn_cell, kvt, isn, E) * ar;
     }
   }
   if (icount != 0) {
+    double weight = m.ns_weight[i];
     u = u / weight;
     v = v / weight;
   }
