@@ -16,7 +16,8 @@ __global__ void k_friction(DevMesh, DevState, DevDims, int, double, double);
 __global__ void k_elm_substeps(DevMesh, DevState, DevDims, double, double, int, int);
 __global__ void k_elm(DevMesh, DevState, DevDims, double, double, double, int *);
 __global__ void k_elm_copy(DevState, long);
-__global__ void k_mom_rhs(DevMesh, DevState, DevForcing, DevDims, MomParams);
+__global__ void k_cell_uv(DevMesh, DevState, DevWork, DevDims);
+__global__ void k_mom_rhs(DevMesh, DevState, DevForcing, DevWork, DevDims, MomParams);
 __global__ void k_mom_solve(DevMesh, DevState, DevForcing, DevWork, DevDims, MomParams, int *);
 __global__ void k_river_velocity(DevMesh, DevState, DevForcing, DevDims, double *);
 __global__ void k_fs_face(DevMesh, DevState, DevWork, DevDims, double);
@@ -91,7 +92,7 @@ struct gomgpu_ctx {
   std::vector<void *> allocs;
   size_t bytes = 0;  // device memory held by the context
   // threads per block of the tunable launches (GOMGPU_LAUNCH_*); none changes results
-  int lb[GOMGPU_NUM_LAUNCH] = {128, 256, 128, 256, 128, 128, 256, 256, 128, 256};
+  int lb[GOMGPU_NUM_LAUNCH] = {128, 256, 64, 256, 128, 256, 256, 256, 128, 256};
   int pcg_blocks = 0;      // co-resident capacity of the PCG kernel
   int pcg_cap = 0;         // tuned cap of the PCG grid (0 = capacity)
   int pcg_barrier = 1;     // 0 cooperative-groups sync, 1 custom barrier
@@ -416,7 +417,8 @@ extern "C" int gomgpu_create(const gomgpu_mesh *mesh, const gomgpu_params *param
         (rc = dalloc(c, &w.qbk, NT * L * QB)) || (rc = dalloc(c, &w.node_new, NT * L * P)) ||
         (rc = dalloc(c, &w.face_new, NT * L * J)) || (rc = dalloc(c, &w.budget, 2 * ((E + 255) / 256) + 2)) ||
         (rc = dalloc(c, &w.err, 2)) || (rc = dalloc(c, &w.turb_kvf, (L + 1) * J)) ||
-        (rc = dalloc(c, &w.tri_gam, L * (E > J ? E : J))) || (rc = dalloc(c, &w.tvd_x, NT * L * E)))
+        (rc = dalloc(c, &w.tri_gam, L * (E > J ? E : J))) || (rc = dalloc(c, &w.tvd_x, NT * L * E)) ||
+        (rc = dalloc(c, &w.cell_u, L * E)) || (rc = dalloc(c, &w.cell_v, L * E)))
       goto fail;
     if (p.turbulence_flag == 3 &&
         ((rc = dalloc(c, &w.tk_av, (L + 1) * J)) || (rc = dalloc(c, &w.tk_kv, (L + 1) * J)) ||
@@ -685,7 +687,8 @@ static int stage_impl(gomgpu_ctx *c, int id) {
     case GOMGPU_STAGE_MOMENTUM: {
       MomParams q{p.dt, p.gravity, p.theta, p.rho_o, p.smagorinsky_parameter, p.Ah_0, p.Kh_0, p.baroclinic_flag};
       KT_BEGIN(KT_MOMENTUM);
-      k_mom_rhs<<<nblk((long)d.L * d.J, LB(MOM_RHS)), LB(MOM_RHS), 0, st>>>(c->m, c->s, c->f, d, q);
+      k_cell_uv<<<nblk((long)d.L * d.E, BS), BS, 0, st>>>(c->m, c->s, c->w, d);
+      k_mom_rhs<<<nblk((long)d.L * d.J, LB(MOM_RHS)), LB(MOM_RHS), 0, st>>>(c->m, c->s, c->f, c->w, d, q);
       k_mom_solve<<<nblk(d.J, LB(MOM_SOLVE)), LB(MOM_SOLVE), 0, st>>>(c->m, c->s, c->f, c->w, d, q, err);
       KT_END(KT_MOMENTUM);
       break;
@@ -734,7 +737,7 @@ static int stage_impl(gomgpu_ctx *c, int id) {
         k_tvd_coef_face<<<nblk((long)d.J * d.L, BS), BS, 0, st>>>(c->m, c->s, c->w, d, a);
         k_tvd_coef_cell<<<nblk((long)d.E * (d.L + 1), BS), BS, 0, st>>>(c->m, c->s, c->w, d, a);
         if (p.h_flux_limiter > 0 || p.v_flux_limiter > 0) {
-          k_tvd_lim_cell<<<nblk(d.E, LB(TVD_LIM)), LB(TVD_LIM), 0, st>>>(c->m, c->s, c->w, d);
+          k_tvd_lim_cell<<<nblk((long)(d.L + 1) * d.E, LB(TVD_LIM)), LB(TVD_LIM), 0, st>>>(c->m, c->s, c->w, d);
           k_tvd_lim_face<<<nblk((long)d.J * d.L, BS), BS, 0, st>>>(c->m, c->s, c->w, d, p.h_flux_limiter);
           k_tvd_psi_cell<<<nblk((long)d.E * (d.L + 1), BS), BS, 0, st>>>(c->m, c->s, c->w, d, p.h_flux_limiter);
         }
@@ -1257,6 +1260,7 @@ extern "C" long long gomgpu_footprint(const gomgpu_params *params) {
   add(NT * L * P, D); add(NT * L * J, D);                   // node_new, face_new
   add(2 * ((E + 255) / 256) + 2, D); add(2, sizeof(int));   // budget, err
   add((L + 1) * J, D); add(L * (E > J ? E : J), D); add(NT * L * E, D);  // turb_kvf, tri_gam, tvd_x
+  add(L * E, D); add(L * E, D);                                           // cell_u, cell_v
   if (p.turbulence_flag == 3) {
     for (int i = 0; i < 4; i++) add((L + 1) * J, D);
     for (int i = 0; i < 6; i++) add(L * J, D);

@@ -424,7 +424,26 @@ __global__ void k_elm_copy(DevState s, long n) {
 // tridiagonal_solver.f90 with the three right-hand sides [G1, G2, dz], in place in the
 // output arrays (the elimination factors go to w.tri_gam). Every value is computed by the
 // same expression sequence as the reference's per-face loop.
-__global__ void k_mom_rhs(DevMesh m, DevState s, DevForcing f, DevDims d, MomParams q) {
+// Cell-layer means of the node velocities at mid-layer (the cell terms of the Smagorinsky
+// stencil, solve_momentum_equation.f90:280-300), in the reference's node order; one thread
+// per (layer, cell).
+__global__ void k_cell_uv(DevMesh m, DevState s, DevWork w, DevDims d) {
+  long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  const int E = d.E, P = d.P;
+  if (t >= (long)d.L * E) return;
+  int ie = (int)(t % E), kk = (int)(t / E) + 1;
+  int N = m.tri_or_quad[ie];
+  double ut = 0.0, vt = 0.0;
+  for (int ll = 1; ll <= N; ll++) {
+    int nd = TB(m.nodenum_at_cell, ll, ie, E) - 1;
+    ut = ut + (A0(s.u_node, kk, nd, P) + A0(s.u_node, kk - 1, nd, P)) * 0.5 / N;
+    vt = vt + (A0(s.v_node, kk, nd, P) + A0(s.v_node, kk - 1, nd, P)) * 0.5 / N;
+  }
+  AL(w.cell_u, kk, ie, E) = ut;
+  AL(w.cell_v, kk, ie, E) = vt;
+}
+
+__global__ void k_mom_rhs(DevMesh m, DevState s, DevForcing f, DevWork w, DevDims d, MomParams q) {
   long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
   const int J = d.J, E = d.E, P = d.P, L = d.L;
   if (t >= (long)L * J) return;
@@ -472,13 +491,7 @@ __global__ void k_mom_rhs(DevMesh m, DevState s, DevForcing f, DevDims d, MomPar
     }
     for (int l = 1; l <= 2; l++) {
       int ie = TB(m.adj_cellnum_at_face, l, j, J) - 1;
-      int N = m.tri_or_quad[ie];
-      double ut = 0.0, vt = 0.0;
-      for (int ll = 1; ll <= N; ll++) {
-        int nd = TB(m.nodenum_at_cell, ll, ie, E) - 1;
-        ut = ut + (A0(s.u_node, kk, nd, P) + A0(s.u_node, kk - 1, nd, P)) * 0.5 / N;
-        vt = vt + (A0(s.v_node, kk, nd, P) + A0(s.v_node, kk - 1, nd, P)) * 0.5 / N;
-      }
+      double ut = AL(w.cell_u, kk, ie, E), vt = AL(w.cell_v, kk, ie, E);
       temp[2 * l + 3] = ut * ct + vt * st;
       temp[2 * l + 4] = -ut * st + vt * ct;
     }

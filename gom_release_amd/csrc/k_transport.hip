@@ -148,33 +148,59 @@ __device__ __forceinline__ double limiter(int which, double r, double phi) {
   return PHI - phi;
 }
 
-// Limiter ratios per cell column (solve_transport_equation_v19.f90:624-842): the vertical
-// ratios r_ik, and the per-layer upwind sums that the face ratios r_jk need.
+// Horizontal upwind sums of one cell-layer (solve_transport_equation_v19.f90:700-760): the
+// inflow-weighted differences sum1 and the inflows sum2, over the cell's active faces in
+// slot order.
+__device__ __forceinline__ void lim_hsums(const DevMesh &m, const DevWork &w, const DevDims &d, int e, int k,
+                                          double *sum1, double *sum2) {
+  const int E = d.E, L2 = d.L + 2, J = d.J;
+  sum1[0] = sum1[1] = 0.0;
+  sum2[0] = sum2[1] = 0.0;
+  int N = m.tri_or_quad[e];
+  for (int l = 1; l <= N; l++) {
+    int j = TB(m.facenum_at_cell, l, e, E) - 1, ii = TB(m.adj_cellnum_at_cell, l, e, E);
+    if (face_active(m, j)) {
+      double Q2 = AL(w.Qj, k, j, J) * (double)TB(m.sign_in_outflow, l, e, E);
+      if (Q2 < 0.0) {
+        for (int t = 0; t < d.NT; t++) {
+          double ci = con_across(m, w, d, e, ii, k, t);
+          sum1[t] = sum1[t] + fabs(Q2) * (w.con[cix(t, k, e, L2, E)] - ci);
+          sum2[t] = sum2[t] + fabs(Q2);
+        }
+      }
+    }
+  }
+}
+
+// Limiter ratios (solve_transport_equation_v19.f90:624-842), one thread per (level, cell):
+// the per-layer upwind sums s1a, s2a that the face ratios r_jk need (layer k), and the
+// vertical ratio r_ik (level k). As in the reference, the vertical ratios reuse the
+// horizontal sums left over from the column's top layer; each thread recomputes those in
+// the same order.
 __global__ void k_tvd_lim_cell(DevMesh m, DevState s, DevWork w, DevDims d) {
-  int e = blockIdx.x * blockDim.x + threadIdx.x;
-  const int E = d.E, L = d.L, L1 = d.L + 1, L2 = d.L + 2, J = d.J;
-  if (e >= E) return;
+  long q = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  const int E = d.E, L = d.L, L1 = d.L + 1, L2 = d.L + 2;
+  if (q >= (long)L1 * E) return;
+  int e = (int)(q % E), k = (int)(q / E);
   int tl = s.top_layer_at_element[e], bl = m.bottom_layer_at_element[e];
-  for (int t = 0; t < d.NT; t++)
-    for (int k = 0; k <= L; k++) w.ri[zix(t, k, e, L1, E)] = 1.0;
-  if (tl == 0) return;
-  int nv = tl - bl + 1, N = m.tri_or_quad[e];
-  double sum1[2] = {0, 0}, sum2[2] = {0, 0};
-  for (int k = bl; k <= tl; k++) {
-    double s1a[2] = {0, 0}, s2a[2] = {0, 0};
-    sum1[0] = sum1[1] = 0.0;
-    sum2[0] = sum2[1] = 0.0;
+  if (tl == 0 || k < bl - 1 || k > tl) {
+    for (int t = 0; t < d.NT; t++) w.ri[zix(t, k, e, L1, E)] = 1.0;
+    return;
+  }
+  int nv = tl - bl + 1;
+  if (k >= bl) {  // s1a, s2a of layer k
+    double s1a[2] = {0, 0}, s2a[2] = {0, 0}, sum1[2], sum2[2];
     for (int t = 0; t < d.NT; t++) {
       double c0 = w.con[cix(t, k, e, L2, E)];
       if (nv != 1) {
         if (k == bl) {
-          double q = A0(w.Qi, k, e, E);
-          double r2 = fabs(0.5 * (q + fabs(q)));
+          double qq = A0(w.Qi, k, e, E);
+          double r2 = fabs(0.5 * (qq + fabs(qq)));
           s1a[t] = r2 * (c0 - w.con[cix(t, k + 1, e, L2, E)]);
           s2a[t] = r2;
         } else if (k == tl) {
-          double q = A0(w.Qi, k - 1, e, E);
-          double r1 = fabs(0.5 * (q + fabs(q)));
+          double qq = A0(w.Qi, k - 1, e, E);
+          double r1 = fabs(0.5 * (qq + fabs(qq)));
           s1a[t] = r1 * (c0 - w.con[cix(t, k - 1, e, L2, E)]);
           s2a[t] = r1;
         } else {
@@ -185,46 +211,38 @@ __global__ void k_tvd_lim_cell(DevMesh m, DevState s, DevWork w, DevDims d) {
         }
       }
     }
-    for (int l = 1; l <= N; l++) {
-      int j = TB(m.facenum_at_cell, l, e, E) - 1, ii = TB(m.adj_cellnum_at_cell, l, e, E);
-      if (face_active(m, j)) {
-        double Q2 = AL(w.Qj, k, j, J) * (double)TB(m.sign_in_outflow, l, e, E);
-        if (Q2 < 0.0) {
-          for (int t = 0; t < d.NT; t++) {
-            double ci = con_across(m, w, d, e, ii, k, t);
-            sum1[t] = sum1[t] + fabs(Q2) * (w.con[cix(t, k, e, L2, E)] - ci);
-            sum2[t] = sum2[t] + fabs(Q2);
-          }
-        }
-      }
-    }
+    lim_hsums(m, w, d, e, k, sum1, sum2);
     for (int t = 0; t < d.NT; t++) {
       w.s1a[nix(t, k, e, L, E)] = s1a[t] + sum1[t];
       w.s2a[nix(t, k, e, L, E)] = s2a[t] + sum2[t];
     }
   }
+  // r_ik of level k: computed for bl <= k <= tl - 1, 1.0 at bl - 1 and tl
+  if (k == tl || k == bl - 1) {
+    for (int t = 0; t < d.NT; t++) w.ri[zix(t, k, e, L1, E)] = 1.0;
+    return;
+  }
+  double sum1[2], sum2[2];
+  lim_hsums(m, w, d, e, tl, sum1, sum2);
   for (int t = 0; t < d.NT; t++) {
+    double r;
     if (A0(w.Qi, tl, e, E) >= 0.0) {
-      for (int k = bl; k <= tl - 1; k++) {
-        double qm = A0(w.Qi, k - 1, e, E);
-        double c0 = w.con[cix(t, k, e, L2, E)];
-        double rt1 = sum1[t] + fabs(qm) * (c0 - w.con[cix(t, k - 1, e, L2, E)]);
-        double rt2 = sum2[t] + fabs(qm);
-        double rt = (w.con[cix(t, k + 1, e, L2, E)] - c0) * rt2;
-        w.ri[zix(t, k, e, L1, E)] = (rt == 0.0) ? 1.0 : rt1 / rt;
-      }
-    } else {
-      for (int k = bl + 1; k <= tl; k++) {
-        double qk = A0(w.Qi, k, e, E);
-        double c0 = w.con[cix(t, k, e, L2, E)];
-        double rt1 = sum1[t] + fabs(qk) * (c0 - w.con[cix(t, k + 1, e, L2, E)]);
-        double rt2 = sum2[t] + fabs(qk);
-        double rt = (w.con[cix(t, k - 1, e, L2, E)] - c0) * rt2;
-        w.ri[zix(t, k - 1, e, L1, E)] = (rt == 0.0) ? 1.0 : rt1 / rt;
-      }
+      double qm = A0(w.Qi, k - 1, e, E);
+      double c0 = w.con[cix(t, k, e, L2, E)];
+      double rt1 = sum1[t] + fabs(qm) * (c0 - w.con[cix(t, k - 1, e, L2, E)]);
+      double rt2 = sum2[t] + fabs(qm);
+      double rt = (w.con[cix(t, k + 1, e, L2, E)] - c0) * rt2;
+      r = (rt == 0.0) ? 1.0 : rt1 / rt;
+    } else {  // the reference's k runs bl+1..tl and writes level k-1
+      int kk = k + 1;
+      double qk = A0(w.Qi, kk, e, E);
+      double c0 = w.con[cix(t, kk, e, L2, E)];
+      double rt1 = sum1[t] + fabs(qk) * (c0 - w.con[cix(t, kk + 1, e, L2, E)]);
+      double rt2 = sum2[t] + fabs(qk);
+      double rt = (w.con[cix(t, kk - 1, e, L2, E)] - c0) * rt2;
+      r = (rt == 0.0) ? 1.0 : rt1 / rt;
     }
-    w.ri[zix(t, tl, e, L1, E)] = 1.0;
-    w.ri[zix(t, bl - 1, e, L1, E)] = 1.0;
+    w.ri[zix(t, k, e, L1, E)] = r;
   }
 }
 

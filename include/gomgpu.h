@@ -270,7 +270,7 @@ enum {
   GOMGPU_LAUNCH_MOM_SOLVE = 2,  /* k_mom_solve, one thread per face column */
   GOMGPU_LAUNCH_TVD_RHS = 3,    /* k_tvd_rhs */
   GOMGPU_LAUNCH_TVD_SOLVE = 4,  /* k_tvd_solve */
-  GOMGPU_LAUNCH_TVD_LIM = 5,    /* k_tvd_lim_cell */
+  GOMGPU_LAUNCH_TVD_LIM = 5,    /* k_tvd_lim_cell, one thread per (level, cell) */
   GOMGPU_LAUNCH_NODE_VEL = 6,   /* k_node_vel */
   GOMGPU_LAUNCH_TURB_PREP = 7,  /* k_turb_prep, k_turb_post */
   GOMGPU_LAUNCH_TURB_SOLVE = 8, /* k_turb_solve */
