@@ -14,7 +14,7 @@
 #include "../../include/gomgpu.h"
 
 #define GOM_LMAX 64  // largest supported maxlayer (per-thread column arrays)
-#define PCG_MAX_SLOT_BLOCKS 1024  // most blocks of a slot-exchange PCG launch
+#define PCG_MAX_SLOT_BLOCKS 256  // most blocks of one system in a slot-exchange PCG launch
 
 struct DevMesh {
 #define DM_R(name, ext) const double *name;

@@ -598,10 +598,14 @@ static void pcg_args(gomgpu_ctx *c, PcgArgs &a) {
   a.tol = c->p.error_tolerance;
 }
 
-// Blocks of the register-resident solve for n rows at 512 threads and up to 4 rows per
-// thread: a function of n alone, so a system gets the same partition alone and in a batch.
+// Blocks of the register-resident solve for n rows at 512 threads: the fewest rows per
+// thread (1, 2 or 4) that cover the system with at most 256 blocks, then as many blocks as
+// that needs. A function of n alone, so a system gets the same partition alone and in a
+// batch. Above 4 x 256 x 512 rows the result exceeds 256 (the memory-resident solve).
 static int pcg_blocks_for(long n) {
-  int nb = (int)((n + 4L * 512 - 1) / (4L * 512));
+  long r = (n + 256L * 512 - 1) / (256L * 512);
+  r = r <= 1 ? 1 : r <= 2 ? 2 : 4;
+  int nb = (int)((n + r * 512 - 1) / (r * 512));
   return nb < 1 ? 1 : nb;
 }
 
@@ -615,12 +619,14 @@ static int fs_pcg(gomgpu_ctx *c) {
     hipError_t le = hipErrorInvalidValue;
     if (c->pcg_mode >= 1 && c->pcg_mode <= 3) {
       int rmax = c->pcg_bs >= 1024 ? 2 : c->pcg_bs >= 512 ? 4 : 8;
-      int nb = c->pcg_nb > 0 ? c->pcg_nb : (int)((d.E + (long)rmax * c->pcg_bs - 1) / ((long)rmax * c->pcg_bs));
+      int nb = c->pcg_nb > 0 ? c->pcg_nb
+               : c->pcg_bs == 512 ? pcg_blocks_for(d.E)
+                                  : (int)((d.E + (long)rmax * c->pcg_bs - 1) / ((long)rmax * c->pcg_bs));
       if (nb < 1) nb = 1;
       if (nb > 4096) nb = 4096;
       if (!c->pcg_coop && nb > c->ncu) nb = c->ncu;  // plain launch: at most one block per CU
       if (c->pcg_mode == 2 || c->pcg_mode == 3) {
-        int slots = c->pcg_mode == 3 && nb <= c->pcg_bs && nb <= PCG_MAX_SLOT_BLOCKS;
+        int slots = c->pcg_mode == 3 && nb <= PCG_MAX_SLOT_BLOCKS;
         if (nb <= gom_pcg_cg1_capacity(c->device, d.E, nb, c->pcg_bs, slots))
           le = gom_launch_pcg_cg1(&a, nb, c->pcg_bs, slots, c->pcg_coop, st);
       } else if (nb <= gom_pcg_reg_capacity(c->device, d.E, nb, c->pcg_bs)) {

@@ -480,11 +480,10 @@ extern "C" int gom_pcg_reg_capacity(int device, int n, int nb, int bs) {
 // Slot exchange of the per-block partial sums, used instead of a grid barrier inside the
 // single-barrier PCG loop. slots holds 3 x nb pairs of 64-bit words; a pair is either the
 // sentinel or the block's (gamma, delta) partials of one iteration. In iteration it block b
-// (1) resets its pair of iteration it+1 to the sentinel, (2) releases (its row values and
-// the reset become visible), (3) stores its pair of iteration it, then polls the nb pairs of
-// iteration it until none is the sentinel and acquires. A pair is reset only after every
-// block has published the iteration after the one it held, i.e. after every block has read
-// it. The totals are summed in the same order as grid_total_bs.
+// (1) resets its pair of iteration it+1 to the sentinel, (2) waits until its row values and
+// the reset are performed, (3) stores its pair of iteration it, then polls the nb pairs of
+// iteration it until none is the sentinel. A pair is reset only after every block has
+// published the iteration after the one it held, i.e. after every block has read it.
 #define PCG_SLOT_SENTINEL 0x7ff4c0ffee0f1a6bull
 #define PCG_CANONICAL_NAN 0x7ff8000000000000ull
 
@@ -498,12 +497,45 @@ __device__ __forceinline__ unsigned long long slot_bits(double v) {
 // host reports the abort. With all blocks co-resident this never triggers.
 #define PCG_SLOT_SPIN_LIMIT (1u << 22)
 
-// One iteration's reduction and exchange with three block barriers: the block's (gamma,
-// delta) partials (S1), the poll of the nb published pairs (S2) and the broadcast of the
-// totals (S3). LDS regions: [0,NW) and [NW,2NW) partials, [2NW,4NW) polled sums, [4NW,4NW+2)
-// totals; each region is rewritten only after a barrier that follows its last read. The
-// sums run in the same order as block_sum_bs / grid_total_bs, so the totals equal those of
-// slot_exchange.
+// Row values published for the neighbouring rows (r, w, s) go through relaxed agent-scope
+// atomics. On gfx950 these are the agent-coherent (sc1) forms of the plain load and store:
+// a store is performed at agent scope once the issuing wave's vmcnt reaches zero, and a load
+// reads the agent-coherent value. Ordering them by that wait (writer) and by the control
+// dependence on the polled slot (reader) is what an agent-scope release/acquire pair does for
+// atomic data; the L2 write-back and invalidate that such fences add for non-atomic data are
+// not needed, and they cost more than the rest of the exchange.
+__device__ __forceinline__ void pub_st(double *p, double v) {
+  __hip_atomic_store((unsigned long long *)p, (unsigned long long)__double_as_longlong(v), __ATOMIC_RELAXED,
+                     __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ double pub_ld(const double *p) {
+  return __longlong_as_double(
+      (long long)__hip_atomic_load((const unsigned long long *)p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
+}
+
+// Pairs polled per lane of wave 0: at most 64 * PCG_SLOT_POLL blocks per system.
+#define PCG_SLOT_POLL (PCG_MAX_SLOT_BLOCKS / 64)
+
+// Resets this block's pair of slot (it+1)%3 at the top of iteration it (thread 0). Every
+// block has read that pair's previous value (iteration it-2): this block has seen every
+// block's iteration it-1 pair, which a block publishes only after reading all pairs of it-2.
+__device__ __forceinline__ void cg1_reset_slot(unsigned long long *slots, int b, int nb, int it) {
+  if (threadIdx.x == 0) {
+    unsigned long long *rst = slots + 2 * ((size_t)((it + 1) % 3) * nb + b);
+    __hip_atomic_store(rst, PCG_SLOT_SENTINEL, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    __hip_atomic_store(rst + 1, PCG_SLOT_SENTINEL, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  }
+}
+
+// One iteration's reduction and exchange. (1) Every wave waits for its stores (the slot
+// reset and the row publications of this iteration are then performed at agent scope) and
+// leaves its partial sums in LDS; block barrier S1. (2) Thread 0 sums the wave partials and
+// publishes the block's (gamma, delta) pair in slot it%3. (3) Wave 0 polls the nb pairs of
+// that slot, lane l owning pairs l, l+64, ..., until none is the sentinel, adds its pairs in
+// that order and reduces over the wave; block barrier S2 broadcasts the totals. Every block
+// adds the same values in the same order, so all agree bit for bit. LDS: [0,2NW) wave
+// partials, [2NW,2NW+2) totals; each is rewritten only after a barrier that follows its last
+// read.
 template <int BS>
 __device__ __forceinline__ void cg1_exchange(unsigned long long *slots, int b, int nb, int it, double pg, double pd,
                                              double *lds, int *abort, double &gamma, double &delta) {
@@ -515,30 +547,38 @@ __device__ __forceinline__ void cg1_exchange(unsigned long long *slots, int b, i
     lds[wid] = pg;
     lds[NW + wid] = pd;
   }
-  __builtin_amdgcn_s_waitcnt(0);  // this wave's row-value stores (and loads) complete
+  __builtin_amdgcn_s_waitcnt(0);  // this wave's slot reset and row publications performed
   __syncthreads();                // S1
-  const int cur = it % 3, nxt = (it + 1) % 3;
-  if (threadIdx.x == 0) {
-    double bg = 0.0, bd = 0.0;
-    for (int w = 0; w < NW; w++) bg += lds[w];
-    for (int w = 0; w < NW; w++) bd += lds[NW + w];
-    unsigned long long *rst = slots + 2 * ((size_t)nxt * nb + b);
-    __hip_atomic_store(rst, PCG_SLOT_SENTINEL, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    __hip_atomic_store(rst + 1, PCG_SLOT_SENTINEL, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
-    unsigned long long *out = slots + 2 * ((size_t)cur * nb + b);
-    __hip_atomic_store(out, slot_bits(bg), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    __hip_atomic_store(out + 1, slot_bits(bd), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-  }
-  double vg = 0.0, vd = 0.0;
-  if ((int)threadIdx.x < nb) {
-    const unsigned long long *src = slots + 2 * ((size_t)cur * nb + threadIdx.x);
-    unsigned long long x, y;
+  const int cur = it % 3;
+  if (wid == 0) {
+    if (lane == 0) {
+      double bg = 0.0, bd = 0.0;
+      for (int w = 0; w < NW; w++) bg += lds[w];
+      for (int w = 0; w < NW; w++) bd += lds[NW + w];
+      unsigned long long *out = slots + 2 * ((size_t)cur * nb + b);
+      __hip_atomic_store(out, slot_bits(bg), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      __hip_atomic_store(out + 1, slot_bits(bd), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
+    const unsigned long long *src = slots + 2 * (size_t)cur * nb;
+    unsigned long long xs[PCG_SLOT_POLL], ys[PCG_SLOT_POLL];
+    unsigned pending = 0;
+#pragma unroll
+    for (int j = 0; j < PCG_SLOT_POLL; j++) {
+      xs[j] = ys[j] = 0;
+      if (lane + 64 * j < nb) pending |= 1u << j;
+    }
     unsigned spins = 0;
-    for (;;) {
-      x = __hip_atomic_load(src, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-      y = __hip_atomic_load(src + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-      if (x != PCG_SLOT_SENTINEL && y != PCG_SLOT_SENTINEL) break;
+    while (pending) {
+#pragma unroll
+      for (int j = 0; j < PCG_SLOT_POLL; j++)
+        if (pending >> j & 1u) {
+          xs[j] = __hip_atomic_load(src + 2 * (lane + 64 * j), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+          ys[j] = __hip_atomic_load(src + 2 * (lane + 64 * j) + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        }
+#pragma unroll
+      for (int j = 0; j < PCG_SLOT_POLL; j++)
+        if ((pending >> j & 1u) && xs[j] != PCG_SLOT_SENTINEL && ys[j] != PCG_SLOT_SENTINEL) pending &= ~(1u << j);
+      if (!pending) break;
       if ((++spins & 255u) == 0) {
         if (__hip_atomic_load(abort, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0) break;
         if (spins >= PCG_SLOT_SPIN_LIMIT) {
@@ -548,33 +588,35 @@ __device__ __forceinline__ void cg1_exchange(unsigned long long *slots, int b, i
       }
       __builtin_amdgcn_s_sleep(1);
     }
-    vg = 0.0 + __longlong_as_double(x);
-    vd = 0.0 + __longlong_as_double(y);
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
-  }
-  vg = wave_sum(vg);
-  vd = wave_sum(vd);
-  if (lane == 0) {
-    lds[2 * NW + wid] = vg;
-    lds[3 * NW + wid] = vd;
+    double vg = 0.0, vd = 0.0;
+#pragma unroll
+    for (int j = 0; j < PCG_SLOT_POLL; j++)
+      if (lane + 64 * j < nb) {
+        vg = vg + __longlong_as_double(xs[j]);
+        vd = vd + __longlong_as_double(ys[j]);
+      }
+    vg = wave_sum(vg);
+    vd = wave_sum(vd);
+    if (lane == 0) {
+      lds[2 * NW] = vg;
+      lds[2 * NW + 1] = vd;
+    }
   }
   __syncthreads();  // S2
-  if (threadIdx.x == 0) {
-    double tg = 0.0, td = 0.0;
-    for (int w = 0; w < NW; w++) tg += lds[2 * NW + w];
-    for (int w = 0; w < NW; w++) td += lds[3 * NW + w];
-    lds[4 * NW] = tg;
-    lds[4 * NW + 1] = td;
-  }
-  __syncthreads();  // S3
-  gamma = lds[4 * NW];
-  delta = lds[4 * NW + 1];
+  gamma = lds[2 * NW];
+  delta = lds[2 * NW + 1];
 }
 
 #define CG1_LDS(BS) (4 * ((BS) / 64) + 2 > 48 ? 4 * ((BS) / 64) + 2 : 48)
 
 // Block b of nb of one system: the body shared by the single-system kernel and the batched
-// kernel (where b and nb are member-local).
+// kernel (where b and nb are member-local). The rows a thread owns are row0 + t + k*BS for
+// k < R; slices k*BS >= rows_per_block are empty for the whole block, so a launch with a
+// larger R than the partition needs runs the same rows with the same arithmetic (the batched
+// kernel relies on this). The neighbour values of an iteration are loaded before any of the
+// row arithmetic (all rows at once for R <= 2, one row at a time for R = 4 to stay within
+// the register budget), so a thread waits for a few memory latencies per iteration instead
+// of one per neighbour.
 template <int R, int BS, bool SLOTS>
 __device__ __forceinline__ void cg1_body(const PcgArgs &a, const int b, const int nb, double *lds,
                                          unsigned *gen0) {
@@ -596,7 +638,7 @@ __device__ __forceinline__ void cg1_body(const PcgArgs &a, const int b, const in
   // published per-row values, double-buffered by iteration parity: [r | w | s] x 2
   double *pub = a.pub;
   int rid[R], cl[R][4];
-  double dg[R], of[R][4], mi[R], xr[R], rr[R], uu[R], ww[R], pp[R], ss[R];
+  double dg[R], of[R][4], mn[R][4], mi[R], xr[R], rr[R], uu[R], ww[R], pp[R], ss[R];
 #pragma unroll
   for (int k = 0; k < R; k++) {
     int i = row0 + threadIdx.x + k * bs;
@@ -608,6 +650,7 @@ __device__ __forceinline__ void cg1_body(const PcgArgs &a, const int b, const in
     for (int l = 0; l < 4; l++) {
       cl[k][l] = (in && l < nf) ? a.col[ix(l, i, n)] : 0;
       of[k][l] = (in && l < nf) ? a.off[ix(l, i, n)] : 0.0;
+      mn[k][l] = cl[k][l] != 0 ? 1.0 / a.diag[cl[k][l] - 1] : 0.0;  // the neighbour's M^-1
     }
     xr[k] = in ? a.x[i] : 0.0;
     pp[k] = 0.0;
@@ -637,7 +680,7 @@ __device__ __forceinline__ void cg1_body(const PcgArgs &a, const int b, const in
 #pragma unroll
     for (int l = 0; l < 4; l++) {
       int c = cl[k][l];
-      if (c != 0) sum = sum + of[k][l] * (0.0 + (1.0 / a.diag[c - 1]) * pub[c - 1]);
+      if (c != 0) sum = sum + of[k][l] * (0.0 + mn[k][l] * pub[c - 1]);
     }
     ww[k] = sum;
     pg = pg + rr[k] * uu[k];
@@ -663,38 +706,55 @@ __device__ __forceinline__ void cg1_body(const PcgArgs &a, const int b, const in
   const double gamma_0 = gamma;
   double alpha = gamma / delta, beta = 0.0;
   int cgit2 = 0;
+  constexpr int H = R >= 4 ? R : 1, RH = R / H;  // gather phases and rows per phase
   for (int cgit = 1; cgit <= a.maxit; cgit++) {
-    const size_t oin = (size_t)par * 3 * n;       // neighbours' (r, w, s) of this iteration
+    if (SLOTS) cg1_reset_slot(a.slots, b, nb, cgit);
+    const size_t oin = (size_t)par * 3 * n;  // neighbours' (r, w, s) of this iteration
     par ^= 1;
     const size_t oout = (size_t)par * 3 * n;
     pg = 0.0;
     pd = 0.0;
 #pragma unroll
-    for (int k = 0; k < R; k++) {
-      if (rid[k] < 0) continue;
-      pp[k] = uu[k] + beta * pp[k];
-      ss[k] = ww[k] + beta * ss[k];
-      xr[k] = xr[k] + alpha * pp[k];
-      rr[k] = rr[k] - alpha * ss[k];
-      uu[k] = 0.0 + mi[k] * rr[k];
-      double sum = dg[k] * uu[k];
+    for (int h = 0; h < H; h++) {
+      double nr[RH][4], nw[RH][4], ns[RH][4];
 #pragma unroll
-      for (int l = 0; l < 4; l++) {
-        int c = cl[k][l];
-        if (c != 0) {
-          double rj = pub[oin + c - 1], wj = pub[oin + n + c - 1], sj = pub[oin + 2 * (size_t)n + c - 1];
-          double sn = wj + beta * sj;
-          double rn = rj - alpha * sn;
-          double un = 0.0 + (1.0 / a.diag[c - 1]) * rn;
-          sum = sum + of[k][l] * un;
+      for (int q = 0; q < RH; q++) {
+        const int k = h * RH + q;
+        if (k * bs < rows_per_block) {  // uniform over the block
+#pragma unroll
+          for (int l = 0; l < 4; l++) {
+            const size_t j = cl[k][l] != 0 ? (size_t)(cl[k][l] - 1) : 0;  // row 0 stands in for none
+            nr[q][l] = pub_ld(pub + oin + j);
+            nw[q][l] = pub_ld(pub + oin + n + j);
+            ns[q][l] = pub_ld(pub + oin + 2 * (size_t)n + j);
+          }
         }
       }
-      ww[k] = sum;
-      pg = pg + rr[k] * uu[k];
-      pd = pd + ww[k] * uu[k];
-      pub[oout + rid[k]] = rr[k];
-      pub[oout + n + rid[k]] = ww[k];
-      pub[oout + 2 * (size_t)n + rid[k]] = ss[k];
+#pragma unroll
+      for (int q = 0; q < RH; q++) {
+        const int k = h * RH + q;
+        if (rid[k] < 0) continue;
+        pp[k] = uu[k] + beta * pp[k];
+        ss[k] = ww[k] + beta * ss[k];
+        xr[k] = xr[k] + alpha * pp[k];
+        rr[k] = rr[k] - alpha * ss[k];
+        uu[k] = 0.0 + mi[k] * rr[k];
+        double sum = dg[k] * uu[k];
+#pragma unroll
+        for (int l = 0; l < 4; l++) {
+          double sn = nw[q][l] + beta * ns[q][l];
+          double rn = nr[q][l] - alpha * sn;
+          double un = 0.0 + mn[k][l] * rn;
+          double t = sum + of[k][l] * un;
+          sum = cl[k][l] != 0 ? t : sum;
+        }
+        ww[k] = sum;
+        pg = pg + rr[k] * uu[k];
+        pd = pd + ww[k] * uu[k];
+        pub_st(pub + oout + rid[k], rr[k]);
+        pub_st(pub + oout + n + rid[k], ww[k]);
+        pub_st(pub + oout + 2 * (size_t)n + rid[k], ss[k]);
+      }
     }
     double gamma_new, delta_new;
     if (SLOTS) {
@@ -809,6 +869,7 @@ __device__ __forceinline__ void cg1m_body(const PcgArgs &a, const int b, const i
   double alpha = gamma / delta, beta = 0.0;
   int cgit2 = 0, par = 1;
   for (int cgit = 1; cgit <= a.maxit; cgit++) {
+    cg1_reset_slot(a.slots, b, nb, cgit);
     const size_t oin = (size_t)par * 3 * n1;
     par ^= 1;
     const size_t oout = (size_t)par * 3 * n1;
@@ -816,7 +877,7 @@ __device__ __forceinline__ void cg1m_body(const PcgArgs &a, const int b, const i
     pd = 0.0;
     CG1M_ROWS(i) {
       double dg = a.diag[i], mi = 1.0 / dg;
-      double rr = pub[oin + i], ww = pub[oin + n1 + i], ss = pub[oin + n2 + i];
+      double rr = pub_ld(pub + oin + i), ww = pub_ld(pub + oin + n1 + i), ss = pub_ld(pub + oin + n2 + i);
       double uu = 0.0 + mi * rr;
       double pp = uu + beta * a.d0[i];
       ss = ww + beta * ss;
@@ -828,7 +889,8 @@ __device__ __forceinline__ void cg1m_body(const PcgArgs &a, const int b, const i
       for (int l = 0; l < nf; l++) {
         int c = a.col[ix(l, i, n)];
         if (c != 0) {
-          double rj = pub[oin + c - 1], wj = pub[oin + n1 + c - 1], sj = pub[oin + n2 + c - 1];
+          double rj = pub_ld(pub + oin + c - 1), wj = pub_ld(pub + oin + n1 + c - 1),
+                 sj = pub_ld(pub + oin + n2 + c - 1);
           double sn = wj + beta * sj;
           double rn = rj - alpha * sn;
           double un = 0.0 + (1.0 / a.diag[c - 1]) * rn;
@@ -840,9 +902,9 @@ __device__ __forceinline__ void cg1m_body(const PcgArgs &a, const int b, const i
       pd = pd + ww * uu;
       a.d0[i] = pp;
       a.x[i] = xr;
-      pub[oout + i] = rr;
-      pub[oout + n1 + i] = ww;
-      pub[oout + n2 + i] = ss;
+      pub_st(pub + oout + i, rr);
+      pub_st(pub + oout + n1 + i, ww);
+      pub_st(pub + oout + n2 + i, ss);
     }
     double gamma_new, delta_new;
     cg1_exchange<BS>(a.slots, b, nb, cgit, pg, pd, lds, a.info + 2, gamma_new, delta_new);
@@ -962,7 +1024,7 @@ extern "C" hipError_t gom_launch_pcg_cg1(const PcgArgs *args, int nb, int bs, in
   PcgArgs a = *args;
   long per_block = ((long)a.n + nb - 1) / nb;
   int R = (int)((per_block + bs - 1) / bs);
-  if (slots && (nb > bs || nb > PCG_MAX_SLOT_BLOCKS)) return hipErrorInvalidValue;
+  if (slots && nb > PCG_MAX_SLOT_BLOCKS) return hipErrorInvalidValue;
   const void *kfun = pcg_cg1_kernel(bs, R, slots);
   if (!kfun) return hipErrorInvalidValue;
   void *kargs[] = {&a};
