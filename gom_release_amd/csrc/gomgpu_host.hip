@@ -114,6 +114,9 @@ struct gomgpu_ctx {
   int kernel_calls[GOMGPU_NUM_KTIMERS];
   int *h_err = nullptr;  // pinned
   bool pcg_ran = false;  // a free-surface solve ran since the last sync
+  std::vector<gomgpu_batch *> batches;  // batches holding this context (a captured batch step
+                                        // holds its pointers)
+  unsigned stale = 0;  // SWAP_* pairs whose new array was exchanged and not yet rewritten
   double *h_stage = nullptr;  // pinned staging of the forcing arrays
   size_t stage_len = 0;
   hipEvent_t fev = nullptr;   // completion of the last forcing upload
@@ -467,8 +470,11 @@ fail:
   return rc ? rc : GOMGPU_E_HIP;
 }
 
+static void batch_forget(gomgpu_batch *b, gomgpu_ctx *c);
+
 extern "C" void gomgpu_destroy(gomgpu_ctx *c) {
   if (!c) return;
+  while (!c->batches.empty()) batch_forget(c->batches.back(), c);  // leaves its batches
   if (c->st) hipStreamSynchronize(c->st);
   for (void *p : c->allocs) hipFree(p);
   if (c->h_err) hipHostFree(c->h_err);
@@ -483,10 +489,42 @@ extern "C" void gomgpu_destroy(gomgpu_ctx *c) {
   delete c;
 }
 
+// New/old pairs that UPDATE may exchange instead of copying. After an exchange the new
+// array holds the values of two steps back until its writer runs (VELOCITY for the
+// velocities, TRANSPORT for the transported tracers); until then the pair is stale and means
+// new == old, as after the reference's copy. Whatever reads a stale new array first copies
+// old into it (materialize), so every entry point sees the reference's values.
+enum { SWAP_WN = 1, SWAP_UN = 2, SWAP_VN = 4, SWAP_SALT = 8, SWAP_TEMP = 16, SWAP_ALL = 31 };
+
+static void swap_pair(gomgpu_ctx *c, unsigned bit, double ***old_, double ***new_, size_t *n) {
+  const size_t E = c->d.E, J = c->d.J, L = c->d.L;
+  switch (bit) {
+    case SWAP_WN: *old_ = &c->s.wn_cell; *new_ = &c->s.wn_cell_new; *n = (L + 1) * E; break;
+    case SWAP_UN: *old_ = &c->s.un_face; *new_ = &c->s.un_face_new; *n = L * J; break;
+    case SWAP_VN: *old_ = &c->s.vn_face; *new_ = &c->s.vn_face_new; *n = L * J; break;
+    case SWAP_SALT: *old_ = &c->s.salt_cell; *new_ = &c->s.salt_cell_new; *n = L * E; break;
+    default: *old_ = &c->s.temp_cell; *new_ = &c->s.temp_cell_new; *n = L * E; break;
+  }
+}
+
+static int materialize(gomgpu_ctx *c, unsigned mask) {
+  for (unsigned bit = 1; bit <= SWAP_TEMP; bit <<= 1) {
+    if (!(c->stale & mask & bit)) continue;
+    double **o, **nw;
+    size_t n;
+    swap_pair(c, bit, &o, &nw, &n);
+    k_copy<<<nblk((long)n, 256), 256, 0, c->st>>>(*nw, *o, (int)n);
+    HIPCHK(c, hipGetLastError());
+    c->stale &= ~bit;
+  }
+  return 0;
+}
+
 extern "C" int gomgpu_set_state(gomgpu_ctx *c, const gomgpu_state *st) {
   if (!c || !st) return GOMGPU_E_ARG;
   const gomgpu_params &p = c->p;
   int rc;
+  if ((rc = materialize(c, SWAP_ALL))) return rc;  // fields not given keep new == old
 #define UP_S(name, ext)                                                              \
   if (st->name) {                                                                    \
     if ((rc = upload(c, c->s.name, st->name, shape_of(#ext, p)))) return rc;         \
@@ -501,6 +539,7 @@ extern "C" int gomgpu_get_state(gomgpu_ctx *c, gomgpu_state *st) {
   if (!c || !st) return GOMGPU_E_ARG;
   const gomgpu_params &p = c->p;
   int rc;
+  if ((rc = materialize(c, SWAP_ALL))) return rc;
   HIPCHK(c, hipStreamSynchronize(c->st));
 #define DN_S(name, ext)                                                              \
   if (st->name) {                                                                    \
@@ -669,6 +708,11 @@ static int stage_impl(gomgpu_ctx *c, int id) {
   const int BS = LB(STREAM);
   hipStream_t st = c->st;
   int *err = c->w.err;
+  unsigned reads = 0;  // exchanged new arrays the stage reads or writes in part
+  if (id == GOMGPU_STAGE_LAYERS) reads = SWAP_WN;
+  if (id == GOMGPU_STAGE_NODEVEL || id == GOMGPU_STAGE_TURBULENCE) reads = SWAP_UN | SWAP_VN;
+  if (id == GOMGPU_STAGE_TRANSPORT) reads = SWAP_UN | SWAP_WN;
+  if (int rc = materialize(c, reads)) return rc;
   switch (id) {
     case GOMGPU_STAGE_FRICTION:
       k_friction<<<nblk(d.J, BS), BS, 0, st>>>(c->m, c->s, d, p.bf_flag, p.dry_depth, p.von_Karman);
@@ -709,6 +753,7 @@ static int stage_impl(gomgpu_ctx *c, int id) {
     case GOMGPU_STAGE_VELOCITY:
       k_velocity_face<<<nblk(d.J, BS), BS, 0, st>>>(c->m, c->s, c->f, d, p.theta * p.gravity * p.dt);
       k_velocity_w<<<nblk(d.E, BS), BS, 0, st>>>(c->m, c->s, d);
+      c->stale &= ~(unsigned)(SWAP_UN | SWAP_VN | SWAP_WN);  // rewritten in full
       break;
     case GOMGPU_STAGE_LAYERS:
       k_layers_cell<<<nblk(d.E, BS), BS, 0, st>>>(c->m, c->s, d, p.MSL, p.dry_depth, err);
@@ -757,6 +802,7 @@ static int stage_impl(gomgpu_ctx *c, int id) {
       k_tvd_face_out<<<nblk((long)d.J * d.L, BS), BS, 0, st>>>(c->m, c->s, c->w, d);
       long nmax = (long)d.L * (d.E > d.P ? d.E : d.P);
       k_tvd_cellnode_out<<<nblk(nmax, BS), BS, 0, st>>>(c->s, c->w, d);
+      for (int t = 0; t < d.NT; t++) c->stale &= ~(unsigned)(d.tid[t] == 1 ? SWAP_SALT : SWAP_TEMP);
       break;
     }
     case GOMGPU_STAGE_EOS:
@@ -766,20 +812,36 @@ static int stage_impl(gomgpu_ctx *c, int id) {
       k_eos_cell<<<nblk(d.E, BS), BS, 0, st>>>(c->m, c->s, d, err);
       break;
     case GOMGPU_STAGE_UPDATE: {
-      // update_variables.f90:16-31
+      // update_variables.f90:16-31: new -> old. A pair whose new array every step rewrites in
+      // full before reading it -- wn_cell (k_velocity_w), un_face and vn_face
+      // (k_velocity_face), and each transported tracer (k_tvd_cellnode_out) -- exchanges
+      // its two pointers instead of copying (the pair is then stale, see materialize). The
+      // others are copied: eta_cell_new is the next solve's initial guess, and the layer
+      // kernels write dz and dzhalf over the wet range only. Contexts in a batch copy every
+      // pair, since a captured batch step holds their pointers.
+      if (int rc = materialize(c, SWAP_ALL)) return rc;
       size_t E = d.E, J = d.J, P = d.P, L = d.L;
+      const bool swap = c->batches.empty();
+      unsigned swappable = SWAP_WN | SWAP_UN | SWAP_VN;
+      for (int t = 0; p.transport_flag == 1 && t < d.NT; t++) swappable |= d.tid[t] == 1 ? SWAP_SALT : SWAP_TEMP;
+      for (unsigned bit = 1; bit <= SWAP_TEMP; bit <<= 1) {
+        double **o, **nw;
+        size_t n;
+        swap_pair(c, bit, &o, &nw, &n);
+        if (swap && (swappable & bit)) {
+          std::swap(*o, *nw);
+          c->stale |= bit;
+        } else {
+          k_copy<<<nblk((long)n, BS), BS, 0, st>>>(*o, *nw, (int)n);
+        }
+      }
       struct {
         double *dst;
         const double *src;
         size_t n;
       } cp[] = {{c->s.eta_cell, c->s.eta_cell_new, E},
-                {c->s.wn_cell, c->s.wn_cell_new, (L + 1) * E},
                 {c->s.dz_cell, c->s.dz_cell_new, L * E},
                 {c->s.dzhalf_cell, c->s.dzhalf_cell_new, (L + 1) * E},
-                {c->s.salt_cell, c->s.salt_cell_new, L * E},
-                {c->s.temp_cell, c->s.temp_cell_new, L * E},
-                {c->s.un_face, c->s.un_face_new, L * J},
-                {c->s.vn_face, c->s.vn_face_new, L * J},
                 {c->s.dz_face, c->s.dz_face_new, L * J},
                 {c->s.dzhalf_face, c->s.dzhalf_face_new, (L + 1) * J},
                 {c->s.dz_node, c->s.dz_node_new, L * P},
@@ -1128,21 +1190,36 @@ extern "C" int gomgpu_batch_add(gomgpu_batch *b, gomgpu_ctx *c) {
   hipEvent_t ev;
   BHIP(b, hipEventCreateWithFlags(&ev, hipEventDisableTiming));
   c->timing = false;
+  c->batches.push_back(b);
   b->members.push_back(c);
   b->ev_join.push_back(ev);
   b->dirty = true;
   return 0;
 }
 
-extern "C" int gomgpu_batch_remove(gomgpu_batch *b, gomgpu_ctx *c) {
-  if (!b || !c) return GOMGPU_E_ARG;
+// Drops c from b (after b's queued work): the next batch step plans and captures anew.
+static void batch_forget(gomgpu_batch *b, gomgpu_ctx *c) {
+  if (b->st) hipStreamSynchronize(b->st);
   for (size_t i = 0; i < b->members.size(); i++)
     if (b->members[i] == c) {
-      BHIP(b, hipStreamSynchronize(b->st));
       hipEventDestroy(b->ev_join[i]);
       b->members.erase(b->members.begin() + i);
       b->ev_join.erase(b->ev_join.begin() + i);
       b->dirty = true;
+      break;
+    }
+  for (size_t i = 0; i < c->batches.size(); i++)
+    if (c->batches[i] == b) {
+      c->batches.erase(c->batches.begin() + i);
+      break;
+    }
+}
+
+extern "C" int gomgpu_batch_remove(gomgpu_batch *b, gomgpu_ctx *c) {
+  if (!b || !c) return GOMGPU_E_ARG;
+  for (size_t i = 0; i < b->members.size(); i++)
+    if (b->members[i] == c) {
+      batch_forget(b, c);
       return 0;
     }
   return bseterr(b, GOMGPU_E_ARG, "context not in the batch");
@@ -1199,6 +1276,12 @@ extern "C" void gomgpu_batch_destroy(gomgpu_batch *b) {
   if (!b) return;
   if (b->st) hipStreamSynchronize(b->st);
   batch_free_plan(b);
+  for (gomgpu_ctx *c : b->members)
+    for (size_t i = 0; i < c->batches.size(); i++)
+      if (c->batches[i] == b) {
+        c->batches.erase(c->batches.begin() + i);
+        break;
+      }
   for (hipEvent_t e : b->ev_join) hipEventDestroy(e);
   if (b->ev_fork) hipEventDestroy(b->ev_fork);
   if (b->ev_done) hipEventDestroy(b->ev_done);

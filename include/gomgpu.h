@@ -286,6 +286,8 @@ long long gomgpu_footprint(const gomgpu_params *params);
 long long gomgpu_device_bytes(gomgpu_ctx *ctx);
 
 typedef struct gomgpu_batch gomgpu_batch;
+/* Batched contexts: one call steps every member. Destroying a member context removes it from
+ * its batches; gomgpu_batch_destroy leaves the members usable on their own. */
 int gomgpu_batch_create(gomgpu_batch **out);
 int gomgpu_batch_add(gomgpu_batch *batch, gomgpu_ctx *member);
 int gomgpu_batch_remove(gomgpu_batch *batch, gomgpu_ctx *member);
