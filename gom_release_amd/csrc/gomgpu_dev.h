@@ -72,7 +72,7 @@ struct DevWork {
   double *con, *con_new;  // con (0:L+1, E, NT) and con_new (L, E, NT), layer-major
   double *Qj, *dj, *phij; // (L, J)
   double *Qi, *di, *phii; // (0:L, E)
-  double *rj, *ri;        // (L, J, NT), (0:L, E, NT)
+  double *htop;           // (2, E, NT) horizontal upwind sums of each column's top layer
   double *PSIj, *PSIi;
   double *s1a, *s2a;      // (L, E, NT) limiter sums per cell-layer
   double *obk, *qbk;      // (L, OB, NT), (L, QB, NT)

@@ -37,9 +37,8 @@ __global__ void k_tvd_init(DevMesh, DevState, DevForcing, DevWork, DevDims, TvdA
 __global__ void k_tvd_bnd(DevMesh, DevState, DevForcing, DevWork, DevDims, TvdArgs);
 __global__ void k_tvd_coef_face(DevMesh, DevState, DevWork, DevDims, TvdArgs);
 __global__ void k_tvd_coef_cell(DevMesh, DevState, DevWork, DevDims, TvdArgs);
-__global__ void k_tvd_lim_cell(DevMesh, DevState, DevWork, DevDims);
+__global__ void k_tvd_lim_cell(DevMesh, DevState, DevWork, DevDims, int);
 __global__ void k_tvd_lim_face(DevMesh, DevState, DevWork, DevDims, int);
-__global__ void k_tvd_psi_cell(DevMesh, DevState, DevWork, DevDims, int);
 __global__ void k_tvd_rhs(DevMesh, DevState, DevWork, DevDims, TvdArgs);
 __global__ void k_tvd_solve(DevMesh, DevState, DevWork, DevDims, TvdArgs, int *);
 __global__ void k_tvd_copyback(DevMesh, DevState, DevWork, DevDims);
@@ -413,8 +412,8 @@ extern "C" int gomgpu_create(const gomgpu_mesh *mesh, const gomgpu_params *param
     if ((rc = dalloc(c, &w.con, NT * (L + 2) * E)) || (rc = dalloc(c, &w.con_new, NT * L * E)) ||
         (rc = dalloc(c, &w.Qj, L * J)) || (rc = dalloc(c, &w.dj, L * J)) || (rc = dalloc(c, &w.phij, L * J)) ||
         (rc = dalloc(c, &w.Qi, (L + 1) * E)) || (rc = dalloc(c, &w.di, (L + 1) * E)) ||
-        (rc = dalloc(c, &w.phii, (L + 1) * E)) || (rc = dalloc(c, &w.rj, NT * L * J)) ||
-        (rc = dalloc(c, &w.ri, NT * (L + 1) * E)) || (rc = dalloc(c, &w.PSIj, NT * L * J)) ||
+        (rc = dalloc(c, &w.phii, (L + 1) * E)) || (rc = dalloc(c, &w.htop, NT * 2 * E)) ||
+        (rc = dalloc(c, &w.PSIj, NT * L * J)) ||
         (rc = dalloc(c, &w.PSIi, NT * (L + 1) * E)) || (rc = dalloc(c, &w.s1a, NT * L * E)) ||
         (rc = dalloc(c, &w.s2a, NT * L * E)) || (rc = dalloc(c, &w.obk, NT * L * OB)) ||
         (rc = dalloc(c, &w.qbk, NT * L * QB)) || (rc = dalloc(c, &w.node_new, NT * L * P)) ||
@@ -788,9 +787,9 @@ static int stage_impl(gomgpu_ctx *c, int id) {
         k_tvd_coef_face<<<nblk((long)d.J * d.L, BS), BS, 0, st>>>(c->m, c->s, c->w, d, a);
         k_tvd_coef_cell<<<nblk((long)d.E * (d.L + 1), BS), BS, 0, st>>>(c->m, c->s, c->w, d, a);
         if (p.h_flux_limiter > 0 || p.v_flux_limiter > 0) {
-          k_tvd_lim_cell<<<nblk((long)(d.L + 1) * d.E, LB(TVD_LIM)), LB(TVD_LIM), 0, st>>>(c->m, c->s, c->w, d);
+          k_tvd_lim_cell<<<nblk((long)(d.L + 1) * d.E, LB(TVD_LIM)), LB(TVD_LIM), 0, st>>>(c->m, c->s, c->w, d,
+                                                                                         p.h_flux_limiter);
           k_tvd_lim_face<<<nblk((long)d.J * d.L, BS), BS, 0, st>>>(c->m, c->s, c->w, d, p.h_flux_limiter);
-          k_tvd_psi_cell<<<nblk((long)d.E * (d.L + 1), BS), BS, 0, st>>>(c->m, c->s, c->w, d, p.h_flux_limiter);
         }
         KT_BEGIN(KT_TVD_COLUMN);
         k_tvd_rhs<<<nblk((long)d.L * d.E, LB(TVD_RHS)), LB(TVD_RHS), 0, st>>>(c->m, c->s, c->w, d, a);
@@ -1342,7 +1341,7 @@ extern "C" long long gomgpu_footprint(const gomgpu_params *params) {
   add(NT * (L + 2) * E, D); add(NT * L * E, D);             // con, con_new
   add(L * J, D); add(L * J, D); add(L * J, D);              // Qj, dj, phij
   add((L + 1) * E, D); add((L + 1) * E, D); add((L + 1) * E, D);  // Qi, di, phii
-  add(NT * L * J, D); add(NT * (L + 1) * E, D);             // rj, ri
+  add(NT * 2 * E, D);                                       // htop
   add(NT * L * J, D); add(NT * (L + 1) * E, D);             // PSIj, PSIi
   add(NT * L * E, D); add(NT * L * E, D);                   // s1a, s2a
   add(NT * L * OB, D); add(NT * L * QB, D);                 // obk, qbk

@@ -100,27 +100,6 @@ __global__ void k_tvd_coef_face(DevMesh m, DevState s, DevWork w, DevDims d, Tvd
   AL(w.phij, k, j, J) = phi;
 }
 
-// Cell (vertical) coefficients (solve_transport_equation_v19.f90:586-609)
-__global__ void k_tvd_coef_cell(DevMesh m, DevState s, DevWork w, DevDims d, TvdArgs a) {
-  int q = blockIdx.x * blockDim.x + threadIdx.x;
-  const int E = d.E;
-  if (q >= E * (d.L + 1)) return;
-  int e = q % E, k = q / E;
-  double Q = 0.0, dd = 0.0, phi = 0.0;
-  if (k >= m.bottom_layer_at_element[e] && k <= s.top_layer_at_element[e] - 1) {
-    double ar = m.area[e];
-    Q = ar * ((1.0 - a.th) * A0(s.wn_cell, k, e, E) + a.th * A0(s.wn_cell_new, k, e, E));
-    double dh = A0(s.dzhalf_cell, k, e, E);
-    double dzo = dh + (A0(s.dzhalf_cell_new, k, e, E) - dh) * a.sub / a.Nt;
-    double D = ar * A0(s.Kv, k, e, E) / dzo;
-    dd = dmaxf(0.0, D - 0.5 * fabs(Q));
-    phi = (Q == 0.0) ? 1.0 : dminf(1.0, 2.0 * D / fabs(Q));
-  }
-  A0(w.Qi, k, e, E) = Q;
-  A0(w.di, k, e, E) = dd;
-  A0(w.phii, k, e, E) = phi;
-}
-
 // Concentration across face slot l of cell e at layer k (boundary value for no neighbour).
 __device__ __forceinline__ double con_across(const DevMesh &m, const DevWork &w, const DevDims &d, int e,
                                              int ii, int k, int t) {
@@ -172,19 +151,50 @@ __device__ __forceinline__ void lim_hsums(const DevMesh &m, const DevWork &w, co
   }
 }
 
+// Cell (vertical) coefficients (solve_transport_equation_v19.f90:586-609). With a flux
+// limiter, the thread of a wet column's top layer also leaves that layer's horizontal upwind
+// sums in w.htop, which every vertical limiter ratio of the column uses (k_tvd_lim_cell).
+__global__ void k_tvd_coef_cell(DevMesh m, DevState s, DevWork w, DevDims d, TvdArgs a) {
+  int q = blockIdx.x * blockDim.x + threadIdx.x;
+  const int E = d.E;
+  if (q >= E * (d.L + 1)) return;
+  int e = q % E, k = q / E;
+  double Q = 0.0, dd = 0.0, phi = 0.0;
+  if (k >= m.bottom_layer_at_element[e] && k <= s.top_layer_at_element[e] - 1) {
+    double ar = m.area[e];
+    Q = ar * ((1.0 - a.th) * A0(s.wn_cell, k, e, E) + a.th * A0(s.wn_cell_new, k, e, E));
+    double dh = A0(s.dzhalf_cell, k, e, E);
+    double dzo = dh + (A0(s.dzhalf_cell_new, k, e, E) - dh) * a.sub / a.Nt;
+    double D = ar * A0(s.Kv, k, e, E) / dzo;
+    dd = dmaxf(0.0, D - 0.5 * fabs(Q));
+    phi = (Q == 0.0) ? 1.0 : dminf(1.0, 2.0 * D / fabs(Q));
+  }
+  A0(w.Qi, k, e, E) = Q;
+  A0(w.di, k, e, E) = dd;
+  A0(w.phii, k, e, E) = phi;
+  if ((a.hlim > 0 || a.vlim > 0) && k >= 1 && k == s.top_layer_at_element[e]) {
+    double sum1[2], sum2[2];
+    lim_hsums(m, w, d, e, k, sum1, sum2);
+    for (int t = 0; t < d.NT; t++) {
+      w.htop[(size_t)(2 * t) * E + e] = sum1[t];
+      w.htop[(size_t)(2 * t + 1) * E + e] = sum2[t];
+    }
+  }
+}
+
 // Limiter ratios (solve_transport_equation_v19.f90:624-842), one thread per (level, cell):
 // the per-layer upwind sums s1a, s2a that the face ratios r_jk need (layer k), and the
-// vertical ratio r_ik (level k). As in the reference, the vertical ratios reuse the
-// horizontal sums left over from the column's top layer; each thread recomputes those in
-// the same order.
-__global__ void k_tvd_lim_cell(DevMesh m, DevState s, DevWork w, DevDims d) {
+// vertical ratio r_ik (level k) with its limiter PSI_ik (level k). As in the reference, the
+// vertical ratios reuse the horizontal sums left over from the column's top layer, which
+// k_tvd_coef_cell computed once per column (w.htop) in the same order.
+__global__ void k_tvd_lim_cell(DevMesh m, DevState s, DevWork w, DevDims d, int hlim) {
   long q = (long)blockIdx.x * blockDim.x + threadIdx.x;
   const int E = d.E, L = d.L, L1 = d.L + 1, L2 = d.L + 2;
   if (q >= (long)L1 * E) return;
   int e = (int)(q % E), k = (int)(q / E);
   int tl = s.top_layer_at_element[e], bl = m.bottom_layer_at_element[e];
   if (tl == 0 || k < bl - 1 || k > tl) {
-    for (int t = 0; t < d.NT; t++) w.ri[zix(t, k, e, L1, E)] = 1.0;
+    for (int t = 0; t < d.NT; t++) w.PSIi[zix(t, k, e, L1, E)] = 0.0;
     return;
   }
   int nv = tl - bl + 1;
@@ -217,13 +227,16 @@ __global__ void k_tvd_lim_cell(DevMesh m, DevState s, DevWork w, DevDims d) {
       w.s2a[nix(t, k, e, L, E)] = s2a[t] + sum2[t];
     }
   }
-  // r_ik of level k: computed for bl <= k <= tl - 1, 1.0 at bl - 1 and tl
+  // r_ik of level k: computed for bl <= k <= tl - 1 (1.0 at bl - 1 and tl, where PSI_ik is 0)
   if (k == tl || k == bl - 1) {
-    for (int t = 0; t < d.NT; t++) w.ri[zix(t, k, e, L1, E)] = 1.0;
+    for (int t = 0; t < d.NT; t++) w.PSIi[zix(t, k, e, L1, E)] = 0.0;
     return;
   }
   double sum1[2], sum2[2];
-  lim_hsums(m, w, d, e, tl, sum1, sum2);
+  for (int t = 0; t < d.NT; t++) {
+    sum1[t] = w.htop[(size_t)(2 * t) * E + e];
+    sum2[t] = w.htop[(size_t)(2 * t + 1) * E + e];
+  }
   for (int t = 0; t < d.NT; t++) {
     double r;
     if (A0(w.Qi, tl, e, E) >= 0.0) {
@@ -242,7 +255,7 @@ __global__ void k_tvd_lim_cell(DevMesh m, DevState s, DevWork w, DevDims d) {
       double rt = (w.con[cix(t, kk - 1, e, L2, E)] - c0) * rt2;
       r = (rt == 0.0) ? 1.0 : rt1 / rt;
     }
-    w.ri[zix(t, k, e, L1, E)] = r;
+    w.PSIi[zix(t, k, e, L1, E)] = hlim != 0 ? limiter(hlim, r, A0(w.phii, k, e, E)) : 0.0;
   }
 }
 
@@ -275,20 +288,8 @@ __global__ void k_tvd_lim_face(DevMesh m, DevState s, DevWork w, DevDims d, int 
     }
   }
   bool in = (k >= m.bottom_layer_at_face[j] && k <= s.top_layer_at_face[j]);
-  for (int t = 0; t < d.NT; t++) {
-    w.rj[nix(t, k, j, L, J)] = r[t];
-    w.PSIj[nix(t, k, j, L, J)] = (in && hlim != 0) ? limiter(hlim, r[t], AL(w.phij, k, j, J)) : 0.0;
-  }
-}
-
-__global__ void k_tvd_psi_cell(DevMesh m, DevState s, DevWork w, DevDims d, int hlim) {
-  int q = blockIdx.x * blockDim.x + threadIdx.x;
-  const int E = d.E, L1 = d.L + 1;
-  if (q >= E * L1) return;
-  int e = q % E, k = q / E;
-  bool in = (k >= m.bottom_layer_at_element[e] && k <= s.top_layer_at_element[e] - 1);
   for (int t = 0; t < d.NT; t++)
-    w.PSIi[zix(t, k, e, L1, E)] = (in && hlim != 0) ? limiter(hlim, w.ri[zix(t, k, e, L1, E)], A0(w.phii, k, e, E)) : 0.0;
+    w.PSIj[nix(t, k, j, L, J)] = (in && hlim != 0) ? limiter(hlim, r[t], AL(w.phij, k, j, J)) : 0.0;
 }
 
 // Per-column implicit update (solve_transport_equation_v19.f90:921-1303), in two passes.
