@@ -737,7 +737,9 @@ static int stage_impl(gomgpu_ctx *c, int id) {
       MomParams q{p.dt, p.gravity, p.theta, p.rho_o, p.smagorinsky_parameter, p.Ah_0, p.Kh_0, p.baroclinic_flag};
       KT_BEGIN(KT_MOMENTUM);
       k_cell_uv<<<nblk((long)d.L * d.E, BS), BS, 0, st>>>(c->m, c->s, c->w, d);
-      k_mom_rhs<<<nblk((long)d.L * d.J, LB(MOM_RHS)), LB(MOM_RHS), 0, st>>>(c->m, c->s, c->f, c->w, d, q);
+      // 64 faces per block, all their rows; LDS for the baroclinic products of every layer
+      k_mom_rhs<<<nblk(d.J, 64), LB(MOM_RHS), (size_t)2 * d.L * 64 * sizeof(double), st>>>(c->m, c->s, c->f, c->w,
+                                                                                          d, q);
       k_mom_solve<<<nblk(d.J, LB(MOM_SOLVE)), LB(MOM_SOLVE), 0, st>>>(c->m, c->s, c->f, c->w, d, q, err);
       KT_END(KT_MOMENTUM);
       break;
@@ -1359,7 +1361,9 @@ extern "C" long long gomgpu_footprint(const gomgpu_params *params) {
 extern "C" long long gomgpu_device_bytes(gomgpu_ctx *c) { return c ? (long long)c->bytes : -1; }
 
 // ------------------------------------------------------------------ launch shapes
-static int launch_max(int which) { return which == GOMGPU_LAUNCH_TURB_SOLVE ? 256 : 1024; }
+static int launch_max(int which) {
+  return which == GOMGPU_LAUNCH_TURB_SOLVE || which == GOMGPU_LAUNCH_MOM_RHS ? 256 : 1024;
+}
 
 extern "C" int gomgpu_set_launch(gomgpu_ctx *c, int which, int block) {
   if (!c || which < 0 || which >= GOMGPU_NUM_LAUNCH) return GOMGPU_E_ARG;

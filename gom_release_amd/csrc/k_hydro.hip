@@ -443,23 +443,62 @@ __global__ void k_cell_uv(DevMesh m, DevState s, DevWork w, DevDims d) {
   AL(w.cell_v, kk, ie, E) = vt;
 }
 
-__global__ void k_mom_rhs(DevMesh m, DevState s, DevForcing f, DevWork w, DevDims d, MomParams q) {
-  long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+// One block covers 64 consecutive faces (lane = face) and all their rows: wave v of the block
+// handles rows v+1, v+1+W, ... (W waves). The baroclinic sums (solve_momentum_equation.f90:
+// 400-460) of row k add, for the layers k2 = kk..top, (rho2 - rho1) * dz_face(k2) -- halved
+// at k2 = kk -- where the layer is wet on both sides. Those products depend on (face, k2)
+// only, so the block first forms them once per layer into LDS (zero where a side is dry:
+// adding +0.0 to a sum that starts at +0.0 changes nothing, and halving the product equals
+// halving the density difference first, both exact), and each row then adds them from LDS in
+// the reference's order: one gather per layer instead of one per (row, layer).
+__global__ void __launch_bounds__(256) k_mom_rhs(DevMesh m, DevState s, DevForcing f, DevWork w, DevDims d,
+                                                 MomParams q) {
+  extern __shared__ double bct[];  // [2][L][64] baroclinic products (cells, nodes)
   const int J = d.J, E = d.E, P = d.P, L = d.L;
-  if (t >= (long)L * J) return;
-  int j = (int)(t % J), k = (int)(t / J) + 1;
-  int kt = s.top_layer_at_face[j];
-  if (kt == 0) return;
-  int kb = m.bottom_layer_at_face[j];
-  int n = kt - kb + 1;
-  if (k > n) return;
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6, nw = blockDim.x >> 6;
+  const int j = blockIdx.x * 64 + lane;
+  const bool valid = j < J;
+  const int kt = valid ? s.top_layer_at_face[j] : 0;
+  const int kb = valid ? m.bottom_layer_at_face[j] : 1;
+  const int node1 = valid ? TB(m.nodenum_at_face, 1, j, J) - 1 : 0;
+  const int node2 = valid ? TB(m.nodenum_at_face, 2, j, J) - 1 : 0;
+  const int a1 = valid ? TB(m.adj_cellnum_at_face, 1, j, J) : 1;
+  const int a2 = valid ? TB(m.adj_cellnum_at_face, 2, j, J) : 0;
+  if (q.baroclinic == 1) {
+    int cb1 = 0, ct1 = -1, cb2 = 0, ct2c = -1, nb1 = 0, nt1 = -1, nb2 = 0, nt2 = -1;
+    if (valid && kt != 0) {
+      if (a2 != 0) {
+        cb1 = m.bottom_layer_at_element[a1 - 1];
+        ct1 = s.top_layer_at_element[a1 - 1];
+        cb2 = m.bottom_layer_at_element[a2 - 1];
+        ct2c = s.top_layer_at_element[a2 - 1];
+      }
+      nb1 = m.bottom_layer_at_node[node1];
+      nt1 = s.top_layer_at_node[node1];
+      nb2 = m.bottom_layer_at_node[node2];
+      nt2 = s.top_layer_at_node[node2];
+    }
+    for (int k2 = wv + 1; k2 <= L; k2 += nw) {
+      double t1 = 0.0, t2 = 0.0;
+      if (valid && kt != 0 && k2 >= kb && k2 <= kt) {
+        double dz2 = AL(s.dz_face, k2, j, J);
+        if (a2 != 0 && k2 >= cb1 && k2 <= ct1 && k2 >= cb2 && k2 <= ct2c)
+          t1 = (AL(s.rho_cell, k2, a2 - 1, E) - AL(s.rho_cell, k2, a1 - 1, E)) * dz2;
+        if (k2 >= nb1 && k2 <= nt1 && k2 >= nb2 && k2 <= nt2)
+          t2 = (AL(s.rho_node, k2, node2, P) - AL(s.rho_node, k2, node1, P)) * dz2;
+      }
+      bct[(k2 - 1) * 64 + lane] = t1;
+      bct[(L + k2 - 1) * 64 + lane] = t2;
+    }
+    __syncthreads();
+  }
+  if (!valid || kt == 0) return;
+  const int n = kt - kb + 1;
   const double dt = q.dt, g = q.g, th = q.th;
-  int node1 = TB(m.nodenum_at_face, 1, j, J) - 1, node2 = TB(m.nodenum_at_face, 2, j, J) - 1;
-  int a1 = TB(m.adj_cellnum_at_face, 1, j, J), a2 = TB(m.adj_cellnum_at_face, 2, j, J);
-  int bt = m.boundary_type_of_face[j];
-  double dj = m.delta_j[j], Lf = m.face_length[j];
-  double ct = m.cos_theta[j], st = m.sin_theta[j], ct2 = m.cos_theta2[j], st2 = m.sin_theta2[j];
-  double cor = m.coriolis_factor[j];
+  const int bt = m.boundary_type_of_face[j];
+  const double dj = m.delta_j[j], Lf = m.face_length[j];
+  const double ct = m.cos_theta[j], st = m.sin_theta[j], ct2 = m.cos_theta2[j], st2 = m.sin_theta2[j];
+  const double cor = m.coriolis_factor[j];
   // air pressure differences are layer independent
   double airc = 0.0;
   if (a2 != 0) {
@@ -472,116 +511,106 @@ __global__ void k_mom_rhs(DevMesh m, DevState s, DevForcing f, DevWork w, DevDim
     }
     airc = -(dt / q.rho_o * (tp[1] - tp[0]) / dj);
   }
-  double rtemp3a = -(dt / q.rho_o * ((f.air_p1[node2] - f.air_p1[node1]) / Lf));
-  bool wet_nodes = (s.wetdry_node[node1] == 0 && s.wetdry_node[node2] == 0);
-  double rtemp0 = 0.0, rtemp1, rtemp2 = 0.0, rtemp3, rtemp4;
-  int kk = kt - k + 1;
-  double dz = AL(s.dz_face, kk, j, J);
-  double unf = AL(s.un_face, kk, j, J), vnf = AL(s.vn_face, kk, j, J);
-  double G1 = dz * (AL(s.un_ELM, kk, j, J) + cor * dt * vnf);
-  double G2 = dz * (AL(s.vn_ELM, kk, j, J) - cor * dt * unf);
-  if (a2 != 0) {
-    double temp[13];
-    for (int l = 1; l <= 2; l++) {
-      int nd = TB(m.nodenum_at_face, l, j, J) - 1;
-      double ut = (A0(s.u_node, kk, nd, P) + A0(s.u_node, kk - 1, nd, P)) * 0.5;
-      double vt = (A0(s.v_node, kk, nd, P) + A0(s.v_node, kk - 1, nd, P)) * 0.5;
-      temp[2 * l - 1] = ut * ct + vt * st;
-      temp[2 * l] = -ut * st + vt * ct;
+  const double rtemp3a = -(dt / q.rho_o * ((f.air_p1[node2] - f.air_p1[node1]) / Lf));
+  const bool wet_nodes = (s.wetdry_node[node1] == 0 && s.wetdry_node[node2] == 0);
+  const double spinup_bc = f.scal[0];
+  const int ibnd = m.isflowside3[j];
+  for (int k = wv + 1; k <= n; k += nw) {
+    double rtemp0 = 0.0, rtemp1, rtemp2 = 0.0, rtemp3, rtemp4;
+    int kk = kt - k + 1;
+    double dz = AL(s.dz_face, kk, j, J);
+    double unf = AL(s.un_face, kk, j, J), vnf = AL(s.vn_face, kk, j, J);
+    double G1 = dz * (AL(s.un_ELM, kk, j, J) + cor * dt * vnf);
+    double G2 = dz * (AL(s.vn_ELM, kk, j, J) - cor * dt * unf);
+    if (a2 != 0) {
+      double temp[13];
+      for (int l = 1; l <= 2; l++) {
+        int nd = TB(m.nodenum_at_face, l, j, J) - 1;
+        double ut = (A0(s.u_node, kk, nd, P) + A0(s.u_node, kk - 1, nd, P)) * 0.5;
+        double vt = (A0(s.v_node, kk, nd, P) + A0(s.v_node, kk - 1, nd, P)) * 0.5;
+        temp[2 * l - 1] = ut * ct + vt * st;
+        temp[2 * l] = -ut * st + vt * ct;
+      }
+      for (int l = 1; l <= 2; l++) {
+        int ie = TB(m.adj_cellnum_at_face, l, j, J) - 1;
+        double ut = AL(w.cell_u, kk, ie, E), vt = AL(w.cell_v, kk, ie, E);
+        temp[2 * l + 3] = ut * ct + vt * st;
+        temp[2 * l + 4] = -ut * st + vt * ct;
+      }
+      double dudx = (temp[7] - temp[5]) / dj, dvdx = (temp[8] - temp[6]) / dj;
+      double dudy = (temp[3] - temp[1]) / Lf, dvdy = (temp[4] - temp[2]) / Lf;
+      double sx = dvdx + dudy;
+      double hdiff = q.smag * sqrt(dudx * dudx + dvdy * dvdy + 0.5 * (sx * sx));
+      double kh = hdiff * m.area[a1 - 1];
+      hdiff = hdiff + q.Ah0;
+      AL(s.Kh, k, j, J) = kh + q.Kh0;
+      double d2udy = 4.0 * (temp[3] + temp[1] - 2 * unf) / (Lf * Lf);
+      double d2vdy = 4.0 * (temp[4] + temp[2] - 2 * vnf) / (Lf * Lf);
+      for (int l = 1; l <= 2; l++) {
+        int ie = TB(m.adj_cellnum_at_face, l, j, J) - 1;
+        double ddx = m.x_cell[ie] - m.x_face[j], ddy = m.y_cell[ie] - m.y_face[j];
+        double rl = sqrt(ddx * ddx + ddy * ddy);
+        temp[2 * l + 7] = (temp[2 * l + 3] - unf) / rl;
+        temp[2 * l + 8] = (temp[2 * l + 4] - vnf) / rl;
+      }
+      double d2udx = 2.0 * (temp[9] + temp[11]) / dj;
+      double d2vdx = 2.0 * (temp[10] + temp[12]) / dj;
+      G1 = G1 + dz * dt * hdiff * (d2udx + d2udy);
+      G2 = G2 + dz * dt * hdiff * (d2vdx + d2vdy);
     }
-    for (int l = 1; l <= 2; l++) {
-      int ie = TB(m.adj_cellnum_at_face, l, j, J) - 1;
-      double ut = AL(w.cell_u, kk, ie, E), vt = AL(w.cell_v, kk, ie, E);
-      temp[2 * l + 3] = ut * ct + vt * st;
-      temp[2 * l + 4] = -ut * st + vt * ct;
+    if (bt == 0) {
+      rtemp0 = g * dt / dj * (1.0 - th) * (s.eta_cell[a2 - 1] - s.eta_cell[a1 - 1]);
+      rtemp1 = rtemp0 * ct2;
+      rtemp2 = rtemp0 * st2;
+      G1 = G1 - dz * rtemp1;
+    } else if (bt > 0) {
+      int ie = m.ob_element_flag[a1 - 1];
+      rtemp1 = g * dt / dj * (1.0 - th) * (f.eta_at_ob_old[ie - 1] - s.eta_cell[a1 - 1]);
+      G1 = G1 - dz * rtemp1;
+    } else if (bt == -1) {
+      G1 = 0.0;
     }
-    double dudx = (temp[7] - temp[5]) / dj, dvdx = (temp[8] - temp[6]) / dj;
-    double dudy = (temp[3] - temp[1]) / Lf, dvdy = (temp[4] - temp[2]) / Lf;
-    double sx = dvdx + dudy;
-    double hdiff = q.smag * sqrt(dudx * dudx + dvdy * dvdy + 0.5 * (sx * sx));
-    double kh = hdiff * m.area[a1 - 1];
-    hdiff = hdiff + q.Ah0;
-    AL(s.Kh, k, j, J) = kh + q.Kh0;
-    double d2udy = 4.0 * (temp[3] + temp[1] - 2 * unf) / (Lf * Lf);
-    double d2vdy = 4.0 * (temp[4] + temp[2] - 2 * vnf) / (Lf * Lf);
-    for (int l = 1; l <= 2; l++) {
-      int ie = TB(m.adj_cellnum_at_face, l, j, J) - 1;
-      double ddx = m.x_cell[ie] - m.x_face[j], ddy = m.y_cell[ie] - m.y_face[j];
-      double rl = sqrt(ddx * ddx + ddy * ddy);
-      temp[2 * l + 7] = (temp[2 * l + 3] - unf) / rl;
-      temp[2 * l + 8] = (temp[2 * l + 4] - vnf) / rl;
+    if (wet_nodes) {
+      rtemp0 = g * dt / Lf * (1.0 - th) * (s.eta_node[node2] - s.eta_node[node1]);
+      if (bt == 0)
+        G2 = G2 - dz * (rtemp0 + rtemp2);
+      else
+        G2 = G2 - dz * rtemp0;
+    } else if (a2 == 0) {
+      rtemp0 = 0.0;  // defined value for the reference's stale private variable
     }
-    double d2udx = 2.0 * (temp[9] + temp[11]) / dj;
-    double d2vdx = 2.0 * (temp[10] + temp[12]) / dj;
-    G1 = G1 + dz * dt * hdiff * (d2udx + d2udy);
-    G2 = G2 + dz * dt * hdiff * (d2vdx + d2vdy);
-  }
-  if (bt == 0) {
-    rtemp0 = g * dt / dj * (1.0 - th) * (s.eta_cell[a2 - 1] - s.eta_cell[a1 - 1]);
+    if (q.baroclinic == 1) {
+      double sum1 = 0.0, sum2 = 0.0;
+      for (int k2 = kk; k2 <= kt; k2++) {
+        double fct = (k2 == kk) ? 0.5 : 1.0;
+        sum1 = sum1 + fct * bct[(k2 - 1) * 64 + lane];
+        sum2 = sum2 + fct * bct[(L + k2 - 1) * 64 + lane];
+      }
+      double dzl = AL(s.dz_face, k, j, J);
+      rtemp1 = dzl * (-(g * dt / (q.rho_o * dj))) * sum1;
+      rtemp2 = dzl * (-(g * dt / (q.rho_o * Lf))) * sum2;
+      rtemp3 = rtemp1 * ct2;
+      rtemp4 = rtemp1 * st2;
+      G1 = G1 + rtemp3 * spinup_bc;
+      G2 = G2 + (rtemp2 + rtemp4) * spinup_bc;
+    }
+    if (k == 1) {
+      G1 = G1 + dt * f.wind_stress_normal[j];
+      G2 = G2 + dt * f.wind_stress_tangnt[j];
+    }
+    G1 = G1 + 0.0;  // equilibrium tide term (zero without tidal potential species)
+    G2 = G2 + 0.0;
+    if (a2 != 0) rtemp0 = airc;
     rtemp1 = rtemp0 * ct2;
     rtemp2 = rtemp0 * st2;
-    G1 = G1 - dz * rtemp1;
-  } else if (bt > 0) {
-    int ie = m.ob_element_flag[a1 - 1];
-    rtemp1 = g * dt / dj * (1.0 - th) * (f.eta_at_ob_old[ie - 1] - s.eta_cell[a1 - 1]);
-    G1 = G1 - dz * rtemp1;
-  } else if (bt == -1) {
-    G1 = 0.0;
+    G1 = G1 + dz * rtemp1;
+    rtemp3 = rtemp3a;
+    G2 = G2 + dz * (rtemp3 + rtemp2);
+    if (bt == -1) G1 = 0.0;
+    if (ibnd > 0) G1 = G1 + dz * f.Qu_boundary[ibnd - 1];
+    AL(s.AinvG1, k, j, J) = G1;
+    AL(s.AinvG2, k, j, J) = G2;
   }
-  if (wet_nodes) {
-    rtemp0 = g * dt / Lf * (1.0 - th) * (s.eta_node[node2] - s.eta_node[node1]);
-    if (bt == 0)
-      G2 = G2 - dz * (rtemp0 + rtemp2);
-    else
-      G2 = G2 - dz * rtemp0;
-  } else if (a2 == 0) {
-    rtemp0 = 0.0;  // defined value for the reference's stale private variable
-  }
-  if (q.baroclinic == 1) {
-    double sum1 = 0.0, sum2 = 0.0;
-    int cb1 = 0, ct1 = -1, cb2 = 0, ct2c = -1;
-    if (a2 != 0) {
-      cb1 = m.bottom_layer_at_element[a1 - 1];
-      ct1 = s.top_layer_at_element[a1 - 1];
-      cb2 = m.bottom_layer_at_element[a2 - 1];
-      ct2c = s.top_layer_at_element[a2 - 1];
-    }
-    int nb1 = m.bottom_layer_at_node[node1], nt1 = s.top_layer_at_node[node1];
-    int nb2 = m.bottom_layer_at_node[node2], nt2 = s.top_layer_at_node[node2];
-    for (int k2 = kk; k2 <= kt; k2++) {
-      double fct = (k2 == kk) ? 0.5 : 1.0;
-      double dz2 = AL(s.dz_face, k2, j, J);
-      if (a2 != 0 && k2 >= cb1 && k2 <= ct1 && k2 >= cb2 && k2 <= ct2c)
-        sum1 = sum1 + fct * (AL(s.rho_cell, k2, a2 - 1, E) - AL(s.rho_cell, k2, a1 - 1, E)) * dz2;
-      if (k2 >= nb1 && k2 <= nt1 && k2 >= nb2 && k2 <= nt2)
-        sum2 = sum2 + fct * (AL(s.rho_node, k2, node2, P) - AL(s.rho_node, k2, node1, P)) * dz2;
-    }
-    double dzl = AL(s.dz_face, k, j, J);
-    rtemp1 = dzl * (-(g * dt / (q.rho_o * dj))) * sum1;
-    rtemp2 = dzl * (-(g * dt / (q.rho_o * Lf))) * sum2;
-    rtemp3 = rtemp1 * ct2;
-    rtemp4 = rtemp1 * st2;
-    const double spinup_bc = f.scal[0];
-    G1 = G1 + rtemp3 * spinup_bc;
-    G2 = G2 + (rtemp2 + rtemp4) * spinup_bc;
-  }
-  if (k == 1) {
-    G1 = G1 + dt * f.wind_stress_normal[j];
-    G2 = G2 + dt * f.wind_stress_tangnt[j];
-  }
-  G1 = G1 + 0.0;  // equilibrium tide term (zero without tidal potential species)
-  G2 = G2 + 0.0;
-  if (a2 != 0) rtemp0 = airc;
-  rtemp1 = rtemp0 * ct2;
-  rtemp2 = rtemp0 * st2;
-  G1 = G1 + dz * rtemp1;
-  rtemp3 = rtemp3a;
-  G2 = G2 + dz * (rtemp3 + rtemp2);
-  if (bt == -1) G1 = 0.0;
-  int ibnd = m.isflowside3[j];
-  if (ibnd > 0) G1 = G1 + dz * f.Qu_boundary[ibnd - 1];
-  AL(s.AinvG1, k, j, J) = G1;
-  AL(s.AinvG2, k, j, J) = G2;
 }
 
 // Tridiagonal row r (0-based, surface first) of face j: solve_momentum_equation.f90:120-169.
