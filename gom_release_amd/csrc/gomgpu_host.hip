@@ -354,6 +354,33 @@ extern "C" int gomgpu_create(const gomgpu_mesh *mesh, const gomgpu_params *param
     NS_UP(weight, ns_weight, double, dp)
 #undef NS_UP
   }
+  {
+    // cell records of the ELM walk: node coordinates and ids in slot order, 4 per cell
+    const long E = p.maxele;
+    if (!mesh->nodenum_at_cell || !mesh->x_node || !mesh->y_node) {
+      rc = seterr(c, GOMGPU_E_ARG, "mesh connectivity missing");
+      goto fail;
+    }
+    std::vector<double> cx(4 * E, 0.0), cy(4 * E, 0.0);
+    std::vector<int32_t> cn(4 * E, 0);
+    for (long e = 0; e < E; e++) {
+      int N = mesh->tri_or_quad[e];
+      for (int l = 0; l < N && l < 4; l++) {
+        int nd = mesh->nodenum_at_cell[l + e * 4];
+        cn[4 * e + l] = nd;
+        cx[4 * e + l] = mesh->x_node[nd - 1];
+        cy[4 * e + l] = mesh->y_node[nd - 1];
+      }
+    }
+    int32_t *ip;
+    double *dp;
+    if ((rc = dalloc(c, &dp, cx.size())) || (rc = upload(c, dp, cx.data(), Shape{1, (long)cx.size()}))) goto fail;
+    c->m.cg_x = dp;
+    if ((rc = dalloc(c, &dp, cy.size())) || (rc = upload(c, dp, cy.data(), Shape{1, (long)cy.size()}))) goto fail;
+    c->m.cg_y = dp;
+    if ((rc = dalloc(c, &ip, cn.size())) || (rc = upload(c, ip, cn.data(), Shape{1, (long)cn.size()}))) goto fail;
+    c->m.cg_node = ip;
+  }
   // state
 #define AL_SR(name, ext)                                                             \
   {                                                                                  \
@@ -1322,6 +1349,8 @@ extern "C" long long gomgpu_footprint(const gomgpu_params *params) {
     add(2 * NB * P, sizeof(double)); add(2 * NB * P, sizeof(double));
     add(NB * P, sizeof(double)); add(P, sizeof(double));
   }
+  add(4 * (size_t)p.maxele, sizeof(double)); add(4 * (size_t)p.maxele, sizeof(double));  // ELM cell records
+  add(4 * (size_t)p.maxele, sizeof(int32_t));
   GOMGPU_STATE_REAL(FP_R)
   GOMGPU_STATE_INT(FP_I)
   GOMGPU_FORCING_REAL(FP_R)

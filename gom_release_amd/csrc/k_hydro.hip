@@ -163,9 +163,22 @@ struct ElmCtx {
   double MSL;
 };
 
+// One visited cell of the walk: its node count and its cell record (the node coordinates and
+// ids, the values of x_node, y_node and nodenum_at_cell in slot order), read where used.
+struct CellRec {
+  int o;  // 4 * cell: the record's first slot
+  int N;
+};
+
+__device__ __forceinline__ void cell_rec(const DevMesh &m, int e, CellRec &r) {
+  r.N = m.tri_or_quad[e];
+  r.o = 4 * e;
+}
+
 // search_straight_line.f90:15-399 (mode 1): straight-line walk from (x0,y0) to (xt,yt).
-__device__ int search_line(const ElmCtx &c, int &nnel, int &jlev, double time, double x0, double y0,
-                           double z0, double &xt, double &yt, double &zt, int &nfl) {
+// On return rc holds the record of the end cell nnel.
+__device__ __forceinline__ int search_line(const ElmCtx &c, int &nnel, int &jlev, double time, double x0, double y0,
+                           double z0, double &xt, double &yt, double &zt, int &nfl, CellRec &rc) {
   const DevMesh &m = c.m;
   const DevState &s = c.s;
   const int E = c.d.E, J = c.d.J, L = c.d.L;
@@ -176,11 +189,12 @@ __device__ int search_line(const ElmCtx &c, int &nnel, int &jlev, double time, d
   nfl = 0;
   double trm = time;
   double area1 = 0.0, area2 = 0.0;
-  int N = m.tri_or_quad[e0];
+  cell_rec(m, e0, rc);
+  int N = rc.N;
   for (int l = 1; l <= N; l++) {
-    int n1 = TB(m.nodenum_at_cell, l, e0, E) - 1, n2 = TB(m.nodenum_at_cell, sen(N, l, 1), e0, E) - 1;
-    area1 = area1 + fabs(carea(m.x_node[n1], m.x_node[n2], x0, m.y_node[n1], m.y_node[n2], y0));
-    area2 = area2 + fabs(carea(m.x_node[n1], m.x_node[n2], xt, m.y_node[n1], m.y_node[n2], yt));
+    int n1 = l - 1, n2 = sen(N, l, 1) - 1;
+    area1 = area1 + fabs(carea(m.cg_x[rc.o + n1], m.cg_x[rc.o + n2], x0, m.cg_y[rc.o + n1], m.cg_y[rc.o + n2], y0));
+    area2 = area2 + fabs(carea(m.cg_x[rc.o + n1], m.cg_x[rc.o + n2], xt, m.cg_y[rc.o + n1], m.cg_y[rc.o + n2], yt));
   }
   double ae = fabs(area1 - m.area[e0]) / m.area[e0];
   if (ae > small06) return GOMGPU_ERR_ELM_NOT_IN_CELL;
@@ -195,8 +209,8 @@ __device__ int search_line(const ElmCtx &c, int &nnel, int &jlev, double time, d
     int iflag = 0, nel_j = 0;
     double xin = 0.0, yin = 0.0;
     for (int l = 1; l <= N; l++) {
-      int jd1 = TB(m.nodenum_at_cell, sen(N, l, 1), e0, E) - 1, jd2 = TB(m.nodenum_at_cell, sen(N, l, 2), e0, E) - 1;
-      iflag = check_isect(xcg, xt, m.x_node[jd1], m.x_node[jd2], ycg, yt, m.y_node[jd1], m.y_node[jd2], xin, yin);
+      int jd1 = sen(N, l, 1) - 1, jd2 = sen(N, l, 2) - 1;
+      iflag = check_isect(xcg, xt, m.cg_x[rc.o + jd1], m.cg_x[rc.o + jd2], ycg, yt, m.cg_y[rc.o + jd1], m.cg_y[rc.o + jd2], xin, yin);
       if (iflag == 1) {
         nel_j = l;
         break;
@@ -216,9 +230,9 @@ __device__ int search_line(const ElmCtx &c, int &nnel, int &jlev, double time, d
         break;
       }
       e0 = nel - 1;
-      N = m.tri_or_quad[e0];
-      int md1 = TB(m.nodenum_at_cell, sen(N, nel_j, 1), e0, E);
-      int md2 = TB(m.nodenum_at_cell, sen(N, nel_j, 2), e0, E);
+      N = rc.N;
+      int md1 = m.cg_node[rc.o + sen(N, nel_j, 1) - 1];
+      int md2 = m.cg_node[rc.o + sen(N, nel_j, 2) - 1];
       double dist = sqrt((xin - xt) * (xin - xt) + (yin - yt) * (yin - yt));
       if (dist / pathl > 1.0 + 1.0e-4) return GOMGPU_ERR_ELM_OVERSHOT;
       zin = zt - dist / pathl * (zt - zin);
@@ -243,11 +257,12 @@ __device__ int search_line(const ElmCtx &c, int &nnel, int &jlev, double time, d
       }
       nel = adj;
       e0 = nel - 1;
-      N = m.tri_or_quad[e0];
+      cell_rec(m, e0, rc);
+      N = rc.N;
       area1 = 0.0;
       for (int l = 1; l <= N; l++) {
-        int k1 = TB(m.nodenum_at_cell, l, e0, E) - 1, k2 = TB(m.nodenum_at_cell, sen(N, l, 1), e0, E) - 1;
-        area1 = area1 + fabs(carea(m.x_node[k1], m.x_node[k2], xt, m.y_node[k1], m.y_node[k2], yt));
+        int k1 = l - 1, k2 = sen(N, l, 1) - 1;
+        area1 = area1 + fabs(carea(m.cg_x[rc.o + k1], m.cg_x[rc.o + k2], xt, m.cg_y[rc.o + k1], m.cg_y[rc.o + k2], yt));
       }
       ae = fabs(area1 - m.area[e0]) / m.area[e0];
       if (ae < small06) {
@@ -256,10 +271,10 @@ __device__ int search_line(const ElmCtx &c, int &nnel, int &jlev, double time, d
       }
       int found = 0;
       for (int l = 1; l <= N; l++) {
-        int jd1 = TB(m.nodenum_at_cell, sen(N, l, 1), e0, E), jd2 = TB(m.nodenum_at_cell, sen(N, l, 2), e0, E);
+        int i1 = sen(N, l, 1) - 1, i2 = sen(N, l, 2) - 1;
+        int jd1 = m.cg_node[rc.o + i1], jd2 = m.cg_node[rc.o + i2];
         if ((jd1 == md1 && jd2 == md2) || (jd2 == md1 && jd1 == md2)) continue;
-        iflag = check_isect(xcg, xt, m.x_node[jd1 - 1], m.x_node[jd2 - 1], ycg, yt, m.y_node[jd1 - 1],
-                            m.y_node[jd2 - 1], xin, yin);
+        iflag = check_isect(xcg, xt, m.cg_x[rc.o + i1], m.cg_x[rc.o + i2], ycg, yt, m.cg_y[rc.o + i1], m.cg_y[rc.o + i2], xin, yin);
         if (iflag == 1) {
           nel_j = l;
           found = 1;
@@ -287,7 +302,7 @@ __device__ int search_line(const ElmCtx &c, int &nnel, int &jlev, double time, d
 }
 
 // ELM_backtrace_v1.f90:15-200 (i_which_backtrack = 1)
-__device__ int backtrace(const ElmCtx &c, int bt_step, double bt_dt, double &uu, double &vv, double &ww,
+__device__ __forceinline__ int backtrace(const ElmCtx &c, int bt_step, double bt_dt, double &uu, double &vv, double &ww,
                          double x0, double y0, double z0, int &nnel, int &jlev) {
   const DevMesh &m = c.m;
   const DevState &s = c.s;
@@ -298,16 +313,17 @@ __device__ int backtrace(const ElmCtx &c, int bt_step, double bt_dt, double &uu,
     yt = y0 - bt_dt * vv;
     zt = z0 - bt_dt * ww;
     int iflqs1 = 0;
-    int err = search_line(c, nnel, jlev, bt_dt, x0, y0, z0, xt, yt, zt, iflqs1);
+    CellRec rc;
+    int err = search_line(c, nnel, jlev, bt_dt, x0, y0, z0, xt, yt, zt, iflqs1, rc);
     if (err) return err;
-    int e0 = nnel - 1, N = m.tri_or_quad[e0], jl = jlev;
+    int e0 = nnel - 1, N = rc.N, jl = jlev;
     double zup = (jl == s.top_layer_at_element[e0]) ? c.MSL + s.eta_cell[e0] : m.z_level[jl];
     if (zt > zup) return GOMGPU_ERR_ELM_VERTICAL;
     double zrat = (zup - zt) / AL(s.dz_cell, jl, e0, E);
     double vxn[4], vyn[4], vzn[4], st[4];
     int nd[4];
     for (int l = 0; l < N; l++) {
-      nd[l] = TB(m.nodenum_at_cell, l + 1, e0, E) - 1;
+      nd[l] = m.cg_node[rc.o + l] - 1;
       double ux1 = A0(s.u_node, jl, nd[l], P), ux0 = A0(s.u_node, jl - 1, nd[l], P);
       double uy1 = A0(s.v_node, jl, nd[l], P), uy0 = A0(s.v_node, jl - 1, nd[l], P);
       double uz1 = A0(s.w_node, jl, nd[l], P), uz0 = A0(s.w_node, jl - 1, nd[l], P);
@@ -315,8 +331,8 @@ __device__ int backtrace(const ElmCtx &c, int bt_step, double bt_dt, double &uu,
       vyn[l] = uy1 * (1 - zrat) + uy0 * zrat;
       vzn[l] = uz1 * (1 - zrat) + uz0 * zrat;
     }
-    double X1 = m.x_node[nd[0]], X2 = m.x_node[nd[1]], X3 = m.x_node[nd[2]];
-    double Y1 = m.y_node[nd[0]], Y2 = m.y_node[nd[1]], Y3 = m.y_node[nd[2]];
+    double X1 = m.cg_x[rc.o + 0], X2 = m.cg_x[rc.o + 1], X3 = m.cg_x[rc.o + 2];
+    double Y1 = m.cg_y[rc.o + 0], Y2 = m.cg_y[rc.o + 1], Y3 = m.cg_y[rc.o + 2];
     double ar = m.area[e0];
     if (N == 3) {
       st[0] = carea(xt, X2, X3, yt, Y2, Y3) / ar;
@@ -330,7 +346,7 @@ __device__ int backtrace(const ElmCtx &c, int bt_step, double bt_dt, double &uu,
         st[2] = 1.0 - st[0] - st[1];
       }
     } else {
-      double X4 = m.x_node[nd[3]], Y4 = m.y_node[nd[3]];
+      double X4 = m.cg_x[rc.o + 3], Y4 = m.cg_y[rc.o + 3];
       double aa1 = carea(X1, X2, xt, Y1, Y2, yt), aa2 = carea(X2, X3, xt, Y2, Y3, yt);
       double aa3 = carea(X3, X4, xt, Y3, Y4, yt), aa4 = carea(X4, X1, xt, Y4, Y1, yt);
       double aa = fabs(aa1) + fabs(aa2) + fabs(aa3) + fabs(aa4);
@@ -635,6 +651,7 @@ __device__ __forceinline__ void mom_row(const DevState &s, const double *gamma_t
   }
 }
 
+#define MS_CHUNK 4
 __global__ void k_mom_solve(DevMesh m, DevState s, DevForcing f, DevWork w, DevDims d, MomParams q, int *err) {
   int j = blockIdx.x * blockDim.x + threadIdx.x;
   const int J = d.J, L = d.L;
@@ -653,7 +670,11 @@ __global__ void k_mom_solve(DevMesh m, DevState s, DevForcing f, DevWork w, DevD
   int kb = m.bottom_layer_at_face[j];
   int n = kt - kb + 1;
   int bt = m.boundary_type_of_face[j];
-  // forward elimination (rows 0..n-1, RHS 3 is dz of the row's layer)
+  // forward elimination (rows 0..n-1, RHS 3 is dz of the row's layer). Row r >= 1 (layer
+  // k = kt - r) has a = the c of row r-1 (the same expression of the same inputs), c from
+  // layer k-1, b from its own dz. Rows are taken in chunks of MS_CHUNK whose inputs are all
+  // loaded (branch-free, indexes clamped into the column) before the recurrence runs, so a
+  // thread waits for one memory latency per chunk rather than per row.
   double a, b, c, cprev;
   mom_row(s, f.Gamma_T, j, J, kt, kb, 0, dt, a, b, c);
   int e = 0;
@@ -669,37 +690,68 @@ __global__ void k_mom_solve(DevMesh m, DevState s, DevForcing f, DevWork w, DevD
     AL(s.AinvG2, 1, j, J) = x1;
     AL(s.AinvDeltaZ2, 1, j, J) = x2;
     cprev = c;
-    for (int r = 1; r < n; r++) {
-      mom_row(s, f.Gamma_T, j, J, kt, kb, r, dt, a, b, c);
-      double gam = cprev / den;
-      den = b - a * gam;
-      if (den == 0.0) {
-        e = GOMGPU_ERR_TRIDIAG_PIVOT;
-        break;
+    const double gbdt = s.Gamma_B[j] * dt;
+    for (int r0 = 1; r0 < n && !e; r0 += MS_CHUNK) {
+      double avm[MS_CHUNK], dzhm[MS_CHUNK], dzk[MS_CHUNK], g1[MS_CHUNK], g2[MS_CHUNK];
+#pragma unroll
+      for (int i = 0; i < MS_CHUNK; i++) {
+        const int r = r0 + i < n ? r0 + i : n - 1, k = kt - r;
+        avm[i] = A0(s.Av, k - 1, j, J);
+        dzhm[i] = A0(s.dzhalf_face, k - 1, j, J);
+        dzk[i] = AL(s.dz_face, k, j, J);
+        g1[i] = AL(s.AinvG1, r + 1, j, J);
+        g2[i] = AL(s.AinvG2, r + 1, j, J);
       }
-      x0 = (AL(s.AinvG1, r + 1, j, J) - a * x0) / den;
-      x1 = (AL(s.AinvG2, r + 1, j, J) - a * x1) / den;
-      x2 = (AL(s.dz_face, kt - r, j, J) - a * x2) / den;
-      AL(s.AinvG1, r + 1, j, J) = x0;
-      AL(s.AinvG2, r + 1, j, J) = x1;
-      AL(s.AinvDeltaZ2, r + 1, j, J) = x2;
-      AL(w.tri_gam, r + 1, j, J) = gam;
-      cprev = c;
+#pragma unroll
+      for (int i = 0; i < MS_CHUNK; i++) {
+        const int r = r0 + i;
+        if (r >= n || e) continue;
+        a = cprev;
+        c = r < n - 1 ? -(dt * avm[i] / dzhm[i]) : 0.0;
+        b = r == n - 1 ? -a + dzk[i] + gbdt : -a + dzk[i] - c;
+        double gam = cprev / den;
+        den = b - a * gam;
+        if (den == 0.0) {
+          e = GOMGPU_ERR_TRIDIAG_PIVOT;
+          continue;
+        }
+        x0 = (g1[i] - a * x0) / den;
+        x1 = (g2[i] - a * x1) / den;
+        x2 = (dzk[i] - a * x2) / den;
+        AL(s.AinvG1, r + 1, j, J) = x0;
+        AL(s.AinvG2, r + 1, j, J) = x1;
+        AL(s.AinvDeltaZ2, r + 1, j, J) = x2;
+        AL(w.tri_gam, r + 1, j, J) = gam;
+        cprev = c;
+      }
     }
   }
   if (e) {
     latch(err, e, j + 1);
     return;
   }
-  // back substitution
-  for (int r = n - 2; r >= 0; r--) {
-    double gam = AL(w.tri_gam, r + 2, j, J);
-    x0 = AL(s.AinvG1, r + 1, j, J) - gam * x0;
-    x1 = AL(s.AinvG2, r + 1, j, J) - gam * x1;
-    x2 = AL(s.AinvDeltaZ2, r + 1, j, J) - gam * x2;
-    AL(s.AinvG1, r + 1, j, J) = x0;
-    AL(s.AinvG2, r + 1, j, J) = x1;
-    AL(s.AinvDeltaZ2, r + 1, j, J) = x2;
+  // back substitution, in chunks likewise
+  for (int r0 = n - 2; r0 >= 0; r0 -= MS_CHUNK) {
+    double gm[MS_CHUNK], h0[MS_CHUNK], h1[MS_CHUNK], h2[MS_CHUNK];
+#pragma unroll
+    for (int i = 0; i < MS_CHUNK; i++) {
+      const int r = r0 - i >= 0 ? r0 - i : 0;
+      gm[i] = AL(w.tri_gam, r + 2, j, J);
+      h0[i] = AL(s.AinvG1, r + 1, j, J);
+      h1[i] = AL(s.AinvG2, r + 1, j, J);
+      h2[i] = AL(s.AinvDeltaZ2, r + 1, j, J);
+    }
+#pragma unroll
+    for (int i = 0; i < MS_CHUNK; i++) {
+      const int r = r0 - i;
+      if (r < 0) continue;
+      x0 = h0[i] - gm[i] * x0;
+      x1 = h1[i] - gm[i] * x1;
+      x2 = h2[i] - gm[i] * x2;
+      AL(s.AinvG1, r + 1, j, J) = x0;
+      AL(s.AinvG2, r + 1, j, J) = x1;
+      AL(s.AinvDeltaZ2, r + 1, j, J) = x2;
+    }
   }
   bool land = (bt == -1);
   bool zero_g1 = land && m.isflowside3[j] == 0;
