@@ -30,11 +30,12 @@ struct DevMesh {
   // layer; [slot][node] like the other tables. ns_weight is the node's sum of fw/L.
   const int32_t *ns_face, *ns_fkb, *ns_cell, *ns_ckb;
   const double *ns_Lf, *ns_fw, *ns_area, *ns_weight;
-  // Cell records of the ELM walk, built at create: the x and y of a cell's nodes and the
-  // node ids (1-based) in slot order, 4 per cell, contiguous per cell ([cell][slot]), so a
-  // visited cell costs three aligned loads instead of a chain of indirect ones.
+  // Cell records of the ELM walk, built at create: the x and y of a cell's nodes, the node
+  // ids and the neighbour cells in slot order, 4 per cell, contiguous per cell
+  // ([cell][slot]), so a visited cell costs a few aligned loads instead of a chain of
+  // indirect ones.
   const double *cg_x, *cg_y;
-  const int32_t *cg_node;
+  const int32_t *cg_node, *cg_adj;  // node ids and neighbour cells (1-based, 0 = none)
 };
 
 struct DevState {

@@ -361,13 +361,18 @@ extern "C" int gomgpu_create(const gomgpu_mesh *mesh, const gomgpu_params *param
       rc = seterr(c, GOMGPU_E_ARG, "mesh connectivity missing");
       goto fail;
     }
+    if (!mesh->adj_cellnum_at_cell) {
+      rc = seterr(c, GOMGPU_E_ARG, "mesh connectivity missing");
+      goto fail;
+    }
     std::vector<double> cx(4 * E, 0.0), cy(4 * E, 0.0);
-    std::vector<int32_t> cn(4 * E, 0);
+    std::vector<int32_t> cn(4 * E, 0), ca(4 * E, 0);
     for (long e = 0; e < E; e++) {
       int N = mesh->tri_or_quad[e];
       for (int l = 0; l < N && l < 4; l++) {
         int nd = mesh->nodenum_at_cell[l + e * 4];
         cn[4 * e + l] = nd;
+        ca[4 * e + l] = mesh->adj_cellnum_at_cell[l + e * 4];
         cx[4 * e + l] = mesh->x_node[nd - 1];
         cy[4 * e + l] = mesh->y_node[nd - 1];
       }
@@ -380,6 +385,8 @@ extern "C" int gomgpu_create(const gomgpu_mesh *mesh, const gomgpu_params *param
     c->m.cg_y = dp;
     if ((rc = dalloc(c, &ip, cn.size())) || (rc = upload(c, ip, cn.data(), Shape{1, (long)cn.size()}))) goto fail;
     c->m.cg_node = ip;
+    if ((rc = dalloc(c, &ip, ca.size())) || (rc = upload(c, ip, ca.data(), Shape{1, (long)ca.size()}))) goto fail;
+    c->m.cg_adj = ip;
   }
   // state
 #define AL_SR(name, ext)                                                             \
@@ -1350,7 +1357,7 @@ extern "C" long long gomgpu_footprint(const gomgpu_params *params) {
     add(NB * P, sizeof(double)); add(P, sizeof(double));
   }
   add(4 * (size_t)p.maxele, sizeof(double)); add(4 * (size_t)p.maxele, sizeof(double));  // ELM cell records
-  add(4 * (size_t)p.maxele, sizeof(int32_t));
+  add(4 * (size_t)p.maxele, sizeof(int32_t)); add(4 * (size_t)p.maxele, sizeof(int32_t));
   GOMGPU_STATE_REAL(FP_R)
   GOMGPU_STATE_INT(FP_I)
   GOMGPU_FORCING_REAL(FP_R)
@@ -1391,7 +1398,8 @@ extern "C" long long gomgpu_device_bytes(gomgpu_ctx *c) { return c ? (long long)
 
 // ------------------------------------------------------------------ launch shapes
 static int launch_max(int which) {
-  return which == GOMGPU_LAUNCH_TURB_SOLVE || which == GOMGPU_LAUNCH_MOM_RHS ? 256 : 1024;
+  return which == GOMGPU_LAUNCH_TURB_SOLVE || which == GOMGPU_LAUNCH_MOM_RHS || which == GOMGPU_LAUNCH_ELM ? 256
+                                                                                                     : 1024;
 }
 
 extern "C" int gomgpu_set_launch(gomgpu_ctx *c, int which, int block) {

@@ -90,7 +90,7 @@ __device__ __forceinline__ int check_isect(double x1, double x2, double x3, doub
 }
 
 // bilinear_interpolation.f90:8-121; returns 0 or GOMGPU_ERR_BILINEAR
-__device__ int bilinear(double x1, double x2, double x3, double x4, double y1, double y2, double y3,
+__device__ __forceinline__ int bilinear(double x1, double x2, double x3, double x4, double y1, double y2, double y3,
                         double y4, double x, double y, double *shapef) {
   const double small3 = (double)1.e-5f;
   const double lim = (double)1.1f;
@@ -163,42 +163,92 @@ struct ElmCtx {
   double MSL;
 };
 
-// One visited cell of the walk: its node count and its cell record (the node coordinates and
-// ids, the values of x_node, y_node and nodenum_at_cell in slot order), read where used.
+// One visited cell of the walk, read from the cell records with a few aligned loads issued
+// together: node count, area, node coordinates, node ids and neighbours in slot order (the
+// values of tri_or_quad, area, x_node/y_node, nodenum_at_cell and adj_cellnum_at_cell).
 struct CellRec {
-  int o;  // 4 * cell: the record's first slot
   int N;
+  double ar;
+  double X[4], Y[4];
+  int nd[4], adj[4];
 };
 
 __device__ __forceinline__ void cell_rec(const DevMesh &m, int e, CellRec &r) {
   r.N = m.tri_or_quad[e];
-  r.o = 4 * e;
+  r.ar = m.area[e];
+  const double2 *px = (const double2 *)(m.cg_x + 4 * (size_t)e), *py = (const double2 *)(m.cg_y + 4 * (size_t)e);
+  const double2 x01 = px[0], x23 = px[1], y01 = py[0], y23 = py[1];
+  const int4 nn = *(const int4 *)(m.cg_node + 4 * (size_t)e), aa = *(const int4 *)(m.cg_adj + 4 * (size_t)e);
+  r.X[0] = x01.x, r.X[1] = x01.y, r.X[2] = x23.x, r.X[3] = x23.y;
+  r.Y[0] = y01.x, r.Y[1] = y01.y, r.Y[2] = y23.x, r.Y[3] = y23.y;
+  r.nd[0] = nn.x, r.nd[1] = nn.y, r.nd[2] = nn.z, r.nd[3] = nn.w;
+  r.adj[0] = aa.x, r.adj[1] = aa.y, r.adj[2] = aa.z, r.adj[3] = aa.w;
+}
+
+// Element i (0..3) of a register-held 4-array: a select chain, so the array stays in
+// registers (a dynamic index would move it to scratch memory).
+template <typename T>
+__device__ __forceinline__ T pick4(const T (&a)[4], int i) {
+  return i == 0 ? a[0] : i == 1 ? a[1] : i == 2 ? a[2] : a[3];
+}
+
+// Sum over the cell's edges l = 1..N of |area(node l, node sen(l,1), point)|, in slot order
+// (the containment test of search_straight_line.f90:60-100). Unrolled over 4 slots; the
+// terms of slots beyond N are not added.
+__device__ __forceinline__ double area_sum(const CellRec &g, double px, double py) {
+  double sum = 0.0;
+#pragma unroll
+  for (int l = 1; l <= 4; l++) {
+    const double xb = l < g.N ? g.X[l & 3] : g.X[0], yb = l < g.N ? g.Y[l & 3] : g.Y[0];
+    const double t = sum + fabs(carea(g.X[l - 1], xb, px, g.Y[l - 1], yb, py));
+    sum = l <= g.N ? t : sum;
+  }
+  return sum;
+}
+
+// The first edge l (slots sen(l,1), sen(l,2), l = 1..N in order) that the segment
+// (xa,ya)-(xb,yb) meets, skipping the edge with node ids {md1, md2} when md1 > 0
+// (search_straight_line.f90:144-169, 320-350). Every edge is tested; the lowest hit wins,
+// which is the edge the reference's loop stops at. Returns l, or 0 if none; (xin, yin) is
+// the intersection point of that edge.
+__device__ __forceinline__ int first_edge(const CellRec &g, double xa, double xb, double ya, double yb, int md1,
+                                          int md2, double &xin, double &yin) {
+  int hit_l = 0;
+#pragma unroll
+  for (int l = 4; l >= 1; l--) {
+    const int i1 = l < g.N ? l : 0;                                  // sen(N, l, 1) - 1
+    const int i2 = l + 2 > g.N ? l + 1 - g.N : l + 1;                // sen(N, l, 2) - 1
+    const int jd1 = pick4(g.nd, i1), jd2 = pick4(g.nd, i2);
+    const bool skip = md1 > 0 && ((jd1 == md1 && jd2 == md2) || (jd2 == md1 && jd1 == md2));
+    double xi = 0.0, yi = 0.0;
+    const int hit = check_isect(xa, xb, pick4(g.X, i1), pick4(g.X, i2), ya, yb, pick4(g.Y, i1), pick4(g.Y, i2), xi, yi);
+    if (l <= g.N && !skip && hit == 1) {
+      hit_l = l;
+      xin = xi;
+      yin = yi;
+    }
+  }
+  return hit_l;
 }
 
 // search_straight_line.f90:15-399 (mode 1): straight-line walk from (x0,y0) to (xt,yt).
 // On return rc holds the record of the end cell nnel.
 __device__ __forceinline__ int search_line(const ElmCtx &c, int &nnel, int &jlev, double time, double x0, double y0,
-                           double z0, double &xt, double &yt, double &zt, int &nfl, CellRec &rc) {
+                                           double z0, double &xt, double &yt, double &zt, int &nfl, CellRec &rc) {
   const DevMesh &m = c.m;
   const DevState &s = c.s;
   const int E = c.d.E, J = c.d.J, L = c.d.L;
   const double small06 = (double)1.e-6f, MSL = c.MSL;
   int nel = nnel;  // 1-based
   int e0 = nel - 1;
+  cell_rec(m, e0, rc);
   if (s.top_layer_at_element[e0] == 0 || MSL - m.h_cell[e0] > MSL + s.eta_cell[e0]) return GOMGPU_ERR_ELM_DRY_START;
   nfl = 0;
   double trm = time;
-  double area1 = 0.0, area2 = 0.0;
-  cell_rec(m, e0, rc);
-  int N = rc.N;
-  for (int l = 1; l <= N; l++) {
-    int n1 = l - 1, n2 = sen(N, l, 1) - 1;
-    area1 = area1 + fabs(carea(m.cg_x[rc.o + n1], m.cg_x[rc.o + n2], x0, m.cg_y[rc.o + n1], m.cg_y[rc.o + n2], y0));
-    area2 = area2 + fabs(carea(m.cg_x[rc.o + n1], m.cg_x[rc.o + n2], xt, m.cg_y[rc.o + n1], m.cg_y[rc.o + n2], yt));
-  }
-  double ae = fabs(area1 - m.area[e0]) / m.area[e0];
+  double area1 = area_sum(rc, x0, y0), area2 = area_sum(rc, xt, yt);
+  double ae = fabs(area1 - rc.ar) / rc.ar;
   if (ae > small06) return GOMGPU_ERR_ELM_NOT_IN_CELL;
-  ae = fabs(area2 - m.area[e0]) / m.area[e0];
+  ae = fabs(area2 - rc.ar) / rc.ar;
   if (ae < small06) {
     nnel = nel;
   } else {
@@ -206,17 +256,9 @@ __device__ __forceinline__ int search_line(const ElmCtx &c, int &nnel, int &jlev
     double ycg = (1.0 - 1.0e-4) * y0 + 1.0e-4 * m.y_cell[e0];
     double pathl = sqrt((xt - xcg) * (xt - xcg) + (yt - ycg) * (yt - ycg));
     if ((xcg == xt && ycg == yt) || pathl == 0.0) return GOMGPU_ERR_ELM_ZERO_PATH;
-    int iflag = 0, nel_j = 0;
     double xin = 0.0, yin = 0.0;
-    for (int l = 1; l <= N; l++) {
-      int jd1 = sen(N, l, 1) - 1, jd2 = sen(N, l, 2) - 1;
-      iflag = check_isect(xcg, xt, m.cg_x[rc.o + jd1], m.cg_x[rc.o + jd2], ycg, yt, m.cg_y[rc.o + jd1], m.cg_y[rc.o + jd2], xin, yin);
-      if (iflag == 1) {
-        nel_j = l;
-        break;
-      }
-    }
-    if (iflag == 0) return GOMGPU_ERR_ELM_NO_EDGE;
+    int nel_j = first_edge(rc, xcg, xt, ycg, yt, 0, 0, xin, yin);
+    if (nel_j == 0) return GOMGPU_ERR_ELM_NO_EDGE;
     double zin = z0;
     int iteration = 0;
     for (;;) {
@@ -230,17 +272,21 @@ __device__ __forceinline__ int search_line(const ElmCtx &c, int &nnel, int &jlev
         break;
       }
       e0 = nel - 1;
-      N = rc.N;
-      int md1 = m.cg_node[rc.o + sen(N, nel_j, 1) - 1];
-      int md2 = m.cg_node[rc.o + sen(N, nel_j, 2) - 1];
+      const int N = rc.N;
+      const int md1 = pick4(rc.nd, sen(N, nel_j, 1) - 1);
+      const int md2 = pick4(rc.nd, sen(N, nel_j, 2) - 1);
+      const int adj = pick4(rc.adj, nel_j - 1);
+      // the neighbour's record and wet flag, loaded together (row 1 stands in for none)
+      CellRec nx;
+      cell_rec(m, adj > 0 ? adj - 1 : 0, nx);
+      const int adj_top = s.top_layer_at_element[adj > 0 ? adj - 1 : 0];
       double dist = sqrt((xin - xt) * (xin - xt) + (yin - yt) * (yin - yt));
       if (dist / pathl > 1.0 + 1.0e-4) return GOMGPU_ERR_ELM_OVERSHOT;
       zin = zt - dist / pathl * (zt - zin);
       trm = trm * dist / pathl;
       pathl = sqrt((xin - xt) * (xin - xt) + (yin - yt) * (yin - yt));
       if (pathl == 0.0 || trm == 0.0) return GOMGPU_ERR_ELM_TARGET;
-      int adj = TB(m.adj_cellnum_at_cell, nel_j, e0, E);
-      if (adj == 0 || s.top_layer_at_element[adj - 1] == 0) {
+      if (adj == 0 || adj_top == 0) {
         int isd = TB(m.facenum_at_cell, nel_j, e0, E) - 1;
         if (TB(m.nodenum_at_face, 1, isd, J) + TB(m.nodenum_at_face, 2, isd, J) != md1 + md2)
           return GOMGPU_ERR_ELM_WRONG_SIDE;
@@ -257,31 +303,15 @@ __device__ __forceinline__ int search_line(const ElmCtx &c, int &nnel, int &jlev
       }
       nel = adj;
       e0 = nel - 1;
-      cell_rec(m, e0, rc);
-      N = rc.N;
-      area1 = 0.0;
-      for (int l = 1; l <= N; l++) {
-        int k1 = l - 1, k2 = sen(N, l, 1) - 1;
-        area1 = area1 + fabs(carea(m.cg_x[rc.o + k1], m.cg_x[rc.o + k2], xt, m.cg_y[rc.o + k1], m.cg_y[rc.o + k2], yt));
-      }
-      ae = fabs(area1 - m.area[e0]) / m.area[e0];
+      rc = nx;
+      area1 = area_sum(rc, xt, yt);
+      ae = fabs(area1 - rc.ar) / rc.ar;
       if (ae < small06) {
         nnel = nel;
         break;
       }
-      int found = 0;
-      for (int l = 1; l <= N; l++) {
-        int i1 = sen(N, l, 1) - 1, i2 = sen(N, l, 2) - 1;
-        int jd1 = m.cg_node[rc.o + i1], jd2 = m.cg_node[rc.o + i2];
-        if ((jd1 == md1 && jd2 == md2) || (jd2 == md1 && jd1 == md2)) continue;
-        iflag = check_isect(xcg, xt, m.cg_x[rc.o + i1], m.cg_x[rc.o + i2], ycg, yt, m.cg_y[rc.o + i1], m.cg_y[rc.o + i2], xin, yin);
-        if (iflag == 1) {
-          nel_j = l;
-          found = 1;
-          break;
-        }
-      }
-      if (!found) return GOMGPU_ERR_ELM_NO_EDGE;
+      nel_j = first_edge(rc, xcg, xt, ycg, yt, md1, md2, xin, yin);
+      if (nel_j == 0) return GOMGPU_ERR_ELM_NO_EDGE;
     }
   }
   e0 = nnel - 1;
@@ -289,12 +319,16 @@ __device__ __forceinline__ int search_line(const ElmCtx &c, int &nnel, int &jlev
   double z = dminf(dmaxf(zt, MSL - m.h_cell[e0]), MSL + s.eta_cell[e0]);
   zt = z;
   if (z <= m.z_level[0] || z > m.z_level[L]) return GOMGPU_ERR_ELM_VERTICAL;
-  for (int k = 0; k <= L - 1; k++) {
-    if (z > m.z_level[k] && z <= m.z_level[k + 1]) {
-      jlev = k + 1;
-      break;
-    }
+  // the level k with z_level[k] < z <= z_level[k+1] (z_level increases strictly): bisection
+  int lo = 0, hi = L;  // z_level[lo] < z <= z_level[hi]
+  while (hi - lo > 1) {
+    int mid = (lo + hi) >> 1;
+    if (z > m.z_level[mid])
+      lo = mid;
+    else
+      hi = mid;
   }
+  jlev = lo + 1;
   int bl = m.bottom_layer_at_element[e0], tl = s.top_layer_at_element[e0];
   int jl = jlev < bl ? bl : jlev;
   jlev = jl > tl ? tl : jl;
@@ -323,7 +357,7 @@ __device__ __forceinline__ int backtrace(const ElmCtx &c, int bt_step, double bt
     double vxn[4], vyn[4], vzn[4], st[4];
     int nd[4];
     for (int l = 0; l < N; l++) {
-      nd[l] = m.cg_node[rc.o + l] - 1;
+      nd[l] = pick4(rc.nd, l) - 1;
       double ux1 = A0(s.u_node, jl, nd[l], P), ux0 = A0(s.u_node, jl - 1, nd[l], P);
       double uy1 = A0(s.v_node, jl, nd[l], P), uy0 = A0(s.v_node, jl - 1, nd[l], P);
       double uz1 = A0(s.w_node, jl, nd[l], P), uz0 = A0(s.w_node, jl - 1, nd[l], P);
@@ -331,9 +365,9 @@ __device__ __forceinline__ int backtrace(const ElmCtx &c, int bt_step, double bt
       vyn[l] = uy1 * (1 - zrat) + uy0 * zrat;
       vzn[l] = uz1 * (1 - zrat) + uz0 * zrat;
     }
-    double X1 = m.cg_x[rc.o + 0], X2 = m.cg_x[rc.o + 1], X3 = m.cg_x[rc.o + 2];
-    double Y1 = m.cg_y[rc.o + 0], Y2 = m.cg_y[rc.o + 1], Y3 = m.cg_y[rc.o + 2];
-    double ar = m.area[e0];
+    double X1 = rc.X[0], X2 = rc.X[1], X3 = rc.X[2];
+    double Y1 = rc.Y[0], Y2 = rc.Y[1], Y3 = rc.Y[2];
+    double ar = rc.ar;
     if (N == 3) {
       st[0] = carea(xt, X2, X3, yt, Y2, Y3) / ar;
       st[1] = carea(X1, xt, X3, Y1, yt, Y3) / ar;
@@ -346,7 +380,7 @@ __device__ __forceinline__ int backtrace(const ElmCtx &c, int bt_step, double bt
         st[2] = 1.0 - st[0] - st[1];
       }
     } else {
-      double X4 = m.cg_x[rc.o + 3], Y4 = m.cg_y[rc.o + 3];
+      double X4 = rc.X[3], Y4 = rc.Y[3];
       double aa1 = carea(X1, X2, xt, Y1, Y2, yt), aa2 = carea(X2, X3, xt, Y2, Y3, yt);
       double aa3 = carea(X3, X4, xt, Y3, Y4, yt), aa4 = carea(X4, X1, xt, Y4, Y1, yt);
       double aa = fabs(aa1) + fabs(aa2) + fabs(aa3) + fabs(aa4);
@@ -371,7 +405,8 @@ __device__ __forceinline__ int backtrace(const ElmCtx &c, int bt_step, double bt
 }
 
 // solve_nonlinear_advection.f90:92-202; one thread per (layer, face).
-__global__ void k_elm(DevMesh m, DevState s, DevDims d, double MSL, double dt, double onoff, int *err) {
+__global__ void __launch_bounds__(256) k_elm(DevMesh m, DevState s, DevDims d, double MSL, double dt, double onoff,
+                                             int *err) {
   int t = blockIdx.x * blockDim.x + threadIdx.x;
   const int J = d.J, E = d.E, P = d.P;
   if (t >= J * d.L) return;

@@ -25,7 +25,7 @@ STAGE_IDS = {name: i for i, name in enumerate(abi.STAGES)}
 # tunable launch shapes (GOMGPU_LAUNCH_* of include/gomgpu.h) and their largest block
 LAUNCH = {"elm": 0, "mom_rhs": 1, "mom_solve": 2, "tvd_rhs": 3, "tvd_solve": 4, "tvd_lim": 5,
           "node_vel": 6, "turb_prep": 7, "turb_solve": 8, "stream": 9}
-LAUNCH_MAX = {k: (256 if k in ("turb_solve", "mom_rhs") else 1024) for k in LAUNCH}
+LAUNCH_MAX = {k: (256 if k in ("turb_solve", "mom_rhs", "elm") else 1024) for k in LAUNCH}
 
 
 class GomGpuError(RuntimeError):

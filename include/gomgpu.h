@@ -263,7 +263,7 @@ void gomgpu_destroy(gomgpu_ctx *ctx);
 /* Launch shapes the autotuner may change: threads per block of the kernels whose threads
  * each compute their own outputs, so no choice changes any result (the PCG partition is a
  * fixed function of the system size and is not tunable). Powers of two from 64 to 1024
- * (256 for MOM_RHS and TURB_SOLVE). No counterpart in the reference. */
+ * (256 for ELM, MOM_RHS and TURB_SOLVE). No counterpart in the reference. */
 enum {
   GOMGPU_LAUNCH_ELM = 0,        /* k_elm, one thread per (layer, face) */
   GOMGPU_LAUNCH_MOM_RHS = 1,    /* k_mom_rhs, 64 faces per block, one wave per row stride */

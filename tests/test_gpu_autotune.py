@@ -49,8 +49,11 @@ def test_launch_limits(gpu_available):
     for bad in (32, 100, 2048):
         with pytest.raises(host.GomGpuError):
             sc.model.set_launch("elm", bad)
-    with pytest.raises(host.GomGpuError):
-        sc.model.set_launch("turb_solve", 512)
-    sc.model.set_launch("elm", 512)
-    assert sc.model.get_launch("elm") == 512
+    for which in ("turb_solve", "mom_rhs", "elm"):   # kernels compiled for at most 256 threads
+        with pytest.raises(host.GomGpuError):
+            sc.model.set_launch(which, 512)
+    sc.model.set_launch("elm", 256)
+    assert sc.model.get_launch("elm") == 256
+    sc.model.set_launch("tvd_rhs", 512)
+    assert sc.model.get_launch("tvd_rhs") == 512
     sc.model.close()
