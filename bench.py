@@ -156,6 +156,8 @@ def ensemble_sample(n_mem: int = 8, steps: int = 10, seed: int = 7):
                                      tide_phase=float(rng.uniform(-180, 180)), turbulence_flag=3)
              for i in range(n_mem)]
     scs = [driver.Scenario(c, wind=(8.0, 2.0)) for c in cases]
+    for sc in scs:   # the batch partition, alone and batched alike
+        sc.model.set_pcg_rows(serving.BATCH_PCG_ROWS)
     # serial: each member alone
     for sc in scs:
         sc.run(2)

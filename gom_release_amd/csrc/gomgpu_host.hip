@@ -101,6 +101,7 @@ struct gomgpu_ctx {
                            // k_pcg_cg1m above its register capacity), 4 always k_pcg_cg1m
   int pcg_bs = 512;        // threads per block of k_pcg_reg / k_pcg_cg1
   int pcg_nb = 0;          // blocks of k_pcg_reg (0 = rows / (4 * bs))
+  int pcg_rows = 0;        // rows per thread of the register-resident solve (0 = automatic)
   int ncu = 0;
   int last_iters = 0;
   std::string err;
@@ -670,13 +671,18 @@ static void pcg_args(gomgpu_ctx *c, PcgArgs &a) {
   a.tol = c->p.error_tolerance;
 }
 
-// Blocks of the register-resident solve for n rows at 512 threads: the fewest rows per
-// thread (1, 2 or 4) that cover the system with at most 256 blocks, then as many blocks as
-// that needs. A function of n alone, so a system gets the same partition alone and in a
-// batch. Above 4 x 256 x 512 rows the result exceeds 256 (the memory-resident solve).
-static int pcg_blocks_for(long n) {
-  long r = (n + 256L * 512 - 1) / (256L * 512);
-  r = r <= 1 ? 1 : r <= 2 ? 2 : 4;
+// Blocks of the register-resident solve for n rows at 512 threads. rows = 0 (automatic):
+// the fewest rows per thread (1, 2 or 4) that cover the system with at most 256 blocks,
+// which spreads a lone system's row work over the most CUs; rows = 1, 2 or 4 fixes it
+// (4 packs more batch members into one launch). A function of n and the context's setting
+// alone, so a system gets the same partition alone and in a batch. Above 4 x 256 x 512
+// rows the result exceeds 256 (the memory-resident solve).
+static int pcg_blocks_for(long n, int rows) {
+  long r = rows;
+  if (r <= 0) {
+    r = (n + 256L * 512 - 1) / (256L * 512);
+    r = r <= 1 ? 1 : r <= 2 ? 2 : 4;
+  }
   int nb = (int)((n + r * 512 - 1) / (r * 512));
   return nb < 1 ? 1 : nb;
 }
@@ -692,7 +698,7 @@ static int fs_pcg(gomgpu_ctx *c) {
     if (c->pcg_mode >= 1 && c->pcg_mode <= 3) {
       int rmax = c->pcg_bs >= 1024 ? 2 : c->pcg_bs >= 512 ? 4 : 8;
       int nb = c->pcg_nb > 0 ? c->pcg_nb
-               : c->pcg_bs == 512 ? pcg_blocks_for(d.E)
+               : c->pcg_bs == 512 ? pcg_blocks_for(d.E, c->pcg_rows)
                                   : (int)((d.E + (long)rmax * c->pcg_bs - 1) / ((long)rmax * c->pcg_bs));
       if (nb < 1) nb = 1;
       if (nb > 4096) nb = 4096;
@@ -1102,7 +1108,7 @@ static int batch_plan(gomgpu_batch *b) {
   std::vector<std::vector<gomgpu_ctx *>> grp;
   int total = 0;
   for (gomgpu_ctx *c : b->members) {
-    int nb = pcg_blocks_for(c->d.E);
+    int nb = pcg_blocks_for(c->d.E, c->pcg_rows);
     if (nb > b->capacity || nb > PCG_MAX_SLOT_BLOCKS) {
       b->solo.push_back(c);
       continue;
@@ -1124,7 +1130,7 @@ static int batch_plan(gomgpu_batch *b) {
       PcgArgs a;
       pcg_args(c, a);
       args.push_back(a);
-      int nb = pcg_blocks_for(c->d.E);
+      int nb = pcg_blocks_for(c->d.E, c->pcg_rows);
       base.push_back(off);
       nbm.push_back(nb);
       off += (nb + 7) / 8 * 8;
@@ -1407,6 +1413,15 @@ extern "C" int gomgpu_set_launch(gomgpu_ctx *c, int which, int block) {
   if (block < 64 || block > launch_max(which) || (block & (block - 1)) != 0)
     return seterr(c, GOMGPU_E_ARG, "block size must be a power of two in [64, max]");
   c->lb[which] = block;
+  return 0;
+}
+
+extern "C" int gomgpu_set_pcg_rows(gomgpu_ctx *c, int rows) {
+  if (!c) return GOMGPU_E_ARG;
+  if (rows != 0 && rows != 1 && rows != 2 && rows != 4)
+    return seterr(c, GOMGPU_E_ARG, "PCG rows per thread must be 0 (automatic), 1, 2 or 4");
+  c->pcg_rows = rows;
+  for (gomgpu_batch *b : c->batches) b->dirty = true;  // its batches re-plan
   return 0;
 }
 

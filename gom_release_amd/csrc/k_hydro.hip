@@ -1190,6 +1190,7 @@ __global__ void k_face_levels(DevMesh m, DevState s, DevDims d) {
 // (level, node). The node's faces and cells come from the static stencil tables (one
 // coalesced load each instead of the reference's cell -> face lookups), visited in the
 // reference's order; the weight sum is precomputed in that order.
+#define NV_CHUNK 4
 __global__ void k_node_vel(DevMesh m, DevState s, DevDims d) {
   int t = blockIdx.x * blockDim.x + threadIdx.x;
   const int P = d.P, E = d.E, J = d.J;
@@ -1199,28 +1200,56 @@ __global__ void k_node_vel(DevMesh m, DevState s, DevDims d) {
   double u = 0.0, v = 0.0;
   int icount = 0;
   double w = 0.0, tw = 0.0;
-  for (int jj = 0; jj < nadj; jj++) {
+  // the adjacent cells in chunks of NV_CHUNK: each chunk's stencil entries, then the wet
+  // tops they index, then the level values are loaded as three batches before the sums,
+  // which run in the reference's order (slots beyond nadj are not added)
+  for (int j0 = 0; j0 < nadj; j0 += NV_CHUNK) {
+    int ifn[2 * NV_CHUNK], isn[NV_CHUNK], ftop[2 * NV_CHUNK], ctop[NV_CHUNK];
 #pragma unroll
-    for (int q = 0; q < 2; q++) {
-      size_t o = (size_t)(2 * jj + q) * P + i;
-      int ifn = m.ns_face[o];
-      int kt = s.top_layer_at_face[ifn];
-      if (kt != 0 && k >= m.ns_fkb[o] - 1) {
-        int kvt = k < kt ? k : kt;
-        double Lf = m.ns_Lf[o], fw = m.ns_fw[o];
-        u = u + A0(s.u_face_level, kvt, ifn, J) / Lf * fw;
-        v = v + A0(s.v_face_level, kvt, ifn, J) / Lf * fw;
-        icount++;
-      }
+    for (int q = 0; q < NV_CHUNK; q++) {
+      const int jj = j0 + q < nadj ? j0 + q : j0;
+      ifn[2 * q] = m.ns_face[(size_t)(2 * jj) * P + i];
+      ifn[2 * q + 1] = m.ns_face[(size_t)(2 * jj + 1) * P + i];
+      isn[q] = m.ns_cell[(size_t)jj * P + i];
     }
-    size_t oc = (size_t)jj * P + i;
-    int isn = m.ns_cell[oc];
-    int te = s.top_layer_at_element[isn];
-    if (te != 0 && k >= m.ns_ckb[oc] - 1) {
-      int kvt = k < te ? k : te;
-      double ar = m.ns_area[oc];
-      tw = tw + ar;
-      w = w + A0(s.wn_cell, kvt, isn, E) * ar;
+#pragma unroll
+    for (int q = 0; q < NV_CHUNK; q++) {
+      ftop[2 * q] = s.top_layer_at_face[ifn[2 * q]];
+      ftop[2 * q + 1] = s.top_layer_at_face[ifn[2 * q + 1]];
+      ctop[q] = s.top_layer_at_element[isn[q]];
+    }
+    double fu[2 * NV_CHUNK], fv[2 * NV_CHUNK], cw[NV_CHUNK];
+#pragma unroll
+    for (int f = 0; f < 2 * NV_CHUNK; f++) {
+      const int kvt = k < ftop[f] ? k : ftop[f];
+      fu[f] = A0(s.u_face_level, kvt, ifn[f], J);
+      fv[f] = A0(s.v_face_level, kvt, ifn[f], J);
+    }
+#pragma unroll
+    for (int q = 0; q < NV_CHUNK; q++) {
+      const int kvt = k < ctop[q] ? k : ctop[q];
+      cw[q] = A0(s.wn_cell, kvt, isn[q], E);
+    }
+#pragma unroll
+    for (int q = 0; q < NV_CHUNK; q++) {
+      const int jj = j0 + q;
+      if (jj >= nadj) continue;
+#pragma unroll
+      for (int h = 0; h < 2; h++) {
+        size_t o = (size_t)(2 * jj + h) * P + i;
+        if (ftop[2 * q + h] != 0 && k >= m.ns_fkb[o] - 1) {
+          double Lf = m.ns_Lf[o], fw = m.ns_fw[o];
+          u = u + fu[2 * q + h] / Lf * fw;
+          v = v + fv[2 * q + h] / Lf * fw;
+          icount++;
+        }
+      }
+      size_t oc = (size_t)jj * P + i;
+      if (ctop[q] != 0 && k >= m.ns_ckb[oc] - 1) {
+        double ar = m.ns_area[oc];
+        tw = tw + ar;
+        w = w + cw[q] * ar;
+      }
     }
   }
   if (icount != 0) {

@@ -59,6 +59,7 @@ def lib():
         L.gomgpu_destroy.restype = None
         L.gomgpu_set_launch.argtypes = [vp, C.c_int, C.c_int]
         L.gomgpu_get_launch.argtypes = [vp, C.c_int]
+        L.gomgpu_set_pcg_rows.argtypes = [vp, C.c_int]
         L.gomgpu_footprint.argtypes = [C.POINTER(abi.Params)]
         L.gomgpu_footprint.restype = C.c_longlong
         L.gomgpu_device_bytes.argtypes = [vp]
@@ -79,7 +80,8 @@ def lib():
 EXPORTED = ["gomgpu_create", "gomgpu_set_state", "gomgpu_set_forcing", "gomgpu_stage",
             "gomgpu_step", "gomgpu_run", "gomgpu_get_state", "gomgpu_budgets",
             "gomgpu_pcg_iterations", "gomgpu_sync", "gomgpu_last_error", "gomgpu_timings",
-            "gomgpu_destroy", "gomgpu_set_launch", "gomgpu_get_launch", "gomgpu_footprint",
+            "gomgpu_destroy", "gomgpu_set_launch", "gomgpu_get_launch", "gomgpu_set_pcg_rows",
+            "gomgpu_footprint",
             "gomgpu_device_bytes", "gomgpu_batch_create", "gomgpu_batch_add", "gomgpu_batch_remove",
             "gomgpu_batch_size", "gomgpu_batch_step", "gomgpu_batch_sync", "gomgpu_batch_last_error",
             "gomgpu_batch_destroy"]
@@ -165,6 +167,10 @@ class GpuModel:
 
     def set_launch(self, which: str, block: int) -> None:
         self._check(self._L.gomgpu_set_launch(self._ctx, LAUNCH[which], int(block)))
+
+    def set_pcg_rows(self, rows: int) -> None:
+        """Rows per thread of the PCG partition (0 automatic, or 1, 2, 4; 4 for batching)."""
+        self._check(self._L.gomgpu_set_pcg_rows(self._ctx, int(rows)))
 
     def get_launch(self, which: str) -> int:
         return int(self._L.gomgpu_get_launch(self._ctx, LAUNCH[which]))

@@ -279,6 +279,13 @@ enum {
 };
 int gomgpu_set_launch(gomgpu_ctx *ctx, int which, int block);
 int gomgpu_get_launch(gomgpu_ctx *ctx, int which);
+/* Rows per thread of the register-resident PCG partition: 0 (default) picks the fewest of
+ * 1, 2, 4 that fit 256 blocks, spreading a lone system over the most compute units; 4 packs
+ * more batch members into one launch. Unlike the launch shapes this changes how the dot
+ * products are associated (results agree to rounding, not bit for bit, across settings); a
+ * context gives the same results alone and in a batch for the same setting. No
+ * counterpart in the reference. */
+int gomgpu_set_pcg_rows(gomgpu_ctx *ctx, int rows);
 
 /* Device bytes gomgpu_create would allocate for these parameters (admission by HBM
  * footprint), and the bytes a context holds; -1 on bad arguments. */

@@ -40,9 +40,10 @@ def _state(sc):
     return s
 
 
-def _alone(case, nsteps):
+def _alone(case, nsteps, rows=0):
     from gom_release_amd.driver import Scenario
     sc = Scenario(case, wind=WIND)
+    sc.model.set_pcg_rows(rows)
     sc.run(nsteps)
     sc.model.sync()
     out = _state(sc)
@@ -56,14 +57,15 @@ def _assert_same(got, ref, tag):
         assert bad.size == 0, f"{tag} {f}: {bad.size} values differ, first {bad[:5]}"
 
 
-@pytest.mark.parametrize("graph", [1, 0])
-def test_batch_equals_alone(gpu_available, graph):
+@pytest.mark.parametrize("graph,rows", [(1, 0), (0, 0), (1, 4)])
+def test_batch_equals_alone(gpu_available, graph, rows):
+    """rows: the PCG partition (0 automatic; 4, the batch-packing one), the same alone."""
     from gom_release_amd import host
     from gom_release_amd.driver import Scenario
-    print(f"[gpu-batch] graph={graph}", flush=True)
+    print(f"[gpu-batch] graph={graph} rows={rows}", flush=True)
     cases = _cases()
     n = 4
-    refs = [_alone(c, n) for c in cases]
+    refs = [_alone(c, n, rows) for c in cases]
     os.environ["GOMGPU_BATCH_GRAPH"] = str(graph)
     try:
         b = host.GpuBatch()
@@ -71,6 +73,7 @@ def test_batch_equals_alone(gpu_available, graph):
         os.environ.pop("GOMGPU_BATCH_GRAPH", None)
     scs = [Scenario(c, wind=WIND) for c in cases]
     for sc in scs:
+        sc.model.set_pcg_rows(rows)
         b.add(sc.model)
     for _ in range(n):
         for sc in scs:

@@ -39,8 +39,10 @@ def _requests(n_steps=4):
 
 
 def _alone(req):
+    from gom_release_amd import serving
     from gom_release_amd.driver import Scenario
     sc = Scenario(req.case, wind=req.wind)
+    sc.model.set_pcg_rows(serving.BATCH_PCG_ROWS)   # the served scenarios' partition
     sc.run(req.nsteps)
     sc.model.sync()
     s = abi.state_arrays(sc.params)
