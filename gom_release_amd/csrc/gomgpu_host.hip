@@ -55,6 +55,7 @@ __global__ void k_turb_post(DevMesh, DevState, DevWork, DevDims, TurbParams);
 __global__ void k_turb_kv(DevMesh, DevState, DevWork, DevDims);
 __global__ void k_budget(DevMesh, DevState, DevDims, double *);
 __global__ void k_copy(double *, const double *, int);
+__global__ void k_update_dz(double *, const double *, double *, const double *, const int32_t *, const int32_t *, int, int);
 __global__ void k_fill(double *, double, long);
 extern "C" hipError_t gom_launch_pcg(const PcgArgs *args, int max_blocks, int cooperative, hipStream_t st);
 extern "C" int gom_pcg_max_blocks(int device);
@@ -117,6 +118,11 @@ struct gomgpu_ctx {
   std::vector<gomgpu_batch *> batches;  // batches holding this context (a captured batch step
                                         // holds its pointers)
   unsigned stale = 0;  // SWAP_* pairs whose new array was exchanged and not yet rewritten
+  // The layer thicknesses are copied new -> old over the levels the last LAYERS stage wrote,
+  // which is exact when new == old held everywhere before it: not after set_state, nor
+  // when LAYERS ran other than once since the last update (then every entry is copied).
+  bool dz_full = true;
+  int layers_since_update = 0;
   double *h_stage = nullptr;  // pinned staging of the forcing arrays
   size_t stage_len = 0;
   hipEvent_t fev = nullptr;   // completion of the last forcing upload
@@ -559,6 +565,7 @@ extern "C" int gomgpu_set_state(gomgpu_ctx *c, const gomgpu_state *st) {
   const gomgpu_params &p = c->p;
   int rc;
   if ((rc = materialize(c, SWAP_ALL))) return rc;  // fields not given keep new == old
+  c->dz_full = true;
 #define UP_S(name, ext)                                                              \
   if (st->name) {                                                                    \
     if ((rc = upload(c, c->s.name, st->name, shape_of(#ext, p)))) return rc;         \
@@ -797,6 +804,7 @@ static int stage_impl(gomgpu_ctx *c, int id) {
       c->stale &= ~(unsigned)(SWAP_UN | SWAP_VN | SWAP_WN);  // rewritten in full
       break;
     case GOMGPU_STAGE_LAYERS:
+      c->layers_since_update++;
       k_layers_cell<<<nblk(d.E, BS), BS, 0, st>>>(c->m, c->s, d, p.MSL, p.dry_depth, err);
       k_layers_face<<<nblk(d.J, BS), BS, 0, st>>>(c->m, c->s, d, p.MSL, p.dry_depth, err);
       k_layers_node<<<nblk(d.P, BS), BS, 0, st>>>(c->m, c->s, d, p.MSL, p.dry_depth, err);
@@ -858,8 +866,9 @@ static int stage_impl(gomgpu_ctx *c, int id) {
       // (k_velocity_face), and each transported tracer (k_tvd_cellnode_out) -- exchanges
       // its two pointers instead of copying (the pair is then stale, see materialize). The
       // others are copied: eta_cell_new is the next solve's initial guess, and the layer
-      // kernels write dz and dzhalf over the wet range only. Contexts in a batch copy every
-      // pair, since a captured batch step holds their pointers.
+      // kernels write dz and dzhalf over the wet range only, so only that range is copied
+      // (k_update_dz). Contexts in a batch copy every pair in full, since a captured batch
+      // step holds their pointers and replays without these decisions.
       if (int rc = materialize(c, SWAP_ALL)) return rc;
       size_t E = d.E, J = d.J, P = d.P, L = d.L;
       const bool swap = c->batches.empty();
@@ -876,18 +885,32 @@ static int stage_impl(gomgpu_ctx *c, int id) {
           k_copy<<<nblk((long)n, BS), BS, 0, st>>>(*o, *nw, (int)n);
         }
       }
-      struct {
-        double *dst;
-        const double *src;
-        size_t n;
-      } cp[] = {{c->s.eta_cell, c->s.eta_cell_new, E},
-                {c->s.dz_cell, c->s.dz_cell_new, L * E},
-                {c->s.dzhalf_cell, c->s.dzhalf_cell_new, (L + 1) * E},
-                {c->s.dz_face, c->s.dz_face_new, L * J},
-                {c->s.dzhalf_face, c->s.dzhalf_face_new, (L + 1) * J},
-                {c->s.dz_node, c->s.dz_node_new, L * P},
-                {c->s.dzhalf_node, c->s.dzhalf_node_new, (L + 1) * P}};
-      for (auto &x : cp) k_copy<<<nblk((long)x.n, BS), BS, 0, st>>>(x.dst, x.src, (int)x.n);
+      k_copy<<<nblk((long)E, BS), BS, 0, st>>>(c->s.eta_cell, c->s.eta_cell_new, (int)E);
+      if (!swap || c->dz_full || c->layers_since_update != 1) {  // batch members: always all
+        struct {
+          double *dst;
+          const double *src;
+          size_t n;
+        } cp[] = {{c->s.dz_cell, c->s.dz_cell_new, L * E},
+                  {c->s.dzhalf_cell, c->s.dzhalf_cell_new, (L + 1) * E},
+                  {c->s.dz_face, c->s.dz_face_new, L * J},
+                  {c->s.dzhalf_face, c->s.dzhalf_face_new, (L + 1) * J},
+                  {c->s.dz_node, c->s.dz_node_new, L * P},
+                  {c->s.dzhalf_node, c->s.dzhalf_node_new, (L + 1) * P}};
+        for (auto &x : cp) k_copy<<<nblk((long)x.n, BS), BS, 0, st>>>(x.dst, x.src, (int)x.n);
+      } else {
+        k_update_dz<<<nblk((long)(L + 1) * E, BS), BS, 0, st>>>(c->s.dz_cell, c->s.dz_cell_new, c->s.dzhalf_cell,
+                                                              c->s.dzhalf_cell_new, c->s.top_layer_at_element,
+                                                              c->m.bottom_layer_at_element, (int)E, (int)L);
+        k_update_dz<<<nblk((long)(L + 1) * J, BS), BS, 0, st>>>(c->s.dz_face, c->s.dz_face_new, c->s.dzhalf_face,
+                                                              c->s.dzhalf_face_new, c->s.top_layer_at_face,
+                                                              c->m.bottom_layer_at_face, (int)J, (int)L);
+        k_update_dz<<<nblk((long)(L + 1) * P, BS), BS, 0, st>>>(c->s.dz_node, c->s.dz_node_new, c->s.dzhalf_node,
+                                                              c->s.dzhalf_node_new, c->s.top_layer_at_node,
+                                                              c->m.bottom_layer_at_node, (int)P, (int)L);
+      }
+      c->dz_full = false;
+      c->layers_since_update = 0;
       k_bars<<<nblk(d.P, BS), BS, 0, st>>>(c->m, c->s, d);
       break;
     }

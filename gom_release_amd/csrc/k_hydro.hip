@@ -1094,6 +1094,23 @@ __global__ void k_layers_face(DevMesh m, DevState s, DevDims d, double MSL, doub
 }
 
 // update_vertical_layers_1.f90:203-254 (nodes)
+// update_variables.f90:16-31 for one entity's layer thicknesses: new -> old over the levels
+// the layer kernels above write (dz on [bottom, top], dzhalf on [bottom-1, top] of a wet
+// entity). Every other entry of the new arrays still holds what the previous update copied,
+// so there new and old already agree. One thread per (level 0..L, entity).
+__global__ void k_update_dz(double *dz, const double *dz_new, double *dzh, const double *dzh_new,
+                            const int32_t *top, const int32_t *bot, int n, int L) {
+  long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= (long)(L + 1) * n) return;
+  int i = (int)(t % n), k = (int)(t / n);
+  int tp = top[i];
+  if (tp == 0) return;
+  int b = bot[i];
+  if (k < b - 1 || k > tp) return;
+  A0(dzh, k, i, n) = A0(dzh_new, k, i, n);
+  if (k >= b) AL(dz, k, i, n) = AL(dz_new, k, i, n);
+}
+
 __global__ void k_layers_node(DevMesh m, DevState s, DevDims d, double MSL, double dry, int *err) {
   int i = blockIdx.x * blockDim.x + threadIdx.x;
   const int P = d.P, L = d.L;
