@@ -7,6 +7,7 @@ on the device; only O(boundary) forcing crosses the bus each step.
 """
 from __future__ import annotations
 
+import math
 from typing import Optional
 
 import numpy as np
@@ -67,10 +68,37 @@ class HostForcing:
         return f
 
 
+def _compact(arrays):
+    """Arrays whose entries are all one bit pattern travel as (value, size): most of a cold
+    start (Av, Kv, q2, xl, uniform tracers, zeros) is constant, and the pickled transfer from
+    a preparation worker is otherwise gigabytes."""
+    out = {}
+    for k, v in arrays.items():
+        bits = v.view(np.uint64) if v.dtype == np.float64 else v
+        if v.size and (bits == bits[0]).all():
+            out[k] = ("const", v.dtype.str, v.size, v[0].item())
+        else:
+            out[k] = v
+    return out
+
+
+def _expand(dst, packed):
+    for k, v in packed.items():
+        if isinstance(v, tuple):
+            _, dt, n, val = v
+            a = dst.arr[k]
+            if a.dtype != np.dtype(dt) or a.size != n:
+                raise ValueError(f"{k}: packed array does not match the parameters")
+            if not (val == 0 and not (isinstance(val, float) and math.copysign(1.0, val) < 0)):
+                a[...] = val   # the arrays start as +0
+        else:
+            dst.arr[k] = v
+
+
 def pack_prepared(case: Case, precision: int = 0):
-    """prep.build in a picklable form (for preparing scenarios in worker processes)."""
+    """prep.build in a picklable, compact form (for preparing scenarios in worker processes)."""
     p, m, st = prep.build(case, precision)
-    return abi.params_to_dict(p), dict(m.arr), dict(st.arr)
+    return abi.params_to_dict(p), _compact(m.arr), _compact(st.arr)
 
 
 def unpack_prepared(packed):
@@ -78,10 +106,8 @@ def unpack_prepared(packed):
     p = abi.params_from_dict(pd)
     m = abi.mesh_arrays(p)
     s = abi.state_arrays(p)
-    for k, v in marr.items():
-        m.arr[k] = v
-    for k, v in sarr.items():
-        s.arr[k] = v
+    _expand(m, marr)
+    _expand(s, sarr)
     return p, m, s
 
 

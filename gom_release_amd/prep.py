@@ -249,30 +249,29 @@ def build(case: Case, precision: int = 0) -> Tuple["abi.Params", "abi.Arrays", "
     dj_int = np.sqrt((x_cell[c2] - x_cell[c1]) ** 2 + (y_cell[c2] - y_cell[c1]) ** 2)
     dj_bnd = np.sqrt((x_cell[c1] - x_face) ** 2 + (y_cell[c1] - y_face) ** 2) * 2.0
     delta_j = np.where(inter, dj_int, dj_bnd)
-    cos_t = np.empty(J)
-    sin_t = np.empty(J)
-    cos_t2 = np.empty(J)
-    sin_t2 = np.empty(J)
-    xc_l, yc_l = x_cell.tolist(), y_cell.tolist()
-    for j, (a_x1, a_x2, a_y1, a_y2, cA, cB) in enumerate(zip(xn1.tolist(), xn2.tolist(), yn1.tolist(),
-                                                           yn2.tolist(), c1.tolist(), fnb.tolist())):
-        th0 = math.atan2(a_x1 - a_x2, a_y2 - a_y1)
-        cos_t[j] = math.cos(th0)
-        sin_t[j] = math.sin(th0)
-        th1 = th0 if cB == 0 else math.atan2(yc_l[cB - 1] - yc_l[cA], xc_l[cB - 1] - xc_l[cA])
-        th2 = th1 - th0
-        cos_t2[j] = math.cos(th2)
-        sin_t2[j] = math.sin(th2)
+    # the angles through the C library's atan2/cos/sin (math), as the reference's libm does;
+    # numpy's vectorised versions may differ in the last bit
+    def libm(f, *args):
+        return np.fromiter(map(f, *(a.tolist() for a in args)), np.float64, J)
+    th0 = libm(math.atan2, xn1 - xn2, yn2 - yn1)
+    cos_t = libm(math.cos, th0)
+    sin_t = libm(math.sin, th0)
+    th1 = np.where(inter, libm(math.atan2, y_cell[c2] - y_cell[c1], x_cell[c2] - x_cell[c1]), th0)
+    th2 = th1 - th0
+    cos_t2 = libm(math.cos, th2)
+    sin_t2 = libm(math.sin, th2)
 
     # ---- boundaries (set_geometry_2.f90:20-82, read_main_inp.f90:841-887)
-    pair_face = {}
-    for j in range(J):
-        pair_face.setdefault((int(n1f[j]), int(n2f[j])), j + 1)
+    pkey = n1f.astype(np.int64) * (P + 1) + n2f
+    porder = np.argsort(pkey, kind="stable")   # equal keys keep face order: the first wins
+    pkey_sorted = pkey[porder]
 
     def find_cell_face(a, b):
-        j = pair_face.get((a, b), 0)
-        if j == 0:
+        key = int(a) * (P + 1) + int(b)
+        at = int(np.searchsorted(pkey_sorted, key))
+        if at >= J or pkey_sorted[at] != key:
             raise ValueError(f"boundary nodes {a},{b} are not a counter-clockwise cell edge")
+        j = int(porder[at]) + 1
         return int(fcell[j - 1]), j
     nob = len(case.open_boundaries)
     ob_cell_id = np.zeros(max(nob, 1), np.int32)
@@ -341,38 +340,40 @@ def build(case: Case, precision: int = 0) -> Tuple["abi.Params", "abi.Arrays", "
 
 
 def _layers0(h, eta, bl, z, L, MSL, dry, kind):
-    """update_vertical_layers_0.f90 for one entity class; returns (top, dz, dzhalf)."""
+    """update_vertical_layers_0.f90 for one entity class; returns (top, dz, dzhalf).
+
+    Whole-array form of the reference's per-entity statements (each value is the same
+    expression, so the result is the same bit for bit), in the ABI's layer-fastest order."""
     n = h.shape[0]
-    dz = np.zeros((L, n))
-    dzh = np.zeros((L + 1, n))
-    top = np.zeros(n, np.int32)
     tw = h + eta
     tmp = MSL + eta
     wet = tw > dry
-    for k in range(L - 1, -1, -1):
-        hit = wet & (top == 0) & (tmp > z[k]) & (tmp <= z[k + 1])
-        top = np.where(hit, k + 1, top)
-        dz[k, hit] = (tmp - z[k])[hit]
+    # top layer: the k with z[k] < MSL + eta <= z[k+1] (z increases)
+    kk = np.searchsorted(z, tmp, side="left") - 1
+    top = np.where(wet & (kk >= 0) & (kk <= L - 1), kk + 1, 0).astype(np.int32)
     if np.any(wet & (top < bl)):
         raise ValueError("initial surface below the bottom layer")
-    idx = np.arange(n)
     one = wet & (top == bl)
     many = wet & (top > bl)
-    dz[top[one] - 1, idx[one]] = tw[one]
+    idx = np.arange(n)
+    # interior layers of the deeper columns, then the top and bottom layers (disjoint rows)
+    K = np.arange(1, L + 1)[:, None]
+    dz = np.where(many & (K > bl) & (K < top), (z[1:] - z[:-1])[:, None], 0.0)
+    w, o, mm = idx[wet], idx[one], idx[many]
+    dz[top[w] - 1, w] = tmp[w] - z[top[w] - 1]
+    dz[top[o] - 1, o] = tw[o]
+    dz[bl[mm] - 1, mm] = z[bl[mm]] - (MSL - h[mm])
+    dzh = np.zeros((L + 1, n))
+    if L > 1:
+        K1 = np.arange(1, L)[:, None]
+        dzh[1:L] = np.where(many & (K1 >= bl) & (K1 <= top - 1), (dz[1:L] + dz[0:L - 1]) * 0.5, 0.0)
     if kind == "face":
-        dzh[top[one] - 1, idx[one]] = tw[one]
-    dzh[top[one], idx[one]] = tw[one]
-    dz[bl[many] - 1, idx[many]] = z[bl[many]] - (MSL - h[many])
-    for k in range(1, L + 1):
-        mid = many & (k > bl) & (k < top)
-        dz[k - 1, mid] = z[k] - z[k - 1]
-    for k in range(1, L):
-        mid = many & (k >= bl) & (k <= top - 1)
-        dzh[k, mid] = (dz[k, mid] + dz[k - 1, mid]) * 0.5
+        dzh[top[o] - 1, o] = tw[o]
+    dzh[top[o], o] = tw[o]
     if kind in ("face", "node"):
-        dzh[top[many], idx[many]] = dz[top[many] - 1, idx[many]]
-        dzh[bl[many] - 1, idx[many]] = dz[bl[many] - 1, idx[many]]
-    return top, dz, dzh
+        dzh[top[mm], mm] = dz[top[mm] - 1, mm]
+        dzh[bl[mm] - 1, mm] = dz[bl[mm] - 1, mm]
+    return top, np.asfortranarray(dz), np.asfortranarray(dzh)
 
 
 def cold_start(case: Case, p: "abi.Params", mesh: "abi.Arrays") -> "abi.Arrays":
@@ -403,47 +404,47 @@ def cold_start(case: Case, p: "abi.Params", mesh: "abi.Arrays") -> "abi.Arrays":
     s["dzhalf_node"] = dzhn
     nsub = {0: case.elm_sub_iter, 1: 1}.get(case.elm_backtrace_flag, 0)
     s["num_sub_elm_iteration"] = np.full(L * P, nsub, np.int32)
-    av = np.zeros((L + 1, J))
+    av = np.zeros((L + 1, J), order="F")
     av[1:, :] = case.av0
     s["Av"] = av
-    kv = np.zeros((L + 1, E))
+    kv = np.zeros((L + 1, E), order="F")
     kv[1:, :] = case.kv0
     s["Kv"] = kv
     # two-equation closure state: k at its floor, length scale at its minimum
-    s["q2"] = np.full((L + 1, J), case.q2min)
-    s["xl"] = np.full((L + 1, J), case.xlmin)
+    s["q2"] = np.full((L + 1, J), case.q2min, order="F")
+    s["xl"] = np.full((L + 1, J), case.xlmin, order="F")
     # tracers (allocate_variables.f90:...: salt 0, temp 20; read_salt_init.f90:25-48)
-    salt_node = np.zeros((L, P))
-    temp_node = np.full((L, P), 20.0)
+    salt_node = np.zeros((L, P), order="F")
+    temp_node = np.full((L, P), 20.0, order="F")
     if case.is_salt and case.salt_init is not None:
-        salt_node = np.asarray(case.salt_init, dtype=np.float64).T.copy()
+        salt_node = np.asarray(case.salt_init, dtype=np.float64).T.copy(order="F")
     elif case.baroclinic_flag == 1 and not case.is_salt:
         salt_node[:] = case.ref_salt
     if case.is_temp and case.temp_init is not None:
-        temp_node = np.asarray(case.temp_init, dtype=np.float64).T.copy()
+        temp_node = np.asarray(case.temp_init, dtype=np.float64).T.copy(order="F")
     elif case.baroclinic_flag == 1 and not case.is_temp:
         temp_node[:] = case.ref_temp
     nf = mesh.view2("nodenum_at_face")
 
     def face_mean(a):
-        return (a[:, nf[0] - 1] + a[:, nf[1] - 1]) * 0.5
+        return np.asfortranarray((a[:, nf[0] - 1] + a[:, nf[1] - 1]) * 0.5)
 
     def cell_mean(a):
-        acc = np.zeros((L, E))
+        acc = np.zeros((L, E), order="F")
         for l in range(4):
             m_ = l < N
             acc = np.where(m_[None, :], acc + a[:, np.maximum(nc[l], 1) - 1], acc)
-        return acc / N[None, :]
+        return np.asfortranarray(acc / N[None, :])
     if case.is_salt and case.salt_init is not None:
         salt_face, salt_cell = face_mean(salt_node), cell_mean(salt_node)
     else:
-        salt_face = np.full((L, J), salt_node[0, 0] if P else 0.0)
-        salt_cell = np.full((L, E), salt_node[0, 0] if P else 0.0)
+        salt_face = np.full((L, J), salt_node[0, 0] if P else 0.0, order="F")
+        salt_cell = np.full((L, E), salt_node[0, 0] if P else 0.0, order="F")
     if case.is_temp and case.temp_init is not None:
         temp_face, temp_cell = face_mean(temp_node), cell_mean(temp_node)
     else:
-        temp_face = np.full((L, J), temp_node[0, 0] if P else 20.0)
-        temp_cell = np.full((L, E), temp_node[0, 0] if P else 20.0)
+        temp_face = np.full((L, J), temp_node[0, 0] if P else 20.0, order="F")
+        temp_cell = np.full((L, E), temp_node[0, 0] if P else 20.0, order="F")
     for nm, v in (("salt_node", salt_node), ("salt_face", salt_face), ("salt_cell", salt_cell),
                   ("salt_cell_new", salt_cell), ("temp_node", temp_node), ("temp_face", temp_face),
                   ("temp_cell", temp_cell), ("temp_cell_new", temp_cell)):
