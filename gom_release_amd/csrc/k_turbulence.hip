@@ -217,23 +217,36 @@ __global__ void __launch_bounds__(256) k_turb_solve(DevMesh m, DevState s, DevFo
   double xlbot = dmaxf(p.xlmin2, AL(s.dz_face_new, kb, j, J) / 2.0 * (double)0.4f);
   int nq = kt - kb + 1;
   double cm3 = p.cmiu0 * p.cmiu0 * p.cmiu0;
-  // ---- k equation (rows top down; row k <-> layer kl = kt - k + 1)
-  double den = 0.0, cprev = 0.0, x = 0.0;
+  // ---- the k and psi equations (:233-276 and :281-347) in one top-down pass: the two
+  // systems share their inputs and are independent, so each row's inputs are read once and
+  // both eliminations advance together (rows top down; row k <-> layer kl = kt - k + 1).
+  // The k system's factors go to w.tri_gam (by row), the psi system's to w.tk_shear (by
+  // layer, after the row's shear has been read); solutions in w.tk_q2tmp and w.tk_x.
+  const double cpow = gls_pow(p.cmiu0, p.rpub);
+  double den = 0.0, cprev = 0.0, x = 0.0;     // k system
+  double denp = 0.0, cprevp = 0.0, xp = 0.0;  // psi system
   for (int k = 1; k <= nq; k++) {
     int kl = kt - k + 1;
     double dzk = AL(s.dz_face_new, kl, j, J);
+    double qdk = A0(w.tk_qd, kl, j, J), qd2k = A0(w.tk_qd2, kl, j, J);
+    double dzhk = A0(s.dzhalf_face_new, kl, j, J), dzhm = A0(s.dzhalf_face_new, kl - 1, j, J);
+    double qdm = A0(w.tk_qd, kl - 1, j, J), qd2m = A0(w.tk_qd2, kl - 1, j, J);
+    double q2bt = AL(w.tk_q2bt, kl, j, J), xlbt = AL(w.tk_xlbt, kl, j, J);
+    double avk = A0(w.tk_av, kl, j, J), avm = A0(w.tk_av, kl - 1, j, J);
+    double kvk = A0(w.tk_kv, kl, j, J), kvm = A0(w.tk_kv, kl - 1, j, J);
+    double shear = AL(w.tk_shear, kl, j, J), rz = AL(w.tk_rz, kl, j, J);
+    // k row
     double eleft = 0.0, eright = 0.0, a = 0.0, c = 0.0;
     if (k > 1) {
-      eleft = -(A0(w.tk_qd, kl, j, J) * dt / A0(s.dzhalf_face_new, kl, j, J));
+      eleft = -(qdk * dt / dzhk);
       a = eleft;
     }
     if (k < nq) {
-      eright = -(A0(w.tk_qd, kl - 1, j, J) * dt / A0(s.dzhalf_face_new, kl - 1, j, J));
+      eright = -(qdm * dt / dzhm);
       c = eright;
     }
-    double q2bt = AL(w.tk_q2bt, kl, j, J), xlbt = AL(w.tk_xlbt, kl, j, J);
-    double prod = (A0(w.tk_av, kl, j, J) + A0(w.tk_av, kl - 1, j, J)) / 2.0 * AL(w.tk_shear, kl, j, J);
-    double buoy = (A0(w.tk_kv, kl, j, J) + A0(w.tk_kv, kl - 1, j, J)) / 2.0 * AL(w.tk_rz, kl, j, J);
+    double prod = (avk + avm) / 2.0 * shear;
+    double buoy = (kvk + kvm) / 2.0 * rz;
     double diss = cm3 * sqrt(q2bt) / xlbt;
     double pplus, pminus;
     if (prod + buoy > 0.0) {
@@ -264,32 +277,17 @@ __global__ void __launch_bounds__(256) k_turb_solve(DevMesh m, DevState s, DevFo
     }
     AL(w.tk_q2tmp, kl, j, J) = x;
     cprev = c;
-  }
-  {  // back substitution and the boundary values (:268-278)
-    int kl = kb;  // row nq
-    AL(w.tk_q2tmp, kl, j, J) = kl == kt ? q2fs : (kl == kb ? q2bot : dmaxf(x, p.q2min));
-    for (int k = nq - 1; k >= 1; k--) {
-      kl = kt - k + 1;
-      x = AL(w.tk_q2tmp, kl, j, J) - AL(w.tri_gam, k + 1, j, J) * x;
-      AL(w.tk_q2tmp, kl, j, J) = kl == kt ? q2fs : (kl == kb ? q2bot : dmaxf(x, p.q2min));
-    }
-  }
-  // ---- psi equation (:281-347)
-  for (int k = 1; k <= nq; k++) {
-    int kl = kt - k + 1;
-    double dzk = AL(s.dz_face_new, kl, j, J);
-    double eleft = 0.0, eright = 0.0, a = 0.0, c = 0.0;
-    double qd2k = A0(w.tk_qd2, kl, j, J);
+    // psi row
+    double eleftp = 0.0, erightp = 0.0, ap = 0.0, cp = 0.0;
     if (k > 1) {
-      eleft = -(qd2k * dt / A0(s.dzhalf_face_new, kl, j, J));
-      a = eleft;
+      eleftp = -(qd2k * dt / dzhk);
+      ap = eleftp;
     }
     if (k < nq) {
-      eright = -(A0(w.tk_qd2, kl - 1, j, J) * dt / A0(s.dzhalf_face_new, kl - 1, j, J));
-      c = eright;
+      erightp = -(qd2m * dt / dzhm);
+      cp = erightp;
     }
-    double q2bt = AL(w.tk_q2bt, kl, j, J), xlbt = AL(w.tk_xlbt, kl, j, J), rz = AL(w.tk_rz, kl, j, J);
-    double bb = dzk - eleft - eright;
+    double bb = dzk - eleftp - erightp;
     double cpsi2p;
     if (p.model == GOMGPU_GLS_MY || p.model == GOMGPU_GLS_KL) {
       double zctr = m.z_level[kl] - dzk / 2.0;
@@ -301,9 +299,9 @@ __global__ void __launch_bounds__(256) k_turb_solve(DevMesh m, DevState s, DevFo
     } else {
       cpsi2p = p.cpsi2;
     }
-    double diss = cpsi2p * cm3 * sqrt(q2bt) / xlbt;
-    bb = bb + dt * dzk * diss;
-    double prod = p.cpsi1 * (A0(w.tk_av, kl, j, J) + A0(w.tk_av, kl - 1, j, J)) / 2.0 * AL(w.tk_shear, kl, j, J);
+    double dissp = cpsi2p * cm3 * sqrt(q2bt) / xlbt;
+    bb = bb + dt * dzk * dissp;
+    double prodp = p.cpsi1 * (avk + avm) / 2.0 * shear;
     double cpsi3;
     if (p.model == GOMGPU_GLS_MY) {
       cpsi3 = (double)0.9f;
@@ -317,52 +315,57 @@ __global__ void __launch_bounds__(256) k_turb_solve(DevMesh m, DevState s, DevFo
         default: cpsi3 = (double)0.1f; break;
       }
     }
-    double buoy = cpsi3 * (A0(w.tk_kv, kl, j, J) + A0(w.tk_kv, kl - 1, j, J)) / 2.0 * rz;
-    double pplus, pminus;
-    if (prod + buoy > 0.0) {
-      pplus = prod + buoy;
-      pminus = 0.0;
+    double buoyp = cpsi3 * (kvk + kvm) / 2.0 * rz;
+    double pplusp, pminusp;
+    if (prodp + buoyp > 0.0) {
+      pplusp = prodp + buoyp;
+      pminusp = 0.0;
     } else {
-      pplus = 0.0;
-      pminus = -prod - buoy;
+      pplusp = 0.0;
+      pminusp = -prodp - buoyp;
     }
-    bb = bb + pminus / q2bt * dt * dzk;
+    bb = bb + pminusp / q2bt * dt * dzk;
     if (kl == kb || kl == kt) bb = bb + dt * (double)0.4f * p.rnub * qd2k / xlbt;
-    double tmp = gls_pow(p.cmiu0, p.rpub) * gls_pow(q2bt, p.rmub) * gls_pow(xlbt, p.rnub);
-    double r = dzk * tmp * (1.0 + dt * pplus / q2bt);
+    double tmp = cpow * gls_pow(q2bt, p.rmub) * gls_pow(xlbt, p.rnub);
+    double rp = dzk * tmp * (1.0 + dt * pplusp / q2bt);
     if (k == 1) {
       if (bb == 0.0) {
         latch(err, GOMGPU_ERR_TRIDIAG_PIVOT, j + 1);
         return;
       }
-      den = bb;
-      x = r / den;
+      denp = bb;
+      xp = rp / denp;
     } else {
-      double gam = cprev / den;
-      den = bb - a * gam;
-      if (den == 0.0) {
+      double gam = cprevp / denp;
+      denp = bb - ap * gam;
+      if (denp == 0.0) {
         latch(err, GOMGPU_ERR_TRIDIAG_PIVOT, j + 1);
         return;
       }
-      x = (r - a * x) / den;
-      AL(w.tri_gam, k, j, J) = gam;
+      xp = (rp - ap * xp) / denp;
+      AL(w.tk_shear, kl, j, J) = gam;  // this row's shear has been read
     }
-    AL(w.tk_x, kl, j, J) = x;
-    cprev = c;
+    AL(w.tk_x, kl, j, J) = xp;
+    cprevp = cp;
   }
-  // back substitution and the length scale (:351-368)
+  // ---- back substitutions, bottom up, both systems (:268-278, :351-368): the k solution
+  // with its boundary values, then the length scale from the psi solution of the same row
+  const double cpow_inv = gls_pow(p.cmiu0, -p.rpub);
   for (int k = nq; k >= 1; k--) {
     int kl = kt - k + 1;
-    if (k < nq) x = AL(w.tk_x, kl, j, J) - AL(w.tri_gam, k + 1, j, J) * x;
-    double q2l = dmaxf(x, p.psimin);
-    double q2tmp = AL(w.tk_q2tmp, kl, j, J), rz = AL(w.tk_rz, kl, j, J);
+    if (k < nq) x = AL(w.tk_q2tmp, kl, j, J) - AL(w.tri_gam, k + 1, j, J) * x;
+    const double q2tmp = kl == kt ? q2fs : (kl == kb ? q2bot : dmaxf(x, p.q2min));
+    AL(w.tk_q2tmp, kl, j, J) = q2tmp;
+    if (k < nq) xp = AL(w.tk_x, kl, j, J) - AL(w.tk_shear, kl - 1, j, J) * xp;
+    double q2l = dmaxf(xp, p.psimin);
+    double rz = AL(w.tk_rz, kl, j, J);
     double xv;
     if (kl == kt)
       xv = xlfs;
     else if (kl == kb)
       xv = xlbot;
     else
-      xv = gls_pow(q2l * gls_pow(p.cmiu0, -p.rpub) * gls_pow(q2tmp, -p.rmub), 1.0 / p.rnub);
+      xv = gls_pow(q2l * cpow_inv * gls_pow(q2tmp, -p.rmub), 1.0 / p.rnub);
     if (rz < 0.0) {
       double upper = sqrt(-(double)0.56f * q2tmp / rz);
       xv = dminf(xv, upper);
