@@ -35,10 +35,16 @@ TURBULENCE = {"off": dict(turbulence_flag=0),
               "gls": dict(turbulence_flag=3, turbulence_model="kl", stability_function="GA")}
 
 
-def make_case(cells: str, turbulence: str = "gls"):
+# The PCG stop test of pre_conj_grad.f90:199 (sqrt(delta^2/delta_0^2) < tol) at the value of
+# the reference's own project input (GOM_Rv1.1.0/Projects/MB_test/2_baroclinic_test/input/
+# main.inp:232, card C6: max_iteration_pcg 1000, error_tolerance 5.e-6).
+PCG_TOLERANCE = 5.0e-6
+
+
+def make_case(cells: str, turbulence: str = "gls", pcg_tolerance: float = PCG_TOLERANCE):
     from gom_release_amd import meshgen
     kw = dict(nlayer=20, dt=120.0, river_q=500.0, trans_sub_iter=2, h_flux_limiter=1, v_flux_limiter=1,
-              **TURBULENCE[turbulence])
+              error_tolerance=pcg_tolerance, max_iteration_pcg=1000, **TURBULENCE[turbulence])
     if cells == "200k":
         return meshgen.estuary_case(ncol=400, nrep=100, **kw)
     if cells == "20k":
@@ -206,6 +212,8 @@ def main() -> int:
     ap.add_argument("--cells", default="200k")
     ap.add_argument("--turbulence", choices=sorted(TURBULENCE), default="gls",
                     help="vertical closure of the workload (gls: k-kl GLS, Galperin stability functions)")
+    ap.add_argument("--pcg-tolerance", type=float, default=PCG_TOLERANCE,
+                    help="PCG stop tolerance on r.M^-1 r / r0.M^-1 r0 (default: the reference's main.inp)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--profile-steps", type=int, default=5)
     ap.add_argument("--no-autotune", action="store_true", help="keep the default launch shapes")
@@ -228,7 +236,7 @@ def main() -> int:
         copy_calibration(args.copy_calibration << 20)
         print(f"[bench] calibration copy of {args.copy_calibration} MiB done", file=sys.stderr, flush=True)
     wind = (10.0, 0.0)
-    case = make_case(args.cells, args.turbulence)
+    case = make_case(args.cells, args.turbulence, args.pcg_tolerance)
     prec = 0 if args.precision == "fp64" else 1
     print(f"[bench] building {case.maxele}-cell x {case.maxlayer}-layer case", file=sys.stderr, flush=True)
     sc = Scenario(case, precision=prec, wind=wind)
@@ -294,7 +302,8 @@ def main() -> int:
         "config": {"workload": f"synthetic estuary, {case.maxele} mixed tri/quad cells x {case.maxlayer} "
                                f"z-layers, dt={case.dt:g} s, TVD minmod x2 sub-cycles, M2+K1, 500 m3/s river, "
                                f"10 m/s wind, " + ("GLS k-kl turbulence closure (Galperin)"
-                                                   if args.turbulence == "gls" else "constant Av/Kv"),
+                                                   if args.turbulence == "gls" else "constant Av/Kv")
+                               + f", PCG tolerance {case.error_tolerance:g}",
                    "turbulence": args.turbulence,
                    "cells": case.maxele, "faces": sc.params.maxface, "nodes": case.maxnod,
                    "layers": case.maxlayer, "active_cell_layers": int(0.5 * (a0 + a1)),

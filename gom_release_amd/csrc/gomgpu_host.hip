@@ -899,13 +899,13 @@ static int stage_impl(gomgpu_ctx *c, int id) {
                   {c->s.dzhalf_node, c->s.dzhalf_node_new, (L + 1) * P}};
         for (auto &x : cp) k_copy<<<nblk((long)x.n, BS), BS, 0, st>>>(x.dst, x.src, (int)x.n);
       } else {
-        k_update_dz<<<nblk((long)(L + 1) * E, BS), BS, 0, st>>>(c->s.dz_cell, c->s.dz_cell_new, c->s.dzhalf_cell,
+        k_update_dz<<<nblk((long)E, BS), BS, 0, st>>>(c->s.dz_cell, c->s.dz_cell_new, c->s.dzhalf_cell,
                                                               c->s.dzhalf_cell_new, c->s.top_layer_at_element,
                                                               c->m.bottom_layer_at_element, (int)E, (int)L);
-        k_update_dz<<<nblk((long)(L + 1) * J, BS), BS, 0, st>>>(c->s.dz_face, c->s.dz_face_new, c->s.dzhalf_face,
+        k_update_dz<<<nblk((long)J, BS), BS, 0, st>>>(c->s.dz_face, c->s.dz_face_new, c->s.dzhalf_face,
                                                               c->s.dzhalf_face_new, c->s.top_layer_at_face,
                                                               c->m.bottom_layer_at_face, (int)J, (int)L);
-        k_update_dz<<<nblk((long)(L + 1) * P, BS), BS, 0, st>>>(c->s.dz_node, c->s.dz_node_new, c->s.dzhalf_node,
+        k_update_dz<<<nblk((long)P, BS), BS, 0, st>>>(c->s.dz_node, c->s.dz_node_new, c->s.dzhalf_node,
                                                               c->s.dzhalf_node_new, c->s.top_layer_at_node,
                                                               c->m.bottom_layer_at_node, (int)P, (int)L);
       }

@@ -1097,18 +1097,32 @@ __global__ void k_layers_face(DevMesh m, DevState s, DevDims d, double MSL, doub
 // update_variables.f90:16-31 for one entity's layer thicknesses: new -> old over the levels
 // the layer kernels above write (dz on [bottom, top], dzhalf on [bottom-1, top] of a wet
 // entity). Every other entry of the new arrays still holds what the previous update copied,
-// so there new and old already agree. One thread per (level 0..L, entity).
+// so there new and old already agree. One thread per entity walking its levels in chunks of
+// four (loads of a chunk issued before its stores); a level row is contiguous across the
+// threads of a wave.
 __global__ void k_update_dz(double *dz, const double *dz_new, double *dzh, const double *dzh_new,
                             const int32_t *top, const int32_t *bot, int n, int L) {
-  long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (t >= (long)(L + 1) * n) return;
-  int i = (int)(t % n), k = (int)(t / n);
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
   int tp = top[i];
   if (tp == 0) return;
   int b = bot[i];
-  if (k < b - 1 || k > tp) return;
-  A0(dzh, k, i, n) = A0(dzh_new, k, i, n);
-  if (k >= b) AL(dz, k, i, n) = AL(dz_new, k, i, n);
+  for (int k0 = b - 1; k0 <= tp; k0 += 4) {
+    double h[4], z[4];
+#pragma unroll
+    for (int q = 0; q < 4; q++) {
+      int k = k0 + q <= tp ? k0 + q : tp;
+      h[q] = A0(dzh_new, k, i, n);
+      z[q] = AL(dz_new, k >= 1 ? k : 1, i, n);
+    }
+#pragma unroll
+    for (int q = 0; q < 4; q++) {
+      int k = k0 + q;
+      if (k > tp) break;
+      A0(dzh, k, i, n) = h[q];
+      if (k >= b) AL(dz, k, i, n) = z[q];
+    }
+  }
 }
 
 __global__ void k_layers_node(DevMesh m, DevState s, DevDims d, double MSL, double dry, int *err) {
