@@ -8,6 +8,7 @@ on the device; only O(boundary) forcing crosses the bus each step.
 from __future__ import annotations
 
 import math
+import os
 from typing import Optional
 
 import numpy as np
@@ -25,7 +26,7 @@ class HostForcing:
         self.case, self.params, self.mesh = case, params, mesh
         self.forcing = abi.ForcingArrays(params)
         if wind[0] or wind[1]:
-            gt, wsn, wst = fz.wind_stress(mesh, *wind)
+            gt, wsn, wst = fz.wind_stress(mesh, *wind, rho_a=case.rho_a, rho_o=case.rho_o)
             self.forcing["Gamma_T"] = gt
             self.forcing["wind_stress_normal"] = wsn
             self.forcing["wind_stress_tangnt"] = wst
@@ -45,7 +46,7 @@ class HostForcing:
         f.scalars["elapsed_time"] = elapsed
         f.scalars["spinup_function_tide"] = sp_tide
         f.scalars["spinup_function_baroclinic"] = sp_bc
-        old, new = fz.boundary_eta(c, it, sp_tide)
+        old, new = fz.boundary_eta(c, it, sp_tide, self.julian_day)
         f["eta_at_ob_old"] = old
         f["eta_at_ob_new"] = new
         ob, qb = fz.tvd_boundary_concentrations(c, self.julian_day, sp_bc, self.params.maxtran2,
@@ -115,7 +116,7 @@ class Scenario:
     """One scenario advanced on the device."""
 
     def __init__(self, case: Case, precision: int = 0, wind=(0.0, 0.0), model: Optional[host.GpuModel] = None,
-                 prepared=None):
+                 prepared=None, model_factory=None):
         self.case = case
         if prepared is None:
             self.params, self.mesh, state0 = prep.build(case, precision)
@@ -123,7 +124,9 @@ class Scenario:
             self.params, self.mesh, state0 = unpack_prepared(prepared)
         if case.rivers:
             self.params.river_on_device = 1
-        self.model = model or host.GpuModel(self.mesh, self.params)
+        self.model = model or (model_factory or host.GpuModel)(self.mesh, self.params)
+        # a model that does not derive the river face velocity itself gets it from the host
+        self.host_river = bool(case.rivers) and getattr(self.model, "host_river_velocity", False)
         self.model.set_state(state0)
         self.model.stage("eos")          # initial density, as prepare_gom does
         self.host_forcing = HostForcing(case, self.params, self.mesh, wind)
@@ -134,16 +137,60 @@ class Scenario:
     def stage_forcing(self) -> None:
         """Host forcing of the next step, uploaded asynchronously (before a batch step)."""
         self.it += 1
-        self.host_forcing.advance(self.it)
-        self.model.set_forcing(self.forcing, only=DYNAMIC)
+        if self.host_river:
+            self.host_forcing.advance(self.it, eta_node=self.fields(["eta_node"])["eta_node"])
+            self.model.set_forcing(self.forcing, only=DYNAMIC + ["Qu_boundary"])
+        else:
+            self.host_forcing.advance(self.it)
+            self.model.set_forcing(self.forcing, only=DYNAMIC)
 
     def step(self) -> None:
         self.stage_forcing()
         self.model.step()
+        if self.outputs is not None:
+            self.write_outputs()
 
     def run(self, n: int) -> None:
         for _ in range(n):
             self.step()
+
+    # ---- output files (prepare_gom.f90:216-227 opens the dumps; main.f90:274-295 writes)
+    outputs = None
+
+    def fields(self, names) -> "abi.Arrays":
+        """The named state fields, read back from the device."""
+        s = abi.state_arrays(self.params)
+        self.model.get_state(s, only=list(names))
+        return s
+
+    def open_outputs(self, out_dir: str) -> None:
+        """Write the case's state dumps and restart files into `out_dir` after each step."""
+        from . import gomout
+        os.makedirs(out_dir, exist_ok=True)
+        self.outputs = out_dir
+        self.dumps = [gomout.DumpWriter(out_dir, kind, spec, self.params.maxlayer)
+                      for kind, spec in (("2D", self.case.dump2d), ("3D", self.case.dump3d))
+                      if spec is not None]
+
+    def write_outputs(self) -> list:
+        """After step `it`: the dumps the schedule asks for, then the restart file when
+        restart_out is -1 at the last step or 1 every restart_freq steps. Returns the
+        files written."""
+        from . import gomout
+        c, it = self.case, self.it
+        out = []
+        for d in self.dumps:
+            if d.step(it, self.host_forcing.julian_day, self.fields):
+                out.append(d.files[-1])
+        if (c.restart_out == -1 and it == c.ndt) or (c.restart_out == 1 and it % c.restart_freq == 0):
+            out.append(gomout.write_restart(gomout.restart_name(self.outputs, it),
+                                            self.fields(gomout.RESTART_FIELDS)))
+        return out
+
+    def close_outputs(self) -> None:
+        for d in getattr(self, "dumps", []):
+            d.close()
+        self.outputs = None
 
     def active_layers(self):
         """(face-layers, cell-layers, node-layers) of the wet entities in the current state."""

@@ -19,13 +19,16 @@ DEG2RAD = math.pi / 180.0
 
 def series_value(s: Series, u2: float, ref_diff_days: int = 0) -> Optional[float]:
     """Linear interpolation of a series at time u2 [s], as the reference does it
-    (solve_transport_equation_v19.f90:245-258, compute_boundary_eta.f90:96-108)."""
-    t = [v * 86400.0 for v in s.times_days]
+    (solve_transport_equation_v19.f90:245-258, compute_boundary_eta.f90:96-108): the first
+    interval whose ends bracket u2; None when no interval does."""
+    t = s.seconds()
+    v = s.scaled()
+    shift = ref_diff_days * 86400.0
     for i in range(1, len(t)):
-        u1 = t[i - 1] - ref_diff_days * 86400.0
-        u3 = t[i] - ref_diff_days * 86400.0
+        u1 = t[i - 1] - shift
+        u3 = t[i] - shift
         if u1 <= u2 <= u3:
-            v1, v3 = float(s.values[i - 1]), float(s.values[i])
+            v1, v3 = v[i - 1], v[i]
             return (u2 - u1) * (v3 - v1) / (u3 - u1) + v1
     return None
 
@@ -49,25 +52,45 @@ def julian_day0(case: Case) -> float:
     return jd + (d - 1) + h / 24.0 + mi / 1440.0 + s / 86400.0
 
 
-def boundary_eta(case: Case, it: int, sp_tide: float):
-    """eta_at_ob_old/new for step `it` (compute_boundary_eta.f90:72-90, harmonic type)."""
+def boundary_eta(case: Case, it: int, sp_tide: float, julian_day: Optional[float] = None):
+    """eta_at_ob_old/new for step `it` (compute_boundary_eta.f90:72-124): harmonic
+    boundaries (type 1) and elevation series (type 2, at julian_day*86400 - dt and
+    julian_day*86400). A series that does not cover the time keeps the value of the
+    boundary before it in the same call, as the reference's local variable does."""
     dt = case.dt
     elapsed = it * dt
     nob = len(case.open_boundaries)
     old = np.zeros(max(nob, 1))
     new = np.zeros(max(nob, 1))
+    rdd = case.reference_diff_days()
+    v2_old = v2_new = 0.0
+    memo = {}
     for i, b in enumerate(case.open_boundaries):
-        if b.eta_type != 1:
-            raise NotImplementedError("only harmonic open boundaries are generated")
-        shift, cons = case.harmonics[b.harmonic_id - 1]
-        o = 0.0
-        n = 0.0
-        for (per, amp, ph, fac, v0, _nm) in cons:
-            w = 2.0 * math.pi / (per * 3600.0)
-            o = o + sp_tide * amp * fac * math.cos(w * (elapsed - dt) + v0 * DEG2RAD - ph * DEG2RAD - shift * DEG2RAD)
-            n = n + sp_tide * amp * fac * math.cos(w * elapsed + v0 * DEG2RAD - ph * DEG2RAD - shift * DEG2RAD)
-        old[i] = o
-        new[i] = n
+        if b.eta_type == 1:
+            shift, cons = case.harmonics[b.harmonic_id - 1]
+            o = 0.0
+            n = 0.0
+            for (per, amp, ph, fac, v0, _nm) in cons:
+                w = 2.0 * math.pi / (per * 3600.0)
+                o = o + sp_tide * amp * fac * math.cos(w * (elapsed - dt) + v0 * DEG2RAD - ph * DEG2RAD - shift * DEG2RAD)
+                n = n + sp_tide * amp * fac * math.cos(w * elapsed + v0 * DEG2RAD - ph * DEG2RAD - shift * DEG2RAD)
+            old[i] = o
+            new[i] = n
+        elif b.eta_type == 2:
+            if julian_day is None:
+                raise ValueError("elevation-series boundaries need the julian day")
+            if b.eta_ser_id not in memo:
+                ser = case.eta_series[b.eta_ser_id - 1]
+                memo[b.eta_ser_id] = (series_value(ser, julian_day * 86400.0 - dt, rdd),
+                                      series_value(ser, julian_day * 86400.0, rdd))
+            vo, vn = memo[b.eta_ser_id]
+            v2_old = v2_old if vo is None else vo
+            v2_new = v2_new if vn is None else vn
+            old[i] = sp_tide * v2_old
+            new[i] = sp_tide * v2_new
+        else:
+            raise NotImplementedError("ob_eta_type -1 (elevation radiated from interior cells) "
+                                      "needs the device state on the host each step")
     return old, new
 
 
@@ -129,6 +152,6 @@ def river_discharge(case: Case, julian_day: float):
     q = np.zeros(max(len(case.rivers), 1))
     u2 = julian_day * 86400.0
     for i, r in enumerate(case.rivers):
-        v = series_value(case.q_series[r.q_ser_id - 1], u2)
+        v = series_value(case.q_series[r.q_ser_id - 1], u2, case.reference_diff_days())
         q[i] = (v if v is not None else 0.0) * r.portion
     return q

@@ -37,7 +37,8 @@ def params_for(case: Case, precision: int = 0) -> "abi.Params":
     tid = ([1] if case.is_salt else []) + ([2] if case.is_temp else [])
     tid += [0, 0]
     d = dict(
-        maxnod=case.maxnod, maxele=case.maxele, maxface=0, maxlayer=L, max_no_neighbor_node=10,
+        maxnod=case.maxnod, maxele=case.maxele, maxface=0, maxlayer=L,
+        max_no_neighbor_node=int(case.max_no_neighbor_node),
         num_ob_cell=len(case.open_boundaries), num_Qb_cell=len(case.rivers),
         maxtran2=ntr if case.transport_flag == 1 else 0, tran_id1=tid[0], tran_id2=tid[1],
         advection_flag=case.advection_flag, ELM_backtrace_flag=case.elm_backtrace_flag,
@@ -47,7 +48,8 @@ def params_for(case: Case, precision: int = 0) -> "abi.Params":
         h_flux_limiter=case.h_flux_limiter, v_flux_limiter=case.v_flux_limiter,
         heat_option=1 if case.is_temp else 0, max_iteration_pcg=case.max_iteration_pcg,
         precision=precision,
-        dt=case.dt, theta=case.theta, gravity=9.81, rho_o=1025.0, MSL=case.msl,
+        dt=case.dt, theta=case.theta, gravity=float(case.gravity),
+        rho_o=float(case.rho_o), MSL=case.msl,
         dry_depth=case.dry_depth, adv_onoff_depth=case.adv_onoff_depth,
         smagorinsky_parameter=case.smagorinsky, Ah_0=case.ah0, Kh_0=case.kh0,
         von_Karman=case.von_karman, error_tolerance=case.error_tolerance,
@@ -97,13 +99,13 @@ def build(case: Case, precision: int = 0) -> Tuple["abi.Params", "abi.Arrays", "
     """Return (params, mesh, state) in the ABI layout for a generated case."""
     x = np.asarray(case.x, dtype=np.float64)
     y = np.asarray(case.y, dtype=np.float64)
-    h_node = np.asarray(case.depth, dtype=np.float64) + 0.0
+    h_node = np.asarray(case.depth, dtype=np.float64) + float(case.h_node_adjust)   # read_node_inp.f90:64
     P = x.shape[0]
     cells = np.asarray(case.cells, dtype=np.int64)
     E = cells.shape[0]
     N = np.where(cells[:, 3] != 0, 4, 3)
     L = case.maxlayer
-    NB = 10
+    NB = int(case.max_no_neighbor_node)
     nc = cells.T.copy()                         # (4, E), 1-based node ids
 
     # ---- area (read_cell_inp.f90:67-80)
@@ -335,7 +337,7 @@ def build(case: Case, precision: int = 0) -> Tuple["abi.Params", "abi.Arrays", "
                 ob_cell_id=ob_cell_id, Qb_cell_id=qb_cell)
     for k, v in vals.items():
         mesh[k] = v
-    state = cold_start(case, p, mesh)
+    state = hot_start(case, p, mesh) if case.restart_in == 1 else cold_start(case, p, mesh)
     return p, mesh, state
 
 
@@ -449,4 +451,22 @@ def cold_start(case: Case, p: "abi.Params", mesh: "abi.Arrays") -> "abi.Arrays":
                   ("salt_cell_new", salt_cell), ("temp_node", temp_node), ("temp_face", temp_face),
                   ("temp_cell", temp_cell), ("temp_cell_new", temp_cell)):
         s[nm] = v
+    return s
+
+
+def hot_start(case: Case, p: "abi.Params", mesh: "abi.Arrays") -> "abi.Arrays":
+    """Initial state after prepare_gom with restart_in = 1 (EOS excluded).
+
+    prepare_gom.f90:80-92 builds the layers from the surface as it stands before the
+    restart is read (zero), and setup_hot_start.f90:14-62 then overwrites only the restart
+    fields. The initial tracer files are not read (read_main_inp.f90:316-321), so the
+    tracers the restart does not hold keep their allocation values (salt 0, temperature
+    20; read_main_inp.f90:290-303 for a tracer that is not transported)."""
+    from dataclasses import replace
+    from .gomout import RESTART_FIELDS
+    if case.hot_start is None:
+        raise ValueError("restart_in = 1 needs the restart records (Case.hot_start)")
+    s = cold_start(replace(case, salt_init=None, temp_init=None), p, mesh)
+    for f in RESTART_FIELDS:
+        s[f] = case.hot_start[f]
     return s

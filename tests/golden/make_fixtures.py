@@ -11,7 +11,10 @@ The reference is the GOM v1.1.0 Fortran code compiled from /root/reference by
 oracle/Makefile (`make -C oracle ref`) and driven by oracle/ref_harness.f90; it runs with
 one OpenMP thread and zero-filled automatic arrays (see oracle/refdump.py).
 
-Usage: python tests/golden/make_fixtures.py [outdir]
+formats_estuary.npz holds the restart and dump files the reference wrote (as bytes), the
+states they were written from, and the start-up state of a hot start from that restart.
+
+Usage: python tests/golden/make_fixtures.py [outdir [fixture ...]]
 """
 from __future__ import annotations
 
@@ -112,6 +115,53 @@ def make(name: str, spec, outdir: str) -> str:
     return path
 
 
+FORMAT_FIELDS = ["eta_cell", "eta_cell_new", "eta_node", "un_face", "vn_face", "u_node", "v_node",
+                 "wn_cell", "w_node", "salt_cell", "temp_cell", "rho_cell", "salt_node", "temp_node",
+                 "rho_node", "salt_face", "temp_face", "rho_face", "ubar_node", "vbar_node",
+                 "sbar_node", "tbar_node", "rbar_node", "julian_day"]
+
+
+def make_formats(outdir: str) -> str:
+    from gom_release_amd import gomout
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    from golden_io import FORMATS, formats_cases
+    a, b_of = formats_cases()
+    out = {}
+    with tempfile.TemporaryDirectory() as tmp:
+        ra = os.path.join(tmp, "A")
+        gomio.write_project(a, ra)
+        refdump.run_reference(ra, os.path.join(ra, "dump"), 1, a.ndt)
+        for it in range(1, a.ndt + 1):
+            d = refdump.read_dump(os.path.join(ra, "dump", f"s{it}_post.bin"))
+            for k in FORMAT_FIELDS:
+                out[f"A/s{it}_post/{k}"] = d[k]
+        for fn in sorted(os.listdir(os.path.join(ra, "output"))):
+            if fn.startswith(("restart_", "dump2D_", "dump3D_")):
+                out[f"A/files/{fn}"] = np.frombuffer(open(os.path.join(ra, "output", fn), "rb").read(), np.uint8)
+        rfile = os.path.join(ra, "output", f"restart_{a.ndt:07d}.out")
+        b = b_of(gomout.read_restart(rfile, a.maxele, a.maxnod, a.maxlayer))
+        rb = os.path.join(tmp, "B")
+        gomio.write_project(b, rb)
+        # the hot start reads the reference's own restart file
+        with open(rfile, "rb") as fi, open(os.path.join(rb, "input", "restart.inp"), "wb") as fo:
+            fo.write(fi.read())
+        refdump.run_reference(rb, os.path.join(rb, "dump"), 1, b.ndt)
+        for k, v in refdump.read_dump(os.path.join(rb, "dump", "s0_post.bin")).items():
+            out[f"B/s0_post/{k}"] = v
+        for it in range(1, b.ndt + 1):
+            d = refdump.read_dump(os.path.join(rb, "dump", f"s{it}_post.bin"))
+            for k in FORMAT_FIELDS:
+                out[f"B/s{it}_post/{k}"] = d[k]
+        for fn in sorted(os.listdir(os.path.join(rb, "output"))):
+            if fn.startswith(("restart_", "dump2D_", "dump3D_")):
+                out[f"B/files/{fn}"] = np.frombuffer(open(os.path.join(rb, "output", fn), "rb").read(), np.uint8)
+    path = os.path.join(outdir, "formats_estuary.npz")
+    np.savez_compressed(path, **out)
+    with open(os.path.join(outdir, "formats_estuary.json"), "w") as f:
+        json.dump(dict(case="formats_estuary", kind=FORMATS["kind"], args=FORMATS["args"]), f, indent=1)
+    return path
+
+
 def provenance(outdir: str) -> None:
     ver = subprocess.run(["/opt/rocm/bin/amdflang", "--version"], capture_output=True, text=True).stdout
     meta = dict(reference="GOM_Rv1.1.0/source", compiler=ver.splitlines()[0],
@@ -126,6 +176,11 @@ if __name__ == "__main__":
     outdir = sys.argv[1] if len(sys.argv) > 1 else os.path.dirname(os.path.abspath(__file__))
     os.makedirs(outdir, exist_ok=True)
     provenance(outdir)
+    only = sys.argv[2:] if len(sys.argv) > 2 else None
     for name, spec in CASES.items():
-        p = make(name, spec, outdir)
-        print(name, os.path.getsize(p) // 1024, "KiB")
+        if only is None or name in only:
+            p = make(name, spec, outdir)
+            print(name, os.path.getsize(p) // 1024, "KiB")
+    if only is None or "formats_estuary" in only:
+        p = make_formats(outdir)
+        print("formats_estuary", os.path.getsize(p) // 1024, "KiB")

@@ -73,3 +73,43 @@ def apply(state: "abi.Arrays", values: Dict[str, np.ndarray]) -> None:
     for k, v in values.items():
         if k in state.arr:
             state[k] = v
+
+
+# Output formats and hot start (formats_estuary.npz): run A writes a restart at its last
+# step and state dumps (2-D formatted with a file roll-over after two dumps, 3-D
+# unformatted every second step); run B starts hot from A's restart file and writes the
+# other forms, with a restart after every step.
+FORMATS = dict(kind="estuary", args=dict(ncol=8, nrep=2, nlayer=4, ndt=4, river_q=50.0,
+                                          dry_land=True, tide_phase=-90.0))
+
+
+def formats_cases():
+    """(case A, function making case B from A's restart fields)."""
+    from dataclasses import replace
+    from gom_release_amd import gomio, meshgen
+    a = meshgen.estuary_case(**FORMATS["args"])
+    a.restart_out = -1
+    a.dump2d = gomio.DumpSpec(binary=0, time=1, file_freq=2, frequency=1)
+    a.dump3d = gomio.DumpSpec(binary=1, time=4, file_freq=99999999, frequency=2)
+
+    def b_of(restart):
+        return replace(a, ndt=2, restart_in=1, hot_start=restart, restart_out=1, restart_freq=1,
+                       dump2d=gomio.DumpSpec(binary=1, time=3, file_freq=99999999, frequency=1),
+                       dump3d=gomio.DumpSpec(binary=0, time=2, file_freq=99999999, frequency=1))
+    return a, b_of
+
+
+class FormatsGolden:
+    """formats_estuary.npz: reference-written files (bytes) and the states behind them."""
+
+    def __init__(self):
+        z = np.load(os.path.join(GOLDEN, "formats_estuary.npz"))
+        self.data = {k: z[k] for k in z.files}
+
+    def state(self, run: str, tag: str) -> Dict[str, np.ndarray]:
+        pre = f"{run}/{tag}/"
+        return {k[len(pre):]: _flat(v) for k, v in self.data.items() if k.startswith(pre)}
+
+    def files(self, run: str) -> Dict[str, bytes]:
+        pre = f"{run}/files/"
+        return {k[len(pre):]: v.tobytes() for k, v in self.data.items() if k.startswith(pre)}
