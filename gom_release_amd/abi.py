@@ -58,8 +58,12 @@ class Forcing(C.Structure):
     _fields_ = [(n, C.c_double) for n in FORCING_SCALARS] + [(n, PD) for n, _ in FORCING_REAL]
 
 
+class ForcingOut(C.Structure):
+    _fields_ = [(n, PD) for n, _ in FORCING_REAL]
+
+
 STAGES = ["friction", "elm", "momentum", "freesurf", "velocity", "layers", "nodevel",
-          "transport", "eos", "update", "turbulence"]
+          "transport", "eos", "update", "turbulence", "wind"]
 
 
 def extent(code: str, p: Params) -> int:
@@ -74,6 +78,7 @@ def extent(code: str, p: Params) -> int:
         "L1E": (L + 1) * E, "LP": L * P, "L1P": (L + 1) * P,
         "NTOB": nt * max(p.maxtran2, 1) * max(p.num_ob_cell, 1),
         "NTQB": nt * max(p.maxtran2, 1) * max(p.num_Qb_cell, 1),
+        "HP": P if p.hurricane_flag == 1 else 1, "HJ": J if p.hurricane_flag == 1 else 1,
     }
     return tab[code]
 
@@ -158,6 +163,7 @@ class ForcingArrays(Arrays):
         self.scalars = {n: 0.0 for n in FORCING_SCALARS}
         self.scalars["spinup_function_tide"] = 1.0
         self.scalars["spinup_function_baroclinic"] = 1.0
+        self.scalars["spinup_function_wind"] = 1.0
 
     def struct(self, only=None):
         s = super().struct(only)

@@ -14,6 +14,7 @@
 #include "../../include/gomgpu.h"
 
 #define GOM_LMAX 64  // largest supported maxlayer (per-thread column arrays)
+#define GOM_NSCAL 8  // per-step device scalars (DevForcing::scal)
 #define PCG_MAX_SLOT_BLOCKS 256  // most blocks of one system in a slot-exchange PCG launch
 
 struct DevMesh {
@@ -52,7 +53,8 @@ struct DevForcing {
   GOMGPU_FORCING_REAL(DF_R)
 #undef DF_R
   // per-step scalars in device memory (so a captured step stays valid): [0] baroclinic
-  // spin-up factor, [1] tidal spin-up factor, [2] elapsed time
+  // spin-up factor, [1] tidal spin-up factor, [2] elapsed time, [4] wind spin-up factor,
+  // [5] hurricane phase (0 keep, 1 interpolate, 2 next snapshot)
   const double *scal;
 };
 

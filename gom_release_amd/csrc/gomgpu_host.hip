@@ -20,6 +20,8 @@ __global__ void k_cell_uv(DevMesh, DevState, DevWork, DevDims);
 __global__ void k_mom_rhs(DevMesh, DevState, DevForcing, DevWork, DevDims, MomParams);
 __global__ void k_mom_solve(DevMesh, DevState, DevForcing, DevWork, DevDims, MomParams, int *);
 __global__ void k_river_velocity(DevMesh, DevState, DevForcing, DevDims, double *);
+__global__ void k_hurricane_nodes(DevForcing, int, double);
+__global__ void k_wind_faces(DevMesh, DevForcing, int, double, int);
 __global__ void k_fs_face(DevMesh, DevState, DevWork, DevDims, double);
 __global__ void k_fs_cell(DevMesh, DevState, DevForcing, DevWork, DevDims, double, double, int *);
 __global__ void k_eta_node(DevMesh, DevState, DevDims);
@@ -72,9 +74,11 @@ enum { KT_PCG = 0, KT_ELM = 1, KT_MOMENTUM = 2, KT_TVD_COLUMN = 3 };
 static const char *kKernelNames[GOMGPU_NUM_KTIMERS] = {"k_pcg", "k_elm", "k_momentum", "k_tvd_column"};
 static const char *kStageNames[GOMGPU_NUM_STAGES] = {"friction", "elm",       "momentum", "freesurf",
                                                       "velocity", "layers",    "nodevel",  "transport",
-                                                      "eos",      "update",    "turbulence"};
-// Order of the stages in one step (main.f90:95-272, the closure before the transport).
+                                                      "eos",      "update",    "turbulence", "wind"};
+// Order of the stages in one step (main.f90:95-272, the closure before the transport; the
+// wind stage of main.f90:113-137 first, no-op without hurricane fields).
 static const int kStepOrder[GOMGPU_NUM_STAGES] = {
+    GOMGPU_STAGE_WIND,
     GOMGPU_STAGE_FRICTION, GOMGPU_STAGE_ELM,        GOMGPU_STAGE_MOMENTUM,  GOMGPU_STAGE_FREESURF,
     GOMGPU_STAGE_VELOCITY, GOMGPU_STAGE_LAYERS,     GOMGPU_STAGE_NODEVEL,   GOMGPU_STAGE_TURBULENCE,
     GOMGPU_STAGE_TRANSPORT, GOMGPU_STAGE_EOS,       GOMGPU_STAGE_UPDATE};
@@ -169,6 +173,8 @@ static Shape shape_of(const char *code, const gomgpu_params &p) {
   if (c == "L1P") return {L + 1, P};
   if (c == "NTOB") return {1, Nt * NT * OB};
   if (c == "NTQB") return {1, Nt * NT * QB};
+  if (c == "HP") return {1, p.hurricane_flag == 1 ? P : 1};
+  if (c == "HJ") return {1, p.hurricane_flag == 1 ? J : 1};
   return {0, 0};
 }
 
@@ -239,6 +245,9 @@ static int validate(const gomgpu_params *p, std::string &why) {
     if (!(p->q2min > 0.0) || !(p->xlmin2 > 0.0) || !(p->eps_min > 0.0) || p->rnub == 0.0)
       return why = "closure needs q2min > 0, xlmin2 > 0, eps_min > 0 and rnub != 0", 1;
   }
+  if (p->hurricane_flag != 0 && p->hurricane_flag != 1) return why = "hurricane_flag must be 0 or 1", 1;
+  if (p->hurricane_flag == 1 && p->hurricane_read_interval < 1)
+    return why = "hurricane_read_interval (hurricane_dt / dt) must be at least 1", 1;
   if (p->precision == GOMGPU_MIXED) return why = "GOMGPU_MIXED is not implemented in this build", 1;
   if (p->precision != GOMGPU_FP64) return why = "bad precision", 1;
   return 0;
@@ -416,8 +425,8 @@ extern "C" int gomgpu_create(const gomgpu_mesh *mesh, const gomgpu_params *param
 #undef AL_FR
   {
     double *dp;
-    const double init[4] = {1.0, 1.0, 0.0, 0.0};
-    if ((rc = dalloc(c, &dp, 4))) goto fail;
+    const double init[GOM_NSCAL] = {1.0, 1.0, 0.0, 0.0, 1.0, 0.0, 0.0, 0.0};
+    if ((rc = dalloc(c, &dp, GOM_NSCAL))) goto fail;
     if (hipMemcpyAsync(dp, init, sizeof init, hipMemcpyHostToDevice, c->st) != hipSuccess ||
         hipStreamSynchronize(c->st) != hipSuccess) {
       rc = seterr(c, GOMGPU_E_HIP, "hipMemcpy failed");
@@ -489,7 +498,7 @@ extern "C" int gomgpu_create(const gomgpu_mesh *mesh, const gomgpu_params *param
   }
     GOMGPU_FORCING_REAL(SZ_F)
 #undef SZ_F
-    tot += 4;  // the per-step scalars
+    tot += GOM_NSCAL;  // the per-step scalars
     c->stage_len = tot;
     if (hipHostMalloc((void **)&c->h_stage, tot * sizeof(double)) != hipSuccess ||
         hipEventCreateWithFlags(&c->fev, hipEventDisableTiming) != hipSuccess ||
@@ -592,6 +601,20 @@ extern "C" int gomgpu_get_state(gomgpu_ctx *c, gomgpu_state *st) {
   return 0;
 }
 
+extern "C" int gomgpu_get_forcing(gomgpu_ctx *c, gomgpu_forcing_out *fo) {
+  if (!c || !fo) return GOMGPU_E_ARG;
+  const gomgpu_params &p = c->p;
+  int rc;
+  HIPCHK(c, hipStreamSynchronize(c->st));
+#define DN_F(name, ext)                                                                       \
+  if (fo->name) {                                                                             \
+    if ((rc = download(c, fo->name, (double *)c->f.name, shape_of(#ext, p)))) return rc;      \
+  }
+  GOMGPU_FORCING_REAL(DN_F)
+#undef DN_F
+  return 0;
+}
+
 // Forcing uploads are asynchronous: the arrays go through one pinned staging buffer; the
 // host waits only for the previous forcing copy before reusing it, so it can run one step
 // ahead of the device.
@@ -615,12 +638,19 @@ extern "C" int gomgpu_set_forcing(gomgpu_ctx *c, const gomgpu_forcing *f) {
   }
   GOMGPU_FORCING_REAL(UP_F)
 #undef UP_F
-  if (off + 4 > c->stage_len) return seterr(c, GOMGPU_E_ARG, "forcing staging overflow");
+  if (off + GOM_NSCAL > c->stage_len) return seterr(c, GOMGPU_E_ARG, "forcing staging overflow");
+  if (p.hurricane_flag == 1 && !(f->hurricane_phase == 0.0 || f->hurricane_phase == 1.0 || f->hurricane_phase == 2.0))
+    return seterr(c, GOMGPU_E_ARG, "hurricane_phase must be 0, 1 or 2");
   c->h_stage[off] = f->spinup_function_baroclinic;
   c->h_stage[off + 1] = f->spinup_function_tide;
   c->h_stage[off + 2] = f->elapsed_time;
   c->h_stage[off + 3] = 0.0;
-  HIPCHK(c, hipMemcpyAsync((double *)c->f.scal, c->h_stage + off, 4 * sizeof(double), hipMemcpyHostToDevice, c->st));
+  c->h_stage[off + 4] = f->spinup_function_wind;
+  c->h_stage[off + 5] = f->hurricane_phase;
+  c->h_stage[off + 6] = 0.0;
+  c->h_stage[off + 7] = 0.0;
+  HIPCHK(c, hipMemcpyAsync((double *)c->f.scal, c->h_stage + off, GOM_NSCAL * sizeof(double), hipMemcpyHostToDevice,
+                           c->st));
   HIPCHK(c, hipEventRecord(c->fev, c->st));
   return 0;
 }
@@ -914,6 +944,14 @@ static int stage_impl(gomgpu_ctx *c, int id) {
       k_bars<<<nblk(d.P, BS), BS, 0, st>>>(c->m, c->s, d);
       break;
     }
+    case GOMGPU_STAGE_WIND: {
+      // read_hurricane_ser_1.f90 (node fields and face winds) and the hurricane branch of
+      // calculate_wind_stress.f90:88-114; the step's phase and spin-up are device scalars
+      if (p.hurricane_flag != 1) break;
+      k_hurricane_nodes<<<nblk(d.P, BS), BS, 0, st>>>(c->f, d.P, (double)p.hurricane_read_interval);
+      k_wind_faces<<<nblk(d.J, BS), BS, 0, st>>>(c->m, c->f, d.J, p.rho_a / p.rho_o, p.wind_formular);
+      break;
+    }
     case GOMGPU_STAGE_TURBULENCE: {
       if (p.turbulence_flag == 0) break;
       TurbParams tp;
@@ -1172,7 +1210,7 @@ static int batch_plan(gomgpu_batch *b) {
   return 0;
 }
 
-static const int kPreStages[] = {GOMGPU_STAGE_FRICTION, GOMGPU_STAGE_ELM, GOMGPU_STAGE_MOMENTUM};
+static const int kPreStages[] = {GOMGPU_STAGE_WIND, GOMGPU_STAGE_FRICTION, GOMGPU_STAGE_ELM, GOMGPU_STAGE_MOMENTUM};
 static const int kPostStages[] = {GOMGPU_STAGE_VELOCITY,   GOMGPU_STAGE_LAYERS,    GOMGPU_STAGE_NODEVEL,
                                   GOMGPU_STAGE_TURBULENCE, GOMGPU_STAGE_TRANSPORT, GOMGPU_STAGE_EOS,
                                   GOMGPU_STAGE_UPDATE};
@@ -1392,7 +1430,7 @@ extern "C" long long gomgpu_footprint(const gomgpu_params *params) {
   GOMGPU_FORCING_REAL(FP_R)
 #undef FP_R
 #undef FP_I
-  add(4, sizeof(double));  // per-step scalars
+  add(GOM_NSCAL, sizeof(double));  // per-step scalars
   long E = p.maxele, J = p.maxface, P = p.maxnod, L = p.maxlayer;
   long NT = (p.transport_flag == 1 ? p.maxtran2 : 0);
   NT = NT > 0 ? NT : 1;

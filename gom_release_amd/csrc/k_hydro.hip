@@ -1313,3 +1313,78 @@ __global__ void k_river_velocity(DevMesh m, DevState s, DevForcing f, DevDims d,
     }
   }
 }
+
+// ---- hurricane fields and wind stress (read_hurricane_ser_1.f90, calculate_wind_stress.f90)
+// Node fields of one step. Phase 2 (every hurricane_read_interval steps) moves the later
+// snapshot into the earlier pair and the interpolated field, then takes the next snapshot
+// (read_hurricane_ser_1.f90:18-46); phase 1 adds one interval's increment to the
+// interpolated field (read_hurricane_ser_1.f90:48-60), so the field is the running sum the
+// reference accumulates; phase 0 (outside the hurricane window) leaves everything.
+__global__ void k_hurricane_nodes(DevForcing f, int P, double ninterval) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= P) return;
+  const double phase = f.scal[5];
+  double *u0 = const_cast<double *>(f.wind_u0), *v0 = const_cast<double *>(f.wind_v0);
+  double *p0 = const_cast<double *>(f.air_p0);
+  double *u1 = const_cast<double *>(f.wind_u1), *v1 = const_cast<double *>(f.wind_v1);
+  double *p1 = const_cast<double *>(f.air_p1);
+  double *u2 = const_cast<double *>(f.wind_u2), *v2 = const_cast<double *>(f.wind_v2);
+  double *p2 = const_cast<double *>(f.air_p2);
+  if (phase == 2.0) {
+    const double a = u2[i], b = v2[i], c = p2[i];
+    u0[i] = a;
+    v0[i] = b;
+    p0[i] = c;
+    u1[i] = a;
+    v1[i] = b;
+    p1[i] = c;
+    u2[i] = f.wind_u_next[i];
+    v2[i] = f.wind_v_next[i];
+    p2[i] = f.air_p_next[i];
+  } else if (phase == 1.0) {
+    u1[i] = u1[i] + (u2[i] - u0[i]) / ninterval;
+    v1[i] = v1[i] + (v2[i] - v0[i]) / ninterval;
+    p1[i] = p1[i] + (p2[i] - p0[i]) / ninterval;
+  }
+}
+
+// Face winds (the mean of the face's two node winds, inside the hurricane window) and the
+// wind stress of every face (calculate_wind_stress.f90:88-114): drag coefficient of the
+// chosen formula capped at 0.003 (a default-real literal in the reference), stress along
+// the face normal and tangent, times the wind spin-up factor.
+__global__ void k_wind_faces(DevMesh m, DevForcing f, int J, double rho_ratio, int formula) {
+  int j = blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= J) return;
+  const double phase = f.scal[5], spin = f.scal[4];
+  double *uf_ = const_cast<double *>(f.wind_u_at_face), *vf_ = const_cast<double *>(f.wind_v_at_face);
+  double uf, vf;
+  if (phase != 0.0) {
+    int n1 = TB(m.nodenum_at_face, 1, j, J) - 1, n2 = TB(m.nodenum_at_face, 2, j, J) - 1;
+    uf = (f.wind_u1[n1] + f.wind_u1[n2]) * 0.5;
+    vf = (f.wind_v1[n1] + f.wind_v1[n2]) * 0.5;
+    uf_[j] = uf;
+    vf_[j] = vf;
+  } else {
+    uf = uf_[j];
+    vf = vf_[j];
+  }
+  const double c = m.cos_theta[j], s = m.sin_theta[j];
+  const double ws = sqrt(uf * uf + vf * vf);
+  const double wn = uf * c + vf * s;
+  const double wt = -(uf * s) + vf * c;
+  double a = 0.61, b = 0.063;
+  if (formula == 1) {
+    a = 0.75;
+    b = 0.067;
+  } else if (formula == 3) {
+    a = 0.80;
+    b = 0.065;
+  }
+  double cda = 0.001 * (a + b * ws);
+  const double cap = (double)0.003f;
+  cda = cap < cda ? cap : cda;
+  const double g = cda * rho_ratio * ws;
+  const_cast<double *>(f.Gamma_T)[j] = g;
+  const_cast<double *>(f.wind_stress_normal)[j] = g * wn * spin;
+  const_cast<double *>(f.wind_stress_tangnt)[j] = g * wt * spin;
+}

@@ -32,6 +32,23 @@ class HostForcing:
             self.forcing["wind_stress_tangnt"] = wst
         self.julian_day = fz.julian_day0(case)
         self.tran_ids = [params.tran_id1, params.tran_id2]
+        self.uploads = list(DYNAMIC)      # forcing arrays the coming step needs uploaded
+        if case.hurricane_flag == 1:
+            # read_hurricane_ser_0.f90: the first snapshot pair; the interpolated field,
+            # air_p1 and the face winds start at zero (allocate_variables.f90:507-519)
+            from .gomio import hurricane_snapshots
+            self._snaps = hurricane_snapshots(case)
+            f = self.forcing
+            for tag in ("0", "2"):
+                u, v, pr = self._next_snapshot(0)
+                f["wind_u" + tag], f["wind_v" + tag], f["air_p" + tag] = u, v, pr
+            self.window = fz.hurricane_window(case)
+
+    def _next_snapshot(self, it: int):
+        try:
+            return next(self._snaps)
+        except StopIteration:
+            raise RuntimeError(f"the hurricane fields end before step {it}") from None
 
     def advance(self, it: int, eta_node: Optional[np.ndarray] = None) -> "abi.ForcingArrays":
         """Fill the forcing of step `it`. With eta_node the river face velocity is computed
@@ -46,6 +63,13 @@ class HostForcing:
         f.scalars["elapsed_time"] = elapsed
         f.scalars["spinup_function_tide"] = sp_tide
         f.scalars["spinup_function_baroclinic"] = sp_bc
+        self.uploads = list(DYNAMIC)
+        if c.hurricane_flag == 1:
+            ph = fz.hurricane_phase(it, self.julian_day, self.window, self.params.hurricane_read_interval)
+            f.scalars["hurricane_phase"] = float(ph)
+            if ph == 2:
+                f["wind_u_next"], f["wind_v_next"], f["air_p_next"] = self._next_snapshot(it)
+                self.uploads += ["wind_u_next", "wind_v_next", "air_p_next"]
         old, new = fz.boundary_eta(c, it, sp_tide, self.julian_day)
         f["eta_at_ob_old"] = old
         f["eta_at_ob_new"] = new
@@ -137,12 +161,13 @@ class Scenario:
     def stage_forcing(self) -> None:
         """Host forcing of the next step, uploaded asynchronously (before a batch step)."""
         self.it += 1
+        hf = self.host_forcing
         if self.host_river:
-            self.host_forcing.advance(self.it, eta_node=self.fields(["eta_node"])["eta_node"])
-            self.model.set_forcing(self.forcing, only=DYNAMIC + ["Qu_boundary"])
+            hf.advance(self.it, eta_node=self.fields(["eta_node"])["eta_node"])
+            self.model.set_forcing(self.forcing, only=hf.uploads + ["Qu_boundary"])
         else:
-            self.host_forcing.advance(self.it)
-            self.model.set_forcing(self.forcing, only=DYNAMIC)
+            hf.advance(self.it)
+            self.model.set_forcing(self.forcing, only=hf.uploads)
 
     def step(self) -> None:
         self.stage_forcing()

@@ -42,14 +42,47 @@ def spinup(flag: int, period_days: float, elapsed: float) -> float:
     return 1.0
 
 
-def julian_day0(case: Case) -> float:
-    """Day-of-year (0-based, fractional) of the start time, julian.f90."""
-    y, mo, d, h, mi, s = case.start
+def julian(y: int, mo: int, d: int, h: int = 0, mi: int = 0, s: int = 0) -> float:
+    """Day of the year, 0-based and fractional (julian.f90:15-57)."""
     cum = [0, 31, 59, 90, 120, 151, 181, 212, 243, 273, 304, 334][mo - 1]
     jd = float(cum)
     if y % 4 == 0 and y != 1900 and mo >= 3:
         jd = jd + 1
     return jd + (d - 1) + h / 24.0 + mi / 1440.0 + s / 86400.0
+
+
+def julian_day0(case: Case) -> float:
+    """Day-of-year (0-based, fractional) of the start time, julian.f90."""
+    return julian(*case.start)
+
+
+def _year_days(y1: int, y2: int) -> int:
+    """Days from the start of year y1 to that of y2 as find_hurricane_start_end_jday.f90:
+    17-30 counts them (364 for a year divisible by 4)."""
+    d = 0
+    while y1 != y2:
+        d += 365 if y1 % 4 != 0 else 364
+        y1 += 1
+    return d
+
+
+def hurricane_window(case: Case):
+    """(hurricane_start_jday, hurricane_end_jday) of find_hurricane_start_end_jday.f90,
+    on the model's julian-day axis."""
+    y0 = case.start[0]
+    a = case.hurricane_start
+    b = case.hurricane_end
+    return (_year_days(y0, a[0]) + julian(a[0], a[1], a[2]),
+            _year_days(y0, b[0]) + julian(b[0], b[1], b[2]))
+
+
+def hurricane_phase(it: int, julian_day: float, window, read_interval: int) -> int:
+    """What main.f90:116-123 and read_hurricane_ser_1.f90:16-17 do at step `it`: 0 outside
+    the hurricane window, 2 when the step reads the next snapshot (mod(it, interval) = 0),
+    else 1 (interpolate)."""
+    if not (window[0] <= julian_day <= window[1]):
+        return 0
+    return 2 if it % read_interval == 0 else 1
 
 
 def boundary_eta(case: Case, it: int, sp_tide: float, julian_day: Optional[float] = None):

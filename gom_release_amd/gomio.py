@@ -165,6 +165,18 @@ class Case:
     restart_out: int = 0
     restart_freq: int = 99999999
     hot_start: Optional[Dict[str, np.ndarray]] = None    # restart records (gomout.read_restart)
+    # C30: drag formula (1 Large-Pond, 2 Wu, 3 ...) and wind spin-up
+    wind_formular: int = 2
+    wind_spinup: int = 0
+    wind_spinup_period: float = 0.0
+    # C31 / C31_1: hurricane wind and pressure fields (formatted, linear in time), read every
+    # hurricane_dt seconds between the start and end dates; `hurricane` is a list of node
+    # snapshots (u [m/s], v [m/s], p [hPa]) or the path of a hurricane_ser.inp
+    hurricane_flag: int = 0
+    hurricane_dt: int = 21600
+    hurricane_start: Tuple[int, int, int] = (1900, 1, 1)
+    hurricane_end: Tuple[int, int, int] = (2999, 12, 31)
+    hurricane: Optional[object] = None
     # C44 / C45
     dump2d: Optional[DumpSpec] = None
     dump3d: Optional[DumpSpec] = None
@@ -271,6 +283,10 @@ def write_project(case: Case, root: str) -> str:
                     f.write(f"  {_r(per)} {_r(amp)} {_r(ph)} {_r(fac)} {_r(v0)} {nm}\n")
     if case.eta_series and any(b.eta_type == 2 for b in case.open_boundaries):
         _write_series_file(os.path.join(inp, "eta_ser1.inp"), case.eta_series, False)
+    if case.hurricane_flag == 1:
+        if case.hurricane is None:
+            raise ValueError("hurricane_flag = 1 needs the hurricane snapshots (Case.hurricane)")
+        write_hurricane_ser(os.path.join(inp, "hurricane_ser.inp"), hurricane_snapshots(case, pascal=False))
     if case.restart_in == 1:
         if case.hot_start is None:
             raise ValueError("restart_in = 1 needs the restart records (Case.hot_start)")
@@ -362,9 +378,13 @@ def _main_inp(c: Case) -> str:
     card("C25\tno_Etide_species Etide_cutoff_depth")
     data(0, 40.0)
     card("C30\twind_flag airp_flag num_windp_ser wind_formular wind_spinup wind_spinup_period")
-    data(0, 0, 0, 2, 0, 0.0)
+    data(0, 0, 0, int(c.wind_formular), int(c.wind_spinup), float(c.wind_spinup_period))
     card("C31\thurricane_flag hurricane_data_type hurricane_interp_method hurricane_dt")
-    data(0, 1, 1, 21600)
+    data(int(c.hurricane_flag), 1, 1, int(c.hurricane_dt))
+    if c.hurricane_flag == 1:
+        card("C31_1\thurricane start year month day / end year month day")
+        data(*[int(v) for v in c.hurricane_start])
+        data(*[int(v) for v in c.hurricane_end])
     card("C32\tholland_flag holland_start_jday holland_end_jday")
     data(0, 0.0, 0.0)
     card("C40\tcheck_grid")
@@ -889,9 +909,10 @@ def write_hurricane_ser(path: str, snapshots) -> None:
                 f.write(f"  {i + 1} {_r(u[i])} {_r(v[i])} {_r(p[i])}\n")
 
 
-def read_hurricane_ser(path: str, maxnod: int) -> Iterator[Tuple[np.ndarray, np.ndarray, np.ndarray]]:
-    """Snapshots (u, v, p[Pa]) of a formatted hurricane_ser.inp in file order; the
-    pressure is converted from hPa with ``* 100`` as read_hurricane_ser_0.f90:34-35 does."""
+def read_hurricane_ser(path: str, maxnod: int, pascal: bool = True) -> Iterator[Tuple[np.ndarray, np.ndarray, np.ndarray]]:
+    """Snapshots (u, v, p) of a formatted hurricane_ser.inp in file order; with `pascal`
+    the pressure is converted from hPa with ``* 100`` as read_hurricane_ser_0.f90:34-35
+    does."""
     rd = ListReader.from_file(path)
     rd.skip_header()
     while True:
@@ -905,8 +926,21 @@ def read_hurricane_ser(path: str, maxnod: int) -> Iterator[Tuple[np.ndarray, np.
         p = np.empty(maxnod)
         for i in range(maxnod):
             _id, u[i], v[i], p[i] = rd.read("irrr")
-            p[i] = p[i] * 100.0
-        yield u, v, p
+        yield u, v, (p * 100.0 if pascal else p)
+
+
+def hurricane_snapshots(case: "Case", pascal: bool = True):
+    """The node snapshots of a case's hurricane fields in time order, from its list or its
+    hurricane_ser.inp; pressures in Pa (``p * 100``, read_hurricane_ser_0.f90:35) when
+    `pascal`, else in hPa as stored."""
+    src = case.hurricane
+    if isinstance(src, str):
+        yield from read_hurricane_ser(src, case.maxnod, pascal)
+        return
+    for (u, v, p) in src:
+        p = np.asarray(p, dtype=np.float64)
+        yield (np.asarray(u, dtype=np.float64), np.asarray(v, dtype=np.float64),
+               p * 100.0 if pascal else p)
 
 
 # --------------------------------------------------------------------------- project
@@ -988,7 +1022,12 @@ def unsupported(pr: Project) -> List[str]:
         (m.num_SS_cell > 0, "C24 source/sink cells"),
         (m.no_Etide_species > 0, "C25 equilibrium tide"),
         (m.wind_flag != 0 or m.airp_flag != 0, "C30 wind/air-pressure series"),
-        (m.hurricane_flag != 0, "C31 hurricane wind and pressure fields"),
+        (m.hurricane_flag != 0 and m.hurricane_data_type != 1,
+         "C31 unformatted hurricane fields (hurricane_data_type 2)"),
+        (m.hurricane_flag != 0 and m.hurricane_interp_method != 1,
+         "C31 hurricane interpolation in space (hurricane_interp_method 2)"),
+        (m.hurricane_flag != 0 and "hurricane_ser.inp" not in pr.present,
+         "C31 hurricane fields: input/hurricane_ser.inp is missing"),
         (m.holland_flag != 0, "C32 Holland storm surge"),
     ]
     for bad, what in checks:
@@ -1043,6 +1082,11 @@ def case_from_project(pr: Project, strict: bool = True) -> Tuple[Case, List[str]
     c.baroclinic_spinup, c.baroclinic_spinup_period = m.baroclinic_spinup, m.baroclinic_spinup_period
     c.rivers = [RiverBoundary(r[1], r[2], r[3], r[4], r[5], r[6]) for r in m.q_rows]
     c.q_series = list(pr.q_series)
+    c.wind_formular, c.wind_spinup, c.wind_spinup_period = m.wind_formular, m.wind_spinup, m.wind_spinup_period
+    if m.hurricane_flag == 1 and not any(b.startswith("C31") for b in bad):
+        c.hurricane_flag, c.hurricane_dt = 1, m.hurricane_dt
+        c.hurricane_start, c.hurricane_end = m.hurricane_start, m.hurricane_end
+        c.hurricane = os.path.join(pr.root, "input", "hurricane_ser.inp")
     for tag, attr in (("2D", "dump2d"), ("3D", "dump3d")):
         if m[f"IS{tag}_dump_switch"] == 1:
             setattr(c, attr, DumpSpec(m[f"IS{tag}_dump_binary"], m[f"IS{tag}_dump_time"],

@@ -46,6 +46,7 @@ def lib():
         L.gomgpu_set_state.argtypes = [vp, C.POINTER(abi.State)]
         L.gomgpu_get_state.argtypes = [vp, C.POINTER(abi.State)]
         L.gomgpu_set_forcing.argtypes = [vp, C.POINTER(abi.Forcing)]
+        L.gomgpu_get_forcing.argtypes = [vp, C.POINTER(abi.ForcingOut)]
         L.gomgpu_stage.argtypes = [vp, C.c_int]
         L.gomgpu_step.argtypes = [vp]
         L.gomgpu_run.argtypes = [vp, C.c_int]
@@ -78,7 +79,7 @@ def lib():
 
 
 EXPORTED = ["gomgpu_create", "gomgpu_set_state", "gomgpu_set_forcing", "gomgpu_stage",
-            "gomgpu_step", "gomgpu_run", "gomgpu_get_state", "gomgpu_budgets",
+            "gomgpu_step", "gomgpu_run", "gomgpu_get_state", "gomgpu_get_forcing", "gomgpu_budgets",
             "gomgpu_pcg_iterations", "gomgpu_sync", "gomgpu_last_error", "gomgpu_timings",
             "gomgpu_destroy", "gomgpu_set_launch", "gomgpu_get_launch", "gomgpu_set_pcg_rows",
             "gomgpu_footprint",
@@ -127,6 +128,14 @@ class GpuModel:
     def set_forcing(self, forcing: "abi.ForcingArrays", only: Optional[Sequence[str]] = None) -> None:
         f = forcing.struct(only)
         self._check(self._L.gomgpu_set_forcing(self._ctx, C.byref(f)))
+
+    def get_forcing(self, forcing: "abi.ForcingArrays", only: Optional[Sequence[str]] = None) -> None:
+        """Download the device's forcing arrays (the wind stage's outputs among them)."""
+        out = abi.ForcingOut()
+        for name, a in forcing.arr.items():
+            if only is None or name in only:
+                setattr(out, name, a.ctypes.data_as(abi.PD))
+        self._check(self._L.gomgpu_get_forcing(self._ctx, C.byref(out)))
 
     # -------------------------------------------------------------- execution
     def stage(self, name: str) -> None:

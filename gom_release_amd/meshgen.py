@@ -185,3 +185,38 @@ def estuary_case(ncol: int = 40, nrep: int = 4, nlayer: int = 5, dx: float = 100
     for k, v in kw.items():
         setattr(case, k, v)
     return case
+
+
+def holland_snapshots(x: np.ndarray, y: np.ndarray, times: Sequence[float], centre0: Tuple[float, float],
+                      velocity: Tuple[float, float], p_min: float = 950.0, p_env: float = 1013.0,
+                      r_max: float = 40.0e3, b: float = 1.5, lat: float = 30.0, rho_air: float = 1.15,
+                      surface_factor: float = 0.8):
+    """Node snapshots (u [m/s], v [m/s], p [hPa]) of a translating Holland (1980) cyclone,
+    one per time in `times` [s], for hurricane_ser.inp.
+
+    Holland, G. J. (1980), Mon. Wea. Rev. 108, 1212-1218: with dp = p_env - p_min and
+    s = (r_max / r)^b, the surface pressure is p = p_min + dp exp(-s) and the gradient wind
+    V = sqrt(b dp s exp(-s) / rho_air + (r f / 2)^2) - r f / 2. The surface wind is
+    `surface_factor` V, counter-clockwise about the centre (northern hemisphere), plus the
+    translation velocity of the centre. The centre starts at `centre0` [m] and moves at
+    `velocity` [m/s]."""
+    x = np.asarray(x, dtype=np.float64)
+    y = np.asarray(y, dtype=np.float64)
+    f = 2.0 * 7.292e-5 * np.sin(np.radians(lat))
+    dp = (p_env - p_min) * 100.0                      # Pa
+    out = []
+    for t in times:
+        cx = centre0[0] + velocity[0] * t
+        cy = centre0[1] + velocity[1] * t
+        dx, dy = x - cx, y - cy
+        r = np.hypot(dx, dy)
+        rs = np.maximum(r, 1.0)
+        s = (r_max / rs) ** b
+        es = np.exp(-s)
+        p = p_min + (p_env - p_min) * es              # hPa
+        vg = np.sqrt(b * dp * s * es / rho_air + (rs * f / 2.0) ** 2) - rs * f / 2.0
+        vs = np.where(r > 1.0, surface_factor * vg, 0.0)
+        u = -vs * dy / rs + velocity[0]
+        v = vs * dx / rs + velocity[1]
+        out.append((u, v, p))
+    return out

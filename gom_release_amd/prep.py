@@ -56,7 +56,11 @@ def params_for(case: Case, precision: int = 0) -> "abi.Params":
         # read_main_inp.f90:262, with (0.5 - 1.0e-4) folded in single precision
         denominator_min_for_matrix=min(case.dry_depth * float(np.float32(0.5) - np.float32(1.0e-4)),
                                        float(dz.min()) * float(np.float32(0.5) - np.float32(1.0e-4))),
-        delta_z_min=float(dz.min()))
+        delta_z_min=float(dz.min()),
+        # C30/C31 (read_hurricane_ser_0.f90:29: hurricane_read_interval = int(hurricane_dt/dt))
+        hurricane_flag=int(case.hurricane_flag), wind_formular=int(case.wind_formular),
+        hurricane_read_interval=int(case.hurricane_dt / case.dt) if case.hurricane_flag == 1 else 0,
+        rho_a=float(case.rho_a))
     d.update(turbulence_params(case))
     return abi.params_from_dict(d)
 

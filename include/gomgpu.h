@@ -5,6 +5,9 @@
  * through the module mod_global_variables (GOM_Rv1.1.0/source/main.f90:179-272). This ABI
  * replaces them one for one, with the module arrays passed explicitly:
  *
+ *   gomgpu_stage(ctx, GOMGPU_STAGE_WIND)      read_hurricane_ser_1 +      main.f90:113-137
+ *                                             calculate_wind_stress (hurricane_flag = 1 only;
+ *                                             first in gomgpu_step)
  *   gomgpu_stage(ctx, GOMGPU_STAGE_FRICTION)  calculate_bottom_friction   main.f90:95
  *   gomgpu_stage(ctx, GOMGPU_STAGE_ELM)       solve_nonlinear_advection   main.f90:167-180
  *   gomgpu_stage(ctx, GOMGPU_STAGE_MOMENTUM)  solve_momentum_equation     main.f90:193
@@ -46,7 +49,8 @@ extern "C" {
 /* ---------------------------------------------------------------------------------------
  * Field lists. Each entry is X(name, extent) where extent names the array shape:
  *   P nodes, E cells, J faces, L maxlayer, L1 maxlayer+1, NB max_no_neighbor_node,
- *   NB1 max_no_neighbor_node+1, OB num_ob_cell, QB num_Qb_cell.
+ *   NB1 max_no_neighbor_node+1, OB num_ob_cell, QB num_Qb_cell; HP and HJ are P and J
+ *   when hurricane_flag = 1 and 1 otherwise.
  * ------------------------------------------------------------------------------------- */
 
 /* Static geometry, from set_geometry_1.f90 / set_geometry_2.f90 / read_*_inp.f90. */
@@ -95,11 +99,21 @@ extern "C" {
  * (solve_transport_equation_v19.f90:229-534): con_at_ob is (Nt, maxtran2, OB) and
  * con_at_Qb (Nt, maxtran2, QB), fastest index first. With params.river_on_device = 1 the
  * river face velocity Qu_boundary is computed on the device from the discharge Q_add and
- * the current face depth (compute_boundary_velocity.f90:32-46) instead of being passed. */
+ * the current face depth (compute_boundary_velocity.f90:32-46) instead of being passed.
+ * With params.hurricane_flag = 1 the wind stage keeps the hurricane fields on the device
+ * (read_hurricane_ser_1.f90): snapshot pair wind_u0/v0/air_p0 and wind_u2/v2/air_p2, the
+ * interpolated wind_u1/v1/air_p1 and the face winds; the host passes them once at the
+ * start (u0 = first snapshot, u2 = second, the rest zero) and the next snapshot in
+ * wind_u/v/air_p_next before a step whose hurricane_phase is 2. Gamma_T and the wind
+ * stress are then outputs of that stage (calculate_wind_stress.f90:88-114). */
 #define GOMGPU_FORCING_REAL(X) \
   X(Gamma_T, J) X(wind_stress_normal, J) X(wind_stress_tangnt, J) X(air_p1, P) \
   X(eta_at_ob_old, OB) X(eta_at_ob_new, OB) X(Qu_boundary, QB) X(phi_sz, LE) \
-  X(con_at_ob, NTOB) X(con_at_Qb, NTQB) X(Q_add, QB)
+  X(con_at_ob, NTOB) X(con_at_Qb, NTQB) X(Q_add, QB) \
+  X(wind_u0, HP) X(wind_v0, HP) X(air_p0, HP) X(wind_u1, HP) X(wind_v1, HP) \
+  X(wind_u2, HP) X(wind_v2, HP) X(air_p2, HP) \
+  X(wind_u_next, HP) X(wind_v_next, HP) X(air_p_next, HP) \
+  X(wind_u_at_face, HJ) X(wind_v_at_face, HJ)
 
 /* Scalars of main.inp and set-up (read_main_inp.f90). */
 #define GOMGPU_PARAMS_INT(X) \
@@ -109,7 +123,8 @@ extern "C" {
   X(bf_flag) X(baroclinic_flag) X(ana_density) X(i_density_flag) \
   X(transport_flag) X(trans_sub_iter) X(h_flux_limiter) X(v_flux_limiter) \
   X(heat_option) X(max_iteration_pcg) X(precision) X(river_on_device) \
-  X(turbulence_flag) X(turbulence_model) X(stability_function)
+  X(turbulence_flag) X(turbulence_model) X(stability_function) \
+  X(hurricane_flag) X(hurricane_read_interval) X(wind_formular)
 
 #define GOMGPU_PARAMS_REAL(X) \
   X(dt) X(theta) X(gravity) X(rho_o) X(MSL) X(dry_depth) X(adv_onoff_depth) \
@@ -117,10 +132,15 @@ extern "C" {
   X(denominator_min_for_matrix) X(delta_z_min) \
   X(q2min) X(psimin) X(eps_min) X(bgdiff) X(diffmax) X(xlmin1) X(xlmin2) \
   X(cmiu0) X(cpsi1) X(cpsi2) X(rpub) X(rmub) X(rnub) X(schk) X(schpsi) \
-  X(h1_pp) X(h2_pp) X(vdmax_pp1) X(vdmax_pp2) X(vdmin_pp1) X(vdmin_pp2) X(tdmin_pp)
+  X(h1_pp) X(h2_pp) X(vdmax_pp1) X(vdmax_pp2) X(vdmin_pp1) X(vdmin_pp2) X(tdmin_pp) \
+  X(rho_a)
 
+/* hurricane_phase of a step (main.f90:116-123): 0 outside the hurricane window (the face
+ * winds are kept), 1 interpolate (read_hurricane_ser_1.f90:48-60), 2 take the next
+ * snapshot (read_hurricane_ser_1.f90:18-46, every hurricane_read_interval steps). */
 #define GOMGPU_FORCING_SCALARS(X) \
-  X(elapsed_time) X(spinup_function_tide) X(spinup_function_baroclinic)
+  X(elapsed_time) X(spinup_function_tide) X(spinup_function_baroclinic) \
+  X(spinup_function_wind) X(hurricane_phase)
 
 #define GOMGPU_DECL_RPTR(name, ext) double *name;
 #define GOMGPU_DECL_IPTR(name, ext) int32_t *name;
@@ -146,10 +166,17 @@ typedef struct gomgpu_state {
   GOMGPU_STATE_INT(GOMGPU_DECL_IPTR)
 } gomgpu_state;
 
+/* Fields may be NULL in gomgpu_set_forcing (left as they are on the device). */
 typedef struct gomgpu_forcing {
   GOMGPU_FORCING_SCALARS(GOMGPU_DECL_REAL)
   GOMGPU_FORCING_REAL(GOMGPU_DECL_CRPTR)
 } gomgpu_forcing;
+
+/* Destination of gomgpu_get_forcing: the forcing arrays as they are on the device
+ * (the outputs of the wind stage among them); NULL fields are not copied. */
+typedef struct gomgpu_forcing_out {
+  GOMGPU_FORCING_REAL(GOMGPU_DECL_RPTR)
+} gomgpu_forcing_out;
 
 /* Vertical turbulence closure (gomgpu_params.turbulence_flag; turbulence_closure_v1.f90):
  * 0 constant Av/Kv, 1 Richardson step function, 2 Pacanowski-Philander, 3 two-equation
@@ -176,7 +203,8 @@ enum gomgpu_stage_id {
   GOMGPU_STAGE_EOS = 8,
   GOMGPU_STAGE_UPDATE = 9,
   GOMGPU_STAGE_TURBULENCE = 10,
-  GOMGPU_NUM_STAGES = 11
+  GOMGPU_STAGE_WIND = 11,
+  GOMGPU_NUM_STAGES = 12
 };
 
 enum gomgpu_status {
@@ -233,6 +261,10 @@ int gomgpu_step(gomgpu_ctx *ctx);
 int gomgpu_run(gomgpu_ctx *ctx, int nsteps);
 /* Synchronise and download the non-NULL fields. */
 int gomgpu_get_state(gomgpu_ctx *ctx, gomgpu_state *state);
+/* Synchronise and download the non-NULL forcing arrays (the wind stage writes Gamma_T,
+ * the wind stress, air_p1 and the hurricane fields on the device; a Fortran host reads
+ * them back for its output, as write_tser_out_files.f90:312 reads air_p1). */
+int gomgpu_get_forcing(gomgpu_ctx *ctx, gomgpu_forcing_out *forcing);
 /* Total volume sum(area*(h+eta)) over wet cells and salt sum(S*area*dz) in FP64. */
 int gomgpu_budgets(gomgpu_ctx *ctx, double *volume, double *salt);
 /* Iterations taken by the last PCG solve (0 if it did not converge). */

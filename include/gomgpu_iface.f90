@@ -15,6 +15,7 @@ module gomgpu_iface
   integer(c_int), parameter :: GOMGPU_STAGE_EOS = 8
   integer(c_int), parameter :: GOMGPU_STAGE_UPDATE = 9
   integer(c_int), parameter :: GOMGPU_STAGE_TURBULENCE = 10
+  integer(c_int), parameter :: GOMGPU_STAGE_WIND = 11
   integer(c_int), parameter :: GOMGPU_FP64 = 0
   integer(c_int), parameter :: GOMGPU_E_ARG = 1
   integer(c_int), parameter :: GOMGPU_MIXED = 1
@@ -49,17 +50,20 @@ module gomgpu_iface
       & ELM_min_iter, ELM_max_iter, bf_flag, baroclinic_flag, ana_density, i_density_flag, &
       & transport_flag, trans_sub_iter, h_flux_limiter, v_flux_limiter, heat_option, &
       & max_iteration_pcg, precision, river_on_device, turbulence_flag, turbulence_model, &
-      & stability_function = 0
+      & stability_function, hurricane_flag, hurricane_read_interval, wind_formular = 0
     real(c_double) :: dt, theta, gravity, rho_o, MSL, dry_depth, adv_onoff_depth, &
       & smagorinsky_parameter, Ah_0, Kh_0, von_Karman, error_tolerance, &
       & denominator_min_for_matrix, delta_z_min, q2min, psimin, eps_min, bgdiff, diffmax, xlmin1, &
       & xlmin2, cmiu0, cpsi1, cpsi2, rpub, rmub, rnub, schk, schpsi, h1_pp, h2_pp, vdmax_pp1, &
-      & vdmax_pp2, vdmin_pp1, vdmin_pp2, tdmin_pp = 0.0d0
+      & vdmax_pp2, vdmin_pp1, vdmin_pp2, tdmin_pp, rho_a = 0.0d0
   end type gomgpu_params
   type, bind(C) :: gomgpu_forcing
-    real(c_double) :: elapsed_time, spinup_function_tide, spinup_function_baroclinic = 0.0d0
+    real(c_double) :: elapsed_time, spinup_function_tide, spinup_function_baroclinic, &
+      & spinup_function_wind, hurricane_phase = 0.0d0
     type(c_ptr) :: Gamma_T, wind_stress_normal, wind_stress_tangnt, air_p1, eta_at_ob_old, &
-      & eta_at_ob_new, Qu_boundary, phi_sz, con_at_ob, con_at_Qb, Q_add = c_null_ptr
+      & eta_at_ob_new, Qu_boundary, phi_sz, con_at_ob, con_at_Qb, Q_add, wind_u0, wind_v0, air_p0, &
+      & wind_u1, wind_v1, wind_u2, wind_v2, air_p2, wind_u_next, wind_v_next, air_p_next, &
+      & wind_u_at_face, wind_v_at_face = c_null_ptr
   end type gomgpu_forcing
 
   interface

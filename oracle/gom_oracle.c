@@ -2171,8 +2171,80 @@ int oracle_update(const gomgpu_mesh *m, const gomgpu_params *p, gomgpu_state *s,
   return 0;
 }
 
+/* read_hurricane_ser_1.f90:13-63 (the snapshot read replaced by f->wind_*_next) and the
+ * hurricane branch of calculate_wind_stress.f90:88-114; f->hurricane_phase selects what
+ * main.f90:116-123 does this step (0 outside the window, 1 interpolate, 2 next snapshot).
+ * The forcing arrays are this stage's state and outputs, written in place. */
+int oracle_wind(const gomgpu_mesh *m, const gomgpu_params *p, gomgpu_state *s,
+                const gomgpu_forcing *f) {
+  (void)s;
+  if (p->hurricane_flag != 1) return 0;
+  const int P = p->maxnod, J = p->maxface;
+  double *u0 = (double *)f->wind_u0, *v0 = (double *)f->wind_v0, *p0 = (double *)f->air_p0;
+  double *u1 = (double *)f->wind_u1, *v1 = (double *)f->wind_v1, *p1 = (double *)f->air_p1;
+  double *u2 = (double *)f->wind_u2, *v2 = (double *)f->wind_v2, *p2 = (double *)f->air_p2;
+  double *uf = (double *)f->wind_u_at_face, *vf = (double *)f->wind_v_at_face;
+  const double phase = f->hurricane_phase;
+  if (phase == 2.0) {
+    for (int i = 1; i <= P; i++) {
+      R1(u0, i) = R1(u2, i);
+      R1(v0, i) = R1(v2, i);
+      R1(p0, i) = R1(p2, i);
+      R1(u1, i) = R1(u2, i);
+      R1(v1, i) = R1(v2, i);
+      R1(p1, i) = R1(p2, i);
+    }
+    for (int i = 1; i <= P; i++) {
+      R1(u2, i) = R1(f->wind_u_next, i);
+      R1(v2, i) = R1(f->wind_v_next, i);
+      R1(p2, i) = R1(f->air_p_next, i);
+    }
+  } else if (phase == 1.0) {
+    /* real(hurricane_read_interval): a default real, exact for these counts */
+    const double rn = (double)(float)p->hurricane_read_interval;
+    for (int i = 1; i <= P; i++) {
+      double inc = (R1(u2, i) - R1(u0, i)) / rn;
+      R1(u1, i) = R1(u1, i) + inc;
+      inc = (R1(v2, i) - R1(v0, i)) / rn;
+      R1(v1, i) = R1(v1, i) + inc;
+      inc = (R1(p2, i) - R1(p0, i)) / rn;
+      R1(p1, i) = R1(p1, i) + inc;
+    }
+  }
+  if (phase != 0.0) {
+    for (int j = 1; j <= J; j++) {
+      int n1 = I2(m->nodenum_at_face, 1, j), n2 = I2(m->nodenum_at_face, 2, j);
+      R1(uf, j) = (R1(u1, n1) + R1(u1, n2)) * 0.5;
+      R1(vf, j) = (R1(v1, n1) + R1(v1, n2)) * 0.5;
+    }
+  }
+  const double rho_ratio = p->rho_a / p->rho_o;
+  for (int j = 1; j <= J; j++) {
+    double ws = sqrt(R1(uf, j) * R1(uf, j) + R1(vf, j) * R1(vf, j));
+    double wn = R1(uf, j) * R1(m->cos_theta, j) + R1(vf, j) * R1(m->sin_theta, j);
+    double wt = -R1(uf, j) * R1(m->sin_theta, j) + R1(vf, j) * R1(m->cos_theta, j);
+    double cda;
+    if (p->wind_formular == 1)
+      cda = 0.001 * (0.75 + 0.067 * ws);
+    else if (p->wind_formular == 3)
+      cda = 0.001 * (0.80 + 0.065 * ws);
+    else
+      cda = 0.001 * (0.61 + 0.063 * ws);
+    cda = dmin((double)0.003f, cda);
+    double g = cda * rho_ratio * ws;
+    ((double *)f->Gamma_T)[j - 1] = g;
+    ((double *)f->wind_stress_normal)[j - 1] = g * wn * f->spinup_function_wind;
+    ((double *)f->wind_stress_tangnt)[j - 1] = g * wt * f->spinup_function_wind;
+  }
+  return 0;
+}
+
 int oracle_step(const gomgpu_mesh *m, const gomgpu_params *p, gomgpu_state *s,
                 const gomgpu_forcing *f) {
+  if (p->hurricane_flag == 1) {
+    int err = oracle_wind(m, p, s, f);
+    if (err) return err;
+  }
   int (*stages[])(const gomgpu_mesh *, const gomgpu_params *, gomgpu_state *, const gomgpu_forcing *) = {
       oracle_friction, oracle_elm, oracle_momentum, oracle_freesurf, oracle_velocity,
       oracle_layers, oracle_nodevel, oracle_turbulence, oracle_transport, oracle_eos, oracle_update};

@@ -53,6 +53,8 @@ int oracle_turbulence(const gomgpu_mesh *m, const gomgpu_params *p, gomgpu_state
                       const gomgpu_forcing *f);
 /* Quasi-equilibrium stability functions Sm, Sh at Gh (GOMGPU_STAB_GA or _KC). */
 void oracle_stability(int kind, double gh, double *sm, double *sh);
+int oracle_wind(const gomgpu_mesh *m, const gomgpu_params *p, gomgpu_state *s,
+                const gomgpu_forcing *f);
 /* All stages in the order of main.f90:95-272 (friction first), with the turbulence
  * closure between the node velocities and the transport when turbulence_flag != 0. */
 int oracle_step(const gomgpu_mesh *m, const gomgpu_params *p, gomgpu_state *s,

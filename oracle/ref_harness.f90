@@ -298,6 +298,13 @@ contains
       call put_i0('num_ob_cell', num_ob_cell); call put_i0('num_Qb_cell', num_Qb_cell)
       call put_i0('num_WR_cell', num_WR_cell); call put_i0('num_SS_cell', num_SS_cell)
       call put_i0('i_density_flag', i_density_flag)
+      call put_r0('rho_a', rho_a); call put_i0('wind_formular', wind_formular)
+      call put_i0('hurricane_flag', hurricane_flag)
+      if (hurricane_flag == 1) then
+         call put_i0('hurricane_read_interval', hurricane_read_interval)
+      else
+         call put_i0('hurricane_read_interval', 0)
+      end if
       call put_i1('tran_id', tran_id)
       call put_i2('start_end_node', reshape(start_end_node, [4, 12]), 1)
       call put_r1('z_level', z_level); call put_r1('delta_z', delta_z)
@@ -356,6 +363,13 @@ contains
       call put_r1('wind_stress_normal', wind_stress_normal)
       call put_r1('wind_stress_tangnt', wind_stress_tangnt)
       call put_r1('air_p1', air_p1)
+      call put_r0('spinup_function_wind', spinup_function_wind)
+      if (hurricane_flag == 1) then
+         call put_r1('wind_u0', wind_u0); call put_r1('wind_v0', wind_v0); call put_r1('air_p0', air_p0)
+         call put_r1('wind_u1', wind_u1); call put_r1('wind_v1', wind_v1)
+         call put_r1('wind_u2', wind_u2); call put_r1('wind_v2', wind_v2); call put_r1('air_p2', air_p2)
+         call put_r1('wind_u_at_face', wind_u_at_face); call put_r1('wind_v_at_face', wind_v_at_face)
+      end if
       if (num_ob_cell > 0) then
          call put_r1('eta_at_ob_old', eta_at_ob_old); call put_r1('eta_at_ob_new', eta_at_ob_new)
       end if

@@ -110,6 +110,10 @@ def params_from_geom(g: Dict[str, np.ndarray]):
         smagorinsky_parameter=g["smagorinsky_parameter"], Ah_0=g["Ah_0"], Kh_0=g["Kh_0"],
         von_Karman=g["von_Karman"], error_tolerance=g["error_tolerance"],
         denominator_min_for_matrix=g["denominator_min_for_matrix"], delta_z_min=g["delta_z_min"])
+    # dumped by the harness since the wind stage exists (older fixtures lack them)
+    for k in ("rho_a", "wind_formular", "hurricane_flag", "hurricane_read_interval"):
+        if k in g:
+            d[k] = g[k]
     return abi.params_from_dict(d)
 
 
@@ -151,6 +155,8 @@ def forcing_from_dump(d: Dict[str, np.ndarray], p, con_at_ob=None, con_at_Qb=Non
     f.scalars["elapsed_time"] = float(d["elapsed_time"])
     f.scalars["spinup_function_tide"] = float(d["spinup_function_tide"])
     f.scalars["spinup_function_baroclinic"] = float(d["spinup_function_baroclinic"])
+    if "spinup_function_wind" in d:
+        f.scalars["spinup_function_wind"] = float(d["spinup_function_wind"])
     if con_at_ob is not None:
         f["con_at_ob"] = con_at_ob
     if con_at_Qb is not None:

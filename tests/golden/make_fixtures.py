@@ -39,7 +39,9 @@ from gom_release_amd import gomio, meshgen  # noqa: E402
 STAGE_OUTPUTS = {
     "pre": ["Cdb", "Gamma_B", "Gamma_T", "wind_stress_normal", "wind_stress_tangnt", "air_p1",
             "eta_at_ob_old", "eta_at_ob_new", "Qu_boundary", "phi_sz", "it", "elapsed_time",
-            "julian_day", "spinup_function_tide", "spinup_function_baroclinic"],
+            "julian_day", "spinup_function_tide", "spinup_function_baroclinic",
+            "spinup_function_wind", "wind_u0", "wind_v0", "air_p0", "wind_u1", "wind_v1",
+            "wind_u2", "wind_v2", "air_p2", "wind_u_at_face", "wind_v_at_face"],
     "elm": ["un_ELM", "vn_ELM", "num_sub_elm_iteration"],
     "mom": ["AinvG1", "AinvG2", "AinvDeltaZ1", "AinvDeltaZ2", "Kh"],
     "fs": ["eta_cell_new", "eta_node", "wetdry_node"],
@@ -57,7 +59,9 @@ STAGE_OUTPUTS = {
 STAGE_ORDER = ["pre", "elm", "mom", "fs", "vel", "lay", "nodev", "tvd", "eos", "post"]
 FORCING_ONLY = {"Gamma_T", "wind_stress_normal", "wind_stress_tangnt", "air_p1", "eta_at_ob_old",
                 "eta_at_ob_new", "Qu_boundary", "phi_sz", "it", "elapsed_time", "julian_day",
-                "spinup_function_tide", "spinup_function_baroclinic"}
+                "spinup_function_tide", "spinup_function_baroclinic", "spinup_function_wind",
+                "wind_u0", "wind_v0", "air_p0", "wind_u1", "wind_v1", "wind_u2", "wind_v2",
+                "air_p2", "wind_u_at_face", "wind_v_at_face"}
 
 CASES = {
     "channel_16x8x5": dict(kind="channel", args=dict(nx=16, ny=8, nlayer=5, ndt=3), steps=[1, 2, 3]),
@@ -66,13 +70,18 @@ CASES = {
                                      elm_max_iter=10, dry_land=True, river_q=50.0,
                                      tide_phase=-90.0),
                            steps=[1, 2, 44, 45, 46]),
+    # Holland cyclone fields read every third step; the hurricane window opens at step 4
+    "hurricane_estuary": dict(kind="estuary",
+                              args=dict(ncol=6, nrep=2, nlayer=3, ndt=8, dry_land=True,
+                                        tide_phase=-90.0, river_q=30.0),
+                              hurricane=True, steps=[1, 2, 3, 4, 5, 6, 7, 8]),
 }
 
 
 def make_case(spec):
-    if spec["kind"] == "channel":
-        return meshgen.channel_case(**spec["args"])
-    return meshgen.estuary_case(**spec["args"])
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    from golden_io import case_of
+    return case_of(spec)
 
 
 def check_outputs(prev, cur, stage):
@@ -110,6 +119,8 @@ def make(name: str, spec, outdir: str) -> str:
     path = os.path.join(outdir, name + ".npz")
     np.savez_compressed(path, **out)
     meta = dict(case=name, kind=spec["kind"], args=spec["args"], steps=steps)
+    if spec.get("hurricane"):
+        meta["hurricane"] = True
     with open(os.path.join(outdir, name + ".json"), "w") as f:
         json.dump(meta, f, indent=1)
     return path

@@ -13,7 +13,7 @@ GOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
 STAGES = [("friction", "pre"), ("elm", "elm"), ("momentum", "mom"), ("freesurf", "fs"),
           ("velocity", "vel"), ("layers", "lay"), ("nodevel", "nodev"), ("transport", "tvd"),
           ("eos", "eos"), ("update", "post")]
-FIXTURES = ["channel_16x8x5", "estuary_wetdry"]
+FIXTURES = ["channel_16x8x5", "estuary_wetdry", "hurricane_estuary"]
 
 
 def _flat(a):
@@ -64,9 +64,51 @@ class Golden:
         return out
 
     def case(self):
-        from gom_release_amd import meshgen
-        m = self.meta
-        return meshgen.channel_case(**m["args"]) if m["kind"] == "channel" else meshgen.estuary_case(**m["args"])
+        return case_of(self.meta)
+
+    def step_forcing(self, it: int, case) -> "abi.ForcingArrays":
+        """The forcing a whole step `it` starts from: as `forcing`, but with the hurricane
+        fields as they were before the step (the step's wind stage recomputes the wind
+        stress and air pressure), the snapshot the step reads, and the step's phase."""
+        f = self.forcing(it, case)
+        if self.params.hurricane_flag != 1:
+            return f
+        from gom_release_amd import forcing as fz
+        prev = self.group(f"s{it}_in")
+        pre = self.group(f"s{it}_pre")
+        for n in HURRICANE_FIELDS:
+            f[n] = _flat(prev[n])
+        f["wind_u_next"], f["wind_v_next"], f["air_p_next"] = (_flat(pre["wind_u2"]), _flat(pre["wind_v2"]),
+                                                                _flat(pre["air_p2"]))
+        f.scalars["hurricane_phase"] = float(fz.hurricane_phase(it, float(pre["julian_day"]),
+                                                                fz.hurricane_window(case),
+                                                                self.params.hurricane_read_interval))
+        return f
+
+
+HURRICANE_FIELDS = ["wind_u0", "wind_v0", "air_p0", "wind_u1", "wind_v1", "air_p1", "wind_u2",
+                    "wind_v2", "air_p2", "wind_u_at_face", "wind_v_at_face"]
+WIND_OUTPUTS = HURRICANE_FIELDS + ["Gamma_T", "wind_stress_normal", "wind_stress_tangnt"]
+
+
+def case_of(spec):
+    """The generated case of a fixture description (kind, args, hurricane)."""
+    from gom_release_amd import meshgen
+    if spec["kind"] == "channel":
+        case = meshgen.channel_case(**spec["args"])
+    else:
+        case = meshgen.estuary_case(**spec["args"])
+    if spec.get("hurricane"):
+        # a Holland cyclone (950 hPa, r_max 300 m) crossing the small estuary at 5 m/s,
+        # fields every third step; the model starts at 23:53 so the window opens at step 4
+        case.start = (2020, 1, 1, 23, 53, 0)
+        case.hurricane_flag, case.hurricane_dt = 1, int(3 * case.dt)
+        case.hurricane_start, case.hurricane_end = (2020, 1, 2), (2020, 1, 3)
+        x0, y0 = float(case.x.min()), float(0.5 * (case.y.min() + case.y.max()))
+        times = [k * case.hurricane_dt for k in range(case.ndt // 3 + 3)]
+        case.hurricane = meshgen.holland_snapshots(case.x, case.y, times, (x0 - 200.0, y0 - 150.0),
+                                                   (5.0, 1.0), p_min=950.0, r_max=300.0, lat=case.lat)
+    return case
 
 
 def apply(state: "abi.Arrays", values: Dict[str, np.ndarray]) -> None:

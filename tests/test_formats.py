@@ -331,11 +331,12 @@ def test_reads_the_reference_project():
     assert pr.q_series[0].values[0] == 287.45 and pr.q_series[0].interp == 1
     bad = gomio.unsupported(pr)
     assert bad == ["C7_3 surface heat flux (heat_option with air_ser.inp)",
-                   "C31 hurricane wind and pressure fields"]
+                   "C31 hurricane fields: input/hurricane_ser.inp is missing"]
     with pytest.raises(NotImplementedError):
         gomio.case_from_project(pr)
     case, dropped = gomio.case_from_project(pr, strict=False)
     assert dropped == bad and case.is_temp == 0 and len(case.open_boundaries) == 75
+    assert case.hurricane_flag == 0
 
 
 # ------------------------------------------- the host driver's output files, on the oracle

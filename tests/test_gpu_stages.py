@@ -72,7 +72,7 @@ def test_gpu_whole_step_matches_reference(gpu_available, name):
     model = _model(g)
     for it in g.steps:
         model.set_state(g.state_in(it))
-        model.set_forcing(g.forcing(it, case))
+        model.set_forcing(g.step_forcing(it, case))
         model.step()
         model.sync()
         out = abi.state_arrays(g.params)

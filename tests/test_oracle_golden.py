@@ -37,7 +37,7 @@ def test_oracle_whole_steps_bitwise(name):
     case = g.case()
     for it in g.steps:
         state = g.state_in(it)
-        o = Oracle(g.mesh, g.params, state, g.forcing(it, case))
+        o = Oracle(g.mesh, g.params, state, g.step_forcing(it, case))
         assert o.step() == 0
         for field, ref in g.expected(it, "post").items():
             assert np.array_equal(state[field], ref), (name, it, field)
