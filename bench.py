@@ -41,18 +41,28 @@ TURBULENCE = {"off": dict(turbulence_flag=0),
 PCG_TOLERANCE = 5.0e-6
 
 
-def make_case(cells: str, turbulence: str = "gls", pcg_tolerance: float = PCG_TOLERANCE):
+def make_case(cells: str, turbulence: str = "gls", pcg_tolerance: float = PCG_TOLERANCE,
+              storm_steps: int = 0):
+    """The estuary of `cells`; with storm_steps > 0 the storm-surge physics of the 1.2M-cell
+    configuration: a Holland cyclone (950 hPa, r_max 40 km, B 1.5) crossing at 5 m/s, its
+    fields every 10 steps for storm_steps steps, and stratified temperature (10 C) and
+    salinity (3 psu) transported."""
     from gom_release_amd import meshgen
     kw = dict(nlayer=20, dt=120.0, river_q=500.0, trans_sub_iter=2, h_flux_limiter=1, v_flux_limiter=1,
               error_tolerance=pcg_tolerance, max_iteration_pcg=1000, **TURBULENCE[turbulence])
     if cells == "200k":
-        return meshgen.estuary_case(ncol=400, nrep=100, **kw)
-    if cells == "20k":
-        return meshgen.estuary_case(ncol=100, nrep=40, **kw)
-    if cells == "1200k":  # the size of the 1.2M-cell x 30-layer configuration
+        case = meshgen.estuary_case(ncol=400, nrep=100, **kw)
+    elif cells == "20k":
+        case = meshgen.estuary_case(ncol=100, nrep=40, **kw)
+    elif cells == "1200k":  # the size of the 1.2M-cell x 30-layer configuration
         kw.update(nlayer=30, dt=60.0)
-        return meshgen.estuary_case(ncol=980, nrep=245, **kw)
-    raise ValueError(cells)
+        case = meshgen.estuary_case(ncol=980, nrep=245, **kw)
+    else:
+        raise ValueError(cells)
+    if storm_steps > 0:
+        meshgen.add_cyclone(case, storm_steps, seed=3, p_min=950.0, r_max=40.0e3, b=1.5, speed=5.0)
+        meshgen.stratify(case, d_temp=10.0, d_salt=3.0)
+    return case
 
 
 def pcg_bytes_per_iteration(mesh, params) -> float:
@@ -139,29 +149,35 @@ def serving_sample(n_req: int = 12, rate: float = 2.0, steps: int = 30, seed: in
     sizes = _log_uniform(rng, 10e3, 200e3, n_req)
     layers = rng.integers(10, 31, n_req)
     arrivals = serving.poisson_trace(n_req, rate, seed=seed)
-    reqs = [serving.Request(i, serving.estuary_of_size(sizes[i], int(layers[i]), seed=100 + i, dt=60.0,
-                                                       river_q=200.0, tide_phase=float(rng.uniform(-180, 180)),
-                                                       turbulence_flag=3),
-                            steps, arrivals[i], wind=(8.0, 2.0)) for i in range(n_req)]
+    from gom_release_amd import meshgen
+    reqs = [serving.Request(i, meshgen.add_cyclone(
+                serving.estuary_of_size(sizes[i], int(layers[i]), seed=100 + i, dt=60.0, river_q=200.0,
+                                        tide_phase=float(rng.uniform(-180, 180)), turbulence_flag=3),
+                steps, seed=300 + i, p_min=float(rng.uniform(940.0, 990.0))),
+                            steps, arrivals[i]) for i in range(n_req)]
     results, summary = serving.ServingLoop(64 << 30, workers=4).run(reqs)
     summary.update({"lambda_per_s": rate, "steps_per_request": steps,
                     "cells_min_max": [min(r.cells for r in results), max(r.cells for r in results)],
-                    "sample": f"{n_req} requests of 10k-200k cells x 10-30 layers, {steps} steps each "
-                              f"(the 24 h scenarios shortened), GLS closure, 64 GiB budget"})
+                    "sample": f"{n_req} storm-surge requests of 10k-200k cells x 10-30 layers, {steps} steps "
+                              f"each (the 24 h scenarios shortened), own Holland cyclone (940-990 hPa) and "
+                              f"tidal phase, GLS closure, 64 GiB budget"})
     return summary
 
 
 def ensemble_sample(n_mem: int = 8, steps: int = 10, seed: int = 7):
-    """Members of 20k-200k cells and 10-30 layers with their own tidal phase: stepped
-    together in one batch (fused PCG, captured graph) against one after another."""
-    from gom_release_amd import driver, host, serving
+    """Members of 20k-200k cells and 10-30 layers with their own tidal phase and their own
+    Holland cyclone (track and 940-990 hPa central pressure): stepped together in one batch
+    (fused PCG, captured graph) against one after another."""
+    from gom_release_amd import driver, host, meshgen, serving
     rng = np.random.default_rng(seed)
     sizes = _log_uniform(rng, 20e3, 200e3, n_mem)
     layers = rng.integers(10, 31, n_mem)
-    cases = [serving.estuary_of_size(sizes[i], int(layers[i]), seed=200 + i, dt=60.0, river_q=200.0,
-                                     tide_phase=float(rng.uniform(-180, 180)), turbulence_flag=3)
+    cases = [meshgen.add_cyclone(
+                 serving.estuary_of_size(sizes[i], int(layers[i]), seed=200 + i, dt=60.0, river_q=200.0,
+                                         tide_phase=float(rng.uniform(-180, 180)), turbulence_flag=3),
+                 2 * steps + 4, seed=400 + i, p_min=float(rng.uniform(940.0, 990.0)))
              for i in range(n_mem)]
-    scs = [driver.Scenario(c, wind=(8.0, 2.0)) for c in cases]
+    scs = [driver.Scenario(c) for c in cases]
     for sc in scs:   # the batch partition, alone and batched alike
         sc.model.set_pcg_rows(serving.BATCH_PCG_ROWS)
     # serial: each member alone
@@ -212,6 +228,8 @@ def main() -> int:
     ap.add_argument("--cells", default="200k")
     ap.add_argument("--turbulence", choices=sorted(TURBULENCE), default="gls",
                     help="vertical closure of the workload (gls: k-kl GLS, Galperin stability functions)")
+    ap.add_argument("--storm", action="store_true",
+                    help="cyclone forcing and stratified T/S (the physics of the 1.2M-cell configuration)")
     ap.add_argument("--pcg-tolerance", type=float, default=PCG_TOLERANCE,
                     help="PCG stop tolerance on r.M^-1 r / r0.M^-1 r0 (default: the reference's main.inp)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
@@ -235,8 +253,9 @@ def main() -> int:
     if args.copy_calibration > 0:
         copy_calibration(args.copy_calibration << 20)
         print(f"[bench] calibration copy of {args.copy_calibration} MiB done", file=sys.stderr, flush=True)
-    wind = (10.0, 0.0)
-    case = make_case(args.cells, args.turbulence, args.pcg_tolerance)
+    wind = (0.0, 0.0) if args.storm else (10.0, 0.0)
+    case = make_case(args.cells, args.turbulence, args.pcg_tolerance,
+                     storm_steps=(4000 + args.warmup + args.steps + args.profile_steps) if args.storm else 0)
     prec = 0 if args.precision == "fp64" else 1
     print(f"[bench] building {case.maxele}-cell x {case.maxlayer}-layer case", file=sys.stderr, flush=True)
     sc = Scenario(case, precision=prec, wind=wind)
@@ -301,7 +320,9 @@ def main() -> int:
         "data": "synthetic",
         "config": {"workload": f"synthetic estuary, {case.maxele} mixed tri/quad cells x {case.maxlayer} "
                                f"z-layers, dt={case.dt:g} s, TVD minmod x2 sub-cycles, M2+K1, 500 m3/s river, "
-                               f"10 m/s wind, " + ("GLS k-kl turbulence closure (Galperin)"
+                               + ("Holland cyclone (950 hPa, r_max 40 km, B 1.5, 5 m/s; fields every 10 steps), "
+                                  "stratified T/S (10 C, 3 psu) transported, " if args.storm else "10 m/s wind, ")
+                               + ("GLS k-kl turbulence closure (Galperin)"
                                                    if args.turbulence == "gls" else "constant Av/Kv")
                                + f", PCG tolerance {case.error_tolerance:g}",
                    "turbulence": args.turbulence,

@@ -187,12 +187,9 @@ def estuary_case(ncol: int = 40, nrep: int = 4, nlayer: int = 5, dx: float = 100
     return case
 
 
-def holland_snapshots(x: np.ndarray, y: np.ndarray, times: Sequence[float], centre0: Tuple[float, float],
-                      velocity: Tuple[float, float], p_min: float = 950.0, p_env: float = 1013.0,
-                      r_max: float = 40.0e3, b: float = 1.5, lat: float = 30.0, rho_air: float = 1.15,
-                      surface_factor: float = 0.8):
+class HollandTrack:
     """Node snapshots (u [m/s], v [m/s], p [hPa]) of a translating Holland (1980) cyclone,
-    one per time in `times` [s], for hurricane_ser.inp.
+    one per time in `times` [s], generated as they are iterated (hurricane_ser.inp order).
 
     Holland, G. J. (1980), Mon. Wea. Rev. 108, 1212-1218: with dp = p_env - p_min and
     s = (r_max / r)^b, the surface pressure is p = p_min + dp exp(-s) and the gradient wind
@@ -200,23 +197,87 @@ def holland_snapshots(x: np.ndarray, y: np.ndarray, times: Sequence[float], cent
     `surface_factor` V, counter-clockwise about the centre (northern hemisphere), plus the
     translation velocity of the centre. The centre starts at `centre0` [m] and moves at
     `velocity` [m/s]."""
-    x = np.asarray(x, dtype=np.float64)
-    y = np.asarray(y, dtype=np.float64)
-    f = 2.0 * 7.292e-5 * np.sin(np.radians(lat))
-    dp = (p_env - p_min) * 100.0                      # Pa
-    out = []
-    for t in times:
-        cx = centre0[0] + velocity[0] * t
-        cy = centre0[1] + velocity[1] * t
-        dx, dy = x - cx, y - cy
+
+    def __init__(self, x, y, times: Sequence[float], centre0: Tuple[float, float],
+                 velocity: Tuple[float, float], p_min: float = 950.0, p_env: float = 1013.0,
+                 r_max: float = 40.0e3, b: float = 1.5, lat: float = 30.0, rho_air: float = 1.15,
+                 surface_factor: float = 0.8):
+        self.x = np.asarray(x, dtype=np.float64)
+        self.y = np.asarray(y, dtype=np.float64)
+        self.times = [float(t) for t in times]
+        self.centre0, self.velocity = tuple(centre0), tuple(velocity)
+        self.p_min, self.p_env, self.r_max, self.b = p_min, p_env, r_max, b
+        self.lat, self.rho_air, self.surface_factor = lat, rho_air, surface_factor
+
+    def __len__(self) -> int:
+        return len(self.times)
+
+    def at(self, t: float):
+        f = 2.0 * 7.292e-5 * np.sin(np.radians(self.lat))
+        dp = (self.p_env - self.p_min) * 100.0               # Pa
+        cx = self.centre0[0] + self.velocity[0] * t
+        cy = self.centre0[1] + self.velocity[1] * t
+        dx, dy = self.x - cx, self.y - cy
         r = np.hypot(dx, dy)
         rs = np.maximum(r, 1.0)
-        s = (r_max / rs) ** b
+        s = (self.r_max / rs) ** self.b
         es = np.exp(-s)
-        p = p_min + (p_env - p_min) * es              # hPa
-        vg = np.sqrt(b * dp * s * es / rho_air + (rs * f / 2.0) ** 2) - rs * f / 2.0
-        vs = np.where(r > 1.0, surface_factor * vg, 0.0)
-        u = -vs * dy / rs + velocity[0]
-        v = vs * dx / rs + velocity[1]
-        out.append((u, v, p))
-    return out
+        p = self.p_min + (self.p_env - self.p_min) * es      # hPa
+        vg = np.sqrt(self.b * dp * s * es / self.rho_air + (rs * f / 2.0) ** 2) - rs * f / 2.0
+        vs = np.where(r > 1.0, self.surface_factor * vg, 0.0)
+        u = -vs * dy / rs + self.velocity[0]
+        v = vs * dx / rs + self.velocity[1]
+        return u, v, p
+
+    def __iter__(self):
+        for t in self.times:
+            yield self.at(t)
+
+
+def holland_snapshots(x, y, times: Sequence[float], centre0: Tuple[float, float],
+                      velocity: Tuple[float, float], p_min: float = 950.0, p_env: float = 1013.0,
+                      r_max: float = 40.0e3, b: float = 1.5, lat: float = 30.0, rho_air: float = 1.15,
+                      surface_factor: float = 0.8) -> HollandTrack:
+    """The HollandTrack of these parameters (snapshots generated when iterated)."""
+    return HollandTrack(x, y, times, centre0, velocity, p_min, p_env, r_max, b, lat, rho_air, surface_factor)
+
+
+def add_cyclone(case: Case, steps: int, seed: int = 0, p_min: float = 950.0, r_max: float = 40.0e3,
+                b: float = 1.5, speed: float = 5.0, hurricane_dt: Optional[int] = None) -> Case:
+    """Give a case Holland-cyclone forcing for `steps` steps: hurricane fields every
+    `hurricane_dt` seconds (default 10 steps) from the model start, a track entering from a
+    seeded side and crossing the domain at `speed` m/s; the uniform wind is replaced."""
+    rng = np.random.default_rng(seed)
+    cx, cy = 0.5 * (case.x.min() + case.x.max()), 0.5 * (case.y.min() + case.y.max())
+    ang = rng.uniform(0.0, 2.0 * np.pi)
+    heading = ang + np.pi + rng.uniform(-np.pi / 6, np.pi / 6)
+    start = (cx + 1.5 * r_max * np.cos(ang), cy + 1.5 * r_max * np.sin(ang))
+    vel = (speed * np.cos(heading), speed * np.sin(heading))
+    hdt = int(hurricane_dt if hurricane_dt is not None else 10 * case.dt)
+    times = [k * float(hdt) for k in range(int(steps * case.dt // hdt) + 3)]
+    case.hurricane_flag, case.hurricane_dt = 1, hdt
+    y, mo, d = case.start[:3]
+    case.hurricane_start, case.hurricane_end = (y, mo, d), (y + 1, mo, d)
+    case.hurricane = holland_snapshots(case.x, case.y, times, start, vel, p_min=p_min, r_max=r_max, b=b,
+                                       lat=case.lat)
+    return case
+
+
+def stratify(case: Case, d_temp: float = 10.0, d_salt: float = 3.0, t_bottom: float = 15.0) -> Case:
+    """Stratified temperature and salinity: temperature from t_bottom at the bottom layer
+    to t_bottom + d_temp at the top; salinity raised towards the bottom by up to d_salt,
+    in proportion to the along-axis salinity. Turns on temperature transport (without a
+    surface heat flux)."""
+    L = case.maxlayer
+    frac = np.arange(L) / max(L - 1, 1)                          # 0 bottom .. 1 top
+    case.is_temp = 1
+    case.temp_init = np.repeat((t_bottom + d_temp * frac)[None, :], case.maxnod, axis=0)
+    case.temp_series = [Series([-1000.0, 100000.0], [t_bottom + d_temp, t_bottom + d_temp])]
+    for bnd in case.open_boundaries:
+        bnd.temp_ser_id = 1
+    for r in case.rivers:
+        r.temp_ser_id = 1
+    if case.salt_init is not None:
+        s = np.asarray(case.salt_init, dtype=np.float64)
+        case.salt_init = s + d_salt * (1.0 - frac)[None, :] * (s / max(float(s.max()), 1e-30))
+    return case

@@ -24,6 +24,9 @@ def test_tuned_scenario_continues_identically(gpu_available):
     print("[gpu-tune] tune + compare", flush=True)
     case = meshgen.estuary_case(ncol=60, nrep=20, nlayer=8, river_q=100.0, trans_sub_iter=2,
                                 h_flux_limiter=1, v_flux_limiter=1, turbulence_flag=3)
+    # cyclone fields read every third step: tuning must leave the wind stage's device
+    # fields and the host's snapshot stream where they were
+    meshgen.add_cyclone(case, 40, seed=4, r_max=20e3, p_min=970.0, hurricane_dt=int(3 * case.dt))
     a = Scenario(case, wind=(6.0, 2.0))
     a.run(2)
     cfg = autotune.tune(a, steps=1, candidates=(64, 512))

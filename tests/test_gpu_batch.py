@@ -25,12 +25,16 @@ def gpu_available():
 
 def _cases():
     from gom_release_amd import meshgen
+    # the second member has cyclone forcing (its wind stage runs in the batch), the third
+    # transports a stratified temperature as well
+    storm = meshgen.estuary_case(ncol=20, nrep=3, nlayer=6, ndt=20, elm_backtrace_flag=1, elm_max_iter=10,
+                                 dry_land=True, river_q=50.0, tide_phase=-90.0, turbulence_flag=3)
+    meshgen.add_cyclone(storm, 40, seed=1, r_max=2000.0, hurricane_dt=int(3 * storm.dt))
     return [
         meshgen.channel_case(nx=16, ny=8, nlayer=5, ndt=20),
-        meshgen.estuary_case(ncol=20, nrep=3, nlayer=6, ndt=20, elm_backtrace_flag=1, elm_max_iter=10,
-                             dry_land=True, river_q=50.0, tide_phase=-90.0, turbulence_flag=3),
-        meshgen.estuary_case(ncol=60, nrep=20, nlayer=8, ndt=20, river_q=100.0, trans_sub_iter=2,
-                             h_flux_limiter=1, v_flux_limiter=1),
+        storm,
+        meshgen.stratify(meshgen.estuary_case(ncol=60, nrep=20, nlayer=8, ndt=20, river_q=100.0, trans_sub_iter=2,
+                                              h_flux_limiter=1, v_flux_limiter=1)),
     ]
 
 
