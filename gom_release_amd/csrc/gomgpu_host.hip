@@ -39,11 +39,11 @@ __global__ void k_tvd_init(DevMesh, DevState, DevForcing, DevWork, DevDims, TvdA
 __global__ void k_tvd_bnd(DevMesh, DevState, DevForcing, DevWork, DevDims, TvdArgs);
 __global__ void k_tvd_coef_face(DevMesh, DevState, DevWork, DevDims, TvdArgs);
 __global__ void k_tvd_coef_cell(DevMesh, DevState, DevWork, DevDims, TvdArgs);
-__global__ void k_tvd_lim_cell(DevMesh, DevState, DevWork, DevDims, int);
-__global__ void k_tvd_lim_face(DevMesh, DevState, DevWork, DevDims, int);
+__global__ void k_tvd_lim_cell(DevMesh, DevState, DevWork, DevDims, int, int);
+__global__ void k_tvd_lim_face(DevMesh, DevState, DevWork, DevDims, int, int);
 __global__ void k_tvd_rhs(DevMesh, DevState, DevWork, DevDims, TvdArgs);
 __global__ void k_tvd_solve(DevMesh, DevState, DevWork, DevDims, TvdArgs, int *);
-__global__ void k_tvd_copyback(DevMesh, DevState, DevWork, DevDims);
+__global__ void k_tvd_copyback(DevMesh, DevState, DevWork, DevDims, int);
 __global__ void k_tvd_node(DevMesh, DevState, DevWork, DevDims);
 __global__ void k_tvd_face_out(DevMesh, DevState, DevWork, DevDims);
 __global__ void k_tvd_cellnode_out(DevState, DevWork, DevDims);
@@ -868,14 +868,15 @@ static int stage_impl(gomgpu_ctx *c, int id) {
         k_tvd_coef_cell<<<nblk((long)d.E * (d.L + 1), BS), BS, 0, st>>>(c->m, c->s, c->w, d, a);
         if (p.h_flux_limiter > 0 || p.v_flux_limiter > 0) {
           k_tvd_lim_cell<<<nblk((long)(d.L + 1) * d.E, LB(TVD_LIM)), LB(TVD_LIM), 0, st>>>(c->m, c->s, c->w, d,
-                                                                                         p.h_flux_limiter);
-          k_tvd_lim_face<<<nblk((long)d.J * d.L, BS), BS, 0, st>>>(c->m, c->s, c->w, d, p.h_flux_limiter);
+                                                                                         p.h_flux_limiter, sub == 0);
+          k_tvd_lim_face<<<nblk((long)d.J * d.L, BS), BS, 0, st>>>(c->m, c->s, c->w, d, p.h_flux_limiter,
+                                                                   sub == 0);
         }
         KT_BEGIN(KT_TVD_COLUMN);
         k_tvd_rhs<<<nblk((long)d.L * d.E, LB(TVD_RHS)), LB(TVD_RHS), 0, st>>>(c->m, c->s, c->w, d, a);
         k_tvd_solve<<<nblk(d.E, LB(TVD_SOLVE)), LB(TVD_SOLVE), 0, st>>>(c->m, c->s, c->w, d, a, err);
         KT_END(KT_TVD_COLUMN);
-        k_tvd_copyback<<<nblk(nE2, BS), BS, 0, st>>>(c->m, c->s, c->w, d);
+        k_tvd_copyback<<<nblk(nE2, BS), BS, 0, st>>>(c->m, c->s, c->w, d, sub == 0);
       }
       k_tvd_node<<<nblk(d.P, BS), BS, 0, st>>>(c->m, c->s, c->w, d);
       k_tvd_face_out<<<nblk((long)d.J * d.L, BS), BS, 0, st>>>(c->m, c->s, c->w, d);

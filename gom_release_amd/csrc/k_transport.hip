@@ -87,7 +87,9 @@ __global__ void k_tvd_coef_face(DevMesh m, DevState s, DevWork w, DevDims d, Tvd
   if (q >= J * d.L) return;
   int j = q % J, k = q / J + 1;
   double Q = 0.0, dd = 0.0, phi = 0.0;
-  if (k >= m.bottom_layer_at_face[j] && k <= s.top_layer_at_face[j]) {
+  const bool in = k >= m.bottom_layer_at_face[j] && k <= s.top_layer_at_face[j];
+  if (!in && a.sub > 0) return;  // zero since the first sub-cycle (the layers are fixed)
+  if (in) {
     double dzf = AL(s.dz_face, k, j, J);
     double dzo = dzf + (AL(s.dz_face_new, k, j, J) - dzf) * a.sub / a.Nt;
     Q = m.face_length[j] * dzo * ((1.0 - a.th) * AL(s.un_face, k, j, J) + a.th * AL(s.un_face_new, k, j, J));
@@ -160,7 +162,12 @@ __global__ void k_tvd_coef_cell(DevMesh m, DevState s, DevWork w, DevDims d, Tvd
   if (q >= E * (d.L + 1)) return;
   int e = q % E, k = q / E;
   double Q = 0.0, dd = 0.0, phi = 0.0;
-  if (k >= m.bottom_layer_at_element[e] && k <= s.top_layer_at_element[e] - 1) {
+  const int tl = s.top_layer_at_element[e];
+  const bool in = k >= m.bottom_layer_at_element[e] && k <= tl - 1;
+  // levels outside the range stay zero after the first sub-cycle; the top layer's thread
+  // still runs for the limiter sums
+  if (!in && a.sub > 0 && !((a.hlim > 0 || a.vlim > 0) && k >= 1 && k == tl)) return;
+  if (in) {
     double ar = m.area[e];
     Q = ar * ((1.0 - a.th) * A0(s.wn_cell, k, e, E) + a.th * A0(s.wn_cell_new, k, e, E));
     double dh = A0(s.dzhalf_cell, k, e, E);
@@ -169,10 +176,12 @@ __global__ void k_tvd_coef_cell(DevMesh m, DevState s, DevWork w, DevDims d, Tvd
     dd = dmaxf(0.0, D - 0.5 * fabs(Q));
     phi = (Q == 0.0) ? 1.0 : dminf(1.0, 2.0 * D / fabs(Q));
   }
-  A0(w.Qi, k, e, E) = Q;
-  A0(w.di, k, e, E) = dd;
-  A0(w.phii, k, e, E) = phi;
-  if ((a.hlim > 0 || a.vlim > 0) && k >= 1 && k == s.top_layer_at_element[e]) {
+  if (in || a.sub == 0) {
+    A0(w.Qi, k, e, E) = Q;
+    A0(w.di, k, e, E) = dd;
+    A0(w.phii, k, e, E) = phi;
+  }
+  if ((a.hlim > 0 || a.vlim > 0) && k >= 1 && k == tl) {
     double sum1[2], sum2[2];
     lim_hsums(m, w, d, e, k, sum1, sum2);
     for (int t = 0; t < d.NT; t++) {
@@ -187,14 +196,15 @@ __global__ void k_tvd_coef_cell(DevMesh m, DevState s, DevWork w, DevDims d, Tvd
 // vertical ratio r_ik (level k) with its limiter PSI_ik (level k). As in the reference, the
 // vertical ratios reuse the horizontal sums left over from the column's top layer, which
 // k_tvd_coef_cell computed once per column (w.htop) in the same order.
-__global__ void k_tvd_lim_cell(DevMesh m, DevState s, DevWork w, DevDims d, int hlim) {
+__global__ void k_tvd_lim_cell(DevMesh m, DevState s, DevWork w, DevDims d, int hlim, int first) {
   long q = (long)blockIdx.x * blockDim.x + threadIdx.x;
   const int E = d.E, L = d.L, L1 = d.L + 1, L2 = d.L + 2;
   if (q >= (long)L1 * E) return;
   int e = (int)(q % E), k = (int)(q / E);
   int tl = s.top_layer_at_element[e], bl = m.bottom_layer_at_element[e];
-  if (tl == 0 || k < bl - 1 || k > tl) {
-    for (int t = 0; t < d.NT; t++) w.PSIi[zix(t, k, e, L1, E)] = 0.0;
+  if (tl == 0 || k < bl - 1 || k > tl) {  // zero from the first sub-cycle on
+    if (first)
+      for (int t = 0; t < d.NT; t++) w.PSIi[zix(t, k, e, L1, E)] = 0.0;
     return;
   }
   int nv = tl - bl + 1;
@@ -229,7 +239,8 @@ __global__ void k_tvd_lim_cell(DevMesh m, DevState s, DevWork w, DevDims d, int 
   }
   // r_ik of level k: computed for bl <= k <= tl - 1 (1.0 at bl - 1 and tl, where PSI_ik is 0)
   if (k == tl || k == bl - 1) {
-    for (int t = 0; t < d.NT; t++) w.PSIi[zix(t, k, e, L1, E)] = 0.0;
+    if (first)
+      for (int t = 0; t < d.NT; t++) w.PSIi[zix(t, k, e, L1, E)] = 0.0;
     return;
   }
   double sum1[2], sum2[2];
@@ -262,11 +273,17 @@ __global__ void k_tvd_lim_cell(DevMesh m, DevState s, DevWork w, DevDims d, int 
 // Face ratio r_jk and PSI_jk. In the reference each adjacent cell whose outflow the face
 // carries (Q2 >= 0 from its side) writes r_jk in cell order, so the higher cell id wins
 // a tie (Q = 0) -- the same writer is chosen here.
-__global__ void k_tvd_lim_face(DevMesh m, DevState s, DevWork w, DevDims d, int hlim) {
+__global__ void k_tvd_lim_face(DevMesh m, DevState s, DevWork w, DevDims d, int hlim, int first) {
   int q = blockIdx.x * blockDim.x + threadIdx.x;
   const int J = d.J, E = d.E, L = d.L, L2 = d.L + 2;
   if (q >= J * L) return;
   int j = q % J, k = q / J + 1;
+  const bool in = (k >= m.bottom_layer_at_face[j] && k <= s.top_layer_at_face[j]);
+  if (!in || hlim == 0) {  // PSI_jk is zero there; written once, in the first sub-cycle
+    if (first)
+      for (int t = 0; t < d.NT; t++) w.PSIj[nix(t, k, j, L, J)] = 0.0;
+    return;
+  }
   double r[2] = {1.0, 1.0};
   if (face_active(m, j)) {
     double Q = AL(w.Qj, k, j, J);
@@ -287,9 +304,7 @@ __global__ void k_tvd_lim_face(DevMesh m, DevState s, DevWork w, DevDims d, int 
       }
     }
   }
-  bool in = (k >= m.bottom_layer_at_face[j] && k <= s.top_layer_at_face[j]);
-  for (int t = 0; t < d.NT; t++)
-    w.PSIj[nix(t, k, j, L, J)] = (in && hlim != 0) ? limiter(hlim, r[t], AL(w.phij, k, j, J)) : 0.0;
+  for (int t = 0; t < d.NT; t++) w.PSIj[nix(t, k, j, L, J)] = limiter(hlim, r[t], AL(w.phij, k, j, J));
 }
 
 // Per-column implicit update (solve_transport_equation_v19.f90:921-1303), in two passes.
@@ -472,14 +487,17 @@ __global__ void k_tvd_solve(DevMesh m, DevState s, DevWork w, DevDims d, TvdArgs
 }
 
 // con <- con_new in the active range, zero elsewhere (solve_transport_equation_v19.f90:1352-1371)
-__global__ void k_tvd_copyback(DevMesh m, DevState s, DevWork w, DevDims d) {
+// (out of the range the zeros written after the first sub-cycle stay)
+__global__ void k_tvd_copyback(DevMesh m, DevState s, DevWork w, DevDims d, int first) {
   int q = blockIdx.x * blockDim.x + threadIdx.x;
   const int E = d.E, L = d.L, L2 = d.L + 2;
   if (q >= E * L2) return;
   int e = q % E, k = q / E;
   int bl = m.bottom_layer_at_element[e], tl = s.top_layer_at_element[e];
+  const bool in = k >= bl && k <= tl;
+  if (!in && !first) return;
   for (int t = 0; t < d.NT; t++)
-    w.con[cix(t, k, e, L2, E)] = (k >= bl && k <= tl) ? w.con_new[nix(t, k, e, L, E)] : 0.0;
+    w.con[cix(t, k, e, L2, E)] = in ? w.con_new[nix(t, k, e, L, E)] : 0.0;
 }
 
 // Node values: area*dz weighted cell means, top down (solve_transport_equation_v19.f90:1403-1453)
