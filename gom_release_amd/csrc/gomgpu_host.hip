@@ -843,7 +843,7 @@ static int stage_impl(gomgpu_ctx *c, int id) {
       break;
     case GOMGPU_STAGE_NODEVEL:
       k_face_levels<<<nblk(d.J, BS), BS, 0, st>>>(c->m, c->s, d);
-      k_node_vel<<<nblk((long)d.P * (d.L + 1), LB(NODE_VEL)), LB(NODE_VEL), 0, st>>>(c->m, c->s, d);
+      k_node_vel<<<nblk((long)d.P * ((d.L + GOM_NV_LEVELS) / GOM_NV_LEVELS), LB(NODE_VEL)), LB(NODE_VEL), 0, st>>>(c->m, c->s, d);
       break;
     case GOMGPU_STAGE_TRANSPORT: {
       if (p.transport_flag != 1 || d.NT == 0) break;

@@ -1221,27 +1221,46 @@ __global__ void k_face_levels(DevMesh m, DevState s, DevDims d) {
 // (level, node). The node's faces and cells come from the static stencil tables (one
 // coalesced load each instead of the reference's cell -> face lookups), visited in the
 // reference's order; the weight sum is precomputed in that order.
-#define NV_CHUNK 4
+#define NV_CHUNK 2
+#define NV_LEVELS GOM_NV_LEVELS
 __global__ void k_node_vel(DevMesh m, DevState s, DevDims d) {
+  // NV_LEVELS consecutive levels of one node per thread: the node's stencil entries are
+  // loaded once per chunk of adjacent cells and serve every level of the thread
+  const int P = d.P, E = d.E, J = d.J, L1 = d.L + 1;
+  const int nlc = (L1 + NV_LEVELS - 1) / NV_LEVELS;
   int t = blockIdx.x * blockDim.x + threadIdx.x;
-  const int P = d.P, E = d.E, J = d.J;
-  if (t >= P * (d.L + 1)) return;
-  int i = t % P, k = t / P;
-  int nadj = m.adj_cells_at_node[i];
-  double u = 0.0, v = 0.0;
-  int icount = 0;
-  double w = 0.0, tw = 0.0;
-  // the adjacent cells in chunks of NV_CHUNK: each chunk's stencil entries, then the wet
-  // tops they index, then the level values are loaded as three batches before the sums,
-  // which run in the reference's order (slots beyond nadj are not added)
+  if (t >= P * nlc) return;
+  const int i = t % P, k0 = (t / P) * NV_LEVELS;
+  const int nlev = L1 - k0 < NV_LEVELS ? L1 - k0 : NV_LEVELS;
+  const int nadj = m.adj_cells_at_node[i];
+  double u[NV_LEVELS], v[NV_LEVELS], w[NV_LEVELS], tw[NV_LEVELS];
+  int icount[NV_LEVELS];
+#pragma unroll
+  for (int z = 0; z < NV_LEVELS; z++) {
+    u[z] = v[z] = w[z] = tw[z] = 0.0;
+    icount[z] = 0;
+  }
+  // the adjacent cells in chunks of NV_CHUNK: each chunk's stencil entries and the wet tops
+  // they index are loaded once, then per level the face and cell values, and the sums run
+  // in the reference's order (slots beyond nadj are not added)
   for (int j0 = 0; j0 < nadj; j0 += NV_CHUNK) {
-    int ifn[2 * NV_CHUNK], isn[NV_CHUNK], ftop[2 * NV_CHUNK], ctop[NV_CHUNK];
+    int ifn[2 * NV_CHUNK], isn[NV_CHUNK], ftop[2 * NV_CHUNK], ctop[NV_CHUNK], fkb[2 * NV_CHUNK], ckb[NV_CHUNK];
+    double lf[2 * NV_CHUNK], fw[2 * NV_CHUNK], car[NV_CHUNK];
 #pragma unroll
     for (int q = 0; q < NV_CHUNK; q++) {
       const int jj = j0 + q < nadj ? j0 + q : j0;
-      ifn[2 * q] = m.ns_face[(size_t)(2 * jj) * P + i];
-      ifn[2 * q + 1] = m.ns_face[(size_t)(2 * jj + 1) * P + i];
-      isn[q] = m.ns_cell[(size_t)jj * P + i];
+#pragma unroll
+      for (int h = 0; h < 2; h++) {
+        const size_t o = (size_t)(2 * jj + h) * P + i;
+        ifn[2 * q + h] = m.ns_face[o];
+        fkb[2 * q + h] = m.ns_fkb[o];
+        lf[2 * q + h] = m.ns_Lf[o];
+        fw[2 * q + h] = m.ns_fw[o];
+      }
+      const size_t oc = (size_t)jj * P + i;
+      isn[q] = m.ns_cell[oc];
+      ckb[q] = m.ns_ckb[oc];
+      car[q] = m.ns_area[oc];
     }
 #pragma unroll
     for (int q = 0; q < NV_CHUNK; q++) {
@@ -1249,49 +1268,55 @@ __global__ void k_node_vel(DevMesh m, DevState s, DevDims d) {
       ftop[2 * q + 1] = s.top_layer_at_face[ifn[2 * q + 1]];
       ctop[q] = s.top_layer_at_element[isn[q]];
     }
-    double fu[2 * NV_CHUNK], fv[2 * NV_CHUNK], cw[NV_CHUNK];
 #pragma unroll
-    for (int f = 0; f < 2 * NV_CHUNK; f++) {
-      const int kvt = k < ftop[f] ? k : ftop[f];
-      fu[f] = A0(s.u_face_level, kvt, ifn[f], J);
-      fv[f] = A0(s.v_face_level, kvt, ifn[f], J);
-    }
+    for (int z = 0; z < NV_LEVELS; z++) {
+      const int k = k0 + (z < nlev ? z : 0);   // levels past the top of the range repeat k0
+      double fu[2 * NV_CHUNK], fv[2 * NV_CHUNK], cw[NV_CHUNK];
 #pragma unroll
-    for (int q = 0; q < NV_CHUNK; q++) {
-      const int kvt = k < ctop[q] ? k : ctop[q];
-      cw[q] = A0(s.wn_cell, kvt, isn[q], E);
-    }
+      for (int f = 0; f < 2 * NV_CHUNK; f++) {
+        const int kvt = k < ftop[f] ? k : ftop[f];
+        fu[f] = A0(s.u_face_level, kvt, ifn[f], J);
+        fv[f] = A0(s.v_face_level, kvt, ifn[f], J);
+      }
 #pragma unroll
-    for (int q = 0; q < NV_CHUNK; q++) {
-      const int jj = j0 + q;
-      if (jj >= nadj) continue;
+      for (int q = 0; q < NV_CHUNK; q++) {
+        const int kvt = k < ctop[q] ? k : ctop[q];
+        cw[q] = A0(s.wn_cell, kvt, isn[q], E);
+      }
 #pragma unroll
-      for (int h = 0; h < 2; h++) {
-        size_t o = (size_t)(2 * jj + h) * P + i;
-        if (ftop[2 * q + h] != 0 && k >= m.ns_fkb[o] - 1) {
-          double Lf = m.ns_Lf[o], fw = m.ns_fw[o];
-          u = u + fu[2 * q + h] / Lf * fw;
-          v = v + fv[2 * q + h] / Lf * fw;
-          icount++;
+      for (int q = 0; q < NV_CHUNK; q++) {
+        if (j0 + q >= nadj) continue;
+#pragma unroll
+        for (int h = 0; h < 2; h++) {
+          const int f = 2 * q + h;
+          if (ftop[f] != 0 && k >= fkb[f] - 1) {
+            u[z] = u[z] + fu[f] / lf[f] * fw[f];
+            v[z] = v[z] + fv[f] / lf[f] * fw[f];
+            icount[z]++;
+          }
+        }
+        if (ctop[q] != 0 && k >= ckb[q] - 1) {
+          tw[z] = tw[z] + car[q];
+          w[z] = w[z] + cw[q] * car[q];
         }
       }
-      size_t oc = (size_t)jj * P + i;
-      if (ctop[q] != 0 && k >= m.ns_ckb[oc] - 1) {
-        double ar = m.ns_area[oc];
-        tw = tw + ar;
-        w = w + cw[q] * ar;
-      }
     }
   }
-  if (icount != 0) {
-    double weight = m.ns_weight[i];
-    u = u / weight;
-    v = v / weight;
+  const double weight = m.ns_weight[i];
+#pragma unroll
+  for (int z = 0; z < NV_LEVELS; z++) {
+    if (z >= nlev) continue;
+    const int k = k0 + z;
+    double uu = u[z], vv = v[z], ww = w[z];
+    if (icount[z] != 0) {
+      uu = uu / weight;
+      vv = vv / weight;
+    }
+    if (tw[z] != 0.0) ww = ww / tw[z];
+    A0(s.u_node, k, i, P) = uu;
+    A0(s.v_node, k, i, P) = vv;
+    A0(s.w_node, k, i, P) = ww;
   }
-  if (tw != 0.0) w = w / tw;
-  A0(s.u_node, k, i, P) = u;
-  A0(s.v_node, k, i, P) = v;
-  A0(s.w_node, k, i, P) = w;
 }
 
 // River face velocity from the discharge (compute_boundary_velocity.f90:32-46):
