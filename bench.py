@@ -243,7 +243,7 @@ def main() -> int:
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build measures one GPU only (got --gpus {args.gpus}); "
-              "multi-GPU sharding is described in DESIGN.md but not run.", file=sys.stderr)
+              "the step would shard by domain decomposition, which is not built.", file=sys.stderr)
         return 2
     from gom_release_amd import host
     from gom_release_amd.driver import Scenario
