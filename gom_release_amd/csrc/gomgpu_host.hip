@@ -878,7 +878,8 @@ static int stage_impl(gomgpu_ctx *c, int id) {
         KT_END(KT_TVD_COLUMN);
         k_tvd_copyback<<<nblk(nE2, BS), BS, 0, st>>>(c->m, c->s, c->w, d, sub == 0);
       }
-      k_tvd_node<<<nblk(d.P, BS), BS, 0, st>>>(c->m, c->s, c->w, d);
+      k_tvd_node<<<nblk((long)d.P * ((d.L + GOM_TVD_NODE_LEVELS - 1) / GOM_TVD_NODE_LEVELS), BS), BS, 0, st>>>(
+          c->m, c->s, c->w, d);
       k_tvd_face_out<<<nblk((long)d.J * d.L, BS), BS, 0, st>>>(c->m, c->s, c->w, d);
       long nmax = (long)d.L * (d.E > d.P ? d.E : d.P);
       k_tvd_cellnode_out<<<nblk(nmax, BS), BS, 0, st>>>(c->s, c->w, d);
@@ -994,7 +995,7 @@ static int stage_impl(gomgpu_ctx *c, int id) {
       } else {
         k_turb_ri<<<nblk((long)d.L * d.J, BS), BS, 0, st>>>(c->m, c->s, c->w, d, tp);
       }
-      k_turb_kv<<<nblk((long)(d.L + 1) * d.E, BS), BS, 0, st>>>(c->m, c->s, c->w, d);
+      k_turb_kv<<<nblk((long)((d.L + GOM_KV_LEVELS) / GOM_KV_LEVELS) * d.E, BS), BS, 0, st>>>(c->m, c->s, c->w, d);
       break;
     }
     default:

@@ -500,32 +500,59 @@ __global__ void k_tvd_copyback(DevMesh m, DevState s, DevWork w, DevDims d, int 
     w.con[cix(t, k, e, L2, E)] = in ? w.con_new[nix(t, k, e, L, E)] : 0.0;
 }
 
-// Node values: area*dz weighted cell means, top down (solve_transport_equation_v19.f90:1403-1453)
-__global__ void k_tvd_node(DevMesh m, DevState s, DevWork w, DevDims d) {
-  int i = blockIdx.x * blockDim.x + threadIdx.x;
+// Node values: area*dz weighted cell means, top down (solve_transport_equation_v19.f90:1403-1453).
+// The mean of one level; `none` is set when no adjacent cell is wet at that level.
+__device__ __forceinline__ double tvd_node_mean(const DevMesh &m, const DevState &s, const DevWork &w,
+                                                const DevDims &d, int i, int k, int t, bool &none) {
   const int P = d.P, E = d.E, L = d.L;
-  if (i >= P) return;
-  int tn = s.top_layer_at_node[i], bn = m.bottom_layer_at_node[i];
-  int nadj = m.adj_cells_at_node[i];
+  const int nadj = m.adj_cells_at_node[i];
+  double s11 = 0.0, s22 = 0.0;
+  int count1 = 0;
+  for (int l = 1; l <= nadj; l++) {
+    int ii = TB(m.adj_cellnum_at_node, l, i, P) - 1;
+    if (k >= m.bottom_layer_at_element[ii] && k <= s.top_layer_at_element[ii]) {
+      double wgt = m.area[ii] * AL(s.dz_cell_new, k, ii, E);
+      s11 = s11 + wgt;
+      s22 = s22 + wgt * w.con_new[nix(t, k, ii, L, E)];
+    } else {
+      count1++;
+    }
+  }
+  none = count1 == nadj;
+  return (s11 == 0.0) ? 0.0 : s22 / s11;
+}
+
+// One thread per (node, chunk of TVD_NODE_LEVELS levels), the chunk's levels top down. A
+// level whose adjacent cells are all dry takes the value of the level above it (unless it
+// is the node's top level); at the top of a chunk that value is found by walking up.
+#define TVD_NODE_LEVELS GOM_TVD_NODE_LEVELS
+__global__ void k_tvd_node(DevMesh m, DevState s, DevWork w, DevDims d) {
+  const int P = d.P, L = d.L;
+  const int nch = (L + TVD_NODE_LEVELS - 1) / TVD_NODE_LEVELS;
+  long q = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= (long)P * nch) return;
+  const int i = (int)(q % P), c = (int)(q / P);
+  const int khi = L - c * TVD_NODE_LEVELS;                  // the chunk's top level
+  const int klo = khi - TVD_NODE_LEVELS + 1 > 1 ? khi - TVD_NODE_LEVELS + 1 : 1;
+  const int tn = s.top_layer_at_node[i], bn = m.bottom_layer_at_node[i];
   for (int t = 0; t < d.NT; t++) {
     double above = 0.0;
-    for (int k = L; k >= 1; k--) {
+    if (khi < tn && khi >= bn) {  // the value of level khi + 1, as the loop above would leave it
+      int ku = khi + 1;
+      bool none;
+      double v = tvd_node_mean(m, s, w, d, i, ku, t, none);
+      while (none && ku != tn) {
+        ku++;
+        v = tvd_node_mean(m, s, w, d, i, ku, t, none);
+      }
+      above = v;
+    }
+    for (int k = khi; k >= klo; k--) {
       double v = 0.0;
       if (k <= tn && k >= bn) {
-        double s11 = 0.0, s22 = 0.0;
-        int count1 = 0;
-        for (int l = 1; l <= nadj; l++) {
-          int ii = TB(m.adj_cellnum_at_node, l, i, P) - 1;
-          if (k >= m.bottom_layer_at_element[ii] && k <= s.top_layer_at_element[ii]) {
-            double wgt = m.area[ii] * AL(s.dz_cell_new, k, ii, E);
-            s11 = s11 + wgt;
-            s22 = s22 + wgt * w.con_new[nix(t, k, ii, L, E)];
-          } else {
-            count1++;
-          }
-        }
-        v = (s11 == 0.0) ? 0.0 : s22 / s11;
-        if (count1 == nadj && k != tn) v = above;
+        bool none;
+        v = tvd_node_mean(m, s, w, d, i, k, t, none);
+        if (none && k != tn) v = above;
       }
       w.node_new[nix(t, k, i, L, P)] = v;
       above = v;

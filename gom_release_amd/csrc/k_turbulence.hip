@@ -430,22 +430,35 @@ __global__ void k_turb_post(DevMesh m, DevState s, DevWork w, DevDims d, TurbPar
 // Face diffusivity -> cell Kv: mean over the cell's wet faces at the level clamped to each
 // face's active range (1.0e-2 for a cell with no wet face); one thread per (level, cell).
 __global__ void k_turb_kv(DevMesh m, DevState s, DevWork w, DevDims d) {
+  // GOM_KV_LEVELS consecutive levels of one cell per thread: the cell's faces and their wet
+  // ranges are read once for all of them
+  const int E = d.E, J = d.J, L1 = d.L + 1;
+  const int nch = (L1 + GOM_KV_LEVELS - 1) / GOM_KV_LEVELS;
   long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
-  const int E = d.E, J = d.J, L = d.L;
-  if (t >= (long)(L + 1) * E) return;
-  int i = (int)(t % E), k = (int)(t / E);
+  if (t >= (long)nch * E) return;
+  const int i = (int)(t % E), k0 = (int)(t / E) * GOM_KV_LEVELS;
   if (s.top_layer_at_element[i] == 0) return;
-  int N = m.tri_or_quad[i];
-  double sum = 0.0;
-  int cnt = 0;
-  for (int l = 1; l <= N; l++) {
-    int f = TB(m.facenum_at_cell, l, i, E) - 1;
-    int kt = s.top_layer_at_face[f];
-    if (kt == 0) continue;
-    int kb = m.bottom_layer_at_face[f];
-    int kk = k < kb ? kb : (k > kt ? kt : k);
-    sum = sum + A0(w.turb_kvf, kk, f, J);
-    cnt++;
+  const int N = m.tri_or_quad[i];
+  int fc[4], fkt[4], fkb[4];
+#pragma unroll
+  for (int l = 0; l < 4; l++) {
+    fc[l] = l < N ? TB(m.facenum_at_cell, l + 1, i, E) - 1 : 0;
+    fkt[l] = l < N ? s.top_layer_at_face[fc[l]] : 0;
+    fkb[l] = l < N ? m.bottom_layer_at_face[fc[l]] : 1;
   }
-  A0(s.Kv, k, i, E) = cnt == 0 ? (double)1.0e-2f : sum / cnt;
+#pragma unroll
+  for (int z = 0; z < GOM_KV_LEVELS; z++) {
+    const int k = k0 + z;
+    if (k >= L1) continue;
+    double sum = 0.0;
+    int cnt = 0;
+#pragma unroll
+    for (int l = 0; l < 4; l++) {
+      if (l >= N || fkt[l] == 0) continue;
+      const int kk = k < fkb[l] ? fkb[l] : (k > fkt[l] ? fkt[l] : k);
+      sum = sum + A0(w.turb_kvf, kk, fc[l], J);
+      cnt++;
+    }
+    A0(s.Kv, k, i, E) = cnt == 0 ? (double)1.0e-2f : sum / cnt;
+  }
 }
