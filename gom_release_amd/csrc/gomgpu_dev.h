@@ -18,6 +18,7 @@
 #define GOM_NV_LEVELS 3  // node levels per thread of k_node_vel
 #define GOM_TVD_NODE_LEVELS 4  // node levels per thread of k_tvd_node
 #define GOM_KV_LEVELS 3  // cell levels per thread of k_turb_kv
+#define GOM_CUV_LAYERS 4  // cell layers per thread of k_cell_uv
 #define PCG_MAX_SLOT_BLOCKS 256  // most blocks of one system in a slot-exchange PCG launch
 
 struct DevMesh {

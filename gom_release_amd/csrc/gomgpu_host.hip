@@ -813,7 +813,8 @@ static int stage_impl(gomgpu_ctx *c, int id) {
     case GOMGPU_STAGE_MOMENTUM: {
       MomParams q{p.dt, p.gravity, p.theta, p.rho_o, p.smagorinsky_parameter, p.Ah_0, p.Kh_0, p.baroclinic_flag};
       KT_BEGIN(KT_MOMENTUM);
-      k_cell_uv<<<nblk((long)d.L * d.E, BS), BS, 0, st>>>(c->m, c->s, c->w, d);
+      k_cell_uv<<<nblk((long)((d.L + GOM_CUV_LAYERS - 1) / GOM_CUV_LAYERS) * d.E, BS), BS, 0, st>>>(c->m, c->s, c->w,
+                                                                                                 d);
       // 64 faces per block, all their rows; LDS for the baroclinic products of every layer
       k_mom_rhs<<<nblk(d.J, 64), LB(MOM_RHS), (size_t)2 * d.L * 64 * sizeof(double), st>>>(c->m, c->s, c->f, c->w,
                                                                                           d, q);

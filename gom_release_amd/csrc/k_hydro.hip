@@ -479,19 +479,41 @@ __global__ void k_elm_copy(DevState s, long n) {
 // stencil, solve_momentum_equation.f90:280-300), in the reference's node order; one thread
 // per (layer, cell).
 __global__ void k_cell_uv(DevMesh m, DevState s, DevWork w, DevDims d) {
+  // GOM_CUV_LAYERS consecutive layers of one cell per thread: the node ids are read once and
+  // each node level serves the two layers it bounds
+  const int E = d.E, P = d.P, L = d.L;
+  const int nch = (L + GOM_CUV_LAYERS - 1) / GOM_CUV_LAYERS;
   long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
-  const int E = d.E, P = d.P;
-  if (t >= (long)d.L * E) return;
-  int ie = (int)(t % E), kk = (int)(t / E) + 1;
-  int N = m.tri_or_quad[ie];
-  double ut = 0.0, vt = 0.0;
-  for (int ll = 1; ll <= N; ll++) {
-    int nd = TB(m.nodenum_at_cell, ll, ie, E) - 1;
-    ut = ut + (A0(s.u_node, kk, nd, P) + A0(s.u_node, kk - 1, nd, P)) * 0.5 / N;
-    vt = vt + (A0(s.v_node, kk, nd, P) + A0(s.v_node, kk - 1, nd, P)) * 0.5 / N;
+  if (t >= (long)nch * E) return;
+  const int ie = (int)(t % E), k1 = (int)(t / E) * GOM_CUV_LAYERS + 1;
+  const int N = m.tri_or_quad[ie];
+  int nd[4];
+#pragma unroll
+  for (int ll = 0; ll < 4; ll++) nd[ll] = ll < N ? TB(m.nodenum_at_cell, ll + 1, ie, E) - 1 : nd[0];
+  double ul[4], vl[4];  // the nodes' values at the level below the current layer
+#pragma unroll
+  for (int ll = 0; ll < 4; ll++) {
+    ul[ll] = A0(s.u_node, k1 - 1, nd[ll], P);
+    vl[ll] = A0(s.v_node, k1 - 1, nd[ll], P);
   }
-  AL(w.cell_u, kk, ie, E) = ut;
-  AL(w.cell_v, kk, ie, E) = vt;
+#pragma unroll
+  for (int z = 0; z < GOM_CUV_LAYERS; z++) {
+    const int kk = k1 + z;
+    if (kk > L) continue;
+    double ut = 0.0, vt = 0.0;
+#pragma unroll
+    for (int ll = 0; ll < 4; ll++) {
+      const double uu = A0(s.u_node, kk, nd[ll], P), vv = A0(s.v_node, kk, nd[ll], P);
+      if (ll < N) {
+        ut = ut + (uu + ul[ll]) * 0.5 / N;
+        vt = vt + (vv + vl[ll]) * 0.5 / N;
+      }
+      ul[ll] = uu;
+      vl[ll] = vv;
+    }
+    AL(w.cell_u, kk, ie, E) = ut;
+    AL(w.cell_v, kk, ie, E) = vt;
+  }
 }
 
 // One block covers 64 consecutive faces (lane = face) and all their rows: wave v of the block
