@@ -98,8 +98,7 @@ struct DevWork {
   double *cell_u, *cell_v;  // (L, E) cell means of the mid-layer node velocities (momentum)
   // two-equation closure (allocated when turbulence_flag == 3): level diffusivities of the
   // current state (0:L, J) and layer quantities / solutions (L, J)
-  double *tk_av, *tk_kv, *tk_qd, *tk_qd2;
-  double *tk_q2bt, *tk_xlbt, *tk_shear, *tk_rz, *tk_q2tmp, *tk_x;
+  double *tk_shear, *tk_rz, *tk_q2tmp, *tk_x;
   // budgets
   double *budget;         // [0] volume, [1] salt
   int *err;               // [0] code, [1] entity

@@ -51,9 +51,7 @@ __global__ void k_eos_nodeface(DevMesh, DevState, DevDims, double, int, int, int
 __global__ void k_eos_cell(DevMesh, DevState, DevDims, int *);
 __global__ void k_bars(DevMesh, DevState, DevDims);
 __global__ void k_turb_ri(DevMesh, DevState, DevWork, DevDims, TurbParams);
-__global__ void k_turb_prep(DevMesh, DevState, DevWork, DevDims, TurbParams, int *);
-__global__ void __launch_bounds__(256) k_turb_solve(DevMesh, DevState, DevForcing, DevWork, DevDims, TurbParams, int *);
-__global__ void k_turb_post(DevMesh, DevState, DevWork, DevDims, TurbParams);
+__global__ void __launch_bounds__(256) k_turb_column(DevMesh, DevState, DevForcing, DevWork, DevDims, TurbParams, int *);
 __global__ void k_turb_kv(DevMesh, DevState, DevWork, DevDims);
 __global__ void k_budget(DevMesh, DevState, DevDims, double *);
 __global__ void k_copy(double *, const double *, int);
@@ -473,10 +471,7 @@ extern "C" int gomgpu_create(const gomgpu_mesh *mesh, const gomgpu_params *param
         (rc = dalloc(c, &w.cell_u, L * E)) || (rc = dalloc(c, &w.cell_v, L * E)))
       goto fail;
     if (p.turbulence_flag == 3 &&
-        ((rc = dalloc(c, &w.tk_av, (L + 1) * J)) || (rc = dalloc(c, &w.tk_kv, (L + 1) * J)) ||
-         (rc = dalloc(c, &w.tk_qd, (L + 1) * J)) || (rc = dalloc(c, &w.tk_qd2, (L + 1) * J)) ||
-         (rc = dalloc(c, &w.tk_q2bt, L * J)) || (rc = dalloc(c, &w.tk_xlbt, L * J)) ||
-         (rc = dalloc(c, &w.tk_shear, L * J)) || (rc = dalloc(c, &w.tk_rz, L * J)) ||
+        ((rc = dalloc(c, &w.tk_shear, L * J)) || (rc = dalloc(c, &w.tk_rz, L * J)) ||
          (rc = dalloc(c, &w.tk_q2tmp, L * J)) || (rc = dalloc(c, &w.tk_x, L * J))))
       goto fail;
   }
@@ -989,12 +984,9 @@ static int stage_impl(gomgpu_ctx *c, int id) {
       tp.model = p.turbulence_model;
       tp.stab = p.stability_function;
       if (tp.flag == 3) {
-        long nlev = (long)(d.L + 1) * d.J;
-        k_turb_prep<<<nblk(nlev, LB(TURB_PREP)), LB(TURB_PREP), 0, st>>>(c->m, c->s, c->w, d, tp, err);
-        k_turb_solve<<<nblk(d.J, LB(TURB_SOLVE)), LB(TURB_SOLVE), 0, st>>>(c->m, c->s, c->f, c->w, d, tp, err);
-        k_turb_post<<<nblk(nlev, LB(TURB_PREP)), LB(TURB_PREP), 0, st>>>(c->m, c->s, c->w, d, tp);
+        k_turb_column<<<nblk(d.J, LB(TURB_SOLVE)), LB(TURB_SOLVE), 0, st>>>(c->m, c->s, c->f, c->w, d, tp, err);
       } else {
-        k_turb_ri<<<nblk((long)d.L * d.J, BS), BS, 0, st>>>(c->m, c->s, c->w, d, tp);
+        k_turb_ri<<<nblk((long)d.L * d.J, LB(TURB_PREP)), LB(TURB_PREP), 0, st>>>(c->m, c->s, c->w, d, tp);
       }
       k_turb_kv<<<nblk((long)((d.L + GOM_KV_LEVELS) / GOM_KV_LEVELS) * d.E, BS), BS, 0, st>>>(c->m, c->s, c->w, d);
       break;
@@ -1457,10 +1449,8 @@ extern "C" long long gomgpu_footprint(const gomgpu_params *params) {
   add(2 * ((E + 255) / 256) + 2, D); add(2, sizeof(int));   // budget, err
   add((L + 1) * J, D); add(L * (E > J ? E : J), D); add(NT * L * E, D);  // turb_kvf, tri_gam, tvd_x
   add(L * E, D); add(L * E, D);                                           // cell_u, cell_v
-  if (p.turbulence_flag == 3) {
-    for (int i = 0; i < 4; i++) add((L + 1) * J, D);
-    for (int i = 0; i < 6; i++) add(L * J, D);
-  }
+  if (p.turbulence_flag == 3)
+    for (int i = 0; i < 4; i++) add(L * J, D);  // tk_shear, tk_rz, tk_q2tmp, tk_x
   return (long long)tot;
 }
 

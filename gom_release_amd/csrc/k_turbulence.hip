@@ -2,10 +2,10 @@
 //
 // Scheme 1 (Richardson step) and 2 (Pacanowski-Philander) are per-level algebra. Scheme 3
 // solves the k (q2) and psi equations of the generic length scale closure as two
-// tridiagonal systems per face column, in three passes: per (level, face) the stability
-// functions of the current state and the layer quantities; per face the two Thomas
-// recurrences of tridiagonal_solver.f90 in global work arrays, the length scale and the
-// conversion to whole levels; per (level, face) the new diffusivities. The stability
+// tridiagonal systems per face column, one thread per column: the stability functions of
+// the current state and the layer quantities as the elimination reaches each row, the two
+// Thomas recurrences of tridiagonal_solver.f90, the length scale, the conversion to whole
+// levels and the new diffusivities. The stability
 // functions are the quasi-equilibrium ones of Galperin et al. (1988) or Kantha & Clayson
 // (1994) (the reference's algebraic_stress is undefined). The face diffusivity is then
 // averaged onto cells for the transport (Kv). Each value is computed by the same expression
@@ -133,70 +133,25 @@ __global__ void k_turb_ri(DevMesh m, DevState s, DevWork w, DevDims d, TurbParam
   A0(w.turb_kvf, k, j, J) = kv;
 }
 
-// Scheme 3, pass 1 (turbulence_closure_v1.f90:56-66, 158-215), one thread per (level, face):
-// the diffusivities of the current k, l at whole levels kb..kt (level kb-1 takes level
-// kb's), and at layers kb..kt the half-level k, l, the squared shear, (g/rho_o) drho/dz and
-// the two stop checks.
-__global__ void k_turb_prep(DevMesh m, DevState s, DevWork w, DevDims d, TurbParams p, int *err) {
-  long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
-  const int J = d.J, L = d.L, P = d.P;
-  if (t >= (long)(L + 1) * J) return;
-  int j = (int)(t % J), k = (int)(t / J);
-  int kt = s.top_layer_at_face[j];
-  if (kt == 0) return;
-  int kb = m.bottom_layer_at_face[j];
-  if (k < kb - 1 || k > kt) return;
-  const double g = p.g;
-  {
-    int kl = k == kb - 1 ? kb : k;
-    double vd, td, qd, qd2;
-    alg_stress(p, A0(s.q2, kl, j, J), A0(s.xl, kl, j, J), rz_level(p, s, J, j, kl, kt), vd, td, qd, qd2);
-    A0(w.tk_av, k, j, J) = vd;
-    A0(w.tk_kv, k, j, J) = td;
-    A0(w.tk_qd, k, j, J) = qd;
-    A0(w.tk_qd2, k, j, J) = qd2;
-  }
-  if (k < kb) return;
-  double q2bt = (A0(s.q2, k - 1, j, J) + A0(s.q2, k, j, J)) / 2.0;
-  double xlbt = (A0(s.xl, k - 1, j, J) + A0(s.xl, k, j, J)) / 2.0;
-  if (xlbt < p.xlmin2) latch(err, GOMGPU_ERR_TURB_XL, j + 1);
-  int n1 = TB(m.nodenum_at_face, 1, j, J) - 1, n2 = TB(m.nodenum_at_face, 2, j, J) - 1;
-  double udown = (A0(s.u_node, k - 1, n1, P) + A0(s.u_node, k - 1, n2, P)) / 2.0;
-  double uup = (A0(s.u_node, k, n1, P) + A0(s.u_node, k, n2, P)) / 2.0;
-  double vdown = (A0(s.v_node, k - 1, n1, P) + A0(s.v_node, k - 1, n2, P)) / 2.0;
-  double vup = (A0(s.v_node, k, n1, P) + A0(s.v_node, k, n2, P)) / 2.0;
-  double dzk = AL(s.dz_face_new, k, j, J);
-  double dudz = (uup - udown) / dzk, dvdz = (vup - vdown) / dzk;
-  double shear = dudz * dudz + dvdz * dvdz;
-  double drhodz;
-  if (k == kt || k == kb) {
-    drhodz = 0.0;
-  } else {
-    double rho_up = AL(s.rho_face, k, j, J) + dzk / 2.0 / A0(s.dzhalf_face_new, k, j, J) *
-                                                  (AL(s.rho_face, k + 1, j, J) - AL(s.rho_face, k, j, J));
-    double rho_down = AL(s.rho_face, k - 1, j, J) + AL(s.dz_face_new, k - 1, j, J) / 2.0 /
-                                                        A0(s.dzhalf_face_new, k - 1, j, J) *
-                                                        (AL(s.rho_face, k, j, J) - AL(s.rho_face, k - 1, j, J));
-    drhodz = (rho_up - rho_down) / dzk;
-  }
-  AL(w.tk_q2bt, k, j, J) = q2bt;
-  AL(w.tk_xlbt, k, j, J) = xlbt;
-  AL(w.tk_shear, k, j, J) = shear;
-  AL(w.tk_rz, k, j, J) = g / p.rho_o * drhodz;
-  int a1 = TB(m.adj_cellnum_at_face, 1, j, J), a2 = TB(m.adj_cellnum_at_face, 2, j, J);
-  double etam = a2 != 0 ? dmaxf(s.eta_cell_new[a1 - 1], s.eta_cell_new[a2 - 1]) : s.eta_cell_new[a1 - 1];
-  double hf = m.h_face[j];
-  double zctr = (k == kt) ? p.MSL + etam - dzk / 2.0 : m.z_level[k] - dzk / 2.0;
-  double dists = p.MSL + etam - zctr, distb = zctr - (p.MSL - hf);
-  double xlmax = (double)0.4f * dists * distb / (hf + etam);
-  if (xlmax <= 0.0) latch(err, GOMGPU_ERR_TURB_DIST, j + 1);
+// The stability-function diffusivities of one whole level (turbulence_closure_v1.f90:56-66):
+// level kb-1 takes level kb's state.
+__device__ __forceinline__ void level_diff(const TurbParams &p, const DevState &s, int J, int j, int l, int kb,
+                                           int kt, double &av, double &kv, double &qd, double &qd2) {
+  const int lc = l == kb - 1 ? kb : l;
+  alg_stress(p, A0(s.q2, lc, j, J), A0(s.xl, lc, j, J), rz_level(p, s, J, j, lc, kt), av, kv, qd, qd2);
 }
 
-// Scheme 3, pass 2 (turbulence_closure_v1.f90:219-390), one thread per face: the k and psi
-// tridiagonal systems (Thomas in place in w.tk_x / w.tk_q2tmp, factors in w.tri_gam), the
-// length scale with Galperin's clipping, and k, l back at whole levels.
-__global__ void __launch_bounds__(256) k_turb_solve(DevMesh m, DevState s, DevForcing f, DevWork w, DevDims d, TurbParams p,
-                             int *err) {
+// Scheme 3 (turbulence_closure_v1.f90:56-66, 158-427), one thread per face column:
+// * the two stop checks of the layer quantities (:158-215);
+// * the k and psi tridiagonal systems (Thomas in place in w.tk_q2tmp / w.tk_x, factors in
+//   w.tri_gam and w.tk_shear), each row's inputs -- the level diffusivities of the current
+//   k, l, the half-level k, l, the squared shear and (g/rho_o) drho/dz -- formed as the
+//   top-down elimination reaches it, with the upper level's values and node velocities
+//   carried from the row above;
+// * the length scale with Galperin's clipping and k, l back at whole levels (:351-390);
+// * the new diffusivities at every level, clamped to the column's range (:397-427).
+__global__ void __launch_bounds__(256) k_turb_column(DevMesh m, DevState s, DevForcing f, DevWork w, DevDims d,
+                                                     TurbParams p, int *err) {
   int j = blockIdx.x * blockDim.x + threadIdx.x;
   const int J = d.J;
   if (j >= J) return;
@@ -217,24 +172,56 @@ __global__ void __launch_bounds__(256) k_turb_solve(DevMesh m, DevState s, DevFo
   double xlbot = dmaxf(p.xlmin2, AL(s.dz_face_new, kb, j, J) / 2.0 * (double)0.4f);
   int nq = kt - kb + 1;
   double cm3 = p.cmiu0 * p.cmiu0 * p.cmiu0;
+  const int P = d.P;
+  const int n1 = TB(m.nodenum_at_face, 1, j, J) - 1, n2 = TB(m.nodenum_at_face, 2, j, J) - 1;
+  // the stop checks of every layer first, as the reference's pass over the layers precedes
+  // its solve
+  for (int kl = kb; kl <= kt; kl++) {
+    double xlbt = (A0(s.xl, kl - 1, j, J) + A0(s.xl, kl, j, J)) / 2.0;
+    if (xlbt < p.xlmin2) latch(err, GOMGPU_ERR_TURB_XL, j + 1);
+    double dzk = AL(s.dz_face_new, kl, j, J);
+    double zctr = (kl == kt) ? p.MSL + etam - dzk / 2.0 : m.z_level[kl] - dzk / 2.0;
+    double dists = p.MSL + etam - zctr, distb = zctr - (p.MSL - hf);
+    double xlmax = (double)0.4f * dists * distb / (hf + etam);
+    if (xlmax <= 0.0) latch(err, GOMGPU_ERR_TURB_DIST, j + 1);
+  }
+  // level kt's diffusivities and node velocities, carried down the rows
+  double avk, kvk, qdk, qd2k;
+  level_diff(p, s, J, j, kt, kb, kt, avk, kvk, qdk, qd2k);
+  double uup = (A0(s.u_node, kt, n1, P) + A0(s.u_node, kt, n2, P)) / 2.0;
+  double vup = (A0(s.v_node, kt, n1, P) + A0(s.v_node, kt, n2, P)) / 2.0;
   // ---- the k and psi equations (:233-276 and :281-347) in one top-down pass: the two
   // systems share their inputs and are independent, so each row's inputs are read once and
   // both eliminations advance together (rows top down; row k <-> layer kl = kt - k + 1).
   // The k system's factors go to w.tri_gam (by row), the psi system's to w.tk_shear (by
-  // layer, after the row's shear has been read); solutions in w.tk_q2tmp and w.tk_x.
+  // layer); solutions in w.tk_q2tmp and w.tk_x.
   const double cpow = gls_pow(p.cmiu0, p.rpub);
   double den = 0.0, cprev = 0.0, x = 0.0;     // k system
   double denp = 0.0, cprevp = 0.0, xp = 0.0;  // psi system
   for (int k = 1; k <= nq; k++) {
     int kl = kt - k + 1;
     double dzk = AL(s.dz_face_new, kl, j, J);
-    double qdk = A0(w.tk_qd, kl, j, J), qd2k = A0(w.tk_qd2, kl, j, J);
     double dzhk = A0(s.dzhalf_face_new, kl, j, J), dzhm = A0(s.dzhalf_face_new, kl - 1, j, J);
-    double qdm = A0(w.tk_qd, kl - 1, j, J), qd2m = A0(w.tk_qd2, kl - 1, j, J);
-    double q2bt = AL(w.tk_q2bt, kl, j, J), xlbt = AL(w.tk_xlbt, kl, j, J);
-    double avk = A0(w.tk_av, kl, j, J), avm = A0(w.tk_av, kl - 1, j, J);
-    double kvk = A0(w.tk_kv, kl, j, J), kvm = A0(w.tk_kv, kl - 1, j, J);
-    double shear = AL(w.tk_shear, kl, j, J), rz = AL(w.tk_rz, kl, j, J);
+    double avm, kvm, qdm, qd2m;  // level kl-1
+    level_diff(p, s, J, j, kl - 1, kb, kt, avm, kvm, qdm, qd2m);
+    // layer kl (turbulence_closure_v1.f90:158-205)
+    double q2bt = (A0(s.q2, kl - 1, j, J) + A0(s.q2, kl, j, J)) / 2.0;
+    double xlbt = (A0(s.xl, kl - 1, j, J) + A0(s.xl, kl, j, J)) / 2.0;
+    double udown = (A0(s.u_node, kl - 1, n1, P) + A0(s.u_node, kl - 1, n2, P)) / 2.0;
+    double vdown = (A0(s.v_node, kl - 1, n1, P) + A0(s.v_node, kl - 1, n2, P)) / 2.0;
+    double dudz = (uup - udown) / dzk, dvdz = (vup - vdown) / dzk;
+    double shear = dudz * dudz + dvdz * dvdz;
+    double drhodz;
+    if (kl == kt || kl == kb) {
+      drhodz = 0.0;
+    } else {
+      double rho_up = AL(s.rho_face, kl, j, J) + dzk / 2.0 / dzhk * (AL(s.rho_face, kl + 1, j, J) - AL(s.rho_face, kl, j, J));
+      double rho_down = AL(s.rho_face, kl - 1, j, J) + AL(s.dz_face_new, kl - 1, j, J) / 2.0 / dzhm *
+                                                           (AL(s.rho_face, kl, j, J) - AL(s.rho_face, kl - 1, j, J));
+      drhodz = (rho_up - rho_down) / dzk;
+    }
+    double rz = p.g / p.rho_o * drhodz;
+    AL(w.tk_rz, kl, j, J) = rz;
     // k row
     double eleft = 0.0, eright = 0.0, a = 0.0, c = 0.0;
     if (k > 1) {
@@ -343,10 +330,16 @@ __global__ void __launch_bounds__(256) k_turb_solve(DevMesh m, DevState s, DevFo
         return;
       }
       xp = (rp - ap * xp) / denp;
-      AL(w.tk_shear, kl, j, J) = gam;  // this row's shear has been read
+      AL(w.tk_shear, kl, j, J) = gam;
     }
     AL(w.tk_x, kl, j, J) = xp;
     cprevp = cp;
+    avk = avm;
+    kvk = kvm;
+    qdk = qdm;
+    qd2k = qd2m;
+    uup = udown;
+    vup = vdown;
   }
   // ---- back substitutions, bottom up, both systems (:268-278, :351-368): the k solution
   // with its boundary values, then the length scale from the psi solution of the same row
@@ -399,32 +392,24 @@ __global__ void __launch_bounds__(256) k_turb_solve(DevMesh m, DevState s, DevFo
     A0(s.q2, k, j, J) = q;
     A0(s.xl, k, j, J) = xv;
   }
-}
-
-// Scheme 3, pass 3 (turbulence_closure_v1.f90:397-427), one thread per (level, face): the
-// new diffusivities at levels kb..kt and their extension above and below the column.
-__global__ void k_turb_post(DevMesh m, DevState s, DevWork w, DevDims d, TurbParams p) {
-  long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
-  const int J = d.J, L = d.L;
-  if (t >= (long)(L + 1) * J) return;
-  int j = (int)(t % J), k = (int)(t / J);
-  int kt = s.top_layer_at_face[j];
-  if (kt == 0) return;
-  int kb = m.bottom_layer_at_face[j];
-  if (k <= kb - 2) {
-    A0(s.q2, k, j, J) = A0(s.q2, kb - 1, j, J);
-    A0(s.xl, k, j, J) = A0(s.xl, kb - 1, j, J);
+  // the new diffusivities (:397-427): k, l extended above and below the column, and at every
+  // level the stability functions of the level clamped to kb..kt
+  for (int k = 0; k <= d.L; k++) {
+    if (k <= kb - 2) {
+      A0(s.q2, k, j, J) = A0(s.q2, kb - 1, j, J);
+      A0(s.xl, k, j, J) = A0(s.xl, kb - 1, j, J);
+    }
+    if (k > kt) {
+      A0(s.q2, k, j, J) = A0(s.q2, kt, j, J);
+      A0(s.xl, k, j, J) = A0(s.xl, kt, j, J);
+    }
+    if (k == 0) continue;
+    const int kl = k < kb ? kb : (k > kt ? kt : k);
+    double vd, td, qd, qd2;
+    alg_stress(p, A0(s.q2, kl, j, J), A0(s.xl, kl, j, J), rz_level(p, s, J, j, kl, kt), vd, td, qd, qd2);
+    A0(s.Av, k, j, J) = vd;
+    A0(w.turb_kvf, k, j, J) = td;
   }
-  if (k > kt) {
-    A0(s.q2, k, j, J) = A0(s.q2, kt, j, J);
-    A0(s.xl, k, j, J) = A0(s.xl, kt, j, J);
-  }
-  if (k == 0) return;
-  int kl = k < kb ? kb : (k > kt ? kt : k);
-  double vd, td, qd, qd2;
-  alg_stress(p, A0(s.q2, kl, j, J), A0(s.xl, kl, j, J), rz_level(p, s, J, j, kl, kt), vd, td, qd, qd2);
-  A0(s.Av, k, j, J) = vd;
-  A0(w.turb_kvf, k, j, J) = td;
 }
 
 // Face diffusivity -> cell Kv: mean over the cell's wet faces at the level clamped to each

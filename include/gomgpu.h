@@ -304,8 +304,8 @@ enum {
   GOMGPU_LAUNCH_TVD_SOLVE = 4,  /* k_tvd_solve */
   GOMGPU_LAUNCH_TVD_LIM = 5,    /* k_tvd_lim_cell, one thread per (level, cell) */
   GOMGPU_LAUNCH_NODE_VEL = 6,   /* k_node_vel */
-  GOMGPU_LAUNCH_TURB_PREP = 7,  /* k_turb_prep, k_turb_post */
-  GOMGPU_LAUNCH_TURB_SOLVE = 8, /* k_turb_solve */
+  GOMGPU_LAUNCH_TURB_PREP = 7,  /* k_turb_ri, one thread per (layer, face) (closure schemes 1, 2) */
+  GOMGPU_LAUNCH_TURB_SOLVE = 8, /* k_turb_column, one thread per face column (scheme 3) */
   GOMGPU_LAUNCH_STREAM = 9,     /* every other per-entity kernel */
   GOMGPU_NUM_LAUNCH = 10
 };
