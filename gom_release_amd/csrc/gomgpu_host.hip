@@ -16,6 +16,10 @@ __global__ void k_friction(DevMesh, DevState, DevDims, int, double, double);
 __global__ void k_elm_substeps(DevMesh, DevState, DevDims, double, double, int, int);
 __global__ void k_elm(DevMesh, DevState, DevDims, double, double, double, int *);
 __global__ void k_elm_copy(DevState, long);
+__global__ void k_elm_tracer(DevMesh, DevState, DevForcing, DevWork, DevDims, double, double, double, int, int *);
+__global__ void k_elm_column(DevMesh, DevState, DevWork, DevDims, double, int, int, int *);
+__global__ void k_elm_node(DevMesh, DevState, DevDims);
+__global__ void k_elm_face(DevMesh, DevState, DevDims);
 __global__ void k_cell_uv(DevMesh, DevState, DevWork, DevDims);
 __global__ void k_mom_rhs(DevMesh, DevState, DevForcing, DevWork, DevDims, MomParams);
 __global__ void k_mom_solve(DevMesh, DevState, DevForcing, DevWork, DevDims, MomParams, int *);
@@ -233,6 +237,10 @@ static int validate(const gomgpu_params *p, std::string &why) {
                    "temperature result from tracer slot 2)", 1;
     if (p->maxtran2 == 2 && p->tran_id2 != 2) return why = "tracer 2 must be temperature", 1;
     if (p->heat_option == 2) return why = "heat_option 2 is not supported", 1;
+    if (p->transport_solver < 0 || p->transport_solver > 2) return why = "transport_solver must be 1 or 2", 1;
+    if (p->transport_solver == 2 && p->maxtran2 != 2)
+      return why = "the ELM transport solver (transport_solver 2) carries salinity and temperature: maxtran2 "
+                   "must be 2", 1;
   }
   if (p->baroclinic_flag == 1 && p->ana_density != 0 && p->ana_density != 1)
     return why = "ana_density must be 0 or 1", 1;
@@ -843,6 +851,22 @@ static int stage_impl(gomgpu_ctx *c, int id) {
       break;
     case GOMGPU_STAGE_TRANSPORT: {
       if (p.transport_flag != 1 || d.NT == 0) break;
+      if (p.transport_solver == 2) {
+        // ELM transport: the backtraced, diffused right-hand sides per cell-layer, the column
+        // solves, node and face means. A dry column keeps its new values, so a stale pair
+        // (new standing for old) has old copied into such a column.
+        const int copy_salt = (c->stale & SWAP_SALT) != 0, copy_temp = (c->stale & SWAP_TEMP) != 0;
+        const int Nt = p.trans_sub_iter > 0 ? p.trans_sub_iter : 1;
+        KT_BEGIN(KT_TVD_COLUMN);
+        k_elm_tracer<<<nblk((long)d.E * d.L, 256), 256, 0, st>>>(c->m, c->s, c->f, c->w, d, p.MSL, p.dt, p.rho_o, Nt,
+                                                                 err);
+        k_elm_column<<<nblk(d.E, 64), 64, 0, st>>>(c->m, c->s, c->w, d, p.dt, copy_salt, copy_temp, err);
+        KT_END(KT_TVD_COLUMN);
+        k_elm_node<<<nblk(d.P, BS), BS, 0, st>>>(c->m, c->s, d);
+        k_elm_face<<<nblk((long)d.J * d.L, BS), BS, 0, st>>>(c->m, c->s, d);
+        c->stale &= ~(unsigned)(SWAP_SALT | SWAP_TEMP);
+        break;
+      }
       TvdArgs a;
       a.Nt = p.trans_sub_iter > 0 ? p.trans_sub_iter : 1;
       a.dt = p.dt;

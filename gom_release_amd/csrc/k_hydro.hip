@@ -335,9 +335,15 @@ __device__ __forceinline__ int search_line(const ElmCtx &c, int &nnel, int &jlev
   return 0;
 }
 
+// The end of a backtrace: the foot point, the top of its layer and the foot's interpolation
+// weights in the cell that holds it (the inputs of the tracer interpolation).
+struct ElmFoot {
+  double xt, yt, zt, zup, st[4];
+};
+
 // ELM_backtrace_v1.f90:15-200 (i_which_backtrack = 1)
 __device__ __forceinline__ int backtrace(const ElmCtx &c, int bt_step, double bt_dt, double &uu, double &vv, double &ww,
-                         double x0, double y0, double z0, int &nnel, int &jlev) {
+                         double x0, double y0, double z0, int &nnel, int &jlev, ElmFoot &ft) {
   const DevMesh &m = c.m;
   const DevState &s = c.s;
   const int E = c.d.E, P = c.d.P;
@@ -354,7 +360,7 @@ __device__ __forceinline__ int backtrace(const ElmCtx &c, int bt_step, double bt
     double zup = (jl == s.top_layer_at_element[e0]) ? c.MSL + s.eta_cell[e0] : m.z_level[jl];
     if (zt > zup) return GOMGPU_ERR_ELM_VERTICAL;
     double zrat = (zup - zt) / AL(s.dz_cell, jl, e0, E);
-    double vxn[4], vyn[4], vzn[4], st[4];
+    double vxn[4], vyn[4], vzn[4], st[4] = {0.0, 0.0, 0.0, 0.0};
     int nd[4];
     for (int l = 0; l < N; l++) {
       nd[l] = pick4(rc.nd, l) - 1;
@@ -396,6 +402,12 @@ __device__ __forceinline__ int backtrace(const ElmCtx &c, int bt_step, double bt
       vv = vv + vyn[l] * st[l];
       ww = ww + vzn[l] * st[l];
     }
+    ft.xt = xt;
+    ft.yt = yt;
+    ft.zt = zt;
+    ft.zup = zup;
+#pragma unroll
+    for (int l = 0; l < 4; l++) ft.st[l] = st[l];
     if (iflqs1 == 1) break;
     x0 = xt;
     y0 = yt;
@@ -454,7 +466,8 @@ __global__ void __launch_bounds__(256) k_elm(DevMesh m, DevState s, DevDims d, d
   double bt_dt = dt / bt_step;
   ElmCtx c{m, s, d, MSL};
   int nnel = ie0, jlev = k;
-  int e = backtrace(c, bt_step, bt_dt, uu, vv, ww, m.x_face[j], m.y_face[j], z0, nnel, jlev);
+  ElmFoot ft;
+  int e = backtrace(c, bt_step, bt_dt, uu, vv, ww, m.x_face[j], m.y_face[j], z0, nnel, jlev, ft);
   if (e) latch(err, e, j + 1);
   AL(s.un_ELM, k, j, J) = uu * ct + vv * st;
   AL(s.vn_ELM, k, j, J) = -uu * st + vv * ct;
@@ -466,6 +479,418 @@ __global__ void k_elm_copy(DevState s, long n) {
   if (t >= n) return;
   s.un_ELM[t] = s.un_face[t];
   s.vn_ELM[t] = s.vn_face[t];
+}
+
+// ================================================================ ELM scalar transport
+// solve_transport_equation_ELM_v8.f90:7-346 (transport_solver 2). Salt and temperature of
+// each wet cell-layer are backtraced from the layer centre, take the surface heat source and
+// an explicit horizontal diffusion term, then vertical diffusion implicit per column; the
+// node values are area*dz weighted cell means, the face values node means.
+
+// Node (or face) value blended between layers k1 and k2 of an (L, n) field.
+__device__ __forceinline__ double zblend(const double *a, int k1, int k2, double zrat2, int e, int n) {
+  return AL(a, k1, e, n) * (1 - zrat2) + AL(a, k2, e, n) * zrat2;
+}
+
+// The tracers at a backtraced foot (ELM_backtrace_v1.f90:216-473, which = 2). A boundary
+// cell gives its boundary values; otherwise the node values at the two levels bracketing zt
+// are weighted by the foot's interpolation weights (or averaged over the wet nodes), and a
+// wet sub-element holding the foot -- for a triangle the four sub-triangles through the face
+// midpoints, for a quad the four sub-quads through the face midpoints and the centre --
+// replaces that by its own interpolation. Boundary values are sub-cycle 0 of con_at_ob /
+// con_at_Qb, salt in tracer slot 1 and temperature in slot 2. nnel is 1-based.
+__device__ __forceinline__ int elm_tracer_foot(const DevMesh &m, const DevState &s, const DevForcing &f,
+                                               const DevDims &d, int Nt, int nnel, int jlev, const ElmFoot &ft,
+                                               double &tt, double &ss) {
+  const int E = d.E, J = d.J, P = d.P, NT = d.NT;
+  const int e0 = nnel - 1;
+  const double small_06 = (double)1.e-6f;
+  tt = 0.0;
+  ss = 0.0;
+  const int ib1 = m.ob_element_flag[e0], ib2 = m.Qb_element_flag[e0];
+  if (ib1 > 0) {
+    ss = f.con_at_ob[(size_t)Nt * (0 + (size_t)NT * (ib1 - 1))];
+    tt = f.con_at_ob[(size_t)Nt * (1 + (size_t)NT * (ib1 - 1))];
+    return 0;
+  }
+  if (ib2 > 0) {
+    ss = f.con_at_Qb[(size_t)Nt * (0 + (size_t)NT * (ib2 - 1))];
+    tt = f.con_at_Qb[(size_t)Nt * (1 + (size_t)NT * (ib2 - 1))];
+    return 0;
+  }
+  const double xt = ft.xt, yt = ft.yt, zt = ft.zt;
+  const double zanchor = ft.zup - AL(s.dz_cell, jlev, e0, E) * 0.5;
+  const int bl = m.bottom_layer_at_element[e0];
+  int k1, k2;
+  double zrat2;
+  if (jlev == s.top_layer_at_element[e0] && zt > zanchor) {
+    k1 = jlev;
+    k2 = jlev;
+    zrat2 = 0.5;
+  } else if (zt > zanchor) {
+    k1 = jlev;
+    k2 = jlev + 1;
+    zrat2 = (zt - zanchor) / A0(s.dzhalf_cell, jlev, e0, E);
+  } else {
+    k1 = jlev - 1 > bl ? jlev - 1 : bl;
+    k2 = jlev;
+    zrat2 = 1.0 - (zanchor - zt) / A0(s.dzhalf_cell, k1, e0, E);
+  }
+  zrat2 = dmaxf(0.0, dminf(1.0, zrat2));
+  const int N = m.tri_or_quad[e0];
+  int icount = 0;
+  double ta = 0.0, sa = 0.0, t_xi[4] = {0.0, 0.0, 0.0, 0.0}, s_xi[4] = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+  for (int l = 0; l < 4; l++) {
+    if (l < N) {
+      const int nd = TB(m.nodenum_at_cell, l + 1, e0, E) - 1;
+      if (s.top_layer_at_node[nd] != 0) {
+        icount++;
+        t_xi[l] = zblend(s.temp_node, k1, k2, zrat2, nd, P);
+        s_xi[l] = zblend(s.salt_node, k1, k2, zrat2, nd, P);
+        ta = ta + t_xi[l];
+        sa = sa + s_xi[l];
+      }
+    }
+  }
+  if (icount == 0) return GOMGPU_ERR_ELM_ALL_DRY;
+  ta = ta / icount;
+  sa = sa / icount;
+  if (icount != N) {
+    tt = ta;
+    ss = sa;
+  } else {
+#pragma unroll
+    for (int l = 0; l < 4; l++) {
+      if (l < N) {
+        tt = tt + t_xi[l] * ft.st[l];
+        ss = ss + s_xi[l] * ft.st[l];
+      }
+    }
+  }
+  int found = 0;
+  const double ar = m.area[e0];
+  if (N == 3) {
+    for (int l = 1; l <= 4; l++) {
+      int n1, f2, f3;
+      double X1, Y1;
+      if (l <= 3) {
+        n1 = TB(m.nodenum_at_cell, l, e0, E);
+        f2 = TB(m.facenum_at_cell, sen(N, l, 2), e0, E);
+        f3 = TB(m.facenum_at_cell, sen(N, l, 1), e0, E);
+        X1 = m.x_node[n1 - 1];
+        Y1 = m.y_node[n1 - 1];
+      } else {
+        n1 = TB(m.facenum_at_cell, 1, e0, E);  // the centre sub-triangle: a face midpoint first
+        f2 = TB(m.facenum_at_cell, 2, e0, E);
+        f3 = TB(m.facenum_at_cell, 3, e0, E);
+        X1 = m.x_face[n1 - 1];
+        Y1 = m.y_face[n1 - 1];
+      }
+      const double X2 = m.x_face[f2 - 1], Y2 = m.y_face[f2 - 1], X3 = m.x_face[f3 - 1], Y3 = m.y_face[f3 - 1];
+      const double aa1 = carea(xt, X2, X3, yt, Y2, Y3);
+      const double aa2 = carea(X1, xt, X3, Y1, yt, Y3);
+      const double aa3 = carea(X1, X2, xt, Y1, Y2, yt);
+      const double aa = fabs(aa1) + fabs(aa2) + fabs(aa3);
+      const double subrat = fabs(aa - ar / 4) * 4 / ar;
+      if (subrat < 100 * small_06) {
+        found = 1;
+        double sig0 = aa1 * 4 / ar, sig1 = aa2 * 4 / ar, sig2;
+        sig0 = dmaxf(0.0, dminf(1.0, sig0));
+        sig1 = dmaxf(0.0, dminf(1.0, sig1));
+        if (sig0 + sig1 > 1) {
+          sig2 = 0;
+          sig1 = 1 - sig0;
+        } else {
+          sig2 = 1 - sig0 - sig1;
+        }
+        bool wet = l <= 3 ? s.top_layer_at_node[n1 - 1] != 0 : s.top_layer_at_face[n1 - 1] != 0;
+        wet = wet && s.top_layer_at_face[f2 - 1] != 0 && s.top_layer_at_face[f3 - 1] != 0;
+        if (wet) {
+          const double t0 = l <= 3 ? zblend(s.temp_node, k1, k2, zrat2, n1 - 1, P)
+                                   : zblend(s.temp_face, k1, k2, zrat2, n1 - 1, J);
+          const double s0 = l <= 3 ? zblend(s.salt_node, k1, k2, zrat2, n1 - 1, P)
+                                   : zblend(s.salt_face, k1, k2, zrat2, n1 - 1, J);
+          const double t1 = zblend(s.temp_face, k1, k2, zrat2, f2 - 1, J);
+          const double t2 = zblend(s.temp_face, k1, k2, zrat2, f3 - 1, J);
+          const double s1 = zblend(s.salt_face, k1, k2, zrat2, f2 - 1, J);
+          const double s2 = zblend(s.salt_face, k1, k2, zrat2, f3 - 1, J);
+          tt = t0 * sig0 + t1 * sig1 + t2 * sig2;
+          ss = s0 * sig0 + s1 * sig1 + s2 * sig2;
+          break;
+        }
+      }
+    }
+  } else {
+    const double XC = m.x_cell[e0], YC = m.y_cell[e0];
+    for (int l = 1; l <= 4; l++) {
+      const int n1 = TB(m.nodenum_at_cell, l, e0, E);
+      const int f2 = TB(m.facenum_at_cell, sen(N, l, 3), e0, E);
+      const int f4 = TB(m.facenum_at_cell, sen(N, l, 2), e0, E);
+      const double X1 = m.x_node[n1 - 1], Y1 = m.y_node[n1 - 1];
+      const double X2 = m.x_face[f2 - 1], Y2 = m.y_face[f2 - 1];
+      const double X4 = m.x_face[f4 - 1], Y4 = m.y_face[f4 - 1];
+      const double aa1 = carea(X1, X2, xt, Y1, Y2, yt);
+      const double aa2 = carea(X2, XC, xt, Y2, YC, yt);
+      const double aa3 = carea(XC, X4, xt, YC, Y4, yt);
+      const double aa4 = carea(X4, X1, xt, Y4, Y1, yt);
+      const double bb1 = carea(X1, X2, X4, Y1, Y2, Y4);
+      const double bb2 = carea(X2, XC, X4, Y2, YC, Y4);
+      if (bb1 <= 0 || bb2 <= 0) return GOMGPU_ERR_ELM_NEG_SUBCELL;
+      const double bb = bb1 + bb2;
+      const double aa = fabs(aa1) + fabs(aa2) + fabs(aa3) + fabs(aa4);
+      const double subrat = fabs(aa - bb) / bb;
+      if (subrat < 4.0 * small_06) {
+        found = 1;
+        double sq[4];
+        const int e2 = bilinear(X1, X2, XC, X4, Y1, Y2, YC, Y4, xt, yt, sq);
+        if (e2) return e2;
+        if (s.top_layer_at_node[n1 - 1] != 0 && s.top_layer_at_face[f2 - 1] != 0 &&
+            s.top_layer_at_face[f4 - 1] != 0) {
+          const double t0 = zblend(s.temp_node, k1, k2, zrat2, n1 - 1, P);
+          const double t1 = zblend(s.temp_face, k1, k2, zrat2, f2 - 1, J);
+          const double t3 = zblend(s.temp_face, k1, k2, zrat2, f4 - 1, J);
+          const double s0 = zblend(s.salt_node, k1, k2, zrat2, n1 - 1, P);
+          const double s1 = zblend(s.salt_face, k1, k2, zrat2, f2 - 1, J);
+          const double s3 = zblend(s.salt_face, k1, k2, zrat2, f4 - 1, J);
+          tt = 0.0;
+          ss = 0.0;
+          tt = tt + t0 * sq[0];
+          ss = ss + s0 * sq[0];
+          tt = tt + t1 * sq[1];
+          ss = ss + s1 * sq[1];
+          tt = tt + ta * sq[2];
+          ss = ss + sa * sq[2];
+          tt = tt + t3 * sq[3];
+          ss = ss + s3 * sq[3];
+        }
+      }
+    }
+  }
+  if (!found) return GOMGPU_ERR_ELM_NO_SUBCELL;
+  return 0;
+}
+
+// One thread per (layer, cell): the backtraced tracers of a wet cell-layer with the surface
+// heat source (solve_transport_equation_ELM_v8.f90:40-140) and the explicit horizontal
+// diffusion through each face towards the face's second cell (:150-190), into w.tvd_x:
+// tracer slot 0 the temperature right-hand side, slot 1 the salt one. As in the reference the
+// salt differences enter the temperature row and the temperature differences the salt row.
+__global__ void __launch_bounds__(256) k_elm_tracer(DevMesh m, DevState s, DevForcing f, DevWork w, DevDims d,
+                                                    double MSL, double dt, double rho_o, int Nt, int *err) {
+  const long q = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  const int E = d.E, J = d.J, P = d.P, L = d.L;
+  if (q >= (long)E * L) return;
+  const int i = (int)(q % E), k = (int)(q / E) + 1;
+  const int tl = s.top_layer_at_element[i], bl = m.bottom_layer_at_element[i];
+  if (tl == 0 || k < bl || k > tl) return;
+  const int N = m.tri_or_quad[i];
+  const double cp = 4186.0;
+  double z0;
+  if (k == tl)
+    z0 = MSL + s.eta_cell[i] - AL(s.dz_cell, k, i, E) * 0.5;
+  else
+    z0 = m.z_level[k] - AL(s.dz_cell, k, i, E) * 0.5;
+  double u_temp = 0.0, v_temp = 0.0;
+  int bt_step = 0;
+  for (int l = 1; l <= N; l++) {
+    const int nd = TB(m.nodenum_at_cell, l, i, E) - 1;
+    u_temp = u_temp + (A0(s.u_node, k, nd, P) + A0(s.u_node, k - 1, nd, P)) * 0.5;
+    v_temp = v_temp + (A0(s.v_node, k, nd, P) + A0(s.v_node, k - 1, nd, P)) * 0.5;
+    bt_step = bt_step + AL(s.num_sub_elm_iteration, k, nd, P);
+  }
+  u_temp = u_temp / N;
+  v_temp = v_temp / N;
+  bt_step = bt_step / N;
+  double uu = u_temp, vv = v_temp;
+  double ww = (A0(s.wn_cell, k, i, E) + A0(s.wn_cell, k - 1, i, E)) * 0.5;
+  const double vmag = sqrt(uu * uu + vv * vv + ww * ww);
+  double tc, sc;
+  if (vmag <= (double)1.e-4f) {
+    tc = AL(s.temp_cell, k, i, E);
+    sc = AL(s.salt_cell, k, i, E);
+  } else {
+    const double bt_dt = dt / bt_step;
+    ElmCtx c{m, s, d, MSL};
+    ElmFoot ft;
+    int nnel = i + 1, jlev = k;
+    int e = backtrace(c, bt_step, bt_dt, uu, vv, ww, m.x_cell[i], m.y_cell[i], z0, nnel, jlev, ft);
+    if (!e) e = elm_tracer_foot(m, s, f, d, Nt, nnel, jlev, ft, tc, sc);
+    if (e) {
+      latch(err, e, i + 1);
+      return;
+    }
+  }
+  if (f.phi_sz) tc = tc + dt * AL(f.phi_sz, k, i, E) / (rho_o * cp * AL(s.dz_cell, k, i, E));
+  double sum1 = 0.0, sum2 = 0.0;
+  const double si = AL(s.salt_cell, k, i, E), ti = AL(s.temp_cell, k, i, E);
+  for (int l = 1; l <= N; l++) {
+    const int j = TB(m.facenum_at_cell, l, i, E) - 1;
+    const int ie = TB(m.adj_cellnum_at_face, 2, j, J);
+    if (ie != 0) {
+      const double rt = m.face_length[j] * AL(s.dz_face, k, j, J) * AL(s.Kh, k, j, J) / m.delta_j[j];
+      sum1 = sum1 + rt * (AL(s.salt_cell, k, ie - 1, E) - si);
+      sum2 = sum2 + rt * (AL(s.temp_cell, k, ie - 1, E) - ti);
+    }
+  }
+  sum1 = sum1 * dt / (m.area[i] * AL(s.dz_cell, k, i, E));
+  sum2 = sum2 * dt / (m.area[i] * AL(s.dz_cell, k, i, E));
+  w.tvd_x[(size_t)(k - 1) * E + i] = tc + sum1;
+  w.tvd_x[((size_t)L + (k - 1)) * E + i] = sc + sum2;
+}
+
+// Row r (1-based, surface first) of a wet column's vertical diffusion system
+// (solve_transport_equation_ELM_v8.f90:200-250); the top row's upper coefficient is
+// -dt + Kv/dzhalf as the reference writes it.
+__device__ __forceinline__ void elm_row(const DevState &s, int i, int E, int tl, int bl, int r, double dt,
+                                        double &av, double &bv, double &cv) {
+  const int nv = tl - bl + 1;
+  av = 0.0;
+  cv = 0.0;
+  if (tl == bl) {
+    bv = AL(s.dz_cell, tl, i, E);
+  } else if (r == 1) {
+    cv = -dt + A0(s.Kv, tl - 1, i, E) / A0(s.dzhalf_cell, tl - 1, i, E);
+    bv = AL(s.dz_cell, tl, i, E) - cv;
+  } else if (r == nv) {
+    av = -dt * A0(s.Kv, bl, i, E) / A0(s.dzhalf_cell, bl, i, E);
+    bv = -av + AL(s.dz_cell, bl, i, E);
+  } else {
+    const int k = tl - r + 1;
+    av = -dt * A0(s.Kv, k, i, E) / A0(s.dzhalf_cell, k, i, E);
+    cv = -dt * A0(s.Kv, k - 1, i, E) / A0(s.dzhalf_cell, k - 1, i, E);
+    bv = -av + AL(s.dz_cell, k, i, E) - cv;
+  }
+}
+
+// One thread per cell column: the Thomas recurrence of tridiagonal_solver.f90 with the
+// temperature and salt right-hand sides (dz times the rows of w.tvd_x, solved in place,
+// elimination factors in w.tri_gam), the solution into temp/salt_cell_new and the values
+// extended below the bottom and above the top layer. A dry column keeps its new values;
+// copy_salt / copy_temp say that the new array still holds an older state and stands for
+// the old one, which such a column then receives.
+__global__ void k_elm_column(DevMesh m, DevState s, DevWork w, DevDims d, double dt, int copy_salt, int copy_temp,
+                             int *err) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  const int E = d.E, L = d.L;
+  if (i >= E) return;
+  const int tl = s.top_layer_at_element[i], bl = m.bottom_layer_at_element[i];
+  if (tl == 0) {
+    for (int k = 1; k <= L; k++) {
+      if (copy_temp) AL(s.temp_cell_new, k, i, E) = AL(s.temp_cell, k, i, E);
+      if (copy_salt) AL(s.salt_cell_new, k, i, E) = AL(s.salt_cell, k, i, E);
+    }
+    return;
+  }
+  const int nv = tl - bl + 1;
+  double *xt = w.tvd_x, *xs = w.tvd_x + (size_t)L * E;
+  double av, bv, cv, cprev;
+  elm_row(s, i, E, tl, bl, 1, dt, av, bv, cv);
+  if (bv == 0.0) {
+    latch(err, GOMGPU_ERR_TRIDIAG_PIVOT, i + 1);
+    return;
+  }
+  double den = bv;
+  double dz = AL(s.dz_cell, tl, i, E);
+  double vt = dz * AL(xt, tl, i, E) / den, vs = dz * AL(xs, tl, i, E) / den;
+  AL(xt, tl, i, E) = vt;
+  AL(xs, tl, i, E) = vs;
+  cprev = cv;
+  for (int r = 2; r <= nv; r++) {
+    const int k = tl - r + 1;
+    elm_row(s, i, E, tl, bl, r, dt, av, bv, cv);
+    const double gam = cprev / den;
+    den = bv - av * gam;
+    if (den == 0.0) {
+      latch(err, GOMGPU_ERR_TRIDIAG_PIVOT, i + 1);
+      return;
+    }
+    dz = AL(s.dz_cell, k, i, E);
+    vt = (dz * AL(xt, k, i, E) - av * vt) / den;
+    vs = (dz * AL(xs, k, i, E) - av * vs) / den;
+    AL(xt, k, i, E) = vt;
+    AL(xs, k, i, E) = vs;
+    AL(w.tri_gam, r, i, E) = gam;
+    cprev = cv;
+  }
+  // back substitution: layers bl .. tl
+  AL(s.temp_cell_new, bl, i, E) = vt;
+  AL(s.salt_cell_new, bl, i, E) = vs;
+  for (int k = 1; k < bl; k++) {
+    AL(s.temp_cell_new, k, i, E) = vt;
+    AL(s.salt_cell_new, k, i, E) = vs;
+  }
+  for (int r = nv - 1; r >= 1; r--) {
+    const int k = tl - r + 1;
+    const double gam = AL(w.tri_gam, r + 1, i, E);
+    vt = AL(xt, k, i, E) - gam * vt;
+    vs = AL(xs, k, i, E) - gam * vs;
+    AL(s.temp_cell_new, k, i, E) = vt;
+    AL(s.salt_cell_new, k, i, E) = vs;
+  }
+  for (int k = tl + 1; k <= L; k++) {
+    AL(s.temp_cell_new, k, i, E) = vt;
+    AL(s.salt_cell_new, k, i, E) = vs;
+  }
+}
+
+// Node values (solve_transport_equation_ELM_v8.f90:270-320), one thread per node, top down:
+// the area*dz_new weighted mean of the adjacent cells wet at the level; a level whose
+// adjacent cells are all dry takes the value of the level above (unless it is the node's
+// top level); then the values extended below the bottom and above the top. Dry nodes keep
+// their values.
+__global__ void k_elm_node(DevMesh m, DevState s, DevDims d) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  const int P = d.P, E = d.E, L = d.L;
+  if (i >= P) return;
+  const int tn = s.top_layer_at_node[i], bn = m.bottom_layer_at_node[i];
+  if (tn == 0) return;
+  const int nadj = m.adj_cells_at_node[i];
+  double tabove = 0.0, sabove = 0.0;
+  for (int k = tn; k >= bn; k--) {
+    double sum0 = 0.0, sum1 = 0.0, sum2 = 0.0;
+    int icount = 0;
+    for (int l = 1; l <= nadj; l++) {
+      const int ie = TB(m.adj_cellnum_at_node, l, i, P) - 1;
+      if (k >= m.bottom_layer_at_element[ie] && k <= s.top_layer_at_element[ie]) {
+        const double wgt = m.area[ie] * AL(s.dz_cell_new, k, ie, E);
+        sum0 = sum0 + wgt;
+        sum1 = sum1 + wgt * AL(s.temp_cell_new, k, ie, E);
+        sum2 = sum2 + wgt * AL(s.salt_cell_new, k, ie, E);
+      } else {
+        icount++;
+      }
+    }
+    double tv = sum0 == 0.0 ? 0.0 : sum1 / sum0, sv = sum0 == 0.0 ? 0.0 : sum2 / sum0;
+    if (icount == nadj && k != tn) {
+      tv = tabove;
+      sv = sabove;
+    }
+    AL(s.temp_node, k, i, P) = tv;
+    AL(s.salt_node, k, i, P) = sv;
+    tabove = tv;
+    sabove = sv;
+  }
+  for (int k = 1; k < bn; k++) {
+    AL(s.temp_node, k, i, P) = tabove;
+    AL(s.salt_node, k, i, P) = sabove;
+  }
+  const double ttop = AL(s.temp_node, tn, i, P), stop = AL(s.salt_node, tn, i, P);
+  for (int k = tn + 1; k <= L; k++) {
+    AL(s.temp_node, k, i, P) = ttop;
+    AL(s.salt_node, k, i, P) = stop;
+  }
+}
+
+// Face values: the mean of the face's two nodes at every layer (:325-335).
+__global__ void k_elm_face(DevMesh m, DevState s, DevDims d) {
+  const long q = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  const int J = d.J, P = d.P;
+  if (q >= (long)J * d.L) return;
+  const int j = (int)(q % J), k = (int)(q / J) + 1;
+  const int n1 = TB(m.nodenum_at_face, 1, j, J) - 1, n2 = TB(m.nodenum_at_face, 2, j, J) - 1;
+  AL(s.salt_face, k, j, J) = (AL(s.salt_node, k, n1, P) + AL(s.salt_node, k, n2, P)) * 0.5;
+  AL(s.temp_face, k, j, J) = (AL(s.temp_node, k, n1, P) + AL(s.temp_node, k, n2, P)) * 0.5;
 }
 
 // ================================================================ momentum

@@ -73,8 +73,13 @@ class HostForcing:
         old, new = fz.boundary_eta(c, it, sp_tide, self.julian_day)
         f["eta_at_ob_old"] = old
         f["eta_at_ob_new"] = new
-        ob, qb = fz.tvd_boundary_concentrations(c, self.julian_day, sp_bc, self.params.maxtran2,
-                                                self.tran_ids)
+        if c.transport_solver == 2:
+            nqb = max(len(c.rivers), 1)
+            prev = self.forcing["con_at_Qb"].reshape(-1, 2, nqb, order="F")[0]
+            ob, qb = fz.elm_boundary_concentrations(c, self.julian_day, sp_bc, prev)
+        else:
+            ob, qb = fz.tvd_boundary_concentrations(c, self.julian_day, sp_bc, self.params.maxtran2,
+                                                    self.tran_ids)
         f["con_at_ob"] = ob
         f["con_at_Qb"] = qb
         if c.rivers:

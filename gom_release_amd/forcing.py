@@ -156,6 +156,43 @@ def tvd_boundary_concentrations(case: Case, julian_day: float, sp_baroclinic: fl
     return ob.ravel(order="F"), qb.ravel(order="F")
 
 
+def elm_boundary_concentrations(case: Case, julian_day: float, sp_baroclinic: float, qb_prev=None):
+    """The boundary values of the ELM transport (find_openboundary_salt_temp_v3.f90:17-97),
+    as con_at_ob (1, 2, OB) and con_at_Qb (1, 2, QB) in Fortran order: salt in tracer slot 1, temperature in slot 2.
+
+    An open boundary takes its series value (reference_diff_days applied) or ref_salt /
+    ref_temp when it has none; a river cell takes its series value only when it has one and
+    otherwise keeps its previous value (`qb_prev`, (2, QB), zero at the start). A series that
+    does not cover the time keeps the value of the boundary before it in the same call."""
+    nt = 1      # the ELM solver reads no sub-cycle count: the arrays keep extent 1
+    nob = max(len(case.open_boundaries), 1)
+    nqb = max(len(case.rivers), 1)
+    ob = np.zeros((nt, 2, nob))
+    qb = np.zeros((nt, 2, nqb))
+    u2 = julian_day * 86400.0
+    rdd = case.reference_diff_days()
+    v2 = 0.0
+    for i, b in enumerate(case.open_boundaries):
+        for t, (on, sid, ser, ref) in enumerate(((case.is_salt, b.salt_ser_id, case.salt_series, case.ref_salt),
+                                                 (case.is_temp, b.temp_ser_id, case.temp_series, case.ref_temp))):
+            if on == 1 and sid > 0:
+                v = series_value(ser[sid - 1], u2, rdd)
+                v2 = v2 if v is None else v
+            else:
+                v2 = ref
+            ob[0, t, i] = sp_baroclinic * v2
+    if qb_prev is not None:
+        qb[0] = np.asarray(qb_prev, dtype=np.float64).reshape(2, nqb)
+    for i, r in enumerate(case.rivers):
+        for t, (on, sid, ser) in enumerate(((case.is_salt, r.salt_ser_id, case.salt_series),
+                                            (case.is_temp, r.temp_ser_id, case.temp_series))):
+            if on == 1 and sid > 0:
+                v = series_value(ser[sid - 1], u2, rdd)
+                v2 = v2 if v is None else v
+                qb[0, t, i] = sp_baroclinic * v2
+    return ob.ravel(order="F"), qb.ravel(order="F")
+
+
 def wind_stress(mesh, u: float, v: float, rho_a: float = 1.29, rho_o: float = 1025.0,
                 formula: int = 2, spinup_wind: float = 1.0):
     """Face wind stress for a uniform wind (calculate_wind_stress.f90:19-66, wind_flag 1).

@@ -130,6 +130,7 @@ class Case:
     max_iteration_pcg: int = 1000
     error_tolerance: float = 1e-10
     transport_flag: int = 1
+    transport_solver: int = 1      # 1 TVD (solve_transport_equation_v19), 2 ELM (_ELM_v8)
     trans_sub_iter: int = 1
     h_flux_limiter: int = 0
     v_flux_limiter: int = 0
@@ -337,10 +338,11 @@ def _main_inp(c: Case) -> str:
     card("C6\tmax_iteration_pcg error_tolerance pcg_result_show")
     data(c.max_iteration_pcg, c.error_tolerance, 0)
     card("C7\ttransport_flag transport_solver")
-    data(c.transport_flag, 1)
+    data(c.transport_flag, int(c.transport_solver))
     if c.transport_flag == 1:
-        card("C7_1\ttrans_sub_iter h_flux_limiter v_flux_limiter")
-        data(c.trans_sub_iter, c.h_flux_limiter, c.v_flux_limiter)
+        if c.transport_solver == 1:
+            card("C7_1\ttrans_sub_iter h_flux_limiter v_flux_limiter")
+            data(c.trans_sub_iter, c.h_flux_limiter, c.v_flux_limiter)
         card("C7_2\tis_tran num_tran_ser tran_ser_shape")
         data(c.is_salt, len(c.salt_series) if c.is_salt else 0, 1)
         data(c.is_temp, len(c.temp_series) if c.is_temp else 0, 1)
@@ -1012,7 +1014,7 @@ def unsupported(pr: Project) -> List[str]:
         (m.ana_depth != 0, "C1 ana_depth (analytical depth)"),
         (m.voronoi != 0, "C1 voronoi cell centres"),
         (m.bf_varying != 0, "C5_02 bf_varying 1 (bottom_roughness.inp)"),
-        (m.transport_flag == 1 and m.transport_solver != 1, "C7 transport_solver 2 (ELM transport)"),
+        (m.transport_flag == 1 and m.transport_solver not in (1, 2), "C7 transport_solver not 1 or 2"),
         (m.transport_flag == 1 and m.is_tran[1] == 1 and m.is_tran[0] == 0,
          "C7_2 temperature without salinity"),
         (m.transport_flag == 1 and m.is_tran[1] == 1,
@@ -1068,6 +1070,7 @@ def case_from_project(pr: Project, strict: bool = True) -> Tuple[Case, List[str]
     c.smagorinsky, c.ah0, c.kh0, c.av0, c.kv0 = m.smagorinsky_parameter, m.Ah_0, m.Kh_0, m.Av_0, m.Kv_0
     c.max_iteration_pcg, c.error_tolerance = m.max_iteration_pcg, m.error_tolerance
     c.transport_flag = m.transport_flag
+    c.transport_solver = m.transport_solver if m.transport_flag == 1 else 1
     if m.transport_flag == 1:
         c.trans_sub_iter = m.get("trans_sub_iter", 1)
         c.h_flux_limiter = m.get("h_flux_limiter", 0)

@@ -35,6 +35,9 @@ def params_for(case: Case, precision: int = 0) -> "abi.Params":
     dz = z[1:] - z[:-1]
     ntr = (1 if case.is_salt else 0) + (1 if case.is_temp else 0)
     tid = ([1] if case.is_salt else []) + ([2] if case.is_temp else [])
+    if case.transport_flag == 1 and case.transport_solver == 2:
+        # the ELM solver carries salt and temperature both (solve_transport_equation_ELM_v8.f90)
+        ntr, tid = 2, [1, 2]
     tid += [0, 0]
     d = dict(
         maxnod=case.maxnod, maxele=case.maxele, maxface=0, maxlayer=L,
@@ -44,8 +47,11 @@ def params_for(case: Case, precision: int = 0) -> "abi.Params":
         advection_flag=case.advection_flag, ELM_backtrace_flag=case.elm_backtrace_flag,
         ELM_min_iter=case.elm_min_iter, ELM_max_iter=case.elm_max_iter, bf_flag=case.bf_flag,
         baroclinic_flag=case.baroclinic_flag, ana_density=case.ana_density, i_density_flag=0,
-        transport_flag=case.transport_flag, trans_sub_iter=case.trans_sub_iter,
-        h_flux_limiter=case.h_flux_limiter, v_flux_limiter=case.v_flux_limiter,
+        transport_flag=case.transport_flag,
+        # the ELM solver reads no TVD card (C7_1), whose values stay zero
+        trans_sub_iter=case.trans_sub_iter if case.transport_solver != 2 else 0,
+        h_flux_limiter=case.h_flux_limiter if case.transport_solver != 2 else 0,
+        v_flux_limiter=case.v_flux_limiter if case.transport_solver != 2 else 0,
         heat_option=1 if case.is_temp else 0, max_iteration_pcg=case.max_iteration_pcg,
         precision=precision,
         dt=case.dt, theta=case.theta, gravity=float(case.gravity),
@@ -60,6 +66,7 @@ def params_for(case: Case, precision: int = 0) -> "abi.Params":
         # C30/C31 (read_hurricane_ser_0.f90:29: hurricane_read_interval = int(hurricane_dt/dt))
         hurricane_flag=int(case.hurricane_flag), wind_formular=int(case.wind_formular),
         hurricane_read_interval=int(case.hurricane_dt / case.dt) if case.hurricane_flag == 1 else 0,
+        transport_solver=int(case.transport_solver) if case.transport_flag == 1 else 1,
         rho_a=float(case.rho_a))
     d.update(turbulence_params(case))
     return abi.params_from_dict(d)

@@ -19,6 +19,8 @@
  *                                             (not called by main.f90; runs between NODEVEL and
  *                                             TRANSPORT when turbulence_flag != 0)
  *   gomgpu_stage(ctx, GOMGPU_STAGE_TRANSPORT) solve_transport_equation_v19 main.f90:231
+ *                                             (transport_solver = 1; = 2: the ELM solver
+ *                                             solve_transport_equation_ELM_v8)
  *   gomgpu_stage(ctx, GOMGPU_STAGE_EOS)       calculate_density_full/_linear main.f90:261,264
  *   gomgpu_stage(ctx, GOMGPU_STAGE_UPDATE)    update_variables            main.f90:272
  *   gomgpu_step(ctx)                          all of the above, in that order
@@ -100,6 +102,10 @@ extern "C" {
  * con_at_Qb (Nt, maxtran2, QB), fastest index first. With params.river_on_device = 1 the
  * river face velocity Qu_boundary is computed on the device from the discharge Q_add and
  * the current face depth (compute_boundary_velocity.f90:32-46) instead of being passed.
+ * With transport_solver = 2 (the ELM solver, which carries salt and temperature both, so
+ * maxtran2 = 2) sub-cycle 0 of con_at_ob and con_at_Qb holds the boundary values of
+ * find_openboundary_salt_temp_v3.f90 (salt_at_obck / temp_at_obck, salt_at_Qbc /
+ * temp_at_Qbc).
  * With params.hurricane_flag = 1 the wind stage keeps the hurricane fields on the device
  * (read_hurricane_ser_1.f90): snapshot pair wind_u0/v0/air_p0 and wind_u2/v2/air_p2, the
  * interpolated wind_u1/v1/air_p1 and the face winds; the host passes them once at the
@@ -124,7 +130,7 @@ extern "C" {
   X(transport_flag) X(trans_sub_iter) X(h_flux_limiter) X(v_flux_limiter) \
   X(heat_option) X(max_iteration_pcg) X(precision) X(river_on_device) \
   X(turbulence_flag) X(turbulence_model) X(stability_function) \
-  X(hurricane_flag) X(hurricane_read_interval) X(wind_formular)
+  X(hurricane_flag) X(hurricane_read_interval) X(wind_formular) X(transport_solver)
 
 #define GOMGPU_PARAMS_REAL(X) \
   X(dt) X(theta) X(gravity) X(rho_o) X(MSL) X(dry_depth) X(adv_onoff_depth) \
@@ -241,7 +247,10 @@ enum gomgpu_model_error {
   GOMGPU_ERR_EOS_DRY_NODES = 20,     /* calculate_density_full.f90:223-225 */
   GOMGPU_ERR_TURB_XL = 21,           /* turbulence_closure_v1.f90:163-170 */
   GOMGPU_ERR_TURB_DIST = 22,         /* turbulence_closure_v1.f90:209-214 */
-  GOMGPU_ERR_TURB_NEGATIVE = 23      /* turbulence_closure_v1.f90:384-388 */
+  GOMGPU_ERR_TURB_NEGATIVE = 23,     /* turbulence_closure_v1.f90:384-388 */
+  GOMGPU_ERR_ELM_ALL_DRY = 24,       /* ELM_backtrace_v1.f90:275-279 (tracer foot in a dry cell) */
+  GOMGPU_ERR_ELM_NEG_SUBCELL = 25,   /* ELM_backtrace_v1.f90:420-424 */
+  GOMGPU_ERR_ELM_NO_SUBCELL = 26     /* ELM_backtrace_v1.f90:463-468 */
 };
 
 typedef struct gomgpu_ctx gomgpu_ctx;

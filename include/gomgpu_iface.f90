@@ -50,7 +50,8 @@ module gomgpu_iface
       & ELM_min_iter, ELM_max_iter, bf_flag, baroclinic_flag, ana_density, i_density_flag, &
       & transport_flag, trans_sub_iter, h_flux_limiter, v_flux_limiter, heat_option, &
       & max_iteration_pcg, precision, river_on_device, turbulence_flag, turbulence_model, &
-      & stability_function, hurricane_flag, hurricane_read_interval, wind_formular = 0
+      & stability_function, hurricane_flag, hurricane_read_interval, wind_formular, &
+      & transport_solver = 0
     real(c_double) :: dt, theta, gravity, rho_o, MSL, dry_depth, adv_onoff_depth, &
       & smagorinsky_parameter, Ah_0, Kh_0, von_Karman, error_tolerance, &
       & denominator_min_for_matrix, delta_z_min, q2min, psimin, eps_min, bgdiff, diffmax, xlmin1, &

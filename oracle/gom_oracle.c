@@ -357,14 +357,16 @@ static int search_line(const elmctx *c, int *nnel, int *jlev, double time, doubl
 }
 
 /* ELM_backtrace_v1.f90:15-200 (i_which_backtrack = 1) */
+/* ELM_backtrace_v1.f90:15-214. With foot non-NULL, the foot point and its interpolation
+ * weights are left there: xt, yt, zt, zup, staint[4]. */
 static int backtrace(const elmctx *c, int jface, int bt_step, double bt_dt, double *uu,
                      double *vv, double *ww, double x0, double y0, double z0, int *nnel,
-                     int *jlev) {
+                     int *jlev, double *foot) {
   const gomgpu_mesh *m = c->m;
   const gomgpu_params *p = c->p;
   gomgpu_state *s = c->s;
   const int L = c->L;
-  double vxl[4][2], vyl[4][2], vzl[4][2], vxn[4], vyn[4], vzn[4], staint[4];
+  double vxl[4][2], vyl[4][2], vzl[4][2], vxn[4], vyn[4], vzn[4], staint[4] = {0, 0, 0, 0};
   double xt = 0, yt = 0, zt = 0;
   for (int idt = 1; idt <= bt_step; idt++) {
     xt = x0 - bt_dt * *uu;
@@ -427,6 +429,13 @@ static int backtrace(const elmctx *c, int jface, int bt_step, double bt_dt, doub
       *uu = *uu + vxn[l] * staint[l];
       *vv = *vv + vyn[l] * staint[l];
       *ww = *ww + vzn[l] * staint[l];
+    }
+    if (foot) {
+      foot[0] = xt;
+      foot[1] = yt;
+      foot[2] = zt;
+      foot[3] = zup;
+      for (int l = 0; l < 4; l++) foot[4 + l] = staint[l];
     }
     if (iflqs1 == 1) break;
     x0 = xt;
@@ -538,7 +547,7 @@ int oracle_elm(const gomgpu_mesh *m, const gomgpu_params *p, gomgpu_state *s,
       } else {
         int bt_step = (int)((RL(s->num_sub_elm_iteration, k, n1) + RL(s->num_sub_elm_iteration, k, n2)) * 0.5);
         double bt_dt = dt / bt_step;
-        int err = backtrace(&c, j, bt_step, bt_dt, &uu, &vv, &ww, x0, y0, z0, &nnel, &jlev);
+        int err = backtrace(&c, j, bt_step, bt_dt, &uu, &vv, &ww, x0, y0, z0, &nnel, &jlev, NULL);
         if (err) return err;
         RL(s->un_ELM, k, j) = uu * ct + vv * st;
         RL(s->vn_ELM, k, j) = -uu * st + vv * ct;
@@ -1201,8 +1210,12 @@ static double limiter(int which, double r, double phi) {
 }
 
 /* solve_transport_equation_v19.f90:7-1489 (TVD; no WR or SS points; heat_option 1) */
+static int oracle_transport_elm(const gomgpu_mesh *m, const gomgpu_params *p, gomgpu_state *s,
+                                const gomgpu_forcing *f);
+
 int oracle_transport(const gomgpu_mesh *m, const gomgpu_params *p, gomgpu_state *s,
                      const gomgpu_forcing *f) {
+  if (p->transport_solver == 2) return oracle_transport_elm(m, p, s, f);
   const int E = p->maxele, J = p->maxface, P = p->maxnod, L = p->maxlayer, NB = p->max_no_neighbor_node;
   const int NT = p->maxtran2, NOB = p->num_ob_cell, NQB = p->num_Qb_cell;
   const int tid[2] = {p->tran_id1, p->tran_id2};
@@ -1670,6 +1683,372 @@ static int eos_column(const gomgpu_mesh *m, const gomgpu_params *p, int L, int f
     if (rcol[k - 1] < 980.0) return fail(GOMGPU_ERR_EOS_RANGE, entity);
   }
   return 0;
+}
+
+/* ---------------------------------------------------- ELM scalar transport ---------- */
+/* The tracers at a backtraced foot point: the which = 2 branch of ELM_backtrace_v1.f90:
+ * 216-473. Boundary cells take the boundary values; elsewhere the node values of the two
+ * levels bracketing zt are blended, weighted by the foot's interpolation weights (or
+ * averaged over the wet nodes), and a wet sub-element of the cell that holds the foot --
+ * the four sub-triangles of a triangle through its face midpoints, the four sub-quads of a
+ * quad through its face midpoints and centre -- replaces that by its own interpolation. */
+static int elm_tracer_foot(const elmctx *c, const gomgpu_forcing *f, int nnel, int jlev,
+                           const double *foot, double *ttint, double *ssint) {
+  const gomgpu_mesh *m = c->m;
+  const gomgpu_params *p = c->p;
+  gomgpu_state *s = c->s;
+  const int L = c->L, NT = p->maxtran2, Nt = p->trans_sub_iter > 0 ? p->trans_sub_iter : 1;
+  const double xt = foot[0], yt = foot[1], zt = foot[2], zup = foot[3];
+  const double small_06 = (double)1.e-6f;
+  double staint[4] = {foot[4], foot[5], foot[6], foot[7]};
+  const int N = R1(m->tri_or_quad, nnel);
+  const int ibnd1 = R1(m->ob_element_flag, nnel), ibnd2 = R1(m->Qb_element_flag, nnel);
+  *ttint = 0.0;
+  *ssint = 0.0;
+  if (ibnd1 > 0) {
+    *ssint = f->con_at_ob[0 + (size_t)Nt * (0 + (size_t)NT * (ibnd1 - 1))];
+    *ttint = f->con_at_ob[0 + (size_t)Nt * (1 + (size_t)NT * (ibnd1 - 1))];
+    return 0;
+  }
+  if (ibnd2 > 0) {
+    *ssint = f->con_at_Qb[0 + (size_t)Nt * (0 + (size_t)NT * (ibnd2 - 1))];
+    *ttint = f->con_at_Qb[0 + (size_t)Nt * (1 + (size_t)NT * (ibnd2 - 1))];
+    return 0;
+  }
+  const double zanchor = zup - RL(s->dz_cell, jlev, nnel) * 0.5;
+  int k1, k2;
+  double zrat2;
+  if (jlev == R1(s->top_layer_at_element, nnel) && zt > zanchor) {
+    k1 = jlev;
+    k2 = jlev;
+    zrat2 = 0.5;
+  } else if (zt > zanchor) {
+    k1 = jlev;
+    k2 = jlev + 1;
+    zrat2 = (zt - zanchor) / R0(s->dzhalf_cell, jlev, nnel);
+  } else {
+    k1 = jlev - 1 > R1(m->bottom_layer_at_element, nnel) ? jlev - 1 : R1(m->bottom_layer_at_element, nnel);
+    k2 = jlev;
+    zrat2 = 1.0 - (zanchor - zt) / R0(s->dzhalf_cell, k1, nnel);
+  }
+  zrat2 = dmax(0.0, dmin(1.0, zrat2));
+#define ZB(arr, k, e) (RL(arr, k1, e) * (1 - zrat2) + RL(arr, k2, e) * zrat2)
+  int icount = 0;
+  double ta = 0.0, sa = 0.0, t_xi[4] = {0, 0, 0, 0}, s_xi[4] = {0, 0, 0, 0};
+  for (int l = 1; l <= N; l++) {
+    int nd = I4(m->nodenum_at_cell, l, nnel);
+    if (R1(s->top_layer_at_node, nd) != 0) {
+      icount++;
+      t_xi[l - 1] = ZB(s->temp_node, k, nd);
+      s_xi[l - 1] = ZB(s->salt_node, k, nd);
+      ta = ta + t_xi[l - 1];
+      sa = sa + s_xi[l - 1];
+    }
+  }
+  if (icount == 0) return fail(GOMGPU_ERR_ELM_ALL_DRY, nnel);
+  if (icount != N) {
+    ta = ta / icount;
+    sa = sa / icount;
+    *ttint = ta;
+    *ssint = sa;
+  } else {
+    ta = ta / icount;
+    sa = sa / icount;
+    *ttint = 0.0;
+    *ssint = 0.0;
+    for (int l = 0; l < N; l++) {
+      *ttint = *ttint + t_xi[l] * staint[l];
+      *ssint = *ssint + s_xi[l] * staint[l];
+    }
+  }
+  int index_1 = 0;
+  const double ar = R1(m->area, nnel);
+  if (N == 3) {
+    for (int l = 1; l <= 4; l++) {
+      double aa1, aa2, aa3, X1, Y1, X2, Y2, X3, Y3;
+      int n1 = 0, f2, f3;
+      if (l <= 3) {
+        n1 = I4(m->nodenum_at_cell, l, nnel);
+        f2 = I4(m->facenum_at_cell, sen(N, l, 2), nnel);
+        f3 = I4(m->facenum_at_cell, sen(N, l, 1), nnel);
+        X1 = R1(m->x_node, n1);
+        Y1 = R1(m->y_node, n1);
+      } else {
+        int f1 = I4(m->facenum_at_cell, 1, nnel);
+        f2 = I4(m->facenum_at_cell, 2, nnel);
+        f3 = I4(m->facenum_at_cell, 3, nnel);
+        X1 = R1(m->x_face, f1);
+        Y1 = R1(m->y_face, f1);
+        n1 = f1;
+      }
+      X2 = R1(m->x_face, f2);
+      Y2 = R1(m->y_face, f2);
+      X3 = R1(m->x_face, f3);
+      Y3 = R1(m->y_face, f3);
+      aa1 = carea(xt, X2, X3, yt, Y2, Y3);
+      aa2 = carea(X1, xt, X3, Y1, yt, Y3);
+      aa3 = carea(X1, X2, xt, Y1, Y2, yt);
+      double aa = fabs(aa1) + fabs(aa2) + fabs(aa3);
+      double subrat = fabs(aa - ar / 4) * 4 / ar;
+      if (subrat < 100 * small_06) {
+        index_1 = 1;
+        double sig[3];
+        sig[0] = aa1 * 4 / ar;
+        sig[1] = aa2 * 4 / ar;
+        sig[0] = dmax(0.0, dmin(1.0, sig[0]));
+        sig[1] = dmax(0.0, dmin(1.0, sig[1]));
+        if (sig[0] + sig[1] > 1) {
+          sig[2] = 0;
+          sig[1] = 1 - sig[0];
+        } else {
+          sig[2] = 1 - sig[0] - sig[1];
+        }
+        int wet = l <= 3 ? (R1(s->top_layer_at_node, n1) != 0) : (R1(s->top_layer_at_face, n1) != 0);
+        wet = wet && R1(s->top_layer_at_face, f2) != 0 && R1(s->top_layer_at_face, f3) != 0;
+        if (wet) {
+          if (l <= 3) {
+            t_xi[0] = ZB(s->temp_node, k, n1);
+            s_xi[0] = ZB(s->salt_node, k, n1);
+          } else {
+            t_xi[0] = ZB(s->temp_face, k, n1);
+            s_xi[0] = ZB(s->salt_face, k, n1);
+          }
+          t_xi[1] = ZB(s->temp_face, k, f2);
+          t_xi[2] = ZB(s->temp_face, k, f3);
+          s_xi[1] = ZB(s->salt_face, k, f2);
+          s_xi[2] = ZB(s->salt_face, k, f3);
+          *ttint = t_xi[0] * sig[0] + t_xi[1] * sig[1] + t_xi[2] * sig[2];
+          *ssint = s_xi[0] * sig[0] + s_xi[1] * sig[1] + s_xi[2] * sig[2];
+          break;
+        }
+      }
+    }
+  } else {
+    const double XC = R1(m->x_cell, nnel), YC = R1(m->y_cell, nnel);
+    for (int l = 1; l <= 4; l++) {
+      int n1 = I4(m->nodenum_at_cell, l, nnel);
+      int f2 = I4(m->facenum_at_cell, sen(N, l, 3), nnel);
+      int f4 = I4(m->facenum_at_cell, sen(N, l, 2), nnel);
+      double X1 = R1(m->x_node, n1), Y1 = R1(m->y_node, n1);
+      double X2 = R1(m->x_face, f2), Y2 = R1(m->y_face, f2);
+      double X4 = R1(m->x_face, f4), Y4 = R1(m->y_face, f4);
+      double aa1 = carea(X1, X2, xt, Y1, Y2, yt);
+      double aa2 = carea(X2, XC, xt, Y2, YC, yt);
+      double aa3 = carea(XC, X4, xt, YC, Y4, yt);
+      double aa4 = carea(X4, X1, xt, Y4, Y1, yt);
+      double bb1 = carea(X1, X2, X4, Y1, Y2, Y4);
+      double bb2 = carea(X2, XC, X4, Y2, YC, Y4);
+      if (bb1 <= 0 || bb2 <= 0) return fail(GOMGPU_ERR_ELM_NEG_SUBCELL, nnel);
+      double bb = bb1 + bb2;
+      double aa = fabs(aa1) + fabs(aa2) + fabs(aa3) + fabs(aa4);
+      double subrat = fabs(aa - bb) / bb;
+      if (subrat < 4.0 * small_06) {
+        index_1 = 1;
+        double csi, etta;
+        int err2 = oracle_bilinear(X1, X2, XC, X4, Y1, Y2, YC, Y4, xt, yt, &csi, &etta, staint);
+        if (err2) return fail(err2, nnel);
+        if (R1(s->top_layer_at_node, n1) != 0 && R1(s->top_layer_at_face, f2) != 0 &&
+            R1(s->top_layer_at_face, f4) != 0) {
+          t_xi[0] = ZB(s->temp_node, k, n1);
+          t_xi[1] = ZB(s->temp_face, k, f2);
+          t_xi[2] = ta;
+          t_xi[3] = ZB(s->temp_face, k, f4);
+          s_xi[0] = ZB(s->salt_node, k, n1);
+          s_xi[1] = ZB(s->salt_face, k, f2);
+          s_xi[2] = sa;
+          s_xi[3] = ZB(s->salt_face, k, f4);
+          *ttint = 0.0;
+          *ssint = 0.0;
+          for (int j = 0; j < 4; j++) {
+            *ttint = *ttint + t_xi[j] * staint[j];
+            *ssint = *ssint + s_xi[j] * staint[j];
+          }
+        }
+      }
+    }
+  }
+#undef ZB
+  if (index_1 == 0) return fail(GOMGPU_ERR_ELM_NO_SUBCELL, nnel);
+  return 0;
+}
+
+/* solve_transport_equation_ELM_v8.f90:7-346: salt and temperature backtraced from each
+ * wet cell-layer centre, explicit horizontal diffusion, implicit vertical diffusion (the
+ * top row's upper coefficient is -dt + Kv/dzhalf as the reference writes it), the values
+ * extended above and below the column, area*dz-weighted node means and face means. Dry
+ * cells and nodes keep their new values. */
+static int oracle_transport_elm(const gomgpu_mesh *m, const gomgpu_params *p, gomgpu_state *s,
+                                const gomgpu_forcing *f) {
+  const int E = p->maxele, P = p->maxnod, J = p->maxface, L = p->maxlayer, NB = p->max_no_neighbor_node;
+  const double dt = p->dt;
+  const double cp = 4186.0;
+  double *tc = calloc((size_t)L * E, sizeof(double)), *sc = calloc((size_t)L * E, sizeof(double));
+  double *trhs = calloc((size_t)L * E, sizeof(double)), *srhs = calloc((size_t)L * E, sizeof(double));
+  double a[65], b[65], cc[65], rhs[2 * 65], sol[2 * 65];  /* rows 1..nv, right-hand sides 65 apart */
+  int ret = 0;
+  elmctx c = {m, p, s, L};
+  for (int i = 1; i <= E && !ret; i++) {
+    int tl = R1(s->top_layer_at_element, i), bl = R1(m->bottom_layer_at_element, i);
+    if (tl == 0) continue;  /* tc, sc stay 0 */
+    int N = R1(m->tri_or_quad, i);
+    for (int k = bl; k <= tl; k++) {
+      int nnel = i, jlev = k;
+      double x0 = R1(m->x_cell, i), y0 = R1(m->y_cell, i), z0;
+      if (k == R1(s->top_layer_at_element, nnel))
+        z0 = p->MSL + R1(s->eta_cell, nnel) - RL(s->dz_cell, k, nnel) * 0.5;
+      else
+        z0 = ZL(k) - RL(s->dz_cell, k, nnel) * 0.5;
+      double u_temp = 0.0, v_temp = 0.0;
+      int bt_step = 0;
+      for (int l = 1; l <= N; l++) {
+        int nd = I4(m->nodenum_at_cell, l, i);
+        u_temp = u_temp + (R0(s->u_node, k, nd) + R0(s->u_node, k - 1, nd)) * 0.5;
+        v_temp = v_temp + (R0(s->v_node, k, nd) + R0(s->v_node, k - 1, nd)) * 0.5;
+        bt_step = bt_step + RL(s->num_sub_elm_iteration, k, nd);
+      }
+      u_temp = u_temp / N;
+      v_temp = v_temp / N;
+      bt_step = bt_step / N;
+      double uu = u_temp, vv = v_temp;
+      double ww = (R0(s->wn_cell, k, i) + R0(s->wn_cell, k - 1, i)) * 0.5;
+      double vmag = sqrt(uu * uu + vv * vv + ww * ww);
+      double t_, s_;
+      if (vmag <= (double)1.e-4f) {
+        t_ = RL(s->temp_cell, k, i);
+        s_ = RL(s->salt_cell, k, i);
+      } else {
+        double bt_dt = dt / bt_step, foot[8];
+        int err = backtrace(&c, i, bt_step, bt_dt, &uu, &vv, &ww, x0, y0, z0, &nnel, &jlev, foot);
+        if (!err) err = elm_tracer_foot(&c, f, nnel, jlev, foot, &t_, &s_);
+        if (err) {
+          ret = err;
+          break;
+        }
+      }
+      t_ = t_ + dt * RL(f->phi_sz, k, i) / (p->rho_o * cp * RL(s->dz_cell, k, i));
+      RL(tc, k, i) = t_;
+      RL(sc, k, i) = s_;
+    }
+  }
+  if (!ret) {
+    for (int i = 1; i <= E; i++) {
+      int tl = R1(s->top_layer_at_element, i), bl = R1(m->bottom_layer_at_element, i), N = R1(m->tri_or_quad, i);
+      for (int k = bl; k <= tl; k++) {
+        double sum1 = 0.0, sum2 = 0.0;
+        for (int l = 1; l <= N; l++) {
+          int j = I4(m->facenum_at_cell, l, i);
+          int ie = I2(m->adj_cellnum_at_face, 2, j);
+          if (ie != 0) {
+            double rt = R1(m->face_length, j) * RL(s->dz_face, k, j) * RL(s->Kh, k, j) / R1(m->delta_j, j);
+            sum1 = sum1 + rt * (RL(s->salt_cell, k, ie) - RL(s->salt_cell, k, i));
+            sum2 = sum2 + rt * (RL(s->temp_cell, k, ie) - RL(s->temp_cell, k, i));
+          }
+        }
+        sum1 = sum1 * dt / (R1(m->area, i) * RL(s->dz_cell, k, i));
+        sum2 = sum2 * dt / (R1(m->area, i) * RL(s->dz_cell, k, i));
+        RL(trhs, k, i) = RL(tc, k, i) + sum1;
+        RL(srhs, k, i) = RL(sc, k, i) + sum2;
+      }
+    }
+    for (int i = 1; i <= E && !ret; i++) {
+      int tl = R1(s->top_layer_at_element, i), bl = R1(m->bottom_layer_at_element, i);
+      if (tl == 0) continue;
+      int nv = tl - bl + 1;
+      for (int k = bl + 1; k <= tl - 1; k++) {
+        int kk = tl - k + 1;
+        a[kk] = -dt * R0(s->Kv, k, i) / R0(s->dzhalf_cell, k, i);
+        cc[kk] = -dt * R0(s->Kv, k - 1, i) / R0(s->dzhalf_cell, k - 1, i);
+        b[kk] = -a[kk] + RL(s->dz_cell, k, i) - cc[kk];
+      }
+      if (tl == bl) {
+        b[1] = RL(s->dz_cell, tl, i);
+      } else {
+        cc[1] = -dt + R0(s->Kv, tl - 1, i) / R0(s->dzhalf_cell, tl - 1, i);
+        b[1] = RL(s->dz_cell, tl, i) - cc[1];
+        a[nv] = -dt * R0(s->Kv, bl, i) / R0(s->dzhalf_cell, bl, i);
+        b[nv] = -a[nv] + RL(s->dz_cell, bl, i);
+      }
+      for (int k = 1; k <= nv; k++) {
+        int kk = tl - k + 1;
+        rhs[k] = RL(s->dz_cell, kk, i) * RL(trhs, kk, i);
+        rhs[65 + k] = RL(s->dz_cell, kk, i) * RL(srhs, kk, i);
+      }
+      int err = oracle_tridiag(nv, 2, 65, a + 1, b + 1, cc + 1, rhs + 1, sol + 1);
+      if (err) {
+        ret = fail(err, i);
+        break;
+      }
+      for (int k = 1; k <= nv; k++) {
+        int klev = tl + 1 - k;
+        RL(s->temp_cell_new, klev, i) = sol[k];
+        RL(s->salt_cell_new, klev, i) = sol[65 + k];
+      }
+      for (int k = 1; k <= bl - 1; k++) {
+        RL(s->temp_cell_new, k, i) = RL(s->temp_cell_new, bl, i);
+        RL(s->salt_cell_new, k, i) = RL(s->salt_cell_new, bl, i);
+      }
+      for (int k = tl + 1; k <= L; k++) {
+        RL(s->temp_cell_new, k, i) = RL(s->temp_cell_new, tl, i);
+        RL(s->salt_cell_new, k, i) = RL(s->salt_cell_new, tl, i);
+      }
+    }
+  }
+  if (!ret) {
+    for (int i = 1; i <= P; i++) {
+      int tn = R1(s->top_layer_at_node, i), bn = R1(m->bottom_layer_at_node, i);
+      int nadj = R1(m->adj_cells_at_node, i);
+      for (int k = tn; k >= bn; k--) {
+        double sum0 = 0.0, sum1 = 0.0, sum2 = 0.0;
+        int icount = 0;
+        for (int l = 1; l <= nadj; l++) {
+          int ie = INB(m->adj_cellnum_at_node, l, i);
+          if (k >= R1(m->bottom_layer_at_element, ie) && k <= R1(s->top_layer_at_element, ie)) {
+            sum0 = sum0 + R1(m->area, ie) * RL(s->dz_cell_new, k, ie);
+            sum1 = sum1 + R1(m->area, ie) * RL(s->dz_cell_new, k, ie) * RL(s->temp_cell_new, k, ie);
+            sum2 = sum2 + R1(m->area, ie) * RL(s->dz_cell_new, k, ie) * RL(s->salt_cell_new, k, ie);
+          } else {
+            icount++;
+          }
+        }
+        if (sum0 == 0.0) {
+          RL(s->temp_node, k, i) = 0.0;
+          RL(s->salt_node, k, i) = 0.0;
+        } else {
+          RL(s->temp_node, k, i) = sum1 / sum0;
+          RL(s->salt_node, k, i) = sum2 / sum0;
+        }
+        if (icount == nadj && k != tn) {
+          RL(s->temp_node, k, i) = RL(s->temp_node, k + 1, i);
+          RL(s->salt_node, k, i) = RL(s->salt_node, k + 1, i);
+        }
+      }
+    }
+    for (int i = 1; i <= P; i++) {
+      int tn = R1(s->top_layer_at_node, i), bn = R1(m->bottom_layer_at_node, i);
+      if (tn == 0) continue;
+      for (int k = 1; k <= bn - 1; k++) {
+        RL(s->temp_node, k, i) = RL(s->temp_node, bn, i);
+        RL(s->salt_node, k, i) = RL(s->salt_node, bn, i);
+      }
+      for (int k = tn + 1; k <= L; k++) {
+        RL(s->temp_node, k, i) = RL(s->temp_node, tn, i);
+        RL(s->salt_node, k, i) = RL(s->salt_node, tn, i);
+      }
+    }
+    for (int j = 1; j <= J; j++) {
+      int n1 = I2(m->nodenum_at_face, 1, j), n2 = I2(m->nodenum_at_face, 2, j);
+      for (int k = 1; k <= L; k++) {
+        RL(s->salt_face, k, j) = (RL(s->salt_node, k, n1) + RL(s->salt_node, k, n2)) * 0.5;
+        RL(s->temp_face, k, j) = (RL(s->temp_node, k, n1) + RL(s->temp_node, k, n2)) * 0.5;
+      }
+    }
+  }
+  (void)NB;
+  free(tc);
+  free(sc);
+  free(trhs);
+  free(srhs);
+  return ret;
 }
 
 /* calculate_density_full.f90:13-247 / calculate_density_linear.f90:10-165 */

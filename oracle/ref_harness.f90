@@ -299,6 +299,7 @@ contains
       call put_i0('num_WR_cell', num_WR_cell); call put_i0('num_SS_cell', num_SS_cell)
       call put_i0('i_density_flag', i_density_flag)
       call put_r0('rho_a', rho_a); call put_i0('wind_formular', wind_formular)
+      call put_i0('transport_solver', transport_solver)
       call put_i0('hurricane_flag', hurricane_flag)
       if (hurricane_flag == 1) then
          call put_i0('hurricane_read_interval', hurricane_read_interval)

@@ -111,9 +111,12 @@ def params_from_geom(g: Dict[str, np.ndarray]):
         von_Karman=g["von_Karman"], error_tolerance=g["error_tolerance"],
         denominator_min_for_matrix=g["denominator_min_for_matrix"], delta_z_min=g["delta_z_min"])
     # dumped by the harness since the wind stage exists (older fixtures lack them)
-    for k in ("rho_a", "wind_formular", "hurricane_flag", "hurricane_read_interval"):
+    for k in ("rho_a", "wind_formular", "hurricane_flag", "hurricane_read_interval", "transport_solver"):
         if k in g:
             d[k] = g[k]
+    if d.get("transport_solver") == 2 and d["transport_flag"] == 1:
+        # the ABI's convention for the ELM solver, which carries both tracers
+        d.update(maxtran2=2, tran_id1=1, tran_id2=2)
     return abi.params_from_dict(d)
 
 

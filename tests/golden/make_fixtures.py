@@ -70,6 +70,11 @@ CASES = {
                                      elm_max_iter=10, dry_land=True, river_q=50.0,
                                      tide_phase=-90.0),
                            steps=[1, 2, 44, 45, 46]),
+    # the ELM transport solver (transport_solver 2) with tide, river and wet/dry banks
+    "elm_estuary": dict(kind="estuary",
+                        args=dict(ncol=12, nrep=3, nlayer=5, ndt=3, dry_land=True, tide_phase=-90.0,
+                                  river_q=40.0, transport_solver=2),
+                        steps=[1, 2, 3]),
     # Holland cyclone fields read every third step; the hurricane window opens at step 4
     "hurricane_estuary": dict(kind="estuary",
                               args=dict(ncol=6, nrep=2, nlayer=3, ndt=8, dry_land=True,

@@ -13,7 +13,7 @@ GOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
 STAGES = [("friction", "pre"), ("elm", "elm"), ("momentum", "mom"), ("freesurf", "fs"),
           ("velocity", "vel"), ("layers", "lay"), ("nodevel", "nodev"), ("transport", "tvd"),
           ("eos", "eos"), ("update", "post")]
-FIXTURES = ["channel_16x8x5", "estuary_wetdry", "hurricane_estuary"]
+FIXTURES = ["channel_16x8x5", "estuary_wetdry", "hurricane_estuary", "elm_estuary"]
 
 
 def _flat(a):
@@ -52,9 +52,14 @@ class Golden:
         if case is not None:
             from gom_release_amd import forcing as fz
             p = self.params
-            ob, qb = fz.tvd_boundary_concentrations(case, float(pre["julian_day"]),
-                                                    float(pre["spinup_function_baroclinic"]),
-                                                    p.maxtran2, [p.tran_id1, p.tran_id2])
+            if p.transport_solver == 2:
+                # the river values without a series keep their zero start in these fixtures
+                ob, qb = fz.elm_boundary_concentrations(case, float(pre["julian_day"]),
+                                                        float(pre["spinup_function_baroclinic"]))
+            else:
+                ob, qb = fz.tvd_boundary_concentrations(case, float(pre["julian_day"]),
+                                                        float(pre["spinup_function_baroclinic"]),
+                                                        p.maxtran2, [p.tran_id1, p.tran_id2])
         return refdump.forcing_from_dump(pre, self.params, ob, qb)
 
     def expected(self, it: int, tag: str) -> Dict[str, np.ndarray]:

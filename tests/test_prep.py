@@ -17,7 +17,7 @@ SKIP_STATE = {"rho_cell", "rho_node", "rho_face", "ubar_node", "vbar_node", "sba
 # closure parameters are not read by read_main_inp.f90, so the reference start-up has none
 SKIP_PARAMS = {"precision"} | set(prep.TURBULENCE_PARAMS)
 # wind-stage parameters, in the fixtures made since the harness dumps them
-OPTIONAL_GEOM = {"rho_a", "wind_formular", "hurricane_flag", "hurricane_read_interval"}
+OPTIONAL_GEOM = {"rho_a", "wind_formular", "hurricane_flag", "hurricane_read_interval", "transport_solver"}
 
 
 @pytest.mark.parametrize("name", FIXTURES)
