@@ -42,14 +42,23 @@ PCG_TOLERANCE = 5.0e-6
 
 
 def make_case(cells: str, turbulence: str = "gls", pcg_tolerance: float = PCG_TOLERANCE,
-              storm_steps: int = 0):
+              storm_steps: int = 0, transport: str = "tvd"):
     """The estuary of `cells`; with storm_steps > 0 the storm-surge physics of the 1.2M-cell
     configuration: a Holland cyclone (950 hPa, r_max 40 km, B 1.5) crossing at 5 m/s, its
     fields every 10 steps for storm_steps steps, and stratified temperature (10 C) and
-    salinity (3 psu) transported."""
+    salinity (3 psu) transported. transport "elm" selects the ELM transport solver
+    (transport_solver 2) in place of the TVD one, with a uniform 20 C temperature and a
+    5 psu river: that scheme adds the horizontal diffusion of temperature to salinity (and of
+    salinity to temperature), which with a cold river drives salinity well below zero at the
+    mouth, and has no clamp, so that a fresh river's salinity dips just below zero even with
+    uniform temperature, where the S^1.5 term of the UNESCO equation is undefined (the
+    linear equation of state is no way out: the reference stops at its first call, on the
+    face densities it never computes)."""
     from gom_release_amd import meshgen
     kw = dict(nlayer=20, dt=120.0, river_q=500.0, trans_sub_iter=2, h_flux_limiter=1, v_flux_limiter=1,
               error_tolerance=pcg_tolerance, max_iteration_pcg=1000, **TURBULENCE[turbulence])
+    if transport == "elm":
+        kw["transport_solver"] = 2
     if cells == "200k":
         case = meshgen.estuary_case(ncol=400, nrep=100, **kw)
     elif cells == "20k":
@@ -59,10 +68,26 @@ def make_case(cells: str, turbulence: str = "gls", pcg_tolerance: float = PCG_TO
         case = meshgen.estuary_case(ncol=980, nrep=245, **kw)
     else:
         raise ValueError(cells)
+    if transport == "elm":
+        elm_tracers(case)
     if storm_steps > 0:
         meshgen.add_cyclone(case, storm_steps, seed=3, p_min=950.0, r_max=40.0e3, b=1.5, speed=5.0)
         meshgen.stratify(case, d_temp=10.0, d_salt=3.0)
     return case
+
+
+def elm_tracers(case, t: float = 20.0, s_river: float = 5.0) -> None:
+    """Tracers the ELM transport scheme keeps in range: temperature t everywhere (at the start,
+    on every open boundary and river), salinity from 35 psu at sea to s_river in the river."""
+    import numpy as np
+    from gom_release_amd import gomio
+    case.is_temp = 1
+    case.temp_init = np.full((case.maxnod, case.maxlayer), t)
+    case.temp_series = [gomio.Series([-1000.0, 100000.0], [t, t])]
+    for b in list(case.open_boundaries) + list(case.rivers):
+        b.temp_ser_id = 1
+    case.salt_init = s_river + np.asarray(case.salt_init) * ((35.0 - s_river) / 35.0)
+    case.salt_series[1] = gomio.Series([-1000.0, 100000.0], [s_river, s_river])
 
 
 def pcg_bytes_per_iteration(mesh, params) -> float:
@@ -230,6 +255,8 @@ def main() -> int:
                     help="vertical closure of the workload (gls: k-kl GLS, Galperin stability functions)")
     ap.add_argument("--storm", action="store_true",
                     help="cyclone forcing and stratified T/S (the physics of the 1.2M-cell configuration)")
+    ap.add_argument("--transport", choices=["tvd", "elm"], default="tvd",
+                    help="scalar transport solver: TVD (transport_solver 1) or ELM (2)")
     ap.add_argument("--pcg-tolerance", type=float, default=PCG_TOLERANCE,
                     help="PCG stop tolerance on r.M^-1 r / r0.M^-1 r0 (default: the reference's main.inp)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
@@ -255,7 +282,8 @@ def main() -> int:
         print(f"[bench] calibration copy of {args.copy_calibration} MiB done", file=sys.stderr, flush=True)
     wind = (0.0, 0.0) if args.storm else (10.0, 0.0)
     case = make_case(args.cells, args.turbulence, args.pcg_tolerance,
-                     storm_steps=(4000 + args.warmup + args.steps + args.profile_steps) if args.storm else 0)
+                     storm_steps=(4000 + args.warmup + args.steps + args.profile_steps) if args.storm else 0,
+                     transport=args.transport)
     prec = 0 if args.precision == "fp64" else 1
     print(f"[bench] building {case.maxele}-cell x {case.maxlayer}-layer case", file=sys.stderr, flush=True)
     sc = Scenario(case, precision=prec, wind=wind)
@@ -311,6 +339,8 @@ def main() -> int:
     roof["step"] = {"algorithmic_bytes": round(bstep), "achieved": round(bstep / (step_ms * 1e-3) / 1e9, 1),
                     "frac": round(bstep / (step_ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
                     "active_face_cell_node_layers": [F, Cl, N]}
+    if args.transport == "elm":     # the byte model counts the TVD transport kernels
+        roof["step"] = None
     vol, salt = model.budgets()
     out = {
         "metric": "cell-layer-steps/s at 1 GPU + %HBM roofline; p50/p99 scenario latency",
@@ -319,13 +349,16 @@ def main() -> int:
         "scaling": "weak", "vs_baseline": None, "dtype": "f64" if prec == 0 else "f32+f64",
         "data": "synthetic",
         "config": {"workload": f"synthetic estuary, {case.maxele} mixed tri/quad cells x {case.maxlayer} "
-                               f"z-layers, dt={case.dt:g} s, TVD minmod x2 sub-cycles, M2+K1, 500 m3/s river, "
+                               f"z-layers, dt={case.dt:g} s, "
+                               + ("ELM scalar transport, uniform 20 C, 5 psu river" if args.transport == "elm"
+                                  else "TVD minmod x2 sub-cycles")
+                               + ", M2+K1, 500 m3/s river, "
                                + ("Holland cyclone (950 hPa, r_max 40 km, B 1.5, 5 m/s; fields every 10 steps), "
                                   "stratified T/S (10 C, 3 psu) transported, " if args.storm else "10 m/s wind, ")
                                + ("GLS k-kl turbulence closure (Galperin)"
                                                    if args.turbulence == "gls" else "constant Av/Kv")
                                + f", PCG tolerance {case.error_tolerance:g}",
-                   "turbulence": args.turbulence,
+                   "turbulence": args.turbulence, "transport": args.transport,
                    "cells": case.maxele, "faces": sc.params.maxface, "nodes": case.maxnod,
                    "layers": case.maxlayer, "active_cell_layers": int(0.5 * (a0 + a1)),
                    "nominal_cell_layer_steps_per_s": case.maxele * case.maxlayer * args.steps / elapsed,

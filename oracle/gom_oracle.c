@@ -1663,7 +1663,12 @@ static int eos_column(const gomgpu_mesh *m, const gomgpu_params *p, int L, int f
   for (int k = bl; k <= tl; k++) {
     double rt = tcol[k - 1], rs = scol[k - 1];
     if (rt < -5.0 || rt > 100.0 || rs < -5.0 || rs > 100.0) return fail(GOMGPU_ERR_EOS_RANGE, entity);
-    if (!do_compute) continue;
+    /* a face of the linear EOS keeps its density (i_density_flag, never set, is 0), and the
+     * check below still reads it (calculate_density_linear.f90:101-110) */
+    if (!do_compute) {
+      if (rcol[k - 1] < 980.0) return fail(GOMGPU_ERR_EOS_RANGE, entity);
+      continue;
+    }
     if (full) {
       rcol[k - 1] = oracle_rho_unesco(rt, rs);
       if (p->MSL - ZL(k) > 100.0) {
