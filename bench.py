@@ -69,25 +69,11 @@ def make_case(cells: str, turbulence: str = "gls", pcg_tolerance: float = PCG_TO
     else:
         raise ValueError(cells)
     if transport == "elm":
-        elm_tracers(case)
+        meshgen.elm_tracers(case)
     if storm_steps > 0:
         meshgen.add_cyclone(case, storm_steps, seed=3, p_min=950.0, r_max=40.0e3, b=1.5, speed=5.0)
         meshgen.stratify(case, d_temp=10.0, d_salt=3.0)
     return case
-
-
-def elm_tracers(case, t: float = 20.0, s_river: float = 5.0) -> None:
-    """Tracers the ELM transport scheme keeps in range: temperature t everywhere (at the start,
-    on every open boundary and river), salinity from 35 psu at sea to s_river in the river."""
-    import numpy as np
-    from gom_release_amd import gomio
-    case.is_temp = 1
-    case.temp_init = np.full((case.maxnod, case.maxlayer), t)
-    case.temp_series = [gomio.Series([-1000.0, 100000.0], [t, t])]
-    for b in list(case.open_boundaries) + list(case.rivers):
-        b.temp_ser_id = 1
-    case.salt_init = s_river + np.asarray(case.salt_init) * ((35.0 - s_river) / 35.0)
-    case.salt_series[1] = gomio.Series([-1000.0, 100000.0], [s_river, s_river])
 
 
 def pcg_bytes_per_iteration(mesh, params) -> float:

@@ -281,3 +281,27 @@ def stratify(case: Case, d_temp: float = 10.0, d_salt: float = 3.0, t_bottom: fl
         s = np.asarray(case.salt_init, dtype=np.float64)
         case.salt_init = s + d_salt * (1.0 - frac)[None, :] * (s / max(float(s.max()), 1e-30))
     return case
+
+
+def elm_tracers(case: Case, t: float = 20.0, s_river: float = 5.0, temp_input: bool = True) -> Case:
+    """Tracers of an estuary_case that the ELM transport scheme (transport_solver 2) keeps in
+    range: temperature t everywhere (at the start, on every open boundary and river) and
+    salinity from 35 psu at sea to s_river in the river. That scheme adds temperature differences to salinity in its
+    horizontal diffusion and has no clamp, so a cold or fresh river drives salinity below
+    zero, where the UNESCO equation of state is undefined. With temp_input False temperature
+    is no input (is_temp 0: no temperature files, and no air series for the heat model,
+    which the reference then reads) and t must be 0, the value a river without a
+    temperature series keeps."""
+    if temp_input:
+        case.is_temp = 1
+        case.temp_init = np.full((case.maxnod, case.maxlayer), t)
+        case.temp_series = [Series([-1000.0, 100000.0], [t, t])]
+        for b in list(case.open_boundaries) + list(case.rivers):
+            b.temp_ser_id = 1
+    else:
+        if t != 0.0:
+            raise ValueError("without temperature input the rivers stay at 0 C")
+        case.is_temp, case.ref_temp = 0, 0.0
+    case.salt_init = s_river + np.asarray(case.salt_init) * ((35.0 - s_river) / 35.0)
+    case.salt_series[1] = Series([-1000.0, 100000.0], [s_river, s_river])
+    return case

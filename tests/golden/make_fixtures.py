@@ -70,11 +70,12 @@ CASES = {
                                      elm_max_iter=10, dry_land=True, river_q=50.0,
                                      tide_phase=-90.0),
                            steps=[1, 2, 44, 45, 46]),
-    # the ELM transport solver (transport_solver 2) with tide, river and wet/dry banks
+    # the ELM transport solver (transport_solver 2) with tide, river and wet/dry banks (the
+    # banks dry from step 43), uniform 0 C (no temperature input) and a 5 psu river
     "elm_estuary": dict(kind="estuary",
-                        args=dict(ncol=12, nrep=3, nlayer=5, ndt=3, dry_land=True, tide_phase=-90.0,
-                                  river_q=40.0, transport_solver=2),
-                        steps=[1, 2, 3]),
+                        args=dict(ncol=20, nrep=3, nlayer=5, ndt=46, dry_land=True, tide_phase=-90.0,
+                                  river_q=50.0, transport_solver=2),
+                        elm_tracers=True, steps=[1, 2, 44, 45, 46]),
     # Holland cyclone fields read every third step; the hurricane window opens at step 4
     "hurricane_estuary": dict(kind="estuary",
                               args=dict(ncol=6, nrep=2, nlayer=3, ndt=8, dry_land=True,
@@ -124,8 +125,9 @@ def make(name: str, spec, outdir: str) -> str:
     path = os.path.join(outdir, name + ".npz")
     np.savez_compressed(path, **out)
     meta = dict(case=name, kind=spec["kind"], args=spec["args"], steps=steps)
-    if spec.get("hurricane"):
-        meta["hurricane"] = True
+    for flag in ("hurricane", "elm_tracers"):
+        if spec.get(flag):
+            meta[flag] = True
     with open(os.path.join(outdir, name + ".json"), "w") as f:
         json.dump(meta, f, indent=1)
     return path

@@ -97,12 +97,14 @@ WIND_OUTPUTS = HURRICANE_FIELDS + ["Gamma_T", "wind_stress_normal", "wind_stress
 
 
 def case_of(spec):
-    """The generated case of a fixture description (kind, args, hurricane)."""
+    """The generated case of a fixture description (kind, args, elm_tracers, hurricane)."""
     from gom_release_amd import meshgen
     if spec["kind"] == "channel":
         case = meshgen.channel_case(**spec["args"])
     else:
         case = meshgen.estuary_case(**spec["args"])
+    if spec.get("elm_tracers"):
+        meshgen.elm_tracers(case, t=0.0, temp_input=False)
     if spec.get("hurricane"):
         # a Holland cyclone (950 hPa, r_max 300 m) crossing the small estuary at 5 m/s,
         # fields every third step; the model starts at 23:53 so the window opens at step 4
