@@ -98,17 +98,20 @@ def boundary_eta(case: Case, it: int, sp_tide: float, julian_day: Optional[float
     rdd = case.reference_diff_days()
     v2_old = v2_new = 0.0
     memo = {}
+    tides = {}  # boundaries share harmonic sets: one sum per set
     for i, b in enumerate(case.open_boundaries):
         if b.eta_type == 1:
-            shift, cons = case.harmonics[b.harmonic_id - 1]
-            o = 0.0
-            n = 0.0
-            for (per, amp, ph, fac, v0, _nm) in cons:
-                w = 2.0 * math.pi / (per * 3600.0)
-                o = o + sp_tide * amp * fac * math.cos(w * (elapsed - dt) + v0 * DEG2RAD - ph * DEG2RAD - shift * DEG2RAD)
-                n = n + sp_tide * amp * fac * math.cos(w * elapsed + v0 * DEG2RAD - ph * DEG2RAD - shift * DEG2RAD)
-            old[i] = o
-            new[i] = n
+            if b.harmonic_id not in tides:
+                shift, cons = case.harmonics[b.harmonic_id - 1]
+                o = 0.0
+                n = 0.0
+                for (per, amp, ph, fac, v0, _nm) in cons:
+                    w = 2.0 * math.pi / (per * 3600.0)
+                    o = o + sp_tide * amp * fac * math.cos(w * (elapsed - dt) + v0 * DEG2RAD - ph * DEG2RAD
+                                                           - shift * DEG2RAD)
+                    n = n + sp_tide * amp * fac * math.cos(w * elapsed + v0 * DEG2RAD - ph * DEG2RAD - shift * DEG2RAD)
+                tides[b.harmonic_id] = (o, n)
+            old[i], new[i] = tides[b.harmonic_id]
         elif b.eta_type == 2:
             if julian_day is None:
                 raise ValueError("elevation-series boundaries need the julian day")
@@ -137,22 +140,28 @@ def tvd_boundary_concentrations(case: Case, julian_day: float, sp_baroclinic: fl
     nqb = max(len(case.rivers), 1)
     ob = np.zeros((nt, max(ntran, 1), nob))
     qb = np.zeros((nt, max(ntran, 1), nqb))
+    memo = {}   # boundaries share series: one interpolation per (tracer, series, sub-cycle)
+
+    def value(tid, sid, u2):
+        key = (tid, sid, u2)
+        if key not in memo:
+            ser = case.salt_series if tid == 1 else case.temp_series
+            v = series_value(ser[sid - 1], u2)
+            memo[key] = sp_baroclinic * (v if v is not None else 0.0)
+        return memo[key]
+
     for sub in range(nt):
         u2 = (julian_day * 86400.0 - case.dt) + (sub + 1) * dt2
         for t in range(ntran):
             tid = tran_ids[t]
             for i, b in enumerate(case.open_boundaries):
                 sid = b.salt_ser_id if tid == 1 else b.temp_ser_id
-                ser = case.salt_series if tid == 1 else case.temp_series
                 if sid > 0:
-                    v = series_value(ser[sid - 1], u2)
-                    ob[sub, t, i] = sp_baroclinic * (v if v is not None else 0.0)
+                    ob[sub, t, i] = value(tid, sid, u2)
             for i, r in enumerate(case.rivers):
                 sid = r.salt_ser_id if tid == 1 else r.temp_ser_id
-                ser = case.salt_series if tid == 1 else case.temp_series
                 if sid > 0:
-                    v = series_value(ser[sid - 1], u2)
-                    qb[sub, t, i] = sp_baroclinic * (v if v is not None else 0.0)
+                    qb[sub, t, i] = value(tid, sid, u2)
     return ob.ravel(order="F"), qb.ravel(order="F")
 
 
