@@ -15,6 +15,7 @@
 
 #define GOM_LMAX 64  // largest supported maxlayer (per-thread column arrays)
 #define GOM_NSCAL 8  // per-step device scalars (DevForcing::scal)
+#define GOM_MAX_FORCING 32  // forcing arrays the ABI can hold (GOMGPU_FORCING_REAL has fewer)
 #define GOM_NV_LEVELS 3  // node levels per thread of k_node_vel
 #define GOM_TVD_NODE_LEVELS 4  // node levels per thread of k_tvd_node
 #define GOM_KV_LEVELS 3  // cell levels per thread of k_turb_kv

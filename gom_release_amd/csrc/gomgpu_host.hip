@@ -59,6 +59,26 @@ __global__ void __launch_bounds__(256) k_turb_column(DevMesh, DevState, DevForci
 __global__ void k_turb_kv(DevMesh, DevState, DevWork, DevDims);
 __global__ void k_budget(DevMesh, DevState, DevDims, double *);
 __global__ void k_copy(double *, const double *, int);
+
+// Segments of the packed forcing upload: segment i goes to dst[i] from offset off[i].
+struct ScatterSegs {
+  double *dst[GOM_MAX_FORCING + 1];
+  long off[GOM_MAX_FORCING + 1], n[GOM_MAX_FORCING + 1];
+  int count;
+};
+
+#define CNT_F(name, ext) +1
+static_assert(0 GOMGPU_FORCING_REAL(CNT_F) <= GOM_MAX_FORCING, "GOM_MAX_FORCING below the forcing array count");
+#undef CNT_F
+
+// One block row (blockIdx.y) per segment, grid-strided over its values.
+__global__ void k_scatter(ScatterSegs s, const double *src) {
+  const int g = blockIdx.y;
+  const long n = s.n[g], o = s.off[g];
+  double *dst = s.dst[g];
+  for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x)
+    dst[i] = src[o + i];
+}
 __global__ void k_update_dz(double *, const double *, double *, const double *, const int32_t *, const int32_t *, int, int);
 __global__ void k_fill(double *, double, long);
 extern "C" hipError_t gom_launch_pcg(const PcgArgs *args, int max_blocks, int cooperative, hipStream_t st);
@@ -130,6 +150,7 @@ struct gomgpu_ctx {
   bool dz_full = true;
   int layers_since_update = 0;
   double *h_stage = nullptr;  // pinned staging of the forcing arrays
+  double *d_stage = nullptr;  // its device image: one copy, then k_scatter to the fields
   size_t stage_len = 0;
   hipEvent_t fev = nullptr;   // completion of the last forcing upload
 };
@@ -509,6 +530,7 @@ extern "C" int gomgpu_create(const gomgpu_mesh *mesh, const gomgpu_params *param
       rc = seterr(c, GOMGPU_E_NOMEM, "cannot allocate the forcing staging buffer");
       goto fail;
     }
+    if ((rc = dalloc(c, &c->d_stage, tot))) goto fail;
   }
   if (hipStreamSynchronize(c->st) != hipSuccess) {
     rc = seterr(c, GOMGPU_E_HIP, "initialisation failed");
@@ -628,15 +650,21 @@ extern "C" int gomgpu_set_forcing(gomgpu_ctx *c, const gomgpu_forcing *f) {
   c->spin_tide = f->spinup_function_tide;
   c->spin_bc = f->spinup_function_baroclinic;
   HIPCHK(c, hipEventSynchronize(c->fev));
+  // the given arrays are packed into the staging buffer, copied in one transfer and spread
+  // to their fields by one kernel (one transfer per array costs more stream time)
   size_t off = 0;
+  ScatterSegs seg;
+  seg.count = 0;
 #define UP_F(name, ext)                                                                              \
   if (f->name) {                                                                                     \
     Shape sh = shape_of(#ext, p);                                                                    \
     size_t n = (size_t)(sh.ld * sh.n);                                                               \
     if (off + n > c->stage_len) return seterr(c, GOMGPU_E_ARG, "forcing staging overflow");         \
     to_device_layout(f->name, c->h_stage + off, sh);                                                 \
-    HIPCHK(c, hipMemcpyAsync((double *)c->f.name, c->h_stage + off, n * sizeof(double),              \
-                             hipMemcpyHostToDevice, c->st));                                         \
+    seg.dst[seg.count] = (double *)c->f.name;                                                        \
+    seg.off[seg.count] = (long)off;                                                                  \
+    seg.n[seg.count] = (long)n;                                                                      \
+    seg.count++;                                                                                     \
     off += n;                                                                                        \
   }
   GOMGPU_FORCING_REAL(UP_F)
@@ -652,9 +680,18 @@ extern "C" int gomgpu_set_forcing(gomgpu_ctx *c, const gomgpu_forcing *f) {
   c->h_stage[off + 5] = f->hurricane_phase;
   c->h_stage[off + 6] = 0.0;
   c->h_stage[off + 7] = 0.0;
-  HIPCHK(c, hipMemcpyAsync((double *)c->f.scal, c->h_stage + off, GOM_NSCAL * sizeof(double), hipMemcpyHostToDevice,
+  seg.dst[seg.count] = (double *)c->f.scal;
+  seg.off[seg.count] = (long)off;
+  seg.n[seg.count] = GOM_NSCAL;
+  seg.count++;
+  long nmax = 0;
+  for (int i = 0; i < seg.count; i++) nmax = seg.n[i] > nmax ? seg.n[i] : nmax;
+  HIPCHK(c, hipMemcpyAsync(c->d_stage, c->h_stage, (off + GOM_NSCAL) * sizeof(double), hipMemcpyHostToDevice,
                            c->st));
   HIPCHK(c, hipEventRecord(c->fev, c->st));
+  long bx = (nmax + 255) / 256;
+  k_scatter<<<dim3((unsigned)(bx < 1024 ? bx : 1024), (unsigned)seg.count), 256, 0, c->st>>>(seg, c->d_stage);
+  HIPCHK(c, hipGetLastError());
   return 0;
 }
 
@@ -1450,6 +1487,13 @@ extern "C" long long gomgpu_footprint(const gomgpu_params *params) {
 #undef FP_R
 #undef FP_I
   add(GOM_NSCAL, sizeof(double));  // per-step scalars
+  {  // device image of the forcing staging buffer
+    size_t st = GOM_NSCAL;
+#define FP_S(name, ext) st += (size_t)(shape_of(#ext, p).ld * shape_of(#ext, p).n);
+    GOMGPU_FORCING_REAL(FP_S)
+#undef FP_S
+    add(st, sizeof(double));
+  }
   long E = p.maxele, J = p.maxface, P = p.maxnod, L = p.maxlayer;
   long NT = (p.transport_flag == 1 ? p.maxtran2 : 0);
   NT = NT > 0 ? NT : 1;
