@@ -12,7 +12,9 @@
  * harness dumps on the committed fixtures (channel and estuary cases, limiters on and off).
  * Building blocks are pinned independently: tridiag against numpy dense solves, PCG against
  * scipy spsolve and a literal numpy transcription of the update order on small matrices,
- * check_intersection/bilinear by closed forms, the EOS against UNESCO-1980 check values.
+ * check_intersection/bilinear by closed forms, the EOS against UNESCO-1980 check values, the
+ * linear EOS by its closed form and the stop it makes on unset face densities. The ELM
+ * scalar transport (transport_solver 2) is pinned by its own fixture, wet and dry.
  * Not pinned by any reference output: the pressure correction of the full EOS (only below
  * 100 m, absent from every fixture) -- "parity unpinned", it is checked against the
  * published UNESCO-1980 high-pressure check value only.
