@@ -167,3 +167,19 @@ struct PcgArgs {
 
 // Launch helpers
 static inline int nblk(long n, int bs) { return (int)((n + bs - 1) / bs); }
+
+// Blocks of the register-resident PCG for n rows at 512 threads. rows = 0 (automatic):
+// the fewest rows per thread (1, 2 or 4) that cover the system with at most 256 blocks,
+// which spreads a lone system's row work over the most CUs; rows = 1, 2 or 4 fixes it
+// (4 packs more batch members into one launch). A function of n and the setting alone, so
+// a system gets the same partition alone and in a batch. Above 4 x 256 x 512 rows the
+// result exceeds 256 (the memory-resident solve).
+static inline int pcg_blocks_for(long n, int rows) {
+  long r = rows;
+  if (r <= 0) {
+    r = (n + 256L * 512 - 1) / (256L * 512);
+    r = r <= 1 ? 1 : r <= 2 ? 2 : 4;
+  }
+  int nb = (int)((n + r * 512 - 1) / (r * 512));
+  return nb < 1 ? 1 : nb;
+}

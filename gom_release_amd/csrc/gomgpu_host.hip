@@ -258,7 +258,7 @@ static int validate(const gomgpu_params *p, std::string &why) {
                    "temperature result from tracer slot 2)", 1;
     if (p->maxtran2 == 2 && p->tran_id2 != 2) return why = "tracer 2 must be temperature", 1;
     if (p->heat_option == 2) return why = "heat_option 2 is not supported", 1;
-    if (p->transport_solver < 0 || p->transport_solver > 2) return why = "transport_solver must be 1 or 2", 1;
+    if (p->transport_solver < 1 || p->transport_solver > 2) return why = "transport_solver must be 1 or 2", 1;
     if (p->transport_solver == 2 && p->maxtran2 != 2)
       return why = "the ELM transport solver (transport_solver 2) carries salinity and temperature: maxtran2 "
                    "must be 2", 1;
@@ -746,22 +746,6 @@ static void pcg_args(gomgpu_ctx *c, PcgArgs &a) {
   a.n = c->d.E;
   a.maxit = c->p.max_iteration_pcg;
   a.tol = c->p.error_tolerance;
-}
-
-// Blocks of the register-resident solve for n rows at 512 threads. rows = 0 (automatic):
-// the fewest rows per thread (1, 2 or 4) that cover the system with at most 256 blocks,
-// which spreads a lone system's row work over the most CUs; rows = 1, 2 or 4 fixes it
-// (4 packs more batch members into one launch). A function of n and the context's setting
-// alone, so a system gets the same partition alone and in a batch. Above 4 x 256 x 512
-// rows the result exceeds 256 (the memory-resident solve).
-static int pcg_blocks_for(long n, int rows) {
-  long r = rows;
-  if (r <= 0) {
-    r = (n + 256L * 512 - 1) / (256L * 512);
-    r = r <= 1 ? 1 : r <= 2 ? 2 : 4;
-  }
-  int nb = (int)((n + r * 512 - 1) / (r * 512));
-  return nb < 1 ? 1 : nb;
 }
 
 static int fs_pcg(gomgpu_ctx *c) {

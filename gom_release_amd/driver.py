@@ -83,7 +83,7 @@ class HostForcing:
         f["con_at_ob"] = ob
         f["con_at_Qb"] = qb
         if c.rivers:
-            q = fz.river_discharge(c, self.julian_day)
+            q = fz.river_discharge(c, self.julian_day, self.forcing["Q_add"])
             f["Q_add"] = q
             if eta_node is not None:
                 m = self.mesh

@@ -133,35 +133,50 @@ def boundary_eta(case: Case, it: int, sp_tide: float, julian_day: Optional[float
 def tvd_boundary_concentrations(case: Case, julian_day: float, sp_baroclinic: float,
                                 ntran: int, tran_ids: Sequence[int]):
     """con_at_ob (Nt, ntran, OB) and con_at_Qb (Nt, ntran, QB), Fortran order, for the
-    sub-cycles of one step (solve_transport_equation_v19.f90:229-397)."""
+    sub-cycles of one step (solve_transport_equation_v19.f90:229-397).
+
+    Series times are shifted by reference_diff_days (:249-250, :270-271, and the river
+    block). The interpolated value is the routine's local v2, which carries over between
+    boundaries, tracers and sub-cycles in the reference's loop order (sub-cycle, then open
+    boundary, then tracer; then river cell, then tracer): a series that does not bracket
+    the time leaves the previous v2 in place, 0 at the start of the call (the reference's
+    is undefined there). A boundary without a series keeps the routine's automatic array
+    value, which is never set in the call: 0."""
     nt = max(case.trans_sub_iter, 1)
     dt2 = case.dt / nt
     nob = max(len(case.open_boundaries), 1)
     nqb = max(len(case.rivers), 1)
     ob = np.zeros((nt, max(ntran, 1), nob))
     qb = np.zeros((nt, max(ntran, 1), nqb))
+    rdd = case.reference_diff_days()
     memo = {}   # boundaries share series: one interpolation per (tracer, series, sub-cycle)
 
-    def value(tid, sid, u2):
+    def interp(tid, sid, u2):
         key = (tid, sid, u2)
         if key not in memo:
             ser = case.salt_series if tid == 1 else case.temp_series
-            v = series_value(ser[sid - 1], u2)
-            memo[key] = sp_baroclinic * (v if v is not None else 0.0)
+            memo[key] = series_value(ser[sid - 1], u2, rdd)
         return memo[key]
 
+    v2 = 0.0
     for sub in range(nt):
         u2 = (julian_day * 86400.0 - case.dt) + (sub + 1) * dt2
-        for t in range(ntran):
-            tid = tran_ids[t]
-            for i, b in enumerate(case.open_boundaries):
-                sid = b.salt_ser_id if tid == 1 else b.temp_ser_id
+        for i, b in enumerate(case.open_boundaries):
+            for t in range(ntran):
+                tid = tran_ids[t]
+                sid = b.salt_ser_id if tid == 1 else (b.temp_ser_id if tid == 2 else 0)
                 if sid > 0:
-                    ob[sub, t, i] = value(tid, sid, u2)
-            for i, r in enumerate(case.rivers):
-                sid = r.salt_ser_id if tid == 1 else r.temp_ser_id
+                    v = interp(tid, sid, u2)
+                    v2 = v2 if v is None else v
+                    ob[sub, t, i] = sp_baroclinic * v2
+        for i, r in enumerate(case.rivers):
+            for t in range(ntran):
+                tid = tran_ids[t]
+                sid = r.salt_ser_id if tid == 1 else (r.temp_ser_id if tid == 2 else 0)
                 if sid > 0:
-                    qb[sub, t, i] = value(tid, sid, u2)
+                    v = interp(tid, sid, u2)
+                    v2 = v2 if v is None else v
+                    qb[sub, t, i] = sp_baroclinic * v2
     return ob.ravel(order="F"), qb.ravel(order="F")
 
 
@@ -226,11 +241,18 @@ def wind_stress(mesh, u: float, v: float, rho_a: float = 1.29, rho_o: float = 10
     return np.full(n, gt), gt * wn * spinup_wind, gt * wt * spinup_wind
 
 
-def river_discharge(case: Case, julian_day: float):
-    """Q_add per river cell (calculate_Q.f90:12-30, linear interpolation)."""
+def river_discharge(case: Case, julian_day: float, prev=None):
+    """Q_add per river cell (calculate_Q.f90:21-45, q_interp_method 1): the series at
+    julian_day shifted by reference_diff_days, times Q_portion. Q_add is a module array:
+    a cell whose series does not bracket the time keeps its previous value (`prev`, zero
+    before the first call, as allocate_variables leaves it)."""
     q = np.zeros(max(len(case.rivers), 1))
+    if prev is not None:
+        q[:] = np.asarray(prev, dtype=np.float64)[:q.size]
     u2 = julian_day * 86400.0
+    rdd = case.reference_diff_days()
     for i, r in enumerate(case.rivers):
-        v = series_value(case.q_series[r.q_ser_id - 1], u2, case.reference_diff_days())
-        q[i] = (v if v is not None else 0.0) * r.portion
+        v = series_value(case.q_series[r.q_ser_id - 1], u2, rdd)
+        if v is not None:
+            q[i] = v * r.portion
     return q

@@ -1031,6 +1031,10 @@ def unsupported(pr: Project) -> List[str]:
         (m.hurricane_flag != 0 and "hurricane_ser.inp" not in pr.present,
          "C31 hurricane fields: input/hurricane_ser.inp is missing"),
         (m.holland_flag != 0, "C32 Holland storm surge"),
+        # read_q_ser.f90:46 leaves the last series' method in the global q_interp_method,
+        # which calculate_Q.f90:21,46,92 branches on; only linear (1) is carried here
+        (bool(pr.q_series) and pr.q_series[-1].interp != 1,
+         "C22 river discharge interpolation other than linear (q_interp_method 2 or 3)"),
     ]
     for bad, what in checks:
         if bad:

@@ -74,6 +74,18 @@ def lib():
         L.gomgpu_batch_last_error.restype = C.c_char_p
         L.gomgpu_batch_destroy.argtypes = [vp]
         L.gomgpu_batch_destroy.restype = None
+        L.gomgpu_pcg_solve.argtypes = [C.c_int, C.c_int, vp, vp, vp, vp, vp, C.c_int, C.c_double,
+                                       C.POINTER(C.c_int), vp]
+        L.gomgpu_pcg_workspace_bytes.argtypes = [C.c_int]
+        L.gomgpu_pcg_workspace_bytes.restype = C.c_longlong
+        L.gomgpu_pcg_last_error.argtypes = []
+        L.gomgpu_pcg_last_error.restype = C.c_char_p
+        # device memory for the entry points that take device pointers, through the HIP
+        # runtime the library itself links
+        L.hipMalloc.argtypes = [C.POINTER(vp), C.c_size_t]
+        L.hipFree.argtypes = [vp]
+        L.hipMemcpy.argtypes = [vp, vp, C.c_size_t, C.c_int]
+        L.hipDeviceSynchronize.argtypes = []
         _lib = L
     return _lib
 
@@ -85,7 +97,10 @@ EXPORTED = ["gomgpu_create", "gomgpu_set_state", "gomgpu_set_forcing", "gomgpu_s
             "gomgpu_footprint",
             "gomgpu_device_bytes", "gomgpu_batch_create", "gomgpu_batch_add", "gomgpu_batch_remove",
             "gomgpu_batch_size", "gomgpu_batch_step", "gomgpu_batch_sync", "gomgpu_batch_last_error",
-            "gomgpu_batch_destroy"]
+            "gomgpu_batch_destroy", "gomgpu_pcg_solve", "gomgpu_pcg_workspace_bytes",
+            "gomgpu_pcg_last_error"]
+
+HIP_H2D, HIP_D2H = 1, 2
 
 
 def footprint(params: "abi.Params") -> int:
@@ -101,6 +116,60 @@ def device_count() -> int:
     n = C.c_int(0)
     rc = lib().hipGetDeviceCount(C.byref(n))
     return n.value if rc == 0 else 0
+
+
+class DeviceArray:
+    """A numpy array copied into device memory (hipMalloc of the library's HIP runtime);
+    `ptr` is the device address, `download()` copies it back."""
+
+    def __init__(self, a: np.ndarray):
+        self.host = np.ascontiguousarray(a)
+        self._L = lib()
+        self.ptr = C.c_void_p()
+        nbytes = max(self.host.nbytes, 1)
+        if self._L.hipMalloc(C.byref(self.ptr), nbytes) != 0:
+            raise GomGpuError(3, f"hipMalloc of {nbytes} bytes failed")
+        if self.host.nbytes and self._L.hipMemcpy(self.ptr, self.host.ctypes.data_as(C.c_void_p),
+                                                  self.host.nbytes, HIP_H2D) != 0:
+            raise GomGpuError(2, "hipMemcpy to the device failed")
+
+    def download(self) -> np.ndarray:
+        out = np.empty_like(self.host)
+        if out.nbytes and self._L.hipMemcpy(out.ctypes.data_as(C.c_void_p), self.ptr, out.nbytes, HIP_D2H) != 0:
+            raise GomGpuError(2, "hipMemcpy from the device failed")
+        return out
+
+    def free(self) -> None:
+        if self.ptr:
+            self._L.hipFree(self.ptr)
+            self.ptr = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+
+def pcg_solve(vals: np.ndarray, cols: np.ndarray, rowstr: np.ndarray, b: np.ndarray, x0: np.ndarray,
+              max_iteration: int, tolerance: float):
+    """pre_conj_grad (pre_conj_grad.f90:9) through gomgpu_pcg_solve: the 1-based CSR system
+    is copied to the device, solved there, and (return code, iterations, x) returned; the
+    code is 0, or 5 (GOMGPU_W_PCG_NOT_CONVERGED); other failures raise."""
+    n = int(b.size)
+    bufs = [DeviceArray(np.asarray(vals, dtype=np.float64)), DeviceArray(np.asarray(cols, dtype=np.int32)),
+            DeviceArray(np.asarray(rowstr, dtype=np.int32)), DeviceArray(np.asarray(b, dtype=np.float64)),
+            DeviceArray(np.asarray(x0, dtype=np.float64))]
+    try:
+        it = C.c_int(0)
+        rc = lib().gomgpu_pcg_solve(n, int(np.asarray(vals).size), *[d.ptr for d in bufs], int(max_iteration),
+                                    float(tolerance), C.byref(it), None)
+        if rc not in (0, 5):
+            raise GomGpuError(rc, lib().gomgpu_pcg_last_error().decode())
+        return rc, it.value, bufs[4].download()
+    finally:
+        for d in bufs:
+            d.free()
 
 
 class GpuModel:

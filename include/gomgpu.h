@@ -333,6 +333,26 @@ int gomgpu_set_pcg_rows(gomgpu_ctx *ctx, int rows);
 long long gomgpu_footprint(const gomgpu_params *params);
 long long gomgpu_device_bytes(gomgpu_ctx *ctx);
 
+/* ---------------------------------------------------------------------------------------
+ * Standalone free-surface solve: replaces pre_conj_grad(na, nz, A, colidx, rowstr, b, x)
+ * (pre_conj_grad.f90:9, called at solve_free_surface_equation.f90:321). Jacobi-
+ * preconditioned CG on a 1-based CSR matrix (A, colidx: nz entries; rowstr: na+1 entries),
+ * right-hand side b and initial guess x (overwritten with the solution), stopping when
+ * sqrt(delta_new^2 / delta_0^2) < error_tolerance (pre_conj_grad.f90:199) or after
+ * max_iteration iterations. All five arrays are DEVICE pointers (the matrix of a caller that
+ * assembles on the GPU). Rows must be free-surface rows: one diagonal entry and at most 4
+ * off-diagonal entries (solve_free_surface_equation.f90:233-258); any other matrix is refused
+ * with GOMGPU_E_ARG. stream is a hipStream_t (NULL: the null stream). The call returns after
+ * the solve; *iterations receives the reference's cgit2 (0 when it did not converge, with
+ * GOMGPU_W_PCG_NOT_CONVERGED returned instead of the reference's silence). The workspace
+ * (gomgpu_pcg_workspace_bytes) is allocated and freed inside the call. gomgpu_pcg_last_error
+ * returns the text of the calling thread's last failure. */
+int gomgpu_pcg_solve(int na, int nz, const double *A, const int32_t *colidx, const int32_t *rowstr,
+                     const double *b, double *x, int max_iteration, double error_tolerance, int *iterations,
+                     void *stream);
+long long gomgpu_pcg_workspace_bytes(int na);
+const char *gomgpu_pcg_last_error(void);
+
 typedef struct gomgpu_batch gomgpu_batch;
 /* Batched contexts: one call steps every member. Destroying a member context removes it from
  * its batches; gomgpu_batch_destroy leaves the members usable on their own. */

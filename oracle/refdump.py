@@ -114,6 +114,8 @@ def params_from_geom(g: Dict[str, np.ndarray]):
     for k in ("rho_a", "wind_formular", "hurricane_flag", "hurricane_read_interval", "transport_solver"):
         if k in g:
             d[k] = g[k]
+    if d["transport_flag"] == 1 and "transport_solver" not in d:
+        d["transport_solver"] = 1   # fixtures dumped before the card was read: the TVD solver
     if d.get("transport_solver") == 2 and d["transport_flag"] == 1:
         # the ABI's convention for the ELM solver, which carries both tracers
         d.update(maxtran2=2, tran_id1=1, tran_id2=2)

@@ -68,6 +68,12 @@ class Golden:
             out[k] = _flat(v)
         return out
 
+    def pcg_iterations(self, it: int):
+        """The reference's PCG iteration count of step `it` (cgit2, pre_conj_grad.f90:
+        224-229), when the fixture recorded it."""
+        v = self.meta.get("pcg_iterations", {}).get(str(it))
+        return None if v is None else int(v)
+
     def case(self):
         return case_of(self.meta)
 

@@ -94,7 +94,10 @@ def _cases():
     ts.harmonics = []
     ts.eta_series = [gomio.Series([0.0, 0.5, 1.0], [0.1, -0.2, 0.3], 86400.0, 3.0, 0.01, 1.5),
                      gomio.Series([-1.0, 40.0], [0.0, 0.25], 3600.0, 0.0, 1.0, 0.0)]
-    ts.q_series = [gomio.Series([0.0, 10.0], [20.0, 30.0], 86400.0, 0.0, 1.0, 0.0, 2)]
+    # the interpolation column round-trips; the last series read sets the reference's global
+    # q_interp_method (read_q_ser.f90:46), which stays linear
+    ts.q_series = [gomio.Series([0.0, 10.0], [20.0, 30.0], 86400.0, 0.0, 1.0, 0.0, 2),
+                   gomio.Series([0.0, 10.0], [5.0, 6.0], 86400.0, 0.0, 1.0, 0.0, 1)]
     ts.start = (2012, 3, 5, 6, 30, 0)
     ts.data_start_year, ts.data_time_shift = 2010, 3
     ts.gravity, ts.rho_o, ts.rho_a, ts.max_no_neighbor_node = 9.80665, 1024.5, 1.225, 12

@@ -3,7 +3,8 @@
 The fixture tests start every step from an uploaded state. Here the device steps on by
 itself, so the update stage exchanges the tracer pairs and the next transport stage meets
 them stale: a dry column then receives the old values (the reference copies new <- old in
-update_variables.f90 and the ELM solver leaves dry columns alone). A small estuary with a
+update_variables.f90 and the ELM solver leaves dry columns alone). The device state is
+read back only every few steps, since a read-back makes every pair current. A small estuary with a
 river and banks that dry from step 43 (20 C, 5 psu river), 46 steps through the host
 driver, against the oracle driven the same way; the tolerances of the other multi-step comparisons (the free-surface PCG
 sums in another order).
@@ -39,18 +40,25 @@ def test_gpu_elm_transport_steps_match_oracle(gpu_available):
     cpu = Scenario(case, wind=(6.0, 2.0), model_factory=OracleModel)
     names = ["eta_cell", "un_face", "salt_cell", "temp_cell", "salt_cell_new", "temp_cell_new", "salt_node",
              "temp_node", "salt_face", "temp_face", "rho_cell", "top_layer_at_element"]
-    dry_seen = 0
+    # The device state is read back only at these steps: a read-back copies every exchanged
+    # pair (old into new), so the steps between two reads, the drying steps 43-45 among
+    # them, run with stale pairs and the transport's copy into dry columns.
+    checks = (15, 30, 42, 46)
+    dry_unread = 0
     for it in range(1, case.ndt + 1):
         if it % 10 == 1:
             print(f"[gpu-elm] step {it}", flush=True)
         gpu.step()
         cpu.step()
-        a, b = gpu.fields(names), cpu.fields(names)
-        dry_seen += int(np.count_nonzero(b["top_layer_at_element"] == 0))
+        b = cpu.fields(names)
+        if it not in checks:
+            dry_unread += int(np.count_nonzero(b["top_layer_at_element"] == 0))
+            continue
+        a = gpu.fields(names)
         for k in names:
             if a[k].dtype != np.float64:
                 assert np.array_equal(a[k], b[k]), (it, k)
             else:
                 np.testing.assert_allclose(a[k], b[k], rtol=1e-7, atol=1e-10, err_msg=f"step {it} {k}")
-    assert dry_seen > 0          # the dry-column path ran
+    assert dry_unread > 0        # dry columns met stale pairs between two read-backs
     gpu.model.close()

@@ -85,3 +85,36 @@ def test_gpu_whole_step_matches_reference(gpu_available, name):
                 else:
                     np.testing.assert_allclose(got, ref, rtol=1e-8, atol=1e-11, err_msg=f"{name} {it} {field}")
     model.close()
+
+
+@pytest.mark.parametrize("name", FIXTURES)
+def test_gpu_freesurf_memory_resident_pcg(gpu_available, name, monkeypatch):
+    """The memory-resident solve k_pcg_cg1m (the one meshes above 524,288 cells and large
+    batch members run) forced on every fixture: eta within the free-surface tolerance, the
+    wet/dry node flags exact, and the reference's iteration count when the fixture holds it."""
+    monkeypatch.setenv("GOMGPU_PCG_MODE", "4")
+    g = Golden(name)
+    case = g.case()
+    model = _model(g)
+    from gom_release_amd import abi
+    for it in g.steps:
+        print(f"[gpu-cg1m] {name} step {it}", flush=True)
+        state = g.state_in(it)
+        for stage, tag in STAGES[:3]:
+            apply(state, g.expected(it, tag))
+        model.set_forcing(g.forcing(it, case))
+        model.set_state(state)
+        model.stage("freesurf")
+        model.sync()
+        out = abi.state_arrays(g.params)
+        model.get_state(out)
+        exp = g.expected(it, "fs")
+        for field in ("eta_cell_new", "eta_node"):
+            np.testing.assert_allclose(out[field], exp[field], rtol=FS_RTOL, atol=FS_ATOL,
+                                       err_msg=f"{name} step {it} {field}")
+        if "wetdry_node" in exp:
+            assert np.array_equal(out["wetdry_node"], exp["wetdry_node"]), (name, it)
+        ref_it = g.pcg_iterations(it)
+        if ref_it is not None:
+            assert model.pcg_iterations() == ref_it, (name, it, model.pcg_iterations(), ref_it)
+    model.close()
