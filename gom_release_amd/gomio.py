@@ -129,6 +129,8 @@ class Case:
     kv0: float = 1e-4
     max_iteration_pcg: int = 1000
     error_tolerance: float = 1e-10
+    pcg_result_show: int = 0       # C6: 1 logs the PCG iteration count (pre_conj_grad.f90:224-229)
+    ishow_frequency: int = 99999999  # C4: ... every ishow_frequency steps
     transport_flag: int = 1
     transport_solver: int = 1      # 1 TVD (solve_transport_equation_v19), 2 ELM (_ELM_v8)
     trans_sub_iter: int = 1
@@ -325,7 +327,7 @@ def _main_inp(c: Case) -> str:
     dsy = y if c.data_start_year is None else int(c.data_start_year)
     data(c.dt, c.ndt, dsy, int(c.data_time_shift), y, mo, d, h, mi, s)
     card("C4\trestart_in restart_out restart_freq ishow ishow_frequency")
-    data(int(c.restart_in), int(c.restart_out), int(c.restart_freq), 0, 99999999)
+    data(int(c.restart_in), int(c.restart_out), int(c.restart_freq), 0, int(c.ishow_frequency))
     card("C5_01\ttheta advection_flag adv_onoff_depth ELM_backtrace_flag ELM_sub_iter ELM_min_iter ELM_max_iter")
     data(c.theta, c.advection_flag, c.adv_onoff_depth, c.elm_backtrace_flag, c.elm_sub_iter,
          c.elm_min_iter, c.elm_max_iter)
@@ -336,7 +338,7 @@ def _main_inp(c: Case) -> str:
     card("C5_04\tSmagorinsky_parameter Ah_0 Kh_0 Av_0 Kv_0")
     data(c.smagorinsky, c.ah0, c.kh0, c.av0, c.kv0)
     card("C6\tmax_iteration_pcg error_tolerance pcg_result_show")
-    data(c.max_iteration_pcg, c.error_tolerance, 0)
+    data(c.max_iteration_pcg, c.error_tolerance, int(c.pcg_result_show))
     card("C7\ttransport_flag transport_solver")
     data(c.transport_flag, int(c.transport_solver))
     if c.transport_flag == 1:
@@ -1073,6 +1075,7 @@ def case_from_project(pr: Project, strict: bool = True) -> Tuple[Case, List[str]
     c.coriolis_option = m.Coriolis_option
     c.smagorinsky, c.ah0, c.kh0, c.av0, c.kv0 = m.smagorinsky_parameter, m.Ah_0, m.Kh_0, m.Av_0, m.Kv_0
     c.max_iteration_pcg, c.error_tolerance = m.max_iteration_pcg, m.error_tolerance
+    c.pcg_result_show, c.ishow_frequency = m.pcg_result_show, m.ishow_frequency
     c.transport_flag = m.transport_flag
     c.transport_solver = m.transport_solver if m.transport_flag == 1 else 1
     if m.transport_flag == 1:

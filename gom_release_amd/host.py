@@ -152,18 +152,20 @@ class DeviceArray:
 
 
 def pcg_solve(vals: np.ndarray, cols: np.ndarray, rowstr: np.ndarray, b: np.ndarray, x0: np.ndarray,
-              max_iteration: int, tolerance: float):
-    """pre_conj_grad (pre_conj_grad.f90:9) through gomgpu_pcg_solve: the 1-based CSR system
-    is copied to the device, solved there, and (return code, iterations, x) returned; the
-    code is 0, or 5 (GOMGPU_W_PCG_NOT_CONVERGED); other failures raise."""
+              max_iteration: int, tolerance: float, noalloc: bool = False):
+    """pre_conj_grad (pre_conj_grad.f90:9) through gomgpu_pcg_solve (or, with `noalloc`,
+    gomgpu_pcg_solve_noalloc): the 1-based CSR system is copied to the device, solved there,
+    and (return code, iterations, x) returned; the code is 0, or 5
+    (GOMGPU_W_PCG_NOT_CONVERGED); other failures raise."""
     n = int(b.size)
     bufs = [DeviceArray(np.asarray(vals, dtype=np.float64)), DeviceArray(np.asarray(cols, dtype=np.int32)),
             DeviceArray(np.asarray(rowstr, dtype=np.int32)), DeviceArray(np.asarray(b, dtype=np.float64)),
             DeviceArray(np.asarray(x0, dtype=np.float64))]
     try:
         it = C.c_int(0)
-        rc = lib().gomgpu_pcg_solve(n, int(np.asarray(vals).size), *[d.ptr for d in bufs], int(max_iteration),
-                                    float(tolerance), C.byref(it), None)
+        fn = lib().gomgpu_pcg_solve_noalloc if noalloc else lib().gomgpu_pcg_solve
+        rc = fn(n, int(np.asarray(vals).size), *[d.ptr for d in bufs], int(max_iteration), float(tolerance),
+                C.byref(it), None)
         if rc not in (0, 5):
             raise GomGpuError(rc, lib().gomgpu_pcg_last_error().decode())
         return rc, it.value, bufs[4].download()

@@ -77,7 +77,9 @@ program gom_ref_harness
    character(len=256) :: dumpdir, arg
    integer :: first_dump, last_dump, nextra, i, j, k
    integer, allocatable :: extra(:)
-   logical :: dumping, dump_now
+   logical :: dumping, dump_now, no_heat
+   character(len=8) :: envv
+   integer :: envlen, envst
    real(dp) :: t0, tstage(12)
    character(len=12), parameter :: stage_name(12) = [character(len=12) :: &
       'forcing', 'elm', 'momentum', 'freesurface', 'velocities', 'layers', &
@@ -98,6 +100,12 @@ program gom_ref_harness
       call get_command_argument(3+i, arg); read(arg,*) extra(i)
    end do
    tstage = 0.0_dp
+   ! GOM_HARNESS_NO_HEAT=1 leaves out the surface heat model (heat_term_by_term reads the
+   ! OpenMP-private copy of its vapour pressure ea before setting it, so its flux is
+   ! undefined): phi_sz then keeps its allocation value 0 and the transport adds no heat.
+   no_heat = .false.
+   call get_environment_variable('GOM_HARNESS_NO_HEAT', envv, envlen, envst)
+   if (envst == 0 .and. envlen > 0) no_heat = envv(1:1) == '1'
 
    ! start-up, as GOM_Rv1.1.0/source/main.f90:12-47
    call omp_set_nested(.true.)
@@ -187,7 +195,7 @@ program gom_ref_harness
       call lap(7)
       if (dump_now) call stage_dump('nodev')
 
-      if (transport_flag == 1 .and. is_temp == 1) then
+      if (transport_flag == 1 .and. is_temp == 1 .and. .not. no_heat) then
          if (heat_option == 1) then
             call heat_term_by_term
          else if (heat_option == 2) then

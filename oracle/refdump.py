@@ -52,8 +52,9 @@ def harness_available() -> bool:
 
 def run_reference(project: str, dumpdir: Optional[str], first: int = 1, last: int = 0,
                   extra: Sequence[int] = (), threads: int = 1, timeout: float = 3600.0,
-                  perturb: int = 255) -> str:
+                  perturb: int = 255, no_heat: bool = False) -> str:
     """Run the reference harness in `project`. With dumpdir None it only times the run.
+    With `no_heat` the harness leaves out the surface heat model (phi_sz stays 0).
 
     MALLOC_PERTURB_=255 makes every heap allocation start zero-filled, so the automatic
     arrays the reference reads before writing (the TVD limiter block) read zeros.
@@ -63,6 +64,8 @@ def run_reference(project: str, dumpdir: Optional[str], first: int = 1, last: in
     env["MALLOC_PERTURB_"] = str(perturb)
     env["GLIBC_TUNABLES"] = "glibc.malloc.tcache_count=0"
     env.setdefault("OMP_STACKSIZE", "512M")
+    if no_heat:
+        env["GOM_HARNESS_NO_HEAT"] = "1"
     if dumpdir is not None:
         os.makedirs(dumpdir, exist_ok=True)
         args = [HARNESS, os.path.abspath(dumpdir), str(first), str(last)] + [str(e) for e in extra]

@@ -81,7 +81,56 @@ CASES = {
                               args=dict(ncol=6, nrep=2, nlayer=3, ndt=8, dry_land=True,
                                         tide_phase=-90.0, river_q=30.0),
                               hurricane=True, steps=[1, 2, 3, 4, 5, 6, 7, 8]),
+    # salinity and temperature (stratified; heat_option 1 with an air series, run with the
+    # heat model left out, so phi_sz = 0) through the van Leer limiter horizontally and
+    # superbee vertically, with the log-law bottom friction (bf_flag 2)
+    "estuary_limiters": dict(kind="estuary",
+                             args=dict(ncol=10, nrep=2, nlayer=5, ndt=4, dry_land=True, tide_phase=-90.0,
+                                       river_q=40.0, h_flux_limiter=2, v_flux_limiter=3, bf_flag=2,
+                                       bf_height=0.003),
+                             stratify=dict(d_temp=6.0, d_salt=2.0, t_bottom=12.0), air_series=True,
+                             steps=[1, 2, 4]),
+    # the other pairing: superbee horizontally, van Leer vertically, salinity alone
+    "estuary_superbee": dict(kind="estuary",
+                             args=dict(ncol=10, nrep=2, nlayer=5, ndt=4, dry_land=True, tide_phase=-90.0,
+                                       river_q=40.0, h_flux_limiter=3, v_flux_limiter=2),
+                             steps=[1, 2, 4]),
 }
+
+# Configuration 1 at its own size, run for its 1,440 steps: only the state after steps 10
+# and 1440 is kept (the per-stage fixtures above cover the stages).
+RUNS = {
+    "channel_64x32x10": dict(kind="channel", args=dict(nx=64, ny=32, nlayer=10, ndt=1440),
+                             keep=[10, 1440]),
+}
+RUN_FIELDS = ["eta_cell", "eta_node", "un_face", "vn_face", "wn_cell", "salt_cell", "temp_cell",
+              "top_layer_at_element", "top_layer_at_face", "u_node"]
+
+# Air series for the heat model (read_air_ser.f90: count, time factor, time shift, then
+# time T_air T_dew wind solar cloud rain evaporation per line), constant over the run.
+AIR_SERIES = """! air_ser.inp: air_data_num air_time_conv air_time_adjust, then per line
+! time T_air T_dew air_wind_speed solar_radiation cloud rain evaporation
+2  86400.0  0.0
+-1000.0  26.0  16.0  4.0  450.0  0.3  0.0  0.0
+100000.0  26.0  16.0  4.0  450.0  0.3  0.0  0.0
+"""
+
+
+def write_extra_inputs(spec, project: str) -> None:
+    if spec.get("air_series"):
+        with open(os.path.join(project, "input", "air_ser.inp"), "w") as f:
+            f.write(AIR_SERIES)
+
+
+def pcg_counts(project: str):
+    """{step: cgit2} from the run log (pre_conj_grad.f90:224-229, pcg_result_show = 1)."""
+    import re
+    out = {}
+    for ln in open(os.path.join(project, "output", "run.log")):
+        m = re.search(r"time step =\s*(\d+), pre_conj_grad.f90 converged at:\s*(-?\d+)", ln)
+        if m:
+            out[int(m.group(1))] = int(m.group(2))
+    return out
 
 
 def make_case(spec):
@@ -102,11 +151,15 @@ def check_outputs(prev, cur, stage):
 
 def make(name: str, spec, outdir: str) -> str:
     case = make_case(spec)
+    case.pcg_result_show, case.ishow_frequency = 1, 1
     steps = spec["steps"]
     with tempfile.TemporaryDirectory() as tmp:
         gomio.write_project(case, tmp)
+        write_extra_inputs(spec, tmp)
         dump = os.path.join(tmp, "dump")
-        refdump.run_reference(tmp, dump, steps[0], steps[0], extra=steps[1:] + [s - 1 for s in steps if s > 1])
+        refdump.run_reference(tmp, dump, steps[0], steps[0], extra=steps[1:] + [s - 1 for s in steps if s > 1],
+                              no_heat=bool(spec.get("air_series")))
+        counts = pcg_counts(tmp)
         out = {}
         g = refdump.read_dump(os.path.join(dump, "geom.bin"))
         for k, v in g.items():
@@ -124,10 +177,39 @@ def make(name: str, spec, outdir: str) -> str:
                 prev = cur
     path = os.path.join(outdir, name + ".npz")
     np.savez_compressed(path, **out)
-    meta = dict(case=name, kind=spec["kind"], args=spec["args"], steps=steps)
-    for flag in ("hurricane", "elm_tracers"):
+    meta = dict(case=name, kind=spec["kind"], args=spec["args"], steps=steps,
+                pcg_iterations={str(it): counts[it] for it in steps})
+    for flag in ("hurricane", "elm_tracers", "air_series"):
         if spec.get(flag):
             meta[flag] = True
+    if spec.get("stratify"):
+        meta["stratify"] = spec["stratify"]
+    with open(os.path.join(outdir, name + ".json"), "w") as f:
+        json.dump(meta, f, indent=1)
+    return path
+
+
+def make_run(name: str, spec, outdir: str) -> str:
+    """A whole run of a case from its cold start: the state after each step in spec["keep"]
+    and the reference's PCG iteration count of every step."""
+    case = make_case(spec)
+    case.pcg_result_show, case.ishow_frequency = 1, 1
+    keep = spec["keep"]
+    out = {}
+    with tempfile.TemporaryDirectory() as tmp:
+        gomio.write_project(case, tmp)
+        write_extra_inputs(spec, tmp)
+        dump = os.path.join(tmp, "dump")
+        refdump.run_reference(tmp, dump, keep[-1], keep[-1], extra=keep[:-1])
+        counts = pcg_counts(tmp)
+        for it in keep:
+            d = refdump.read_dump(os.path.join(dump, f"s{it}_post.bin"))
+            for k in RUN_FIELDS:
+                out[f"s{it}_post/{k}"] = d[k]
+    path = os.path.join(outdir, name + ".npz")
+    np.savez_compressed(path, **out)
+    meta = dict(case=name, kind=spec["kind"], args=spec["args"], keep=keep,
+                pcg_iterations={str(it): counts[it] for it in sorted(counts)})
     with open(os.path.join(outdir, name + ".json"), "w") as f:
         json.dump(meta, f, indent=1)
     return path
@@ -198,6 +280,10 @@ if __name__ == "__main__":
     for name, spec in CASES.items():
         if only is None or name in only:
             p = make(name, spec, outdir)
+            print(name, os.path.getsize(p) // 1024, "KiB")
+    for name, spec in RUNS.items():
+        if only is None or name in only:
+            p = make_run(name, spec, outdir)
             print(name, os.path.getsize(p) // 1024, "KiB")
     if only is None or "formats_estuary" in only:
         p = make_formats(outdir)

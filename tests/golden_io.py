@@ -13,7 +13,8 @@ GOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
 STAGES = [("friction", "pre"), ("elm", "elm"), ("momentum", "mom"), ("freesurf", "fs"),
           ("velocity", "vel"), ("layers", "lay"), ("nodevel", "nodev"), ("transport", "tvd"),
           ("eos", "eos"), ("update", "post")]
-FIXTURES = ["channel_16x8x5", "estuary_wetdry", "hurricane_estuary", "elm_estuary"]
+FIXTURES = ["channel_16x8x5", "estuary_wetdry", "hurricane_estuary", "elm_estuary", "estuary_limiters",
+            "estuary_superbee"]
 
 
 def _flat(a):
@@ -111,6 +112,8 @@ def case_of(spec):
         case = meshgen.estuary_case(**spec["args"])
     if spec.get("elm_tracers"):
         meshgen.elm_tracers(case, t=0.0, temp_input=False)
+    if spec.get("stratify"):
+        meshgen.stratify(case, **spec["stratify"])
     if spec.get("hurricane"):
         # a Holland cyclone (950 hPa, r_max 300 m) crossing the small estuary at 5 m/s,
         # fields every third step; the model starts at 23:53 so the window opens at step 4

@@ -46,6 +46,8 @@ def test_gpu_stages_match_reference(gpu_available, name):
             model.set_state(state)
             model.stage(stage)
             model.sync()
+            if stage == "freesurf" and g.pcg_iterations(it) is not None:
+                assert model.pcg_iterations() == g.pcg_iterations(it), (name, it, model.pcg_iterations())
             out = abi.state_arrays(g.params)
             model.get_state(out)
             for field, ref in g.expected(it, tag).items():

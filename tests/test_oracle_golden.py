@@ -21,6 +21,9 @@ def test_oracle_stages_bitwise(name):
         for stage, tag in STAGES:
             o = Oracle(g.mesh, g.params, state, forcing)
             assert o.run(stage) == 0, (name, it, stage)
+            if stage == "freesurf" and g.pcg_iterations(it) is not None:
+                # the reference's cgit2 (pre_conj_grad.f90:224-229)
+                assert Oracle.pcg_iterations() == g.pcg_iterations(it), (name, it)
             for field, ref in g.expected(it, tag).items():
                 if field not in state.arr:
                     continue
