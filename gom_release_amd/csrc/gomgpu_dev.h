@@ -123,6 +123,64 @@ __device__ __forceinline__ double carea(double x1, double x2, double x3, double 
   return fabs((x1 - x3) * (y2 - y3) - (x2 - x3) * (y1 - y3)) / 2.0;
 }
 
+// ---- correctly rounded natural logarithm (host and device)
+// The reference evaluates log with the host C library, whose log is correctly rounded in
+// all but vanishingly rare cases; the device library's log is not, and differs from it in
+// the last bit for some arguments. gom_log computes log(x) to about 2^-100 relative in
+// double-double arithmetic and rounds once, so it returns the correctly rounded value.
+// Method: x = m 2^e with m in [sqrt(1/2), sqrt(2)), log x = e ln2 + 2 atanh(s),
+// s = (m - 1)/(m + 1), |s| <= 0.1716, atanh(s)/s = sum_k s^(2k)/(2k+1) to k = 22.
+struct GomDD {
+  double hi, lo;
+};
+__host__ __device__ inline GomDD dd_two_sum(double a, double b) {
+  double s = a + b, bb = s - a;
+  return {s, (a - (s - bb)) + (b - bb)};
+}
+__host__ __device__ inline GomDD dd_quick(double a, double b) {  // |a| >= |b|
+  double s = a + b;
+  return {s, b - (s - a)};
+}
+__host__ __device__ inline GomDD dd_add(GomDD a, GomDD b) {
+  GomDD s = dd_two_sum(a.hi, b.hi), t = dd_two_sum(a.lo, b.lo);
+  s.lo = s.lo + t.hi;
+  s = dd_quick(s.hi, s.lo);
+  s.lo = s.lo + t.lo;
+  return dd_quick(s.hi, s.lo);
+}
+__host__ __device__ inline GomDD dd_mul(GomDD a, GomDD b) {
+  double p = a.hi * b.hi;
+  double e = fma(a.hi, b.hi, -p);
+  e = e + (a.hi * b.lo + a.lo * b.hi);
+  return dd_quick(p, e);
+}
+__host__ __device__ inline GomDD dd_div(GomDD a, GomDD b) {
+  double q1 = a.hi / b.hi;
+  GomDD r = dd_add(a, dd_mul(b, GomDD{-q1, 0.0}));
+  double q2 = r.hi / b.hi;
+  r = dd_add(r, dd_mul(b, GomDD{-q2, 0.0}));
+  double q3 = r.hi / b.hi;
+  return dd_add(dd_quick(q1, q2), GomDD{q3, 0.0});
+}
+__host__ __device__ inline double gom_log(double x) {
+  if (!(x > 0.0) || x == x + x || x < 2.2250738585072014e-308) return log(x);  // 0, <0, inf, NaN, subnormal
+  int e;
+  double m = frexp(x, &e);  // m in [0.5, 1)
+  if (m < 0.70710678118654752) {
+    m = m + m;
+    e = e - 1;
+  }
+  const GomDD s = dd_div(GomDD{m - 1.0, 0.0}, dd_two_sum(m, 1.0));  // m - 1 is exact
+  const GomDD s2 = dd_mul(s, s);
+  GomDD sum = dd_div(GomDD{1.0, 0.0}, GomDD{45.0, 0.0});
+  for (int k = 21; k >= 0; k--) sum = dd_add(dd_mul(sum, s2), dd_div(GomDD{1.0, 0.0}, GomDD{2.0 * k + 1.0, 0.0}));
+  GomDD lm = dd_mul(s, sum);
+  lm = GomDD{2.0 * lm.hi, 2.0 * lm.lo};
+  const GomDD ln2{0x1.62e42fefa39efp-1, 0x1.abc9e3b39803fp-56};
+  const GomDD r = dd_add(dd_mul(ln2, GomDD{(double)e, 0.0}), lm);
+  return r.hi + r.lo;
+}
+
 __device__ __forceinline__ double dmaxf(double a, double b) { return a > b ? a : b; }
 __device__ __forceinline__ double dminf(double a, double b) { return a < b ? a : b; }
 

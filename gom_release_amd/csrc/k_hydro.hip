@@ -21,7 +21,8 @@ __global__ void k_friction(DevMesh m, DevState s, DevDims d, int bf_flag, double
     if (bf_flag == 1) {
       cdb = m.Cdb_static[j];
     } else {
-      double lg = log(dmaxf(0.5, A0(s.dzhalf_face, k - 1, j, J)) / m.bf_height[j]);
+      // the reference's host log (correctly rounded), not the device library's
+      double lg = gom_log(dmaxf(0.5, A0(s.dzhalf_face, k - 1, j, J)) / m.bf_height[j]);
       cdb = vk * vk / (lg * lg);
     }
     double u = AL(s.un_face, k, j, J), v = AL(s.vn_face, k, j, J);
@@ -29,6 +30,11 @@ __global__ void k_friction(DevMesh m, DevState s, DevDims d, int bf_flag, double
   }
   s.Cdb[j] = cdb;
   s.Gamma_B[j] = gb;
+}
+
+// gom_log on the host, for checking it against the C library (test use).
+extern "C" void gom_log_array(const double *x, double *y, long n) {
+  for (long i = 0; i < n; i++) y[i] = gom_log(x[i]);
 }
 
 // ================================================================ ELM
