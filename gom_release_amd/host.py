@@ -76,6 +76,7 @@ def lib():
         L.gomgpu_batch_destroy.restype = None
         L.gomgpu_pcg_solve.argtypes = [C.c_int, C.c_int, vp, vp, vp, vp, vp, C.c_int, C.c_double,
                                        C.POINTER(C.c_int), vp]
+        L.gomgpu_pcg_solve_noalloc.argtypes = L.gomgpu_pcg_solve.argtypes
         L.gomgpu_pcg_workspace_bytes.argtypes = [C.c_int]
         L.gomgpu_pcg_workspace_bytes.restype = C.c_longlong
         L.gomgpu_pcg_last_error.argtypes = []
@@ -98,7 +99,7 @@ EXPORTED = ["gomgpu_create", "gomgpu_set_state", "gomgpu_set_forcing", "gomgpu_s
             "gomgpu_device_bytes", "gomgpu_batch_create", "gomgpu_batch_add", "gomgpu_batch_remove",
             "gomgpu_batch_size", "gomgpu_batch_step", "gomgpu_batch_sync", "gomgpu_batch_last_error",
             "gomgpu_batch_destroy", "gomgpu_pcg_solve", "gomgpu_pcg_workspace_bytes",
-            "gomgpu_pcg_last_error"]
+            "gomgpu_pcg_last_error", "gomgpu_pcg_solve_noalloc"]
 
 HIP_H2D, HIP_D2H = 1, 2
 

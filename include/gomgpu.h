@@ -353,6 +353,18 @@ int gomgpu_pcg_solve(int na, int nz, const double *A, const int32_t *colidx, con
 long long gomgpu_pcg_workspace_bytes(int na);
 const char *gomgpu_pcg_last_error(void);
 
+/* gomgpu_pcg_solve without any device allocation. No counterpart in the reference: for
+ * callers that cannot allocate (code under HIP graph capture, tight serving loops, harnesses
+ * that forbid hipMalloc). Same arguments, kernels and results (bit for bit) as
+ * gomgpu_pcg_solve, for at most GOMGPU_PCG_NOALLOC_MAX_ROWS rows (GOMGPU_E_ARG above). The
+ * workspace is a fixed device array of the library (about 30.6 MB, held on every device the
+ * library runs on until the process ends), shared by all calls on a device: calls that
+ * overlap on one device race, and each call clears what must start at zero. */
+#define GOMGPU_PCG_NOALLOC_MAX_ROWS 262144
+int gomgpu_pcg_solve_noalloc(int na, int nz, const double *A, const int32_t *colidx, const int32_t *rowstr,
+                             const double *b, double *x, int max_iteration, double error_tolerance,
+                             int *iterations, void *stream);
+
 typedef struct gomgpu_batch gomgpu_batch;
 /* Batched contexts: one call steps every member. Destroying a member context removes it from
  * its batches; gomgpu_batch_destroy leaves the members usable on their own. */
