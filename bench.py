@@ -1,19 +1,23 @@
 """Benchmark of the GOM semi-implicit step on one MI355X.
 
-Workload: the synthetic estuary of the benchmark configurations -- a mixed orthogonal mesh
-of 200,200 cells (equilateral 100 m triangles stitched to 100x100 m squares), 20 z-layers,
-Gaussian thalweg 20 m -> 2 m with seeded +-0.5 m noise, salinity 0 -> 35 psu along the
-axis, 500 m^3/s river, M2+K1 tide, uniform 10 m/s wind, TVD limiters (minmod) with 2
-sub-cycles, GLS k-kl turbulence closure (Galperin stability functions), dt = 120 s. Inputs are generated (no network) and resident on the device when
-the timed region starts; per step only the O(boundary) host forcing crosses the bus.
+Workload (default, --config storm): the baroclinic storm-surge configuration -- 1,200,990
+mixed orthogonal cells (equilateral 100 m triangles stitched to 100x100 m squares), 30
+z-layers, a seeded fractal bathymetry of 1-40 m, a Holland cyclone (950 hPa, r_max 40 km,
+B 1.5) crossing the domain at 5 m/s with its fields every 10 steps, M2+K1 tide at the open
+boundary, wet/dry, stratified temperature and salinity (10 C, 3 psu) transported by TVD
+(minmod, 4 sub-cycles), GLS k-kl turbulence closure, dt = 60 s. --config estuary selects the
+200,200-cell x 20-layer synthetic estuary (500 m^3/s river, uniform 10 m/s wind, 2
+sub-cycles, dt = 120 s). Inputs are generated (no network) and resident on the device when
+the timed region starts; per step only the O(boundary) host forcing (and a hurricane
+snapshot every 10 steps) crosses the bus.
 
-A "step" is one full semi-implicit time step (friction, ELM, momentum, free surface + PCG,
-velocities, layers, node velocities, transport, EOS, update). The metric is active
-cell-layer-steps per second: sum over wet cells of their layer count, per step.
+A "step" is one full semi-implicit time step (wind, friction, ELM, momentum, free surface +
+PCG, velocities, layers, node velocities, closure, transport, EOS, update). The metric is
+active cell-layer-steps per second: sum over wet cells of their layer count, per step.
 
 Prints one JSON line. Extra fields: "roofline" for the dominant kernel (HBM-bound; bytes
 are algorithmic, durations from HIP events on the context stream) and "cpu_baseline" (the
-CPU restatement of the same step, single thread, on a bounded sample of the same workload).
+CPU restatement of the same step on a bounded sample of the same workload).
 """
 from __future__ import annotations
 
@@ -39,6 +43,17 @@ TURBULENCE = {"off": dict(turbulence_flag=0),
 # the reference's own project input (GOM_Rv1.1.0/Projects/MB_test/2_baroclinic_test/input/
 # main.inp:232, card C6: max_iteration_pcg 1000, error_tolerance 5.e-6).
 PCG_TOLERANCE = 5.0e-6
+
+
+def storm_case(pcg_tolerance: float = PCG_TOLERANCE, turbulence: str = "gls", ncol: int = 980, nrep: int = 245):
+    """The baroclinic storm-surge configuration: 1,200,990 mixed tri/quad cells (100 m), 30
+    z-layers, seeded fractal bathymetry 1-40 m, a Holland cyclone (950 hPa, r_max 40 km,
+    B 1.5) crossing at 5 m/s, M2+K1 tide, wet/dry, stratified T/S (10 C, 3 psu) transported
+    by TVD (minmod, 4 sub-cycles), GLS k-kl closure, dt = 60 s, 2,880 steps."""
+    from gom_release_amd import meshgen
+    return meshgen.storm_surge_case(ncol=ncol, nrep=nrep, nlayer=30, dt=60.0, ndt=2880, h_flux_limiter=1,
+                                    v_flux_limiter=1, error_tolerance=pcg_tolerance, max_iteration_pcg=1000,
+                                    **TURBULENCE[turbulence])
 
 
 def make_case(cells: str, turbulence: str = "gls", pcg_tolerance: float = PCG_TOLERANCE,
@@ -98,7 +113,7 @@ def step_bytes(F: int, C: int, N: int, J: int, E: int, zbar: float, nt: int, ntr
     return b3 + 24.0 * J + E * (12.0 * zbar + 20.0) + iters * E * (12.0 * zbar + 108.0)
 
 
-def cpu_baseline(case, wind, budget_s: float = 15.0, max_steps: int = 12):
+def cpu_baseline(case, wind, budget_s: float = 15.0, max_steps: int = 12, label: str = ""):
     """Time the CPU restatement (oracle, one thread) on the same workload."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     from oracle import Oracle
@@ -124,7 +139,7 @@ def cpu_baseline(case, wind, budget_s: float = 15.0, max_steps: int = 12):
         work += int(np.sum(np.where(top > 0, top - bot + 1, 0)))
         n += 1
     return {"value": work / t_total, "unit": "cell-layer-steps/s", "cores": 1, "kind": "port",
-            "sample": f"first {n} steps of the same 200k-cell x 20-layer estuary, FP64 C restatement, "
+            "sample": f"first {n} steps of the same workload ({label}), FP64 C restatement, "
                       f"single thread, {t_total:.1f} s"}
 
 
@@ -236,6 +251,9 @@ def main() -> int:
     ap.add_argument("--steps", type=int, default=60)
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--precision", choices=["fp64", "mixed"], default="fp64")
+    ap.add_argument("--config", choices=["storm", "estuary"], default="storm",
+                    help="storm: the 1.2M-cell baroclinic storm-surge configuration (default); "
+                         "estuary: the synthetic estuary of --cells")
     ap.add_argument("--cells", default="200k")
     ap.add_argument("--turbulence", choices=sorted(TURBULENCE), default="gls",
                     help="vertical closure of the workload (gls: k-kl GLS, Galperin stability functions)")
@@ -266,10 +284,14 @@ def main() -> int:
     if args.copy_calibration > 0:
         copy_calibration(args.copy_calibration << 20)
         print(f"[bench] calibration copy of {args.copy_calibration} MiB done", file=sys.stderr, flush=True)
-    wind = (0.0, 0.0) if args.storm else (10.0, 0.0)
-    case = make_case(args.cells, args.turbulence, args.pcg_tolerance,
-                     storm_steps=(4000 + args.warmup + args.steps + args.profile_steps) if args.storm else 0,
-                     transport=args.transport)
+    storm_cfg = args.config == "storm"
+    wind = (0.0, 0.0) if (args.storm or storm_cfg) else (10.0, 0.0)
+    if storm_cfg:
+        case = storm_case(args.pcg_tolerance, args.turbulence)
+    else:
+        case = make_case(args.cells, args.turbulence, args.pcg_tolerance,
+                         storm_steps=(4000 + args.warmup + args.steps + args.profile_steps) if args.storm else 0,
+                         transport=args.transport)
     prec = 0 if args.precision == "fp64" else 1
     print(f"[bench] building {case.maxele}-cell x {case.maxlayer}-layer case", file=sys.stderr, flush=True)
     sc = Scenario(case, precision=prec, wind=wind)
@@ -320,7 +342,7 @@ def main() -> int:
     F, Cl, N = sc.active_layers()
     zbar = pcg_bytes_per_iteration(sc.mesh, sc.params) / sc.params.maxele / 12.0 - 9.0
     bstep = step_bytes(F, Cl, N, sc.params.maxface, sc.params.maxele, zbar, max(case.trans_sub_iter, 1),
-                       2, it_mean, closure=case.turbulence_flag != 0)
+                       int(sc.params.maxtran2), it_mean, closure=case.turbulence_flag != 0)
     step_ms = 1e3 * elapsed / args.steps
     roof["step"] = {"algorithmic_bytes": round(bstep), "achieved": round(bstep / (step_ms * 1e-3) / 1e9, 1),
                     "frac": round(bstep / (step_ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
@@ -334,7 +356,15 @@ def main() -> int:
         "warmup": args.warmup, "ms_per_step": 1e3 * elapsed / args.steps, "higher_is_better": True,
         "scaling": "weak", "vs_baseline": None, "dtype": "f64" if prec == 0 else "f32+f64",
         "data": "synthetic",
-        "config": {"workload": f"synthetic estuary, {case.maxele} mixed tri/quad cells x {case.maxlayer} "
+        "config": {"workload": (f"baroclinic storm surge, {case.maxele} mixed tri/quad cells (100 m) x "
+                                f"{case.maxlayer} z-layers, seeded fractal bathymetry 1-40 m, Holland cyclone "
+                                f"(950 hPa, r_max 40 km, B 1.5, 5 m/s; fields every 10 steps), M2+K1, wet/dry, "
+                                f"stratified T/S (10 C, 3 psu) by TVD minmod x{case.trans_sub_iter} sub-cycles, "
+                                f"dt={case.dt:g} s, "
+                                + ("GLS k-kl turbulence closure (Galperin)" if args.turbulence == "gls"
+                                   else "constant Av/Kv")
+                                + f", PCG tolerance {case.error_tolerance:g}") if storm_cfg else (
+                               f"synthetic estuary, {case.maxele} mixed tri/quad cells x {case.maxlayer} "
                                f"z-layers, dt={case.dt:g} s, "
                                + ("ELM scalar transport, uniform 20 C, 5 psu river" if args.transport == "elm"
                                   else "TVD minmod x2 sub-cycles")
@@ -343,7 +373,7 @@ def main() -> int:
                                   "stratified T/S (10 C, 3 psu) transported, " if args.storm else "10 m/s wind, ")
                                + ("GLS k-kl turbulence closure (Galperin)"
                                                    if args.turbulence == "gls" else "constant Av/Kv")
-                               + f", PCG tolerance {case.error_tolerance:g}",
+                               + f", PCG tolerance {case.error_tolerance:g}"),
                    "turbulence": args.turbulence, "transport": args.transport,
                    "cells": case.maxele, "faces": sc.params.maxface, "nodes": case.maxnod,
                    "layers": case.maxlayer, "active_cell_layers": int(0.5 * (a0 + a1)),
@@ -364,7 +394,7 @@ def main() -> int:
         out["config"]["p99_latency_s"] = out["serving"]["p99_latency_s"]
     if not args.no_cpu_baseline:
         print("[bench] timing the CPU restatement", file=sys.stderr, flush=True)
-        out["cpu_baseline"] = cpu_baseline(case, wind)
+        out["cpu_baseline"] = cpu_baseline(case, wind, label=f"{case.maxele} cells x {case.maxlayer} layers")
     print(json.dumps(out))
     return 0
 

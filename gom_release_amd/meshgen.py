@@ -263,6 +263,89 @@ def add_cyclone(case: Case, steps: int, seed: int = 0, p_min: float = 950.0, r_m
     return case
 
 
+def fractal_field(x, y, seed: int, beta: float = 3.4, n: int = 512) -> np.ndarray:
+    """A seeded fractal surface sampled at the points (x, y), scaled to [0, 1]: spectral
+    synthesis on an n x n periodic grid spanning the points' box (amplitude |k|^(-beta/2)
+    with uniform random phases, i.e. a power spectrum |k|^-beta; beta = 3.4 is a fractional
+    Brownian surface of Hurst exponent 0.7), interpolated bilinearly."""
+    rng = np.random.default_rng(seed)
+    kx = np.fft.fftfreq(n)[:, None]
+    ky = np.fft.rfftfreq(n)[None, :]
+    kk = np.hypot(kx, ky)
+    kk[0, 0] = 1.0
+    amp = kk ** (-beta / 2.0)
+    amp[0, 0] = 0.0
+    spec = amp * np.exp(2j * np.pi * rng.uniform(size=amp.shape))
+    g = np.fft.irfft2(spec, s=(n, n))
+    g = (g - g.min()) / (g.max() - g.min())
+    x0, x1, y0, y1 = float(np.min(x)), float(np.max(x)), float(np.min(y)), float(np.max(y))
+    fx = (np.asarray(x) - x0) / max(x1 - x0, 1e-30) * (n - 1)
+    fy = (np.asarray(y) - y0) / max(y1 - y0, 1e-30) * (n - 1)
+    i0 = np.clip(np.floor(fx).astype(np.int64), 0, n - 2)
+    j0 = np.clip(np.floor(fy).astype(np.int64), 0, n - 2)
+    tx, ty = fx - i0, fy - j0
+    v = (g[i0, j0] * (1 - tx) * (1 - ty) + g[i0 + 1, j0] * tx * (1 - ty) + g[i0, j0 + 1] * (1 - tx) * ty
+         + g[i0 + 1, j0 + 1] * tx * ty)
+    return (v - v.min()) / (v.max() - v.min())
+
+
+def storm_surge_case(ncol: int = 980, nrep: int = 245, nlayer: int = 30, dx: float = 100.0, dt: float = 60.0,
+                     ndt: int = 2880, seed: int = 11, p_min: float = 950.0, r_max: float = 40.0e3,
+                     b: float = 1.5, speed: float = 5.0, heading_deg: float = 30.0, d_temp: float = 10.0,
+                     d_salt: float = 3.0, coast_weight: float = 0.55, track_offset: float = 0.0,
+                     surface_gap: float = 4.0, name: str = "storm_surge", **kw) -> Case:
+    """The baroclinic storm-surge configuration: the stitched square/triangle mesh (1,200,990
+    cells at the default size), `nlayer` z-layers, a seeded fractal bathymetry of 1-40 m that
+    deepens towards the open (west) boundary, M2+K1 tide there, wet/dry active, stratified
+    temperature and salinity (`d_temp` C and `d_salt` psu from surface to bottom: 25 -> 15 C,
+    32 -> 35 psu), and a Holland cyclone (`p_min` hPa, `r_max`, B = `b`) moving at `speed`
+    m/s on `heading_deg` (from east, counter-clockwise) whose centre passes `track_offset` m to
+    the left of the middle of the domain half way through the run; its fields every 10 steps.
+    `coast_weight` is the share of the landward ramp in the bathymetry (the rest is the fractal
+    field). The surface layer spans `surface_gap` m below to 6 m above mean sea level, more
+    than the surge range, so the surface never comes near a level interface: a z-level top
+    layer only centimetres thick would take the whole storm stress (currents of tens of m/s,
+    and tracers driven to the transport's clamps). No river."""
+    kw.setdefault("tide_phase", -90.0)     # the tide starts from zero elevation
+    # four transport sub-cycles: the explicit TVD step stays below one cell volume per
+    # sub-cycle on the 100 m triangles up to ~3 m/s
+    kw.setdefault("trans_sub_iter", 4)
+    case = estuary_case(ncol=ncol, nrep=nrep, nlayer=nlayer, dx=dx, dt=dt, ndt=ndt, seed=seed,
+                        river_q=0.0, name=name, **kw)
+    x, y = case.x, case.y
+    coast = (x - x.min()) / (x.max() - x.min())                 # 0 at the open boundary, 1 inland
+    g = coast_weight * (1.0 - coast) + (1.0 - coast_weight) * fractal_field(x, y, seed)
+    g = (g - g.min()) / (g.max() - g.min())
+    case.depth = np.round(1.0 + 39.0 * g, 6)
+    zb = float(np.ceil(case.depth.max() + 0.5))
+    case.z_level = uniform_levels(zb, nlayer, top=6.0, surf_gap=surface_gap)
+    case.rivers, case.q_series = [], []
+    L = nlayer
+    frac = np.arange(L) / max(L - 1, 1)                           # 0 bottom .. 1 top
+    case.salt_init = np.repeat((35.0 - d_salt * frac)[None, :], case.maxnod, axis=0)
+    case.salt_series = [Series([-1000.0, 100000.0], [35.0, 35.0])]
+    case.is_temp = 1
+    t_bottom = 25.0 - d_temp
+    case.temp_init = np.repeat((t_bottom + d_temp * frac)[None, :], case.maxnod, axis=0)
+    case.temp_series = [Series([-1000.0, 100000.0], [25.0, 25.0])]
+    for bnd in case.open_boundaries:
+        bnd.salt_ser_id, bnd.temp_ser_id = 1, 1
+    # the track: through the domain centre at the middle of the run
+    hd = math.radians(heading_deg)
+    vel = (speed * math.cos(hd), speed * math.sin(hd))
+    half = 0.5 * ndt * dt
+    cx, cy = 0.5 * (x.min() + x.max()), 0.5 * (y.min() + y.max())
+    cx, cy = cx - track_offset * math.sin(hd), cy + track_offset * math.cos(hd)   # left of the heading
+    start = (cx - vel[0] * half, cy - vel[1] * half)
+    hdt = int(10 * dt)
+    times = [k * float(hdt) for k in range(int(ndt * dt // hdt) + 3)]
+    case.hurricane_flag, case.hurricane_dt = 1, hdt
+    y0, mo, d = case.start[:3]
+    case.hurricane_start, case.hurricane_end = (y0, mo, d), (y0 + 1, mo, d)
+    case.hurricane = holland_snapshots(x, y, times, start, vel, p_min=p_min, r_max=r_max, b=b, lat=case.lat)
+    return case
+
+
 def stratify(case: Case, d_temp: float = 10.0, d_salt: float = 3.0, t_bottom: float = 15.0) -> Case:
     """Stratified temperature and salinity: temperature from t_bottom at the bottom layer
     to t_bottom + d_temp at the top; salinity raised towards the bottom by up to d_salt,
