@@ -22,6 +22,7 @@ CPU restatement of the same step on a bounded sample of the same workload).
 from __future__ import annotations
 
 import argparse
+import dataclasses
 import json
 import os
 import sys
@@ -245,6 +246,59 @@ def ensemble_sample(n_mem: int = 8, steps: int = 10, seed: int = 7):
             "speedup": (workb / t_batch) / (work / t_serial)}
 
 
+MIXED_FIELDS = ["eta_cell", "salt_cell", "temp_cell", "un_face", "vn_face", "wn_cell", "top_layer_at_element"]
+
+
+def field_errors(a, b) -> dict:
+    """Error of state a (mixed precision) against b (FP64): eta and velocities as relative L2,
+    tracers as the largest absolute difference; wet/dry masks exact or not."""
+    rel = lambda x, y: float(np.linalg.norm(x - y) / max(np.linalg.norm(y), 1e-300))  # noqa: E731
+    return {"eta_rel_l2": rel(a["eta_cell"], b["eta_cell"]),
+            "salt_max_abs_psu": float(np.max(np.abs(a["salt_cell"] - b["salt_cell"]))),
+            "temp_max_abs_C": float(np.max(np.abs(a["temp_cell"] - b["temp_cell"]))),
+            "un_rel_l2": rel(a["un_face"], b["un_face"]), "vn_rel_l2": rel(a["vn_face"], b["vn_face"]),
+            "wn_rel_l2": rel(a["wn_cell"], b["wn_cell"]),
+            "wet_dry_masks_equal": bool(np.array_equal(a["top_layer_at_element"], b["top_layer_at_element"]))}
+
+
+def mixed_sample(case, wind, warmup: int, steps: int, ref_fields, fp64_ms: float, launch) -> dict:
+    """The same workload with GOMGPU_MIXED (FP32 storage of the transport work arrays and of
+    the PCG inner solve, FP64 refinement), the same warm-up and timed steps; its throughput
+    and its state against the FP64 run's after the same steps."""
+    from gom_release_amd.driver import Scenario
+    sc = Scenario(case, precision=1, wind=wind)
+    if launch:
+        for k, v in launch.items():
+            sc.model.set_launch(k, v)
+    sc.run(warmup)
+    sc.model.sync()
+    a0 = sc.active_cell_layers()
+    t0 = time.perf_counter()
+    sc.run(steps)
+    sc.model.sync()
+    el = time.perf_counter() - t0
+    a1 = sc.active_cell_layers()
+    errs = field_errors(sc.fields(MIXED_FIELDS), ref_fields)
+    errs["salt_rms_psu"] = float(np.sqrt(np.mean((sc.fields(["salt_cell"])["salt_cell"] - ref_fields["salt_cell"]) ** 2)))
+    iters = []
+    for _ in range(3):
+        sc.step()
+        iters.append(sc.model.pcg_iterations())
+    sc.model.close()
+    ms = 1e3 * el / steps
+    # the same FP64 step with the PCG stop tolerance halved: how far the FP64 state itself
+    # moves within the solver's tolerance (the scale the mixed-precision errors compare to)
+    tight = dataclasses.replace(case, error_tolerance=0.5 * case.error_tolerance)
+    sc = Scenario(tight, precision=0, wind=wind)
+    sc.run(warmup + steps)
+    sens = field_errors(sc.fields(MIXED_FIELDS), ref_fields)
+    sc.model.close()
+    return {"value": 0.5 * (a0 + a1) * steps / el, "unit": "cell-layer-steps/s", "ms_per_step": ms,
+            "speedup_vs_fp64": fp64_ms / ms, "pcg_inner_iterations": sum(iters) / len(iters),
+            "errors_vs_fp64_after_timed_steps": errs,
+            "fp64_half_tolerance_vs_fp64": sens}
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -268,6 +322,7 @@ def main() -> int:
     ap.add_argument("--no-autotune", action="store_true", help="keep the default launch shapes")
     ap.add_argument("--no-serving", action="store_true", help="skip the serving-loop sample")
     ap.add_argument("--no-ensemble", action="store_true", help="skip the batched-ensemble sample")
+    ap.add_argument("--no-mixed", action="store_true", help="skip the mixed-precision sample")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="also run one device-to-device copy of MIB MiB (calibrates rocprofv3 "
                          "FETCH_SIZE/WRITE_SIZE against known traffic)")
@@ -317,6 +372,7 @@ def main() -> int:
     active = 0.5 * (a0 + a1) * args.steps
     value = active / elapsed
     print(f"[bench] timed {args.steps} steps in {elapsed:.3f} s", file=sys.stderr, flush=True)
+    ref_fields = sc.fields(MIXED_FIELDS) if (prec == 0 and not args.no_mixed) else None
 
     # measurement pass: per-stage and per-kernel HIP event times (synchronising)
     model.timings(True)
@@ -384,6 +440,11 @@ def main() -> int:
         "launch": launch,
     }
     model.close()
+    if ref_fields is not None:
+        print("[bench] mixed-precision sample", file=sys.stderr, flush=True)
+        out["mixed"] = mixed_sample(case, wind, args.warmup, args.steps, ref_fields, 1e3 * elapsed / args.steps,
+                                    launch)
+        ref_fields = None
     if not args.no_ensemble:
         print("[bench] batched ensemble sample", file=sys.stderr, flush=True)
         out["ensemble"] = ensemble_sample()

@@ -221,6 +221,30 @@ struct PcgArgs {
   int n, maxit;
   int barrier_mode;   // 0 cooperative-groups grid sync, 1 custom barrier
   double tol;
+  static constexpr bool kMixed = false;
+};
+
+// The FP32-storage inner solve of the mixed-precision PCG (k_pcg.hip): the matrix, the
+// right-hand side and the published row values (r, w, s) in FP32; the correction it
+// accumulates and the search direction in FP64, as is every operation and reduction. Its
+// stop test is relative to the outer residual: gpart holds the FP64 partial sums of
+// r.M^-1 r of the current outer residual, g0 that of the first.
+struct PcgArgs32 {
+  const float *diag, *off, *b;
+  const int32_t *col, *nfaces;
+  double *x, *d0;
+  float *pub;
+  double *partA, *partB;
+  unsigned long long *slots;
+  int *info;
+  unsigned *bar;
+  int n, maxit;
+  double tol;
+  const double *gpart;  // partial sums of the outer gamma (ngpart of them)
+  int ngpart;
+  int first;            // first refinement: the outer gamma is gamma_0, stored in *g0
+  double *g0;
+  static constexpr bool kMixed = true;
 };
 
 // Launch helpers

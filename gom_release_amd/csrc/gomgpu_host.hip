@@ -39,18 +39,19 @@ __global__ void k_layers_check_cell(DevMesh, DevState, DevDims, double, int *);
 __global__ void k_layers_check_fn(DevMesh, DevState, DevDims, double, int *);
 __global__ void k_face_levels(DevMesh, DevState, DevDims);
 __global__ void k_node_vel(DevMesh, DevState, DevDims);
-__global__ void k_tvd_init(DevMesh, DevState, DevForcing, DevWork, DevDims, TvdArgs);
-__global__ void k_tvd_bnd(DevMesh, DevState, DevForcing, DevWork, DevDims, TvdArgs);
-__global__ void k_tvd_coef_face(DevMesh, DevState, DevWork, DevDims, TvdArgs);
-__global__ void k_tvd_coef_cell(DevMesh, DevState, DevWork, DevDims, TvdArgs);
-__global__ void k_tvd_lim_cell(DevMesh, DevState, DevWork, DevDims, int, int);
-__global__ void k_tvd_lim_face(DevMesh, DevState, DevWork, DevDims, int, int);
-__global__ void k_tvd_rhs(DevMesh, DevState, DevWork, DevDims, TvdArgs);
-__global__ void k_tvd_solve(DevMesh, DevState, DevWork, DevDims, TvdArgs, int *);
-__global__ void k_tvd_copyback(DevMesh, DevState, DevWork, DevDims, int);
-__global__ void k_tvd_node(DevMesh, DevState, DevWork, DevDims);
-__global__ void k_tvd_face_out(DevMesh, DevState, DevWork, DevDims);
-__global__ void k_tvd_cellnode_out(DevState, DevWork, DevDims);
+// transport launchers (k_transport.hip): f32 selects the FP32-storage instantiation
+void tvd_launch_init(bool, long, int, hipStream_t, DevMesh, DevState, DevForcing, DevWork, DevDims, TvdArgs);
+void tvd_launch_bnd(bool, long, int, hipStream_t, DevMesh, DevState, DevForcing, DevWork, DevDims, TvdArgs);
+void tvd_launch_coef_face(bool, long, int, hipStream_t, DevMesh, DevState, DevWork, DevDims, TvdArgs);
+void tvd_launch_coef_cell(bool, long, int, hipStream_t, DevMesh, DevState, DevWork, DevDims, TvdArgs);
+void tvd_launch_lim_cell(bool, long, int, hipStream_t, DevMesh, DevState, DevWork, DevDims, int, int);
+void tvd_launch_lim_face(bool, long, int, hipStream_t, DevMesh, DevState, DevWork, DevDims, int, int);
+void tvd_launch_rhs(bool, long, int, hipStream_t, DevMesh, DevState, DevWork, DevDims, TvdArgs);
+void tvd_launch_solve(bool, long, int, hipStream_t, DevMesh, DevState, DevWork, DevDims, TvdArgs, int *);
+void tvd_launch_copyback(bool, long, int, hipStream_t, DevMesh, DevState, DevWork, DevDims, int);
+void tvd_launch_node(bool, long, int, hipStream_t, DevMesh, DevState, DevWork, DevDims);
+void tvd_launch_face_out(bool, long, int, hipStream_t, DevMesh, DevState, DevWork, DevDims);
+void tvd_launch_cellnode_out(bool, long, int, hipStream_t, DevState, DevWork, DevDims);
 __global__ void k_eos_nodeface(DevMesh, DevState, DevDims, double, int, int, int *);
 __global__ void k_eos_cell(DevMesh, DevState, DevDims, int *);
 __global__ void k_bars(DevMesh, DevState, DevDims);
@@ -91,6 +92,9 @@ extern "C" int gom_pcg_cg1_capacity(int device, int n, int nb, int bs, int slots
 extern "C" int gom_pcg_cg1m_blocks(long n);
 extern "C" hipError_t gom_launch_pcg_cg1m(const PcgArgs *args, int nb, int cooperative, hipStream_t st);
 extern "C" int gom_pcg_cg1m_capacity(int device);
+extern "C" hipError_t gom_launch_pcg_mixed(const PcgArgs *a64, const PcgArgs32 *m32, int cooperative,
+                                           hipStream_t st);
+extern "C" int gom_pcg_cg1m32_capacity(int device);
 
 enum { KT_PCG = 0, KT_ELM = 1, KT_MOMENTUM = 2, KT_TVD_COLUMN = 3 };
 static const char *kKernelNames[GOMGPU_NUM_KTIMERS] = {"k_pcg", "k_elm", "k_momentum", "k_tvd_column"};
@@ -275,8 +279,7 @@ static int validate(const gomgpu_params *p, std::string &why) {
   if (p->hurricane_flag != 0 && p->hurricane_flag != 1) return why = "hurricane_flag must be 0 or 1", 1;
   if (p->hurricane_flag == 1 && p->hurricane_read_interval < 1)
     return why = "hurricane_read_interval (hurricane_dt / dt) must be at least 1", 1;
-  if (p->precision == GOMGPU_MIXED) return why = "GOMGPU_MIXED is not implemented in this build", 1;
-  if (p->precision != GOMGPU_FP64) return why = "bad precision", 1;
+  if (p->precision != GOMGPU_FP64 && p->precision != GOMGPU_MIXED) return why = "bad precision", 1;
   return 0;
 }
 
@@ -748,13 +751,49 @@ static void pcg_args(gomgpu_ctx *c, PcgArgs &a) {
   a.tol = c->p.error_tolerance;
 }
 
+// The FP32 side of the mixed-precision solve, on the context's PCG buffers: the FP32 matrix
+// in pcg_minv (diagonal) and the upper half of pcg_pub (4 slots), the FP32 published values in
+// its lower half, the FP32 residual in pcg_r, the FP64 correction and direction in pcg_s and
+// pcg_d0,
+// the outer partial sums in the third partial buffer and gamma_0 in the fourth.
+static void pcg_args32(gomgpu_ctx *c, const PcgArgs &a, PcgArgs32 &m) {
+  const long E = c->d.E;
+  const long pstride = c->pcg_blocks > 4096 ? c->pcg_blocks : 4096;
+  m.diag = (const float *)c->w.pcg_minv;
+  m.off = (const float *)(c->w.pcg_pub + 3 * E);
+  m.b = (const float *)c->w.pcg_r;
+  m.col = a.col;
+  m.nfaces = a.nfaces;
+  m.x = c->w.pcg_s;
+  m.d0 = c->w.pcg_d0;
+  m.pub = (float *)c->w.pcg_pub;
+  m.partA = c->w.pcg_part;
+  m.partB = c->w.pcg_part + pstride;
+  m.slots = a.slots;
+  m.info = a.info;
+  m.bar = a.bar;
+  m.n = a.n;
+  m.maxit = a.maxit;
+  m.tol = a.tol;
+  m.gpart = c->w.pcg_part + 2 * pstride;
+  m.ngpart = 0;
+  m.first = 1;
+  m.g0 = c->w.pcg_part + 3 * pstride;
+}
+
 static int fs_pcg(gomgpu_ctx *c) {
   const DevDims &d = c->d;
   hipStream_t st = c->st;
   PcgArgs a;
   pcg_args(c, a);
   KT_BEGIN(KT_PCG);
-  {
+  if (c->p.precision == GOMGPU_MIXED) {
+    PcgArgs32 m;
+    pcg_args32(c, a, m);
+    if (gom_pcg_cg1m_blocks(d.E) > gom_pcg_cg1m32_capacity(c->device))
+      return seterr(c, GOMGPU_E_HIP, "the mixed-precision PCG grid does not fit the device");
+    HIPCHK(c, gom_launch_pcg_mixed(&a, &m, c->pcg_coop, st));
+  } else {
     hipError_t le = hipErrorInvalidValue;
     if (c->pcg_mode >= 1 && c->pcg_mode <= 3) {
       int rmax = c->pcg_bs >= 1024 ? 2 : c->pcg_bs >= 512 ? 4 : 8;
@@ -899,31 +938,31 @@ static int stage_impl(gomgpu_ctx *c, int id) {
       a.heat = p.heat_option;
       a.sub = 0;
       long nE2 = (long)d.E * (d.L + 2);
+      const bool f32 = p.precision == GOMGPU_MIXED;
       k_fill<<<nblk((long)d.NT * d.L * d.E, BS), BS, 0, st>>>(c->w.con_new, 0.0, (long)d.NT * d.L * d.E);
-      k_tvd_init<<<nblk(nE2, BS), BS, 0, st>>>(c->m, c->s, c->f, c->w, d, a);
+      tvd_launch_init(f32, nE2, BS, st, c->m, c->s, c->f, c->w, d, a);
       for (int sub = 0; sub < a.Nt; sub++) {
         a.sub = sub;
         long nb = (long)d.L * (d.NOB + d.NQB);
-        if (nb > 0) k_tvd_bnd<<<nblk(nb, BS), BS, 0, st>>>(c->m, c->s, c->f, c->w, d, a);
-        k_tvd_coef_face<<<nblk((long)d.J * d.L, BS), BS, 0, st>>>(c->m, c->s, c->w, d, a);
-        k_tvd_coef_cell<<<nblk((long)d.E * (d.L + 1), BS), BS, 0, st>>>(c->m, c->s, c->w, d, a);
+        if (nb > 0) tvd_launch_bnd(f32, nb, BS, st, c->m, c->s, c->f, c->w, d, a);
+        tvd_launch_coef_face(f32, (long)d.J * d.L, BS, st, c->m, c->s, c->w, d, a);
+        tvd_launch_coef_cell(f32, (long)d.E * (d.L + 1), BS, st, c->m, c->s, c->w, d, a);
         if (p.h_flux_limiter > 0 || p.v_flux_limiter > 0) {
-          k_tvd_lim_cell<<<nblk((long)(d.L + 1) * d.E, LB(TVD_LIM)), LB(TVD_LIM), 0, st>>>(c->m, c->s, c->w, d,
-                                                                                         p.h_flux_limiter, sub == 0);
-          k_tvd_lim_face<<<nblk((long)d.J * d.L, BS), BS, 0, st>>>(c->m, c->s, c->w, d, p.h_flux_limiter,
-                                                                   sub == 0);
+          tvd_launch_lim_cell(f32, (long)(d.L + 1) * d.E, LB(TVD_LIM), st, c->m, c->s, c->w, d, p.h_flux_limiter,
+                              sub == 0);
+          tvd_launch_lim_face(f32, (long)d.J * d.L, BS, st, c->m, c->s, c->w, d, p.h_flux_limiter, sub == 0);
         }
         KT_BEGIN(KT_TVD_COLUMN);
-        k_tvd_rhs<<<nblk((long)d.L * d.E, LB(TVD_RHS)), LB(TVD_RHS), 0, st>>>(c->m, c->s, c->w, d, a);
-        k_tvd_solve<<<nblk(d.E, LB(TVD_SOLVE)), LB(TVD_SOLVE), 0, st>>>(c->m, c->s, c->w, d, a, err);
+        tvd_launch_rhs(f32, (long)d.L * d.E, LB(TVD_RHS), st, c->m, c->s, c->w, d, a);
+        tvd_launch_solve(f32, d.E, LB(TVD_SOLVE), st, c->m, c->s, c->w, d, a, err);
         KT_END(KT_TVD_COLUMN);
-        k_tvd_copyback<<<nblk(nE2, BS), BS, 0, st>>>(c->m, c->s, c->w, d, sub == 0);
+        tvd_launch_copyback(f32, nE2, BS, st, c->m, c->s, c->w, d, sub == 0);
       }
-      k_tvd_node<<<nblk((long)d.P * ((d.L + GOM_TVD_NODE_LEVELS - 1) / GOM_TVD_NODE_LEVELS), BS), BS, 0, st>>>(
-          c->m, c->s, c->w, d);
-      k_tvd_face_out<<<nblk((long)d.J * d.L, BS), BS, 0, st>>>(c->m, c->s, c->w, d);
+      tvd_launch_node(f32, (long)d.P * ((d.L + GOM_TVD_NODE_LEVELS - 1) / GOM_TVD_NODE_LEVELS), BS, st, c->m, c->s,
+                      c->w, d);
+      tvd_launch_face_out(f32, (long)d.J * d.L, BS, st, c->m, c->s, c->w, d);
       long nmax = (long)d.L * (d.E > d.P ? d.E : d.P);
-      k_tvd_cellnode_out<<<nblk(nmax, BS), BS, 0, st>>>(c->s, c->w, d);
+      tvd_launch_cellnode_out(f32, nmax, BS, st, c->s, c->w, d);
       for (int t = 0; t < d.NT; t++) c->stale &= ~(unsigned)(d.tid[t] == 1 ? SWAP_SALT : SWAP_TEMP);
       break;
     }
@@ -1210,7 +1249,7 @@ static int batch_plan(gomgpu_batch *b) {
   int total = 0;
   for (gomgpu_ctx *c : b->members) {
     int nb = pcg_blocks_for(c->d.E, c->pcg_rows);
-    if (nb > b->capacity || nb > PCG_MAX_SLOT_BLOCKS) {
+    if (nb > b->capacity || nb > PCG_MAX_SLOT_BLOCKS || c->p.precision == GOMGPU_MIXED) {
       b->solo.push_back(c);
       continue;
     }
@@ -1271,6 +1310,12 @@ static int batch_enqueue(gomgpu_batch *b) {
   for (gomgpu_ctx *c : b->solo) {  // the solve each of them runs alone (fs_pcg)
     PcgArgs a;
     pcg_args(c, a);
+    if (c->p.precision == GOMGPU_MIXED) {
+      PcgArgs32 m;
+      pcg_args32(c, a, m);
+      BHIP(b, gom_launch_pcg_mixed(&a, &m, 0, b->st));
+      continue;
+    }
     int nb = gom_pcg_cg1m_blocks(c->d.E);
     if (nb <= gom_pcg_cg1m_capacity(c->device)) {
       BHIP(b, gom_launch_pcg_cg1m(&a, nb, 0, b->st));

@@ -512,6 +512,12 @@ __device__ __forceinline__ double pub_ld(const double *p) {
   return __longlong_as_double(
       (long long)__hip_atomic_load((const unsigned long long *)p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
 }
+__device__ __forceinline__ void pub_st(float *p, double v) {
+  __hip_atomic_store((unsigned *)p, (unsigned)__float_as_uint((float)v), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ double pub_ld(const float *p) {
+  return (double)__uint_as_float(__hip_atomic_load((const unsigned *)p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
+}
 
 // Pairs polled per lane of wave 0: at most 64 * PCG_SLOT_POLL blocks per system.
 #define PCG_SLOT_POLL (PCG_MAX_SLOT_BLOCKS / 64)
@@ -803,9 +809,8 @@ __global__ void __launch_bounds__(BS) k_pcg_cg1(PcgArgs a) {
 // k*BS (the assignment of the register-resident kernel) each iteration; x and the search
 // direction p stay in a.x and a.d0, the published (r, w, s) in a.pub, the matrix is
 // streamed. Per row the arithmetic is that of cg1_body statement for statement.
-template <int BS>
-__device__ __forceinline__ void cg1m_body(const PcgArgs &a, const int b, const int nb, double *lds,
-                                          unsigned *gen0) {
+template <int BS, typename A>
+__device__ __forceinline__ void cg1m_body(const A &a, const int b, const int nb, double *lds, unsigned *gen0) {
   if (threadIdx.x == 0) *gen0 = __hip_atomic_load(&a.bar[1], __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT);
   __syncthreads();
   unsigned gen = *gen0;
@@ -821,10 +826,24 @@ __device__ __forceinline__ void cg1m_body(const PcgArgs &a, const int b, const i
   if (threadIdx.x < 6)
     __hip_atomic_store(a.slots + 2 * ((size_t)(threadIdx.x >> 1) * nb + b) + (threadIdx.x & 1),
                        PCG_SLOT_SENTINEL, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-  double *pub = a.pub;
+  auto *pub = a.pub;
   const size_t n1 = (size_t)n, n2 = 2 * (size_t)n;
 #define CG1M_ROWS(i)                                                                     \
   for (int loc = threadIdx.x, i = row0 + loc; loc < rows_per_block && i < n; loc += BS, i += BS)
+  double tol = a.tol;
+  if constexpr (A::kMixed) {
+    // the outer gamma of this refinement (every block sums the partials in one order) and the
+    // inner tolerance that brings the outer ratio below tol; nothing to do once it is there
+    double gcur = 0.0;
+    for (int q = 0; q < a.ngpart; q++) gcur = gcur + a.gpart[q];
+    const double g0 = a.first ? gcur : *a.g0;
+    if (a.first && b == 0 && threadIdx.x == 0) *a.g0 = gcur;
+    if (!(gcur > 0.0) || sqrt(gcur * gcur / (g0 * g0)) < a.tol) {
+      if (b == 0 && threadIdx.x == 0) a.info[1] = 1;
+      return;  // the correction stays 0 (zeroed with the residual)
+    }
+    tol = a.tol * (g0 / gcur);
+  }
   // r0 = b - A x0 (pre_conj_grad.f90:48-60); publish r0; p = 0
   CG1M_ROWS(i) {
     double dg = a.diag[i];
@@ -913,20 +932,32 @@ __device__ __forceinline__ void cg1m_body(const PcgArgs &a, const int b, const i
     gamma = gamma_new;
     beta = beta_new;
     alpha = alpha_new;
-    if (sqrt(gamma * gamma / (gamma_0 * gamma_0)) < a.tol) {
+    if constexpr (A::kMixed) cgit2 = cgit;  // iterations run, converged or not
+    if (sqrt(gamma * gamma / (gamma_0 * gamma_0)) < tol) {
       cgit2 = cgit;
       break;
     }
   }
 #undef CG1M_ROWS
   if (b == 0 && threadIdx.x == 0) {
-    a.info[0] = cgit2;
-    a.info[1] = cgit2 > 0 ? 1 : 0;
+    if constexpr (A::kMixed) {
+      a.info[0] = a.info[0] + cgit2;  // the outer check (k_pcg_mixed_final) sets info[1]
+    } else {
+      a.info[0] = cgit2;
+      a.info[1] = cgit2 > 0 ? 1 : 0;
+    }
   }
 }
 
 template <int BS>
 __global__ void __launch_bounds__(BS) k_pcg_cg1m(PcgArgs a) {
+  __shared__ double lds[CG1_LDS(BS)];
+  __shared__ unsigned gen0;
+  cg1m_body<BS>(a, blockIdx.x, gridDim.x, lds, &gen0);
+}
+
+template <int BS>
+__global__ void __launch_bounds__(BS) k_pcg_cg1m32(PcgArgs32 a) {
   __shared__ double lds[CG1_LDS(BS)];
   __shared__ unsigned gen0;
   cg1m_body<BS>(a, blockIdx.x, gridDim.x, lds, &gen0);
@@ -1039,6 +1070,107 @@ extern "C" int gom_pcg_cg1_capacity(int device, int n, int nb, int bs, int slots
   if (!k) return 0;
   int per_cu = 0, ncu = 0;
   if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k, bs, 0) != hipSuccess) return 0;
+  if (hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, device) != hipSuccess) return 0;
+  return per_cu * ncu;
+}
+
+// ---------------------------------------------------------------------------------------
+// Mixed-precision PCG (GOMGPU_MIXED): iterative refinement of the same Jacobi-preconditioned
+// system. Each refinement: the FP64 residual r = b - A x and its gamma = r.M^-1 r partials
+// (k_pcg_mixed_res, which also zeroes the correction and, the first time, writes the FP32
+// matrix); the FP32-storage single-reduction solve of A e = r to the inner tolerance that
+// takes the outer ratio below tol (k_pcg_cg1m32); x = x + e in FP64 (k_pcg_mixed_update).
+// A final residual decides convergence by the reference's test sqrt(g^2/g0^2) < tol
+// (pre_conj_grad.f90:199) on the FP64 residual. The sequence is fixed (no host round trip):
+// once converged, further refinements return at once.
+#define PCG_MIXED_REFINE 3
+#define PCG_RES_BS 256
+#define PCG_RES_BLOCKS 1024
+
+__global__ void __launch_bounds__(PCG_RES_BS) k_pcg_mixed_res(PcgArgs a, PcgArgs32 m, int first, int write_r) {
+  __shared__ double lds[PCG_RES_BS / 64];
+  const int n = a.n;
+  double g = 0.0;
+  for (int i = blockIdx.x * PCG_RES_BS + threadIdx.x; i < n; i += gridDim.x * PCG_RES_BS) {
+    const double dg = a.diag[i];
+    double sum = dg * a.x[i];
+    const int nf = a.nfaces[i];
+    for (int l = 0; l < nf; l++) {
+      const int c = a.col[ix(l, i, n)];
+      if (c != 0) sum = sum + a.off[ix(l, i, n)] * a.x[c - 1];
+    }
+    const double r = a.b[i] - sum;
+    g = g + r * (r / dg);
+    if (write_r) {
+      ((float *)m.b)[i] = (float)r;
+      m.x[i] = 0.0;
+    }
+    if (first) {
+      ((float *)m.diag)[i] = (float)dg;
+      for (int l = 0; l < 4; l++) ((float *)m.off)[ix(l, i, n)] = (float)a.off[ix(l, i, n)];
+    }
+  }
+  g = wave_sum(g);
+  if ((threadIdx.x & 63) == 0) lds[threadIdx.x >> 6] = g;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double t = 0.0;
+    for (int w = 0; w < PCG_RES_BS / 64; w++) t += lds[w];
+    ((double *)m.gpart)[blockIdx.x] = t;
+  }
+}
+
+__global__ void k_pcg_mixed_update(PcgArgs a, PcgArgs32 m) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < a.n) a.x[i] = a.x[i] + m.x[i];
+}
+
+__global__ void k_pcg_mixed_final(PcgArgs32 m, double tol) {
+  if (threadIdx.x != 0) return;
+  double g = 0.0;
+  for (int q = 0; q < m.ngpart; q++) g = g + m.gpart[q];
+  const double g0 = *m.g0;
+  m.info[1] = (g0 > 0.0 && sqrt(g * g / (g0 * g0)) < tol) || g0 == 0.0 ? 1 : 0;
+}
+
+__global__ void k_pcg_mixed_init(int *info) {
+  if (threadIdx.x == 0) {
+    info[0] = 0;
+    info[1] = 0;
+    info[2] = 0;
+  }
+}
+
+// The sequence on the buffers of a PCG context: m holds the FP32 arrays and the partials.
+extern "C" hipError_t gom_launch_pcg_mixed(const PcgArgs *a64, const PcgArgs32 *m32, int cooperative,
+                                           hipStream_t st) {
+  PcgArgs a = *a64;
+  PcgArgs32 m = *m32;
+  const int nbr = nblk(a.n, PCG_RES_BS) < PCG_RES_BLOCKS ? nblk(a.n, PCG_RES_BS) : PCG_RES_BLOCKS;
+  m.ngpart = nbr;
+  const int nb = gom_pcg_cg1m_blocks(a.n);
+  if (nb > CG1M_BS || nb > PCG_MAX_SLOT_BLOCKS) return hipErrorInvalidValue;
+  k_pcg_mixed_init<<<1, 64, 0, st>>>(m.info);
+  for (int r = 0; r < PCG_MIXED_REFINE; r++) {
+    m.first = r == 0;
+    k_pcg_mixed_res<<<nbr, PCG_RES_BS, 0, st>>>(a, m, r == 0, 1);
+    void *kargs[] = {&m};
+    const void *k = (const void *)k_pcg_cg1m32<CG1M_BS>;
+    hipError_t e = cooperative ? hipLaunchCooperativeKernel(k, dim3(nb), dim3(CG1M_BS), kargs, 0, st)
+                               : hipLaunchKernel(k, dim3(nb), dim3(CG1M_BS), kargs, 0, st);
+    if (e != hipSuccess) return e;
+    k_pcg_mixed_update<<<nblk(a.n, 256), 256, 0, st>>>(a, m);
+  }
+  k_pcg_mixed_res<<<nbr, PCG_RES_BS, 0, st>>>(a, m, 0, 0);
+  k_pcg_mixed_final<<<1, 64, 0, st>>>(m, a.tol);
+  return hipGetLastError();
+}
+
+extern "C" int gom_pcg_cg1m32_capacity(int device) {
+  int per_cu = 0, ncu = 0;
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, (const void *)k_pcg_cg1m32<CG1M_BS>, CG1M_BS, 0) !=
+      hipSuccess)
+    return 0;
   if (hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, device) != hipSuccess) return 0;
   return per_cu * ncu;
 }

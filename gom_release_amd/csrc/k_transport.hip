@@ -8,6 +8,45 @@
 #include "gomgpu_dev.h"
 
 
+// The transport's work arrays, typed. W = double is the reference's FP64 storage. W = float
+// (GOMGPU_MIXED) stores in FP32 the horizontal limiter arrays (phi and PSI of every
+// face-layer and the per-cell upwind sums that form the face ratios) and the node/face means
+// of the output; every value is widened to FP64 when loaded and the arithmetic is FP64. The
+// concentrations, the fluxes and diffusion coefficients (horizontal and vertical), the
+// vertical limiter and the column solve stay FP64: a thin partial layer's update divides by
+// area * dz, which magnifies a rounding of its fluxes by the layer's Courant number, while
+// the limiter terms multiply concentration differences, not concentrations. The
+// allocations are the FP64 ones; an FP32 array uses their first half.
+template <typename W>
+struct TvdW {
+  double *con, *con_new, *Qj, *dj, *Qi, *di, *phii, *htop, *PSIi, *obk, *qbk, *tri_gam, *tvd_x;
+  W *phij, *PSIj, *s1a, *s2a, *node_new, *face_new;
+};
+template <typename W>
+static TvdW<W> tvd_view(const DevWork &w) {
+  TvdW<W> v;
+  v.con = w.con;
+  v.con_new = w.con_new;
+  v.Qi = w.Qi;
+  v.di = w.di;
+  v.phii = w.phii;
+  v.htop = w.htop;
+  v.PSIi = w.PSIi;
+  v.obk = w.obk;
+  v.qbk = w.qbk;
+  v.tri_gam = w.tri_gam;
+  v.tvd_x = w.tvd_x;
+  v.Qj = w.Qj;
+  v.dj = w.dj;
+  v.phij = (W *)w.phij;
+  v.PSIj = (W *)w.PSIj;
+  v.s1a = (W *)w.s1a;
+  v.s2a = (W *)w.s2a;
+  v.node_new = (W *)w.node_new;
+  v.face_new = (W *)w.face_new;
+  return v;
+}
+
 __device__ __forceinline__ size_t cix(int t, int k, int e, int L2, int n) {  // con (0:L+1)
   return ((size_t)t * L2 + k) * n + e;
 }
@@ -19,7 +58,8 @@ __device__ __forceinline__ size_t zix(int t, int k, int e, int L1, int n) {  // 
 }
 
 // con_cell from salt/temp (+ surface heat source), solve_transport_equation_v19.f90:88-173
-__global__ void k_tvd_init(DevMesh m, DevState s, DevForcing f, DevWork w, DevDims d, TvdArgs a) {
+template <typename W>
+__global__ void k_tvd_init(DevMesh m, DevState s, DevForcing f, TvdW<W> w, DevDims d, TvdArgs a) {
   int q = blockIdx.x * blockDim.x + threadIdx.x;
   const int E = d.E, L = d.L, L2 = d.L + 2;
   if (q >= E * L2) return;
@@ -43,7 +83,8 @@ __global__ void k_tvd_init(DevMesh m, DevState s, DevForcing f, DevWork w, DevDi
 
 // Boundary concentrations of open-boundary and river cells for sub-cycle a.sub
 // (solve_transport_equation_v19.f90:291-397); one thread per (layer, boundary cell).
-__global__ void k_tvd_bnd(DevMesh m, DevState s, DevForcing f, DevWork w, DevDims d, TvdArgs a) {
+template <typename W>
+__global__ void k_tvd_bnd(DevMesh m, DevState s, DevForcing f, TvdW<W> w, DevDims d, TvdArgs a) {
   int q = blockIdx.x * blockDim.x + threadIdx.x;
   const int L = d.L, E = d.E, J = d.J, L2 = d.L + 2;
   int nob = d.NOB, nqb = d.NQB;
@@ -81,7 +122,8 @@ __global__ void k_tvd_bnd(DevMesh m, DevState s, DevForcing f, DevWork w, DevDim
 }
 
 // Face coefficients Q, d, phi (solve_transport_equation_v19.f90:552-581)
-__global__ void k_tvd_coef_face(DevMesh m, DevState s, DevWork w, DevDims d, TvdArgs a) {
+template <typename W>
+__global__ void k_tvd_coef_face(DevMesh m, DevState s, TvdW<W> w, DevDims d, TvdArgs a) {
   int q = blockIdx.x * blockDim.x + threadIdx.x;
   const int J = d.J;
   if (q >= J * d.L) return;
@@ -103,7 +145,8 @@ __global__ void k_tvd_coef_face(DevMesh m, DevState s, DevWork w, DevDims d, Tvd
 }
 
 // Concentration across face slot l of cell e at layer k (boundary value for no neighbour).
-__device__ __forceinline__ double con_across(const DevMesh &m, const DevWork &w, const DevDims &d, int e,
+template <typename W>
+__device__ __forceinline__ double con_across(const DevMesh &m, const TvdW<W> &w, const DevDims &d, int e,
                                              int ii, int k, int t) {
   const int E = d.E, L = d.L, L2 = d.L + 2;
   if (ii > 0) return w.con[cix(t, k, ii - 1, L2, E)];
@@ -132,7 +175,8 @@ __device__ __forceinline__ double limiter(int which, double r, double phi) {
 // Horizontal upwind sums of one cell-layer (solve_transport_equation_v19.f90:700-760): the
 // inflow-weighted differences sum1 and the inflows sum2, over the cell's active faces in
 // slot order.
-__device__ __forceinline__ void lim_hsums(const DevMesh &m, const DevWork &w, const DevDims &d, int e, int k,
+template <typename W>
+__device__ __forceinline__ void lim_hsums(const DevMesh &m, const TvdW<W> &w, const DevDims &d, int e, int k,
                                           double *sum1, double *sum2) {
   const int E = d.E, L2 = d.L + 2, J = d.J;
   sum1[0] = sum1[1] = 0.0;
@@ -156,7 +200,8 @@ __device__ __forceinline__ void lim_hsums(const DevMesh &m, const DevWork &w, co
 // Cell (vertical) coefficients (solve_transport_equation_v19.f90:586-609). With a flux
 // limiter, the thread of a wet column's top layer also leaves that layer's horizontal upwind
 // sums in w.htop, which every vertical limiter ratio of the column uses (k_tvd_lim_cell).
-__global__ void k_tvd_coef_cell(DevMesh m, DevState s, DevWork w, DevDims d, TvdArgs a) {
+template <typename W>
+__global__ void k_tvd_coef_cell(DevMesh m, DevState s, TvdW<W> w, DevDims d, TvdArgs a) {
   int q = blockIdx.x * blockDim.x + threadIdx.x;
   const int E = d.E;
   if (q >= E * (d.L + 1)) return;
@@ -196,7 +241,8 @@ __global__ void k_tvd_coef_cell(DevMesh m, DevState s, DevWork w, DevDims d, Tvd
 // vertical ratio r_ik (level k) with its limiter PSI_ik (level k). As in the reference, the
 // vertical ratios reuse the horizontal sums left over from the column's top layer, which
 // k_tvd_coef_cell computed once per column (w.htop) in the same order.
-__global__ void k_tvd_lim_cell(DevMesh m, DevState s, DevWork w, DevDims d, int hlim, int first) {
+template <typename W>
+__global__ void k_tvd_lim_cell(DevMesh m, DevState s, TvdW<W> w, DevDims d, int hlim, int first) {
   long q = (long)blockIdx.x * blockDim.x + threadIdx.x;
   const int E = d.E, L = d.L, L1 = d.L + 1, L2 = d.L + 2;
   if (q >= (long)L1 * E) return;
@@ -273,7 +319,8 @@ __global__ void k_tvd_lim_cell(DevMesh m, DevState s, DevWork w, DevDims d, int 
 // Face ratio r_jk and PSI_jk. In the reference each adjacent cell whose outflow the face
 // carries (Q2 >= 0 from its side) writes r_jk in cell order, so the higher cell id wins
 // a tie (Q = 0) -- the same writer is chosen here.
-__global__ void k_tvd_lim_face(DevMesh m, DevState s, DevWork w, DevDims d, int hlim, int first) {
+template <typename W>
+__global__ void k_tvd_lim_face(DevMesh m, DevState s, TvdW<W> w, DevDims d, int hlim, int first) {
   int q = blockIdx.x * blockDim.x + threadIdx.x;
   const int J = d.J, E = d.E, L = d.L, L2 = d.L + 2;
   if (q >= J * L) return;
@@ -313,7 +360,8 @@ __global__ void k_tvd_lim_face(DevMesh m, DevState s, DevWork w, DevDims d, int 
 // the layer); k_tvd_solve, one thread per cell column: vertical diffusion implicit (the
 // Thomas recurrence of tridiagonal_solver.f90 with NT right-hand sides, in place in
 // w.tvd_x, elimination factors in w.tri_gam), then the clamp to [0, 50] into con_new.
-__global__ void k_tvd_rhs(DevMesh m, DevState s, DevWork w, DevDims d, TvdArgs a) {
+template <typename W>
+__global__ void k_tvd_rhs(DevMesh m, DevState s, TvdW<W> w, DevDims d, TvdArgs a) {
   long q = (long)blockIdx.x * blockDim.x + threadIdx.x;
   const int E = d.E, L = d.L, L1 = d.L + 1, L2 = d.L + 2, J = d.J;
   if (q >= (long)L * E) return;
@@ -391,7 +439,8 @@ __global__ void k_tvd_rhs(DevMesh m, DevState s, DevWork w, DevDims d, TvdArgs a
 }
 
 // Tridiagonal row r (0-based, surface first) of cell e: solve_transport_equation_v19.f90:957-1025
-__device__ __forceinline__ void tvd_row(const DevState &s, const DevWork &w, int e, int E, int tl, int bl, int r,
+template <typename W>
+__device__ __forceinline__ void tvd_row(const DevState &s, const TvdW<W> &w, int e, int E, int tl, int bl, int r,
                                         double ar, double dt2, int sub, int Nt, double &av, double &bv, double &cv) {
   int nv = tl - bl + 1;
   av = 0.0;
@@ -420,7 +469,8 @@ __device__ __forceinline__ void tvd_row(const DevState &s, const DevWork &w, int
   }
 }
 
-__global__ void k_tvd_solve(DevMesh m, DevState s, DevWork w, DevDims d, TvdArgs a, int *err) {
+template <typename W>
+__global__ void k_tvd_solve(DevMesh m, DevState s, TvdW<W> w, DevDims d, TvdArgs a, int *err) {
   int e = blockIdx.x * blockDim.x + threadIdx.x;
   const int E = d.E, L = d.L;
   if (e >= E) return;
@@ -488,7 +538,8 @@ __global__ void k_tvd_solve(DevMesh m, DevState s, DevWork w, DevDims d, TvdArgs
 
 // con <- con_new in the active range, zero elsewhere (solve_transport_equation_v19.f90:1352-1371)
 // (out of the range the zeros written after the first sub-cycle stay)
-__global__ void k_tvd_copyback(DevMesh m, DevState s, DevWork w, DevDims d, int first) {
+template <typename W>
+__global__ void k_tvd_copyback(DevMesh m, DevState s, TvdW<W> w, DevDims d, int first) {
   int q = blockIdx.x * blockDim.x + threadIdx.x;
   const int E = d.E, L = d.L, L2 = d.L + 2;
   if (q >= E * L2) return;
@@ -502,7 +553,8 @@ __global__ void k_tvd_copyback(DevMesh m, DevState s, DevWork w, DevDims d, int 
 
 // Node values: area*dz weighted cell means, top down (solve_transport_equation_v19.f90:1403-1453).
 // The mean of one level; `none` is set when no adjacent cell is wet at that level.
-__device__ __forceinline__ double tvd_node_mean(const DevMesh &m, const DevState &s, const DevWork &w,
+template <typename W>
+__device__ __forceinline__ double tvd_node_mean(const DevMesh &m, const DevState &s, const TvdW<W> &w,
                                                 const DevDims &d, int i, int k, int t, bool &none) {
   const int P = d.P, E = d.E, L = d.L;
   const int nadj = m.adj_cells_at_node[i];
@@ -526,7 +578,8 @@ __device__ __forceinline__ double tvd_node_mean(const DevMesh &m, const DevState
 // level whose adjacent cells are all dry takes the value of the level above it (unless it
 // is the node's top level); at the top of a chunk that value is found by walking up.
 #define TVD_NODE_LEVELS GOM_TVD_NODE_LEVELS
-__global__ void k_tvd_node(DevMesh m, DevState s, DevWork w, DevDims d) {
+template <typename W>
+__global__ void k_tvd_node(DevMesh m, DevState s, TvdW<W> w, DevDims d) {
   const int P = d.P, L = d.L;
   const int nch = (L + TVD_NODE_LEVELS - 1) / TVD_NODE_LEVELS;
   long q = (long)blockIdx.x * blockDim.x + threadIdx.x;
@@ -561,7 +614,8 @@ __global__ void k_tvd_node(DevMesh m, DevState s, DevWork w, DevDims d) {
 }
 
 // Face values and the copy to the model arrays (solve_transport_equation_v19.f90:1457-1482)
-__global__ void k_tvd_face_out(DevMesh m, DevState s, DevWork w, DevDims d) {
+template <typename W>
+__global__ void k_tvd_face_out(DevMesh m, DevState s, TvdW<W> w, DevDims d) {
   int q = blockIdx.x * blockDim.x + threadIdx.x;
   const int J = d.J, P = d.P, L = d.L;
   if (q >= J * L) return;
@@ -569,13 +623,14 @@ __global__ void k_tvd_face_out(DevMesh m, DevState s, DevWork w, DevDims d) {
   int n1 = TB(m.nodenum_at_face, 1, j, J) - 1, n2 = TB(m.nodenum_at_face, 2, j, J) - 1;
   bool in = k >= m.bottom_layer_at_face[j] && k <= s.top_layer_at_face[j];
   for (int t = 0; t < d.NT; t++) {
-    double v = in ? (w.node_new[nix(t, k, n1, L, P)] + w.node_new[nix(t, k, n2, L, P)]) * 0.5 : 0.0;
+    double v = in ? ((double)w.node_new[nix(t, k, n1, L, P)] + w.node_new[nix(t, k, n2, L, P)]) * 0.5 : 0.0;
     double *dst = d.tid[t] == 1 ? s.salt_face : s.temp_face;
     AL(dst, k, j, J) = v;
   }
 }
 
-__global__ void k_tvd_cellnode_out(DevState s, DevWork w, DevDims d) {
+template <typename W>
+__global__ void k_tvd_cellnode_out(DevState s, TvdW<W> w, DevDims d) {
   int q = blockIdx.x * blockDim.x + threadIdx.x;
   const int E = d.E, P = d.P, L = d.L;
   long nc = (long)E * L, nn = (long)P * L;
@@ -764,3 +819,64 @@ __global__ void k_budget(DevMesh m, DevState s, DevDims d, double *part) {
     part[2 * blockIdx.x + 1] = ls[0];
   }
 }
+
+// ---- launchers: the FP64 or the FP32-storage instantiation of each transport kernel
+#define TVD_GRID(n, bs) dim3(nblk((n), (bs))), dim3(bs), 0, st
+void tvd_launch_init(bool f32, long n, int bs, hipStream_t st, DevMesh m, DevState s, DevForcing f, DevWork w,
+                     DevDims d, TvdArgs a) {
+  if (f32) k_tvd_init<float><<<TVD_GRID(n, bs)>>>(m, s, f, tvd_view<float>(w), d, a);
+  else k_tvd_init<double><<<TVD_GRID(n, bs)>>>(m, s, f, tvd_view<double>(w), d, a);
+}
+void tvd_launch_bnd(bool f32, long n, int bs, hipStream_t st, DevMesh m, DevState s, DevForcing f, DevWork w,
+                    DevDims d, TvdArgs a) {
+  if (f32) k_tvd_bnd<float><<<TVD_GRID(n, bs)>>>(m, s, f, tvd_view<float>(w), d, a);
+  else k_tvd_bnd<double><<<TVD_GRID(n, bs)>>>(m, s, f, tvd_view<double>(w), d, a);
+}
+void tvd_launch_coef_face(bool f32, long n, int bs, hipStream_t st, DevMesh m, DevState s, DevWork w, DevDims d,
+                          TvdArgs a) {
+  if (f32) k_tvd_coef_face<float><<<TVD_GRID(n, bs)>>>(m, s, tvd_view<float>(w), d, a);
+  else k_tvd_coef_face<double><<<TVD_GRID(n, bs)>>>(m, s, tvd_view<double>(w), d, a);
+}
+void tvd_launch_coef_cell(bool f32, long n, int bs, hipStream_t st, DevMesh m, DevState s, DevWork w, DevDims d,
+                          TvdArgs a) {
+  if (f32) k_tvd_coef_cell<float><<<TVD_GRID(n, bs)>>>(m, s, tvd_view<float>(w), d, a);
+  else k_tvd_coef_cell<double><<<TVD_GRID(n, bs)>>>(m, s, tvd_view<double>(w), d, a);
+}
+void tvd_launch_lim_cell(bool f32, long n, int bs, hipStream_t st, DevMesh m, DevState s, DevWork w, DevDims d,
+                         int hlim, int first) {
+  if (f32) k_tvd_lim_cell<float><<<TVD_GRID(n, bs)>>>(m, s, tvd_view<float>(w), d, hlim, first);
+  else k_tvd_lim_cell<double><<<TVD_GRID(n, bs)>>>(m, s, tvd_view<double>(w), d, hlim, first);
+}
+void tvd_launch_lim_face(bool f32, long n, int bs, hipStream_t st, DevMesh m, DevState s, DevWork w, DevDims d,
+                         int hlim, int first) {
+  if (f32) k_tvd_lim_face<float><<<TVD_GRID(n, bs)>>>(m, s, tvd_view<float>(w), d, hlim, first);
+  else k_tvd_lim_face<double><<<TVD_GRID(n, bs)>>>(m, s, tvd_view<double>(w), d, hlim, first);
+}
+void tvd_launch_rhs(bool f32, long n, int bs, hipStream_t st, DevMesh m, DevState s, DevWork w, DevDims d,
+                    TvdArgs a) {
+  if (f32) k_tvd_rhs<float><<<TVD_GRID(n, bs)>>>(m, s, tvd_view<float>(w), d, a);
+  else k_tvd_rhs<double><<<TVD_GRID(n, bs)>>>(m, s, tvd_view<double>(w), d, a);
+}
+void tvd_launch_solve(bool f32, long n, int bs, hipStream_t st, DevMesh m, DevState s, DevWork w, DevDims d,
+                      TvdArgs a, int *err) {
+  if (f32) k_tvd_solve<float><<<TVD_GRID(n, bs)>>>(m, s, tvd_view<float>(w), d, a, err);
+  else k_tvd_solve<double><<<TVD_GRID(n, bs)>>>(m, s, tvd_view<double>(w), d, a, err);
+}
+void tvd_launch_copyback(bool f32, long n, int bs, hipStream_t st, DevMesh m, DevState s, DevWork w, DevDims d,
+                         int first) {
+  if (f32) k_tvd_copyback<float><<<TVD_GRID(n, bs)>>>(m, s, tvd_view<float>(w), d, first);
+  else k_tvd_copyback<double><<<TVD_GRID(n, bs)>>>(m, s, tvd_view<double>(w), d, first);
+}
+void tvd_launch_node(bool f32, long n, int bs, hipStream_t st, DevMesh m, DevState s, DevWork w, DevDims d) {
+  if (f32) k_tvd_node<float><<<TVD_GRID(n, bs)>>>(m, s, tvd_view<float>(w), d);
+  else k_tvd_node<double><<<TVD_GRID(n, bs)>>>(m, s, tvd_view<double>(w), d);
+}
+void tvd_launch_face_out(bool f32, long n, int bs, hipStream_t st, DevMesh m, DevState s, DevWork w, DevDims d) {
+  if (f32) k_tvd_face_out<float><<<TVD_GRID(n, bs)>>>(m, s, tvd_view<float>(w), d);
+  else k_tvd_face_out<double><<<TVD_GRID(n, bs)>>>(m, s, tvd_view<double>(w), d);
+}
+void tvd_launch_cellnode_out(bool f32, long n, int bs, hipStream_t st, DevState s, DevWork w, DevDims d) {
+  if (f32) k_tvd_cellnode_out<float><<<TVD_GRID(n, bs)>>>(s, tvd_view<float>(w), d);
+  else k_tvd_cellnode_out<double><<<TVD_GRID(n, bs)>>>(s, tvd_view<double>(w), d);
+}
+#undef TVD_GRID
