@@ -362,11 +362,13 @@ def main() -> int:
     model.sync()
     print(f"[bench] warmup done ({args.warmup} steps)", file=sys.stderr, flush=True)
     a0 = sc.active_cell_layers()
+    model.elm_near_edge(reset=True)
     model.sync()
     t0 = time.perf_counter()
     sc.run(args.steps)
     model.sync()
     t1 = time.perf_counter()
+    near = model.elm_near_edge()
     a1 = sc.active_cell_layers()
     elapsed = t1 - t0
     active = 0.5 * (a0 + a1) * args.steps
@@ -438,6 +440,9 @@ def main() -> int:
                    "volume_m3": vol, "salt_psu_m3": salt},
         "roofline": roof,
         "launch": launch,
+        # ELM departure points within 1e-12 of a cell edge over the timed steps (their
+        # departure cell rests on the last bit of a coordinate), of the active face-layers
+        "elm_near_edge": {"momentum": near[0], "tracer": near[1], "face_layer_steps": F * args.steps},
     }
     model.close()
     if ref_fields is not None:

@@ -103,6 +103,10 @@ struct DevWork {
   // budgets
   double *budget;         // [0] volume, [1] salt
   int *err;               // [0] code, [1] entity
+  // departure points within 1e-12 of a cell edge, counted since creation or the last reset:
+  // [0] momentum ELM, [1] tracer ELM
+  unsigned long long *elm_near;
+  int32_t *elm_dep;       // departure cells then levels per face-layer (2 L J), NULL unless recorded
 };
 
 // ------------------------------------------------------------------ index helpers

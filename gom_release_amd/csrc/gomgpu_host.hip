@@ -14,7 +14,7 @@
 // kernels (k_hydro.hip, k_transport.hip, k_pcg.hip)
 __global__ void k_friction(DevMesh, DevState, DevDims, int, double, double);
 __global__ void k_elm_substeps(DevMesh, DevState, DevDims, double, double, int, int);
-__global__ void k_elm(DevMesh, DevState, DevDims, double, double, double, int *);
+__global__ void k_elm(DevMesh, DevState, DevDims, double, double, double, int *, int32_t *, unsigned long long *);
 __global__ void k_elm_copy(DevState, long);
 __global__ void k_elm_tracer(DevMesh, DevState, DevForcing, DevWork, DevDims, double, double, double, int, int *);
 __global__ void k_elm_column(DevMesh, DevState, DevWork, DevDims, double, int, int, int *);
@@ -157,6 +157,8 @@ struct gomgpu_ctx {
   double *d_stage = nullptr;  // its device image: one copy, then k_scatter to the fields
   size_t stage_len = 0;
   hipEvent_t fev = nullptr;   // completion of the last forcing upload
+  int32_t *dep_buf = nullptr; // departure cells and levels of the momentum ELM (allocated on
+                              // the first gomgpu_elm_record; w.elm_dep points here while on)
 };
 
 static int seterr(gomgpu_ctx *c, int code, const std::string &msg) {
@@ -498,7 +500,7 @@ extern "C" int gomgpu_create(const gomgpu_mesh *mesh, const gomgpu_params *param
         (rc = dalloc(c, &w.s2a, NT * L * E)) || (rc = dalloc(c, &w.obk, NT * L * OB)) ||
         (rc = dalloc(c, &w.qbk, NT * L * QB)) || (rc = dalloc(c, &w.node_new, NT * L * P)) ||
         (rc = dalloc(c, &w.face_new, NT * L * J)) || (rc = dalloc(c, &w.budget, 2 * ((E + 255) / 256) + 2)) ||
-        (rc = dalloc(c, &w.err, 2)) || (rc = dalloc(c, &w.turb_kvf, (L + 1) * J)) ||
+        (rc = dalloc(c, &w.err, 2)) || (rc = dalloc(c, &w.elm_near, 2)) || (rc = dalloc(c, &w.turb_kvf, (L + 1) * J)) ||
         (rc = dalloc(c, &w.tri_gam, L * (E > J ? E : J))) || (rc = dalloc(c, &w.tvd_x, NT * L * E)) ||
         (rc = dalloc(c, &w.cell_u, L * E)) || (rc = dalloc(c, &w.cell_v, L * E)))
       goto fail;
@@ -869,7 +871,8 @@ static int stage_impl(gomgpu_ctx *c, int id) {
           k_elm_substeps<<<nblk((long)d.P * d.L, BS), BS, 0, st>>>(c->m, c->s, d, p.dt, p.delta_z_min,
                                                                    p.ELM_min_iter, p.ELM_max_iter);
         KT_BEGIN(KT_ELM);
-        k_elm<<<nblk((long)d.J * d.L, LB(ELM)), LB(ELM), 0, st>>>(c->m, c->s, d, p.MSL, p.dt, p.adv_onoff_depth, err);
+        k_elm<<<nblk((long)d.J * d.L, LB(ELM)), LB(ELM), 0, st>>>(c->m, c->s, d, p.MSL, p.dt, p.adv_onoff_depth, err,
+                                                                  c->w.elm_dep, c->w.elm_near);
         KT_END(KT_ELM);
       }
       break;
@@ -1544,6 +1547,7 @@ extern "C" long long gomgpu_footprint(const gomgpu_params *params) {
   add(NT * L * OB, D); add(NT * L * QB, D);                 // obk, qbk
   add(NT * L * P, D); add(NT * L * J, D);                   // node_new, face_new
   add(2 * ((E + 255) / 256) + 2, D); add(2, sizeof(int));   // budget, err
+  add(2, sizeof(unsigned long long));                       // elm_near
   add((L + 1) * J, D); add(L * (E > J ? E : J), D); add(NT * L * E, D);  // turb_kvf, tri_gam, tvd_x
   add(L * E, D); add(L * E, D);                                           // cell_u, cell_v
   if (p.turbulence_flag == 3)
@@ -1564,6 +1568,45 @@ extern "C" int gomgpu_set_launch(gomgpu_ctx *c, int which, int block) {
   if (block < 64 || block > launch_max(which) || (block & (block - 1)) != 0)
     return seterr(c, GOMGPU_E_ARG, "block size must be a power of two in [64, max]");
   c->lb[which] = block;
+  return 0;
+}
+
+extern "C" int gomgpu_elm_record(gomgpu_ctx *c, int record) {
+  if (!c) return GOMGPU_E_ARG;
+  if (record && !c->dep_buf) {
+    int rc = dalloc(c, &c->dep_buf, 2 * (size_t)c->d.L * c->d.J);
+    if (rc) return rc;
+    HIPCHK(c, hipStreamSynchronize(c->st));
+  }
+  int32_t *want = record ? c->dep_buf : nullptr;
+  if (want != c->w.elm_dep) {
+    c->w.elm_dep = want;
+    for (gomgpu_batch *b : c->batches) b->dirty = true;  // a captured step holds the pointer
+  }
+  return 0;
+}
+
+extern "C" int gomgpu_elm_departures(gomgpu_ctx *c, int32_t *nnel, int32_t *jlev) {
+  if (!c) return GOMGPU_E_ARG;
+  if (!c->dep_buf) return seterr(c, GOMGPU_E_ARG, "ELM departures are not recorded (gomgpu_elm_record)");
+  HIPCHK(c, hipStreamSynchronize(c->st));  // leaves the latched status to gomgpu_sync
+  const size_t n = (size_t)c->d.L * c->d.J;
+  if (nnel) HIPCHK(c, hipMemcpy(nnel, c->dep_buf, n * sizeof(int32_t), hipMemcpyDeviceToHost));
+  if (jlev) HIPCHK(c, hipMemcpy(jlev, c->dep_buf + n, n * sizeof(int32_t), hipMemcpyDeviceToHost));
+  return 0;
+}
+
+extern "C" int gomgpu_elm_near_edge(gomgpu_ctx *c, long long *counts, int reset) {
+  if (!c || !counts) return GOMGPU_E_ARG;
+  HIPCHK(c, hipStreamSynchronize(c->st));
+  unsigned long long h[2];
+  HIPCHK(c, hipMemcpy(h, c->w.elm_near, sizeof h, hipMemcpyDeviceToHost));
+  counts[0] = (long long)h[0];
+  counts[1] = (long long)h[1];
+  if (reset) {
+    const unsigned long long zero[2] = {0, 0};
+    HIPCHK(c, hipMemcpy(c->w.elm_near, zero, sizeof zero, hipMemcpyHostToDevice));
+  }
   return 0;
 }
 

@@ -212,6 +212,23 @@ __device__ __forceinline__ double area_sum(const CellRec &g, double px, double p
   return sum;
 }
 
+// Whether (px, py) lies within 1e-12 of an edge of the cell, relative to the edge's length:
+// |(B - A) x (P - A)| <= 1e-12 |B - A|^2 for some edge A -> B. The containment and crossing
+// tests (search_straight_line.f90:60-100, check_intersection.f90:34-48, inclusive bounds)
+// can assign such a point to either cell sharing the edge on the last bit of its
+// coordinates, so its departure cell is reproducible only by the same arithmetic.
+__device__ __forceinline__ bool near_edge(const CellRec &g, double px, double py) {
+  bool near = false;
+#pragma unroll
+  for (int l = 0; l < 4; l++) {
+    const double xb = l + 1 < g.N ? g.X[(l + 1) & 3] : g.X[0], yb = l + 1 < g.N ? g.Y[(l + 1) & 3] : g.Y[0];
+    const double ex = xb - g.X[l], ey = yb - g.Y[l];
+    const double cr = ex * (py - g.Y[l]) - ey * (px - g.X[l]);
+    near = near || (l < g.N && fabs(cr) <= 1.0e-12 * (ex * ex + ey * ey));
+  }
+  return near;
+}
+
 // The first edge l (slots sen(l,1), sen(l,2), l = 1..N in order) that the segment
 // (xa,ya)-(xb,yb) meets, skipping the edge with node ids {md1, md2} when md1 > 0
 // (search_straight_line.f90:144-169, 320-350). Every edge is tested; the lowest hit wins,
@@ -347,9 +364,10 @@ struct ElmFoot {
   double xt, yt, zt, zup, st[4];
 };
 
-// ELM_backtrace_v1.f90:15-200 (i_which_backtrack = 1)
+// ELM_backtrace_v1.f90:15-200 (i_which_backtrack = 1). near is set when the end point of
+// any sub-step lies within 1e-12 of an edge of its cell (near_edge).
 __device__ __forceinline__ int backtrace(const ElmCtx &c, int bt_step, double bt_dt, double &uu, double &vv, double &ww,
-                         double x0, double y0, double z0, int &nnel, int &jlev, ElmFoot &ft) {
+                         double x0, double y0, double z0, int &nnel, int &jlev, ElmFoot &ft, bool &near) {
   const DevMesh &m = c.m;
   const DevState &s = c.s;
   const int E = c.d.E, P = c.d.P;
@@ -362,6 +380,7 @@ __device__ __forceinline__ int backtrace(const ElmCtx &c, int bt_step, double bt
     CellRec rc;
     int err = search_line(c, nnel, jlev, bt_dt, x0, y0, z0, xt, yt, zt, iflqs1, rc);
     if (err) return err;
+    near = near || near_edge(rc, xt, yt);
     int e0 = nnel - 1, N = rc.N, jl = jlev;
     double zup = (jl == s.top_layer_at_element[e0]) ? c.MSL + s.eta_cell[e0] : m.z_level[jl];
     if (zt > zup) return GOMGPU_ERR_ELM_VERTICAL;
@@ -422,15 +441,22 @@ __device__ __forceinline__ int backtrace(const ElmCtx &c, int bt_step, double bt
   return 0;
 }
 
-// solve_nonlinear_advection.f90:92-202; one thread per (layer, face).
+// solve_nonlinear_advection.f90:92-202; one thread per (layer, face). Departures within 1e-12
+// of a cell edge are counted in near_count. With dep non-NULL, the departure cell (negated
+// for such a point) and level of face-layer t = (k-1) J + j go to dep[t] and dep[L J + t]
+// (0 where nothing is backtraced).
 __global__ void __launch_bounds__(256) k_elm(DevMesh m, DevState s, DevDims d, double MSL, double dt, double onoff,
-                                             int *err) {
+                                             int *err, int32_t *dep, unsigned long long *near_count) {
   int t = blockIdx.x * blockDim.x + threadIdx.x;
   const int J = d.J, E = d.E, P = d.P;
   if (t >= J * d.L) return;
   int j = t % J, k = t / J + 1;
   int kb = m.bottom_layer_at_face[j], kt = s.top_layer_at_face[j];
   double un = AL(s.un_face, k, j, J), vn = AL(s.vn_face, k, j, J);
+  if (dep) {
+    dep[t] = 0;
+    dep[(size_t)J * d.L + t] = 0;
+  }
   if (k < kb || k > kt) {
     AL(s.un_ELM, k, j, J) = 0.0;
     AL(s.vn_ELM, k, j, J) = 0.0;
@@ -473,8 +499,14 @@ __global__ void __launch_bounds__(256) k_elm(DevMesh m, DevState s, DevDims d, d
   ElmCtx c{m, s, d, MSL};
   int nnel = ie0, jlev = k;
   ElmFoot ft;
-  int e = backtrace(c, bt_step, bt_dt, uu, vv, ww, m.x_face[j], m.y_face[j], z0, nnel, jlev, ft);
+  bool near = false;
+  int e = backtrace(c, bt_step, bt_dt, uu, vv, ww, m.x_face[j], m.y_face[j], z0, nnel, jlev, ft, near);
   if (e) latch(err, e, j + 1);
+  if (near) atomicAdd(near_count, 1ull);
+  if (dep) {
+    dep[t] = near ? -nnel : nnel;
+    dep[(size_t)J * d.L + t] = jlev;
+  }
   AL(s.un_ELM, k, j, J) = uu * ct + vv * st;
   AL(s.vn_ELM, k, j, J) = -uu * st + vv * ct;
 }
@@ -720,7 +752,9 @@ __global__ void __launch_bounds__(256) k_elm_tracer(DevMesh m, DevState s, DevFo
     ElmCtx c{m, s, d, MSL};
     ElmFoot ft;
     int nnel = i + 1, jlev = k;
-    int e = backtrace(c, bt_step, bt_dt, uu, vv, ww, m.x_cell[i], m.y_cell[i], z0, nnel, jlev, ft);
+    bool near = false;
+    int e = backtrace(c, bt_step, bt_dt, uu, vv, ww, m.x_cell[i], m.y_cell[i], z0, nnel, jlev, ft, near);
+    if (near) atomicAdd(w.elm_near + 1, 1ull);
     if (!e) e = elm_tracer_foot(m, s, f, d, Nt, nnel, jlev, ft, tc, sc);
     if (e) {
       latch(err, e, i + 1);

@@ -53,6 +53,9 @@ def lib():
         L.gomgpu_sync.argtypes = [vp]
         L.gomgpu_budgets.argtypes = [vp, C.POINTER(C.c_double), C.POINTER(C.c_double)]
         L.gomgpu_pcg_iterations.argtypes = [vp]
+        L.gomgpu_elm_record.argtypes = [vp, C.c_int]
+        L.gomgpu_elm_departures.argtypes = [vp, vp, vp]
+        L.gomgpu_elm_near_edge.argtypes = [vp, C.POINTER(C.c_longlong), C.c_int]
         L.gomgpu_last_error.argtypes = [vp]
         L.gomgpu_last_error.restype = C.c_char_p
         L.gomgpu_timings.argtypes = [vp, C.c_int, C.POINTER(C.c_char_p), C.POINTER(C.c_double), C.c_int]
@@ -99,7 +102,8 @@ EXPORTED = ["gomgpu_create", "gomgpu_set_state", "gomgpu_set_forcing", "gomgpu_s
             "gomgpu_device_bytes", "gomgpu_batch_create", "gomgpu_batch_add", "gomgpu_batch_remove",
             "gomgpu_batch_size", "gomgpu_batch_step", "gomgpu_batch_sync", "gomgpu_batch_last_error",
             "gomgpu_batch_destroy", "gomgpu_pcg_solve", "gomgpu_pcg_workspace_bytes",
-            "gomgpu_pcg_last_error", "gomgpu_pcg_solve_noalloc"]
+            "gomgpu_pcg_last_error", "gomgpu_pcg_solve_noalloc", "gomgpu_elm_record",
+            "gomgpu_elm_departures", "gomgpu_elm_near_edge"]
 
 HIP_H2D, HIP_D2H = 1, 2
 
@@ -233,6 +237,26 @@ class GpuModel:
         v, s = C.c_double(), C.c_double()
         self._check(self._L.gomgpu_budgets(self._ctx, C.byref(v), C.byref(s)))
         return v.value, s.value
+
+    def elm_record(self, on: bool = True):
+        """Keep (on) or stop keeping the momentum ELM's departure cell and level per face-layer."""
+        self._check(self._L.gomgpu_elm_record(self._ctx, 1 if on else 0))
+
+    def elm_departures(self):
+        """(nnel, jlev) of the last recorded ELM stage, each [maxlayer, maxface] (layer k-1,
+        face j-1); nnel is 1-based and negated for a point within 1e-12 of a cell edge, 0
+        where nothing was backtraced."""
+        L, J = self.params.maxlayer, self.params.maxface
+        nnel = np.zeros((L, J), dtype=np.int32)
+        jlev = np.zeros((L, J), dtype=np.int32)
+        self._check(self._L.gomgpu_elm_departures(self._ctx, nnel.ctypes.data, jlev.ctypes.data))
+        return nnel, jlev
+
+    def elm_near_edge(self, reset: bool = False):
+        """Departure points within 1e-12 of a cell edge: (momentum ELM, tracer ELM)."""
+        c = (C.c_longlong * 2)()
+        self._check(self._L.gomgpu_elm_near_edge(self._ctx, c, 1 if reset else 0))
+        return int(c[0]), int(c[1])
 
     def timings(self, enable: bool = True) -> Dict[str, float]:
         names = (C.c_char_p * 16)()

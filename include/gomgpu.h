@@ -35,9 +35,10 @@
  *
  * Errors: every function returns 0 on success, a positive GOMGPU_E* code on failure. A
  * failing device check (the reference's `stop` conditions) is latched with the first
- * failing entity and reported by gomgpu_last_error. The PCG never fails silently:
- * non-convergence is reported as GOMGPU_W_PCG_NOT_CONVERGED by gomgpu_step, and the
- * iteration count of the last solve is available from gomgpu_pcg_iterations.
+ * failing entity and reported by gomgpu_last_error. The PCG never fails silently: a solve
+ * cut off by max_iteration_pcg is reported as GOMGPU_W_PCG_NOT_CONVERGED by the next
+ * gomgpu_sync (once; the step carries on with the last iterate, as the reference does),
+ * and the iteration count of the last solve is available from gomgpu_pcg_iterations.
  */
 #ifndef GOMGPU_H
 #define GOMGPU_H
@@ -278,6 +279,21 @@ int gomgpu_get_forcing(gomgpu_ctx *ctx, gomgpu_forcing_out *forcing);
 int gomgpu_budgets(gomgpu_ctx *ctx, double *volume, double *salt);
 /* Iterations taken by the last PCG solve (0 if it did not converge). */
 int gomgpu_pcg_iterations(gomgpu_ctx *ctx);
+/* Departure points of the momentum ELM (solve_nonlinear_advection.f90:92-202 calling
+ * ELM_backtrace_v1.f90:15-200; the reference keeps nnel and jlev in locals of that loop).
+ * A departure point within 1e-12 of an edge of its cell, relative to the edge's length,
+ * can fall in either cell sharing the edge on the last bit of its coordinates; such points
+ * are counted. gomgpu_elm_record(ctx, 1) makes every later ELM stage keep, for face-layer
+ * t = (k-1)*maxface + (j-1), the departure cell nnel (1-based, negated for a point within
+ * 1e-12 of an edge, 0 where nothing was backtraced) and level jlev (0 likewise); record = 0
+ * stops keeping them. gomgpu_elm_departures waits for the context's queued work and
+ * downloads the last kept nnel and jlev (maxlayer*maxface each; either may be NULL).
+ * gomgpu_elm_near_edge waits likewise and returns the points within 1e-12 of an edge since
+ * creation or the last reset: counts[0] momentum ELM, counts[1] tracer ELM
+ * (transport_solver 2); reset != 0 zeroes them after reading. */
+int gomgpu_elm_record(gomgpu_ctx *ctx, int record);
+int gomgpu_elm_departures(gomgpu_ctx *ctx, int32_t *nnel, int32_t *jlev);
+int gomgpu_elm_near_edge(gomgpu_ctx *ctx, long long *counts, int reset);
 /* Wait for all queued work; returns the latched model error as GOMGPU_E_MODEL. */
 int gomgpu_sync(gomgpu_ctx *ctx);
 /* Text of the last error (static storage of the context). */
@@ -329,7 +345,8 @@ int gomgpu_get_launch(gomgpu_ctx *ctx, int which);
 int gomgpu_set_pcg_rows(gomgpu_ctx *ctx, int rows);
 
 /* Device bytes gomgpu_create would allocate for these parameters (admission by HBM
- * footprint), and the bytes a context holds; -1 on bad arguments. */
+ * footprint), and the bytes a context holds (gomgpu_elm_record adds 8*maxlayer*maxface);
+ * -1 on bad arguments. */
 long long gomgpu_footprint(const gomgpu_params *params);
 long long gomgpu_device_bytes(gomgpu_ctx *ctx);
 

@@ -358,12 +358,38 @@ static int search_line(const elmctx *c, int *nnel, int *jlev, double time, doubl
   return 0;
 }
 
+/* Departure bookkeeping (no counterpart in the reference, where nnel and jlev are locals of
+ * solve_nonlinear_advection.f90:92-202): points within 1e-12 of an edge of their cell,
+ * counted per ELM ([0] momentum, [1] tracer), and the momentum ELM's departure cell
+ * (negated for such a point) and level per face-layer, [k-1][j-1], when recording. */
+static long long g_near[2];
+static int32_t *g_dep;
+
+void oracle_elm_record(int32_t *dep) { g_dep = dep; }
+long long oracle_elm_near_edge(int which) { return which == 0 || which == 1 ? g_near[which] : -1; }
+void oracle_elm_near_reset(void) { g_near[0] = g_near[1] = 0; }
+
+/* Whether (px, py) is within 1e-12 of an edge of cell nel relative to the edge's length:
+ * |(B - A) x (P - A)| <= 1e-12 |B - A|^2 for an edge A -> B, in node-slot order. */
+static int near_edge(const gomgpu_mesh *m, int nel, double px, double py) {
+  const int N = R1(m->tri_or_quad, nel);
+  for (int l = 1; l <= N; l++) {
+    int na = I4(m->nodenum_at_cell, l, nel), nb = I4(m->nodenum_at_cell, l < N ? l + 1 : 1, nel);
+    double ax = R1(m->x_node, na), ay = R1(m->y_node, na);
+    double ex = R1(m->x_node, nb) - ax, ey = R1(m->y_node, nb) - ay;
+    double cr = ex * (py - ay) - ey * (px - ax);
+    if (fabs(cr) <= 1.0e-12 * (ex * ex + ey * ey)) return 1;
+  }
+  return 0;
+}
+
 /* ELM_backtrace_v1.f90:15-200 (i_which_backtrack = 1) */
 /* ELM_backtrace_v1.f90:15-214. With foot non-NULL, the foot point and its interpolation
- * weights are left there: xt, yt, zt, zup, staint[4]. */
+ * weights are left there: xt, yt, zt, zup, staint[4]. *near is set when the end point of a
+ * sub-step is within 1e-12 of an edge of its cell (near_edge). */
 static int backtrace(const elmctx *c, int jface, int bt_step, double bt_dt, double *uu,
                      double *vv, double *ww, double x0, double y0, double z0, int *nnel,
-                     int *jlev, double *foot) {
+                     int *jlev, double *foot, int *near) {
   const gomgpu_mesh *m = c->m;
   const gomgpu_params *p = c->p;
   gomgpu_state *s = c->s;
@@ -377,6 +403,7 @@ static int backtrace(const elmctx *c, int jface, int bt_step, double bt_dt, doub
     int iflqs1 = 0;
     int err = search_line(c, nnel, jlev, bt_dt, x0, y0, z0, &xt, &yt, &zt, &iflqs1, jface);
     if (err) return err;
+    if (near_edge(m, *nnel, xt, yt)) *near = 1;
     int nel = *nnel, N = R1(m->tri_or_quad, nel), jl = *jlev;
     for (int l = 1; l <= N; l++) {
       int nd = I4(m->nodenum_at_cell, l, nel);
@@ -464,6 +491,7 @@ int oracle_elm(const gomgpu_mesh *m, const gomgpu_params *p, gomgpu_state *s,
     s->un_ELM[i] = 0.0;
     s->vn_ELM[i] = 0.0;
   }
+  if (g_dep) memset(g_dep, 0, 2 * (size_t)L * J * sizeof(int32_t));
   const double dt = p->dt;
   if (p->ELM_backtrace_flag == 1) {
     for (int i = 1; i <= P; i++) {
@@ -549,8 +577,14 @@ int oracle_elm(const gomgpu_mesh *m, const gomgpu_params *p, gomgpu_state *s,
       } else {
         int bt_step = (int)((RL(s->num_sub_elm_iteration, k, n1) + RL(s->num_sub_elm_iteration, k, n2)) * 0.5);
         double bt_dt = dt / bt_step;
-        int err = backtrace(&c, j, bt_step, bt_dt, &uu, &vv, &ww, x0, y0, z0, &nnel, &jlev, NULL);
+        int near = 0;
+        int err = backtrace(&c, j, bt_step, bt_dt, &uu, &vv, &ww, x0, y0, z0, &nnel, &jlev, NULL, &near);
         if (err) return err;
+        g_near[0] += near;
+        if (g_dep) {
+          g_dep[(size_t)(k - 1) * J + (j - 1)] = near ? -nnel : nnel;
+          g_dep[(size_t)L * J + (size_t)(k - 1) * J + (j - 1)] = jlev;
+        }
         RL(s->un_ELM, k, j) = uu * ct + vv * st;
         RL(s->vn_ELM, k, j) = -uu * st + vv * ct;
       }
@@ -1925,7 +1959,9 @@ static int oracle_transport_elm(const gomgpu_mesh *m, const gomgpu_params *p, go
         s_ = RL(s->salt_cell, k, i);
       } else {
         double bt_dt = dt / bt_step, foot[8];
-        int err = backtrace(&c, i, bt_step, bt_dt, &uu, &vv, &ww, x0, y0, z0, &nnel, &jlev, foot);
+        int near = 0;
+        int err = backtrace(&c, i, bt_step, bt_dt, &uu, &vv, &ww, x0, y0, z0, &nnel, &jlev, foot, &near);
+        g_near[1] += near;
         if (!err) err = elm_tracer_foot(&c, f, nnel, jlev, foot, &t_, &s_);
         if (err) {
           ret = err;

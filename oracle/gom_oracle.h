@@ -75,6 +75,13 @@ int oracle_bilinear(double x1, double x2, double x3, double x4, double y1, doubl
 double oracle_rho_unesco(double t, double s);
 /* UNESCO-1980 secant-bulk-modulus correction, pres in bar (calculate_density_full.f90:75-101). */
 double oracle_rho_pressure(double rho_surface, double t, double s, double pres);
+/* Departures of the ELM: with dep non-NULL (2*L*J ints), oracle_elm writes per face-layer
+ * [k-1][j-1] the departure cell nnel (1-based, negated for a point within 1e-12 of an edge
+ * of its cell, 0 where nothing is backtraced), then the levels jlev; NULL stops it. The
+ * points within 1e-12 of an edge are counted: which 0 momentum ELM, 1 tracer ELM. */
+void oracle_elm_record(int32_t *dep);
+long long oracle_elm_near_edge(int which);
+void oracle_elm_near_reset(void);
 /* Last PCG iteration count and the first failing entity of the last error. */
 int oracle_last_pcg_iterations(void);
 int oracle_last_error_entity(void);

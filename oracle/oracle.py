@@ -9,6 +9,8 @@ import ctypes as C
 import os
 import subprocess
 
+import numpy as np
+
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB = os.path.join(HERE, "liboracle.so")
 _lib = None
@@ -41,6 +43,11 @@ def lib():
                                     C.c_void_p, C.c_int, C.c_double]
         _lib.oracle_check_intersection.argtypes = [C.c_double] * 8 + [C.c_void_p, C.c_void_p]
         _lib.oracle_bilinear.argtypes = [C.c_double] * 10 + [C.c_void_p] * 3
+        _lib.oracle_elm_record.argtypes = [C.c_void_p]
+        _lib.oracle_elm_record.restype = None
+        _lib.oracle_elm_near_edge.argtypes = [C.c_int]
+        _lib.oracle_elm_near_edge.restype = C.c_longlong
+        _lib.oracle_elm_near_reset.restype = None
     return _lib
 
 
@@ -60,6 +67,31 @@ class Oracle:
 
     def step(self) -> int:
         return self.run("step")
+
+    def elm_departures(self):
+        """Run the ELM stage recording its departures: returns (rc, nnel, jlev), each array
+        [maxlayer, maxface], nnel negated for a point within 1e-12 of a cell edge."""
+        L, J = self.params.maxlayer, self.params.maxface
+        dep = np.zeros((2, L, J), dtype=np.int32)
+        lib().oracle_elm_record(dep.ctypes.data)
+        try:
+            rc = self.run("elm")
+        finally:
+            lib().oracle_elm_record(None)
+        return rc, dep[0], dep[1]
+
+    @staticmethod
+    def elm_near_edge(reset: bool = False):
+        """Points within 1e-12 of a cell edge: (momentum ELM, tracer ELM)."""
+        out = (int(lib().oracle_elm_near_edge(0)), int(lib().oracle_elm_near_edge(1)))
+        if reset:
+            lib().oracle_elm_near_reset()
+        return out
+
+    @staticmethod
+    def last_error_entity() -> int:
+        """The first failing entity of the last stage that stopped."""
+        return lib().oracle_last_error_entity()
 
     @staticmethod
     def pcg_iterations() -> int:

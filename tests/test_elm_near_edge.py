@@ -1,0 +1,92 @@
+"""The oracle's accounting of ELM departure points within 1e-12 of a cell edge.
+
+A departure point within 1e-12 of an edge of its cell (relative to the edge's length) can
+be placed in either cell sharing the edge on the last bit of its coordinates: the
+containment test (search_straight_line.f90:60-100) and the crossing test
+(check_intersection.f90:34-48, inclusive bounds) decide it. Such points are counted, and
+the recorded departure cell is negated for them.
+
+Known answer: a uniform flow U along x in a channel of square quads. The faces on the
+horizontal edges start on their edge and are carried along it, so every one of their
+departures is on an edge; the faces on the vertical edges are carried 0.15 of a cell into
+the interior (U dt = 30 m, dx = 200 m) and none is near an edge. A cross-flow delta moves
+the first group off their edge by delta dt: at 1e-13 of the edge length they are still
+counted, at 1e-11 they are not, which fixes the threshold and its normalisation (a distance
+held to 1e-12 m, or to 1e-12 of the length rather than of its square, fails one case).
+"""
+from __future__ import annotations
+
+import numpy as np
+import pytest
+
+from gom_release_amd import driver, meshgen, prep
+from golden_io import FIXTURES, Golden, apply
+from oracle import Oracle
+
+U = 0.5
+DX = 200.0
+DT = 60.0
+
+
+def uniform_channel(cross: float):
+    """(oracle, params, mesh, state) of the channel with node velocity (U, 0) and face
+    velocities of (U, cross)."""
+    case = meshgen.channel_case(nx=8, ny=4, nlayer=3, depth=5.0, dx=DX, dt=DT, ndt=2)
+    p, m, s = prep.build(case)
+    hf = driver.HostForcing(case, p, m, (0.0, 0.0))
+    o = Oracle(m, p, s, hf.forcing)
+    assert o.run("layers") == 0
+    L = p.maxlayer
+    ct, st = np.asarray(m["cos_theta"]), np.asarray(m["sin_theta"])
+    s["u_node"] = np.full_like(s["u_node"], U)
+    s["v_node"] = np.zeros_like(s["v_node"])
+    s["w_node"] = np.zeros_like(s["w_node"])
+    # (uu, vv) = R(theta) (un, vn): un = U ct + d st, vn = -U st + d ct
+    un = U * ct + cross * st
+    vn = -U * st + cross * ct
+    s["un_face"] = np.repeat(un[:, None], L, axis=1).ravel()
+    s["vn_face"] = np.repeat(vn[:, None], L, axis=1).ravel()
+    return o, p, m, s
+
+
+@pytest.mark.parametrize("rel,flagged", [(0.0, True), (1e-13, True), (1e-11, False)])
+def test_uniform_flow_known_answer(rel, flagged):
+    o, p, m, s = uniform_channel(rel * DX / DT)
+    Oracle.elm_near_edge(reset=True)
+    rc, nnel, jlev = o.elm_departures()
+    assert rc == 0
+    horiz = np.abs(np.asarray(m["cos_theta"])) < 0.5       # edges along x
+    assert np.all(nnel != 0)                               # every face-layer is backtraced
+    assert np.all(jlev >= 1) and np.all(jlev <= p.maxlayer)
+    near = nnel < 0
+    if flagged:
+        assert np.array_equal(near, np.broadcast_to(horiz, near.shape))
+        assert Oracle.elm_near_edge()[0] == int(horiz.sum()) * p.maxlayer
+    else:
+        assert not near.any()
+        assert Oracle.elm_near_edge()[0] == 0
+    # the departure cell is a neighbour of the face (a step of 0.15 cell crosses at most one edge)
+    adj = np.asarray(m["adj_cellnum_at_face"]).reshape(-1, 2)
+    for k in range(p.maxlayer):
+        for j in range(p.maxface):
+            assert abs(nnel[k, j]) in adj[j], (k, j, nnel[k, j])
+
+
+@pytest.mark.parametrize("name", FIXTURES)
+def test_recording_leaves_the_stage_unchanged(name):
+    """With its departures recorded, the ELM stage still reproduces the reference."""
+    g = Golden(name)
+    case = g.case()
+    it = g.steps[-1]
+    state = g.state_in(it)
+    apply(state, g.expected(it, "pre"))
+    o = Oracle(g.mesh, g.params, state, g.forcing(it, case))
+    rc, nnel, jlev = o.elm_departures()
+    assert rc == 0
+    for field, ref in g.expected(it, "elm").items():
+        if field in state.arr:
+            assert np.array_equal(state[field], ref), field
+    back = nnel != 0
+    assert back.any()
+    assert np.array_equal(back, jlev != 0)
+    assert np.all(np.abs(nnel[back]) <= g.params.maxele)
