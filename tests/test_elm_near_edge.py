@@ -90,3 +90,68 @@ def test_recording_leaves_the_stage_unchanged(name):
     assert back.any()
     assert np.array_equal(back, jlev != 0)
     assert np.all(np.abs(nnel[back]) <= g.params.maxele)
+
+
+def _cells_containing(px, py, cx, cy, nn, tol=1e-9):
+    """Brute-force point location: for each point, the 1-based cells (counter-clockwise
+    polygons, vertices cx/cy [E, 4], nn of them used) that contain it, and whether it lies
+    within tol (relative to the edge length) of some cell's edge."""
+    E = cx.shape[0]
+    inside = np.ones((px.size, E), dtype=bool)
+    near = np.zeros(px.size, dtype=bool)
+    for l in range(4):
+        use = l < nn
+        l2 = np.where(l + 1 < nn, l + 1, 0)
+        ax, ay = cx[:, l], cy[:, l]
+        bx, by = cx[np.arange(E), l2], cy[np.arange(E), l2]
+        ex, ey = bx - ax, by - ay
+        cr = ex[None, :] * (py[:, None] - ay[None, :]) - ey[None, :] * (px[:, None] - ax[None, :])
+        inside &= ~use[None, :] | (cr >= 0.0)
+        near |= (use[None, :] & (np.abs(cr) <= tol * (ex * ex + ey * ey)[None, :])).any(axis=1)
+    return inside, near
+
+
+def test_uniform_flow_departure_cells_by_brute_force():
+    """Uniform flow (0.25, 0.15) m/s over the mixed triangle/quad estuary mesh for one
+    step: every departure point is x_face - u dt, and its cell is the one brute-force
+    point location finds (points that leave the domain or lie within 1e-9 of an edge are
+    left out); w being 0, the level is the start level clamped to the departure cell's
+    layers (search_straight_line.f90:370-399)."""
+    uu, vv, dt = 0.25, 0.15, 120.0
+    case = meshgen.estuary_case(ncol=24, nrep=2, nlayer=3, river_q=0.0, noise=0.0, dt=dt, ndt=2)
+    p, m, s = prep.build(case)
+    hf = driver.HostForcing(case, p, m, (0.0, 0.0))
+    o = Oracle(m, p, s, hf.forcing)
+    assert o.run("layers") == 0
+    L, J, E = p.maxlayer, p.maxface, p.maxele
+    ct, st = np.asarray(m["cos_theta"]), np.asarray(m["sin_theta"])
+    s["u_node"] = np.full_like(s["u_node"], uu)
+    s["v_node"] = np.full_like(s["v_node"], vv)
+    s["w_node"] = np.zeros_like(s["w_node"])
+    s["un_face"] = np.repeat((uu * ct + vv * st)[:, None], L, axis=1).ravel()
+    s["vn_face"] = np.repeat((-uu * st + vv * ct)[:, None], L, axis=1).ravel()
+    rc, nnel, jlev = o.elm_departures()
+    assert rc == 0
+    xn, yn = np.asarray(m["x_node"]), np.asarray(m["y_node"])
+    nodes = np.asarray(m["nodenum_at_cell"]).reshape(E, 4)
+    nn = np.asarray(m["tri_or_quad"])
+    cx = xn[np.maximum(nodes, 1) - 1]
+    cy = yn[np.maximum(nodes, 1) - 1]
+    px = np.asarray(m["x_face"]) - uu * dt
+    py = np.asarray(m["y_face"]) - vv * dt
+    inside, near = _cells_containing(px, py, cx, cy, nn)
+    found = inside.sum(axis=1)
+    checked = 0
+    for j in range(J):
+        for k in range(L):
+            if nnel[k, j] == 0:
+                continue
+            if found[j] == 0 or near[j]:
+                continue                                 # outside the domain, or on an edge
+            assert found[j] == 1
+            cell = int(np.flatnonzero(inside[j])[0]) + 1
+            assert abs(nnel[k, j]) == cell, (k, j)
+            lo, hi = int(m["bottom_layer_at_element"][cell - 1]), int(s["top_layer_at_element"][cell - 1])
+            assert jlev[k, j] == min(max(k + 1, lo), hi), (k, j)
+            checked += 1
+    assert checked > 0.8 * np.count_nonzero(nnel)
