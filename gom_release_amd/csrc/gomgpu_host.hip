@@ -1417,6 +1417,13 @@ extern "C" int gomgpu_batch_remove(gomgpu_batch *b, gomgpu_ctx *c) {
 
 extern "C" int gomgpu_batch_size(gomgpu_batch *b) { return b ? (int)b->members.size() : -1; }
 
+extern "C" int gomgpu_batch_layout(gomgpu_batch *b, int *groups, int *solo) {
+  if (!b) return GOMGPU_E_ARG;
+  if (groups) *groups = (int)b->groups.size();
+  if (solo) *solo = (int)b->solo.size();
+  return 0;
+}
+
 extern "C" int gomgpu_batch_step(gomgpu_batch *b) {
   if (!b) return GOMGPU_E_ARG;
   if (b->members.empty()) return 0;

@@ -71,6 +71,7 @@ def lib():
         L.gomgpu_batch_create.argtypes = [C.POINTER(vp)]
         for fn in ("gomgpu_batch_add", "gomgpu_batch_remove"):
             getattr(L, fn).argtypes = [vp, vp]
+        L.gomgpu_batch_layout.argtypes = [vp, C.POINTER(C.c_int), C.POINTER(C.c_int)]
         for fn in ("gomgpu_batch_size", "gomgpu_batch_step", "gomgpu_batch_sync"):
             getattr(L, fn).argtypes = [vp]
         L.gomgpu_batch_last_error.argtypes = [vp]
@@ -100,7 +101,8 @@ EXPORTED = ["gomgpu_create", "gomgpu_set_state", "gomgpu_set_forcing", "gomgpu_s
             "gomgpu_destroy", "gomgpu_set_launch", "gomgpu_get_launch", "gomgpu_set_pcg_rows",
             "gomgpu_footprint",
             "gomgpu_device_bytes", "gomgpu_batch_create", "gomgpu_batch_add", "gomgpu_batch_remove",
-            "gomgpu_batch_size", "gomgpu_batch_step", "gomgpu_batch_sync", "gomgpu_batch_last_error",
+            "gomgpu_batch_size", "gomgpu_batch_layout", "gomgpu_batch_step", "gomgpu_batch_sync",
+            "gomgpu_batch_last_error",
             "gomgpu_batch_destroy", "gomgpu_pcg_solve", "gomgpu_pcg_workspace_bytes",
             "gomgpu_pcg_last_error", "gomgpu_pcg_solve_noalloc", "gomgpu_elm_record",
             "gomgpu_elm_departures", "gomgpu_elm_near_edge"]
@@ -327,6 +329,12 @@ class GpuBatch:
 
     def step(self) -> None:
         self._check(self._L.gomgpu_batch_step(self._b))
+
+    def layout(self):
+        """(fused PCG launches, members solved alone) of the last step."""
+        g, s = C.c_int(), C.c_int()
+        self._check(self._L.gomgpu_batch_layout(self._b, C.byref(g), C.byref(s)))
+        return g.value, s.value
 
     def sync(self) -> None:
         self._check(self._L.gomgpu_batch_sync(self._b))

@@ -97,6 +97,7 @@ class ServingLoop:
         futures = {}
         used = 0
         nxt = 0
+        most_groups = most_solo = 0   # fused PCG launches and lone solves in one step
         # fresh interpreters (never a fork of this process, which holds the GPU); the
         # workers do host work only
         ctx = multiprocessing.get_context("spawn")
@@ -138,6 +139,8 @@ class ServingLoop:
                     a.sc.stage_forcing()
                 batch.step()
                 batch.sync()
+                g, so = batch.layout()
+                most_groups, most_solo = max(most_groups, g), max(most_solo, so)
                 for a in active:
                     a.steps += 1
                 done = [a for a in active if a.steps >= a.req.nsteps]
@@ -164,7 +167,8 @@ class ServingLoop:
         work = sum(r.active_cell_layer_steps for r in results)
         summary = {"requests": len(results), "p50_latency_s": float(np.percentile(lat, 50)),
                    "p99_latency_s": float(np.percentile(lat, 99)), "makespan_s": makespan,
-                   "cell_layer_steps_per_s": work / makespan if makespan > 0 else 0.0}
+                   "cell_layer_steps_per_s": work / makespan if makespan > 0 else 0.0,
+                   "most_fused_pcg_launches": most_groups, "most_lone_pcg_solves": most_solo}
         return sorted(results, key=lambda r: r.rid), summary
 
 

@@ -389,6 +389,11 @@ int gomgpu_batch_create(gomgpu_batch **out);
 int gomgpu_batch_add(gomgpu_batch *batch, gomgpu_ctx *member);
 int gomgpu_batch_remove(gomgpu_batch *batch, gomgpu_ctx *member);
 int gomgpu_batch_size(gomgpu_batch *batch);
+/* How the last step solved the members' free surfaces: *groups fused launches of
+ * register-resident solves (members packed while their blocks fit the co-resident grid) and
+ * *solo members solved alone (more blocks than fit, or GOMGPU_MIXED). 0 and 0 before the
+ * first step. */
+int gomgpu_batch_layout(gomgpu_batch *batch, int *groups, int *solo);
 /* One step of every member (asynchronous). */
 int gomgpu_batch_step(gomgpu_batch *batch);
 /* Wait for the batch; returns the first member error (gomgpu_sync semantics). */

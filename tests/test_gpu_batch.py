@@ -120,3 +120,36 @@ def test_continuous_batching_members_join_and_leave(gpu_available):
     b.close()
     for sc in scs.values():
         sc.model.close()
+
+
+def test_fused_groups_and_a_lone_solve(gpu_available):
+    """Members of uneven size: two of about 300k cells (each half of the co-resident PCG
+    grid, so they land in different fused launches), a small one packed beside the second,
+    and one of 600k cells (more blocks than the grid holds: solved by its own
+    memory-resident launch in the batch step). Each equals its run alone bit for bit."""
+    from gom_release_amd import host, meshgen, serving
+    from gom_release_amd.driver import Scenario
+    cases = [meshgen.channel_case(nx=600, ny=500, nlayer=4, ndt=4),
+             serving.estuary_of_size(300000, 4, seed=5, river_q=80.0, ndt=4),
+             meshgen.channel_case(nx=16, ny=8, nlayer=5, ndt=4),
+             serving.estuary_of_size(600000, 4, seed=6, river_q=80.0, ndt=4)]
+    n = 3
+    refs = [_alone(c, n) for c in cases]
+    print("[gpu-batch] uneven members alone done", flush=True)
+    b = host.GpuBatch()
+    scs = [Scenario(c, wind=WIND) for c in cases]
+    for sc in scs:
+        b.add(sc.model)
+    for _ in range(n):
+        for sc in scs:
+            sc.stage_forcing()
+        b.step()
+    b.sync()
+    groups, solo = b.layout()
+    print(f"[gpu-batch] {groups} fused PCG launches, {solo} lone", flush=True)
+    assert (groups, solo) == (2, 1)
+    for i, sc in enumerate(scs):
+        _assert_same(_state(sc), refs[i], f"member {i} ({sc.params.maxele} cells)")
+    b.close()
+    for sc in scs:
+        sc.model.close()

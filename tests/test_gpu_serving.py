@@ -71,3 +71,25 @@ def test_served_results_equal_alone(gpu_available, budget_mib):
     for r in results:  # footprint of everything active when r was admitted
         live = [x.footprint for x in results if x.admitted <= r.admitted < x.completed]
         assert sum(live) <= budget
+
+
+@pytest.mark.parametrize("rate", [20.0, 1.5])
+def test_large_request_served_alone_in_the_batch(gpu_available, rate):
+    """Requests of 8k-40k cells and one of 600k cells (more PCG rows than one fused launch
+    holds: it is solved by its own memory-resident launch inside the batch step), arriving
+    as Poisson processes of 20/s and 1.5/s. Every result equals the scenario alone."""
+    from gom_release_amd import serving
+    sizes = [8000, 600000, 40000, 15000]
+    arr = serving.poisson_trace(len(sizes), rate, seed=7)
+    reqs = [serving.Request(i, serving.estuary_of_size(sz, 4, seed=20 + i, river_q=50.0), 3, arr[i],
+                            wind=(5.0, 1.0), keep_state=KEEP) for i, sz in enumerate(sizes)]
+    print(f"[gpu-serve] rate {rate}/s, arrivals {[round(a, 3) for a in arr]}", flush=True)
+    results, summary = serving.ServingLoop(1 << 36, workers=2).run(reqs, log=lambda m: print(m, flush=True))
+    print(f"[gpu-serve] {summary}", flush=True)
+    assert summary["requests"] == len(reqs)
+    assert summary["most_lone_pcg_solves"] >= 1
+    assert max(r.cells for r in results) > 524288
+    for r, req in zip(results, reqs):
+        ref = _alone(req)
+        for f in KEEP:
+            assert np.array_equal(r.state[f], ref[f]), (req.rid, f)
