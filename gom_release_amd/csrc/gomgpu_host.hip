@@ -285,21 +285,35 @@ static int validate(const gomgpu_params *p, std::string &why) {
   return 0;
 }
 
+// Why the last gomgpu_create on this thread failed (gomgpu_last_error(NULL)).
+static thread_local std::string g_create_err;
+
 extern "C" int gomgpu_create(const gomgpu_mesh *mesh, const gomgpu_params *params, gomgpu_ctx **out) {
-  if (!mesh || !params || !out) return GOMGPU_E_ARG;
+  if (!mesh || !params || !out) {
+    g_create_err = "gomgpu_create: NULL mesh, params or out";
+    return GOMGPU_E_ARG;
+  }
   *out = nullptr;
+  g_create_err.clear();
   std::string why;
   if (validate(params, why)) {
-    fprintf(stderr, "gomgpu_create: %s\n", why.c_str());
+    g_create_err = "gomgpu_create: " + why;
+    fprintf(stderr, "%s\n", g_create_err.c_str());
     return GOMGPU_E_ARG;
   }
   gomgpu_ctx *c = new gomgpu_ctx();
   c->p = *params;
   const gomgpu_params &p = c->p;
   int rc;
-  if (hipGetDevice(&c->device) != hipSuccess || hipStreamCreateWithFlags(&c->st, hipStreamNonBlocking) != hipSuccess) {
-    delete c;
-    return GOMGPU_E_HIP;
+  {
+    hipError_t e = hipGetDevice(&c->device);
+    if (e == hipSuccess) e = hipStreamCreateWithFlags(&c->st, hipStreamNonBlocking);
+    if (e != hipSuccess) {
+      g_create_err = std::string("gomgpu_create: no usable HIP device: ") + hipGetErrorString(e);
+      fprintf(stderr, "%s\n", g_create_err.c_str());
+      delete c;
+      return GOMGPU_E_HIP;
+    }
   }
   c->d.P = p.maxnod;
   c->d.E = p.maxele;
@@ -544,7 +558,8 @@ extern "C" int gomgpu_create(const gomgpu_mesh *mesh, const gomgpu_params *param
   *out = c;
   return 0;
 fail:
-  fprintf(stderr, "gomgpu_create: %s\n", c->err.c_str());
+  g_create_err = "gomgpu_create: " + c->err;
+  fprintf(stderr, "%s\n", g_create_err.c_str());
   gomgpu_destroy(c);
   return rc ? rc : GOMGPU_E_HIP;
 }
@@ -1166,7 +1181,10 @@ extern "C" int gomgpu_budgets(gomgpu_ctx *c, double *volume, double *salt) {
   return 0;
 }
 
-extern "C" const char *gomgpu_last_error(gomgpu_ctx *c) { return c ? c->err.c_str() : "null context"; }
+extern "C" const char *gomgpu_last_error(gomgpu_ctx *c) {
+  if (c) return c->err.c_str();
+  return g_create_err.empty() ? "null context" : g_create_err.c_str();
+}
 
 extern "C" int gomgpu_timings(gomgpu_ctx *c, int enable, const char **names, double *ms, int n) {
   if (!c) return GOMGPU_E_ARG;

@@ -296,7 +296,8 @@ int gomgpu_elm_departures(gomgpu_ctx *ctx, int32_t *nnel, int32_t *jlev);
 int gomgpu_elm_near_edge(gomgpu_ctx *ctx, long long *counts, int reset);
 /* Wait for all queued work; returns the latched model error as GOMGPU_E_MODEL. */
 int gomgpu_sync(gomgpu_ctx *ctx);
-/* Text of the last error (static storage of the context). */
+/* Text of the last error (static storage of the context). With ctx NULL: why the last
+ * gomgpu_create on the calling thread failed. */
 const char *gomgpu_last_error(gomgpu_ctx *ctx);
 /* Event timing. enable != 0 turns it on (turning it on resets the kernel averages);
  * fills up to n (name, milliseconds) pairs: the GOMGPU_NUM_STAGES stage times of the last

@@ -59,3 +59,48 @@ def test_module_compiles_links_and_matches_c_sizes():
     for st in ("gomgpu_mesh", "gomgpu_state", "gomgpu_params", "gomgpu_forcing"):
         assert int(got[st]) == lay[(st, "size")], st
     assert int(got["create_rc"]) == 1  # GOMGPU_E_ARG
+
+
+BRIDGE = os.path.join(ROOT, "gom_release_amd", "fortran", "gomgpu_bridge.f90")
+REFSRC = "/root/reference/GOM_Rv1.1.0/source"
+SHIMMED = ["solve_nonlinear_advection", "solve_momentum_equation", "solve_free_surface_equation",
+           "solve_velocities", "update_vertical_layers_1", "calculate_velocity_at_node",
+           "solve_transport_equation_v19", "solve_transport_equation_ELM_v8", "calculate_density_full",
+           "calculate_density_linear", "update_variables"]
+
+
+def test_committed_bridge_is_generated():
+    assert open(BRIDGE).read() == fortran_binding.bridge_text()
+
+
+@pytest.mark.skipif(FC is None or not os.path.isdir(REFSRC), reason="needs amdflang and the reference sources")
+def test_reference_main_links_against_the_shims():
+    """GOM's own main.f90, unchanged, linked with every reference object but the eleven
+    hot-path ones, the bridge, the shims and libgomgpu.so (`make -C oracle gpu-main`).
+    The shims are the only definitions of the reference's hot-path names, and the run goes
+    through the reference's start-up to its first hot-path call (prepare_gom's node
+    velocities of a cold start), where the bridge creates the GPU context: without a GPU it
+    stops there with the library's reason, written to the run log."""
+    from gom_release_amd import gomio, meshgen
+    if not os.path.exists(host.LIB_PATH):
+        pytest.skip("libgomgpu.so not built")
+    subprocess.run(["make", "-s", "-C", os.path.join(ROOT, "oracle"), "gpu-main"], check=True,
+                   capture_output=True, timeout=1800)
+    exe = os.path.join(ROOT, "oracle", "_ref", "gom_gpu_main")
+    syms = subprocess.run(["nm", exe], check=True, capture_output=True, text=True).stdout.split("\n")
+    for name in SHIMMED:
+        defs = [ln for ln in syms if ln.endswith(" T " + name.lower() + "_")]
+        assert len(defs) == 1, (name, defs)
+    undef = {ln.split()[-1] for ln in syms if " U " in ln}
+    for fn in ("gomgpu_create", "gomgpu_set_state", "gomgpu_set_forcing", "gomgpu_stage", "gomgpu_get_state",
+               "gomgpu_sync", "gomgpu_last_error"):
+        assert fn in undef, fn
+    case = meshgen.estuary_case(ncol=20, nrep=3, nlayer=4, ndt=3, river_q=50.0)
+    with tempfile.TemporaryDirectory() as d:
+        proj = gomio.write_project(case, d)
+        env = dict(os.environ, OMP_NUM_THREADS="1", HIP_VISIBLE_DEVICES="-1")
+        p = subprocess.run([exe], cwd=proj, capture_output=True, text=True, timeout=300, env=env)
+        log = open(os.path.join(proj, "output", "run.log")).read()
+    assert "Time stepping begins" in p.stdout            # the reference's start-up ran
+    assert "gomgpu error" in log and "(gomgpu_create)" in log, log[-600:]
+    assert "gomgpu_create: no usable HIP device" in p.stderr
