@@ -115,13 +115,18 @@ def step_bytes(F: int, C: int, N: int, J: int, E: int, zbar: float, nt: int, ntr
 
 
 def cpu_baseline(case, wind, budget_s: float = 15.0, max_steps: int = 12, label: str = ""):
-    """Time the CPU restatement (oracle, one thread) on the same workload."""
+    """Time the CPU restatement on the same workload: the oracle's OpenMP build
+    (liboracle_omp.so, the same bits as the serial build), with OMP_NUM_THREADS threads or,
+    unset, one per CPU the process may use."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    nproc = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else os.cpu_count()
+    threads = int(os.environ.get("OMP_NUM_THREADS") or nproc)
+    os.environ["OMP_NUM_THREADS"] = str(threads)     # read when the OpenMP runtime loads
     from oracle import Oracle
     from gom_release_amd import driver, prep
     p, m, s = prep.build(case)
     hf = driver.HostForcing(case, p, m, wind)
-    o = Oracle(m, p, s, hf.forcing)
+    o = Oracle(m, p, s, hf.forcing, threaded=True)
     if o.run("eos") != 0:
         raise RuntimeError("oracle eos failed")
     t_total = 0.0
@@ -139,9 +144,10 @@ def cpu_baseline(case, wind, budget_s: float = 15.0, max_steps: int = 12, label:
             raise RuntimeError(f"oracle step failed: {rc}")
         work += int(np.sum(np.where(top > 0, top - bot + 1, 0)))
         n += 1
-    return {"value": work / t_total, "unit": "cell-layer-steps/s", "cores": 1, "kind": "port",
-            "sample": f"first {n} steps of the same workload ({label}), FP64 C restatement, "
-                      f"single thread, {t_total:.1f} s"}
+    return {"value": work / t_total, "unit": "cell-layer-steps/s", "cores": threads, "nproc": nproc,
+            "kind": "port",
+            "sample": f"first {n} steps of the same workload ({label}), FP64 C restatement built with "
+                      f"OpenMP, {threads} threads, {t_total:.1f} s"}
 
 
 def copy_calibration(nbytes: int) -> None:

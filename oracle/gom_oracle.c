@@ -18,6 +18,12 @@
  * Not pinned by any reference output: the pressure correction of the full EOS (only below
  * 100 m, absent from every fixture) -- "parity unpinned", it is checked against the
  * published UNESCO-1980 high-pressure check value only.
+ *
+ * Threads: liboracle_omp.so is this file built with -fopenmp, liboracle.so without. The
+ * parallel loops run over independent entities (faces, cells, nodes) with private scratch,
+ * and every sum keeps its serial order, so both builds give the same bits
+ * (tests/test_oracle_threads.py). A failing entity in a parallel loop is recorded and the
+ * lowest one reported, which is the one the serial loop stops at.
  */
 #include "gom_oracle.h"
 
@@ -36,7 +42,7 @@
 #define ZL(k) (m->z_level[(k)])
 
 static int g_last_iters = 0;
-static int g_err_entity = 0;
+static _Thread_local int g_err_entity = 0; /* set by the calling thread */
 
 int oracle_last_pcg_iterations(void) { return g_last_iters; }
 int oracle_last_error_entity(void) { return g_err_entity; }
@@ -44,6 +50,21 @@ int oracle_last_error_entity(void) { return g_err_entity; }
 static int fail(int code, int entity) {
   g_err_entity = entity;
   return code;
+}
+
+/* The first failure of a parallel loop over entities: the lowest entity and its code. */
+typedef struct {
+  int code, entity;
+} loop_err;
+
+static void loop_fail(loop_err *e, int code, int entity) {
+#pragma omp critical(oracle_loop_err)
+  {
+    if (e->code == 0 || entity < e->entity) {
+      e->code = code;
+      e->entity = entity;
+    }
+  }
 }
 
 /* start_end_node(N, l, w): set_geometry_1.f90:78-97 */
@@ -494,6 +515,7 @@ int oracle_elm(const gomgpu_mesh *m, const gomgpu_params *p, gomgpu_state *s,
   if (g_dep) memset(g_dep, 0, 2 * (size_t)L * J * sizeof(int32_t));
   const double dt = p->dt;
   if (p->ELM_backtrace_flag == 1) {
+#pragma omp parallel for schedule(static)
     for (int i = 1; i <= P; i++) {
       int nadj = R1(m->adj_cells_at_node, i);
       for (int k = 1; k <= L; k++) {
@@ -532,6 +554,8 @@ int oracle_elm(const gomgpu_mesh *m, const gomgpu_params *p, gomgpu_state *s,
   }
   elmctx c = {m, p, s, L};
   const double vmin = (double)1.e-4f;
+  loop_err e = {0, 0};
+#pragma omp parallel for schedule(dynamic, 64)
   for (int j = 1; j <= J; j++) {
     int n1 = I2(m->nodenum_at_face, 1, j), n2 = I2(m->nodenum_at_face, 2, j);
     int kb = R1(m->bottom_layer_at_face, j), kt = R1(s->top_layer_at_face, j);
@@ -579,8 +603,14 @@ int oracle_elm(const gomgpu_mesh *m, const gomgpu_params *p, gomgpu_state *s,
         double bt_dt = dt / bt_step;
         int near = 0;
         int err = backtrace(&c, j, bt_step, bt_dt, &uu, &vv, &ww, x0, y0, z0, &nnel, &jlev, NULL, &near);
-        if (err) return err;
-        g_near[0] += near;
+        if (err) {
+          loop_fail(&e, err, j);
+          break;
+        }
+        if (near) {
+#pragma omp atomic
+          g_near[0]++;
+        }
         if (g_dep) {
           g_dep[(size_t)(k - 1) * J + (j - 1)] = near ? -nnel : nnel;
           g_dep[(size_t)L * J + (size_t)(k - 1) * J + (j - 1)] = jlev;
@@ -590,6 +620,7 @@ int oracle_elm(const gomgpu_mesh *m, const gomgpu_params *p, gomgpu_state *s,
       }
     }
   }
+  if (e.code) return fail(e.code, e.entity);
   return 0;
 }
 
@@ -599,19 +630,18 @@ int oracle_momentum(const gomgpu_mesh *m, const gomgpu_params *p, gomgpu_state *
                     const gomgpu_forcing *f) {
   const int J = p->maxface, L = p->maxlayer;
   const double dt = p->dt, g = p->gravity, th = p->theta;
-  double a[64], b[64], cc[64], rhs[3 * 64], sol[3 * 64], G1[64], G2[64], temp[13];
   for (size_t i = 0; i < (size_t)L * J; i++) {
     s->AinvDeltaZ1[i] = 0.0;
     s->AinvDeltaZ2[i] = 0.0;
     s->AinvG1[i] = 0.0;
     s->AinvG2[i] = 0.0;
   }
-  memset(a, 0, sizeof a);
-  memset(b, 0, sizeof b);
-  memset(cc, 0, sizeof cc);
+  loop_err ferr = {0, 0};
+#pragma omp parallel for schedule(static)
   for (int j = 1; j <= J; j++) {
     int kt = R1(s->top_layer_at_face, j);
     if (kt == 0) continue;
+    double a[64] = {0}, b[64] = {0}, cc[64] = {0}, rhs[3 * 64], sol[3 * 64], G1[64], G2[64], temp[13];
     int kb = R1(m->bottom_layer_at_face, j);
     int node1 = I2(m->nodenum_at_face, 1, j), node2 = I2(m->nodenum_at_face, 2, j);
     int n = kt - kb + 1;
@@ -756,7 +786,10 @@ int oracle_momentum(const gomgpu_mesh *m, const gomgpu_params *p, gomgpu_state *
       rhs[(k - 1) + 2 * 64] = dz;
     }
     int err = oracle_tridiag(n, 3, 64, a + 1, b + 1, cc + 1, rhs, sol);
-    if (err) return fail(err, j);
+    if (err) {
+      loop_fail(&ferr, err, j);
+      continue;
+    }
     for (int k = 1; k <= n; k++) {
       RL(s->AinvG1, k, j) = sol[(k - 1)];
       RL(s->AinvG2, k, j) = sol[(k - 1) + 64];
@@ -770,6 +803,7 @@ int oracle_momentum(const gomgpu_mesh *m, const gomgpu_params *p, gomgpu_state *
     if (bt == -1)
       for (int k = 1; k <= n; k++) RL(s->AinvDeltaZ1, k, j) = 0.0;
   }
+  if (ferr.code) return fail(ferr.code, ferr.entity);
   return 0;
 }
 
@@ -794,6 +828,7 @@ int oracle_pcg(int n, const double *A, const int32_t *col, const int32_t *rowstr
   double delta_0 = delta_new;
   int cgit2 = 0;
   for (int cgit = 1; cgit <= max_iter; cgit++) {
+#pragma omp parallel for schedule(static)
     for (int j = 1; j <= n; j++) {
       double sum1 = 0.0;
       for (int k = rowstr[j - 1]; k <= rowstr[j] - 1; k++) sum1 = sum1 + A[k - 1] * d[col[k - 1] - 1];
@@ -802,15 +837,17 @@ int oracle_pcg(int n, const double *A, const int32_t *col, const int32_t *rowstr
     double dq = 0.0;
     for (int i = 0; i < n; i++) dq = dq + d[i] * q[i];
     double alpha = delta_new / dq;
+#pragma omp parallel for schedule(static)
     for (int i = 0; i < n; i++) {
       x[i] = x[i] + alpha * d[i];
       r[i] = r[i] - alpha * q[i];
+      sv[i] = 0.0 + Minv[i] * r[i];
     }
-    for (int i = 0; i < n; i++) sv[i] = 0.0 + Minv[i] * r[i];
     double delta_old = delta_new;
     delta_new = 0.0;
     for (int i = 0; i < n; i++) delta_new = delta_new + r[i] * sv[i];
     double beta = delta_new / delta_old;
+#pragma omp parallel for schedule(static)
     for (int i = 0; i < n; i++) d[i] = sv[i] + beta * d[i];
     if (sqrt(delta_new * delta_new / (delta_0 * delta_0)) < tol) {
       cgit2 = cgit;
@@ -839,7 +876,9 @@ int oracle_freesurf(const gomgpu_mesh *m, const gomgpu_params *p, gomgpu_state *
   double *x = calloc((size_t)E, sizeof(double));
   int ret = 0;
 #define SPM(l, i) sparsem[(l) + (size_t)((i) - 1) * 5]
-  for (int i = 1; i <= E && !ret; i++) {
+  loop_err aerr = {0, 0};
+#pragma omp parallel for schedule(static)
+  for (int i = 1; i <= E; i++) {
     int N = R1(m->tri_or_quad, i);
     SPM(0, i) = R1(m->area, i);
     for (int l = 1; l <= N; l++) {
@@ -853,7 +892,7 @@ int oracle_freesurf(const gomgpu_mesh *m, const gomgpu_params *p, gomgpu_state *
         c = g * (dt * dt) * (th * th) * R1(m->face_length, j) / R1(m->delta_j, j) * c;
         c = c * R1(m->cos_theta2, j);
         if (c < 0.0) {
-          ret = fail(GOMGPU_ERR_NOT_SPD, i);
+          loop_fail(&aerr, GOMGPU_ERR_NOT_SPD, i);
           break;
         }
         SPM(0, i) = SPM(0, i) + c;
@@ -891,6 +930,7 @@ int oracle_freesurf(const gomgpu_mesh *m, const gomgpu_params *p, gomgpu_state *
       }
     }
   }
+  if (aerr.code) ret = fail(aerr.code, aerr.entity);
   if (!ret) {
     int nz = 0;
     for (int i = 1; i <= E; i++) {
@@ -912,6 +952,7 @@ int oracle_freesurf(const gomgpu_mesh *m, const gomgpu_params *p, gomgpu_state *
     ia[E] = nz + 1;
     g_last_iters = oracle_pcg(E, a1, ja, ia, rhs1, x, p->max_iteration_pcg, p->error_tolerance);
     for (int i = 1; i <= E; i++) R1(s->eta_cell_new, i) = x[i - 1];
+#pragma omp parallel for schedule(static)
     for (int i = 1; i <= P; i++) {
       R1(s->wetdry_node, i) = 0;
       double sum1 = 0.0, sum0 = 0.0;
@@ -926,6 +967,8 @@ int oracle_freesurf(const gomgpu_mesh *m, const gomgpu_params *p, gomgpu_state *
       }
       if (R1(s->wetdry_node, i) == 0) R1(s->eta_node, i) = sum1 / sum0;
     }
+    /* reads the wet nodes, writes the dry ones */
+#pragma omp parallel for schedule(static)
     for (int i = 1; i <= P; i++) {
       if (R1(s->wetdry_node, i) == 1) {
         double sum0 = 0.0;
@@ -957,6 +1000,7 @@ int oracle_velocity(const gomgpu_mesh *m, const gomgpu_params *p, gomgpu_state *
                     const gomgpu_forcing *f) {
   const int J = p->maxface, E = p->maxele, L = p->maxlayer;
   const double dt = p->dt, g = p->gravity, th = p->theta;
+#pragma omp parallel for schedule(static)
   for (int j = 1; j <= J; j++) {
     int n1 = I2(m->nodenum_at_face, 1, j), n2 = I2(m->nodenum_at_face, 2, j);
     int kt = R1(s->top_layer_at_face, j);
@@ -1011,6 +1055,7 @@ int oracle_velocity(const gomgpu_mesh *m, const gomgpu_params *p, gomgpu_state *
       RL(s->vn_face_new, k, j) = RL(s->vn_face_new, kt, j);
     }
   }
+#pragma omp parallel for schedule(static)
   for (int i = 1; i <= E; i++) {
     int t = R1(s->top_layer_at_element, i), b = R1(m->bottom_layer_at_element, i);
     if (t == 0) {
@@ -1157,12 +1202,13 @@ int oracle_nodevel(const gomgpu_mesh *m, const gomgpu_params *p, gomgpu_state *s
                    const gomgpu_forcing *f) {
   (void)f;
   const int J = p->maxface, P = p->maxnod, L = p->maxlayer, NB = p->max_no_neighbor_node;
-  double uf[66], vf[66];
+#pragma omp parallel for schedule(static)
   for (int j = 1; j <= J; j++) {
     int kt = R1(s->top_layer_at_face, j);
     if (kt == 0) continue;
     int kb = R1(m->bottom_layer_at_face, j);
     double ct = R1(m->cos_theta, j), st = R1(m->sin_theta, j);
+    double uf[66], vf[66];
     for (int k = kb; k <= kt; k++) {
       double un = RL(s->un_face_new, k, j), vn = RL(s->vn_face_new, k, j);
       uf[k] = un * ct - vn * st;
@@ -1178,6 +1224,7 @@ int oracle_nodevel(const gomgpu_mesh *m, const gomgpu_params *p, gomgpu_state *s
       R0(s->v_face_level, k, j) = vf[k] + r * (vf[k + 1] - vf[k]);
     }
   }
+#pragma omp parallel for schedule(static)
   for (int i = 1; i <= P; i++) {
     int nadj = R1(m->adj_cells_at_node, i);
     for (int k = 0; k <= L; k++) {
@@ -1208,6 +1255,7 @@ int oracle_nodevel(const gomgpu_mesh *m, const gomgpu_params *p, gomgpu_state *s
       R0(s->v_node, k, i) = v;
     }
   }
+#pragma omp parallel for schedule(static)
   for (int i = 1; i <= P; i++) {
     int nadj = R1(m->adj_cells_at_node, i);
     for (int k = 0; k <= L; k++) {
@@ -1269,8 +1317,6 @@ int oracle_transport(const gomgpu_mesh *m, const gomgpu_params *p, gomgpu_state 
   double *PSIj = calloc((size_t)L * J * NT, sizeof(double)), *PSIi = calloc((size_t)(L + 1) * E * NT, sizeof(double));
   double *obk = calloc((size_t)L * (NOB > 0 ? NOB : 1) * NT, sizeof(double));
   double *qbk = calloc((size_t)L * (NQB > 0 ? NQB : 1) * NT, sizeof(double));
-  double oqb[64][2];
-  double a[64], b[64], cc[64], rhs[2 * 64], sol[2 * 64];
   int ret = 0;
 #define CON(k, i, t) con[(k) + (size_t)((i) - 1) * (L + 2) + (size_t)(t) * LC]
 #define CNEW(k, i, t) con_new[((k) - 1) + (size_t)((i) - 1) * L + (size_t)(t) * L * E]
@@ -1283,6 +1329,7 @@ int oracle_transport(const gomgpu_mesh *m, const gomgpu_params *p, gomgpu_state 
 #define OBK(k, i, t) obk[((k) - 1) + (size_t)((i) - 1) * L + (size_t)(t) * L * (NOB > 0 ? NOB : 1)]
 #define QBK(k, i, t) qbk[((k) - 1) + (size_t)((i) - 1) * L + (size_t)(t) * L * (NQB > 0 ? NQB : 1)]
   for (int t = 0; t < NT; t++) {
+#pragma omp parallel for schedule(static)
     for (int i = 1; i <= E; i++) {
       const double *src = (tid[t] == 1) ? s->salt_cell : s->temp_cell;
       for (int k = 1; k <= L; k++) CON(k, i, t) = RL(src, k, i);
@@ -1334,6 +1381,7 @@ int oracle_transport(const gomgpu_mesh *m, const gomgpu_params *p, gomgpu_state 
         for (int t = 0; t < NT; t++) QBK(k, i, t) = f->con_at_Qb[sub + (size_t)Nt * (t + (size_t)NT * (i - 1))];
     }
     /* face and cell coefficients (:549-611) */
+#pragma omp parallel for schedule(static)
     for (int j = 1; j <= J; j++) {
       int bl = R1(m->bottom_layer_at_face, j), tl = R1(s->top_layer_at_face, j);
       for (int k = bl; k <= tl; k++) {
@@ -1346,6 +1394,7 @@ int oracle_transport(const gomgpu_mesh *m, const gomgpu_params *p, gomgpu_state 
         FJ(phij, k, j) = (Q == 0.0) ? 1.0 : fmin2(1.0, 2.0 * D / fabs(Q));
       }
     }
+#pragma omp parallel for schedule(static)
     for (int i = 1; i <= E; i++) {
       int bl = R1(m->bottom_layer_at_element, i), tl = R1(s->top_layer_at_element, i);
       for (int k = bl; k <= tl - 1; k++) {
@@ -1360,11 +1409,14 @@ int oracle_transport(const gomgpu_mesh *m, const gomgpu_params *p, gomgpu_state 
     }
     /* limiter ratios (:615-904) */
     if (p->h_flux_limiter > 0 || p->v_flux_limiter > 0) {
+#pragma omp parallel for schedule(static)
       for (int i = 1; i <= E; i++) {
         int tl = R1(s->top_layer_at_element, i), bl = R1(m->bottom_layer_at_element, i);
         if (tl == 0) continue;
         int nv = tl - bl + 1, N = R1(m->tri_or_quad, i);
+        double oqb[64][2];
         int iob = R1(m->ob_element_flag, i), iqb = R1(m->Qb_element_flag, i);
+        for (int k = bl; k <= tl; k++) oqb[k][0] = oqb[k][1] = 0.0;
         if (iob > 0)
           for (int k = bl; k <= tl; k++)
             for (int t = 0; t < NT; t++) oqb[k][t] = OBK(k, iob, t);
@@ -1417,7 +1469,9 @@ int oracle_transport(const gomgpu_mesh *m, const gomgpu_params *p, gomgpu_state 
           for (int l = 1; l <= N; l++) {
             int j = I4(m->facenum_at_cell, l, i), ii = I4(m->adj_cellnum_at_cell, l, i);
             if (R1(m->boundary_type_of_face, j) != -1 || R1(m->isflowside3, j) > 0) {
-              if (Q2[l] >= 0.0) {
+              /* without flow both cells of the face write it; the serial loop keeps the
+                 higher-numbered cell's value */
+              if (Q2[l] > 0.0 || (Q2[l] == 0.0 && ii < i)) {
                 for (int t = 0; t < NT; t++) {
                   double ci = ii > 0 ? CON(k, ii, t) : oqb[k][t];
                   double rt = s2a[t] * (ci - CON(k, i, t));
@@ -1448,12 +1502,14 @@ int oracle_transport(const gomgpu_mesh *m, const gomgpu_params *p, gomgpu_state 
           RI(bl - 1, i, t) = 1.0;
         }
       }
+#pragma omp parallel for schedule(static)
       for (int i = 1; i <= E; i++) {
         int bl = R1(m->bottom_layer_at_element, i), tl = R1(s->top_layer_at_element, i);
         for (int k = bl; k <= tl - 1; k++)
           for (int t = 0; t < NT; t++)
             PSI_(k, i, t) = (p->h_flux_limiter == 0) ? 0.0 : limiter(p->h_flux_limiter, RI(k, i, t), FI(phii, k, i));
       }
+#pragma omp parallel for schedule(static)
       for (int j = 1; j <= J; j++) {
         int bl = R1(m->bottom_layer_at_face, j), tl = R1(s->top_layer_at_face, j);
         for (int k = bl; k <= tl; k++)
@@ -1462,11 +1518,14 @@ int oracle_transport(const gomgpu_mesh *m, const gomgpu_params *p, gomgpu_state 
       }
     }
     /* per-column update (:909-1346) */
-    for (int i = 1; i <= E && !ret; i++) {
+    loop_err col_err = {0, 0};
+#pragma omp parallel for schedule(static)
+    for (int i = 1; i <= E; i++) {
       int bl = R1(m->bottom_layer_at_element, i), tl = R1(s->top_layer_at_element, i);
       if (tl == 0) continue;
       int nv = tl - bl + 1, N = R1(m->tri_or_quad, i);
       double ar = R1(m->area, i);
+      double oqb[64][2], a[64], b[64], cc[64], rhs[2 * 64], sol[2 * 64];
       for (int k = bl + 1; k <= tl - 1; k++) {
         int kk = tl + 1 - k;
         a[kk] = -dt2 * FI(di, k, i);
@@ -1486,6 +1545,7 @@ int oracle_transport(const gomgpu_mesh *m, const gomgpu_params *p, gomgpu_state 
         b[nv] = -a[nv] + ar * dzn;
       }
       int iob = R1(m->ob_element_flag, i), iqb = R1(m->Qb_element_flag, i);
+      for (int k = bl; k <= tl; k++) oqb[k][0] = oqb[k][1] = 0.0;
       if (iob > 0)
         for (int k = bl; k <= tl; k++)
           for (int t = 0; t < NT; t++) oqb[k][t] = OBK(k, iob, t);
@@ -1558,8 +1618,8 @@ int oracle_transport(const gomgpu_mesh *m, const gomgpu_params *p, gomgpu_state 
       }
       int err = oracle_tridiag(nv, NT, 64, a + 1, b + 1, cc + 1, rhs, sol);
       if (err) {
-        ret = fail(err, i);
-        break;
+        loop_fail(&col_err, err, i);
+        continue;
       }
       for (int k = bl; k <= tl; k++) {
         int kk = tl - k + 1;
@@ -1574,6 +1634,11 @@ int oracle_transport(const gomgpu_mesh *m, const gomgpu_params *p, gomgpu_state 
         }
       }
     }
+    if (col_err.code) {
+      ret = fail(col_err.code, col_err.entity);
+      break;
+    }
+#pragma omp parallel for schedule(static)
     for (int i = 1; i <= E; i++) {
       int bl = R1(m->bottom_layer_at_element, i), tl = R1(s->top_layer_at_element, i);
       for (int t = 0; t < NT; t++) {
@@ -1585,6 +1650,7 @@ int oracle_transport(const gomgpu_mesh *m, const gomgpu_params *p, gomgpu_state 
   }
   if (!ret) {
     /* node and face values (:1403-1468) */
+#pragma omp parallel for schedule(static)
     for (int i = 1; i <= P; i++) {
       int tn = R1(s->top_layer_at_node, i), bn = R1(m->bottom_layer_at_node, i);
       int nadj = R1(m->adj_cells_at_node, i);
@@ -1608,6 +1674,7 @@ int oracle_transport(const gomgpu_mesh *m, const gomgpu_params *p, gomgpu_state 
         }
       }
     }
+#pragma omp parallel for schedule(static)
     for (int j = 1; j <= J; j++) {
       int n1 = I2(m->nodenum_at_face, 1, j), n2 = I2(m->nodenum_at_face, 2, j);
       for (int k = R1(m->bottom_layer_at_face, j); k <= R1(s->top_layer_at_face, j); k++)
@@ -2102,6 +2169,8 @@ int oracle_eos(const gomgpu_mesh *m, const gomgpu_params *p, gomgpu_state *s,
   const int E = p->maxele, J = p->maxface, P = p->maxnod, L = p->maxlayer;
   const int full = (p->ana_density == 0);
   if (p->ana_density != 0 && p->ana_density != 1) return GOMGPU_E_ARG;
+  loop_err e = {0, 0};
+#pragma omp parallel for schedule(static)
   for (int i = 1; i <= P; i++) {
     int tl = R1(s->top_layer_at_node, i);
     if (tl == 0) continue;
@@ -2109,10 +2178,15 @@ int oracle_eos(const gomgpu_mesh *m, const gomgpu_params *p, gomgpu_state *s,
     double *r = &RL(s->rho_node, 1, i);
     int err = eos_column(m, p, L, full, bl, tl, &RL(s->temp_node, 1, i), &RL(s->salt_node, 1, i), r,
                          &RL(s->dz_node, 1, i), i, 1);
-    if (err) return err;
+    if (err) {
+      loop_fail(&e, err, i);
+      continue;
+    }
     for (int k = 1; k <= bl - 1; k++) r[k - 1] = r[bl - 1];
     for (int k = tl + 1; k <= L; k++) r[k - 1] = r[tl - 1];
   }
+  if (e.code) return fail(e.code, e.entity);
+#pragma omp parallel for schedule(static)
   for (int j = 1; j <= J; j++) {
     int tl = R1(s->top_layer_at_face, j);
     if (tl == 0) continue;
@@ -2120,10 +2194,15 @@ int oracle_eos(const gomgpu_mesh *m, const gomgpu_params *p, gomgpu_state *s,
     double *r = &RL(s->rho_face, 1, j);
     int err = eos_column(m, p, L, full, bl, tl, &RL(s->temp_face, 1, j), &RL(s->salt_face, 1, j), r,
                          &RL(s->dz_face, 1, j), j, full || p->i_density_flag == 1);
-    if (err) return err;
+    if (err) {
+      loop_fail(&e, err, j);
+      continue;
+    }
     for (int k = 1; k <= bl - 1; k++) r[k - 1] = r[bl - 1];
     for (int k = tl + 1; k <= L; k++) r[k - 1] = r[tl - 1];
   }
+  if (e.code) return fail(e.code, e.entity);
+#pragma omp parallel for schedule(static)
   for (int i = 1; i <= E; i++) {
     int tl = R1(s->top_layer_at_element, i);
     if (tl == 0) continue;
@@ -2138,12 +2217,16 @@ int oracle_eos(const gomgpu_mesh *m, const gomgpu_params *p, gomgpu_state *s,
           r = r + RL(s->rho_node, k, nd);
         }
       }
-      if (icount == 0) return fail(GOMGPU_ERR_EOS_DRY_NODES, i);
+      if (icount == 0) {
+        loop_fail(&e, GOMGPU_ERR_EOS_DRY_NODES, i);
+        break;
+      }
       RL(s->rho_cell, k, i) = r / icount;
     }
     for (int k = 1; k <= bl - 1; k++) RL(s->rho_cell, k, i) = RL(s->rho_cell, bl, i);
     for (int k = tl + 1; k <= L; k++) RL(s->rho_cell, k, i) = RL(s->rho_cell, tl, i);
   }
+  if (e.code) return fail(e.code, e.entity);
   return 0;
 }
 
@@ -2231,6 +2314,7 @@ static double rz_level(const gomgpu_params *p, const gomgpu_state *s, int L, int
  * fallback of the node mean at turbulence_closure_v1.f90:456-460). */
 static void kv_cells(const gomgpu_mesh *m, const gomgpu_params *p, gomgpu_state *s, const double *kvf) {
   const int E = p->maxele, L = p->maxlayer;
+#pragma omp parallel for schedule(static)
   for (int i = 1; i <= E; i++) {
     if (R1(s->top_layer_at_element, i) == 0) continue;
     int N = R1(m->tri_or_quad, i);
@@ -2262,9 +2346,12 @@ int oracle_turbulence(const gomgpu_mesh *m, const gomgpu_params *p, gomgpu_state
   const double dt = p->dt, g = p->gravity;
   const double fs_coef = (double)powf(16.6f, 2.0f / 3.0f);  /* 16.6**(2.0/3), default real */
   int err = 0;
-  for (int j = 1; j <= J && !err; j++) {
+  loop_err e = {0, 0};
+#pragma omp parallel for schedule(dynamic, 64)
+  for (int j = 1; j <= J; j++) {
     int kt = R1(s->top_layer_at_face, j);
     if (kt == 0) continue;
+    int ferr = 0;
     int kb = R1(m->bottom_layer_at_face, j);
     double *KV = &kvf[(size_t)(j - 1) * (L + 1)];
     if (flag == 1 || flag == 2) {
@@ -2329,7 +2416,7 @@ int oracle_turbulence(const gomgpu_mesh *m, const gomgpu_params *p, gomgpu_state
       q2bt[k] = (R0(s->q2, k - 1, j) + R0(s->q2, k, j)) / 2.0;
       xlbt[k] = (R0(s->xl, k - 1, j) + R0(s->xl, k, j)) / 2.0;
       if (xlbt[k] < p->xlmin2) {
-        err = fail(GOMGPU_ERR_TURB_XL, j);
+        ferr = GOMGPU_ERR_TURB_XL;
         break;
       }
       double udown = (R0(s->u_node, k - 1, n1) + R0(s->u_node, k - 1, n2)) / 2.0;
@@ -2351,7 +2438,10 @@ int oracle_turbulence(const gomgpu_mesh *m, const gomgpu_params *p, gomgpu_state
       }
       rzbt[k] = g / p->rho_o * drhodz;
     }
-    if (err) break;
+    if (ferr) {
+      loop_fail(&e, ferr, j);
+      continue;
+    }
     int a1 = I2(m->adj_cellnum_at_face, 1, j), a2 = I2(m->adj_cellnum_at_face, 2, j);
     double etam = a2 != 0 ? dmax(R1(s->eta_cell_new, a1), R1(s->eta_cell_new, a2)) : R1(s->eta_cell_new, a1);
     double hf = R1(m->h_face, j);
@@ -2361,11 +2451,14 @@ int oracle_turbulence(const gomgpu_mesh *m, const gomgpu_params *p, gomgpu_state
       double dists = p->MSL + etam - zctr, distb = zctr - (p->MSL - hf);
       double xlmax = (double)0.4f * dists * distb / (hf + etam);
       if (xlmax <= 0.0) {
-        err = fail(GOMGPU_ERR_TURB_DIST, j);
+        ferr = GOMGPU_ERR_TURB_DIST;
         break;
       }
     }
-    if (err) break;
+    if (ferr) {
+      loop_fail(&e, ferr, j);
+      continue;
+    }
     /* boundary values (:219-227) */
     double wn = f->wind_stress_normal[j - 1], wt = f->wind_stress_tangnt[j - 1];
     double q2fs = fs_coef * sqrt(wn * wn + wt * wt) / 2.0;
@@ -2407,10 +2500,10 @@ int oracle_turbulence(const gomgpu_mesh *m, const gomgpu_params *p, gomgpu_state
       a[0] = 0.0;
       c[nq - 1] = 0.0;
     }
-    err = oracle_tridiag(nq, 1, 66, a, b, c, rhs, sol);
-    if (err) {
-      err = fail(err, j);
-      break;
+    ferr = oracle_tridiag(nq, 1, 66, a, b, c, rhs, sol);
+    if (ferr) {
+      loop_fail(&e, ferr, j);
+      continue;
     }
     for (int k = 1; k <= nq; k++) { /* :268-278 */
       int kl = kt - k + 1;
@@ -2475,10 +2568,10 @@ int oracle_turbulence(const gomgpu_mesh *m, const gomgpu_params *p, gomgpu_state
       a[0] = 0.0;
       c[nq - 1] = 0.0;
     }
-    err = oracle_tridiag(nq, 1, 66, a, b, c, rhs, sol);
-    if (err) {
-      err = fail(err, j);
-      break;
+    ferr = oracle_tridiag(nq, 1, 66, a, b, c, rhs, sol);
+    if (ferr) {
+      loop_fail(&e, ferr, j);
+      continue;
     }
     for (int k = 1; k <= nq; k++) { /* :351-368 */
       int kl = kt - k + 1;
@@ -2511,14 +2604,17 @@ int oracle_turbulence(const gomgpu_mesh *m, const gomgpu_params *p, gomgpu_state
         q = q2tmp[k] + r * (q2tmp[k + 1] - q2tmp[k]);
         x = xltmp[k] + r * (xltmp[k + 1] - xltmp[k]);
         if (q < 0.0 || x < p->xlmin2) {
-          err = fail(GOMGPU_ERR_TURB_NEGATIVE, j);
+          ferr = GOMGPU_ERR_TURB_NEGATIVE;
           break;
         }
       }
       R0(s->q2, k, j) = q;
       R0(s->xl, k, j) = x;
     }
-    if (err) break;
+    if (ferr) {
+      loop_fail(&e, ferr, j);
+      continue;
+    }
     for (int k = kb; k <= kt; k++) { /* new diffusivities (:397-405) */
       double vd, td, qd, qd2;
       alg_stress(p, R0(s->q2, k, j), R0(s->xl, k, j), rz_level(p, s, L, j, k, kt), &vd, &td, &qd, &qd2);
@@ -2540,6 +2636,7 @@ int oracle_turbulence(const gomgpu_mesh *m, const gomgpu_params *p, gomgpu_state
       KV[k] = KV[kt];
     }
   }
+  if (e.code) err = fail(e.code, e.entity);
   if (!err) kv_cells(m, p, s, kvf);
   free(kvf);
   return err;

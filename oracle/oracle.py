@@ -13,20 +13,24 @@ import numpy as np
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB = os.path.join(HERE, "liboracle.so")
-_lib = None
+LIB_OMP = os.path.join(HERE, "liboracle_omp.so")   # the same source threaded with OpenMP
+_libs = {}
 
 
 def build() -> None:
-    subprocess.run(["make", "-C", HERE, "liboracle.so"], check=True,
+    subprocess.run(["make", "-C", HERE, "liboracle.so", "liboracle_omp.so"], check=True,
                    stdout=subprocess.DEVNULL)
 
 
-def lib():
-    global _lib
-    if _lib is None:
-        if not os.path.exists(LIB):
+def lib(threaded: bool = False):
+    """The serial build, or with threaded the OpenMP build (OMP_NUM_THREADS threads), which
+    gives the same bits."""
+    path = LIB_OMP if threaded else LIB
+    if path not in _libs:
+        if not os.path.exists(path):
             build()
-        _lib = C.CDLL(LIB)
+        _lib = C.CDLL(path)
+        _libs[path] = _lib
         from gom_release_amd import abi
         args = [C.POINTER(abi.Mesh), C.POINTER(abi.Params), C.POINTER(abi.State),
                 C.POINTER(abi.Forcing)]
@@ -48,21 +52,24 @@ def lib():
         _lib.oracle_elm_near_edge.argtypes = [C.c_int]
         _lib.oracle_elm_near_edge.restype = C.c_longlong
         _lib.oracle_elm_near_reset.restype = None
-    return _lib
+    return _libs[path]
 
 
 class Oracle:
-    """Runs oracle stages on numpy-backed ABI arrays (mesh, params, state, forcing)."""
+    """Runs oracle stages on numpy-backed ABI arrays (mesh, params, state, forcing). With
+    threaded the stages run in the OpenMP build; the static accessors (iteration counts,
+    error entities, near-edge counts) read the serial build's."""
 
-    def __init__(self, mesh, params, state, forcing):
+    def __init__(self, mesh, params, state, forcing, threaded: bool = False):
         self.mesh, self.params, self.state, self.forcing = mesh, params, state, forcing
         self._m = mesh.struct()
         self._s = state.struct()
+        self._lib = lib(threaded)
 
     def run(self, stage: str) -> int:
         f = self.forcing.struct()
-        rc = getattr(lib(), "oracle_" + stage)(C.byref(self._m), C.byref(self.params),
-                                               C.byref(self._s), C.byref(f))
+        rc = getattr(self._lib, "oracle_" + stage)(C.byref(self._m), C.byref(self.params),
+                                                   C.byref(self._s), C.byref(f))
         return rc
 
     def step(self) -> int:
@@ -73,11 +80,11 @@ class Oracle:
         [maxlayer, maxface], nnel negated for a point within 1e-12 of a cell edge."""
         L, J = self.params.maxlayer, self.params.maxface
         dep = np.zeros((2, L, J), dtype=np.int32)
-        lib().oracle_elm_record(dep.ctypes.data)
+        self._lib.oracle_elm_record(dep.ctypes.data)
         try:
             rc = self.run("elm")
         finally:
-            lib().oracle_elm_record(None)
+            self._lib.oracle_elm_record(None)
         return rc, dep[0], dep[1]
 
     @staticmethod
