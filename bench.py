@@ -92,6 +92,17 @@ def make_case(cells: str, turbulence: str = "gls", pcg_tolerance: float = PCG_TO
     return case
 
 
+# HBM bytes of one launch of k_pcg_cg1m (the solve of the storm-surge configuration's
+# 1,200,990 rows), from rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE runs of
+# `bench.py --counter-pass 3 --copy-calibration 1024` (three solves of 58, 62 and 68
+# iterations), each counter scaled by its reading of the 1 GiB copy (FETCH_SIZE x2.0,
+# WRITE_SIZE x1.0): a part per launch plus a part per iteration, least squares over the three
+# launches (residuals below 1e-4).
+PCG_TRAFFIC_CELLS = 1200990
+PCG_TRAFFIC_LAUNCH = 2.33982e8
+PCG_TRAFFIC_PER_ITERATION = 2.273454e8
+
+
 def pcg_bytes_per_iteration(mesh, params) -> float:
     """Algorithmic bytes of one PCG iteration: E*(12*zbar + 108), zbar = nnz per row."""
     import numpy as np
@@ -332,6 +343,9 @@ def main() -> int:
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="also run one device-to-device copy of MIB MiB (calibrates rocprofv3 "
                          "FETCH_SIZE/WRITE_SIZE against known traffic)")
+    ap.add_argument("--counter-pass", type=int, default=0, metavar="STEPS",
+                    help="instead of the benchmark, run STEPS steps (default launch shapes) and print "
+                         "each step's PCG iteration count, for normalising rocprofv3 --pmc counters")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build measures one GPU only (got --gpus {args.gpus}); "
@@ -357,6 +371,15 @@ def main() -> int:
     print(f"[bench] building {case.maxele}-cell x {case.maxlayer}-layer case", file=sys.stderr, flush=True)
     sc = Scenario(case, precision=prec, wind=wind)
     model = sc.model
+    if args.counter_pass > 0:
+        its = []
+        for _ in range(args.counter_pass):
+            sc.step()
+            its.append(model.pcg_iterations())
+            print(f"[bench] counter pass step {len(its)}: {its[-1]} PCG iterations", file=sys.stderr, flush=True)
+        print(json.dumps({"counter_pass": {"steps": len(its), "pcg_iterations": its, "cells": case.maxele}}))
+        model.close()
+        return 0
     launch = None
     if not args.no_autotune:
         from gom_release_amd import autotune
@@ -398,9 +421,13 @@ def main() -> int:
     pcg_bytes = pcg_bytes_per_iteration(sc.mesh, sc.params) * it_mean
     pcg_ms = kernels["k_pcg"]
     achieved = pcg_bytes / (pcg_ms * 1e-3) / 1e9
+    traffic = None
+    if sc.params.maxele == PCG_TRAFFIC_CELLS and prec == 0:
+        traffic = round(PCG_TRAFFIC_LAUNCH + PCG_TRAFFIC_PER_ITERATION * it_mean)
+    kname = "k_pcg_cg1m" if sc.params.maxele > 524288 else "k_pcg_cg1"
     roof = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-            "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
-            "kernel": "k_pcg_cg1 (PCG solve; timer k_pcg)", "pcg_iterations": round(it_mean, 1),
+            "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
+            "kernel": f"{kname} (PCG solve; timer k_pcg)", "pcg_iterations": round(it_mean, 1),
             "kernel_ms": {k: round(v, 4) for k, v in kernels.items()},
             "stage_ms": {k: round(v, 4) for k, v in stages.items()}}
     F, Cl, N = sc.active_layers()

@@ -70,6 +70,26 @@ struct DevDims {
 };
 
 // Work arrays of the step (device-owned, sized at create time).
+// Register budget of the latency-bound column and gather kernels: the compiler keeps each
+// within what the given number of waves per SIMD leaves (amdgpu_waves_per_eu), trading
+// registers for occupancy.
+#ifndef GOM_WAVES_ELM
+#define GOM_WAVES_ELM 4
+#endif
+#ifndef GOM_WAVES_MOM_RHS
+#define GOM_WAVES_MOM_RHS 1
+#endif
+#ifndef GOM_WAVES_MOM_SOLVE
+#define GOM_WAVES_MOM_SOLVE 1
+#endif
+#ifndef GOM_WAVES_NODE_VEL
+#define GOM_WAVES_NODE_VEL 1
+#endif
+#ifndef GOM_WAVES_TURB
+#define GOM_WAVES_TURB 4
+#endif
+#define GOM_WAVES(n) __attribute__((amdgpu_waves_per_eu(n)))
+
 struct DevWork {
   // free surface / PCG (E rows, ELL with the diagonal and 4 neighbour slots)
   double *fs_diag, *fs_off, *fs_rhs, *pcg_x, *pcg_r, *pcg_d0, *pcg_d1, *pcg_q, *pcg_s, *pcg_minv;
@@ -83,9 +103,10 @@ struct DevWork {
   double *face_sg;        // per face sum_k dz * AinvG1
   // transport
   double *con, *con_new;  // con (0:L+1, E, NT) and con_new (L, E, NT), layer-major
+  double *con2;           // the sub-cycles' second con buffer (con and con2 alternate)
+  double *htop;           // (2, E, NT) horizontal upwind sums of each column's top layer
   double *Qj, *dj, *phij; // (L, J)
   double *Qi, *di, *phii; // (0:L, E)
-  double *htop;           // (2, E, NT) horizontal upwind sums of each column's top layer
   double *PSIj, *PSIi;
   double *s1a, *s2a;      // (L, E, NT) limiter sums per cell-layer
   double *obk, *qbk;      // (L, OB, NT), (L, QB, NT)

@@ -47,8 +47,8 @@ void tvd_launch_coef_cell(bool, long, int, hipStream_t, DevMesh, DevState, DevWo
 void tvd_launch_lim_cell(bool, long, int, hipStream_t, DevMesh, DevState, DevWork, DevDims, int, int);
 void tvd_launch_lim_face(bool, long, int, hipStream_t, DevMesh, DevState, DevWork, DevDims, int, int);
 void tvd_launch_rhs(bool, long, int, hipStream_t, DevMesh, DevState, DevWork, DevDims, TvdArgs);
-void tvd_launch_solve(bool, long, int, hipStream_t, DevMesh, DevState, DevWork, DevDims, TvdArgs, int *);
-void tvd_launch_copyback(bool, long, int, hipStream_t, DevMesh, DevState, DevWork, DevDims, int);
+void tvd_launch_solve(bool, long, int, hipStream_t, DevMesh, DevState, DevWork, DevDims, TvdArgs, double *, int, int,
+                      int *);
 void tvd_launch_node(bool, long, int, hipStream_t, DevMesh, DevState, DevWork, DevDims);
 void tvd_launch_face_out(bool, long, int, hipStream_t, DevMesh, DevState, DevWork, DevDims);
 void tvd_launch_cellnode_out(bool, long, int, hipStream_t, DevState, DevWork, DevDims);
@@ -509,6 +509,7 @@ extern "C" int gomgpu_create(const gomgpu_mesh *mesh, const gomgpu_params *param
         (rc = dalloc(c, &w.Qj, L * J)) || (rc = dalloc(c, &w.dj, L * J)) || (rc = dalloc(c, &w.phij, L * J)) ||
         (rc = dalloc(c, &w.Qi, (L + 1) * E)) || (rc = dalloc(c, &w.di, (L + 1) * E)) ||
         (rc = dalloc(c, &w.phii, (L + 1) * E)) || (rc = dalloc(c, &w.htop, NT * 2 * E)) ||
+        (rc = dalloc(c, &w.con2, NT * (L + 2) * E)) ||
         (rc = dalloc(c, &w.PSIj, NT * L * J)) ||
         (rc = dalloc(c, &w.PSIi, NT * (L + 1) * E)) || (rc = dalloc(c, &w.s1a, NT * L * E)) ||
         (rc = dalloc(c, &w.s2a, NT * L * E)) || (rc = dalloc(c, &w.obk, NT * L * OB)) ||
@@ -959,22 +960,26 @@ static int stage_impl(gomgpu_ctx *c, int id) {
       const bool f32 = p.precision == GOMGPU_MIXED;
       k_fill<<<nblk((long)d.NT * d.L * d.E, BS), BS, 0, st>>>(c->w.con_new, 0.0, (long)d.NT * d.L * d.E);
       tvd_launch_init(f32, nE2, BS, st, c->m, c->s, c->f, c->w, d, a);
+      // the sub-cycles read con from one buffer and write the next sub-cycle's into the other
+      double *cbuf[2] = {c->w.con, c->w.con2};
       for (int sub = 0; sub < a.Nt; sub++) {
         a.sub = sub;
+        DevWork wv = c->w;
+        wv.con = cbuf[sub & 1];
         long nb = (long)d.L * (d.NOB + d.NQB);
-        if (nb > 0) tvd_launch_bnd(f32, nb, BS, st, c->m, c->s, c->f, c->w, d, a);
-        tvd_launch_coef_face(f32, (long)d.J * d.L, BS, st, c->m, c->s, c->w, d, a);
-        tvd_launch_coef_cell(f32, (long)d.E * (d.L + 1), BS, st, c->m, c->s, c->w, d, a);
+        if (nb > 0) tvd_launch_bnd(f32, nb, BS, st, c->m, c->s, c->f, wv, d, a);
+        tvd_launch_coef_face(f32, (long)d.J * d.L, BS, st, c->m, c->s, wv, d, a);
+        tvd_launch_coef_cell(f32, (long)d.E * (d.L + 1), BS, st, c->m, c->s, wv, d, a);
         if (p.h_flux_limiter > 0 || p.v_flux_limiter > 0) {
-          tvd_launch_lim_cell(f32, (long)(d.L + 1) * d.E, LB(TVD_LIM), st, c->m, c->s, c->w, d, p.h_flux_limiter,
+          tvd_launch_lim_cell(f32, (long)(d.L + 1) * d.E, LB(TVD_LIM), st, c->m, c->s, wv, d, p.h_flux_limiter,
                               sub == 0);
-          tvd_launch_lim_face(f32, (long)d.J * d.L, BS, st, c->m, c->s, c->w, d, p.h_flux_limiter, sub == 0);
+          tvd_launch_lim_face(f32, (long)d.J * d.L, BS, st, c->m, c->s, wv, d, p.h_flux_limiter, sub == 0);
         }
         KT_BEGIN(KT_TVD_COLUMN);
-        tvd_launch_rhs(f32, (long)d.L * d.E, LB(TVD_RHS), st, c->m, c->s, c->w, d, a);
-        tvd_launch_solve(f32, d.E, LB(TVD_SOLVE), st, c->m, c->s, c->w, d, a, err);
+        tvd_launch_rhs(f32, (long)d.L * d.E, LB(TVD_RHS), st, c->m, c->s, wv, d, a);
+        tvd_launch_solve(f32, d.E, LB(TVD_SOLVE), st, c->m, c->s, wv, d, a, cbuf[(sub + 1) & 1], sub < 2,
+                         sub == a.Nt - 1, err);
         KT_END(KT_TVD_COLUMN);
-        tvd_launch_copyback(f32, nE2, BS, st, c->m, c->s, c->w, d, sub == 0);
       }
       tvd_launch_node(f32, (long)d.P * ((d.L + GOM_TVD_NODE_LEVELS - 1) / GOM_TVD_NODE_LEVELS), BS, st, c->m, c->s,
                       c->w, d);
@@ -1566,7 +1571,7 @@ extern "C" long long gomgpu_footprint(const gomgpu_params *params) {
   add(NT * (L + 2) * E, D); add(NT * L * E, D);             // con, con_new
   add(L * J, D); add(L * J, D); add(L * J, D);              // Qj, dj, phij
   add((L + 1) * E, D); add((L + 1) * E, D); add((L + 1) * E, D);  // Qi, di, phii
-  add(NT * 2 * E, D);                                       // htop
+  add(NT * 2 * E, D); add(NT * (L + 2) * E, D);             // htop, con2
   add(NT * L * J, D); add(NT * (L + 1) * E, D);             // PSIj, PSIi
   add(NT * L * E, D); add(NT * L * E, D);                   // s1a, s2a
   add(NT * L * OB, D); add(NT * L * QB, D);                 // obk, qbk

@@ -445,7 +445,7 @@ __device__ __forceinline__ int backtrace(const ElmCtx &c, int bt_step, double bt
 // of a cell edge are counted in near_count. With dep non-NULL, the departure cell (negated
 // for such a point) and level of face-layer t = (k-1) J + j go to dep[t] and dep[L J + t]
 // (0 where nothing is backtraced).
-__global__ void __launch_bounds__(256) k_elm(DevMesh m, DevState s, DevDims d, double MSL, double dt, double onoff,
+__global__ void __launch_bounds__(256) GOM_WAVES(GOM_WAVES_ELM) k_elm(DevMesh m, DevState s, DevDims d, double MSL, double dt, double onoff,
                                              int *err, int32_t *dep, unsigned long long *near_count) {
   int t = blockIdx.x * blockDim.x + threadIdx.x;
   const int J = d.J, E = d.E, P = d.P;
@@ -989,7 +989,7 @@ __global__ void k_cell_uv(DevMesh m, DevState s, DevWork w, DevDims d) {
 // adding +0.0 to a sum that starts at +0.0 changes nothing, and halving the product equals
 // halving the density difference first, both exact), and each row then adds them from LDS in
 // the reference's order: one gather per layer instead of one per (row, layer).
-__global__ void __launch_bounds__(256) k_mom_rhs(DevMesh m, DevState s, DevForcing f, DevWork w, DevDims d,
+__global__ void __launch_bounds__(256) GOM_WAVES(GOM_WAVES_MOM_RHS) k_mom_rhs(DevMesh m, DevState s, DevForcing f, DevWork w, DevDims d,
                                                  MomParams q) {
   extern __shared__ double bct[];  // [2][L][64] baroclinic products (cells, nodes)
   const int J = d.J, E = d.E, P = d.P, L = d.L;
@@ -1174,7 +1174,7 @@ __device__ __forceinline__ void mom_row(const DevState &s, const double *gamma_t
 }
 
 #define MS_CHUNK 4
-__global__ void k_mom_solve(DevMesh m, DevState s, DevForcing f, DevWork w, DevDims d, MomParams q, int *err) {
+__global__ void GOM_WAVES(GOM_WAVES_MOM_SOLVE) k_mom_solve(DevMesh m, DevState s, DevForcing f, DevWork w, DevDims d, MomParams q, int *err) {
   int j = blockIdx.x * blockDim.x + threadIdx.x;
   const int J = d.J, L = d.L;
   if (j >= J) return;
@@ -1710,7 +1710,7 @@ __global__ void k_face_levels(DevMesh m, DevState s, DevDims d) {
 // reference's order; the weight sum is precomputed in that order.
 #define NV_CHUNK 2
 #define NV_LEVELS GOM_NV_LEVELS
-__global__ void k_node_vel(DevMesh m, DevState s, DevDims d) {
+__global__ void GOM_WAVES(GOM_WAVES_NODE_VEL) k_node_vel(DevMesh m, DevState s, DevDims d) {
   // NV_LEVELS consecutive levels of one node per thread: the node's stencil entries are
   // loaded once per chunk of adjacent cells and serve every level of the thread
   const int P = d.P, E = d.E, J = d.J, L1 = d.L + 1;

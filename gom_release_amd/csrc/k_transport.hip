@@ -28,9 +28,9 @@ static TvdW<W> tvd_view(const DevWork &w) {
   v.con = w.con;
   v.con_new = w.con_new;
   v.Qi = w.Qi;
+  v.htop = w.htop;
   v.di = w.di;
   v.phii = w.phii;
-  v.htop = w.htop;
   v.PSIi = w.PSIi;
   v.obk = w.obk;
   v.qbk = w.qbk;
@@ -354,25 +354,18 @@ __global__ void k_tvd_lim_face(DevMesh m, DevState s, TvdW<W> w, DevDims d, int 
   for (int t = 0; t < d.NT; t++) w.PSIj[nix(t, k, j, L, J)] = limiter(hlim, r[t], AL(w.phij, k, j, J));
 }
 
-// Per-column implicit update (solve_transport_equation_v19.f90:921-1303), in two passes.
-// k_tvd_rhs, one thread per (layer, cell): horizontal upwind advection, diffusion and
-// limiter fluxes and the vertical advection/limiter terms, explicit, into w.tvd_x (row of
-// the layer); k_tvd_solve, one thread per cell column: vertical diffusion implicit (the
-// Thomas recurrence of tridiagonal_solver.f90 with NT right-hand sides, in place in
-// w.tvd_x, elimination factors in w.tri_gam), then the clamp to [0, 50] into con_new.
+// Per-column implicit update (solve_transport_equation_v19.f90:921-1303) in two passes.
+// tvd_rhs_layer forms the explicit right-hand side of layer k: horizontal upwind advection,
+// diffusion and limiter fluxes, and the vertical advection and limiter terms.
 template <typename W>
-__global__ void k_tvd_rhs(DevMesh m, DevState s, TvdW<W> w, DevDims d, TvdArgs a) {
-  long q = (long)blockIdx.x * blockDim.x + threadIdx.x;
+__device__ __forceinline__ void tvd_rhs_layer(const DevMesh &m, const DevState &s, const TvdW<W> &w,
+                                              const DevDims &d, const TvdArgs &a, int e, int k, int bl, int tl,
+                                              double *x) {
   const int E = d.E, L = d.L, L1 = d.L + 1, L2 = d.L + 2, J = d.J;
-  if (q >= (long)L * E) return;
-  int e = (int)(q % E), k = (int)(q / E) + 1;
-  int bl = m.bottom_layer_at_element[e], tl = s.top_layer_at_element[e];
-  if (tl == 0 || k < bl || k > tl) return;
   const int NT = d.NT, Nt = a.Nt, sub = a.sub;
   const double dt2 = a.dt2;
   int nv = tl - bl + 1, N = m.tri_or_quad[e];
   double ar = m.area[e];
-  double x[2];
   double sqi[2] = {0, 0}, sqii[2] = {0, 0}, shd[2] = {0, 0}, shl[2] = {0, 0};
   for (int l = 1; l <= N; l++) {
     int j = TB(m.facenum_at_cell, l, e, E) - 1, ii = TB(m.adj_cellnum_at_cell, l, e, E);
@@ -435,7 +428,6 @@ __global__ void k_tvd_rhs(DevMesh m, DevState s, TvdW<W> w, DevDims d, TvdArgs a
       if (a.vlim > 0) x[t] = x[t] + vft;
     }
   }
-  for (int t = 0; t < NT; t++) w.tvd_x[nix(t, k, e, L, E)] = x[t];
 }
 
 // Tridiagonal row r (0-based, surface first) of cell e: solve_transport_equation_v19.f90:957-1025
@@ -469,14 +461,47 @@ __device__ __forceinline__ void tvd_row(const DevState &s, const TvdW<W> &w, int
   }
 }
 
+// The clamp of solve_transport_equation_v19.f90:1310-1322.
+__device__ __forceinline__ double tvd_clamp(double v) {
+  if (v >= 0.0 && v <= 50.0) return v;
+  return v < 0.0 ? 0.0 : 50.0;
+}
+
+// The explicit right-hand sides, one thread per (layer, cell), into w.tvd_x (row of the layer).
 template <typename W>
-__global__ void k_tvd_solve(DevMesh m, DevState s, TvdW<W> w, DevDims d, TvdArgs a, int *err) {
-  int e = blockIdx.x * blockDim.x + threadIdx.x;
+__global__ void k_tvd_rhs(DevMesh m, DevState s, TvdW<W> w, DevDims d, TvdArgs a) {
+  long q = (long)blockIdx.x * blockDim.x + threadIdx.x;
   const int E = d.E, L = d.L;
-  if (e >= E) return;
+  if (q >= (long)L * E) return;
+  int e = (int)(q % E), k = (int)(q / E) + 1;
   int bl = m.bottom_layer_at_element[e], tl = s.top_layer_at_element[e];
+  if (tl == 0 || k < bl || k > tl) return;
+  double x[2];
+  tvd_rhs_layer(m, s, w, d, a, e, k, bl, tl, x);
+  for (int t = 0; t < d.NT; t++) w.tvd_x[nix(t, k, e, L, E)] = x[t];
+}
+
+// One thread per cell column: vertical diffusion implicit, the Thomas recurrence of
+// tridiagonal_solver.f90 with NT right-hand sides in place in w.tvd_x (factors in
+// w.tri_gam), then the clamp to [0, 50]. The result goes to the next sub-cycle's
+// concentration buffer con_next (every level in the first two sub-cycles, so both buffers
+// hold the reference's zeros outside the wet range, solve_transport_equation_v19.f90:
+// 1352-1371; the wet range after that) or, in the last sub-cycle, to con_new.
+template <typename W>
+__global__ void k_tvd_solve(DevMesh m, DevState s, TvdW<W> w, DevDims d, TvdArgs a, double *con_next, int full,
+                            int last, int *err) {
+  int e = blockIdx.x * blockDim.x + threadIdx.x;
+  const int E = d.E, L = d.L, L2 = d.L + 2;
+  if (e >= E) return;
+  const int bl = m.bottom_layer_at_element[e], tl = s.top_layer_at_element[e];
+  const int NT = d.NT;
+  if (!last && full) {  // the levels outside the wet range (all of a dry column)
+    for (int k = 0; k <= L + 1; k++)
+      if (tl == 0 || k < bl || k > tl)
+        for (int t = 0; t < NT; t++) con_next[cix(t, k, e, L2, E)] = 0.0;
+  }
   if (tl == 0) return;
-  const int NT = d.NT, Nt = a.Nt, sub = a.sub;
+  const int Nt = a.Nt, sub = a.sub;
   const double dt2 = a.dt2;
   int nv = tl - bl + 1;
   double ar = m.area[e];
@@ -509,46 +534,25 @@ __global__ void k_tvd_solve(DevMesh m, DevState s, TvdW<W> w, DevDims d, TvdArgs
     AL(w.tri_gam, r + 1, e, E) = gam;
     cprev = cv;
   }
-  // back substitution, then the clamp into con_new (rows top down = layers tl .. bl)
+  // back substitution, then the clamp (rows top down = layers tl .. bl)
   for (int t = 0; t < NT; t++) {
-    double v = x[t];
-    size_t o = nix(t, bl, e, L, E);
-    if (v >= 0.0 && v <= 50.0)
-      w.con_new[o] = v;
-    else if (v < 0.0)
-      w.con_new[o] = 0.0;
-    else if (v > 50.0)
-      w.con_new[o] = 50.0;
+    double v = tvd_clamp(x[t]);
+    if (last)
+      w.con_new[nix(t, bl, e, L, E)] = v;
+    else
+      con_next[cix(t, bl, e, L2, E)] = v;
   }
   for (int r = nv - 2; r >= 0; r--) {
     double gam = AL(w.tri_gam, r + 2, e, E);
     for (int t = 0; t < NT; t++) {
-      size_t o = nix(t, tl - r, e, L, E);
-      x[t] = w.tvd_x[o] - gam * x[t];
-      double v = x[t];
-      if (v >= 0.0 && v <= 50.0)
-        w.con_new[o] = v;
-      else if (v < 0.0)
-        w.con_new[o] = 0.0;
-      else if (v > 50.0)
-        w.con_new[o] = 50.0;
+      x[t] = w.tvd_x[nix(t, tl - r, e, L, E)] - gam * x[t];
+      double v = tvd_clamp(x[t]);
+      if (last)
+        w.con_new[nix(t, tl - r, e, L, E)] = v;
+      else
+        con_next[cix(t, tl - r, e, L2, E)] = v;
     }
   }
-}
-
-// con <- con_new in the active range, zero elsewhere (solve_transport_equation_v19.f90:1352-1371)
-// (out of the range the zeros written after the first sub-cycle stay)
-template <typename W>
-__global__ void k_tvd_copyback(DevMesh m, DevState s, TvdW<W> w, DevDims d, int first) {
-  int q = blockIdx.x * blockDim.x + threadIdx.x;
-  const int E = d.E, L = d.L, L2 = d.L + 2;
-  if (q >= E * L2) return;
-  int e = q % E, k = q / E;
-  int bl = m.bottom_layer_at_element[e], tl = s.top_layer_at_element[e];
-  const bool in = k >= bl && k <= tl;
-  if (!in && !first) return;
-  for (int t = 0; t < d.NT; t++)
-    w.con[cix(t, k, e, L2, E)] = in ? w.con_new[nix(t, k, e, L, E)] : 0.0;
 }
 
 // Node values: area*dz weighted cell means, top down (solve_transport_equation_v19.f90:1403-1453).
@@ -858,14 +862,9 @@ void tvd_launch_rhs(bool f32, long n, int bs, hipStream_t st, DevMesh m, DevStat
   else k_tvd_rhs<double><<<TVD_GRID(n, bs)>>>(m, s, tvd_view<double>(w), d, a);
 }
 void tvd_launch_solve(bool f32, long n, int bs, hipStream_t st, DevMesh m, DevState s, DevWork w, DevDims d,
-                      TvdArgs a, int *err) {
-  if (f32) k_tvd_solve<float><<<TVD_GRID(n, bs)>>>(m, s, tvd_view<float>(w), d, a, err);
-  else k_tvd_solve<double><<<TVD_GRID(n, bs)>>>(m, s, tvd_view<double>(w), d, a, err);
-}
-void tvd_launch_copyback(bool f32, long n, int bs, hipStream_t st, DevMesh m, DevState s, DevWork w, DevDims d,
-                         int first) {
-  if (f32) k_tvd_copyback<float><<<TVD_GRID(n, bs)>>>(m, s, tvd_view<float>(w), d, first);
-  else k_tvd_copyback<double><<<TVD_GRID(n, bs)>>>(m, s, tvd_view<double>(w), d, first);
+                      TvdArgs a, double *con_next, int full, int last, int *err) {
+  if (f32) k_tvd_solve<float><<<TVD_GRID(n, bs)>>>(m, s, tvd_view<float>(w), d, a, con_next, full, last, err);
+  else k_tvd_solve<double><<<TVD_GRID(n, bs)>>>(m, s, tvd_view<double>(w), d, a, con_next, full, last, err);
 }
 void tvd_launch_node(bool f32, long n, int bs, hipStream_t st, DevMesh m, DevState s, DevWork w, DevDims d) {
   if (f32) k_tvd_node<float><<<TVD_GRID(n, bs)>>>(m, s, tvd_view<float>(w), d);

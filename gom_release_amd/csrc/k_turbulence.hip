@@ -150,7 +150,7 @@ __device__ __forceinline__ void level_diff(const TurbParams &p, const DevState &
 //   carried from the row above;
 // * the length scale with Galperin's clipping and k, l back at whole levels (:351-390);
 // * the new diffusivities at every level, clamped to the column's range (:397-427).
-__global__ void __launch_bounds__(256) k_turb_column(DevMesh m, DevState s, DevForcing f, DevWork w, DevDims d,
+__global__ void __launch_bounds__(256) GOM_WAVES(GOM_WAVES_TURB) k_turb_column(DevMesh m, DevState s, DevForcing f, DevWork w, DevDims d,
                                                      TurbParams p, int *err) {
   int j = blockIdx.x * blockDim.x + threadIdx.x;
   const int J = d.J;

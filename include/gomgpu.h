@@ -326,8 +326,8 @@ enum {
   GOMGPU_LAUNCH_ELM = 0,        /* k_elm, one thread per (layer, face) */
   GOMGPU_LAUNCH_MOM_RHS = 1,    /* k_mom_rhs, 64 faces per block, one wave per row stride */
   GOMGPU_LAUNCH_MOM_SOLVE = 2,  /* k_mom_solve, one thread per face column */
-  GOMGPU_LAUNCH_TVD_RHS = 3,    /* k_tvd_rhs */
-  GOMGPU_LAUNCH_TVD_SOLVE = 4,  /* k_tvd_solve */
+  GOMGPU_LAUNCH_TVD_RHS = 3,    /* k_tvd_rhs, one thread per (layer, cell) */
+  GOMGPU_LAUNCH_TVD_SOLVE = 4,  /* k_tvd_solve, one thread per cell column */
   GOMGPU_LAUNCH_TVD_LIM = 5,    /* k_tvd_lim_cell, one thread per (level, cell) */
   GOMGPU_LAUNCH_NODE_VEL = 6,   /* k_node_vel */
   GOMGPU_LAUNCH_TURB_PREP = 7,  /* k_turb_ri, one thread per (layer, face) (closure schemes 1, 2) */
