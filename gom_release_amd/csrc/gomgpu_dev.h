@@ -42,6 +42,10 @@ struct DevMesh {
   // indirect ones.
   const double *cg_x, *cg_y;
   const int32_t *cg_node, *cg_adj;  // node ids and neighbour cells (1-based, 0 = none)
+  // Per cell slot (TB layout, like facenum_at_cell): sign_in_outflow of the slot's face when
+  // the transport counts its flux (boundary_type_of_face != -1 or a river face), else 0 --
+  // one coalesced load in place of the sign and two loads indexed by the face.
+  const int32_t *slot_flow;
 };
 
 struct DevState {

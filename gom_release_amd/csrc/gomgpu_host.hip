@@ -449,6 +449,27 @@ extern "C" int gomgpu_create(const gomgpu_mesh *mesh, const gomgpu_params *param
     c->m.cg_node = ip;
     if ((rc = dalloc(c, &ip, ca.size())) || (rc = upload(c, ip, ca.data(), Shape{1, (long)ca.size()}))) goto fail;
     c->m.cg_adj = ip;
+    // slot_flow: the transport's face test and outflow sign of each cell slot
+    if (!mesh->facenum_at_cell || !mesh->sign_in_outflow || !mesh->boundary_type_of_face || !mesh->isflowside3) {
+      rc = seterr(c, GOMGPU_E_ARG, "mesh connectivity missing");
+      goto fail;
+    }
+    std::vector<int32_t> sf(4 * E, 0);
+    for (long e = 0; e < E; e++) {
+      int N = mesh->tri_or_quad[e];
+      for (int l = 0; l < N && l < 4; l++) {
+        int j = mesh->facenum_at_cell[l + e * 4] - 1;
+        int sg = mesh->sign_in_outflow[l + e * 4];
+        if (j < 0 || j >= p.maxface || (sg != 1 && sg != -1)) {
+          rc = seterr(c, GOMGPU_E_ARG, "sign_in_outflow must be +1 or -1 on every face of every cell");
+          goto fail;
+        }
+        const bool active = mesh->boundary_type_of_face[j] != -1 || mesh->isflowside3[j] > 0;
+        sf[(size_t)l * E + e] = active ? sg : 0;
+      }
+    }
+    if ((rc = dalloc(c, &ip, sf.size())) || (rc = upload(c, ip, sf.data(), Shape{1, (long)sf.size()}))) goto fail;
+    c->m.slot_flow = ip;
   }
   // state
 #define AL_SR(name, ext)                                                             \
@@ -1543,6 +1564,7 @@ extern "C" long long gomgpu_footprint(const gomgpu_params *params) {
   }
   add(4 * (size_t)p.maxele, sizeof(double)); add(4 * (size_t)p.maxele, sizeof(double));  // ELM cell records
   add(4 * (size_t)p.maxele, sizeof(int32_t)); add(4 * (size_t)p.maxele, sizeof(int32_t));
+  add(4 * (size_t)p.maxele, sizeof(int32_t));  // slot_flow
   GOMGPU_STATE_REAL(FP_R)
   GOMGPU_STATE_INT(FP_I)
   GOMGPU_FORCING_REAL(FP_R)

@@ -183,9 +183,10 @@ __device__ __forceinline__ void lim_hsums(const DevMesh &m, const TvdW<W> &w, co
   sum2[0] = sum2[1] = 0.0;
   int N = m.tri_or_quad[e];
   for (int l = 1; l <= N; l++) {
-    int j = TB(m.facenum_at_cell, l, e, E) - 1, ii = TB(m.adj_cellnum_at_cell, l, e, E);
-    if (face_active(m, j)) {
-      double Q2 = AL(w.Qj, k, j, J) * (double)TB(m.sign_in_outflow, l, e, E);
+    const int sa = TB(m.slot_flow, l, e, E);
+    if (sa != 0) {
+      int j = TB(m.facenum_at_cell, l, e, E) - 1, ii = TB(m.adj_cellnum_at_cell, l, e, E);
+      double Q2 = AL(w.Qj, k, j, J) * (double)sa;
       if (Q2 < 0.0) {
         for (int t = 0; t < d.NT; t++) {
           double ci = con_across(m, w, d, e, ii, k, t);
@@ -368,9 +369,10 @@ __device__ __forceinline__ void tvd_rhs_layer(const DevMesh &m, const DevState &
   double ar = m.area[e];
   double sqi[2] = {0, 0}, sqii[2] = {0, 0}, shd[2] = {0, 0}, shl[2] = {0, 0};
   for (int l = 1; l <= N; l++) {
+    const int sa = TB(m.slot_flow, l, e, E);
+    if (sa == 0) continue;
     int j = TB(m.facenum_at_cell, l, e, E) - 1, ii = TB(m.adj_cellnum_at_cell, l, e, E);
-    if (!face_active(m, j)) continue;
-    double Q2 = AL(w.Qj, k, j, J) * (double)TB(m.sign_in_outflow, l, e, E);
+    double Q2 = AL(w.Qj, k, j, J) * (double)sa;
     double dfj = AL(w.dj, k, j, J);
     for (int t = 0; t < NT; t++) {
       double c0 = w.con[cix(t, k, e, L2, E)];
