@@ -109,9 +109,10 @@ struct DevWork {
   double *con, *con_new;  // con (0:L+1, E, NT) and con_new (L, E, NT), layer-major
   double *con2;           // the sub-cycles' second con buffer (con and con2 alternate)
   double *htop;           // (2, E, NT) horizontal upwind sums of each column's top layer
-  double *Qj, *dj, *phij; // (L, J)
+  double *fqd, *fps;      // (2, L, J) each: per face-layer (Q, d) and the PSI of both tracers
+  double *phij;           // (L, J)
   double *Qi, *di, *phii; // (0:L, E)
-  double *PSIj, *PSIi;
+  double *PSIi;
   double *s1a, *s2a;      // (L, E, NT) limiter sums per cell-layer
   double *obk, *qbk;      // (L, OB, NT), (L, QB, NT)
   double *node_new;       // (L, P, NT)

@@ -527,11 +527,10 @@ extern "C" int gomgpu_create(const gomgpu_mesh *mesh, const gomgpu_params *param
         (rc = dalloc(c, &w.face_sg, J)))
       goto fail;
     if ((rc = dalloc(c, &w.con, NT * (L + 2) * E)) || (rc = dalloc(c, &w.con_new, NT * L * E)) ||
-        (rc = dalloc(c, &w.Qj, L * J)) || (rc = dalloc(c, &w.dj, L * J)) || (rc = dalloc(c, &w.phij, L * J)) ||
+        (rc = dalloc(c, &w.fqd, 2 * L * J)) || (rc = dalloc(c, &w.fps, 2 * L * J)) || (rc = dalloc(c, &w.phij, L * J)) ||
         (rc = dalloc(c, &w.Qi, (L + 1) * E)) || (rc = dalloc(c, &w.di, (L + 1) * E)) ||
         (rc = dalloc(c, &w.phii, (L + 1) * E)) || (rc = dalloc(c, &w.htop, NT * 2 * E)) ||
         (rc = dalloc(c, &w.con2, NT * (L + 2) * E)) ||
-        (rc = dalloc(c, &w.PSIj, NT * L * J)) ||
         (rc = dalloc(c, &w.PSIi, NT * (L + 1) * E)) || (rc = dalloc(c, &w.s1a, NT * L * E)) ||
         (rc = dalloc(c, &w.s2a, NT * L * E)) || (rc = dalloc(c, &w.obk, NT * L * OB)) ||
         (rc = dalloc(c, &w.qbk, NT * L * QB)) || (rc = dalloc(c, &w.node_new, NT * L * P)) ||
@@ -1591,10 +1590,10 @@ extern "C" long long gomgpu_footprint(const gomgpu_params *params) {
   add(3, sizeof(int)); add(512, sizeof(unsigned));
   add(J, D); add(J, D); add(J, D);                          // face_sdz, face_sun, face_sg
   add(NT * (L + 2) * E, D); add(NT * L * E, D);             // con, con_new
-  add(L * J, D); add(L * J, D); add(L * J, D);              // Qj, dj, phij
+  add(2 * L * J, D); add(2 * L * J, D); add(L * J, D);      // fqd, fps, phij
   add((L + 1) * E, D); add((L + 1) * E, D); add((L + 1) * E, D);  // Qi, di, phii
   add(NT * 2 * E, D); add(NT * (L + 2) * E, D);             // htop, con2
-  add(NT * L * J, D); add(NT * (L + 1) * E, D);             // PSIj, PSIi
+  add(NT * (L + 1) * E, D);                                 // PSIi
   add(NT * L * E, D); add(NT * L * E, D);                   // s1a, s2a
   add(NT * L * OB, D); add(NT * L * QB, D);                 // obk, qbk
   add(NT * L * P, D); add(NT * L * J, D);                   // node_new, face_new

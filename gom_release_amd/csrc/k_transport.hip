@@ -9,18 +9,19 @@
 
 
 // The transport's work arrays, typed. W = double is the reference's FP64 storage. W = float
-// (GOMGPU_MIXED) stores in FP32 the horizontal limiter arrays (phi and PSI of every
-// face-layer and the per-cell upwind sums that form the face ratios) and the node/face means
-// of the output; every value is widened to FP64 when loaded and the arithmetic is FP64. The
-// concentrations, the fluxes and diffusion coefficients (horizontal and vertical), the
-// vertical limiter and the column solve stay FP64: a thin partial layer's update divides by
-// area * dz, which magnifies a rounding of its fluxes by the layer's Courant number, while
-// the limiter terms multiply concentration differences, not concentrations. The
-// allocations are the FP64 ones; an FP32 array uses their first half.
+// (GOMGPU_MIXED) stores in FP32 the face limiter's phi, the per-cell upwind sums that form
+// the face ratios and the node/face means of the output; every value is widened to FP64
+// when loaded and the arithmetic is FP64. The concentrations, the fluxes and diffusion
+// coefficients (horizontal and vertical), the limiters PSI and the column solve stay FP64:
+// a thin partial layer's update divides by area * dz, which magnifies a rounding of its
+// fluxes by the layer's Courant number. Per face-layer, Q and d are stored together (fqd) and
+// so are the PSI of the two tracers (fps): the right-hand side gathers each with one 16-byte
+// load.
+// The allocations are the FP64 ones; an FP32 array uses their first half.
 template <typename W>
 struct TvdW {
-  double *con, *con_new, *Qj, *dj, *Qi, *di, *phii, *htop, *PSIi, *obk, *qbk, *tri_gam, *tvd_x;
-  W *phij, *PSIj, *s1a, *s2a, *node_new, *face_new;
+  double *con, *con_new, *fqd, *fps, *Qi, *di, *phii, *htop, *PSIi, *obk, *qbk, *tri_gam, *tvd_x;
+  W *phij, *s1a, *s2a, *node_new, *face_new;
 };
 template <typename W>
 static TvdW<W> tvd_view(const DevWork &w) {
@@ -36,10 +37,9 @@ static TvdW<W> tvd_view(const DevWork &w) {
   v.qbk = w.qbk;
   v.tri_gam = w.tri_gam;
   v.tvd_x = w.tvd_x;
-  v.Qj = w.Qj;
-  v.dj = w.dj;
+  v.fqd = w.fqd;
+  v.fps = w.fps;
   v.phij = (W *)w.phij;
-  v.PSIj = (W *)w.PSIj;
   v.s1a = (W *)w.s1a;
   v.s2a = (W *)w.s2a;
   v.node_new = (W *)w.node_new;
@@ -52,6 +52,9 @@ __device__ __forceinline__ size_t cix(int t, int k, int e, int L2, int n) {  // 
 }
 __device__ __forceinline__ size_t nix(int t, int k, int e, int L, int n) {  // (L) fields, k 1-based
   return ((size_t)t * L + (k - 1)) * n + e;
+}
+__device__ __forceinline__ size_t frx(int k, int j, int J) {  // (Q, d) or (PSI 1, PSI 2) of layer k
+  return 2 * ((size_t)(k - 1) * J + j);
 }
 __device__ __forceinline__ size_t zix(int t, int k, int e, int L1, int n) {  // (0:L) fields
   return ((size_t)t * L1 + k) * n + e;
@@ -139,8 +142,7 @@ __global__ void k_tvd_coef_face(DevMesh m, DevState s, TvdW<W> w, DevDims d, Tvd
     dd = dmaxf(0.0, D - 0.5 * fabs(Q));
     phi = (Q == 0.0) ? 1.0 : dminf(1.0, 2.0 * D / fabs(Q));
   }
-  AL(w.Qj, k, j, J) = Q;
-  AL(w.dj, k, j, J) = dd;
+  *(double2 *)(w.fqd + frx(k, j, J)) = make_double2(Q, dd);
   AL(w.phij, k, j, J) = phi;
 }
 
@@ -186,7 +188,7 @@ __device__ __forceinline__ void lim_hsums(const DevMesh &m, const TvdW<W> &w, co
     const int sa = TB(m.slot_flow, l, e, E);
     if (sa != 0) {
       int j = TB(m.facenum_at_cell, l, e, E) - 1, ii = TB(m.adj_cellnum_at_cell, l, e, E);
-      double Q2 = AL(w.Qj, k, j, J) * (double)sa;
+      double Q2 = w.fqd[frx(k, j, J)] * (double)sa;
       if (Q2 < 0.0) {
         for (int t = 0; t < d.NT; t++) {
           double ci = con_across(m, w, d, e, ii, k, t);
@@ -329,12 +331,12 @@ __global__ void k_tvd_lim_face(DevMesh m, DevState s, TvdW<W> w, DevDims d, int 
   const bool in = (k >= m.bottom_layer_at_face[j] && k <= s.top_layer_at_face[j]);
   if (!in || hlim == 0) {  // PSI_jk is zero there; written once, in the first sub-cycle
     if (first)
-      for (int t = 0; t < d.NT; t++) w.PSIj[nix(t, k, j, L, J)] = 0.0;
+      *(double2 *)(w.fps + frx(k, j, J)) = make_double2(0.0, 0.0);
     return;
   }
   double r[2] = {1.0, 1.0};
   if (face_active(m, j)) {
-    double Q = AL(w.Qj, k, j, J);
+    double Q = w.fqd[frx(k, j, J)];
     for (int side = 1; side <= 2; side++) {
       int c = TB(m.adj_cellnum_at_face, side, j, J);
       if (c == 0) continue;
@@ -352,7 +354,9 @@ __global__ void k_tvd_lim_face(DevMesh m, DevState s, TvdW<W> w, DevDims d, int 
       }
     }
   }
-  for (int t = 0; t < d.NT; t++) w.PSIj[nix(t, k, j, L, J)] = limiter(hlim, r[t], AL(w.phij, k, j, J));
+  double psi[2] = {0.0, 0.0};
+  for (int t = 0; t < d.NT; t++) psi[t] = limiter(hlim, r[t], AL(w.phij, k, j, J));
+  *(double2 *)(w.fps + frx(k, j, J)) = make_double2(psi[0], psi[1]);
 }
 
 // Per-column implicit update (solve_transport_equation_v19.f90:921-1303) in two passes.
@@ -372,8 +376,10 @@ __device__ __forceinline__ void tvd_rhs_layer(const DevMesh &m, const DevState &
     const int sa = TB(m.slot_flow, l, e, E);
     if (sa == 0) continue;
     int j = TB(m.facenum_at_cell, l, e, E) - 1, ii = TB(m.adj_cellnum_at_cell, l, e, E);
-    double Q2 = AL(w.Qj, k, j, J) * (double)sa;
-    double dfj = AL(w.dj, k, j, J);
+    const double2 qd = *(const double2 *)(w.fqd + frx(k, j, J));   // Q, d
+    const double2 ps = *(const double2 *)(w.fps + frx(k, j, J));   // PSI of the two tracers
+    double Q2 = qd.x * (double)sa;
+    double dfj = qd.y;
     for (int t = 0; t < NT; t++) {
       double c0 = w.con[cix(t, k, e, L2, E)];
       double ci = con_across(m, w, d, e, ii, k, t);
@@ -382,7 +388,7 @@ __device__ __forceinline__ void tvd_rhs_layer(const DevMesh &m, const DevState &
       else if (Q2 < 0.0)
         sqii[t] = sqii[t] + fabs(Q2) * ci;
       shd[t] = shd[t] + dfj * (ci - c0);
-      shl[t] = shl[t] + w.PSIj[nix(t, k, j, L, J)] * fabs(Q2) * (c0 - ci);
+      shl[t] = shl[t] + (t == 0 ? ps.x : ps.y) * fabs(Q2) * (c0 - ci);
     }
   }
   double dz0 = AL(s.dz_cell, k, e, E);
