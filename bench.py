@@ -343,6 +343,8 @@ def main() -> int:
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="also run one device-to-device copy of MIB MiB (calibrates rocprofv3 "
                          "FETCH_SIZE/WRITE_SIZE against known traffic)")
+    ap.add_argument("--graph", action="store_true",
+                    help="submit each timed step as a captured hipGraph (a batch of this one scenario)")
     ap.add_argument("--counter-pass", type=int, default=0, metavar="STEPS",
                     help="instead of the benchmark, run STEPS steps (default launch shapes) and print "
                          "each step's PCG iteration count, for normalising rocprofv3 --pmc counters")
@@ -387,16 +389,34 @@ def main() -> int:
         launch = autotune.tuned(sc, log=lambda m: print(m, file=sys.stderr, flush=True))
         print(f"[bench] launch shapes tuned in {time.perf_counter() - t_tune:.1f} s: {launch}",
               file=sys.stderr, flush=True)
-    sc.run(args.warmup)
+    graph = None
+    if args.graph:
+        # the step submitted as a captured graph: a batch of this one scenario
+        graph = host.GpuBatch()
+        graph.add(model)
+
+    def run(n):
+        if graph is None:
+            sc.run(n)
+            return
+        for _ in range(n):
+            sc.stage_forcing()
+            graph.step()
+
+    run(args.warmup)
     model.sync()
     print(f"[bench] warmup done ({args.warmup} steps)", file=sys.stderr, flush=True)
     a0 = sc.active_cell_layers()
     model.elm_near_edge(reset=True)
     model.sync()
     t0 = time.perf_counter()
-    sc.run(args.steps)
+    run(args.steps)
+    if graph is not None:
+        graph.sync()
     model.sync()
     t1 = time.perf_counter()
+    if graph is not None:
+        graph.close()
     near = model.elm_near_edge()
     a1 = sc.active_cell_layers()
     elapsed = t1 - t0
