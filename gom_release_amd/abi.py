@@ -63,7 +63,17 @@ class ForcingOut(C.Structure):
 
 
 STAGES = ["friction", "elm", "momentum", "freesurf", "velocity", "layers", "nodevel",
-          "transport", "eos", "update", "turbulence", "wind"]
+          "transport", "eos", "update", "turbulence", "wind", "boundary"]
+
+
+class Boundary(C.Structure):
+    """gomgpu_boundary: the tables of the device boundary forcing (GOMGPU_STAGE_BOUNDARY)."""
+    _fields_ = [("num_harmonic_sets", C.c_int32), ("harmonic_first", PI), ("tidal_phase_shift", PD),
+                ("constituents", PD), ("num_series", C.c_int32), ("series_first", PI), ("series_time", PD),
+                ("series_value", PD), ("num_ob", C.c_int32), ("ob_eta_type", PI), ("ob_eta_id", PI),
+                ("ob_tracer_series", PI), ("num_qb", C.c_int32), ("qb_q_series", PI), ("qb_portion", PD),
+                ("qb_tracer_series", PI), ("is_salt", C.c_int32), ("is_temp", C.c_int32),
+                ("ref_salt", C.c_double), ("ref_temp", C.c_double), ("reference_diff_days", C.c_int32)]
 
 
 def extent(code: str, p: Params) -> int:

@@ -14,7 +14,7 @@ PKG = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(PKG, "csrc")
 OBJDIR = os.path.join(PKG, "_build")
 LIB = os.path.join(PKG, "libgomgpu.so")
-SOURCES = ["k_hydro.hip", "k_pcg.hip", "k_transport.hip", "k_turbulence.hip", "pcg_solve.hip", "pcg_noalloc.hip", "gomgpu_host.hip"]
+SOURCES = ["k_hydro.hip", "k_boundary.hip", "k_pcg.hip", "k_transport.hip", "k_turbulence.hip", "pcg_solve.hip", "pcg_noalloc.hip", "gomgpu_host.hip"]
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = os.environ.get("GOMGPU_ARCH", "gfx950")
 FLAGS = ["-O3", "-std=c++17", "-fPIC", f"--offload-arch={ARCH}", "-ffp-contract=off",

@@ -211,12 +211,116 @@ __host__ __device__ inline double gom_log(double x) {
   return r.hi + r.lo;
 }
 
+// ---- correctly rounded cosine (host and device)
+// The harmonic tide of compute_boundary_eta.f90:72-90 takes cos from the host C library,
+// which rounds correctly in all but vanishingly rare cases; the device library's cos does
+// not always. gom_cos reduces x = n pi/2 + r with pi/2 held in three doubles (the product
+// n P1 exact by an FMA split, x - n P1 exact by Sterbenz), so r carries ~2^-150 absolute
+// error for |n| < 2^30; then cos r or sin r (|r| <= pi/4) by their Taylor series to r^28
+// in double-double, Horner in r^2, rounded once.
+__host__ __device__ inline GomDD dd_two_prod(double a, double b) {
+  double p = a * b;
+  return {p, fma(a, b, -p)};
+}
+__host__ __device__ inline double gom_cos(double x) {
+  if (!(fabs(x) < 1.0e9)) return cos(x);  // inf, NaN and arguments far beyond a tide's
+  // 1/n! as double-double, n = 0..28 (the terms of both series)
+  const double ch[29] = {0x1p+0, 0x1p+0, 0x1p-1, 0x1.5555555555555p-3, 0x1.5555555555555p-5,
+                         0x1.1111111111111p-7, 0x1.6c16c16c16c17p-10, 0x1.a01a01a01a01ap-13,
+                         0x1.a01a01a01a01ap-16, 0x1.71de3a556c734p-19, 0x1.27e4fb7789f5cp-22,
+                         0x1.ae64567f544e4p-26, 0x1.1eed8eff8d898p-29, 0x1.6124613a86d09p-33,
+                         0x1.93974a8c07c9dp-37, 0x1.ae7f3e733b81fp-41, 0x1.ae7f3e733b81fp-45,
+                         0x1.952c77030ad4ap-49, 0x1.6827863b97d97p-53, 0x1.2f49b46814157p-57,
+                         0x1.e542ba4020225p-62, 0x1.71b8ef6dcf572p-66, 0x1.0ce396db7f853p-70,
+                         0x1.761b41316381ap-75, 0x1.f2cf01972f578p-80, 0x1.3f3ccdd165fa9p-84,
+                         0x1.88e85fc6a4e5ap-89, 0x1.d1ab1c2dccea3p-94, 0x1.0a18a2635085dp-98};
+  const double cl[29] = {0.0, 0.0, 0.0, 0x1.5555555555555p-57, 0x1.5555555555555p-59,
+                         0x1.1111111111111p-63, -0x1.f49f49f49f49fp-65, 0x1.a01a01a01a01ap-73,
+                         0x1.a01a01a01a01ap-76, -0x1.c154f8ddc6c00p-73, 0x1.cbbc05b4fa99ap-76,
+                         -0x1.c062e06d1f209p-80, -0x1.2aec959e14c06p-83, 0x1.f28e0cc748ebep-87,
+                         0x1.05d6f8a2efd1fp-92, 0x1.1d8656b0ee8cbp-97, 0x1.1d8656b0ee8cbp-101,
+                         0x1.ac981465ddc6cp-103, 0x1.eec01221a8b0bp-107, 0x1.2650f61dbdcb4p-112,
+                         0x1.ea72b4afe3c2fp-120, -0x1.d043ae40c4647p-120, -0x1.aebcdbd20331cp-124,
+                         -0x1.3423c7d91404fp-130, -0x1.9ada5fcc1ab14p-135, -0x1.58ddadf344487p-139,
+                         -0x1.71c37ebd16540p-143, 0x1.054d0c78aea14p-149, 0x1.b9e2e28e1aa54p-153};
+  const double P1 = 0x1.921fb54442d18p+0, P2 = 0x1.1a62633145c07p-54, P3 = -0x1.f1976b7ed8fbcp-110;
+  const double n = rint(x * 0x1.45f306dc9c883p-1);  // x 2/pi
+  GomDD p1 = dd_two_prod(n, P1), p2 = dd_two_prod(n, P2);
+  GomDD r = dd_two_sum(x - p1.hi, -p1.lo);
+  r = dd_add(r, GomDD{-p2.hi, -p2.lo});
+  r = dd_add(r, GomDD{-(n * P3), 0.0});
+  const GomDD z = dd_mul(r, r);
+  long long q = (long long)n % 4;
+  if (q < 0) q += 4;
+  // cos r = sum_{j<=14} (-1)^j z^j/(2j)!, sin r = r sum_{j<=13} (-1)^j z^j/(2j+1)!: o = 1
+  // (odd quadrants, sin r) takes the odd factorials; Horner from the top term j = 14 - o
+  const int o = (q & 1) ? 1 : 0;
+  GomDD s{(o ? -1.0 : 1.0) * ch[28 - o], (o ? -1.0 : 1.0) * cl[28 - o]};
+  for (int j = 13 - o; j >= 0; j--) {
+    const double sg = (j & 1) ? -1.0 : 1.0;
+    s = dd_add(dd_mul(s, z), GomDD{sg * ch[2 * j + o], sg * cl[2 * j + o]});
+  }
+  if (o) s = dd_mul(s, r);
+  const double v = s.hi + s.lo;
+  return (q == 1 || q == 2) ? -v : v;
+}
+
 __device__ __forceinline__ double dmaxf(double a, double b) { return a > b ? a : b; }
 __device__ __forceinline__ double dminf(double a, double b) { return a < b ? a : b; }
 
 __device__ __forceinline__ void latch(int *err, int code, int entity) {
   if (atomicCAS(err, 0, code) == 0) err[1] = entity;
 }
+
+// The block's share of the grid, XCD-aware. The dispatcher hands block b to XCD b % 8, so
+// consecutive blocks (neighbouring cells, faces and nodes of one layer plane) would land on
+// eight different L2 caches and each would fetch the stencil values the others share. Here
+// each group of 8 * GOM_XCD_RUN blocks is dealt out so that XCD x takes GOM_XCD_RUN
+// consecutive blocks of the group; groups stay in dispatch order, so all XCDs still sweep
+// the same part of the grid together (the work of a layer-major grid varies with the layer).
+// A bijection of [0, gridDim.x) onto itself: the blocks of a last partial group keep their
+// index. Only the placement changes.
+#define GOM_XCDS 8
+#ifndef GOM_XCD_RUN
+#define GOM_XCD_RUN 32
+#endif
+__device__ __forceinline__ unsigned gom_block() {
+  const unsigned b = blockIdx.x, g = GOM_XCDS * GOM_XCD_RUN;
+  if (b >= gridDim.x - gridDim.x % g) return b;
+  return b - b % g + (b % GOM_XCDS) * GOM_XCD_RUN + (b / GOM_XCDS) % GOM_XCD_RUN;
+}
+
+// Column tiles of a (level, entity) grid. Threads are numbered tile by tile, a tile being
+// GOM_TILE consecutive entities over all nlev levels (level-major inside the tile), so the
+// blocks that run together cover whole columns: a column's vertical neighbours and its
+// per-entity tables are read by blocks of one XCD (gom_block) within a short time, from L2,
+// instead of once per layer plane from memory. A wavefront is still 64 consecutive entities
+// of one level, so every field access stays coalesced. The grid holds tile_threads(n, nlev)
+// threads; gom_tile gives this thread's level k (0-based) and entity e, false past the end.
+#define GOM_TILE 64
+static inline long tile_threads(long n, int nlev) { return (n + GOM_TILE - 1) / GOM_TILE * GOM_TILE * nlev; }
+__device__ __forceinline__ bool gom_tile(int n, int nlev, int &k, int &e) {
+  const unsigned q = gom_block() * blockDim.x + threadIdx.x;
+  const unsigned tsz = GOM_TILE * (unsigned)nlev;
+  const unsigned t = q / tsz, i = q - t * tsz;
+  e = (int)(t * GOM_TILE + i % GOM_TILE);
+  k = (int)(i / GOM_TILE);
+  return e < n;
+}
+
+// Device copy of the boundary tables (gomgpu_boundary) and the stage's work arrays.
+struct DevBnd {
+  int nh, ns, nob, nqb;
+  const int *h_first, *s_first, *s_mono;   // s_mono[s]: the series' times never decrease
+  const double *h_shift, *h_con, *s_time, *s_val;
+  const int *ob_type, *ob_id, *ob_ser, *qb_qser, *qb_ser;
+  const double *qb_portion;
+  double *h_old, *h_new;  // each set's sum at elapsed - dt and elapsed
+  double *job_val;        // each series at each time slot of the step
+  int *job_found;
+  double shift, ref_salt, ref_temp;  // shift: reference_diff_days * 86400 s
+  int is_salt, is_temp, any_eta_series, tracers;
+};
 
 // Kernel argument blocks shared by the launchers and the kernels.
 struct MomParams {

@@ -95,16 +95,20 @@ extern "C" int gom_pcg_cg1m_capacity(int device);
 extern "C" hipError_t gom_launch_pcg_mixed(const PcgArgs *a64, const PcgArgs32 *m32, int cooperative,
                                            hipStream_t st);
 extern "C" int gom_pcg_cg1m32_capacity(int device);
+extern "C" hipError_t gom_launch_boundary(const DevBnd *b, const DevForcing *f, const DevDims *d, double dt, int Nt,
+                                          int elm, hipStream_t st);
 
 enum { KT_PCG = 0, KT_ELM = 1, KT_MOMENTUM = 2, KT_TVD_COLUMN = 3 };
 static const char *kKernelNames[GOMGPU_NUM_KTIMERS] = {"k_pcg", "k_elm", "k_momentum", "k_tvd_column"};
 static const char *kStageNames[GOMGPU_NUM_STAGES] = {"friction", "elm",       "momentum", "freesurf",
                                                       "velocity", "layers",    "nodevel",  "transport",
-                                                      "eos",      "update",    "turbulence", "wind"};
+                                                      "eos",      "update",    "turbulence", "wind",
+                                                      "boundary"};
 // Order of the stages in one step (main.f90:95-272, the closure before the transport; the
-// wind stage of main.f90:113-137 first, no-op without hurricane fields).
+// boundary forcing and the wind stage of main.f90:84-161 first, each a no-op unless its
+// tables or hurricane fields are set).
 static const int kStepOrder[GOMGPU_NUM_STAGES] = {
-    GOMGPU_STAGE_WIND,
+    GOMGPU_STAGE_BOUNDARY,       GOMGPU_STAGE_WIND,
     GOMGPU_STAGE_FRICTION, GOMGPU_STAGE_ELM,        GOMGPU_STAGE_MOMENTUM,  GOMGPU_STAGE_FREESURF,
     GOMGPU_STAGE_VELOCITY, GOMGPU_STAGE_LAYERS,     GOMGPU_STAGE_NODEVEL,   GOMGPU_STAGE_TURBULENCE,
     GOMGPU_STAGE_TRANSPORT, GOMGPU_STAGE_EOS,       GOMGPU_STAGE_UPDATE};
@@ -157,6 +161,11 @@ struct gomgpu_ctx {
   double *d_stage = nullptr;  // its device image: one copy, then k_scatter to the fields
   size_t stage_len = 0;
   hipEvent_t fev = nullptr;   // completion of the last forcing upload
+  // boundary forcing on the device (gomgpu_set_boundary): its tables and the sub-cycle
+  // extent of con_at_ob / con_at_Qb
+  bool bnd_on = false;
+  DevBnd bnd{};
+  int bnd_nt = 1;
   int32_t *dep_buf = nullptr; // departure cells and levels of the momentum ELM (allocated on
                               // the first gomgpu_elm_record; w.elm_dep points here while on)
 };
@@ -719,7 +728,7 @@ extern "C" int gomgpu_set_forcing(gomgpu_ctx *c, const gomgpu_forcing *f) {
   c->h_stage[off + 3] = 0.0;
   c->h_stage[off + 4] = f->spinup_function_wind;
   c->h_stage[off + 5] = f->hurricane_phase;
-  c->h_stage[off + 6] = 0.0;
+  c->h_stage[off + 6] = f->julian_day;
   c->h_stage[off + 7] = 0.0;
   seg.dst[seg.count] = (double *)c->f.scal;
   seg.off[seg.count] = (long)off;
@@ -733,6 +742,99 @@ extern "C" int gomgpu_set_forcing(gomgpu_ctx *c, const gomgpu_forcing *f) {
   long bx = (nmax + 255) / 256;
   k_scatter<<<dim3((unsigned)(bx < 1024 ? bx : 1024), (unsigned)seg.count), 256, 0, c->st>>>(seg, c->d_stage);
   HIPCHK(c, hipGetLastError());
+  return 0;
+}
+
+// Boundary tables (gomgpu_boundary): checked, uploaded once; GOMGPU_STAGE_BOUNDARY then
+// computes each step's boundary forcing on the device.
+template <typename T>
+static int upload_vec(gomgpu_ctx *c, const T **dst, const T *src, size_t n) {
+  T *d = nullptr;
+  if (int rc = dalloc(c, &d, n)) return rc;
+  if (n > 0) HIPCHK(c, hipMemcpyAsync(d, src, n * sizeof(T), hipMemcpyHostToDevice, c->st));
+  *dst = d;
+  return 0;
+}
+
+extern "C" int gomgpu_set_boundary(gomgpu_ctx *c, const gomgpu_boundary *t) {
+  if (!c || !t) return GOMGPU_E_ARG;
+  const gomgpu_params &p = c->p;
+  if (c->bnd_on) return seterr(c, GOMGPU_E_ARG, "gomgpu_set_boundary: the tables are already set");
+  if (!c->batches.empty()) return seterr(c, GOMGPU_E_ARG, "gomgpu_set_boundary: the context is in a batch");
+  if (t->num_ob != p.num_ob_cell || t->num_qb != p.num_Qb_cell)
+    return seterr(c, GOMGPU_E_ARG, "gomgpu_set_boundary: num_ob / num_qb differ from num_ob_cell / num_Qb_cell");
+  const int nh = t->num_harmonic_sets, ns = t->num_series, nob = t->num_ob, nqb = t->num_qb;
+  if (nh < 0 || ns < 0) return seterr(c, GOMGPU_E_ARG, "gomgpu_set_boundary: negative table size");
+  if (nh > 0 && (!t->harmonic_first || !t->tidal_phase_shift || t->harmonic_first[0] != 0))
+    return seterr(c, GOMGPU_E_ARG, "gomgpu_set_boundary: harmonic tables missing");
+  for (int h = 0; h < nh; h++)
+    if (t->harmonic_first[h + 1] < t->harmonic_first[h])
+      return seterr(c, GOMGPU_E_ARG, "gomgpu_set_boundary: harmonic_first must not decrease");
+  const int ncon = nh > 0 ? t->harmonic_first[nh] : 0;
+  if (ncon > 0 && !t->constituents) return seterr(c, GOMGPU_E_ARG, "gomgpu_set_boundary: constituents missing");
+  if (ns > 0 && (!t->series_first || t->series_first[0] != 0))
+    return seterr(c, GOMGPU_E_ARG, "gomgpu_set_boundary: series tables missing");
+  for (int s = 0; s < ns; s++)
+    if (t->series_first[s + 1] < t->series_first[s])
+      return seterr(c, GOMGPU_E_ARG, "gomgpu_set_boundary: series_first must not decrease");
+  const int npt = ns > 0 ? t->series_first[ns] : 0;
+  if (npt > 0 && (!t->series_time || !t->series_value))
+    return seterr(c, GOMGPU_E_ARG, "gomgpu_set_boundary: series points missing");
+  const bool tracers = p.transport_flag == 1 && p.maxtran2 > 0;
+  int any_eta_series = 0;
+  for (int i = 0; i < nob; i++) {
+    const int ty = t->ob_eta_type[i], id = t->ob_eta_id[i];
+    if (ty == 1 ? (id < 1 || id > nh) : ty == 2 ? (id < 1 || id > ns) : true)
+      return seterr(c, GOMGPU_E_ARG, "gomgpu_set_boundary: ob_eta_type must be 1 (harmonic set) or 2 (series) "
+                                     "with an id in range");
+    any_eta_series |= ty == 2;
+    for (int k = 0; k < 2 && tracers; k++)
+      if (t->ob_tracer_series[2 * i + k] < 0 || t->ob_tracer_series[2 * i + k] > ns)
+        return seterr(c, GOMGPU_E_ARG, "gomgpu_set_boundary: ob_tracer_series out of range");
+  }
+  for (int i = 0; i < nqb; i++) {
+    if (t->qb_q_series[i] < 0 || t->qb_q_series[i] > ns)
+      return seterr(c, GOMGPU_E_ARG, "gomgpu_set_boundary: qb_q_series out of range");
+    for (int k = 0; k < 2 && tracers; k++)
+      if (t->qb_tracer_series[2 * i + k] < 0 || t->qb_tracer_series[2 * i + k] > ns)
+        return seterr(c, GOMGPU_E_ARG, "gomgpu_set_boundary: qb_tracer_series out of range");
+  }
+  std::vector<int> mono(ns > 0 ? ns : 1, 1);
+  for (int s = 0; s < ns; s++)
+    for (int k = t->series_first[s] + 1; k < t->series_first[s + 1]; k++)
+      if (!(t->series_time[k] >= t->series_time[k - 1])) mono[s] = 0;
+  const int Nt = p.transport_solver == 2 ? 1 : (p.trans_sub_iter > 1 ? p.trans_sub_iter : 1);
+  DevBnd &b = c->bnd;
+  b.nh = nh;
+  b.ns = ns;
+  b.nob = nob;
+  b.nqb = nqb;
+  int rc;
+  std::vector<int> zeros(2 * (size_t)(nob > nqb ? nob : nqb) + 1, 0);
+  if ((rc = upload_vec(c, &b.h_first, t->harmonic_first, nh > 0 ? nh + 1 : 0)) ||
+      (rc = upload_vec(c, &b.h_shift, t->tidal_phase_shift, nh)) ||
+      (rc = upload_vec(c, &b.h_con, t->constituents, 5 * (size_t)ncon)) ||
+      (rc = upload_vec(c, &b.s_first, t->series_first, ns > 0 ? ns + 1 : 0)) ||
+      (rc = upload_vec(c, &b.s_mono, mono.data(), ns)) ||
+      (rc = upload_vec(c, &b.s_time, t->series_time, npt)) || (rc = upload_vec(c, &b.s_val, t->series_value, npt)) ||
+      (rc = upload_vec(c, &b.ob_type, t->ob_eta_type, nob)) || (rc = upload_vec(c, &b.ob_id, t->ob_eta_id, nob)) ||
+      (rc = upload_vec(c, &b.ob_ser, tracers ? t->ob_tracer_series : zeros.data(), 2 * (size_t)nob)) ||
+      (rc = upload_vec(c, &b.qb_qser, t->qb_q_series, nqb)) || (rc = upload_vec(c, &b.qb_portion, t->qb_portion, nqb)) ||
+      (rc = upload_vec(c, &b.qb_ser, tracers ? t->qb_tracer_series : zeros.data(), 2 * (size_t)nqb)))
+    return rc;
+  if ((rc = dalloc(c, &b.h_old, nh)) || (rc = dalloc(c, &b.h_new, nh)) ||
+      (rc = dalloc(c, &b.job_val, (size_t)ns * (2 + Nt))) || (rc = dalloc(c, &b.job_found, (size_t)ns * (2 + Nt))))
+    return rc;
+  HIPCHK(c, hipStreamSynchronize(c->st));
+  b.shift = t->reference_diff_days * 86400.0;
+  b.ref_salt = t->ref_salt;
+  b.ref_temp = t->ref_temp;
+  b.is_salt = t->is_salt;
+  b.is_temp = t->is_temp;
+  b.any_eta_series = any_eta_series;
+  b.tracers = tracers ? 1 : 0;
+  c->bnd_nt = Nt;
+  c->bnd_on = true;
   return 0;
 }
 
@@ -907,7 +1009,7 @@ static int stage_impl(gomgpu_ctx *c, int id) {
           k_elm_substeps<<<nblk((long)d.P * d.L, BS), BS, 0, st>>>(c->m, c->s, d, p.dt, p.delta_z_min,
                                                                    p.ELM_min_iter, p.ELM_max_iter);
         KT_BEGIN(KT_ELM);
-        k_elm<<<nblk((long)d.J * d.L, LB(ELM)), LB(ELM), 0, st>>>(c->m, c->s, d, p.MSL, p.dt, p.adv_onoff_depth, err,
+        k_elm<<<nblk(tile_threads(d.J, d.L), LB(ELM)), LB(ELM), 0, st>>>(c->m, c->s, d, p.MSL, p.dt, p.adv_onoff_depth, err,
                                                                   c->w.elm_dep, c->w.elm_near);
         KT_END(KT_ELM);
       }
@@ -957,7 +1059,7 @@ static int stage_impl(gomgpu_ctx *c, int id) {
         const int copy_salt = (c->stale & SWAP_SALT) != 0, copy_temp = (c->stale & SWAP_TEMP) != 0;
         const int Nt = p.trans_sub_iter > 0 ? p.trans_sub_iter : 1;
         KT_BEGIN(KT_TVD_COLUMN);
-        k_elm_tracer<<<nblk((long)d.E * d.L, 256), 256, 0, st>>>(c->m, c->s, c->f, c->w, d, p.MSL, p.dt, p.rho_o, Nt,
+        k_elm_tracer<<<nblk(tile_threads(d.E, d.L), 256), 256, 0, st>>>(c->m, c->s, c->f, c->w, d, p.MSL, p.dt, p.rho_o, Nt,
                                                                  err);
         k_elm_column<<<nblk(d.E, 64), 64, 0, st>>>(c->m, c->s, c->w, d, p.dt, copy_salt, copy_temp, err);
         KT_END(KT_TVD_COLUMN);
@@ -1067,6 +1169,13 @@ static int stage_impl(gomgpu_ctx *c, int id) {
       c->dz_full = false;
       c->layers_since_update = 0;
       k_bars<<<nblk(d.P, BS), BS, 0, st>>>(c->m, c->s, d);
+      break;
+    }
+    case GOMGPU_STAGE_BOUNDARY: {
+      // compute_boundary_eta, calculate_Q and the transport's boundary concentrations
+      // (k_boundary.hip), from the tables of gomgpu_set_boundary
+      if (!c->bnd_on) break;
+      HIPCHK(c, gom_launch_boundary(&c->bnd, &c->f, &d, p.dt, c->bnd_nt, p.transport_solver == 2, st));
       break;
     }
     case GOMGPU_STAGE_WIND: {
@@ -1335,7 +1444,8 @@ static int batch_plan(gomgpu_batch *b) {
   return 0;
 }
 
-static const int kPreStages[] = {GOMGPU_STAGE_WIND, GOMGPU_STAGE_FRICTION, GOMGPU_STAGE_ELM, GOMGPU_STAGE_MOMENTUM};
+static const int kPreStages[] = {GOMGPU_STAGE_BOUNDARY, GOMGPU_STAGE_WIND, GOMGPU_STAGE_FRICTION, GOMGPU_STAGE_ELM,
+                                 GOMGPU_STAGE_MOMENTUM};
 static const int kPostStages[] = {GOMGPU_STAGE_VELOCITY,   GOMGPU_STAGE_LAYERS,    GOMGPU_STAGE_NODEVEL,
                                   GOMGPU_STAGE_TURBULENCE, GOMGPU_STAGE_TRANSPORT, GOMGPU_STAGE_EOS,
                                   GOMGPU_STAGE_UPDATE};

@@ -12,7 +12,7 @@
 // calculate_bottom_friction.f90:21-62. bf_flag 1 uses the Manning drag of the still-water
 // face depth, computed once on the host with the same libm pow as the reference.
 __global__ void k_friction(DevMesh m, DevState s, DevDims d, int bf_flag, double dry, double vk) {
-  int j = blockIdx.x * blockDim.x + threadIdx.x;
+  int j = gom_block() * blockDim.x + threadIdx.x;
   if (j >= d.J) return;
   const int J = d.J;
   double cdb = 0.0, gb = 0.0;
@@ -36,12 +36,16 @@ __global__ void k_friction(DevMesh m, DevState s, DevDims d, int bf_flag, double
 extern "C" void gom_log_array(const double *x, double *y, long n) {
   for (long i = 0; i < n; i++) y[i] = gom_log(x[i]);
 }
+// gom_cos over a host array (the host build of the device cosine, for its tests)
+extern "C" void gom_cos_array(const double *x, double *y, long n) {
+  for (long i = 0; i < n; i++) y[i] = gom_cos(x[i]);
+}
 
 // ================================================================ ELM
 // Sub-step counts per node-layer: solve_nonlinear_advection.f90:52-87
 __global__ void k_elm_substeps(DevMesh m, DevState s, DevDims d, double dt, double dzmin,
                                int imin, int imax) {
-  int t = blockIdx.x * blockDim.x + threadIdx.x;
+  int t = gom_block() * blockDim.x + threadIdx.x;
   if (t >= d.P * d.L) return;
   const int P = d.P, E = d.E;
   int i = t % P, k = t / P + 1;
@@ -447,10 +451,11 @@ __device__ __forceinline__ int backtrace(const ElmCtx &c, int bt_step, double bt
 // (0 where nothing is backtraced).
 __global__ void __launch_bounds__(256) GOM_WAVES(GOM_WAVES_ELM) k_elm(DevMesh m, DevState s, DevDims d, double MSL, double dt, double onoff,
                                              int *err, int32_t *dep, unsigned long long *near_count) {
-  int t = blockIdx.x * blockDim.x + threadIdx.x;
   const int J = d.J, E = d.E, P = d.P;
-  if (t >= J * d.L) return;
-  int j = t % J, k = t / J + 1;
+  int j, k;
+  if (!gom_tile(J, d.L, k, j)) return;
+  const size_t t = (size_t)k * J + j;  // face-layer (k+1, j)
+  k += 1;
   int kb = m.bottom_layer_at_face[j], kt = s.top_layer_at_face[j];
   double un = AL(s.un_face, k, j, J), vn = AL(s.vn_face, k, j, J);
   if (dep) {
@@ -513,7 +518,7 @@ __global__ void __launch_bounds__(256) GOM_WAVES(GOM_WAVES_ELM) k_elm(DevMesh m,
 
 // main.f90:171-176 (advection off)
 __global__ void k_elm_copy(DevState s, long n) {
-  long t = blockIdx.x * (long)blockDim.x + threadIdx.x;
+  long t = gom_block() * (long)blockDim.x + threadIdx.x;
   if (t >= n) return;
   s.un_ELM[t] = s.un_face[t];
   s.vn_ELM[t] = s.vn_face[t];
@@ -716,10 +721,10 @@ __device__ __forceinline__ int elm_tracer_foot(const DevMesh &m, const DevState 
 // salt differences enter the temperature row and the temperature differences the salt row.
 __global__ void __launch_bounds__(256) k_elm_tracer(DevMesh m, DevState s, DevForcing f, DevWork w, DevDims d,
                                                     double MSL, double dt, double rho_o, int Nt, int *err) {
-  const long q = (long)blockIdx.x * blockDim.x + threadIdx.x;
   const int E = d.E, J = d.J, P = d.P, L = d.L;
-  if (q >= (long)E * L) return;
-  const int i = (int)(q % E), k = (int)(q / E) + 1;
+  int i, k;
+  if (!gom_tile(E, L, k, i)) return;
+  k += 1;
   const int tl = s.top_layer_at_element[i], bl = m.bottom_layer_at_element[i];
   if (tl == 0 || k < bl || k > tl) return;
   const int N = m.tri_or_quad[i];
@@ -811,7 +816,7 @@ __device__ __forceinline__ void elm_row(const DevState &s, int i, int E, int tl,
 // the old one, which such a column then receives.
 __global__ void k_elm_column(DevMesh m, DevState s, DevWork w, DevDims d, double dt, int copy_salt, int copy_temp,
                              int *err) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  const int i = gom_block() * blockDim.x + threadIdx.x;
   const int E = d.E, L = d.L;
   if (i >= E) return;
   const int tl = s.top_layer_at_element[i], bl = m.bottom_layer_at_element[i];
@@ -880,7 +885,7 @@ __global__ void k_elm_column(DevMesh m, DevState s, DevWork w, DevDims d, double
 // top level); then the values extended below the bottom and above the top. Dry nodes keep
 // their values.
 __global__ void k_elm_node(DevMesh m, DevState s, DevDims d) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  const int i = gom_block() * blockDim.x + threadIdx.x;
   const int P = d.P, E = d.E, L = d.L;
   if (i >= P) return;
   const int tn = s.top_layer_at_node[i], bn = m.bottom_layer_at_node[i];
@@ -924,7 +929,7 @@ __global__ void k_elm_node(DevMesh m, DevState s, DevDims d) {
 
 // Face values: the mean of the face's two nodes at every layer (:325-335).
 __global__ void k_elm_face(DevMesh m, DevState s, DevDims d) {
-  const long q = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  const long q = (long)gom_block() * blockDim.x + threadIdx.x;
   const int J = d.J, P = d.P;
   if (q >= (long)J * d.L) return;
   const int j = (int)(q % J), k = (int)(q / J) + 1;
@@ -948,7 +953,7 @@ __global__ void k_cell_uv(DevMesh m, DevState s, DevWork w, DevDims d) {
   // each node level serves the two layers it bounds
   const int E = d.E, P = d.P, L = d.L;
   const int nch = (L + GOM_CUV_LAYERS - 1) / GOM_CUV_LAYERS;
-  long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  long t = (long)gom_block() * blockDim.x + threadIdx.x;
   if (t >= (long)nch * E) return;
   const int ie = (int)(t % E), k1 = (int)(t / E) * GOM_CUV_LAYERS + 1;
   const int N = m.tri_or_quad[ie];
@@ -994,7 +999,7 @@ __global__ void __launch_bounds__(256) GOM_WAVES(GOM_WAVES_MOM_RHS) k_mom_rhs(De
   extern __shared__ double bct[];  // [2][L][64] baroclinic products (cells, nodes)
   const int J = d.J, E = d.E, P = d.P, L = d.L;
   const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6, nw = blockDim.x >> 6;
-  const int j = blockIdx.x * 64 + lane;
+  const int j = gom_block() * 64 + lane;
   const bool valid = j < J;
   const int kt = valid ? s.top_layer_at_face[j] : 0;
   const int kb = valid ? m.bottom_layer_at_face[j] : 1;
@@ -1175,7 +1180,7 @@ __device__ __forceinline__ void mom_row(const DevState &s, const double *gamma_t
 
 #define MS_CHUNK 4
 __global__ void GOM_WAVES(GOM_WAVES_MOM_SOLVE) k_mom_solve(DevMesh m, DevState s, DevForcing f, DevWork w, DevDims d, MomParams q, int *err) {
-  int j = blockIdx.x * blockDim.x + threadIdx.x;
+  int j = gom_block() * blockDim.x + threadIdx.x;
   const int J = d.J, L = d.L;
   if (j >= J) return;
   int kt = s.top_layer_at_face[j];
@@ -1294,7 +1299,7 @@ __global__ void GOM_WAVES(GOM_WAVES_MOM_SOLVE) k_mom_solve(DevMesh m, DevState s
 // ================================================================ free surface
 // Per-face column sums of solve_free_surface_equation.f90:79-86, 140-146 (top down).
 __global__ void k_fs_face(DevMesh m, DevState s, DevWork w, DevDims d, double gdt2th2) {
-  int j = blockIdx.x * blockDim.x + threadIdx.x;
+  int j = gom_block() * blockDim.x + threadIdx.x;
   const int J = d.J;
   if (j >= J) return;
   int kt = s.top_layer_at_face[j];
@@ -1320,7 +1325,7 @@ __global__ void k_fs_face(DevMesh m, DevState s, DevWork w, DevDims d, double gd
 // reference CSR); the initial guess is eta_cell_new (:241).
 __global__ void k_fs_cell(DevMesh m, DevState s, DevForcing f, DevWork w, DevDims d, double dt,
                           double th, int *err) {
-  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  int i = gom_block() * blockDim.x + threadIdx.x;
   const int E = d.E;
   if (i >= E) return;
   int N = m.tri_or_quad[i];
@@ -1359,7 +1364,7 @@ __global__ void k_fs_cell(DevMesh m, DevState s, DevForcing f, DevWork w, DevDim
 
 // Area-weighted node elevation and wet/dry flag: solve_free_surface_equation.f90:353-374
 __global__ void k_eta_node(DevMesh m, DevState s, DevDims d) {
-  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  int i = gom_block() * blockDim.x + threadIdx.x;
   const int P = d.P;
   if (i >= P) return;
   int wet = 0;
@@ -1380,7 +1385,7 @@ __global__ void k_eta_node(DevMesh m, DevState s, DevDims d) {
 
 // Dry-node fill from wet neighbours: solve_free_surface_equation.f90:378-394
 __global__ void k_eta_node_dry(DevMesh m, DevState s, DevDims d) {
-  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  int i = gom_block() * blockDim.x + threadIdx.x;
   const int P = d.P;
   if (i >= P) return;
   if (s.wetdry_node[i] != 1) return;
@@ -1403,7 +1408,7 @@ __global__ void k_eta_node_dry(DevMesh m, DevState s, DevDims d) {
 // update_variables equals vn_face; they are written back from vn_face so the arrays can be
 // double-buffered.
 __global__ void k_velocity_face(DevMesh m, DevState s, DevForcing f, DevDims d, double thgdt) {
-  int j = blockIdx.x * blockDim.x + threadIdx.x;
+  int j = gom_block() * blockDim.x + threadIdx.x;
   const int J = d.J, L = d.L;
   if (j >= J) return;
   int kt = s.top_layer_at_face[j];
@@ -1470,7 +1475,7 @@ __global__ void k_velocity_face(DevMesh m, DevState s, DevForcing f, DevDims d, 
 
 // Vertical velocity per cell column: solve_velocities.f90:190-258
 __global__ void k_velocity_w(DevMesh m, DevState s, DevDims d) {
-  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  int i = gom_block() * blockDim.x + threadIdx.x;
   const int E = d.E, J = d.J, L = d.L;
   if (i >= E) return;
   int t = s.top_layer_at_element[i], b = m.bottom_layer_at_element[i];
@@ -1504,7 +1509,7 @@ __global__ void k_velocity_w(DevMesh m, DevState s, DevDims d) {
 // ================================================================ layers
 // update_vertical_layers_1.f90:40-133 (cells)
 __global__ void k_layers_cell(DevMesh m, DevState s, DevDims d, double MSL, double dry, int *err) {
-  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  int i = gom_block() * blockDim.x + threadIdx.x;
   const int E = d.E, L = d.L;
   if (i >= E) return;
   int tnew = 0;
@@ -1543,7 +1548,7 @@ __global__ void k_layers_cell(DevMesh m, DevState s, DevDims d, double MSL, doub
 
 // update_vertical_layers_1.f90:138-198 (faces)
 __global__ void k_layers_face(DevMesh m, DevState s, DevDims d, double MSL, double dry, int *err) {
-  int j = blockIdx.x * blockDim.x + threadIdx.x;
+  int j = gom_block() * blockDim.x + threadIdx.x;
   const int J = d.J, L = d.L;
   if (j >= J) return;
   int tnew = 0;
@@ -1589,7 +1594,7 @@ __global__ void k_layers_face(DevMesh m, DevState s, DevDims d, double MSL, doub
 // threads of a wave.
 __global__ void k_update_dz(double *dz, const double *dz_new, double *dzh, const double *dzh_new,
                             const int32_t *top, const int32_t *bot, int n, int L) {
-  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  int i = gom_block() * blockDim.x + threadIdx.x;
   if (i >= n) return;
   int tp = top[i];
   if (tp == 0) return;
@@ -1613,7 +1618,7 @@ __global__ void k_update_dz(double *dz, const double *dz_new, double *dzh, const
 }
 
 __global__ void k_layers_node(DevMesh m, DevState s, DevDims d, double MSL, double dry, int *err) {
-  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  int i = gom_block() * blockDim.x + threadIdx.x;
   const int P = d.P, L = d.L;
   if (i >= P) return;
   int tnew = 0;
@@ -1648,7 +1653,7 @@ __global__ void k_layers_node(DevMesh m, DevState s, DevDims d, double MSL, doub
 
 // Element dryness from its nodes and the dz_half checks: update_vertical_layers_1.f90:260-330
 __global__ void k_layers_check_cell(DevMesh m, DevState s, DevDims d, double dmin, int *err) {
-  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  int i = gom_block() * blockDim.x + threadIdx.x;
   const int E = d.E;
   if (i >= E) return;
   int N = m.tri_or_quad[i], nsum1 = 0, nsum2 = 0;
@@ -1665,7 +1670,7 @@ __global__ void k_layers_check_cell(DevMesh m, DevState s, DevDims d, double dmi
 }
 
 __global__ void k_layers_check_fn(DevMesh m, DevState s, DevDims d, double dmin, int *err) {
-  int t = blockIdx.x * blockDim.x + threadIdx.x;
+  int t = gom_block() * blockDim.x + threadIdx.x;
   const int J = d.J, P = d.P;
   if (t < J) {
     for (int k = m.bottom_layer_at_face[t]; k <= s.top_layer_at_face[t] - 1; k++)
@@ -1680,7 +1685,7 @@ __global__ void k_layers_check_fn(DevMesh m, DevState s, DevDims d, double dmin,
 // ================================================================ node velocity
 // Face level velocities: calculate_velocity_at_node.f90:31-60
 __global__ void k_face_levels(DevMesh m, DevState s, DevDims d) {
-  int j = blockIdx.x * blockDim.x + threadIdx.x;
+  int j = gom_block() * blockDim.x + threadIdx.x;
   const int J = d.J;
   if (j >= J) return;
   int kt = s.top_layer_at_face[j];
@@ -1715,7 +1720,7 @@ __global__ void GOM_WAVES(GOM_WAVES_NODE_VEL) k_node_vel(DevMesh m, DevState s, 
   // loaded once per chunk of adjacent cells and serve every level of the thread
   const int P = d.P, E = d.E, J = d.J, L1 = d.L + 1;
   const int nlc = (L1 + NV_LEVELS - 1) / NV_LEVELS;
-  int t = blockIdx.x * blockDim.x + threadIdx.x;
+  int t = gom_block() * blockDim.x + threadIdx.x;
   if (t >= P * nlc) return;
   const int i = t % P, k0 = (t / P) * NV_LEVELS;
   const int nlev = L1 - k0 < NV_LEVELS ? L1 - k0 : NV_LEVELS;
@@ -1809,7 +1814,7 @@ __global__ void GOM_WAVES(GOM_WAVES_NODE_VEL) k_node_vel(DevMesh m, DevState s, 
 // River face velocity from the discharge (compute_boundary_velocity.f90:32-46):
 // Qu = -Q_add / (face_length * (h_face + mean eta of the face nodes)).
 __global__ void k_river_velocity(DevMesh m, DevState s, DevForcing f, DevDims d, double *qu) {
-  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  int i = gom_block() * blockDim.x + threadIdx.x;
   const int J = d.J;
   if (i >= d.NQB) return;
   // the river face is the face of the river cell flagged with this river id
@@ -1833,7 +1838,7 @@ __global__ void k_river_velocity(DevMesh m, DevState s, DevForcing f, DevDims d,
 // interpolated field (read_hurricane_ser_1.f90:48-60), so the field is the running sum the
 // reference accumulates; phase 0 (outside the hurricane window) leaves everything.
 __global__ void k_hurricane_nodes(DevForcing f, int P, double ninterval) {
-  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  int i = gom_block() * blockDim.x + threadIdx.x;
   if (i >= P) return;
   const double phase = f.scal[5];
   double *u0 = const_cast<double *>(f.wind_u0), *v0 = const_cast<double *>(f.wind_v0);
@@ -1865,7 +1870,7 @@ __global__ void k_hurricane_nodes(DevForcing f, int P, double ninterval) {
 // chosen formula capped at 0.003 (a default-real literal in the reference), stress along
 // the face normal and tangent, times the wind spin-up factor.
 __global__ void k_wind_faces(DevMesh m, DevForcing f, int J, double rho_ratio, int formula) {
-  int j = blockIdx.x * blockDim.x + threadIdx.x;
+  int j = gom_block() * blockDim.x + threadIdx.x;
   if (j >= J) return;
   const double phase = f.scal[5], spin = f.scal[4];
   double *uf_ = const_cast<double *>(f.wind_u_at_face), *vf_ = const_cast<double *>(f.wind_v_at_face);

@@ -63,7 +63,7 @@ __device__ __forceinline__ size_t zix(int t, int k, int e, int L1, int n) {  // 
 // con_cell from salt/temp (+ surface heat source), solve_transport_equation_v19.f90:88-173
 template <typename W>
 __global__ void k_tvd_init(DevMesh m, DevState s, DevForcing f, TvdW<W> w, DevDims d, TvdArgs a) {
-  int q = blockIdx.x * blockDim.x + threadIdx.x;
+  int q = gom_block() * blockDim.x + threadIdx.x;
   const int E = d.E, L = d.L, L2 = d.L + 2;
   if (q >= E * L2) return;
   int e = q % E, k = q / E;
@@ -88,7 +88,7 @@ __global__ void k_tvd_init(DevMesh m, DevState s, DevForcing f, TvdW<W> w, DevDi
 // (solve_transport_equation_v19.f90:291-397); one thread per (layer, boundary cell).
 template <typename W>
 __global__ void k_tvd_bnd(DevMesh m, DevState s, DevForcing f, TvdW<W> w, DevDims d, TvdArgs a) {
-  int q = blockIdx.x * blockDim.x + threadIdx.x;
+  int q = gom_block() * blockDim.x + threadIdx.x;
   const int L = d.L, E = d.E, J = d.J, L2 = d.L + 2;
   int nob = d.NOB, nqb = d.NQB;
   if (q < L * nob) {
@@ -127,7 +127,7 @@ __global__ void k_tvd_bnd(DevMesh m, DevState s, DevForcing f, TvdW<W> w, DevDim
 // Face coefficients Q, d, phi (solve_transport_equation_v19.f90:552-581)
 template <typename W>
 __global__ void k_tvd_coef_face(DevMesh m, DevState s, TvdW<W> w, DevDims d, TvdArgs a) {
-  int q = blockIdx.x * blockDim.x + threadIdx.x;
+  int q = gom_block() * blockDim.x + threadIdx.x;
   const int J = d.J;
   if (q >= J * d.L) return;
   int j = q % J, k = q / J + 1;
@@ -205,7 +205,7 @@ __device__ __forceinline__ void lim_hsums(const DevMesh &m, const TvdW<W> &w, co
 // sums in w.htop, which every vertical limiter ratio of the column uses (k_tvd_lim_cell).
 template <typename W>
 __global__ void k_tvd_coef_cell(DevMesh m, DevState s, TvdW<W> w, DevDims d, TvdArgs a) {
-  int q = blockIdx.x * blockDim.x + threadIdx.x;
+  int q = gom_block() * blockDim.x + threadIdx.x;
   const int E = d.E;
   if (q >= E * (d.L + 1)) return;
   int e = q % E, k = q / E;
@@ -246,7 +246,7 @@ __global__ void k_tvd_coef_cell(DevMesh m, DevState s, TvdW<W> w, DevDims d, Tvd
 // k_tvd_coef_cell computed once per column (w.htop) in the same order.
 template <typename W>
 __global__ void k_tvd_lim_cell(DevMesh m, DevState s, TvdW<W> w, DevDims d, int hlim, int first) {
-  long q = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  long q = (long)gom_block() * blockDim.x + threadIdx.x;
   const int E = d.E, L = d.L, L1 = d.L + 1, L2 = d.L + 2;
   if (q >= (long)L1 * E) return;
   int e = (int)(q % E), k = (int)(q / E);
@@ -324,7 +324,7 @@ __global__ void k_tvd_lim_cell(DevMesh m, DevState s, TvdW<W> w, DevDims d, int 
 // a tie (Q = 0) -- the same writer is chosen here.
 template <typename W>
 __global__ void k_tvd_lim_face(DevMesh m, DevState s, TvdW<W> w, DevDims d, int hlim, int first) {
-  int q = blockIdx.x * blockDim.x + threadIdx.x;
+  int q = gom_block() * blockDim.x + threadIdx.x;
   const int J = d.J, E = d.E, L = d.L, L2 = d.L + 2;
   if (q >= J * L) return;
   int j = q % J, k = q / J + 1;
@@ -478,7 +478,7 @@ __device__ __forceinline__ double tvd_clamp(double v) {
 // The explicit right-hand sides, one thread per (layer, cell), into w.tvd_x (row of the layer).
 template <typename W>
 __global__ void k_tvd_rhs(DevMesh m, DevState s, TvdW<W> w, DevDims d, TvdArgs a) {
-  long q = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  long q = (long)gom_block() * blockDim.x + threadIdx.x;
   const int E = d.E, L = d.L;
   if (q >= (long)L * E) return;
   int e = (int)(q % E), k = (int)(q / E) + 1;
@@ -498,7 +498,7 @@ __global__ void k_tvd_rhs(DevMesh m, DevState s, TvdW<W> w, DevDims d, TvdArgs a
 template <typename W>
 __global__ void k_tvd_solve(DevMesh m, DevState s, TvdW<W> w, DevDims d, TvdArgs a, double *con_next, int full,
                             int last, int *err) {
-  int e = blockIdx.x * blockDim.x + threadIdx.x;
+  int e = gom_block() * blockDim.x + threadIdx.x;
   const int E = d.E, L = d.L, L2 = d.L + 2;
   if (e >= E) return;
   const int bl = m.bottom_layer_at_element[e], tl = s.top_layer_at_element[e];
@@ -594,7 +594,7 @@ template <typename W>
 __global__ void k_tvd_node(DevMesh m, DevState s, TvdW<W> w, DevDims d) {
   const int P = d.P, L = d.L;
   const int nch = (L + TVD_NODE_LEVELS - 1) / TVD_NODE_LEVELS;
-  long q = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  long q = (long)gom_block() * blockDim.x + threadIdx.x;
   if (q >= (long)P * nch) return;
   const int i = (int)(q % P), c = (int)(q / P);
   const int khi = L - c * TVD_NODE_LEVELS;                  // the chunk's top level
@@ -628,7 +628,7 @@ __global__ void k_tvd_node(DevMesh m, DevState s, TvdW<W> w, DevDims d) {
 // Face values and the copy to the model arrays (solve_transport_equation_v19.f90:1457-1482)
 template <typename W>
 __global__ void k_tvd_face_out(DevMesh m, DevState s, TvdW<W> w, DevDims d) {
-  int q = blockIdx.x * blockDim.x + threadIdx.x;
+  int q = gom_block() * blockDim.x + threadIdx.x;
   const int J = d.J, P = d.P, L = d.L;
   if (q >= J * L) return;
   int j = q % J, k = q / J + 1;
@@ -643,7 +643,7 @@ __global__ void k_tvd_face_out(DevMesh m, DevState s, TvdW<W> w, DevDims d) {
 
 template <typename W>
 __global__ void k_tvd_cellnode_out(DevState s, TvdW<W> w, DevDims d) {
-  int q = blockIdx.x * blockDim.x + threadIdx.x;
+  int q = gom_block() * blockDim.x + threadIdx.x;
   const int E = d.E, P = d.P, L = d.L;
   long nc = (long)E * L, nn = (long)P * L;
   for (int t = 0; t < d.NT; t++) {
@@ -724,7 +724,7 @@ __device__ void eos_column(const DevMesh &m, double MSL, int full, int compute, 
 }
 
 __global__ void k_eos_nodeface(DevMesh m, DevState s, DevDims d, double MSL, int full, int idens, int *err) {
-  int q = blockIdx.x * blockDim.x + threadIdx.x;
+  int q = gom_block() * blockDim.x + threadIdx.x;
   const int P = d.P, J = d.J, L = d.L;
   if (q < P) {
     int tl = s.top_layer_at_node[q];
@@ -741,7 +741,7 @@ __global__ void k_eos_nodeface(DevMesh m, DevState s, DevDims d, double MSL, int
 }
 
 __global__ void k_eos_cell(DevMesh m, DevState s, DevDims d, int *err) {
-  int e = blockIdx.x * blockDim.x + threadIdx.x;
+  int e = gom_block() * blockDim.x + threadIdx.x;
   const int E = d.E, P = d.P, L = d.L;
   if (e >= E) return;
   int tl = s.top_layer_at_element[e];
@@ -774,7 +774,7 @@ __global__ void k_eos_cell(DevMesh m, DevState s, DevDims d, int *err) {
 // ================================================================ update
 // Depth averages at nodes, update_variables.f90:37-74
 __global__ void k_bars(DevMesh m, DevState s, DevDims d) {
-  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  int i = gom_block() * blockDim.x + threadIdx.x;
   const int P = d.P;
   if (i >= P) return;
   int tl = s.top_layer_at_node[i], bl = m.bottom_layer_at_node[i];

@@ -79,7 +79,7 @@ __device__ __forceinline__ double rz_level(const TurbParams &p, const DevState &
 // Schemes 1 and 2 (turbulence_closure_v1.f90:75-141): per-level algebra, one thread per
 // (level, face).
 __global__ void k_turb_ri(DevMesh m, DevState s, DevWork w, DevDims d, TurbParams p) {
-  long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  long t = (long)gom_block() * blockDim.x + threadIdx.x;
   const int J = d.J, L = d.L;
   if (t >= (long)L * J) return;
   int j = (int)(t % J), k = (int)(t / J) + 1;
@@ -152,7 +152,7 @@ __device__ __forceinline__ void level_diff(const TurbParams &p, const DevState &
 // * the new diffusivities at every level, clamped to the column's range (:397-427).
 __global__ void __launch_bounds__(256) GOM_WAVES(GOM_WAVES_TURB) k_turb_column(DevMesh m, DevState s, DevForcing f, DevWork w, DevDims d,
                                                      TurbParams p, int *err) {
-  int j = blockIdx.x * blockDim.x + threadIdx.x;
+  int j = gom_block() * blockDim.x + threadIdx.x;
   const int J = d.J;
   if (j >= J) return;
   int kt = s.top_layer_at_face[j];
@@ -419,7 +419,7 @@ __global__ void k_turb_kv(DevMesh m, DevState s, DevWork w, DevDims d) {
   // ranges are read once for all of them
   const int E = d.E, J = d.J, L1 = d.L + 1;
   const int nch = (L1 + GOM_KV_LEVELS - 1) / GOM_KV_LEVELS;
-  long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  long t = (long)gom_block() * blockDim.x + threadIdx.x;
   if (t >= (long)nch * E) return;
   const int i = (int)(t % E), k0 = (int)(t / E) * GOM_KV_LEVELS;
   if (s.top_layer_at_element[i] == 0) return;

@@ -33,6 +33,9 @@ class HostForcing:
         self.julian_day = fz.julian_day0(case)
         self.tran_ids = [params.tran_id1, params.tran_id2]
         self.uploads = list(DYNAMIC)      # forcing arrays the coming step needs uploaded
+        # set when the model computes the boundary forcing itself (GOMGPU_STAGE_BOUNDARY):
+        # then a step needs only its scalars
+        self.device_boundary = False
         if case.hurricane_flag == 1:
             # read_hurricane_ser_0.f90: the first snapshot pair; the interpolated field,
             # air_p1 and the face winds start at zero (allocate_variables.f90:507-519)
@@ -63,13 +66,16 @@ class HostForcing:
         f.scalars["elapsed_time"] = elapsed
         f.scalars["spinup_function_tide"] = sp_tide
         f.scalars["spinup_function_baroclinic"] = sp_bc
-        self.uploads = list(DYNAMIC)
+        f.scalars["julian_day"] = self.julian_day
+        self.uploads = [] if self.device_boundary else list(DYNAMIC)
         if c.hurricane_flag == 1:
             ph = fz.hurricane_phase(it, self.julian_day, self.window, self.params.hurricane_read_interval)
             f.scalars["hurricane_phase"] = float(ph)
             if ph == 2:
                 f["wind_u_next"], f["wind_v_next"], f["air_p_next"] = self._next_snapshot(it)
                 self.uploads += ["wind_u_next", "wind_v_next", "air_p_next"]
+        if self.device_boundary:
+            return f
         old, new = fz.boundary_eta(c, it, sp_tide, self.julian_day)
         f["eta_at_ob_old"] = old
         f["eta_at_ob_new"] = new
@@ -145,7 +151,7 @@ class Scenario:
     """One scenario advanced on the device."""
 
     def __init__(self, case: Case, precision: int = 0, wind=(0.0, 0.0), model: Optional[host.GpuModel] = None,
-                 prepared=None, model_factory=None):
+                 prepared=None, model_factory=None, device_boundary: bool = True):
         self.case = case
         if prepared is None:
             self.params, self.mesh, state0 = prep.build(case, precision)
@@ -161,6 +167,11 @@ class Scenario:
         self.host_forcing = HostForcing(case, self.params, self.mesh, wind)
         self.forcing = self.host_forcing.forcing
         self.model.set_forcing(self.forcing)   # static fields once
+        # a model that computes the boundary forcing on the device gets its tables once; the
+        # host then sends only each step's scalars
+        if device_boundary and hasattr(self.model, "set_boundary") and not self.host_river:
+            self.model.set_boundary(fz.boundary_tables(case))
+            self.host_forcing.device_boundary = True
         self.it = 0
 
     def stage_forcing(self) -> None:

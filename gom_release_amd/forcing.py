@@ -256,3 +256,50 @@ def river_discharge(case: Case, julian_day: float, prev=None):
         if v is not None:
             q[i] = v * r.portion
     return q
+
+
+def boundary_tables(case: Case, params=None) -> dict:
+    """The tables of the device boundary forcing (gomgpu_boundary of include/gomgpu.h): the
+    case's harmonic sets, every series (elevation, salinity, temperature, discharge, in that
+    order, in one table, as the times and scaled values series_value reads), and which set or
+    series each open boundary and river cell reads. With them the device computes what
+    boundary_eta, tvd_boundary_concentrations / elm_boundary_concentrations and
+    river_discharge compute here, from the step's scalars alone."""
+    first, shift, cons = [0], [], []
+    for sh, cn in case.harmonics:
+        shift.append(sh)
+        for (per, amp, ph, fac, v0, _nm) in cn:
+            cons.extend([per, amp, fac, v0, ph])
+        first.append(len(cons) // 5)
+    groups = (case.eta_series, case.salt_series, case.temp_series, case.q_series)
+    base = np.cumsum([0] + [len(g) for g in groups])
+    sfirst, stime, sval = [0], [], []
+    for g in groups:
+        for s in g:
+            stime.extend(s.seconds())
+            sval.extend(s.scaled())
+            sfirst.append(len(stime))
+
+    def sid(group, i):
+        return int(base[group] + i) if i > 0 else 0
+
+    obs, rivs = case.open_boundaries, case.rivers
+    for b in obs:
+        if b.eta_type not in (1, 2):
+            raise NotImplementedError("ob_eta_type -1 (elevation radiated from interior cells) needs the device "
+                                      "state on the host each step")
+    return dict(
+        num_harmonic_sets=len(case.harmonics), harmonic_first=np.array(first, np.int32),
+        tidal_phase_shift=np.array(shift, np.float64), constituents=np.array(cons, np.float64),
+        num_series=len(sfirst) - 1, series_first=np.array(sfirst, np.int32),
+        series_time=np.array(stime, np.float64), series_value=np.array(sval, np.float64),
+        num_ob=len(obs),
+        ob_eta_type=np.array([b.eta_type for b in obs], np.int32),
+        ob_eta_id=np.array([b.harmonic_id if b.eta_type == 1 else sid(0, b.eta_ser_id) for b in obs], np.int32),
+        ob_tracer_series=np.array([[sid(1, b.salt_ser_id), sid(2, b.temp_ser_id)] for b in obs], np.int32).ravel(),
+        num_qb=len(rivs),
+        qb_q_series=np.array([sid(3, r.q_ser_id) for r in rivs], np.int32),
+        qb_portion=np.array([r.portion for r in rivs], np.float64),
+        qb_tracer_series=np.array([[sid(1, r.salt_ser_id), sid(2, r.temp_ser_id)] for r in rivs], np.int32).ravel(),
+        is_salt=int(case.is_salt), is_temp=int(case.is_temp), ref_salt=float(case.ref_salt),
+        ref_temp=float(case.ref_temp), reference_diff_days=int(case.reference_diff_days()))

@@ -46,6 +46,7 @@ def lib():
         L.gomgpu_set_state.argtypes = [vp, C.POINTER(abi.State)]
         L.gomgpu_get_state.argtypes = [vp, C.POINTER(abi.State)]
         L.gomgpu_set_forcing.argtypes = [vp, C.POINTER(abi.Forcing)]
+        L.gomgpu_set_boundary.argtypes = [vp, C.POINTER(abi.Boundary)]
         L.gomgpu_get_forcing.argtypes = [vp, C.POINTER(abi.ForcingOut)]
         L.gomgpu_stage.argtypes = [vp, C.c_int]
         L.gomgpu_step.argtypes = [vp]
@@ -95,7 +96,7 @@ def lib():
     return _lib
 
 
-EXPORTED = ["gomgpu_create", "gomgpu_set_state", "gomgpu_set_forcing", "gomgpu_stage",
+EXPORTED = ["gomgpu_create", "gomgpu_set_state", "gomgpu_set_forcing", "gomgpu_set_boundary", "gomgpu_stage",
             "gomgpu_step", "gomgpu_run", "gomgpu_get_state", "gomgpu_get_forcing", "gomgpu_budgets",
             "gomgpu_pcg_iterations", "gomgpu_sync", "gomgpu_last_error", "gomgpu_timings",
             "gomgpu_destroy", "gomgpu_set_launch", "gomgpu_get_launch", "gomgpu_set_pcg_rows",
@@ -108,6 +109,8 @@ EXPORTED = ["gomgpu_create", "gomgpu_set_state", "gomgpu_set_forcing", "gomgpu_s
             "gomgpu_elm_departures", "gomgpu_elm_near_edge"]
 
 HIP_H2D, HIP_D2H = 1, 2
+_BND_INT = {"harmonic_first", "series_first", "ob_eta_type", "ob_eta_id", "ob_tracer_series", "qb_q_series",
+            "qb_tracer_series"}
 
 
 def footprint(params: "abi.Params") -> int:
@@ -206,6 +209,22 @@ class GpuModel:
     def set_forcing(self, forcing: "abi.ForcingArrays", only: Optional[Sequence[str]] = None) -> None:
         f = forcing.struct(only)
         self._check(self._L.gomgpu_set_forcing(self._ctx, C.byref(f)))
+
+    def set_boundary(self, tables: dict) -> None:
+        """Upload the boundary tables (forcing.boundary_tables): from then on every step
+        computes its open-boundary elevations, river discharge and boundary concentrations on
+        the device, and the host uploads only the per-step scalars."""
+        keep = {k: np.ascontiguousarray(v, dtype=np.int32 if k in _BND_INT else np.float64)
+                for k, v in tables.items() if isinstance(v, np.ndarray)}
+        b = abi.Boundary()
+        for name, typ in abi.Boundary._fields_:
+            v = keep.get(name, tables.get(name))
+            if isinstance(v, np.ndarray):
+                setattr(b, name, v.ctypes.data_as(typ))
+            elif v is not None:
+                setattr(b, name, v)
+        self._bnd_keep = keep
+        self._check(self._L.gomgpu_set_boundary(self._ctx, C.byref(b)))
 
     def get_forcing(self, forcing: "abi.ForcingArrays", only: Optional[Sequence[str]] = None) -> None:
         """Download the device's forcing arrays (the wind stage's outputs among them)."""

@@ -5,6 +5,10 @@
  * through the module mod_global_variables (GOM_Rv1.1.0/source/main.f90:179-272). This ABI
  * replaces them one for one, with the module arrays passed explicitly:
  *
+ *   gomgpu_stage(ctx, GOMGPU_STAGE_BOUNDARY)  compute_boundary_eta (ob_eta_type 1, 2), main.f90:84-161
+ *                                             calculate_Q, and the transport's boundary
+ *                                             concentrations (only after gomgpu_set_boundary;
+ *                                             first in gomgpu_step)
  *   gomgpu_stage(ctx, GOMGPU_STAGE_WIND)      read_hurricane_ser_1 +      main.f90:113-137
  *                                             calculate_wind_stress (hurricane_flag = 1 only;
  *                                             first in gomgpu_step)
@@ -147,7 +151,7 @@ extern "C" {
  * snapshot (read_hurricane_ser_1.f90:18-46, every hurricane_read_interval steps). */
 #define GOMGPU_FORCING_SCALARS(X) \
   X(elapsed_time) X(spinup_function_tide) X(spinup_function_baroclinic) \
-  X(spinup_function_wind) X(hurricane_phase)
+  X(spinup_function_wind) X(hurricane_phase) X(julian_day)
 
 #define GOMGPU_DECL_RPTR(name, ext) double *name;
 #define GOMGPU_DECL_IPTR(name, ext) int32_t *name;
@@ -178,6 +182,40 @@ typedef struct gomgpu_forcing {
   GOMGPU_FORCING_SCALARS(GOMGPU_DECL_REAL)
   GOMGPU_FORCING_REAL(GOMGPU_DECL_CRPTR)
 } gomgpu_forcing;
+
+/* Boundary forcing computed on the device by GOMGPU_STAGE_BOUNDARY, once
+ * gomgpu_set_boundary has uploaded these tables: the open-boundary elevations eta_at_ob_old/new
+ * of compute_boundary_eta.f90:72-124 (harmonic sets, ob_eta_type 1, and elevation series, 2),
+ * the river discharge Q_add of calculate_Q.f90:21-45 (q_interp_method 1), and con_at_ob /
+ * con_at_Qb of the transport: the sub-cycles of solve_transport_equation_v19.f90:229-397, or
+ * find_openboundary_salt_temp_v3.f90:17-97 with transport_solver 2. The per-step inputs are
+ * the forcing scalars (elapsed_time, julian_day after its increment, the tide and baroclinic
+ * spin-up factors); the host then uploads no boundary array. Series are 1-based indices into
+ * one table of all series (0 = none); a series is the reference's times [s] and scaled values.
+ * The harmonic terms use a correctly rounded cosine; the C library's cos, which the reference
+ * calls, rounds a few arguments in a thousand the other way (one ulp). */
+typedef struct gomgpu_boundary {
+  int32_t num_harmonic_sets;
+  const int32_t *harmonic_first;     /* (num_harmonic_sets + 1) first constituent of each set, 0-based, then the end */
+  const double *tidal_phase_shift;   /* (num_harmonic_sets) degrees */
+  const double *constituents;        /* (5, constituents) period [h], amplitude [m], nodal factor,
+                                        equilibrium argument [deg], phase [deg] */
+  int32_t num_series;
+  const int32_t *series_first;       /* (num_series + 1) first point of each series, 0-based, then the end */
+  const double *series_time;         /* seconds (Series.seconds of the data year) */
+  const double *series_value;        /* scaled values */
+  int32_t num_ob;                    /* num_ob_cell */
+  const int32_t *ob_eta_type;        /* (num_ob) 1 harmonic set, 2 elevation series */
+  const int32_t *ob_eta_id;          /* (num_ob) harmonic set or series, 1-based */
+  const int32_t *ob_tracer_series;   /* (2, num_ob) salinity and temperature series, 0 = none */
+  int32_t num_qb;                    /* num_Qb_cell */
+  const int32_t *qb_q_series;        /* (num_qb) discharge series, 0 = none */
+  const double *qb_portion;          /* (num_qb) Q_portion */
+  const int32_t *qb_tracer_series;   /* (2, num_qb) */
+  int32_t is_salt, is_temp;          /* the ELM solver's boundaries: open ones without a series take */
+  double ref_salt, ref_temp;         /*   ref_salt / ref_temp */
+  int32_t reference_diff_days;       /* calculate_reference_diff_days.f90: series times shift by it */
+} gomgpu_boundary;
 
 /* Destination of gomgpu_get_forcing: the forcing arrays as they are on the device
  * (the outputs of the wind stage among them); NULL fields are not copied. */
@@ -211,7 +249,8 @@ enum gomgpu_stage_id {
   GOMGPU_STAGE_UPDATE = 9,
   GOMGPU_STAGE_TURBULENCE = 10,
   GOMGPU_STAGE_WIND = 11,
-  GOMGPU_NUM_STAGES = 12
+  GOMGPU_STAGE_BOUNDARY = 12,
+  GOMGPU_NUM_STAGES = 13
 };
 
 enum gomgpu_status {
@@ -262,6 +301,10 @@ int gomgpu_create(const gomgpu_mesh *mesh, const gomgpu_params *params, gomgpu_c
 int gomgpu_set_state(gomgpu_ctx *ctx, const gomgpu_state *state);
 /* Upload the host forcing of the coming step (main.f90:84-161). */
 int gomgpu_set_forcing(gomgpu_ctx *ctx, const gomgpu_forcing *forcing);
+/* Upload the boundary tables: from now on each step computes its boundary forcing on the
+ * device (GOMGPU_STAGE_BOUNDARY). No counterpart in the reference, whose host computes it
+ * (main.f90:84-161); for callers that keep the forcing on the device. */
+int gomgpu_set_boundary(gomgpu_ctx *ctx, const gomgpu_boundary *tables);
 /* Run one stage (for per-stage parity) or the whole step. Both are asynchronous on the
  * context's stream except where a stage needs a device-to-host value (none in FP64). */
 int gomgpu_stage(gomgpu_ctx *ctx, int stage_id);
