@@ -63,7 +63,7 @@ class ForcingOut(C.Structure):
 
 
 STAGES = ["friction", "elm", "momentum", "freesurf", "velocity", "layers", "nodevel",
-          "transport", "eos", "update", "turbulence", "wind", "boundary"]
+          "transport", "eos", "update", "turbulence", "wind", "boundary", "bars"]
 
 
 class Boundary(C.Structure):

@@ -6,6 +6,7 @@
 #include <stdlib.h>
 #include <string.h>
 
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -103,15 +104,16 @@ static const char *kKernelNames[GOMGPU_NUM_KTIMERS] = {"k_pcg", "k_elm", "k_mome
 static const char *kStageNames[GOMGPU_NUM_STAGES] = {"friction", "elm",       "momentum", "freesurf",
                                                       "velocity", "layers",    "nodevel",  "transport",
                                                       "eos",      "update",    "turbulence", "wind",
-                                                      "boundary"};
+                                                      "boundary", "bars"};
 // Order of the stages in one step (main.f90:95-272, the closure before the transport; the
 // boundary forcing and the wind stage of main.f90:84-161 first, each a no-op unless its
 // tables or hurricane fields are set).
-static const int kStepOrder[GOMGPU_NUM_STAGES] = {
+static const int kStepOrder[] = {
     GOMGPU_STAGE_BOUNDARY,       GOMGPU_STAGE_WIND,
     GOMGPU_STAGE_FRICTION, GOMGPU_STAGE_ELM,        GOMGPU_STAGE_MOMENTUM,  GOMGPU_STAGE_FREESURF,
     GOMGPU_STAGE_VELOCITY, GOMGPU_STAGE_LAYERS,     GOMGPU_STAGE_NODEVEL,   GOMGPU_STAGE_TURBULENCE,
     GOMGPU_STAGE_TRANSPORT, GOMGPU_STAGE_EOS,       GOMGPU_STAGE_UPDATE};
+static const int kNumStep = (int)(sizeof(kStepOrder) / sizeof(kStepOrder[0]));
 
 struct gomgpu_ctx {
   int device = 0;
@@ -166,6 +168,21 @@ struct gomgpu_ctx {
   bool bnd_on = false;
   DevBnd bnd{};
   int bnd_nt = 1;
+  // asynchronous state snapshots (gomgpu_snapshot): a device copy of the fields taken on the
+  // step's stream, then copied to pinned host memory on a stream of its own
+  struct SnapItem {
+    void *dst;
+    size_t off;  // bytes into the snapshot buffers
+    long ld, n;
+    bool is_int;
+  };
+  std::mutex snap_mu;
+  hipStream_t snap_st = nullptr;
+  hipEvent_t snap_ev0 = nullptr, snap_ev = nullptr;
+  char *snap_dev = nullptr, *snap_host = nullptr;
+  size_t snap_cap = 0;
+  bool snap_busy = false;
+  std::vector<SnapItem> snap_items;
   int32_t *dep_buf = nullptr; // departure cells and levels of the momentum ELM (allocated on
                               // the first gomgpu_elm_record; w.elm_dep points here while on)
 };
@@ -600,6 +617,14 @@ extern "C" void gomgpu_destroy(gomgpu_ctx *c) {
   if (!c) return;
   while (!c->batches.empty()) batch_forget(c->batches.back(), c);  // leaves its batches
   if (c->st) hipStreamSynchronize(c->st);
+  if (c->snap_st) {
+    hipStreamSynchronize(c->snap_st);
+    hipStreamDestroy(c->snap_st);
+  }
+  if (c->snap_ev0) hipEventDestroy(c->snap_ev0);
+  if (c->snap_ev) hipEventDestroy(c->snap_ev);
+  if (c->snap_dev) hipFree(c->snap_dev);
+  if (c->snap_host) hipHostFree(c->snap_host);
   for (void *p : c->allocs) hipFree(p);
   if (c->h_err) hipHostFree(c->h_err);
   if (c->h_stage) hipHostFree(c->h_stage);
@@ -674,6 +699,83 @@ extern "C" int gomgpu_get_state(gomgpu_ctx *c, gomgpu_state *st) {
   GOMGPU_STATE_INT(DN_S)
 #undef DN_S
   return 0;
+}
+
+// ---- asynchronous snapshots
+// The snapshot's device copies go on the context stream (ordered after the work already
+// queued, a few microseconds per field at HBM speed); the download runs on the snapshot
+// stream, so the steps queued after it do not wait for the bus. One snapshot is in flight per
+// context: a new one first completes the previous.
+static int snapshot_finish(gomgpu_ctx *c) {
+  if (!c->snap_busy) return 0;
+  HIPCHK(c, hipEventSynchronize(c->snap_ev));
+  for (const auto &it : c->snap_items) {
+    Shape sh{it.ld, it.n};
+    if (it.is_int) from_device_layout((const int32_t *)(c->snap_host + it.off), (int32_t *)it.dst, sh);
+    else from_device_layout((const double *)(c->snap_host + it.off), (double *)it.dst, sh);
+  }
+  c->snap_items.clear();
+  c->snap_busy = false;
+  return 0;
+}
+
+extern "C" int gomgpu_snapshot(gomgpu_ctx *c, const gomgpu_state *dst) {
+  if (!c || !dst) return GOMGPU_E_ARG;
+  std::lock_guard<std::mutex> lk(c->snap_mu);
+  int rc;
+  if ((rc = snapshot_finish(c))) return rc;
+  if ((rc = materialize(c, SWAP_ALL))) return rc;
+  const gomgpu_params &p = c->p;
+  std::vector<gomgpu_ctx::SnapItem> items;
+  std::vector<const void *> srcs;
+  size_t total = 0;
+#define SNAP_F(name, ext, isint)                                                          \
+  if (dst->name) {                                                                        \
+    Shape sh = shape_of(#ext, p);                                                         \
+    items.push_back({(void *)dst->name, total, sh.ld, sh.n, isint});                      \
+    srcs.push_back((const void *)c->s.name);                                              \
+    total += ((size_t)(sh.ld * sh.n) * (isint ? sizeof(int32_t) : sizeof(double)) + 255) & ~(size_t)255; \
+  }
+#define SNAP_R(name, ext) SNAP_F(name, ext, false)
+#define SNAP_I(name, ext) SNAP_F(name, ext, true)
+  GOMGPU_STATE_REAL(SNAP_R)
+  GOMGPU_STATE_INT(SNAP_I)
+#undef SNAP_R
+#undef SNAP_I
+#undef SNAP_F
+  if (items.empty()) return 0;
+  if (!c->snap_st) {
+    HIPCHK(c, hipStreamCreateWithFlags(&c->snap_st, hipStreamNonBlocking));
+    HIPCHK(c, hipEventCreateWithFlags(&c->snap_ev0, hipEventDisableTiming));
+    HIPCHK(c, hipEventCreateWithFlags(&c->snap_ev, hipEventDisableTiming));
+  }
+  if (total > c->snap_cap) {
+    if (c->snap_dev) HIPCHK(c, hipFree(c->snap_dev));
+    if (c->snap_host) HIPCHK(c, hipHostFree(c->snap_host));
+    c->snap_dev = c->snap_host = nullptr;
+    c->snap_cap = 0;
+    if (hipMalloc((void **)&c->snap_dev, total) != hipSuccess) return seterr(c, GOMGPU_E_NOMEM, "snapshot hipMalloc failed");
+    if (hipHostMalloc((void **)&c->snap_host, total, hipHostMallocDefault) != hipSuccess)
+      return seterr(c, GOMGPU_E_NOMEM, "snapshot hipHostMalloc failed");
+    c->snap_cap = total;
+  }
+  for (size_t i = 0; i < items.size(); i++) {
+    size_t bytes = (size_t)(items[i].ld * items[i].n) * (items[i].is_int ? sizeof(int32_t) : sizeof(double));
+    HIPCHK(c, hipMemcpyAsync(c->snap_dev + items[i].off, srcs[i], bytes, hipMemcpyDeviceToDevice, c->st));
+  }
+  HIPCHK(c, hipEventRecord(c->snap_ev0, c->st));
+  HIPCHK(c, hipStreamWaitEvent(c->snap_st, c->snap_ev0, 0));
+  HIPCHK(c, hipMemcpyAsync(c->snap_host, c->snap_dev, total, hipMemcpyDeviceToHost, c->snap_st));
+  HIPCHK(c, hipEventRecord(c->snap_ev, c->snap_st));
+  c->snap_items = std::move(items);
+  c->snap_busy = true;
+  return 0;
+}
+
+extern "C" int gomgpu_snapshot_wait(gomgpu_ctx *c) {
+  if (!c) return GOMGPU_E_ARG;
+  std::lock_guard<std::mutex> lk(c->snap_mu);
+  return snapshot_finish(c);
 }
 
 extern "C" int gomgpu_get_forcing(gomgpu_ctx *c, gomgpu_forcing_out *fo) {
@@ -1171,6 +1273,10 @@ static int stage_impl(gomgpu_ctx *c, int id) {
       k_bars<<<nblk(d.P, BS), BS, 0, st>>>(c->m, c->s, d);
       break;
     }
+    case GOMGPU_STAGE_BARS:
+      // calculate_vertically_averaged_values (prepare_gom.f90:172)
+      k_bars<<<nblk(d.P, BS), BS, 0, st>>>(c->m, c->s, d);
+      break;
     case GOMGPU_STAGE_BOUNDARY: {
       // compute_boundary_eta, calculate_Q and the transport's boundary concentrations
       // (k_boundary.hip), from the tables of gomgpu_set_boundary
@@ -1242,14 +1348,14 @@ extern "C" int gomgpu_stage(gomgpu_ctx *c, int id) {
 extern "C" int gomgpu_step(gomgpu_ctx *c) {
   if (!c) return GOMGPU_E_ARG;
   if (c->timing) HIPCHK(c, hipEventRecord(c->ev[0], c->st));
-  for (int pos = 0; pos < GOMGPU_NUM_STAGES; pos++) {
+  for (int pos = 0; pos < kNumStep; pos++) {
     int rc = stage_impl(c, kStepOrder[pos]);
     if (rc) return rc;
     if (c->timing) HIPCHK(c, hipEventRecord(c->ev[pos + 1], c->st));
   }
   if (c->timing) {
-    HIPCHK(c, hipEventSynchronize(c->ev[GOMGPU_NUM_STAGES]));
-    for (int pos = 0; pos < GOMGPU_NUM_STAGES; pos++) {
+    HIPCHK(c, hipEventSynchronize(c->ev[kNumStep]));
+    for (int pos = 0; pos < kNumStep; pos++) {
       float ms = 0;
       hipEventElapsedTime(&ms, c->ev[pos], c->ev[pos + 1]);
       c->stage_ms[kStepOrder[pos]] = ms;

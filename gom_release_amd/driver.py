@@ -163,7 +163,8 @@ class Scenario:
         # a model that does not derive the river face velocity itself gets it from the host
         self.host_river = bool(case.rivers) and getattr(self.model, "host_river_velocity", False)
         self.model.set_state(state0)
-        self.model.stage("eos")          # initial density, as prepare_gom does
+        self.model.stage("eos")          # initial density and depth means, as prepare_gom does
+        self.model.stage("bars")
         self.host_forcing = HostForcing(case, self.params, self.mesh, wind)
         self.forcing = self.host_forcing.forcing
         self.model.set_forcing(self.forcing)   # static fields once
@@ -195,8 +196,10 @@ class Scenario:
         for _ in range(n):
             self.step()
 
-    # ---- output files (prepare_gom.f90:216-227 opens the dumps; main.f90:274-295 writes)
+    # ---- output files (prepare_gom.f90:186-227 opens them; main.f90:274-295 writes)
     outputs = None
+    plots = None
+    async_out = None
 
     def fields(self, names) -> "abi.Arrays":
         """The named state fields, read back from the device."""
@@ -204,14 +207,33 @@ class Scenario:
         self.model.get_state(s, only=list(names))
         return s
 
-    def open_outputs(self, out_dir: str) -> None:
-        """Write the case's state dumps and restart files into `out_dir` after each step."""
-        from . import gomout
+    def open_outputs(self, out_dir: str, asynchronous: bool = True) -> None:
+        """Write the case's output into `out_dir` after each step: state dumps and restart
+        files (gomout), and the time series and Tecplot/VTK contour files of cards C41-C43
+        (gomplot), whose start-up part is written here. With `asynchronous` the contour and
+        time-series files are formatted and written by a thread of their own from device
+        snapshots, while the following steps run."""
+        from . import gomout, gomplot
         os.makedirs(out_dir, exist_ok=True)
         self.outputs = out_dir
         self.dumps = [gomout.DumpWriter(out_dir, kind, spec, self.params.maxlayer)
                       for kind, spec in (("2D", self.case.dump2d), ("3D", self.case.dump3d))
                       if spec is not None]
+        c = self.case
+        self.plots = self.async_out = None
+        if (c.tser is not None and c.tser.stations) or c.out2d is not None or c.out3d is not None:
+            self.plots = gomplot.PlotOutputs(c, self.mesh, self.params, out_dir)
+            self.plots.open(self.host_forcing.julian_day, self.it * c.dt, self.fields(self.plots.fields()))
+            if asynchronous:
+                self.async_out = gomplot.AsyncOutput(self.plots)
+
+    def _snapshot(self, names):
+        """A callable returning the named fields as they are now: a device snapshot when the
+        model takes one (the read happens on the caller of the callable), else a read now."""
+        if hasattr(self.model, "snapshot"):
+            return self.model.snapshot(list(names))
+        s = self.fields(names)
+        return lambda: s
 
     def write_outputs(self) -> list:
         """After step `it`: the dumps the schedule asks for, then the restart file when
@@ -223,14 +245,32 @@ class Scenario:
         for d in self.dumps:
             if d.step(it, self.host_forcing.julian_day, self.fields):
                 out.append(d.files[-1])
+        if self.plots is not None:
+            due = self.plots.due(it)
+            if due:
+                names = []
+                for w in due:
+                    names += [f for f in w.fields() if f not in names]
+                fetch = self._snapshot(names)
+                jd, elapsed = self.host_forcing.julian_day, it * c.dt
+                if self.async_out is not None:
+                    self.async_out.submit(due, jd, elapsed, fetch)
+                else:
+                    self.plots.write(due, jd, elapsed, fetch())
         if (c.restart_out == -1 and it == c.ndt) or (c.restart_out == 1 and it % c.restart_freq == 0):
             out.append(gomout.write_restart(gomout.restart_name(self.outputs, it),
                                             self.fields(gomout.RESTART_FIELDS)))
         return out
 
     def close_outputs(self) -> None:
+        """Close the output files (waiting for the output thread to finish its work)."""
         for d in getattr(self, "dumps", []):
             d.close()
+        if getattr(self, "async_out", None) is not None:
+            self.async_out.close()
+        elif getattr(self, "plots", None) is not None:
+            self.plots.close()
+        self.plots = self.async_out = None
         self.outputs = None
 
     def active_layers(self):

@@ -92,6 +92,34 @@ class DumpSpec:
 
 
 @dataclass
+class ContourSpec:
+    """Cards C42 / C43 (``read_main_inp.f90:1161-1205``): Tecplot and VTK contour output.
+    For 3-D output, `surf` and `full` are IS3D_surf_switch and IS3D_full_switch."""
+    format: int = 1            # 1 Tecplot, 2 VTK, 3 both
+    file_freq: int = 99999999  # outputs per Tecplot file = file_freq // frequency
+    time: int = 4              # time unit: 1 s, 2 min, 3 h, 4 days
+    frequency: int = 1         # every `frequency` steps in [start, end]
+    start: int = 0
+    end: int = 99999999
+    unit_conv: float = 1.0     # coordinate factor (1 m, 0.001 km)
+    variables: Sequence[int] = (1, 1, 1, 1, 1, 1)   # 2-D: eta u v salt temp rho; 3-D: u v w salt temp rho
+    surf: int = 1
+    full: int = 0
+    grid_format: int = 1
+
+
+@dataclass
+class TserSpec:
+    """Card C41 (``read_main_inp.f90:1129-1160``): time series at stations."""
+    stations: Sequence[Tuple[int, int, str]] = ()   # (cell, node, name)
+    hloc: int = 1              # 1 at the cell, 2 at the node
+    time: int = 4
+    frequency: int = 1
+    format: int = 3            # 1 Tecplot, 2 VTK, 3 both
+    fields: Sequence[int] = (1, 1, 1, 1, 1, 1, 0)   # eta H u v salt temp airp
+
+
+@dataclass
 class Case:
     """Everything a GOM project needs for the configurations this package runs."""
     name: str
@@ -183,6 +211,10 @@ class Case:
     # C44 / C45
     dump2d: Optional[DumpSpec] = None
     dump3d: Optional[DumpSpec] = None
+    # C41 - C43: time series and contour output (written by gom_release_amd.gomplot)
+    tser: Optional[TserSpec] = None
+    out2d: Optional[ContourSpec] = None
+    out3d: Optional[ContourSpec] = None
     # vertical turbulence closure (turbulence_closure_v1.f90; not read by read_main_inp.f90,
     # so these never reach the GOM input files): 0 constant Av/Kv, 1 Richardson step,
     # 2 Pacanowski-Philander, 3 GLS; model 'my','kl','ke','kw','ub'; stability 'GA','KC'
@@ -394,21 +426,42 @@ def _main_inp(c: Case) -> str:
     card("C40\tcheck_grid")
     data(0, 0, 0, 1, 0, 0.001)
     card("C41\ttser")
-    data(0, 0, 1, 4, 24, 1)
+    ts = c.tser
+    if ts is None or not ts.stations:
+        data(0, 0, 1, 4, 24, 1)
+    else:
+        data(len(ts.stations), 0, ts.hloc, ts.time, ts.frequency, ts.format)
+        card("C41_1\tserial cell node name")
+        for i, (ce, nd, nm) in enumerate(ts.stations):
+            data(i + 1, int(ce), int(nd), nm)
+        card("C41_2\teta H u v salt temp airp")
+        data(*[int(v) for v in ts.fields])
+    o2 = c.out2d
     card("C42\t2D output")
-    data(0, 0)
-    data(1, 0, 99999999)
-    data(4, 99999999, 0, 99999999, 0.001)
+    if o2 is None:
+        data(0, 0)
+        data(1, 0, 99999999)
+        data(4, 99999999, 0, 99999999, 0.001)
+    else:
+        data(1, 0)
+        data(o2.format, 0, o2.file_freq)
+        data(o2.time, o2.frequency, o2.start, o2.end, float(o2.unit_conv))
     card("C42_1\tvariables")
-    for nm in ("eta", "u", "v", "salt", "temp", "rho"):
-        data(nm, 0)
+    for nm, v in zip(("eta", "u", "v", "salt", "temp", "rho"), o2.variables if o2 else (0,) * 6):
+        data(nm, int(v))
+    o3 = c.out3d
     card("C43\t3D output")
-    data(0, 1, 0)
-    data(1, 0, 99999999)
-    data(4, 99999999, 0, 99999999, 0.001)
+    if o3 is None:
+        data(0, 1, 0)
+        data(1, 0, 99999999)
+        data(4, 99999999, 0, 99999999, 0.001)
+    else:
+        data(o3.full, o3.grid_format, o3.surf)
+        data(o3.format, 0, o3.file_freq)
+        data(o3.time, o3.frequency, o3.start, o3.end, float(o3.unit_conv))
     card("C43_1\tvariables")
-    for nm in ("u", "v", "w", "salt", "temp", "rho"):
-        data(nm, 0)
+    for nm, v in zip(("u", "v", "w", "salt", "temp", "rho"), o3.variables if o3 else (0,) * 6):
+        data(nm, int(v))
     for tag, spec in (("C44\t2D dump", c.dump2d), ("C45\t3D dump", c.dump3d)):
         card(tag)
         sp = spec or DumpSpec()
@@ -1097,6 +1150,17 @@ def case_from_project(pr: Project, strict: bool = True) -> Tuple[Case, List[str]
         c.hurricane_flag, c.hurricane_dt = 1, m.hurricane_dt
         c.hurricane_start, c.hurricane_end = m.hurricane_start, m.hurricane_end
         c.hurricane = os.path.join(pr.root, "input", "hurricane_ser.inp")
+    if m.tser_station_num > 0:
+        c.tser = TserSpec([(int(r[1]), int(r[2]), str(r[3])) for r in m.tser_stations], m.tser_hloc,
+                          m.tser_time, m.tser_frequency, m.tser_format,
+                          (m.tser_eta, m.tser_H, m.tser_u, m.tser_v, m.tser_salt, m.tser_temp, m.tser_airp))
+    if m.IS2D_switch == 1:
+        c.out2d = ContourSpec(m.IS2D_format, m.IS2D_File_freq, m.IS2D_time, m.IS2D_frequency, m.IS2D_start,
+                              m.IS2D_end, m.IS2D_unit_conv, tuple(m.IS2D_variable))
+    if m.IS3D_surf_switch == 1 or m.IS3D_full_switch == 1:
+        c.out3d = ContourSpec(m.IS3D_format, m.IS3D_File_freq, m.IS3D_time, m.IS3D_frequency, m.IS3D_start,
+                              m.IS3D_end, m.IS3D_unit_conv, tuple(m.IS3D_variable), m.IS3D_surf_switch,
+                              m.IS3D_full_switch, m.IS3D_grid_format)
     for tag, attr in (("2D", "dump2d"), ("3D", "dump3d")):
         if m[f"IS{tag}_dump_switch"] == 1:
             setattr(c, attr, DumpSpec(m[f"IS{tag}_dump_binary"], m[f"IS{tag}_dump_time"],

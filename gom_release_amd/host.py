@@ -47,6 +47,8 @@ def lib():
         L.gomgpu_get_state.argtypes = [vp, C.POINTER(abi.State)]
         L.gomgpu_set_forcing.argtypes = [vp, C.POINTER(abi.Forcing)]
         L.gomgpu_set_boundary.argtypes = [vp, C.POINTER(abi.Boundary)]
+        L.gomgpu_snapshot.argtypes = [vp, C.POINTER(abi.State)]
+        L.gomgpu_snapshot_wait.argtypes = [vp]
         L.gomgpu_get_forcing.argtypes = [vp, C.POINTER(abi.ForcingOut)]
         L.gomgpu_stage.argtypes = [vp, C.c_int]
         L.gomgpu_step.argtypes = [vp]
@@ -106,7 +108,7 @@ EXPORTED = ["gomgpu_create", "gomgpu_set_state", "gomgpu_set_forcing", "gomgpu_s
             "gomgpu_batch_last_error",
             "gomgpu_batch_destroy", "gomgpu_pcg_solve", "gomgpu_pcg_workspace_bytes",
             "gomgpu_pcg_last_error", "gomgpu_pcg_solve_noalloc", "gomgpu_elm_record",
-            "gomgpu_elm_departures", "gomgpu_elm_near_edge"]
+            "gomgpu_elm_departures", "gomgpu_elm_near_edge", "gomgpu_snapshot", "gomgpu_snapshot_wait"]
 
 HIP_H2D, HIP_D2H = 1, 2
 _BND_INT = {"harmonic_first", "series_first", "ob_eta_type", "ob_eta_id", "ob_tracer_series", "qb_q_series",
@@ -225,6 +227,19 @@ class GpuModel:
                 setattr(b, name, v)
         self._bnd_keep = keep
         self._check(self._L.gomgpu_set_boundary(self._ctx, C.byref(b)))
+
+    def snapshot(self, names: Sequence[str]):
+        """Start an asynchronous snapshot of the named state fields (gomgpu_snapshot); returns
+        a callable that completes it (gomgpu_snapshot_wait, from any thread) and returns the
+        fields."""
+        s = abi.state_arrays(self.params)
+        st = s.struct(list(names))
+        self._check(self._L.gomgpu_snapshot(self._ctx, C.byref(st)))
+
+        def wait():
+            self._check(self._L.gomgpu_snapshot_wait(self._ctx))
+            return s
+        return wait
 
     def get_forcing(self, forcing: "abi.ForcingArrays", only: Optional[Sequence[str]] = None) -> None:
         """Download the device's forcing arrays (the wind stage's outputs among them)."""

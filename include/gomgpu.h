@@ -28,6 +28,9 @@
  *   gomgpu_stage(ctx, GOMGPU_STAGE_EOS)       calculate_density_full/_linear main.f90:261,264
  *   gomgpu_stage(ctx, GOMGPU_STAGE_UPDATE)    update_variables            main.f90:272
  *   gomgpu_step(ctx)                          all of the above, in that order
+ *   gomgpu_stage(ctx, GOMGPU_STAGE_BARS)      calculate_vertically_averaged_values, the node
+ *                                             depth means (prepare_gom.f90:172, at start-up;
+ *                                             in a step the UPDATE stage forms them)
  *
  * Layout conventions (identical to the reference module arrays, so a Fortran caller passes
  * its allocatables with c_loc): every 2-D array is Fortran column-major (layer, entity),
@@ -250,7 +253,8 @@ enum gomgpu_stage_id {
   GOMGPU_STAGE_TURBULENCE = 10,
   GOMGPU_STAGE_WIND = 11,
   GOMGPU_STAGE_BOUNDARY = 12,
-  GOMGPU_NUM_STAGES = 13
+  GOMGPU_STAGE_BARS = 13,
+  GOMGPU_NUM_STAGES = 14
 };
 
 enum gomgpu_status {
@@ -314,6 +318,16 @@ int gomgpu_step(gomgpu_ctx *ctx);
 int gomgpu_run(gomgpu_ctx *ctx, int nsteps);
 /* Synchronise and download the non-NULL fields. */
 int gomgpu_get_state(gomgpu_ctx *ctx, gomgpu_state *state);
+/* Snapshot of the non-NULL fields of `dst` as they stand after the work already submitted,
+ * without waiting for it: the device copies them on the step's stream and downloads the copy
+ * on a stream of its own, so the following steps can be submitted (and run) meanwhile.
+ * gomgpu_snapshot_wait completes it: it waits for the download and writes the fields into
+ * the arrays `dst` pointed to, which must stay valid until then. One snapshot is in flight
+ * per context (a new one completes the previous first); the two calls may come from
+ * different threads. No counterpart in the reference, which writes its output files from
+ * the host arrays between steps (write_output_files.f90); for asynchronous output. */
+int gomgpu_snapshot(gomgpu_ctx *ctx, const gomgpu_state *dst);
+int gomgpu_snapshot_wait(gomgpu_ctx *ctx);
 /* Synchronise and download the non-NULL forcing arrays (the wind stage writes Gamma_T,
  * the wind stress, air_p1 and the hurricane fields on the device; a Fortran host reads
  * them back for its output, as write_tser_out_files.f90:312 reads air_p1). */

@@ -17,6 +17,7 @@ module gomgpu_iface
   integer(c_int), parameter :: GOMGPU_STAGE_TURBULENCE = 10
   integer(c_int), parameter :: GOMGPU_STAGE_WIND = 11
   integer(c_int), parameter :: GOMGPU_STAGE_BOUNDARY = 12
+  integer(c_int), parameter :: GOMGPU_STAGE_BARS = 13
   integer(c_int), parameter :: GOMGPU_FP64 = 0
   integer(c_int), parameter :: GOMGPU_E_ARG = 1
   integer(c_int), parameter :: GOMGPU_MIXED = 1

@@ -2659,6 +2659,14 @@ int oracle_update(const gomgpu_mesh *m, const gomgpu_params *p, gomgpu_state *s,
   memcpy(s->dzhalf_face, s->dzhalf_face_new, sizeof(double) * (L + 1) * J);
   memcpy(s->dz_node, s->dz_node_new, sizeof(double) * L * P);
   memcpy(s->dzhalf_node, s->dzhalf_node_new, sizeof(double) * (L + 1) * P);
+  return oracle_bars(m, p, s, f);
+}
+
+/* Depth means at nodes: update_variables.f90:37-74, the same as
+ * calculate_vertically_averaged_values, which prepare_gom.f90:172 calls at start-up. */
+int oracle_bars(const gomgpu_mesh *m, const gomgpu_params *p, gomgpu_state *s, const gomgpu_forcing *f) {
+  (void)f;
+  const int P = p->maxnod, L = p->maxlayer;
   for (int i = 1; i <= P; i++) {
     int tl = R1(s->top_layer_at_node, i), bl = R1(m->bottom_layer_at_node, i);
     int nv = tl - bl + 1;

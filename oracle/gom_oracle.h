@@ -48,6 +48,8 @@ int oracle_eos(const gomgpu_mesh *m, const gomgpu_params *p, gomgpu_state *s,
                const gomgpu_forcing *f);
 int oracle_update(const gomgpu_mesh *m, const gomgpu_params *p, gomgpu_state *s,
                   const gomgpu_forcing *f);
+/* The node depth means alone (calculate_vertically_averaged_values, prepare_gom.f90:172). */
+int oracle_bars(const gomgpu_mesh *m, const gomgpu_params *p, gomgpu_state *s, const gomgpu_forcing *f);
 /* Vertical turbulence closure (turbulence_closure_v1.f90:9-492); see gom_oracle.c. */
 int oracle_turbulence(const gomgpu_mesh *m, const gomgpu_params *p, gomgpu_state *s,
                       const gomgpu_forcing *f);

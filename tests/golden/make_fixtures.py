@@ -262,6 +262,32 @@ def make_formats(outdir: str) -> str:
     return path
 
 
+PLOT_PREFIXES = ("tec2D_", "tec3D_surf_", "vtk2D_", "vtk3D_surf_", "eta_tser", "H_tser", "u_tser", "v_tser",
+                 "salt_tser", "temp_tser")
+
+
+def make_plots(outdir: str) -> str:
+    """plots_estuary.npz: the Tecplot, VTK and time-series files of the two runs of
+    golden_io.plots_cases, as the reference build wrote them."""
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    from golden_io import PLOTS, plots_cases
+    out = {}
+    with tempfile.TemporaryDirectory() as tmp:
+        for run, case in plots_cases().items():
+            d = os.path.join(tmp, run)
+            gomio.write_project(case, d)
+            refdump.run_reference(d, os.path.join(d, "dump"), case.ndt, case.ndt)
+            for fn in sorted(os.listdir(os.path.join(d, "output"))):
+                if fn.startswith(PLOT_PREFIXES):
+                    out[f"{run}/files/{fn}"] = np.frombuffer(open(os.path.join(d, "output", fn), "rb").read(),
+                                                             np.uint8)
+    path = os.path.join(outdir, "plots_estuary.npz")
+    np.savez_compressed(path, **out)
+    with open(os.path.join(outdir, "plots_estuary.json"), "w") as f:
+        json.dump(dict(case="plots_estuary", kind=PLOTS["kind"], args=PLOTS["args"]), f, indent=1)
+    return path
+
+
 def provenance(outdir: str) -> None:
     ver = subprocess.run(["/opt/rocm/bin/amdflang", "--version"], capture_output=True, text=True).stdout
     meta = dict(reference="GOM_Rv1.1.0/source", compiler=ver.splitlines()[0],
@@ -288,3 +314,6 @@ if __name__ == "__main__":
     if only is None or "formats_estuary" in only:
         p = make_formats(outdir)
         print("formats_estuary", os.path.getsize(p) // 1024, "KiB")
+    if only is None or "plots_estuary" in only:
+        p = make_plots(outdir)
+        print("plots_estuary", os.path.getsize(p) // 1024, "KiB")

@@ -157,6 +157,42 @@ def formats_cases():
     return a, b_of
 
 
+PLOTS = dict(kind="estuary", args=dict(ncol=12, nrep=2, nlayer=4, ndt=10, river_q=50.0,
+                                        dry_land=True, tide_phase=-90.0))
+
+
+def plots_cases():
+    """The two runs of plots_estuary.npz: A with time series at cells and Tecplot + VTK
+    output in 2-D and of the 3-D surface (files rolling over every two outputs); B with time
+    series at nodes, 2-D Tecplot only in km and minutes, 3-D surface VTK only in seconds."""
+    from dataclasses import replace
+    from gom_release_amd import gomio, meshgen
+    a = meshgen.estuary_case(**PLOTS["args"])
+    stations = [(3, 5, "st_a"), (40, 30, "st_b"), (57, 45, "st_c")]
+    a.tser = gomio.TserSpec(stations=stations, hloc=1, time=4, frequency=1, format=3)
+    a.out2d = gomio.ContourSpec(format=3, file_freq=4, time=4, frequency=2, unit_conv=1.0)
+    a.out3d = gomio.ContourSpec(format=3, file_freq=4, time=3, frequency=2, unit_conv=0.001, surf=1, full=0)
+    b = replace(a, ndt=6,
+                tser=gomio.TserSpec(stations=stations[:2], hloc=2, time=2, frequency=2, format=3),
+                out2d=gomio.ContourSpec(format=1, file_freq=99999999, time=2, frequency=1, start=2, end=5,
+                                        unit_conv=0.001, variables=(1, 0, 1, 1, 0, 1)),
+                out3d=gomio.ContourSpec(format=2, file_freq=99999999, time=1, frequency=3, unit_conv=1.0,
+                                        surf=1, full=0))
+    return {"A": a, "B": b}
+
+
+class PlotsGolden:
+    """plots_estuary.npz: the contour and time-series files the reference wrote (bytes)."""
+
+    def __init__(self):
+        z = np.load(os.path.join(GOLDEN, "plots_estuary.npz"))
+        self.data = {k: z[k] for k in z.files}
+
+    def files(self, run: str) -> Dict[str, bytes]:
+        pre = f"{run}/files/"
+        return {k[len(pre):]: v.tobytes() for k, v in self.data.items() if k.startswith(pre)}
+
+
 class FormatsGolden:
     """formats_estuary.npz: reference-written files (bytes) and the states behind them."""
 
