@@ -295,9 +295,9 @@ class GpuModel:
         return int(c[0]), int(c[1])
 
     def timings(self, enable: bool = True) -> Dict[str, float]:
-        names = (C.c_char_p * 16)()
-        ms = (C.c_double * 16)()
-        n = self._L.gomgpu_timings(self._ctx, 1 if enable else 0, names, ms, 16)
+        names = (C.c_char_p * 32)()
+        ms = (C.c_double * 32)()
+        n = self._L.gomgpu_timings(self._ctx, 1 if enable else 0, names, ms, 32)
         return {names[i].decode(): ms[i] for i in range(n)}
 
     def last_error(self) -> str:
