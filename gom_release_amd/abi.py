@@ -66,6 +66,12 @@ STAGES = ["friction", "elm", "momentum", "freesurf", "velocity", "layers", "node
           "transport", "eos", "update", "turbulence", "wind", "boundary", "bars"]
 
 
+class Holland(C.Structure):
+    """gomgpu_holland: a Holland cyclone generated on the device."""
+    _fields_ = [(n, C.c_double) for n in ("p_min", "p_env", "r_max", "b", "coriolis", "rho_air",
+                                           "surface_factor", "vx", "vy")]
+
+
 class Boundary(C.Structure):
     """gomgpu_boundary: the tables of the device boundary forcing (GOMGPU_STAGE_BOUNDARY)."""
     _fields_ = [("num_harmonic_sets", C.c_int32), ("harmonic_first", PI), ("tidal_phase_shift", PD),

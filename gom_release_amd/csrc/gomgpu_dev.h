@@ -62,8 +62,9 @@ struct DevForcing {
   GOMGPU_FORCING_REAL(DF_R)
 #undef DF_R
   // per-step scalars in device memory (so a captured step stays valid): [0] baroclinic
-  // spin-up factor, [1] tidal spin-up factor, [2] elapsed time, [4] wind spin-up factor,
-  // [5] hurricane phase (0 keep, 1 interpolate, 2 next snapshot)
+  // spin-up factor, [1] tidal spin-up factor, [2] elapsed time, [3] and [7] x and y of a
+  // generated cyclone's centre, [4] wind spin-up factor, [5] hurricane phase (0 keep,
+  // 1 interpolate, 2 next snapshot), [6] julian day
   const double *scal;
 };
 

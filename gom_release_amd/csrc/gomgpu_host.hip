@@ -26,6 +26,7 @@ __global__ void k_mom_rhs(DevMesh, DevState, DevForcing, DevWork, DevDims, MomPa
 __global__ void k_mom_solve(DevMesh, DevState, DevForcing, DevWork, DevDims, MomParams, int *);
 __global__ void k_river_velocity(DevMesh, DevState, DevForcing, DevDims, double *);
 __global__ void k_hurricane_nodes(DevForcing, int, double);
+__global__ void k_holland_next(DevMesh, DevForcing, gomgpu_holland, int);
 __global__ void k_wind_faces(DevMesh, DevForcing, int, double, int);
 __global__ void k_fs_face(DevMesh, DevState, DevWork, DevDims, double);
 __global__ void k_fs_cell(DevMesh, DevState, DevForcing, DevWork, DevDims, double, double, int *);
@@ -168,6 +169,9 @@ struct gomgpu_ctx {
   bool bnd_on = false;
   DevBnd bnd{};
   int bnd_nt = 1;
+  // a Holland cyclone generated on the device (gomgpu_set_holland)
+  bool holland_on = false;
+  gomgpu_holland holland{};
   // asynchronous state snapshots (gomgpu_snapshot): a device copy of the fields taken on the
   // step's stream, then copied to pinned host memory on a stream of its own
   struct SnapItem {
@@ -827,11 +831,11 @@ extern "C" int gomgpu_set_forcing(gomgpu_ctx *c, const gomgpu_forcing *f) {
   c->h_stage[off] = f->spinup_function_baroclinic;
   c->h_stage[off + 1] = f->spinup_function_tide;
   c->h_stage[off + 2] = f->elapsed_time;
-  c->h_stage[off + 3] = 0.0;
+  c->h_stage[off + 3] = f->hurricane_centre_x;
   c->h_stage[off + 4] = f->spinup_function_wind;
   c->h_stage[off + 5] = f->hurricane_phase;
   c->h_stage[off + 6] = f->julian_day;
-  c->h_stage[off + 7] = 0.0;
+  c->h_stage[off + 7] = f->hurricane_centre_y;
   seg.dst[seg.count] = (double *)c->f.scal;
   seg.off[seg.count] = (long)off;
   seg.n[seg.count] = GOM_NSCAL;
@@ -937,6 +941,17 @@ extern "C" int gomgpu_set_boundary(gomgpu_ctx *c, const gomgpu_boundary *t) {
   b.tracers = tracers ? 1 : 0;
   c->bnd_nt = Nt;
   c->bnd_on = true;
+  return 0;
+}
+
+extern "C" int gomgpu_set_holland(gomgpu_ctx *c, const gomgpu_holland *h) {
+  if (!c || !h) return GOMGPU_E_ARG;
+  if (c->p.hurricane_flag != 1) return seterr(c, GOMGPU_E_ARG, "gomgpu_set_holland: hurricane_flag is not 1");
+  if (!c->batches.empty()) return seterr(c, GOMGPU_E_ARG, "gomgpu_set_holland: the context is in a batch");
+  if (!(h->r_max > 0.0) || !(h->rho_air > 0.0) || !(h->p_env > h->p_min))
+    return seterr(c, GOMGPU_E_ARG, "gomgpu_set_holland: needs r_max > 0, rho_air > 0 and p_env > p_min");
+  c->holland = *h;
+  c->holland_on = true;
   return 0;
 }
 
@@ -1288,6 +1303,7 @@ static int stage_impl(gomgpu_ctx *c, int id) {
       // read_hurricane_ser_1.f90 (node fields and face winds) and the hurricane branch of
       // calculate_wind_stress.f90:88-114; the step's phase and spin-up are device scalars
       if (p.hurricane_flag != 1) break;
+      if (c->holland_on) k_holland_next<<<nblk(d.P, BS), BS, 0, st>>>(c->m, c->f, c->holland, d.P);
       k_hurricane_nodes<<<nblk(d.P, BS), BS, 0, st>>>(c->f, d.P, (double)p.hurricane_read_interval);
       k_wind_faces<<<nblk(d.J, BS), BS, 0, st>>>(c->m, c->f, d.J, p.rho_a / p.rho_o, p.wind_formular);
       break;

@@ -1837,6 +1837,28 @@ __global__ void k_river_velocity(DevMesh m, DevState s, DevForcing f, DevDims d,
 // (read_hurricane_ser_1.f90:18-46); phase 1 adds one interval's increment to the
 // interpolated field (read_hurricane_ser_1.f90:48-60), so the field is the running sum the
 // reference accumulates; phase 0 (outside the hurricane window) leaves everything.
+// The next hurricane snapshot of a Holland cyclone (gomgpu_holland), at a step whose phase
+// takes it (scal[5] == 2), for the centre (scal[3], scal[7]); one thread per node. The
+// arithmetic is the host generator's (meshgen.HollandTrack.at), operation for operation.
+__global__ void k_holland_next(DevMesh m, DevForcing f, gomgpu_holland h, int P) {
+  const int i = gom_block() * blockDim.x + threadIdx.x;
+  if (i >= P || f.scal[5] != 2.0) return;
+  const double cx = f.scal[3], cy = f.scal[7];
+  const double dp = (h.p_env - h.p_min) * 100.0;
+  const double dx = m.x_node[i] - cx, dy = m.y_node[i] - cy;
+  const double r = hypot(dx, dy);
+  const double rs = r > 1.0 ? r : 1.0;
+  const double s = pow(h.r_max / rs, h.b);
+  const double es = exp(-s);
+  const double p = h.p_min + (h.p_env - h.p_min) * es;
+  const double rf2 = rs * h.coriolis / 2.0;
+  const double vg = sqrt(h.b * dp * s * es / h.rho_air + rf2 * rf2) - rs * h.coriolis / 2.0;
+  const double vs = r > 1.0 ? h.surface_factor * vg : 0.0;
+  const_cast<double *>(f.wind_u_next)[i] = -vs * dy / rs + h.vx;
+  const_cast<double *>(f.wind_v_next)[i] = vs * dx / rs + h.vy;
+  const_cast<double *>(f.air_p_next)[i] = p * 100.0;
+}
+
 __global__ void k_hurricane_nodes(DevForcing f, int P, double ninterval) {
   int i = gom_block() * blockDim.x + threadIdx.x;
   if (i >= P) return;

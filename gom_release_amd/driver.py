@@ -36,6 +36,10 @@ class HostForcing:
         # set when the model computes the boundary forcing itself (GOMGPU_STAGE_BOUNDARY):
         # then a step needs only its scalars
         self.device_boundary = False
+        # set when the model generates the Holland cyclone's snapshots itself: then a read step
+        # sends the centre of its snapshot, not the node fields
+        self.device_holland = False
+        self._snap_index = 0
         if case.hurricane_flag == 1:
             # read_hurricane_ser_0.f90: the first snapshot pair; the interpolated field,
             # air_p1 and the face winds start at zero (allocate_variables.f90:507-519)
@@ -49,9 +53,11 @@ class HostForcing:
 
     def _next_snapshot(self, it: int):
         try:
-            return next(self._snaps)
+            snap = next(self._snaps)
         except StopIteration:
             raise RuntimeError(f"the hurricane fields end before step {it}") from None
+        self._snap_index += 1
+        return snap
 
     def advance(self, it: int, eta_node: Optional[np.ndarray] = None) -> "abi.ForcingArrays":
         """Fill the forcing of step `it`. With eta_node the river face velocity is computed
@@ -71,7 +77,14 @@ class HostForcing:
         if c.hurricane_flag == 1:
             ph = fz.hurricane_phase(it, self.julian_day, self.window, self.params.hurricane_read_interval)
             f.scalars["hurricane_phase"] = float(ph)
-            if ph == 2:
+            if ph == 2 and self.device_holland:
+                tr = c.hurricane
+                if self._snap_index >= len(tr):
+                    raise RuntimeError(f"the hurricane fields end before step {it}")
+                cx, cy = tr.centre(tr.times[self._snap_index])
+                self._snap_index += 1
+                f.scalars["hurricane_centre_x"], f.scalars["hurricane_centre_y"] = cx, cy
+            elif ph == 2:
                 f["wind_u_next"], f["wind_v_next"], f["air_p_next"] = self._next_snapshot(it)
                 self.uploads += ["wind_u_next", "wind_v_next", "air_p_next"]
         if self.device_boundary:
@@ -151,7 +164,7 @@ class Scenario:
     """One scenario advanced on the device."""
 
     def __init__(self, case: Case, precision: int = 0, wind=(0.0, 0.0), model: Optional[host.GpuModel] = None,
-                 prepared=None, model_factory=None, device_boundary: bool = True):
+                 prepared=None, model_factory=None, device_boundary: bool = True, device_holland: bool = True):
         self.case = case
         if prepared is None:
             self.params, self.mesh, state0 = prep.build(case, precision)
@@ -173,6 +186,12 @@ class Scenario:
         if device_boundary and hasattr(self.model, "set_boundary") and not self.host_river:
             self.model.set_boundary(fz.boundary_tables(case))
             self.host_forcing.device_boundary = True
+        # a Holland cyclone's snapshots generated on the device
+        from .meshgen import HollandTrack
+        if device_holland and case.hurricane_flag == 1 and isinstance(case.hurricane, HollandTrack) \
+                and hasattr(self.model, "set_holland"):
+            self.model.set_holland(case.hurricane)
+            self.host_forcing.device_holland = True
         self.it = 0
 
     def stage_forcing(self) -> None:

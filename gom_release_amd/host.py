@@ -48,6 +48,7 @@ def lib():
         L.gomgpu_set_forcing.argtypes = [vp, C.POINTER(abi.Forcing)]
         L.gomgpu_set_boundary.argtypes = [vp, C.POINTER(abi.Boundary)]
         L.gomgpu_snapshot.argtypes = [vp, C.POINTER(abi.State)]
+        L.gomgpu_set_holland.argtypes = [vp, C.POINTER(abi.Holland)]
         L.gomgpu_snapshot_wait.argtypes = [vp]
         L.gomgpu_get_forcing.argtypes = [vp, C.POINTER(abi.ForcingOut)]
         L.gomgpu_stage.argtypes = [vp, C.c_int]
@@ -108,7 +109,7 @@ EXPORTED = ["gomgpu_create", "gomgpu_set_state", "gomgpu_set_forcing", "gomgpu_s
             "gomgpu_batch_last_error",
             "gomgpu_batch_destroy", "gomgpu_pcg_solve", "gomgpu_pcg_workspace_bytes",
             "gomgpu_pcg_last_error", "gomgpu_pcg_solve_noalloc", "gomgpu_elm_record",
-            "gomgpu_elm_departures", "gomgpu_elm_near_edge", "gomgpu_snapshot", "gomgpu_snapshot_wait"]
+            "gomgpu_elm_departures", "gomgpu_elm_near_edge", "gomgpu_snapshot", "gomgpu_snapshot_wait", "gomgpu_set_holland"]
 
 HIP_H2D, HIP_D2H = 1, 2
 _BND_INT = {"harmonic_first", "series_first", "ob_eta_type", "ob_eta_id", "ob_tracer_series", "qb_q_series",
@@ -227,6 +228,14 @@ class GpuModel:
                 setattr(b, name, v)
         self._bnd_keep = keep
         self._check(self._L.gomgpu_set_boundary(self._ctx, C.byref(b)))
+
+    def set_holland(self, track) -> None:
+        """Generate the hurricane snapshots of a Holland cyclone (meshgen.HollandTrack) on the
+        device: the host then sends each read step's centre instead of the node fields."""
+        h = abi.Holland(p_min=track.p_min, p_env=track.p_env, r_max=track.r_max, b=track.b,
+                        coriolis=track.coriolis(), rho_air=track.rho_air, surface_factor=track.surface_factor,
+                        vx=track.velocity[0], vy=track.velocity[1])
+        self._check(self._L.gomgpu_set_holland(self._ctx, C.byref(h)))
 
     def snapshot(self, names: Sequence[str]):
         """Start an asynchronous snapshot of the named state fields (gomgpu_snapshot); returns

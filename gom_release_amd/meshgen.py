@@ -212,11 +212,16 @@ class HollandTrack:
     def __len__(self) -> int:
         return len(self.times)
 
+    def coriolis(self) -> float:
+        return float(2.0 * 7.292e-5 * np.sin(np.radians(self.lat)))
+
+    def centre(self, t: float):
+        return self.centre0[0] + self.velocity[0] * t, self.centre0[1] + self.velocity[1] * t
+
     def at(self, t: float):
-        f = 2.0 * 7.292e-5 * np.sin(np.radians(self.lat))
+        f = self.coriolis()
         dp = (self.p_env - self.p_min) * 100.0               # Pa
-        cx = self.centre0[0] + self.velocity[0] * t
-        cy = self.centre0[1] + self.velocity[1] * t
+        cx, cy = self.centre(t)
         dx, dy = self.x - cx, self.y - cy
         r = np.hypot(dx, dy)
         rs = np.maximum(r, 1.0)

@@ -154,7 +154,8 @@ extern "C" {
  * snapshot (read_hurricane_ser_1.f90:18-46, every hurricane_read_interval steps). */
 #define GOMGPU_FORCING_SCALARS(X) \
   X(elapsed_time) X(spinup_function_tide) X(spinup_function_baroclinic) \
-  X(spinup_function_wind) X(hurricane_phase) X(julian_day)
+  X(spinup_function_wind) X(hurricane_phase) X(julian_day) \
+  X(hurricane_centre_x) X(hurricane_centre_y)
 
 #define GOMGPU_DECL_RPTR(name, ext) double *name;
 #define GOMGPU_DECL_IPTR(name, ext) int32_t *name;
@@ -219,6 +220,25 @@ typedef struct gomgpu_boundary {
   double ref_salt, ref_temp;         /*   ref_salt / ref_temp */
   int32_t reference_diff_days;       /* calculate_reference_diff_days.f90: series times shift by it */
 } gomgpu_boundary;
+
+/* A Holland (1980) parametric cyclone generated on the device (gomgpu_set_holland): at each
+ * step that takes the next hurricane snapshot (hurricane_phase 2) the wind stage forms
+ * wind_u_next, wind_v_next and air_p_next itself, for the centre the step's forcing scalars
+ * hurricane_centre_x/_y give, instead of the host uploading 3 maxnod values. With
+ * s = (r_max / max(r, 1 m))^b the pressure is p_min + (p_env - p_min) e^-s [hPa, stored x100
+ * as read_hurricane_ser_0.f90:35 stores Pa], the gradient wind
+ * V = sqrt(b dp s e^-s / rho_air + (r f / 2)^2) - r f / 2 with dp = 100 (p_env - p_min) and
+ * f = coriolis, surface_factor V counter-clockwise about the centre (0 within 1 m of it) plus
+ * the translation (vx, vy). No counterpart in the reference, whose Holland routine
+ * (holland_storm_surge.f90) no call reaches; it reads the fields from hurricane_ser.inp. */
+typedef struct gomgpu_holland {
+  double p_min, p_env;          /* hPa */
+  double r_max, b;              /* m, Holland B */
+  double coriolis;              /* f [1/s] */
+  double rho_air;               /* kg/m3 */
+  double surface_factor;
+  double vx, vy;                /* translation velocity [m/s] */
+} gomgpu_holland;
 
 /* Destination of gomgpu_get_forcing: the forcing arrays as they are on the device
  * (the outputs of the wind stage among them); NULL fields are not copied. */
@@ -309,6 +329,10 @@ int gomgpu_set_forcing(gomgpu_ctx *ctx, const gomgpu_forcing *forcing);
  * device (GOMGPU_STAGE_BOUNDARY). No counterpart in the reference, whose host computes it
  * (main.f90:84-161); for callers that keep the forcing on the device. */
 int gomgpu_set_boundary(gomgpu_ctx *ctx, const gomgpu_boundary *tables);
+/* From now on the wind stage generates the next hurricane snapshot on the device
+ * (gomgpu_holland). No counterpart in the reference; for cyclone scenarios that would
+ * otherwise upload every snapshot. */
+int gomgpu_set_holland(gomgpu_ctx *ctx, const gomgpu_holland *cyclone);
 /* Run one stage (for per-stage parity) or the whole step. Both are asynchronous on the
  * context's stream except where a stage needs a device-to-host value (none in FP64). */
 int gomgpu_stage(gomgpu_ctx *ctx, int stage_id);

@@ -62,7 +62,8 @@ module gomgpu_iface
   end type gomgpu_params
   type, bind(C) :: gomgpu_forcing
     real(c_double) :: elapsed_time, spinup_function_tide, spinup_function_baroclinic, &
-      & spinup_function_wind, hurricane_phase, julian_day = 0.0d0
+      & spinup_function_wind, hurricane_phase, julian_day, hurricane_centre_x, &
+      & hurricane_centre_y = 0.0d0
     type(c_ptr) :: Gamma_T, wind_stress_normal, wind_stress_tangnt, air_p1, eta_at_ob_old, &
       & eta_at_ob_new, Qu_boundary, phi_sz, con_at_ob, con_at_Qb, Q_add, wind_u0, wind_v0, air_p0, &
       & wind_u1, wind_v1, wind_u2, wind_v2, air_p2, wind_u_next, wind_v_next, air_p_next, &

@@ -54,7 +54,8 @@ def _c_layout():
                        ("gomgpu_state", [n for n, _ in abi.STATE_REAL + abi.STATE_INT]),
                        ("gomgpu_params", abi.PARAMS_INT + abi.PARAMS_REAL),
                        ("gomgpu_forcing", abi.FORCING_SCALARS + [n for n, _ in abi.FORCING_REAL]),
-                       ("gomgpu_boundary", [n for n, _ in abi.Boundary._fields_])):
+                       ("gomgpu_boundary", [n for n, _ in abi.Boundary._fields_]),
+                       ("gomgpu_holland", [n for n, _ in abi.Holland._fields_])):
         lines.append(f'printf("{st} size %zu\\n", sizeof({st}));')
         for f in fields:
             lines.append(f'printf("{st} {f} %zu\\n", offsetof({st}, {f}));')
@@ -76,7 +77,7 @@ def test_ctypes_structs_match_c_layout():
     lay = _c_layout()
     for st, cls in (("gomgpu_mesh", abi.Mesh), ("gomgpu_state", abi.State),
                     ("gomgpu_params", abi.Params), ("gomgpu_forcing", abi.Forcing),
-                    ("gomgpu_boundary", abi.Boundary)):
+                    ("gomgpu_boundary", abi.Boundary), ("gomgpu_holland", abi.Holland)):
         assert C.sizeof(cls) == lay[(st, "size")], st
         for name, _ in cls._fields_:
             assert getattr(cls, name).offset == lay[(st, name)], (st, name)
