@@ -42,10 +42,12 @@ struct DevMesh {
   // indirect ones.
   const double *cg_x, *cg_y;
   const int32_t *cg_node, *cg_adj;  // node ids and neighbour cells (1-based, 0 = none)
-  // Per cell slot (TB layout, like facenum_at_cell): sign_in_outflow of the slot's face when
-  // the transport counts its flux (boundary_type_of_face != -1 or a river face), else 0 --
-  // one coalesced load in place of the sign and two loads indexed by the face.
-  const int32_t *slot_flow;
+  // Per cell, its 4 slots contiguous ([cell][slot], one 16-byte load each): slot_flow is the
+  // slot face's sign_in_outflow when the transport counts its flux (boundary_type_of_face !=
+  // -1 or a river face), else 0; cg_face is the slot's face id (1-based, 0 past the cell's
+  // faces), facenum_at_cell in this layout. With cg_adj they give a cell's stencil in three
+  // loads instead of twelve.
+  const int32_t *slot_flow, *cg_face;
 };
 
 struct DevState {

@@ -183,16 +183,23 @@ __device__ __forceinline__ void lim_hsums(const DevMesh &m, const TvdW<W> &w, co
   const int E = d.E, L2 = d.L + 2, J = d.J;
   sum1[0] = sum1[1] = 0.0;
   sum2[0] = sum2[1] = 0.0;
-  int N = m.tri_or_quad[e];
-  for (int l = 1; l <= N; l++) {
-    const int sa = TB(m.slot_flow, l, e, E);
+  const int4 fl4 = *(const int4 *)(m.slot_flow + 4 * (size_t)e);
+  const int4 fc4 = *(const int4 *)(m.cg_face + 4 * (size_t)e);
+  const int4 ad4 = *(const int4 *)(m.cg_adj + 4 * (size_t)e);
+  const int fl[4] = {fl4.x, fl4.y, fl4.z, fl4.w}, fc[4] = {fc4.x, fc4.y, fc4.z, fc4.w};
+  const int ad[4] = {ad4.x, ad4.y, ad4.z, ad4.w};
+  double c0[2] = {0.0, 0.0};
+  for (int t = 0; t < d.NT; t++) c0[t] = w.con[cix(t, k, e, L2, E)];
+#pragma unroll
+  for (int l = 0; l < 4; l++) {  // slots past the cell's faces hold flow 0
+    const int sa = fl[l];
     if (sa != 0) {
-      int j = TB(m.facenum_at_cell, l, e, E) - 1, ii = TB(m.adj_cellnum_at_cell, l, e, E);
+      int j = fc[l] - 1, ii = ad[l];
       double Q2 = w.fqd[frx(k, j, J)] * (double)sa;
       if (Q2 < 0.0) {
         for (int t = 0; t < d.NT; t++) {
           double ci = con_across(m, w, d, e, ii, k, t);
-          sum1[t] = sum1[t] + fabs(Q2) * (w.con[cix(t, k, e, L2, E)] - ci);
+          sum1[t] = sum1[t] + fabs(Q2) * (c0[t] - ci);
           sum2[t] = sum2[t] + fabs(Q2);
         }
       }
@@ -369,19 +376,27 @@ __device__ __forceinline__ void tvd_rhs_layer(const DevMesh &m, const DevState &
   const int E = d.E, L = d.L, L1 = d.L + 1, L2 = d.L + 2, J = d.J;
   const int NT = d.NT, Nt = a.Nt, sub = a.sub;
   const double dt2 = a.dt2;
-  int nv = tl - bl + 1, N = m.tri_or_quad[e];
+  int nv = tl - bl + 1;
   double ar = m.area[e];
   double sqi[2] = {0, 0}, sqii[2] = {0, 0}, shd[2] = {0, 0}, shl[2] = {0, 0};
-  for (int l = 1; l <= N; l++) {
-    const int sa = TB(m.slot_flow, l, e, E);
+  const int4 fl4 = *(const int4 *)(m.slot_flow + 4 * (size_t)e);
+  const int4 fc4 = *(const int4 *)(m.cg_face + 4 * (size_t)e);
+  const int4 ad4 = *(const int4 *)(m.cg_adj + 4 * (size_t)e);
+  const int fl[4] = {fl4.x, fl4.y, fl4.z, fl4.w}, fc[4] = {fc4.x, fc4.y, fc4.z, fc4.w};
+  const int ad[4] = {ad4.x, ad4.y, ad4.z, ad4.w};
+  double cc[2] = {0.0, 0.0};
+  for (int t = 0; t < NT; t++) cc[t] = w.con[cix(t, k, e, L2, E)];
+#pragma unroll
+  for (int l = 0; l < 4; l++) {  // slots past the cell's faces hold flow 0
+    const int sa = fl[l];
     if (sa == 0) continue;
-    int j = TB(m.facenum_at_cell, l, e, E) - 1, ii = TB(m.adj_cellnum_at_cell, l, e, E);
+    int j = fc[l] - 1, ii = ad[l];
     const double2 qd = *(const double2 *)(w.fqd + frx(k, j, J));   // Q, d
     const double2 ps = *(const double2 *)(w.fps + frx(k, j, J));   // PSI of the two tracers
     double Q2 = qd.x * (double)sa;
     double dfj = qd.y;
     for (int t = 0; t < NT; t++) {
-      double c0 = w.con[cix(t, k, e, L2, E)];
+      double c0 = cc[t];
       double ci = con_across(m, w, d, e, ii, k, t);
       if (Q2 > 0.0)
         sqi[t] = sqi[t] + fabs(Q2) * c0;
@@ -397,13 +412,13 @@ __device__ __forceinline__ void tvd_rhs_layer(const DevMesh &m, const DevState &
     double h_adv = dt2 * (sqii[t] - sqi[t]);
     double h_diff = dt2 * shd[t];
     double h_flux = 0.5 * dt2 * shl[t];
-    double rhs = ar * dzo * w.con[cix(t, k, e, L2, E)] + h_adv + h_diff;
+    double rhs = ar * dzo * cc[t] + h_adv + h_diff;
     if (a.hlim > 0) rhs = rhs + h_flux;
     x[t] = rhs;
   }
   if (nv != 1) {
     for (int t = 0; t < NT; t++) {
-      double ck = w.con[cix(t, k, e, L2, E)];
+      double ck = cc[t];
       double v_adv, vft = 0.0;
       if (k == bl) {
         double Q = A0(w.Qi, k, e, E), ckp = w.con[cix(t, k + 1, e, L2, E)];
