@@ -115,8 +115,8 @@ struct DevWork {
   double *fqd, *fps;      // (2, L, J) each: per face-layer (Q, d) and the PSI of both tracers
   double *phij;           // (L, J)
   double *Qi, *di, *phii; // (0:L, E)
-  double *PSIi;
-  double *s1a, *s2a;      // (L, E, NT) limiter sums per cell-layer
+  double *PSIi;           // (2, 0:L, E): the vertical limiter of both tracers side by side
+  double *s12;            // (2, NT, L, E): the limiter sums (s1a, s2a) of each tracer per cell-layer
   double *obk, *qbk;      // (L, OB, NT), (L, QB, NT)
   double *node_new;       // (L, P, NT)
   double *face_new;       // (L, J, NT)

@@ -564,8 +564,8 @@ extern "C" int gomgpu_create(const gomgpu_mesh *mesh, const gomgpu_params *param
         (rc = dalloc(c, &w.Qi, (L + 1) * E)) || (rc = dalloc(c, &w.di, (L + 1) * E)) ||
         (rc = dalloc(c, &w.phii, (L + 1) * E)) || (rc = dalloc(c, &w.htop, NT * 2 * E)) ||
         (rc = dalloc(c, &w.con2, NT * (L + 2) * E)) ||
-        (rc = dalloc(c, &w.PSIi, NT * (L + 1) * E)) || (rc = dalloc(c, &w.s1a, NT * L * E)) ||
-        (rc = dalloc(c, &w.s2a, NT * L * E)) || (rc = dalloc(c, &w.obk, NT * L * OB)) ||
+        (rc = dalloc(c, &w.PSIi, 2 * (L + 1) * E)) || (rc = dalloc(c, &w.s12, 2 * NT * L * E)) ||
+        (rc = dalloc(c, &w.obk, NT * L * OB)) ||
         (rc = dalloc(c, &w.qbk, NT * L * QB)) || (rc = dalloc(c, &w.node_new, NT * L * P)) ||
         (rc = dalloc(c, &w.face_new, NT * L * J)) || (rc = dalloc(c, &w.budget, 2 * ((E + 255) / 256) + 2)) ||
         (rc = dalloc(c, &w.err, 2)) || (rc = dalloc(c, &w.elm_near, 2)) || (rc = dalloc(c, &w.turb_kvf, (L + 1) * J)) ||
@@ -1829,8 +1829,8 @@ extern "C" long long gomgpu_footprint(const gomgpu_params *params) {
   add(2 * L * J, D); add(2 * L * J, D); add(L * J, D);      // fqd, fps, phij
   add((L + 1) * E, D); add((L + 1) * E, D); add((L + 1) * E, D);  // Qi, di, phii
   add(NT * 2 * E, D); add(NT * (L + 2) * E, D);             // htop, con2
-  add(NT * (L + 1) * E, D);                                 // PSIi
-  add(NT * L * E, D); add(NT * L * E, D);                   // s1a, s2a
+  add(2 * (L + 1) * E, D);                                  // PSIi
+  add(2 * NT * L * E, D);                                   // s12
   add(NT * L * OB, D); add(NT * L * QB, D);                 // obk, qbk
   add(NT * L * P, D); add(NT * L * J, D);                   // node_new, face_new
   add(2 * ((E + 255) / 256) + 2, D); add(2, sizeof(int));   // budget, err

@@ -21,7 +21,7 @@
 template <typename W>
 struct TvdW {
   double *con, *con_new, *fqd, *fps, *Qi, *di, *phii, *htop, *PSIi, *obk, *qbk, *tri_gam, *tvd_x;
-  W *phij, *s1a, *s2a, *node_new, *face_new;
+  W *phij, *s12, *node_new, *face_new;
 };
 template <typename W>
 static TvdW<W> tvd_view(const DevWork &w) {
@@ -40,8 +40,7 @@ static TvdW<W> tvd_view(const DevWork &w) {
   v.fqd = w.fqd;
   v.fps = w.fps;
   v.phij = (W *)w.phij;
-  v.s1a = (W *)w.s1a;
-  v.s2a = (W *)w.s2a;
+  v.s12 = (W *)w.s12;
   v.node_new = (W *)w.node_new;
   v.face_new = (W *)w.face_new;
   return v;
@@ -56,9 +55,13 @@ __device__ __forceinline__ size_t nix(int t, int k, int e, int L, int n) {  // (
 __device__ __forceinline__ size_t frx(int k, int j, int J) {  // (Q, d) or (PSI 1, PSI 2) of layer k
   return 2 * ((size_t)(k - 1) * J + j);
 }
-__device__ __forceinline__ size_t zix(int t, int k, int e, int L1, int n) {  // (0:L) fields
-  return ((size_t)t * L1 + k) * n + e;
-}
+// PSI_ik of level k (0:L) for the two tracers side by side: one 16-byte load for both
+__device__ __forceinline__ size_t pix(int k, int e, int n) { return 2 * ((size_t)k * n + e); }
+// The upwind sums (s1a, s2a) of layer k (1:L), per tracer t at 2 t, NT tracers per record
+__device__ __forceinline__ size_t six(int k, int e, int n, int NT) { return 2 * (size_t)NT * ((size_t)(k - 1) * n + e); }
+template <typename W> struct Pair;
+template <> struct Pair<double> { typedef double2 T; };
+template <> struct Pair<float> { typedef float2 T; };
 
 // con_cell from salt/temp (+ surface heat source), solve_transport_equation_v19.f90:88-173
 template <typename W>
@@ -259,8 +262,7 @@ __global__ void k_tvd_lim_cell(DevMesh m, DevState s, TvdW<W> w, DevDims d, int 
   int e = (int)(q % E), k = (int)(q / E);
   int tl = s.top_layer_at_element[e], bl = m.bottom_layer_at_element[e];
   if (tl == 0 || k < bl - 1 || k > tl) {  // zero from the first sub-cycle on
-    if (first)
-      for (int t = 0; t < d.NT; t++) w.PSIi[zix(t, k, e, L1, E)] = 0.0;
+    if (first) *(double2 *)(w.PSIi + pix(k, e, E)) = make_double2(0.0, 0.0);
     return;
   }
   int nv = tl - bl + 1;
@@ -288,15 +290,17 @@ __global__ void k_tvd_lim_cell(DevMesh m, DevState s, TvdW<W> w, DevDims d, int 
       }
     }
     lim_hsums(m, w, d, e, k, sum1, sum2);
+    typedef typename Pair<W>::T W2;
     for (int t = 0; t < d.NT; t++) {
-      w.s1a[nix(t, k, e, L, E)] = s1a[t] + sum1[t];
-      w.s2a[nix(t, k, e, L, E)] = s2a[t] + sum2[t];
+      W2 v;
+      v.x = (W)(s1a[t] + sum1[t]);
+      v.y = (W)(s2a[t] + sum2[t]);
+      *(W2 *)(w.s12 + six(k, e, E, d.NT) + 2 * t) = v;
     }
   }
   // r_ik of level k: computed for bl <= k <= tl - 1 (1.0 at bl - 1 and tl, where PSI_ik is 0)
   if (k == tl || k == bl - 1) {
-    if (first)
-      for (int t = 0; t < d.NT; t++) w.PSIi[zix(t, k, e, L1, E)] = 0.0;
+    if (first) *(double2 *)(w.PSIi + pix(k, e, E)) = make_double2(0.0, 0.0);
     return;
   }
   double sum1[2], sum2[2];
@@ -304,6 +308,7 @@ __global__ void k_tvd_lim_cell(DevMesh m, DevState s, TvdW<W> w, DevDims d, int 
     sum1[t] = w.htop[(size_t)(2 * t) * E + e];
     sum2[t] = w.htop[(size_t)(2 * t + 1) * E + e];
   }
+  double psi[2] = {0.0, 0.0};
   for (int t = 0; t < d.NT; t++) {
     double r;
     if (A0(w.Qi, tl, e, E) >= 0.0) {
@@ -322,8 +327,9 @@ __global__ void k_tvd_lim_cell(DevMesh m, DevState s, TvdW<W> w, DevDims d, int 
       double rt = (w.con[cix(t, kk - 1, e, L2, E)] - c0) * rt2;
       r = (rt == 0.0) ? 1.0 : rt1 / rt;
     }
-    w.PSIi[zix(t, k, e, L1, E)] = hlim != 0 ? limiter(hlim, r, A0(w.phii, k, e, E)) : 0.0;
+    psi[t] = hlim != 0 ? limiter(hlim, r, A0(w.phii, k, e, E)) : 0.0;
   }
+  *(double2 *)(w.PSIi + pix(k, e, E)) = make_double2(psi[0], psi[1]);
 }
 
 // Face ratio r_jk and PSI_jk. In the reference each adjacent cell whose outflow the face
@@ -353,10 +359,12 @@ __global__ void k_tvd_lim_face(DevMesh m, DevState s, TvdW<W> w, DevDims d, int 
       double sg = side == 1 ? 1.0 : -1.0;  // sign_in_outflow of face j seen from this cell
       if (Q * sg >= 0.0) {
         int other = TB(m.adj_cellnum_at_face, 3 - side, j, J);
+        typedef typename Pair<W>::T W2;
         for (int t = 0; t < d.NT; t++) {
           double ci = con_across(m, w, d, e, other, k, t);
-          double rt = w.s2a[nix(t, k, e, L, E)] * (ci - w.con[cix(t, k, e, L2, E)]);
-          r[t] = (rt == 0.0) ? 1.0 : w.s1a[nix(t, k, e, L, E)] / rt;
+          const W2 sv = *(const W2 *)(w.s12 + six(k, e, E, d.NT) + 2 * t);  // (s1a, s2a)
+          double rt = (double)sv.y * (ci - w.con[cix(t, k, e, L2, E)]);
+          r[t] = (rt == 0.0) ? 1.0 : (double)sv.x / rt;
         }
       }
     }
@@ -425,7 +433,7 @@ __device__ __forceinline__ void tvd_rhs_layer(const DevMesh &m, const DevState &
         double Ap = fabs(0.5 * (Q + fabs(Q))), Am = fabs(0.5 * (Q - fabs(Q)));
         v_adv = dt2 * (-(Ap * ck) + Am * ckp);
         if (a.vlim > 0) {
-          double ps = w.PSIi[zix(t, k, e, L1, E)];
+          double ps = w.PSIi[pix(k, e, E) + t];
           vft = 0.5 * dt2 * (-(ps * Ap * (ckp - ck)) + ps * Am * (ck - ckp));
         }
       } else if (k == tl) {
@@ -433,7 +441,7 @@ __device__ __forceinline__ void tvd_rhs_layer(const DevMesh &m, const DevState &
         double Ap = fabs(0.5 * (Q + fabs(Q))), Am = fabs(0.5 * (Q - fabs(Q)));
         v_adv = dt2 * (Ap * ckm - Am * ck);
         if (a.vlim > 0) {
-          double ps = w.PSIi[zix(t, k - 1, e, L1, E)];
+          double ps = w.PSIi[pix(k - 1, e, E) + t];
           vft = 0.5 * dt2 * (ps * Ap * (ck - ckm) - ps * Am * (ckm - ck));
         }
       } else {
@@ -443,7 +451,7 @@ __device__ __forceinline__ void tvd_rhs_layer(const DevMesh &m, const DevState &
         double A2p = fabs(0.5 * (Q2 + fabs(Q2))), A2m = fabs(0.5 * (Q2 - fabs(Q2)));
         v_adv = dt2 * (A1p * ckm - A2p * ck + A2m * ckp - A1m * ck);
         if (a.vlim > 0) {
-          double p1 = w.PSIi[zix(t, k - 1, e, L1, E)], p2 = w.PSIi[zix(t, k, e, L1, E)];
+          double p1 = w.PSIi[pix(k - 1, e, E) + t], p2 = w.PSIi[pix(k, e, E) + t];
           vft = 0.5 * dt2 * (p1 * A1p * (ck - ckm) - p2 * A2p * (ckp - ck) + p2 * A2m * (ck - ckp) - p1 * A1m * (ckm - ck));
         }
       }
