@@ -30,12 +30,13 @@ struct DevMesh {
 #undef DM_R
 #undef DM_I
   const double *Cdb_static;  // Manning drag from the still-water depth (bf_flag 1)
-  // Static node stencil of calculate_velocity_at_node.f90:67-138, built at create: for the
-  // jj-th adjacent cell of node i, its two faces touching the node (slots 2jj, 2jj+1) with
-  // their length, boundary factor and bottom layer, and the cell with its area and bottom
-  // layer; [slot][node] like the other tables. ns_weight is the node's sum of fw/L.
-  const int32_t *ns_face, *ns_fkb, *ns_cell, *ns_ckb;
-  const double *ns_Lf, *ns_fw, *ns_area, *ns_weight;
+  // Static node stencil of calculate_velocity_at_node.f90:67-138, built at create, one
+  // record per (jj-th adjacent cell, node i), [jj][node]: ns_face (face 0, face 1, bottom
+  // layer 0, bottom layer 1) of the cell's two faces touching the node, ns_Lf (length 0,
+  // length 1, boundary factor 0, boundary factor 1), ns_cell (the cell, its bottom layer)
+  // and ns_area; each a 16- or 8-byte load. ns_weight is the node's sum of fw/L.
+  const int32_t *ns_face, *ns_cell;
+  const double *ns_Lf, *ns_area, *ns_weight;
   // Cell records of the ELM walk, built at create: the x and y of a cell's nodes, the node
   // ids and the neighbour cells in slot order, 4 per cell, contiguous per cell
   // ([cell][slot]), so a visited cell costs a few aligned loads instead of a chain of

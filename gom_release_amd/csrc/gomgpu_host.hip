@@ -403,8 +403,9 @@ extern "C" int gomgpu_create(const gomgpu_mesh *mesh, const gomgpu_params *param
       rc = seterr(c, GOMGPU_E_ARG, "mesh connectivity missing");
       goto fail;
     }
-    std::vector<int32_t> face(2 * NB * P, 0), fkb(2 * NB * P, 0), cell(NB * P, 0), ckb(NB * P, 0);
-    std::vector<double> Lf(2 * NB * P, 1.0), fw(2 * NB * P, 0.0), area(NB * P, 0.0), weight(P, 0.0);
+    std::vector<int32_t> face(4 * NB * P, 0), cell(2 * NB * P, 0);
+    std::vector<double> Lf(4 * NB * P, 0.0), area(NB * P, 0.0), weight(P, 0.0);
+    for (size_t q = 0; q < Lf.size(); q += 4) Lf[q] = Lf[q + 1] = 1.0;  // lengths of unused slots
     for (long i = 0; i < P; i++) {
       int nadj = mesh->adj_cells_at_node[i];
       double wsum = 0.0;
@@ -417,16 +418,19 @@ extern "C" int gomgpu_create(const gomgpu_mesh *mesh, const gomgpu_params *param
           int sl = v > N ? v - N : v;  // start_end_node (set_geometry_1.f90:78-97)
           int ifn = mesh->facenum_at_cell[(sl - 1) + (long)isn * 4] - 1;
           double w = mesh->adj_cellnum_at_face[1 + (long)ifn * 2] == 0 ? 2.0 : 1.0;
-          long o = (long)(2 * (jj - 1) + (l - (N - 2))) * P + i;
-          face[o] = ifn;
-          fkb[o] = mesh->bottom_layer_at_face[ifn];
-          Lf[o] = mesh->face_length[ifn];
-          fw[o] = w;
+          // one record per (adjacent cell, node): (face 0, face 1, kb 0, kb 1) and
+          // (L 0, L 1, w 0, w 1), the two faces of the cell that touch the node
+          const int h = l - (N - 2);
+          long o = 4 * ((long)(jj - 1) * P + i);
+          face[o + h] = ifn;
+          face[o + 2 + h] = mesh->bottom_layer_at_face[ifn];
+          Lf[o + h] = mesh->face_length[ifn];
+          Lf[o + 2 + h] = w;
           wsum = wsum + w / mesh->face_length[ifn];
         }
         long oc = (long)(jj - 1) * P + i;
-        cell[oc] = isn;
-        ckb[oc] = mesh->bottom_layer_at_element[isn];
+        cell[2 * oc] = isn;
+        cell[2 * oc + 1] = mesh->bottom_layer_at_element[isn];
         area[oc] = mesh->area[isn];
       }
       weight[i] = wsum;
@@ -437,11 +441,8 @@ extern "C" int gomgpu_create(const gomgpu_mesh *mesh, const gomgpu_params *param
     if ((rc = dalloc(c, &ptr, vec.size())) || (rc = upload(c, ptr, vec.data(), Shape{1, (long)vec.size()}))) goto fail; \
     c->m.field = ptr;
     NS_UP(face, ns_face, int32_t, ip)
-    NS_UP(fkb, ns_fkb, int32_t, ip)
     NS_UP(cell, ns_cell, int32_t, ip)
-    NS_UP(ckb, ns_ckb, int32_t, ip)
     NS_UP(Lf, ns_Lf, double, dp)
-    NS_UP(fw, ns_fw, double, dp)
     NS_UP(area, ns_area, double, dp)
     NS_UP(weight, ns_weight, double, dp)
 #undef NS_UP

@@ -1741,17 +1741,20 @@ __global__ void GOM_WAVES(GOM_WAVES_NODE_VEL) k_node_vel(DevMesh m, DevState s, 
 #pragma unroll
     for (int q = 0; q < NV_CHUNK; q++) {
       const int jj = j0 + q < nadj ? j0 + q : j0;
-#pragma unroll
-      for (int h = 0; h < 2; h++) {
-        const size_t o = (size_t)(2 * jj + h) * P + i;
-        ifn[2 * q + h] = m.ns_face[o];
-        fkb[2 * q + h] = m.ns_fkb[o];
-        lf[2 * q + h] = m.ns_Lf[o];
-        fw[2 * q + h] = m.ns_fw[o];
-      }
       const size_t oc = (size_t)jj * P + i;
-      isn[q] = m.ns_cell[oc];
-      ckb[q] = m.ns_ckb[oc];
+      const int4 ff = *(const int4 *)(m.ns_face + 4 * oc);
+      const double2 ll = *(const double2 *)(m.ns_Lf + 4 * oc), ww = *(const double2 *)(m.ns_Lf + 4 * oc + 2);
+      const int2 cc = *(const int2 *)(m.ns_cell + 2 * oc);
+      ifn[2 * q] = ff.x;
+      ifn[2 * q + 1] = ff.y;
+      fkb[2 * q] = ff.z;
+      fkb[2 * q + 1] = ff.w;
+      lf[2 * q] = ll.x;
+      lf[2 * q + 1] = ll.y;
+      fw[2 * q] = ww.x;
+      fw[2 * q + 1] = ww.y;
+      isn[q] = cc.x;
+      ckb[q] = cc.y;
       car[q] = m.ns_area[oc];
     }
 #pragma unroll
