@@ -59,6 +59,7 @@ __global__ void k_eos_cell(DevMesh, DevState, DevDims, int *);
 __global__ void k_bars(DevMesh, DevState, DevDims);
 __global__ void k_turb_ri(DevMesh, DevState, DevWork, DevDims, TurbParams);
 __global__ void __launch_bounds__(256) k_turb_column(DevMesh, DevState, DevForcing, DevWork, DevDims, TurbParams, int *);
+__global__ void __launch_bounds__(256) k_turb_levels(DevMesh, DevState, DevForcing, DevWork, DevDims, TurbParams, int *);
 __global__ void k_turb_kv(DevMesh, DevState, DevWork, DevDims);
 __global__ void k_budget(DevMesh, DevState, DevDims, double *);
 __global__ void k_copy(double *, const double *, int);
@@ -1347,6 +1348,7 @@ static int stage_impl(gomgpu_ctx *c, int id) {
       tp.stab = p.stability_function;
       if (tp.flag == 3) {
         k_turb_column<<<nblk(d.J, LB(TURB_SOLVE)), LB(TURB_SOLVE), 0, st>>>(c->m, c->s, c->f, c->w, d, tp, err);
+        k_turb_levels<<<nblk(d.J, LB(TURB_SOLVE)), LB(TURB_SOLVE), 0, st>>>(c->m, c->s, c->f, c->w, d, tp, err);
       } else {
         k_turb_ri<<<nblk((long)d.L * d.J, LB(TURB_PREP)), LB(TURB_PREP), 0, st>>>(c->m, c->s, c->w, d, tp);
       }

@@ -70,12 +70,6 @@ __device__ __forceinline__ void alg_stress(const TurbParams &p, double q2, doubl
   qd2 = dminf(p.diffmax, dmaxf(p.bgdiff, b));
 }
 
-__device__ __forceinline__ double rz_level(const TurbParams &p, const DevState &s, int J, int j, int k, int kt) {
-  if (k >= kt) return 0.0;
-  double drhodz = (AL(s.rho_face, k + 1, j, J) - AL(s.rho_face, k, j, J)) / A0(s.dzhalf_face_new, k, j, J);
-  return p.g / p.rho_o * drhodz;
-}
-
 // Schemes 1 and 2 (turbulence_closure_v1.f90:75-141): per-level algebra, one thread per
 // (level, face).
 __global__ void k_turb_ri(DevMesh m, DevState s, DevWork w, DevDims d, TurbParams p) {
@@ -133,25 +127,18 @@ __global__ void k_turb_ri(DevMesh m, DevState s, DevWork w, DevDims d, TurbParam
   A0(w.turb_kvf, k, j, J) = kv;
 }
 
-// The stability-function diffusivities of one whole level (turbulence_closure_v1.f90:56-66):
-// level kb-1 takes level kb's state.
-__device__ __forceinline__ void level_diff(const TurbParams &p, const DevState &s, int J, int j, int l, int kb,
-                                           int kt, double &av, double &kv, double &qd, double &qd2) {
-  const int lc = l == kb - 1 ? kb : l;
-  alg_stress(p, A0(s.q2, lc, j, J), A0(s.xl, lc, j, J), rz_level(p, s, J, j, lc, kt), av, kv, qd, qd2);
-}
-
 // Scheme 3 (turbulence_closure_v1.f90:56-66, 158-427), one thread per face column:
 // * the two stop checks of the layer quantities (:158-215);
 // * the k and psi tridiagonal systems (Thomas in place in w.tk_q2tmp / w.tk_x, factors in
 //   w.tri_gam and w.tk_shear), each row's inputs -- the level diffusivities of the current
 //   k, l, the half-level k, l, the squared shear and (g/rho_o) drho/dz -- formed as the
-//   top-down elimination reaches it, with the upper level's values and node velocities
-//   carried from the row above;
-// * the length scale with Galperin's clipping and k, l back at whole levels (:351-390);
-// * the new diffusivities at every level, clamped to the column's range (:397-427).
-__global__ void __launch_bounds__(256) GOM_WAVES(GOM_WAVES_TURB) k_turb_column(DevMesh m, DevState s, DevForcing f, DevWork w, DevDims d,
-                                                     TurbParams p, int *err) {
+//   top-down elimination reaches it; each row reads the level below it once and carries
+//   the level above (its state, diffusivities, density and node velocities) from the row
+//   before, where level kb-1 takes level kb's state;
+// * k_turb_levels then completes the column.
+__global__ void __launch_bounds__(256) GOM_WAVES(GOM_WAVES_TURB) k_turb_column(DevMesh m, DevState s, DevForcing f,
+                                                                                DevWork w, DevDims d, TurbParams p,
+                                                                                int *err) {
   int j = gom_block() * blockDim.x + threadIdx.x;
   const int J = d.J;
   if (j >= J) return;
@@ -162,22 +149,17 @@ __global__ void __launch_bounds__(256) GOM_WAVES(GOM_WAVES_TURB) k_turb_column(D
   int a1 = TB(m.adj_cellnum_at_face, 1, j, J), a2 = TB(m.adj_cellnum_at_face, 2, j, J);
   double etam = a2 != 0 ? dmaxf(s.eta_cell_new[a1 - 1], s.eta_cell_new[a2 - 1]) : s.eta_cell_new[a1 - 1];
   double hf = m.h_face[j];
-  double wn = f.wind_stress_normal[j], wt = f.wind_stress_tangnt[j];
-  double q2fs = p.fs_coef * sqrt(wn * wn + wt * wt) / 2.0;
-  q2fs = dmaxf(q2fs, p.q2min);
-  double ubn = AL(s.un_face_new, kb, j, J), ubt = AL(s.vn_face_new, kb, j, J);
-  double q2bot = p.fs_coef * s.Cdb[j] * sqrt(ubn * ubn + ubt * ubt) / 2.0;
-  q2bot = dmaxf(q2bot, p.q2min);
-  double xlfs = dmaxf(p.xlmin1, AL(s.dz_face_new, kt, j, J) / 2.0 * (double)0.4f);
-  double xlbot = dmaxf(p.xlmin2, AL(s.dz_face_new, kb, j, J) / 2.0 * (double)0.4f);
   int nq = kt - kb + 1;
   double cm3 = p.cmiu0 * p.cmiu0 * p.cmiu0;
   const int P = d.P;
   const int n1 = TB(m.nodenum_at_face, 1, j, J) - 1, n2 = TB(m.nodenum_at_face, 2, j, J) - 1;
   // the stop checks of every layer first, as the reference's pass over the layers precedes
   // its solve
+  double xl_lo = A0(s.xl, kb - 1, j, J);
   for (int kl = kb; kl <= kt; kl++) {
-    double xlbt = (A0(s.xl, kl - 1, j, J) + A0(s.xl, kl, j, J)) / 2.0;
+    const double xl_hi = A0(s.xl, kl, j, J);
+    double xlbt = (xl_lo + xl_hi) / 2.0;
+    xl_lo = xl_hi;
     if (xlbt < p.xlmin2) latch(err, GOMGPU_ERR_TURB_XL, j + 1);
     double dzk = AL(s.dz_face_new, kl, j, J);
     double zctr = (kl == kt) ? p.MSL + etam - dzk / 2.0 : m.z_level[kl] - dzk / 2.0;
@@ -185,9 +167,13 @@ __global__ void __launch_bounds__(256) GOM_WAVES(GOM_WAVES_TURB) k_turb_column(D
     double xlmax = (double)0.4f * dists * distb / (hf + etam);
     if (xlmax <= 0.0) latch(err, GOMGPU_ERR_TURB_DIST, j + 1);
   }
-  // level kt's diffusivities and node velocities, carried down the rows
+  // level kt's state, diffusivities and node velocities, carried down the rows: row k reads
+  // level kl-1 once and takes level kl (and the density at kl+1) from the row above
+  double dzk = AL(s.dz_face_new, kt, j, J), dzhk = A0(s.dzhalf_face_new, kt, j, J);
+  double q2k = A0(s.q2, kt, j, J), xlk = A0(s.xl, kt, j, J);
+  double rhok = AL(s.rho_face, kt, j, J), rhop = 0.0;  // density at kl and kl+1
   double avk, kvk, qdk, qd2k;
-  level_diff(p, s, J, j, kt, kb, kt, avk, kvk, qdk, qd2k);
+  alg_stress(p, q2k, xlk, 0.0, avk, kvk, qdk, qd2k);
   double uup = (A0(s.u_node, kt, n1, P) + A0(s.u_node, kt, n2, P)) / 2.0;
   double vup = (A0(s.v_node, kt, n1, P) + A0(s.v_node, kt, n2, P)) / 2.0;
   // ---- the k and psi equations (:233-276 and :281-347) in one top-down pass: the two
@@ -199,14 +185,24 @@ __global__ void __launch_bounds__(256) GOM_WAVES(GOM_WAVES_TURB) k_turb_column(D
   double den = 0.0, cprev = 0.0, x = 0.0;     // k system
   double denp = 0.0, cprevp = 0.0, xp = 0.0;  // psi system
   for (int k = 1; k <= nq; k++) {
-    int kl = kt - k + 1;
-    double dzk = AL(s.dz_face_new, kl, j, J);
-    double dzhk = A0(s.dzhalf_face_new, kl, j, J), dzhm = A0(s.dzhalf_face_new, kl - 1, j, J);
-    double avm, kvm, qdm, qd2m;  // level kl-1
-    level_diff(p, s, J, j, kl - 1, kb, kt, avm, kvm, qdm, qd2m);
+    const int kl = kt - k + 1, km = kl - 1;
+    const double dzhm = A0(s.dzhalf_face_new, km, j, J);
+    const double q2m = A0(s.q2, km, j, J), xlm = A0(s.xl, km, j, J);
+    double dzm = 0.0, rhom = 0.0;
+    double avm, kvm, qdm, qd2m;  // level kl-1; level kb-1 takes level kb's
+    if (km >= kb) {
+      dzm = AL(s.dz_face_new, km, j, J);
+      rhom = AL(s.rho_face, km, j, J);
+      alg_stress(p, q2m, xlm, p.g / p.rho_o * ((rhok - rhom) / dzhm), avm, kvm, qdm, qd2m);
+    } else {
+      avm = avk;
+      kvm = kvk;
+      qdm = qdk;
+      qd2m = qd2k;
+    }
     // layer kl (turbulence_closure_v1.f90:158-205)
-    double q2bt = (A0(s.q2, kl - 1, j, J) + A0(s.q2, kl, j, J)) / 2.0;
-    double xlbt = (A0(s.xl, kl - 1, j, J) + A0(s.xl, kl, j, J)) / 2.0;
+    double q2bt = (q2m + q2k) / 2.0;
+    double xlbt = (xlm + xlk) / 2.0;
     double udown = (A0(s.u_node, kl - 1, n1, P) + A0(s.u_node, kl - 1, n2, P)) / 2.0;
     double vdown = (A0(s.v_node, kl - 1, n1, P) + A0(s.v_node, kl - 1, n2, P)) / 2.0;
     double dudz = (uup - udown) / dzk, dvdz = (vup - vdown) / dzk;
@@ -215,9 +211,8 @@ __global__ void __launch_bounds__(256) GOM_WAVES(GOM_WAVES_TURB) k_turb_column(D
     if (kl == kt || kl == kb) {
       drhodz = 0.0;
     } else {
-      double rho_up = AL(s.rho_face, kl, j, J) + dzk / 2.0 / dzhk * (AL(s.rho_face, kl + 1, j, J) - AL(s.rho_face, kl, j, J));
-      double rho_down = AL(s.rho_face, kl - 1, j, J) + AL(s.dz_face_new, kl - 1, j, J) / 2.0 / dzhm *
-                                                           (AL(s.rho_face, kl, j, J) - AL(s.rho_face, kl - 1, j, J));
+      double rho_up = rhok + dzk / 2.0 / dzhk * (rhop - rhok);
+      double rho_down = rhom + dzm / 2.0 / dzhm * (rhok - rhom);
       drhodz = (rho_up - rho_down) / dzk;
     }
     double rz = p.g / p.rho_o * drhodz;
@@ -340,9 +335,43 @@ __global__ void __launch_bounds__(256) GOM_WAVES(GOM_WAVES_TURB) k_turb_column(D
     qd2k = qd2m;
     uup = udown;
     vup = vdown;
+    dzk = dzm;
+    dzhk = dzhm;
+    q2k = q2m;
+    xlk = xlm;
+    rhop = rhok;
+    rhok = rhom;
   }
+}
+
+// The rest of scheme 3 (turbulence_closure_v1.f90:268-278, 351-427), one thread per face
+// column, after k_turb_column's elimination: the back substitutions of both systems from
+// the bottom row's solutions (w.tk_q2tmp, w.tk_x at layer kb), the length scale, k and l at
+// whole levels and the new diffusivities. A kernel of its own so that neither half holds
+// the other's registers; the surface and bottom values of k and l are formed again from
+// the same inputs. A column whose elimination stopped at a zero pivot has failed the step
+// (the error is latched) and its values here are not used.
+__global__ void __launch_bounds__(256) GOM_WAVES(GOM_WAVES_TURB) k_turb_levels(DevMesh m, DevState s, DevForcing f,
+                                                                                DevWork w, DevDims d, TurbParams p,
+                                                                                int *err) {
+  int j = gom_block() * blockDim.x + threadIdx.x;
+  const int J = d.J;
+  if (j >= J) return;
+  int kt = s.top_layer_at_face[j];
+  if (kt == 0) return;
+  int kb = m.bottom_layer_at_face[j];
+  double wn = f.wind_stress_normal[j], wt = f.wind_stress_tangnt[j];
+  double q2fs = p.fs_coef * sqrt(wn * wn + wt * wt) / 2.0;
+  q2fs = dmaxf(q2fs, p.q2min);
+  double ubn = AL(s.un_face_new, kb, j, J), ubt = AL(s.vn_face_new, kb, j, J);
+  double q2bot = p.fs_coef * s.Cdb[j] * sqrt(ubn * ubn + ubt * ubt) / 2.0;
+  q2bot = dmaxf(q2bot, p.q2min);
+  double xlfs = dmaxf(p.xlmin1, AL(s.dz_face_new, kt, j, J) / 2.0 * (double)0.4f);
+  double xlbot = dmaxf(p.xlmin2, AL(s.dz_face_new, kb, j, J) / 2.0 * (double)0.4f);
+  const int nq = kt - kb + 1;
   // ---- back substitutions, bottom up, both systems (:268-278, :351-368): the k solution
   // with its boundary values, then the length scale from the psi solution of the same row
+  double x = AL(w.tk_q2tmp, kb, j, J), xp = AL(w.tk_x, kb, j, J);
   const double cpow_inv = gls_pow(p.cmiu0, -p.rpub);
   for (int k = nq; k >= 1; k--) {
     int kl = kt - k + 1;
@@ -368,47 +397,51 @@ __global__ void __launch_bounds__(256) GOM_WAVES(GOM_WAVES_TURB) k_turb_column(D
     // the solution of row k-1 still needs x, so the clipped length goes to the xl array
     A0(s.xl, kl, j, J) = dmaxf(p.xlmin2, dminf(xl_max, xv));
   }
-  // whole levels (:372-390). xl[k] holds layer k's length until level k replaces it; in
-  // ascending order level k reads layers k and k+1 before either is overwritten.
-  for (int k = kb - 1; k <= kt; k++) {
-    double q, xv;
-    if (k == kb - 1) {
-      q = q2bot;
-      xv = xlbot;
-    } else if (k == kt) {
-      q = q2fs;
-      xv = xlfs;
-    } else {
-      double r = AL(s.dz_face_new, k, j, J) / 2.0 / A0(s.dzhalf_face_new, k, j, J);
-      double q2a = AL(w.tk_q2tmp, k, j, J), q2b = AL(w.tk_q2tmp, k + 1, j, J);
-      double xa = A0(s.xl, k, j, J), xb = A0(s.xl, k + 1, j, J);
-      q = q2a + r * (q2b - q2a);
-      xv = xa + r * (xb - xa);
-      if (q < 0.0 || xv < p.xlmin2) {
-        latch(err, GOMGPU_ERR_TURB_NEGATIVE, j + 1);
-        return;
-      }
+  // whole levels (:372-390) and the new diffusivities (:397-427), descending from the
+  // surface. Level k interpolates layers k and k+1: layer k is read before level k replaces
+  // it, and layer k+1 (read before level k+1 replaced it) and the density at k+1 carry from
+  // the level above. Each level's stability functions follow at once; the levels above kt
+  // take level kt's k, l and diffusivities, those below kb level kb-1's k, l and level kb's
+  // diffusivities.
+  double vd, td, qd, qd2;
+  alg_stress(p, q2fs, xlfs, 0.0, vd, td, qd, qd2);
+  for (int k = kt + 1; k <= d.L; k++) {
+    A0(s.q2, k, j, J) = q2fs;
+    A0(s.xl, k, j, J) = xlfs;
+    A0(s.Av, k, j, J) = vd;
+    A0(w.turb_kvf, k, j, J) = td;
+  }
+  double q2b = AL(w.tk_q2tmp, kt, j, J), xb = A0(s.xl, kt, j, J), rho_b = AL(s.rho_face, kt, j, J);
+  A0(s.q2, kt, j, J) = q2fs;
+  A0(s.xl, kt, j, J) = xlfs;
+  A0(s.Av, kt, j, J) = vd;
+  A0(w.turb_kvf, kt, j, J) = td;
+  for (int k = kt - 1; k >= kb; k--) {
+    const double dzh = A0(s.dzhalf_face_new, k, j, J);
+    const double r = AL(s.dz_face_new, k, j, J) / 2.0 / dzh;
+    const double q2a = AL(w.tk_q2tmp, k, j, J), xa = A0(s.xl, k, j, J), rho_a = AL(s.rho_face, k, j, J);
+    const double q = q2a + r * (q2b - q2a);
+    const double xv = xa + r * (xb - xa);
+    if (q < 0.0 || xv < p.xlmin2) {
+      latch(err, GOMGPU_ERR_TURB_NEGATIVE, j + 1);
+      return;
     }
     A0(s.q2, k, j, J) = q;
     A0(s.xl, k, j, J) = xv;
-  }
-  // the new diffusivities (:397-427): k, l extended above and below the column, and at every
-  // level the stability functions of the level clamped to kb..kt
-  for (int k = 0; k <= d.L; k++) {
-    if (k <= kb - 2) {
-      A0(s.q2, k, j, J) = A0(s.q2, kb - 1, j, J);
-      A0(s.xl, k, j, J) = A0(s.xl, kb - 1, j, J);
-    }
-    if (k > kt) {
-      A0(s.q2, k, j, J) = A0(s.q2, kt, j, J);
-      A0(s.xl, k, j, J) = A0(s.xl, kt, j, J);
-    }
-    if (k == 0) continue;
-    const int kl = k < kb ? kb : (k > kt ? kt : k);
-    double vd, td, qd, qd2;
-    alg_stress(p, A0(s.q2, kl, j, J), A0(s.xl, kl, j, J), rz_level(p, s, J, j, kl, kt), vd, td, qd, qd2);
+    alg_stress(p, q, xv, p.g / p.rho_o * ((rho_b - rho_a) / dzh), vd, td, qd, qd2);
     A0(s.Av, k, j, J) = vd;
     A0(w.turb_kvf, k, j, J) = td;
+    q2b = q2a;
+    xb = xa;
+    rho_b = rho_a;
+  }
+  for (int k = 0; k < kb; k++) {
+    A0(s.q2, k, j, J) = q2bot;
+    A0(s.xl, k, j, J) = xlbot;
+    if (k > 0) {
+      A0(s.Av, k, j, J) = vd;
+      A0(w.turb_kvf, k, j, J) = td;
+    }
   }
 }
 
