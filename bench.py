@@ -335,7 +335,8 @@ def main() -> int:
     ap.add_argument("--pcg-tolerance", type=float, default=PCG_TOLERANCE,
                     help="PCG stop tolerance on r.M^-1 r / r0.M^-1 r0 (default: the reference's main.inp)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--profile-steps", type=int, default=5)
+    ap.add_argument("--profile-steps", type=int, default=5,
+                    help="untimed steps after the timed ones, for the stage and kernel timers (>= 1)")
     ap.add_argument("--no-autotune", action="store_true", help="keep the default launch shapes")
     ap.add_argument("--no-serving", action="store_true", help="skip the serving-loop sample")
     ap.add_argument("--no-ensemble", action="store_true", help="skip the batched-ensemble sample")
@@ -349,6 +350,8 @@ def main() -> int:
                     help="instead of the benchmark, run STEPS steps (default launch shapes) and print "
                          "each step's PCG iteration count, for normalising rocprofv3 --pmc counters")
     args = ap.parse_args()
+    if args.profile_steps < 1:
+        ap.error("--profile-steps must be at least 1: the roofline is read from those steps' timers")
     if args.gpus != 1:
         print(f"bench.py: this build measures one GPU only (got --gpus {args.gpus}); "
               "the step would shard by domain decomposition, which is not built.", file=sys.stderr)
