@@ -125,7 +125,8 @@ struct DevWork {
   double *turb_kvf;
   // elimination factors of the column tridiagonal solves, (L, max(E, J))
   double *tri_gam;
-  double *tvd_x;          // (L, E, NT) right-hand sides / solution of the TVD column solve
+  double *tvd_x;          // [layer][cell][tracer] right-hand sides / solution of the TVD column solve
+                          // (the ELM transport keeps [tracer][layer][cell] in it)
   double *cell_u, *cell_v;  // (L, E) cell means of the mid-layer node velocities (momentum)
   // two-equation closure (allocated when turbulence_flag == 3): level diffusivities of the
   // current state (0:L, J) and layer quantities / solutions (L, J)

@@ -55,6 +55,25 @@ __device__ __forceinline__ size_t nix(int t, int k, int e, int L, int n) {  // (
 __device__ __forceinline__ size_t frx(int k, int j, int J) {  // (Q, d) or (PSI 1, PSI 2) of layer k
   return 2 * ((size_t)(k - 1) * J + j);
 }
+// The right-hand sides of the column solve, w.tvd_x: layer k's values of the NT tracers side
+// by side, so that both tracers move in one 16-byte access
+__device__ __forceinline__ void x_load(const double *x, int k, int e, int n, int NT, double v[2]) {
+  const size_t i = (size_t)(k - 1) * n + e;
+  if (NT == 2) {
+    const double2 p = ((const double2 *)x)[i];
+    v[0] = p.x;
+    v[1] = p.y;
+  } else {
+    v[0] = x[i];
+  }
+}
+__device__ __forceinline__ void x_store(double *x, int k, int e, int n, int NT, const double v[2]) {
+  const size_t i = (size_t)(k - 1) * n + e;
+  if (NT == 2)
+    ((double2 *)x)[i] = make_double2(v[0], v[1]);
+  else
+    x[i] = v[0];
+}
 // PSI_ik of level k (0:L) for the two tracers side by side: one 16-byte load for both
 __device__ __forceinline__ size_t pix(int k, int e, int n) { return 2 * ((size_t)k * n + e); }
 // The upwind sums (s1a, s2a) of layer k (1:L), per tracer t at 2 t, NT tracers per record
@@ -461,35 +480,11 @@ __device__ __forceinline__ void tvd_rhs_layer(const DevMesh &m, const DevState &
   }
 }
 
-// Tridiagonal row r (0-based, surface first) of cell e: solve_transport_equation_v19.f90:957-1025
-template <typename W>
-__device__ __forceinline__ void tvd_row(const DevState &s, const TvdW<W> &w, int e, int E, int tl, int bl, int r,
-                                        double ar, double dt2, int sub, int Nt, double &av, double &bv, double &cv) {
-  int nv = tl - bl + 1;
-  av = 0.0;
-  cv = 0.0;
-  if (tl == bl) {
-    double dz0 = AL(s.dz_cell, tl, e, E);
-    double dzn = dz0 + (AL(s.dz_cell_new, tl, e, E) - dz0) * (sub + 1) / Nt;
-    bv = ar * dzn;
-  } else if (r == 0) {
-    cv = -dt2 * A0(w.di, tl - 1, e, E);
-    double dz0 = AL(s.dz_cell, tl, e, E);
-    double dzn = dz0 + (AL(s.dz_cell_new, tl, e, E) - dz0) * (sub + 1) / Nt;
-    bv = ar * dzn - cv;
-  } else if (r == nv - 1) {
-    av = -dt2 * A0(w.di, bl, e, E);
-    double dz0 = AL(s.dz_cell, bl, e, E);
-    double dzn = dz0 + (AL(s.dz_cell_new, bl, e, E) - dz0) * (sub + 1) / Nt;
-    bv = -av + ar * dzn;
-  } else {
-    int k = tl - r;
-    av = -dt2 * A0(w.di, k, e, E);
-    cv = -dt2 * A0(w.di, k - 1, e, E);
-    double dz0 = AL(s.dz_cell, k, e, E);
-    double dzn = dz0 + (AL(s.dz_cell_new, k, e, E) - dz0) * (sub + 1) / Nt;
-    bv = -av + ar * dzn - cv;
-  }
+// Layer k's thickness at the sub-cycle's time: dz_cell moved towards dz_cell_new by
+// (sub+1)/Nt (solve_transport_equation_v19.f90:957-1025)
+__device__ __forceinline__ double tvd_dzn(const DevState &s, int k, int e, int E, int sub, int Nt) {
+  const double dz0 = AL(s.dz_cell, k, e, E);
+  return dz0 + (AL(s.dz_cell_new, k, e, E) - dz0) * (sub + 1) / Nt;
 }
 
 // The clamp of solve_transport_equation_v19.f90:1310-1322.
@@ -509,7 +504,7 @@ __global__ void k_tvd_rhs(DevMesh m, DevState s, TvdW<W> w, DevDims d, TvdArgs a
   if (tl == 0 || k < bl || k > tl) return;
   double x[2];
   tvd_rhs_layer(m, s, w, d, a, e, k, bl, tl, x);
-  for (int t = 0; t < d.NT; t++) w.tvd_x[nix(t, k, e, L, E)] = x[t];
+  x_store(w.tvd_x, k, e, E, d.NT, x);
 }
 
 // One thread per cell column: vertical diffusion implicit, the Thomas recurrence of
@@ -536,34 +531,45 @@ __global__ void k_tvd_solve(DevMesh m, DevState s, TvdW<W> w, DevDims d, TvdArgs
   const double dt2 = a.dt2;
   int nv = tl - bl + 1;
   double ar = m.area[e];
-  double av, bv, cv, cprev, x[2];
-  tvd_row(s, w, e, E, tl, bl, 0, ar, dt2, sub, Nt, av, bv, cv);
+  // Rows top down (row r is layer tl - r), solve_transport_equation_v19.f90:957-1025: row r
+  // has a = -dt2 di(k) below the surface row, c = -dt2 di(k-1) above the bottom row and
+  // b = area dz - a - c. Row r+1's a is row r's c, carried.
+  double cv = 0.0, bv, x[2];
+  if (nv == 1) {
+    bv = ar * tvd_dzn(s, tl, e, E, sub, Nt);
+  } else {
+    cv = -dt2 * A0(w.di, tl - 1, e, E);
+    bv = ar * tvd_dzn(s, tl, e, E, sub, Nt) - cv;
+  }
   if (bv == 0.0) {
     latch(err, GOMGPU_ERR_TRIDIAG_PIVOT, e + 1);
     return;
   }
   double den = bv;
-  for (int t = 0; t < NT; t++) {
-    size_t o = nix(t, tl, e, L, E);
-    x[t] = w.tvd_x[o] / den;
-    w.tvd_x[o] = x[t];
-  }
-  cprev = cv;
+  x_load(w.tvd_x, tl, e, E, NT, x);
+  for (int t = 0; t < NT; t++) x[t] = x[t] / den;
+  x_store(w.tvd_x, tl, e, E, NT, x);
   for (int r = 1; r < nv; r++) {
-    tvd_row(s, w, e, E, tl, bl, r, ar, dt2, sub, Nt, av, bv, cv);
+    const int k = tl - r;
+    const double av = cv, cprev = cv;
+    if (r == nv - 1) {
+      cv = 0.0;
+      bv = -av + ar * tvd_dzn(s, k, e, E, sub, Nt);
+    } else {
+      cv = -dt2 * A0(w.di, k - 1, e, E);
+      bv = -av + ar * tvd_dzn(s, k, e, E, sub, Nt) - cv;
+    }
     double gam = cprev / den;
     den = bv - av * gam;
     if (den == 0.0) {
       latch(err, GOMGPU_ERR_TRIDIAG_PIVOT, e + 1);
       return;
     }
-    for (int t = 0; t < NT; t++) {
-      size_t o = nix(t, tl - r, e, L, E);
-      x[t] = (w.tvd_x[o] - av * x[t]) / den;
-      w.tvd_x[o] = x[t];
-    }
+    double xr[2];
+    x_load(w.tvd_x, k, e, E, NT, xr);
+    for (int t = 0; t < NT; t++) x[t] = (xr[t] - av * x[t]) / den;
+    x_store(w.tvd_x, k, e, E, NT, x);
     AL(w.tri_gam, r + 1, e, E) = gam;
-    cprev = cv;
   }
   // back substitution, then the clamp (rows top down = layers tl .. bl)
   for (int t = 0; t < NT; t++) {
@@ -575,8 +581,10 @@ __global__ void k_tvd_solve(DevMesh m, DevState s, TvdW<W> w, DevDims d, TvdArgs
   }
   for (int r = nv - 2; r >= 0; r--) {
     double gam = AL(w.tri_gam, r + 2, e, E);
+    double xr[2];
+    x_load(w.tvd_x, tl - r, e, E, NT, xr);
     for (int t = 0; t < NT; t++) {
-      x[t] = w.tvd_x[nix(t, tl - r, e, L, E)] - gam * x[t];
+      x[t] = xr[t] - gam * x[t];
       double v = tvd_clamp(x[t]);
       if (last)
         w.con_new[nix(t, tl - r, e, L, E)] = v;
