@@ -112,7 +112,7 @@ struct DevWork {
   // transport
   double *con, *con_new;  // con (0:L+1, E, NT) and con_new (L, E, NT), layer-major
   double *con2;           // the sub-cycles' second con buffer (con and con2 alternate)
-  double *htop;           // (2, E, NT) horizontal upwind sums of each column's top layer
+  double *htop;           // [cell][tracer] (sum1, sum2): horizontal upwind sums of each column's top layer
   double *fqd, *fps;      // (2, L, J) each: per face-layer (Q, d) and the PSI of both tracers
   double *phij;           // (L, J)
   double *Qi, *di, *phii; // (0:L, E)

@@ -74,6 +74,8 @@ __device__ __forceinline__ void x_store(double *x, int k, int e, int n, int NT, 
   else
     x[i] = v[0];
 }
+// The top layer's horizontal upwind sums (sum1, sum2) of tracer t of cell e, w.htop
+__device__ __forceinline__ size_t hix(int t, int e, int NT) { return 2 * ((size_t)e * NT + t); }
 // PSI_ik of level k (0:L) for the two tracers side by side: one 16-byte load for both
 __device__ __forceinline__ size_t pix(int k, int e, int n) { return 2 * ((size_t)k * n + e); }
 // The upwind sums (s1a, s2a) of layer k (1:L), per tracer t at 2 t, NT tracers per record
@@ -198,10 +200,10 @@ __device__ __forceinline__ double limiter(int which, double r, double phi) {
 
 // Horizontal upwind sums of one cell-layer (solve_transport_equation_v19.f90:700-760): the
 // inflow-weighted differences sum1 and the inflows sum2, over the cell's active faces in
-// slot order.
+// slot order; c0 holds the cell-layer's own concentrations.
 template <typename W>
 __device__ __forceinline__ void lim_hsums(const DevMesh &m, const TvdW<W> &w, const DevDims &d, int e, int k,
-                                          double *sum1, double *sum2) {
+                                          const double *c0, double *sum1, double *sum2) {
   const int E = d.E, L2 = d.L + 2, J = d.J;
   sum1[0] = sum1[1] = 0.0;
   sum2[0] = sum2[1] = 0.0;
@@ -210,8 +212,6 @@ __device__ __forceinline__ void lim_hsums(const DevMesh &m, const TvdW<W> &w, co
   const int4 ad4 = *(const int4 *)(m.cg_adj + 4 * (size_t)e);
   const int fl[4] = {fl4.x, fl4.y, fl4.z, fl4.w}, fc[4] = {fc4.x, fc4.y, fc4.z, fc4.w};
   const int ad[4] = {ad4.x, ad4.y, ad4.z, ad4.w};
-  double c0[2] = {0.0, 0.0};
-  for (int t = 0; t < d.NT; t++) c0[t] = w.con[cix(t, k, e, L2, E)];
 #pragma unroll
   for (int l = 0; l < 4; l++) {  // slots past the cell's faces hold flow 0
     const int sa = fl[l];
@@ -259,12 +259,10 @@ __global__ void k_tvd_coef_cell(DevMesh m, DevState s, TvdW<W> w, DevDims d, Tvd
     A0(w.phii, k, e, E) = phi;
   }
   if ((a.hlim > 0 || a.vlim > 0) && k >= 1 && k == tl) {
-    double sum1[2], sum2[2];
-    lim_hsums(m, w, d, e, k, sum1, sum2);
-    for (int t = 0; t < d.NT; t++) {
-      w.htop[(size_t)(2 * t) * E + e] = sum1[t];
-      w.htop[(size_t)(2 * t + 1) * E + e] = sum2[t];
-    }
+    double sum1[2], sum2[2], c0[2] = {0.0, 0.0};
+    for (int t = 0; t < d.NT; t++) c0[t] = w.con[cix(t, k, e, d.L + 2, E)];
+    lim_hsums(m, w, d, e, k, c0, sum1, sum2);
+    for (int t = 0; t < d.NT; t++) *(double2 *)(w.htop + hix(t, e, d.NT)) = make_double2(sum1[t], sum2[t]);
   }
 }
 
@@ -284,31 +282,40 @@ __global__ void k_tvd_lim_cell(DevMesh m, DevState s, TvdW<W> w, DevDims d, int 
     if (first) *(double2 *)(w.PSIi + pix(k, e, E)) = make_double2(0.0, 0.0);
     return;
   }
-  int nv = tl - bl + 1;
-  if (k >= bl) {  // s1a, s2a of layer k
+  if (k == bl - 1) {  // PSI_ik is 0 at bl - 1 (r_ik is 1.0 there)
+    if (first) *(double2 *)(w.PSIi + pix(k, e, E)) = make_double2(0.0, 0.0);
+    return;
+  }
+  // bl <= k <= tl from here. The concentrations of levels k-1, k and k+1 (con holds levels
+  // 0..L+1) and the vertical flows at k-1 and k, read once for both parts.
+  const int nv = tl - bl + 1;
+  double cm[2] = {0.0, 0.0}, c0[2] = {0.0, 0.0}, cp[2] = {0.0, 0.0};
+  for (int t = 0; t < d.NT; t++) {
+    cm[t] = w.con[cix(t, k - 1, e, L2, E)];
+    c0[t] = w.con[cix(t, k, e, L2, E)];
+    cp[t] = w.con[cix(t, k + 1, e, L2, E)];
+  }
+  const double qm = A0(w.Qi, k - 1, e, E), qk = A0(w.Qi, k, e, E);
+  {  // s1a, s2a of layer k
     double s1a[2] = {0, 0}, s2a[2] = {0, 0}, sum1[2], sum2[2];
     for (int t = 0; t < d.NT; t++) {
-      double c0 = w.con[cix(t, k, e, L2, E)];
       if (nv != 1) {
         if (k == bl) {
-          double qq = A0(w.Qi, k, e, E);
-          double r2 = fabs(0.5 * (qq + fabs(qq)));
-          s1a[t] = r2 * (c0 - w.con[cix(t, k + 1, e, L2, E)]);
+          double r2 = fabs(0.5 * (qk + fabs(qk)));
+          s1a[t] = r2 * (c0[t] - cp[t]);
           s2a[t] = r2;
         } else if (k == tl) {
-          double qq = A0(w.Qi, k - 1, e, E);
-          double r1 = fabs(0.5 * (qq + fabs(qq)));
-          s1a[t] = r1 * (c0 - w.con[cix(t, k - 1, e, L2, E)]);
+          double r1 = fabs(0.5 * (qm + fabs(qm)));
+          s1a[t] = r1 * (c0[t] - cm[t]);
           s2a[t] = r1;
         } else {
-          double q1 = A0(w.Qi, k - 1, e, E), q2 = A0(w.Qi, k, e, E);
-          double r1 = fabs(0.5 * (q1 + fabs(q1))), r2 = fabs(0.5 * (q2 + fabs(q2)));
-          s1a[t] = r1 * (c0 - w.con[cix(t, k - 1, e, L2, E)]) + r2 * (c0 - w.con[cix(t, k + 1, e, L2, E)]);
+          double r1 = fabs(0.5 * (qm + fabs(qm))), r2 = fabs(0.5 * (qk + fabs(qk)));
+          s1a[t] = r1 * (c0[t] - cm[t]) + r2 * (c0[t] - cp[t]);
           s2a[t] = r1 + r2;
         }
       }
     }
-    lim_hsums(m, w, d, e, k, sum1, sum2);
+    lim_hsums(m, w, d, e, k, c0, sum1, sum2);
     typedef typename Pair<W>::T W2;
     for (int t = 0; t < d.NT; t++) {
       W2 v;
@@ -317,36 +324,30 @@ __global__ void k_tvd_lim_cell(DevMesh m, DevState s, TvdW<W> w, DevDims d, int 
       *(W2 *)(w.s12 + six(k, e, E, d.NT) + 2 * t) = v;
     }
   }
-  // r_ik of level k: computed for bl <= k <= tl - 1 (1.0 at bl - 1 and tl, where PSI_ik is 0)
-  if (k == tl || k == bl - 1) {
+  // r_ik of level k: computed for bl <= k <= tl - 1 (1.0 at tl, where PSI_ik is 0)
+  if (k == tl) {
     if (first) *(double2 *)(w.PSIi + pix(k, e, E)) = make_double2(0.0, 0.0);
     return;
   }
-  double sum1[2], sum2[2];
-  for (int t = 0; t < d.NT; t++) {
-    sum1[t] = w.htop[(size_t)(2 * t) * E + e];
-    sum2[t] = w.htop[(size_t)(2 * t + 1) * E + e];
-  }
+  const bool up = A0(w.Qi, tl, e, E) >= 0.0;
+  const double phi = A0(w.phii, k, e, E);
   double psi[2] = {0.0, 0.0};
   for (int t = 0; t < d.NT; t++) {
+    const double2 h = *(const double2 *)(w.htop + hix(t, e, d.NT));  // (sum1, sum2)
     double r;
-    if (A0(w.Qi, tl, e, E) >= 0.0) {
-      double qm = A0(w.Qi, k - 1, e, E);
-      double c0 = w.con[cix(t, k, e, L2, E)];
-      double rt1 = sum1[t] + fabs(qm) * (c0 - w.con[cix(t, k - 1, e, L2, E)]);
-      double rt2 = sum2[t] + fabs(qm);
-      double rt = (w.con[cix(t, k + 1, e, L2, E)] - c0) * rt2;
+    if (up) {
+      double rt1 = h.x + fabs(qm) * (c0[t] - cm[t]);
+      double rt2 = h.y + fabs(qm);
+      double rt = (cp[t] - c0[t]) * rt2;
       r = (rt == 0.0) ? 1.0 : rt1 / rt;
-    } else {  // the reference's k runs bl+1..tl and writes level k-1
-      int kk = k + 1;
-      double qk = A0(w.Qi, kk, e, E);
-      double c0 = w.con[cix(t, kk, e, L2, E)];
-      double rt1 = sum1[t] + fabs(qk) * (c0 - w.con[cix(t, kk + 1, e, L2, E)]);
-      double rt2 = sum2[t] + fabs(qk);
-      double rt = (w.con[cix(t, kk - 1, e, L2, E)] - c0) * rt2;
+    } else {  // the reference's k runs bl+1..tl and writes level k-1: level k's ratio from layer k+1
+      double qk1 = A0(w.Qi, k + 1, e, E);
+      double rt1 = h.x + fabs(qk1) * (cp[t] - w.con[cix(t, k + 2, e, L2, E)]);
+      double rt2 = h.y + fabs(qk1);
+      double rt = (c0[t] - cp[t]) * rt2;
       r = (rt == 0.0) ? 1.0 : rt1 / rt;
     }
-    psi[t] = hlim != 0 ? limiter(hlim, r, A0(w.phii, k, e, E)) : 0.0;
+    psi[t] = hlim != 0 ? limiter(hlim, r, phi) : 0.0;
   }
   *(double2 *)(w.PSIi + pix(k, e, E)) = make_double2(psi[0], psi[1]);
 }
