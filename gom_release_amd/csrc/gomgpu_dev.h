@@ -43,12 +43,12 @@ struct DevMesh {
   // indirect ones.
   const double *cg_x, *cg_y;
   const int32_t *cg_node, *cg_adj;  // node ids and neighbour cells (1-based, 0 = none)
-  // Per cell, its 4 slots contiguous ([cell][slot], one 16-byte load each): slot_flow is the
-  // slot face's sign_in_outflow when the transport counts its flux (boundary_type_of_face !=
-  // -1 or a river face), else 0; cg_face is the slot's face id (1-based, 0 past the cell's
-  // faces), facenum_at_cell in this layout. With cg_adj they give a cell's stencil in three
-  // loads instead of twelve.
-  const int32_t *slot_flow, *cg_face;
+  // Per cell, its 4 slots contiguous ([cell][slot], one 16-byte load): where the transport
+  // counts the slot face's flux (boundary_type_of_face != -1 or a river face), the face id
+  // (1-based) shifted left by one with bit 0 set when the face's sign_in_outflow is +1 (clear
+  // for -1); 0 for a face the transport skips and past the cell's faces. With cg_adj it
+  // gives a cell's transport stencil in two loads instead of twelve.
+  const int32_t *slot_face;
 };
 
 struct DevState {
@@ -403,4 +403,12 @@ static inline int pcg_blocks_for(long n, int rows) {
   }
   int nb = (int)((n + r * 512 - 1) / (r * 512));
   return nb < 1 ? 1 : nb;
+}
+
+// A slot_face entry: false for a slot the transport skips, else the face (0-based) and its
+// outflow sign seen from the cell
+__device__ __forceinline__ bool slot_decode(int v, int &j, int &sign) {
+  j = (int)((unsigned)v >> 1) - 1;
+  sign = (v & 1) ? 1 : -1;
+  return v != 0;
 }

@@ -481,12 +481,12 @@ extern "C" int gomgpu_create(const gomgpu_mesh *mesh, const gomgpu_params *param
     c->m.cg_node = ip;
     if ((rc = dalloc(c, &ip, ca.size())) || (rc = upload(c, ip, ca.data(), Shape{1, (long)ca.size()}))) goto fail;
     c->m.cg_adj = ip;
-    // slot_flow: the transport's face test and outflow sign of each cell slot
+    // slot_face: each cell slot's face and outflow sign where the transport counts its flux
     if (!mesh->facenum_at_cell || !mesh->sign_in_outflow || !mesh->boundary_type_of_face || !mesh->isflowside3) {
       rc = seterr(c, GOMGPU_E_ARG, "mesh connectivity missing");
       goto fail;
     }
-    std::vector<int32_t> sf(4 * E, 0), cf(4 * E, 0);
+    std::vector<int32_t> sf(4 * E, 0);
     for (long e = 0; e < E; e++) {
       int N = mesh->tri_or_quad[e];
       for (int l = 0; l < N && l < 4; l++) {
@@ -497,14 +497,11 @@ extern "C" int gomgpu_create(const gomgpu_mesh *mesh, const gomgpu_params *param
           goto fail;
         }
         const bool active = mesh->boundary_type_of_face[j] != -1 || mesh->isflowside3[j] > 0;
-        sf[4 * e + l] = active ? sg : 0;
-        cf[4 * e + l] = j + 1;
+        sf[4 * e + l] = active ? (int32_t)(((uint32_t)(j + 1) << 1) | (sg > 0 ? 1u : 0u)) : 0;
       }
     }
     if ((rc = dalloc(c, &ip, sf.size())) || (rc = upload(c, ip, sf.data(), Shape{1, (long)sf.size()}))) goto fail;
-    c->m.slot_flow = ip;
-    if ((rc = dalloc(c, &ip, cf.size())) || (rc = upload(c, ip, cf.data(), Shape{1, (long)cf.size()}))) goto fail;
-    c->m.cg_face = ip;
+    c->m.slot_face = ip;
   }
   // state
 #define AL_SR(name, ext)                                                             \
@@ -1801,8 +1798,7 @@ extern "C" long long gomgpu_footprint(const gomgpu_params *params) {
   }
   add(4 * (size_t)p.maxele, sizeof(double)); add(4 * (size_t)p.maxele, sizeof(double));  // ELM cell records
   add(4 * (size_t)p.maxele, sizeof(int32_t)); add(4 * (size_t)p.maxele, sizeof(int32_t));
-  add(4 * (size_t)p.maxele, sizeof(int32_t));  // slot_flow
-  add(4 * (size_t)p.maxele, sizeof(int32_t));  // cg_face
+  add(4 * (size_t)p.maxele, sizeof(int32_t));  // slot_face
   GOMGPU_STATE_REAL(FP_R)
   GOMGPU_STATE_INT(FP_I)
   GOMGPU_FORCING_REAL(FP_R)

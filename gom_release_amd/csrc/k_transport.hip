@@ -207,16 +207,14 @@ __device__ __forceinline__ void lim_hsums(const DevMesh &m, const TvdW<W> &w, co
   const int E = d.E, L2 = d.L + 2, J = d.J;
   sum1[0] = sum1[1] = 0.0;
   sum2[0] = sum2[1] = 0.0;
-  const int4 fl4 = *(const int4 *)(m.slot_flow + 4 * (size_t)e);
-  const int4 fc4 = *(const int4 *)(m.cg_face + 4 * (size_t)e);
+  const int4 sf4 = *(const int4 *)(m.slot_face + 4 * (size_t)e);
   const int4 ad4 = *(const int4 *)(m.cg_adj + 4 * (size_t)e);
-  const int fl[4] = {fl4.x, fl4.y, fl4.z, fl4.w}, fc[4] = {fc4.x, fc4.y, fc4.z, fc4.w};
-  const int ad[4] = {ad4.x, ad4.y, ad4.z, ad4.w};
+  const int sf[4] = {sf4.x, sf4.y, sf4.z, sf4.w}, ad[4] = {ad4.x, ad4.y, ad4.z, ad4.w};
 #pragma unroll
-  for (int l = 0; l < 4; l++) {  // slots past the cell's faces hold flow 0
-    const int sa = fl[l];
-    if (sa != 0) {
-      int j = fc[l] - 1, ii = ad[l];
+  for (int l = 0; l < 4; l++) {  // slots past the cell's faces hold 0
+    int j, sa;
+    if (slot_decode(sf[l], j, sa)) {
+      const int ii = ad[l];
       double Q2 = w.fqd[frx(k, j, J)] * (double)sa;
       if (Q2 < 0.0) {
         for (int t = 0; t < d.NT; t++) {
@@ -407,18 +405,16 @@ __device__ __forceinline__ void tvd_rhs_layer(const DevMesh &m, const DevState &
   int nv = tl - bl + 1;
   double ar = m.area[e];
   double sqi[2] = {0, 0}, sqii[2] = {0, 0}, shd[2] = {0, 0}, shl[2] = {0, 0};
-  const int4 fl4 = *(const int4 *)(m.slot_flow + 4 * (size_t)e);
-  const int4 fc4 = *(const int4 *)(m.cg_face + 4 * (size_t)e);
+  const int4 sf4 = *(const int4 *)(m.slot_face + 4 * (size_t)e);
   const int4 ad4 = *(const int4 *)(m.cg_adj + 4 * (size_t)e);
-  const int fl[4] = {fl4.x, fl4.y, fl4.z, fl4.w}, fc[4] = {fc4.x, fc4.y, fc4.z, fc4.w};
-  const int ad[4] = {ad4.x, ad4.y, ad4.z, ad4.w};
+  const int sf[4] = {sf4.x, sf4.y, sf4.z, sf4.w}, ad[4] = {ad4.x, ad4.y, ad4.z, ad4.w};
   double cc[2] = {0.0, 0.0};
   for (int t = 0; t < NT; t++) cc[t] = w.con[cix(t, k, e, L2, E)];
 #pragma unroll
-  for (int l = 0; l < 4; l++) {  // slots past the cell's faces hold flow 0
-    const int sa = fl[l];
-    if (sa == 0) continue;
-    int j = fc[l] - 1, ii = ad[l];
+  for (int l = 0; l < 4; l++) {  // slots past the cell's faces hold 0
+    int j, sa;
+    if (!slot_decode(sf[l], j, sa)) continue;
+    const int ii = ad[l];
     const double2 qd = *(const double2 *)(w.fqd + frx(k, j, J));   // Q, d
     const double2 ps = *(const double2 *)(w.fps + frx(k, j, J));   // PSI of the two tracers
     double Q2 = qd.x * (double)sa;
