@@ -49,6 +49,9 @@ struct DevMesh {
   // for -1); 0 for a face the transport skips and past the cell's faces. With cg_adj it
   // gives a cell's transport stencil in two loads instead of twelve.
   const int32_t *slot_face;
+  // Per face (int2): its two cells (1-based, 0 = none), the first shifted left by one with
+  // bit 0 set where the transport counts the face's flux, as slot_face
+  const int32_t *face_adj;
 };
 
 struct DevState {

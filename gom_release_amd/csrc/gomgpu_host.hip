@@ -502,6 +502,17 @@ extern "C" int gomgpu_create(const gomgpu_mesh *mesh, const gomgpu_params *param
     }
     if ((rc = dalloc(c, &ip, sf.size())) || (rc = upload(c, ip, sf.data(), Shape{1, (long)sf.size()}))) goto fail;
     c->m.slot_face = ip;
+    // face_adj: per face its two cells (1-based, 0 = none), the first shifted left by one
+    // with bit 0 set where the transport counts the face's flux
+    std::vector<int32_t> fa(2 * (size_t)p.maxface);
+    for (long j = 0; j < p.maxface; j++) {
+      const int32_t c1 = mesh->adj_cellnum_at_face[2 * j], c2 = mesh->adj_cellnum_at_face[2 * j + 1];
+      const bool active = mesh->boundary_type_of_face[j] != -1 || mesh->isflowside3[j] > 0;
+      fa[2 * j] = (int32_t)(((uint32_t)c1 << 1) | (active ? 1u : 0u));
+      fa[2 * j + 1] = c2;
+    }
+    if ((rc = dalloc(c, &ip, fa.size())) || (rc = upload(c, ip, fa.data(), Shape{1, (long)fa.size()}))) goto fail;
+    c->m.face_adj = ip;
   }
   // state
 #define AL_SR(name, ext)                                                             \
@@ -1799,6 +1810,7 @@ extern "C" long long gomgpu_footprint(const gomgpu_params *params) {
   add(4 * (size_t)p.maxele, sizeof(double)); add(4 * (size_t)p.maxele, sizeof(double));  // ELM cell records
   add(4 * (size_t)p.maxele, sizeof(int32_t)); add(4 * (size_t)p.maxele, sizeof(int32_t));
   add(4 * (size_t)p.maxele, sizeof(int32_t));  // slot_face
+  add(2 * (size_t)p.maxface, sizeof(int32_t));  // face_adj
   GOMGPU_STATE_REAL(FP_R)
   GOMGPU_STATE_INT(FP_I)
   GOMGPU_FORCING_REAL(FP_R)

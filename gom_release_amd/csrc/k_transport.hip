@@ -183,10 +183,6 @@ __device__ __forceinline__ double con_across(const DevMesh &m, const TvdW<W> &w,
   return 0.0;
 }
 
-__device__ __forceinline__ bool face_active(const DevMesh &m, int j) {
-  return m.boundary_type_of_face[j] != -1 || m.isflowside3[j] > 0;
-}
-
 __device__ __forceinline__ double limiter(int which, double r, double phi) {
   double PHI;
   if (which == 1)
@@ -244,7 +240,12 @@ __global__ void k_tvd_coef_cell(DevMesh m, DevState s, TvdW<W> w, DevDims d, Tvd
   if (!in && a.sub > 0 && !((a.hlim > 0 || a.vlim > 0) && k >= 1 && k == tl)) return;
   if (in) {
     double ar = m.area[e];
-    Q = ar * ((1.0 - a.th) * A0(s.wn_cell, k, e, E) + a.th * A0(s.wn_cell_new, k, e, E));
+    // Q does not change over the sub-cycles (the velocities and the layer range are those
+    // of the step): formed in the first, read back after
+    if (a.sub == 0)
+      Q = ar * ((1.0 - a.th) * A0(s.wn_cell, k, e, E) + a.th * A0(s.wn_cell_new, k, e, E));
+    else
+      Q = A0(w.Qi, k, e, E);
     double dh = A0(s.dzhalf_cell, k, e, E);
     double dzo = dh + (A0(s.dzhalf_cell_new, k, e, E) - dh) * a.sub / a.Nt;
     double D = ar * A0(s.Kv, k, e, E) / dzo;
@@ -252,7 +253,7 @@ __global__ void k_tvd_coef_cell(DevMesh m, DevState s, TvdW<W> w, DevDims d, Tvd
     phi = (Q == 0.0) ? 1.0 : dminf(1.0, 2.0 * D / fabs(Q));
   }
   if (in || a.sub == 0) {
-    A0(w.Qi, k, e, E) = Q;
+    if (a.sub == 0) A0(w.Qi, k, e, E) = Q;
     A0(w.di, k, e, E) = dd;
     A0(w.phii, k, e, E) = phi;
   }
@@ -366,17 +367,19 @@ __global__ void k_tvd_lim_face(DevMesh m, DevState s, TvdW<W> w, DevDims d, int 
     return;
   }
   double r[2] = {1.0, 1.0};
-  if (face_active(m, j)) {
+  const int2 fa = *(const int2 *)(m.face_adj + 2 * (size_t)j);
+  if (fa.x & 1) {  // a face whose flux the transport counts
+    const int cell[2] = {fa.x >> 1, fa.y};
     double Q = w.fqd[frx(k, j, J)];
     for (int side = 1; side <= 2; side++) {
-      int c = TB(m.adj_cellnum_at_face, side, j, J);
+      int c = cell[side - 1];
       if (c == 0) continue;
       int e = c - 1;
       int tl = s.top_layer_at_element[e], bl = m.bottom_layer_at_element[e];
       if (tl == 0 || k < bl || k > tl) continue;
       double sg = side == 1 ? 1.0 : -1.0;  // sign_in_outflow of face j seen from this cell
       if (Q * sg >= 0.0) {
-        int other = TB(m.adj_cellnum_at_face, 3 - side, j, J);
+        int other = cell[2 - side];
         typedef typename Pair<W>::T W2;
         for (int t = 0; t < d.NT; t++) {
           double ci = con_across(m, w, d, e, other, k, t);
