@@ -16,8 +16,12 @@
 #define GOM_LMAX 64  // largest supported maxlayer (per-thread column arrays)
 #define GOM_NSCAL 8  // per-step device scalars (DevForcing::scal)
 #define GOM_MAX_FORCING 32  // forcing arrays the ABI can hold (GOMGPU_FORCING_REAL has fewer)
-#define GOM_NV_LEVELS 3  // node levels per thread of k_node_vel
-#define GOM_TVD_NODE_LEVELS 4  // node levels per thread of k_tvd_node
+#ifndef GOM_NV_LEVELS
+#define GOM_NV_LEVELS 4  // node levels per thread of k_node_vel
+#endif
+#ifndef GOM_TVD_NODE_LEVELS
+#define GOM_TVD_NODE_LEVELS 8  // node levels per thread of k_tvd_node
+#endif
 #define GOM_KV_LEVELS 3  // cell levels per thread of k_turb_kv
 #define GOM_CUV_LAYERS 4  // cell layers per thread of k_cell_uv
 #define PCG_MAX_SLOT_BLOCKS 256  // most blocks of one system in a slot-exchange PCG launch

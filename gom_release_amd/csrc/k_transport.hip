@@ -444,6 +444,12 @@ __device__ __forceinline__ void tvd_rhs_layer(const DevMesh &m, const DevState &
     x[t] = rhs;
   }
   if (nv != 1) {
+    // the vertical limiter of both tracers at levels k-1 and k, one 16-byte load each
+    double2 pm = make_double2(0.0, 0.0), pk = make_double2(0.0, 0.0);
+    if (a.vlim > 0) {
+      if (k > bl) pm = *(const double2 *)(w.PSIi + pix(k - 1, e, E));
+      if (k < tl) pk = *(const double2 *)(w.PSIi + pix(k, e, E));
+    }
     for (int t = 0; t < NT; t++) {
       double ck = cc[t];
       double v_adv, vft = 0.0;
@@ -452,7 +458,7 @@ __device__ __forceinline__ void tvd_rhs_layer(const DevMesh &m, const DevState &
         double Ap = fabs(0.5 * (Q + fabs(Q))), Am = fabs(0.5 * (Q - fabs(Q)));
         v_adv = dt2 * (-(Ap * ck) + Am * ckp);
         if (a.vlim > 0) {
-          double ps = w.PSIi[pix(k, e, E) + t];
+          double ps = t == 0 ? pk.x : pk.y;
           vft = 0.5 * dt2 * (-(ps * Ap * (ckp - ck)) + ps * Am * (ck - ckp));
         }
       } else if (k == tl) {
@@ -460,7 +466,7 @@ __device__ __forceinline__ void tvd_rhs_layer(const DevMesh &m, const DevState &
         double Ap = fabs(0.5 * (Q + fabs(Q))), Am = fabs(0.5 * (Q - fabs(Q)));
         v_adv = dt2 * (Ap * ckm - Am * ck);
         if (a.vlim > 0) {
-          double ps = w.PSIi[pix(k - 1, e, E) + t];
+          double ps = t == 0 ? pm.x : pm.y;
           vft = 0.5 * dt2 * (ps * Ap * (ck - ckm) - ps * Am * (ckm - ck));
         }
       } else {
@@ -470,7 +476,7 @@ __device__ __forceinline__ void tvd_rhs_layer(const DevMesh &m, const DevState &
         double A2p = fabs(0.5 * (Q2 + fabs(Q2))), A2m = fabs(0.5 * (Q2 - fabs(Q2)));
         v_adv = dt2 * (A1p * ckm - A2p * ck + A2m * ckp - A1m * ck);
         if (a.vlim > 0) {
-          double p1 = w.PSIi[pix(k - 1, e, E) + t], p2 = w.PSIi[pix(k, e, E) + t];
+          double p1 = t == 0 ? pm.x : pm.y, p2 = t == 0 ? pk.x : pk.y;
           vft = 0.5 * dt2 * (p1 * A1p * (ck - ckm) - p2 * A2p * (ckp - ck) + p2 * A2m * (ck - ckp) - p1 * A1m * (ckm - ck));
         }
       }
