@@ -19,6 +19,9 @@
 #ifndef GOM_NV_LEVELS
 #define GOM_NV_LEVELS 4  // node levels per thread of k_node_vel
 #endif
+#ifndef GOM_TVD_LIM_LEVELS
+#define GOM_TVD_LIM_LEVELS 1  // levels per thread of k_tvd_lim_cell
+#endif
 #ifndef GOM_TVD_NODE_LEVELS
 #define GOM_TVD_NODE_LEVELS 8  // node levels per thread of k_tvd_node
 #endif

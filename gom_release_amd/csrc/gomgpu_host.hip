@@ -1223,7 +1223,7 @@ static int stage_impl(gomgpu_ctx *c, int id) {
         tvd_launch_coef_face(f32, (long)d.J * d.L, BS, st, c->m, c->s, wv, d, a);
         tvd_launch_coef_cell(f32, (long)d.E * (d.L + 1), BS, st, c->m, c->s, wv, d, a);
         if (p.h_flux_limiter > 0 || p.v_flux_limiter > 0) {
-          tvd_launch_lim_cell(f32, (long)(d.L + 1) * d.E, LB(TVD_LIM), st, c->m, c->s, wv, d, p.h_flux_limiter,
+          tvd_launch_lim_cell(f32, (long)((d.L + GOM_TVD_LIM_LEVELS) / GOM_TVD_LIM_LEVELS) * d.E, LB(TVD_LIM), st, c->m, c->s, wv, d, p.h_flux_limiter,
                               sub == 0);
           tvd_launch_lim_face(f32, (long)d.J * d.L, BS, st, c->m, c->s, wv, d, p.h_flux_limiter, sub == 0);
         }

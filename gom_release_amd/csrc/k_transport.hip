@@ -196,15 +196,15 @@ __device__ __forceinline__ double limiter(int which, double r, double phi) {
 
 // Horizontal upwind sums of one cell-layer (solve_transport_equation_v19.f90:700-760): the
 // inflow-weighted differences sum1 and the inflows sum2, over the cell's active faces in
-// slot order; c0 holds the cell-layer's own concentrations.
+// slot order; sf4 and ad4 are the cell's slot_face and cg_adj records, c0 the cell-layer's
+// own concentrations.
 template <typename W>
 __device__ __forceinline__ void lim_hsums(const DevMesh &m, const TvdW<W> &w, const DevDims &d, int e, int k,
-                                          const double *c0, double *sum1, double *sum2) {
+                                          const int4 &sf4, const int4 &ad4, const double *c0, double *sum1,
+                                          double *sum2) {
   const int E = d.E, L2 = d.L + 2, J = d.J;
   sum1[0] = sum1[1] = 0.0;
   sum2[0] = sum2[1] = 0.0;
-  const int4 sf4 = *(const int4 *)(m.slot_face + 4 * (size_t)e);
-  const int4 ad4 = *(const int4 *)(m.cg_adj + 4 * (size_t)e);
   const int sf[4] = {sf4.x, sf4.y, sf4.z, sf4.w}, ad[4] = {ad4.x, ad4.y, ad4.z, ad4.w};
 #pragma unroll
   for (int l = 0; l < 4; l++) {  // slots past the cell's faces hold 0
@@ -260,23 +260,23 @@ __global__ void k_tvd_coef_cell(DevMesh m, DevState s, TvdW<W> w, DevDims d, Tvd
   if ((a.hlim > 0 || a.vlim > 0) && k >= 1 && k == tl) {
     double sum1[2], sum2[2], c0[2] = {0.0, 0.0};
     for (int t = 0; t < d.NT; t++) c0[t] = w.con[cix(t, k, e, d.L + 2, E)];
-    lim_hsums(m, w, d, e, k, c0, sum1, sum2);
+    const int4 sf4 = *(const int4 *)(m.slot_face + 4 * (size_t)e);
+    const int4 ad4 = *(const int4 *)(m.cg_adj + 4 * (size_t)e);
+    lim_hsums(m, w, d, e, k, sf4, ad4, c0, sum1, sum2);
     for (int t = 0; t < d.NT; t++) *(double2 *)(w.htop + hix(t, e, d.NT)) = make_double2(sum1[t], sum2[t]);
   }
 }
 
-// Limiter ratios (solve_transport_equation_v19.f90:624-842), one thread per (level, cell):
+// Limiter ratios (solve_transport_equation_v19.f90:624-842), the work of one level of a cell:
 // the per-layer upwind sums s1a, s2a that the face ratios r_jk need (layer k), and the
 // vertical ratio r_ik (level k) with its limiter PSI_ik (level k). As in the reference, the
 // vertical ratios reuse the horizontal sums left over from the column's top layer, which
 // k_tvd_coef_cell computed once per column (w.htop) in the same order.
 template <typename W>
-__global__ void k_tvd_lim_cell(DevMesh m, DevState s, TvdW<W> w, DevDims d, int hlim, int first) {
-  long q = (long)gom_block() * blockDim.x + threadIdx.x;
-  const int E = d.E, L = d.L, L1 = d.L + 1, L2 = d.L + 2;
-  if (q >= (long)L1 * E) return;
-  int e = (int)(q % E), k = (int)(q / E);
-  int tl = s.top_layer_at_element[e], bl = m.bottom_layer_at_element[e];
+__device__ __forceinline__ void lim_cell_level(const DevMesh &m, const TvdW<W> &w, const DevDims &d, int hlim,
+                                               int first, int e, int k, int tl, int bl, const int4 &sf4,
+                                               const int4 &ad4) {
+  const int E = d.E, L2 = d.L + 2;
   if (tl == 0 || k < bl - 1 || k > tl) {  // zero from the first sub-cycle on
     if (first) *(double2 *)(w.PSIi + pix(k, e, E)) = make_double2(0.0, 0.0);
     return;
@@ -314,7 +314,7 @@ __global__ void k_tvd_lim_cell(DevMesh m, DevState s, TvdW<W> w, DevDims d, int 
         }
       }
     }
-    lim_hsums(m, w, d, e, k, c0, sum1, sum2);
+    lim_hsums(m, w, d, e, k, sf4, ad4, c0, sum1, sum2);
     typedef typename Pair<W>::T W2;
     for (int t = 0; t < d.NT; t++) {
       W2 v;
@@ -349,6 +349,26 @@ __global__ void k_tvd_lim_cell(DevMesh m, DevState s, TvdW<W> w, DevDims d, int 
     psi[t] = hlim != 0 ? limiter(hlim, r, phi) : 0.0;
   }
   *(double2 *)(w.PSIi + pix(k, e, E)) = make_double2(psi[0], psi[1]);
+}
+
+// One thread per (chunk of TVD_LIM_LEVELS levels, cell): the cell's layer range and slot
+// records are read once for the chunk's levels.
+#define TVD_LIM_LEVELS GOM_TVD_LIM_LEVELS
+template <typename W>
+__global__ void k_tvd_lim_cell(DevMesh m, DevState s, TvdW<W> w, DevDims d, int hlim, int first) {
+  long q = (long)gom_block() * blockDim.x + threadIdx.x;
+  const int E = d.E, L1 = d.L + 1;
+  const int nch = (L1 + TVD_LIM_LEVELS - 1) / TVD_LIM_LEVELS;
+  if (q >= (long)nch * E) return;
+  const int e = (int)(q % E), k0 = (int)(q / E) * TVD_LIM_LEVELS;
+  const int tl = s.top_layer_at_element[e], bl = m.bottom_layer_at_element[e];
+  int4 sf4 = make_int4(0, 0, 0, 0), ad4 = sf4;
+  if (tl != 0 && k0 + TVD_LIM_LEVELS - 1 >= bl && k0 <= tl) {
+    sf4 = *(const int4 *)(m.slot_face + 4 * (size_t)e);
+    ad4 = *(const int4 *)(m.cg_adj + 4 * (size_t)e);
+  }
+  for (int z = 0; z < TVD_LIM_LEVELS && k0 + z < L1; z++)
+    lim_cell_level(m, w, d, hlim, first, e, k0 + z, tl, bl, sf4, ad4);
 }
 
 // Face ratio r_jk and PSI_jk. In the reference each adjacent cell whose outflow the face
